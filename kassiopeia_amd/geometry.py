@@ -1,0 +1,364 @@
+"""Element records and synthetic meshes for the electrostatic BEM path.
+
+Element records follow KEMField's shape data (11 doubles per element):
+triangles and rectangles ``[A, B, P0x, P0y, P0z, N1x, N1y, N1z, N2x, N2y, N2z]``
+(``KEMField/Source/Surfaces/include/KTriangle.hh:9-131``, ``KRectangle.hh``), line
+segments ``[P0x, P0y, P0z, P1x, P1y, P1z, diameter, 0, 0, 0, 0]``
+(``KEMField/Source/Surfaces/include/KLineSegment.hh``).
+
+The meshes are the workloads named by the benchmark configurations:
+
+* ``cube_mesh``: unit cube, each face an n x n grid of squares split into 2 triangles;
+* ``coax_mesh``: two coaxial cylinders (r = 0.5 m, 1.0 m, length 4 m) with annular end
+  caps, tangentially jittered interior vertices;
+* ``ring_stack_mesh``: an axisymmetric stack of annular ring electrodes;
+* ``ring_bore_mesh``: coaxial ring electrodes plus an outer cylinder;
+* ``sphere_mesh``: subdivided icosahedron on the unit sphere.
+"""
+from __future__ import annotations
+
+import dataclasses
+
+import numpy as np
+
+TRIANGLE, RECTANGLE, LINESEGMENT = 0, 1, 2
+DIRICHLET, NEUMANN = 0, 1
+REC = 11
+
+
+@dataclasses.dataclass
+class Mesh:
+    """A surface container: elements in KSurfaceContainer global order."""
+
+    shapes: np.ndarray  # int32[n]
+    recs: np.ndarray  # float64[n, 11]
+    bc_type: np.ndarray  # int32[n]  (0 Dirichlet, 1 Neumann)
+    bc_value: np.ndarray  # float64[n]  (Dirichlet voltage)
+    bc_flux: np.ndarray  # float64[n]  (Neumann normal boundary flux, eps ratio)
+    group: np.ndarray  # int32[n]  (electrode id)
+
+    @property
+    def n(self) -> int:
+        return int(self.shapes.shape[0])
+
+    def areas(self) -> np.ndarray:
+        a = np.empty(self.n)
+        t = self.shapes == TRIANGLE
+        r = self.recs
+        cr = np.cross(r[:, 5:8], r[:, 8:11])
+        a[t] = 0.5 * r[t, 0] * r[t, 1] * np.sqrt(cr[t, 0] ** 2 + cr[t, 1] ** 2 + cr[t, 2] ** 2)
+        q = self.shapes == RECTANGLE
+        a[q] = r[q, 0] * r[q, 1]
+        ln = self.shapes == LINESEGMENT
+        if ln.any():
+            d = r[ln, 0:3] - r[ln, 3:6]
+            a[ln] = np.pi * r[ln, 6] * np.sqrt(d[:, 0] ** 2 + d[:, 1] ** 2 + d[:, 2] ** 2)
+        return a
+
+    def centroids(self) -> np.ndarray:
+        r = self.recs
+        c = np.empty((self.n, 3))
+        t = self.shapes == TRIANGLE
+        c[t] = r[t, 2:5] + (r[t, 0:1] * r[t, 5:8] + r[t, 1:2] * r[t, 8:11]) / 3.0
+        q = self.shapes == RECTANGLE
+        c[q] = r[q, 2:5] + r[q, 0:1] * r[q, 5:8] * 0.5 + r[q, 1:2] * r[q, 8:11] * 0.5
+        ln = self.shapes == LINESEGMENT
+        c[ln] = (r[ln, 0:3] + r[ln, 3:6]) * 0.5
+        return c
+
+    def subset(self, idx) -> "Mesh":
+        idx = np.asarray(idx)
+        return Mesh(self.shapes[idx], self.recs[idx], self.bc_type[idx], self.bc_value[idx],
+                    self.bc_flux[idx], self.group[idx])
+
+    def with_voltages(self, group_volts) -> "Mesh":
+        """Copy with Dirichlet values set per electrode group."""
+        v = np.asarray(group_volts, dtype=np.float64)[self.group]
+        return dataclasses.replace(self, bc_value=np.ascontiguousarray(v))
+
+
+def _norm(v):
+    # evaluation order of KThreeVector::Magnitude: sqrt((x*x + y*y) + z*z)
+    return np.sqrt(v[..., 0] * v[..., 0] + v[..., 1] * v[..., 1] + v[..., 2] * v[..., 2])
+
+
+def records_from_points(p0, p1, p2) -> np.ndarray:
+    """Vectorised KTriangle/KRectangle::SetValues(p0, p1, p2) (KTriangle.cc:18-28):
+    N1 = unit(p1 - p0), A = |p1 - p0|, N2 = unit(p2 - p0), B = |p2 - p0|."""
+    p0 = np.asarray(p0, dtype=np.float64).reshape(-1, 3)
+    p1 = np.asarray(p1, dtype=np.float64).reshape(-1, 3)
+    p2 = np.asarray(p2, dtype=np.float64).reshape(-1, 3)
+    n1 = p1 - p0
+    n2 = p2 - p0
+    a = _norm(n1)
+    b = _norm(n2)
+    rec = np.empty((p0.shape[0], REC))
+    rec[:, 0] = a
+    rec[:, 1] = b
+    rec[:, 2:5] = p0
+    rec[:, 5:8] = n1 / a[:, None]
+    rec[:, 8:11] = n2 / b[:, None]
+    return rec
+
+
+def line_records(p0, p1, diameter) -> np.ndarray:
+    p0 = np.asarray(p0, dtype=np.float64).reshape(-1, 3)
+    p1 = np.asarray(p1, dtype=np.float64).reshape(-1, 3)
+    rec = np.zeros((p0.shape[0], REC))
+    rec[:, 0:3] = p0
+    rec[:, 3:6] = p1
+    rec[:, 6] = diameter
+    return rec
+
+
+def _mesh(recs, shape, volts, group, bc_type=None, flux=None) -> Mesh:
+    n = recs.shape[0]
+    return Mesh(
+        shapes=np.full(n, shape, dtype=np.int32),
+        recs=np.ascontiguousarray(recs),
+        bc_type=np.zeros(n, dtype=np.int32) if bc_type is None else np.asarray(bc_type, np.int32),
+        bc_value=np.ascontiguousarray(np.asarray(volts, dtype=np.float64) * np.ones(n)),
+        bc_flux=np.zeros(n) if flux is None else np.asarray(flux, np.float64),
+        group=np.asarray(group, dtype=np.int32) * np.ones(n, dtype=np.int32),
+    )
+
+
+def concat(*meshes: Mesh) -> Mesh:
+    return Mesh(*[np.concatenate([getattr(m, f.name) for m in meshes])
+                  for f in dataclasses.fields(Mesh)])
+
+
+def _quad_grid_triangles(P):
+    """Split a (nu+1) x (nv+1) grid of vertices P[i, j] into 2 triangles per cell, with
+    (p1 - p0) x (p2 - p0) along the grid's (u x v) orientation."""
+    a = P[:-1, :-1].reshape(-1, 3)
+    b = P[1:, :-1].reshape(-1, 3)
+    c = P[1:, 1:].reshape(-1, 3)
+    d = P[:-1, 1:].reshape(-1, 3)
+    p0 = np.concatenate([a, c])
+    p1 = np.concatenate([b, d])
+    p2 = np.concatenate([c, a])
+    # interleave so the two triangles of one cell are adjacent in container order
+    n = a.shape[0]
+    order = np.stack([np.arange(n), np.arange(n) + n], axis=1).reshape(-1)
+    return p0[order], p1[order], p2[order]
+
+
+def cube_mesh(n: int = 12, edge: float = 1.0, volts: float = 1.0) -> Mesh:
+    """Cube centred at the origin; each face an n x n grid of squares, each square split
+    along a diagonal into two triangles (6*2*n*n triangles), outward normals."""
+    h = edge / 2.0
+    s = np.linspace(-h, h, n + 1)
+    U, V = np.meshgrid(s, s, indexing="ij")
+    faces = []
+    for axis in range(3):
+        for sign in (1.0, -1.0):
+            P = np.zeros((n + 1, n + 1, 3))
+            u_ax, v_ax = (axis + 1) % 3, (axis + 2) % 3
+            if sign < 0:
+                u_ax, v_ax = v_ax, u_ax
+            P[..., axis] = sign * h
+            P[..., u_ax] = U
+            P[..., v_ax] = V
+            faces.append(records_from_points(*_quad_grid_triangles(P)))
+    recs = np.concatenate(faces)
+    return _mesh(recs, TRIANGLE, volts, 0)
+
+
+def cube_rect_mesh(n: int = 12, edge: float = 1.0, volts: float = 1.0) -> Mesh:
+    """Cube with each face an n x n grid of rectangles (KRectangle records)."""
+    h = edge / 2.0
+    s = np.linspace(-h, h, n + 1)
+    recs = []
+    for axis in range(3):
+        for sign in (1.0, -1.0):
+            u_ax, v_ax = (axis + 1) % 3, (axis + 2) % 3
+            if sign < 0:
+                u_ax, v_ax = v_ax, u_ax
+            I, J = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+            p0 = np.zeros((n * n, 3))
+            p1 = np.zeros((n * n, 3))
+            p2 = np.zeros((n * n, 3))
+            for p in (p0, p1, p2):
+                p[:, axis] = sign * h
+            p0[:, u_ax] = s[I.ravel()]
+            p0[:, v_ax] = s[J.ravel()]
+            p1[:, u_ax] = s[I.ravel() + 1]
+            p1[:, v_ax] = s[J.ravel()]
+            p2[:, u_ax] = s[I.ravel()]
+            p2[:, v_ax] = s[J.ravel() + 1]
+            recs.append(records_from_points(p0, p1, p2))
+    return _mesh(np.concatenate(recs), RECTANGLE, volts, 0)
+
+
+def _jitter(rng, shape, amp):
+    return rng.uniform(-amp, amp, size=shape)
+
+
+def _cylinder(radius, z0, z1, nphi, nz, rng, jitter, inward=False):
+    """Cylinder side r = radius, z in [z0, z1]; vertices jittered tangentially (phi, z) by
+    +-jitter x local edge length except on the two rim circles."""
+    phi = np.linspace(0.0, 2.0 * np.pi, nphi + 1)
+    z = np.linspace(z0, z1, nz + 1)
+    PH, Z = np.meshgrid(phi, z, indexing="ij")
+    dphi = 2.0 * np.pi / nphi
+    dz = (z1 - z0) / nz
+    if jitter > 0:
+        jp = _jitter(rng, PH.shape, jitter * dphi)
+        jz = _jitter(rng, Z.shape, jitter * dz)
+        jp[:, 0] = jp[:, -1] = 0.0
+        jz[:, 0] = jz[:, -1] = 0.0
+        jp[-1] = jp[0]  # seam: phi = 2 pi copies phi = 0
+        jz[-1] = jz[0]
+        PH = PH + jp
+        Z = Z + jz
+    P = np.stack([radius * np.cos(PH), radius * np.sin(PH), Z], axis=-1)
+    if inward:
+        P = P[::-1]
+    return records_from_points(*_quad_grid_triangles(P))
+
+
+def _annulus(r0, r1, z, nr, nphi, rng, jitter, up=True):
+    """Flat annulus r in [r0, r1] at height z; jittered tangentially (r, phi) except on rims."""
+    r = np.linspace(r0, r1, nr + 1)
+    phi = np.linspace(0.0, 2.0 * np.pi, nphi + 1)
+    R, PH = np.meshgrid(r, phi, indexing="ij")
+    if jitter > 0:
+        jr = _jitter(rng, R.shape, jitter * (r1 - r0) / nr)
+        jp = _jitter(rng, PH.shape, jitter * 2.0 * np.pi / nphi)
+        jr[0] = jr[-1] = 0.0
+        jp[0] = jp[-1] = 0.0
+        jr[:, -1] = jr[:, 0]
+        jp[:, -1] = jp[:, 0]
+        R = R + jr
+        PH = PH + jp
+    P = np.stack([R * np.cos(PH), R * np.sin(PH), np.full_like(R, z)], axis=-1)
+    if not up:
+        P = P[::-1]
+    return records_from_points(*_quad_grid_triangles(P))
+
+
+def coax_mesh(seed: int = 1234, r_in: float = 0.5, r_out: float = 1.0, length: float = 4.0,
+              v_in: float = -18600.0, v_out: float = 0.0, nphi_in: int = 48, nphi_out: int = 96,
+              nz: int = 62, ncap_r: int = 8, ncap_phi: int = 67, jitter: float = 0.01) -> Mesh:
+    """Two coaxial cylinders with annular end caps: 2*(48*62 + 96*62 + 2*8*67) = 20,000
+    triangles.  Groups: 0 = inner cylinder (v_in); 1 = outer cylinder, 2 = end caps (both
+    v_out).  Interior vertices of each surface are jittered tangentially by up to 1 % of the
+    local edge length (seeded)."""
+    rng = np.random.default_rng(seed)
+    z0, z1 = -length / 2.0, length / 2.0
+    inner = _cylinder(r_in, z0, z1, nphi_in, nz, rng, jitter)
+    outer = _cylinder(r_out, z0, z1, nphi_out, nz, rng, jitter, inward=True)
+    cap_lo = _annulus(r_in, r_out, z0, ncap_r, ncap_phi, rng, jitter, up=True)
+    cap_hi = _annulus(r_in, r_out, z1, ncap_r, ncap_phi, rng, jitter, up=False)
+    return concat(_mesh(inner, TRIANGLE, v_in, 0), _mesh(outer, TRIANGLE, v_out, 1),
+                  _mesh(np.concatenate([cap_lo, cap_hi]), TRIANGLE, v_out, 2))
+
+
+def coax_ring_groups(mesh: Mesh, ngroups: int = 16, length: float = 4.0) -> np.ndarray:
+    """Split a coax mesh into `ngroups` axial ring groups by centroid z (for multi-RHS sets)."""
+    z = mesh.centroids()[:, 2]
+    g = np.floor((z + length / 2.0) / length * ngroups).astype(np.int32)
+    return np.clip(g, 0, ngroups - 1)
+
+
+def ring_stack_mesh(nrings: int = 8, r_min: float = 0.2, r_max: float = 1.0, dz: float = 0.25,
+                    width: float = 0.1, n_target: int = 5000, volts=None) -> Mesh:
+    """Axisymmetric stack of flat annular ring electrodes (inner radius r_i spaced linearly in
+    [r_min, r_max], radial width `width`), one ring per z-level spaced by dz, centred on z = 0.
+    Triangles are distributed over rings in proportion to area, ~n_target in total."""
+    radii = np.linspace(r_min, r_max, nrings)
+    areas = np.pi * ((radii + width) ** 2 - radii ** 2)
+    per = np.maximum(2 * 8 * 2, np.round(n_target * areas / areas.sum())).astype(int)
+    parts = []
+    zs = (np.arange(nrings) - (nrings - 1) / 2.0) * dz
+    for k in range(nrings):
+        nr = 2
+        nphi = max(8, int(round(per[k] / (2 * nr))))
+        recs = _annulus(radii[k], radii[k] + width, zs[k], nr, nphi, None, 0.0)
+        v = 0.0 if volts is None else volts[k]
+        parts.append(_mesh(recs, TRIANGLE, v, k))
+    return concat(*parts)
+
+
+def ring_bore_mesh(nrings: int = 32, r_ring: float = 1.9, ring_width: float = 0.05,
+                   z_span: float = 8.0, r_outer: float = 2.0, n_target: int = 100000, volts=None,
+                   seed: int = 99) -> Mesh:
+    """Coaxial ring electrodes (short cylinders of radius r_ring and axial width ring_width,
+    evenly spaced over z_span) inside a grounded outer cylinder of radius r_outer; about
+    n_target triangles split between rings and the outer cylinder by area."""
+    rng = np.random.default_rng(seed)
+    zc = (np.arange(nrings) - (nrings - 1) / 2.0) * (z_span / nrings)
+    a_ring = 2 * np.pi * r_ring * ring_width
+    a_out = 2 * np.pi * r_outer * (z_span + 0.5)
+    total = nrings * a_ring + a_out
+    n_ring = int(n_target * a_ring / total) // 2
+    n_out = (n_target // 2) - nrings * n_ring
+    parts = []
+    for k in range(nrings):
+        nz = max(1, int(round(np.sqrt(n_ring * ring_width / (2 * np.pi * r_ring)))))
+        nphi = max(8, n_ring // nz)
+        recs = _cylinder(r_ring, zc[k] - ring_width / 2, zc[k] + ring_width / 2, nphi, nz, rng, 0.0)
+        v = 0.0 if volts is None else volts[k]
+        parts.append(_mesh(recs, TRIANGLE, v, k))
+    nz = max(1, int(round(np.sqrt(n_out * (z_span + 0.5) / (2 * np.pi * r_outer)))))
+    nphi = max(8, n_out // nz)
+    recs = _cylinder(r_outer, -(z_span + 0.5) / 2, (z_span + 0.5) / 2, nphi, nz, rng, 0.0, inward=True)
+    parts.append(_mesh(recs, TRIANGLE, 0.0, nrings))
+    return concat(*parts)
+
+
+def sphere_mesh(subdiv: int = 3, radius: float = 1.0, volts: float = 1.0) -> Mesh:
+    """Icosahedron subdivided `subdiv` times and projected to the sphere (20*4**subdiv
+    triangles), outward normals."""
+    t = (1.0 + 5.0 ** 0.5) / 2.0
+    V = [(-1, t, 0), (1, t, 0), (-1, -t, 0), (1, -t, 0), (0, -1, t), (0, 1, t), (0, -1, -t), (0, 1, -t),
+         (t, 0, -1), (t, 0, 1), (-t, 0, -1), (-t, 0, 1)]
+    F = [(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4), (11, 10, 2),
+         (10, 7, 6), (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8), (3, 8, 9), (4, 9, 5), (2, 4, 11),
+         (6, 2, 10), (8, 6, 7), (9, 8, 1)]
+    verts = [np.array(v, dtype=np.float64) / np.linalg.norm(v) for v in V]
+    faces = list(F)
+    for _ in range(subdiv):
+        cache = {}
+        nf = []
+
+        def mid(i, j):
+            key = (min(i, j), max(i, j))
+            if key not in cache:
+                m = verts[i] + verts[j]
+                verts.append(m / np.linalg.norm(m))
+                cache[key] = len(verts) - 1
+            return cache[key]
+
+        for a, b, c in faces:
+            ab, bc, ca = mid(a, b), mid(b, c), mid(c, a)
+            nf += [(a, ab, ca), (b, bc, ab), (c, ca, bc), (ab, bc, ca)]
+        faces = nf
+    Vv = np.array(verts) * radius
+    Fa = np.array(faces)
+    return _mesh(records_from_points(Vv[Fa[:, 0]], Vv[Fa[:, 1]], Vv[Fa[:, 2]]), TRIANGLE, volts, 0)
+
+
+# ------------------------------------------------------------------ field points
+def points_shell(m: int, r_min: float, r_max: float, seed: int) -> np.ndarray:
+    """m points uniform in volume in the spherical shell r_min <= r <= r_max."""
+    rng = np.random.default_rng(seed)
+    u = rng.uniform(r_min ** 3, r_max ** 3, m) ** (1.0 / 3.0)
+    v = rng.normal(size=(m, 3))
+    v /= np.linalg.norm(v, axis=1)[:, None]
+    return np.ascontiguousarray(v * u[:, None])
+
+
+def points_annulus(m: int, r_min: float, r_max: float, z_half: float, seed: int) -> np.ndarray:
+    """m points uniform in volume in r_min < r < r_max, |z| < z_half."""
+    rng = np.random.default_rng(seed)
+    r = np.sqrt(rng.uniform(r_min ** 2, r_max ** 2, m))
+    ph = rng.uniform(0.0, 2.0 * np.pi, m)
+    z = rng.uniform(-z_half, z_half, m)
+    return np.ascontiguousarray(np.stack([r * np.cos(ph), r * np.sin(ph), z], axis=1))
+
+
+def points_axis(m: int, z_min: float, z_max: float) -> np.ndarray:
+    z = np.linspace(z_min, z_max, m)
+    return np.ascontiguousarray(np.stack([np.zeros(m), np.zeros(m), z], axis=1))

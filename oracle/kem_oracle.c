@@ -1,0 +1,1006 @@
+/*
+ * kem_oracle.c -- CPU FP64 restatement of KEMField's electrostatic BEM hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY (the checker, never the product): see kem_oracle.h.
+ *
+ * Written for this project from the algorithms of the cited KEMField files.  The
+ * reference's branch structure, thresholds and quadrature tables are kept because the
+ * last bits of the results depend on them; loops and data flow are this file's own.
+ * Paths are relative to KEMField/Source/ unless they start with Kommon/.
+ */
+#include "kem_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define SQ(x) ((x) * (x))
+
+/* ---------------------------------------------------------------- constants */
+/* Kommon/Base/Utility/KConst_2021.h:37 (pi), :139-146 (eps0, 4 pi eps0);
+   Core/src/KEMConstants.cc:9-12 */
+static const double KO_PI = 3.141592653589793238462643383279502884;
+static const double KO_EPS0 = 8.8541878128E-12;
+#define KO_K (1. / (4. * KO_PI * KO_EPS0))
+
+double ko_pi(void) { return KO_PI; }
+double ko_eps0(void) { return KO_EPS0; }
+double ko_one_over_4pi_eps0(void) { return KO_K; }
+
+static void cross3(const double a[3], const double b[3], double c[3])
+{
+    c[0] = a[1] * b[2] - a[2] * b[1];
+    c[1] = a[2] * b[0] - a[0] * b[2];
+    c[2] = a[0] * b[1] - a[1] * b[0];
+}
+static double dot3(const double a[3], const double b[3]) { return (a[0] * b[0] + a[1] * b[1]) + a[2] * b[2]; }
+static double norm3(const double a[3]) { return sqrt(SQ(a[0]) + SQ(a[1]) + SQ(a[2])); }
+
+/* ---------------------------------------------------------------- geometry */
+/* KTriangle::SetValues(p0,p1,p2): Surfaces/src/KTriangle.cc:18-28 */
+void ko_set_from_points(const double p0[3], const double p1[3], const double p2[3], double rec[KO_REC])
+{
+    double n1[3], n2[3];
+    for (int k = 0; k < 3; k++) {
+        n1[k] = p1[k] - p0[k];
+        n2[k] = p2[k] - p0[k];
+    }
+    double a = norm3(n1), b = norm3(n2);
+    rec[0] = a;
+    rec[1] = b;
+    for (int k = 0; k < 3; k++) {
+        rec[2 + k] = p0[k];
+        /* KThreeVector::Unit() divides by the magnitude */
+        rec[5 + k] = n1[k] / a;
+        rec[8 + k] = n2[k] / b;
+    }
+}
+
+/* unit normal N3 = unit(N1 x N2): KTriangle.cc:16,28 */
+void ko_normal(int shape, const double* d, double n[3])
+{
+    (void)shape;
+    double c[3];
+    cross3(d + 5, d + 8, c);
+    double m = norm3(c);
+    for (int k = 0; k < 3; k++) n[k] = c[k] / m;
+}
+
+/* KTriangle.hh:27-30, KRectangle.hh:27-30, KLineSegment.hh:25-28 */
+double ko_area(int shape, const double* d)
+{
+    if (shape == KO_TRIANGLE) {
+        double c[3];
+        cross3(d + 5, d + 8, c);
+        return .5 * d[0] * d[1] * norm3(c);
+    }
+    if (shape == KO_RECTANGLE) return d[0] * d[1];
+    double l[3] = {d[0] - d[3], d[1] - d[4], d[2] - d[5]};
+    return KO_PI * d[6] * norm3(l);
+}
+
+/* KTriangle.hh:31-34, KRectangle.hh:31-34, KLineSegment.hh:29-32 */
+void ko_centroid(int shape, const double* d, double c[3])
+{
+    for (int k = 0; k < 3; k++) {
+        if (shape == KO_TRIANGLE)
+            c[k] = d[2 + k] + (d[0] * d[5 + k] + d[1] * d[8 + k]) / 3.;
+        else if (shape == KO_RECTANGLE)
+            c[k] = d[2 + k] + d[0] * d[5 + k] * .5 + d[1] * d[8 + k] * .5;
+        else
+            c[k] = (d[k] + d[3 + k]) * .5;
+    }
+}
+
+/* ------------------------------------------------- triangle cubature tables */
+/* Numeric/include/KElectrostaticCubatureTriangleIntegrator.hh:56-258.
+   Published rules: 7-point Radon/Stroud (degree 5), 12-point Gatermann 1988 (degree 7),
+   33-point Papanicolopulos 2011 (degree 12).  Barycentric orbits and normalised weights
+   (sum 1). */
+static double T7a[3], T7b[3], T7w[3];
+static const double T12a[4] = {0.6238226509439084e-1, 0.5522545665692000e-1, 0.3432430294509488e-1,
+                               0.5158423343536001};
+static const double T12b[4] = {0.6751786707392436e-1, 0.3215024938520156, 0.6609491961867980,
+                               0.2777161669764050};
+static const double T12c[4] = {0.8700998678316848, 0.6232720494910644, 0.3047265008681072,
+                               0.2064414986699949};
+static const double T12w[4] = {0.2651702815743450e-1, 0.4388140871444811e-1, 0.2877504278497528e-1,
+                               0.6749318700980879e-1};
+static const double T33a[8] = {4.570749859701478e-01, 1.197767026828138e-01, 2.359249810891690e-02,
+                               7.814843446812914e-01, 9.507072731273288e-01, 1.162960196779266e-01,
+                               2.303415635526714e-02, 2.138249025617059e-02};
+static const double T33b[8] = {2.714625070149261e-01, 4.401116486585931e-01, 4.882037509455416e-01,
+                               1.092578276593543e-01, 2.464636343633559e-02, 2.554542286385173e-01,
+                               2.916556797383410e-01, 1.272797172335894e-01};
+static const double T33w[8] = {6.254121319590276e-02, 4.991833492806094e-02, 2.426683808145203e-02,
+                               2.848605206887754e-02, 7.931642509973638e-03, 4.322736365941421e-02,
+                               2.178358503860756e-02, 1.508367757651144e-02};
+static int tables_ready = 0;
+
+static void init_tables(void)
+{
+    if (tables_ready) return;
+    const double t = 1. / 3., q = sqrt(15), r = (1. + q) / 7., s = (1. - q) / 7.;
+    T7a[0] = t;
+    T7a[1] = t + (2. * t * s);
+    T7a[2] = t + (2. * t * r);
+    T7b[0] = t;
+    T7b[1] = t - (t * s);
+    T7b[2] = t - (t * r);
+    T7w[0] = 9 / 40.;
+    T7w[1] = (155. + q) / 1200.;
+    T7w[2] = (155. - q) / 1200.;
+    tables_ready = 1;
+}
+
+/* one node: Q = a*A + b*B + c*C, component-wise in the reference order */
+static void bary(double a, double b, double c, const double A[3], const double B[3], const double C[3], double* Q)
+{
+    for (int k = 0; k < 3; k++) Q[k] = a * A[k] + b * B[k] + c * C[k];
+}
+
+static void tri_vertices(const double* d, double A[3], double B[3], double C[3])
+{
+    for (int k = 0; k < 3; k++) {
+        A[k] = d[2 + k];
+        B[k] = d[2 + k] + (d[0] * d[5 + k]);
+        C[k] = d[2 + k] + (d[1] * d[8 + k]);
+    }
+}
+
+/* node generation: KElectrostaticCubatureTriangleIntegrator.cc:34-68 (7), :70-115 (12), :212-269 (33).
+   The permutation pattern of each orbit follows the reference so that the node order (and
+   hence the summation order) is the reference's. */
+int ko_tri_nodes(int n, const double* d, double* Q, double* w)
+{
+    init_tables();
+    double A[3], B[3], C[3];
+    tri_vertices(d, A, B, C);
+    int m = 0;
+    if (n == 7) {
+        bary(T7a[0], T7b[0], T7b[0], A, B, C, Q + 3 * m);
+        w[m++] = T7w[0];
+        for (int o = 1; o < 3; o++) {
+            double a = T7a[o], b = T7b[o], c = T7b[o];
+            bary(a, b, c, A, B, C, Q + 3 * m);
+            w[m++] = T7w[o];
+            bary(b, a, c, A, B, C, Q + 3 * m);
+            w[m++] = T7w[o];
+            bary(c, b, a, A, B, C, Q + 3 * m);
+            w[m++] = T7w[o];
+        }
+        return m;
+    }
+    if (n == 12) {
+        for (int o = 0; o < 4; o++) {
+            double a = T12a[o], b = T12b[o], c = T12c[o];
+            bary(a, b, c, A, B, C, Q + 3 * m);
+            w[m++] = T12w[o] * 2.;
+            bary(b, c, a, A, B, C, Q + 3 * m);
+            w[m++] = T12w[o] * 2.;
+            bary(c, a, b, A, B, C, Q + 3 * m);
+            w[m++] = T12w[o] * 2.;
+        }
+        return m;
+    }
+    if (n == 33) {
+        for (int o = 0; o < 8; o++) {
+            double a = T33a[o], b = T33b[o], c = 1. - T33a[o] - T33b[o];
+            bary(a, b, c, A, B, C, Q + 3 * m);
+            w[m++] = T33w[o];
+            bary(b, a, c, A, B, C, Q + 3 * m);
+            w[m++] = T33w[o];
+            bary(c, b, a, A, B, C, Q + 3 * m);
+            w[m++] = T33w[o];
+            if (o >= 5) {
+                bary(a, c, b, A, B, C, Q + 3 * m);
+                w[m++] = T33w[o];
+                bary(c, a, b, A, B, C, Q + 3 * m);
+                w[m++] = T33w[o];
+                bary(b, c, a, A, B, C, Q + 3 * m);
+                w[m++] = T33w[o];
+            }
+        }
+        return m;
+    }
+    return 0;
+}
+
+/* weighted node sum for (E, phi): ElectricFieldAndPotential_TriNP (:387-451) and _RectNP.
+   prefactor = area / (4 pi eps0). */
+static void node_sum(int n, const double* Q, const double* w, double prefac, const double P[3], double out[4])
+{
+    double s[4] = {0., 0., 0., 0.};
+    for (int g = 0; g < n; g++) {
+        double dv[3], r2 = 0.;
+        for (int k = 0; k < 3; k++) {
+            dv[k] = P[k] - Q[3 * g + k];
+            r2 += SQ(dv[k]);
+        }
+        double ir = 1. / sqrt(r2);
+        double f = ir * ir * ir;
+        f = f * w[g];
+        s[0] += dv[0] * f;
+        s[1] += dv[1] * f;
+        s[2] += dv[2] * f;
+        s[3] += w[g] * ir;
+    }
+    for (int k = 0; k < 4; k++) out[k] = prefac * s[k];
+}
+
+static double tri_area_from_rec(const double* d)
+{
+    /* area exactly as in ..._TriNP: 0.5*A*B*|N1 x N2| with the reference's component order */
+    return 0.5 * d[0] * d[1] *
+           sqrt(SQ((d[6] * d[10]) - (d[7] * d[9])) + SQ((d[7] * d[8]) - (d[5] * d[10])) +
+                SQ((d[5] * d[9]) - (d[6] * d[8])));
+}
+
+void ko_tri_cubature(int n, const double* d, const double P[3], double out[4])
+{
+    double Q[99], w[33];
+    int m = ko_tri_nodes(n, d, Q, w);
+    node_sum(m, Q, w, tri_area_from_rec(d) * KO_K, P, out);
+}
+
+/* ---------------------------------------------------------------- solid angle */
+/* KSolidAngle::SolidAngleTriangleAsArray: Surfaces/include/KSolidAngle.hh:58-144 */
+double ko_solid_angle_tri(const double* d, const double P[3])
+{
+    double V[3][3];
+    tri_vertices(d, V[0], V[1], V[2]);
+    double n[3];
+    cross3(d + 5, d + 8, n);
+    double im = 1. / norm3(n);
+    for (int k = 0; k < 3; k++) n[k] *= im;
+    double c[3];
+    ko_centroid(KO_TRIANGLE, d, c);
+    double pc[3] = {P[0] - c[0], P[1] - c[1], P[2] - c[2]};
+    double h = (n[0] * pc[0]) + (n[1] * pc[1]) + (n[2] * pc[2]);
+    double res;
+    if (norm3(pc) <= 1.e-12) {
+        res = 2. * KO_PI;
+    } else {
+        double u[3][3];
+        for (int v = 0; v < 3; v++) {
+            double dv[3] = {V[v][0] - P[0], V[v][1] - P[1], V[v][2] - P[2]};
+            double mi = 1. / norm3(dv);
+            for (int k = 0; k < 3; k++) u[v][k] = mi * dv[k];
+        }
+        double x = 1. + dot3(u[0], u[1]) + dot3(u[0], u[2]) + dot3(u[1], u[2]);
+        double a12[3];
+        cross3(u[1], u[2], a12);
+        double y = fabs(dot3(u[0], a12));
+        res = fabs(2. * atan2(y, x));
+    }
+    if (h < 0.) res *= -1.;
+    return res;
+}
+
+static void rect_vertices(const double* d, double V[4][3])
+{
+    for (int k = 0; k < 3; k++) {
+        V[0][k] = d[2 + k];
+        V[1][k] = d[2 + k] + (d[0] * d[5 + k]);
+        V[2][k] = d[2 + k] + (d[0] * d[5 + k]) + (d[1] * d[8 + k]);
+        V[3][k] = d[2 + k] + (d[1] * d[8 + k]);
+    }
+}
+
+/* KSolidAngle::SolidAngleRectangleAsArray: KSolidAngle.hh:166-305 (two triangles P0P1P2, P2P3P0;
+   in-plane points get 2 pi inside / 0 outside) */
+double ko_solid_angle_rect(const double* d, const double P[3])
+{
+    double V[4][3];
+    rect_vertices(d, V);
+    double c[3], n[3];
+    for (int k = 0; k < 3; k++) c[k] = d[2 + k] + d[0] * d[5 + k] * .5 + d[1] * d[8 + k] * .5;
+    cross3(d + 5, d + 8, n); /* not normalised in the reference (N1, N2 orthonormal) */
+    double h = (n[0] * (P[0] - c[0])) + (n[1] * (P[1] - c[1])) + (n[2] * (P[2] - c[2]));
+    double res;
+    if (fabs(h) < 1.e-12) {
+        double s1[3], s2[3], dp[3];
+        for (int k = 0; k < 3; k++) {
+            s1[k] = d[0] * d[5 + k];
+            s2[k] = d[1] * d[8 + k];
+            dp[k] = d[2 + k] - P[k];
+        }
+        double l1 = (-1) * dot3(dp, s1) / SQ(d[0]);
+        double l2 = (-1) * dot3(dp, s2) / SQ(d[1]);
+        res = (l1 > 0. && l1 < 1. && l2 > 0. && l2 < 1.) ? 2. * KO_PI : 0.;
+    } else {
+        double u[4][3];
+        for (int v = 0; v < 4; v++) {
+            double dv[3] = {V[v][0] - P[0], V[v][1] - P[1], V[v][2] - P[2]};
+            double mi = 1. / norm3(dv);
+            for (int k = 0; k < 3; k++) u[v][k] = mi * dv[k];
+        }
+        double x1 = 1. + dot3(u[0], u[1]) + dot3(u[0], u[2]) + dot3(u[1], u[2]);
+        double a12[3];
+        cross3(u[1], u[2], a12);
+        double y1 = fabs(dot3(u[0], a12));
+        double x2 = 1. + dot3(u[2], u[3]) + dot3(u[2], u[0]) + dot3(u[3], u[0]);
+        double a30[3];
+        cross3(u[3], u[0], a30);
+        double y2 = fabs(dot3(u[2], a30));
+        res = fabs(2. * atan2(y1, x1)) + fabs(2. * atan2(y2, x2));
+    }
+    if (h < 0.) res *= -1.;
+    return res;
+}
+
+/* ------------------------------------------------------ closed-form near field */
+/* Common structure of KElectrostaticRWGTriangleIntegrator.cc and ...RectangleIntegrator.cc:
+   phi = (-h*Omega - sum_e t_e*L_e)/(4 pi eps0), E = (n*Omega + sum_e m_e*L_e)/(4 pi eps0),
+   L_e = ln((r+ + s+)/(r- + s-)) along edge e. */
+typedef struct {
+    int ne;           /* edges: 3 (triangle) or 4 (rectangle) */
+    double V[4][3];   /* vertices in edge order */
+    double u[4][3];   /* unit edge directions */
+    double mid[4][3]; /* edge midpoints */
+    double m[4][3];   /* in-plane outward normals u x n */
+    double n[3];      /* unit plane normal */
+    double c[3];      /* centroid used for h */
+    double sv[4][3];  /* full edge vectors (edge detection) */
+    double invlen[4]; /* 1/edge length (edge detection) */
+} ko_poly;
+
+static void poly_tri(const double* d, ko_poly* p)
+{
+    p->ne = 3;
+    tri_vertices(d, p->V[0], p->V[1], p->V[2]);
+    cross3(d + 5, d + 8, p->n);
+    double im = 1. / norm3(p->n);
+    for (int k = 0; k < 3; k++) p->n[k] = p->n[k] * im;
+    ko_centroid(KO_TRIANGLE, d, p->c);
+    /* IqL*: RWG/src/KElectrostaticRWGTriangleIntegrator.cc:20-100 */
+    double e12[3] = {p->V[2][0] - p->V[1][0], p->V[2][1] - p->V[1][1], p->V[2][2] - p->V[1][2]};
+    double len12 = sqrt(SQ(e12[0]) + SQ(e12[1]) + SQ(e12[2]));
+    double i12 = 1. / len12;
+    double hl[3] = {0.5 * d[0], 0.5 * len12, 0.5 * d[1]};
+    for (int k = 0; k < 3; k++) {
+        p->u[0][k] = d[5 + k];
+        p->u[1][k] = i12 * e12[k];
+        p->u[2][k] = -d[8 + k];
+        /* edge detection vectors: ...TriangleIntegrator.cc:645-657 */
+        p->sv[0][k] = d[0] * d[5 + k];
+        p->sv[1][k] = e12[k];
+        p->sv[2][k] = (-1) * d[1] * d[8 + k];
+    }
+    p->invlen[0] = 1. / d[0];
+    p->invlen[1] = 1. / len12;
+    p->invlen[2] = 1. / d[1];
+    for (int e = 0; e < 3; e++) {
+        for (int k = 0; k < 3; k++) p->mid[e][k] = hl[e] * p->u[e][k] + p->V[e][k];
+        cross3(p->u[e], p->n, p->m[e]);
+    }
+}
+
+static void poly_rect(const double* d, ko_poly* p)
+{
+    p->ne = 4;
+    rect_vertices(d, p->V);
+    cross3(d + 5, d + 8, p->n);
+    double im = 1. / sqrt(SQ(p->n[0]) + SQ(p->n[1]) + SQ(p->n[2]));
+    for (int k = 0; k < 3; k++) p->n[k] = p->n[k] * im;
+    /* RWG rectangle centroid: ...RectangleIntegrator.cc:1212-1214 */
+    for (int k = 0; k < 3; k++) p->c[k] = d[2 + k] + 0.5 * (d[0] * d[5 + k]) + 0.5 * (d[1] * d[8 + k]);
+    for (int k = 0; k < 3; k++) {
+        p->u[0][k] = d[5 + k];
+        p->u[1][k] = d[8 + k];
+        p->u[2][k] = -d[5 + k];
+        p->u[3][k] = -d[8 + k];
+        /* midpoints: ...RectangleIntegrator.cc:60-75 */
+        p->mid[0][k] = (0.5 * d[0] * d[5 + k]) + p->V[0][k];
+        p->mid[1][k] = (0.5 * d[1] * d[8 + k]) + p->V[1][k];
+        p->mid[2][k] = (-0.5 * d[0] * d[5 + k]) + p->V[2][k];
+        p->mid[3][k] = (-0.5 * d[1] * d[8 + k]) + p->V[3][k];
+        p->sv[0][k] = d[0] * d[5 + k];
+        p->sv[1][k] = d[1] * d[8 + k];
+        p->sv[2][k] = -d[0] * d[5 + k];
+        p->sv[3][k] = -d[1] * d[8 + k];
+    }
+    p->invlen[0] = 1. / d[0];
+    p->invlen[1] = 1. / d[1];
+    p->invlen[2] = 1. / d[0];
+    p->invlen[3] = 1. / d[1];
+    for (int e = 0; e < 4; e++) cross3(p->u[e], p->n, p->m[e]);
+}
+
+/* LogArgTaylor: ...TriangleIntegrator.cc:8-18 */
+static double log_arg_taylor(double sM, double dist)
+{
+    double q = fabs(dist / sM);
+    if (q < 1.e-14) q = 1.e-14;
+    return 0.5 * fabs(sM) * SQ(q);
+}
+
+/* line-integral term IqLFieldAndPotential (...TriangleIntegrator.cc:393-604):
+   returns iL[0..2] (field part) and iL[3] (potential part) */
+static void iql(const ko_poly* p, const double P[3], int count, int line, double dist, double iL[4])
+{
+    double D[4][3], r[4], t[4];
+    for (int v = 0; v < p->ne; v++) {
+        for (int k = 0; k < 3; k++) D[v][k] = p->V[v][k] - P[k];
+        r[v] = sqrt(SQ(D[v][0]) + SQ(D[v][1]) + SQ(D[v][2]));
+    }
+    for (int e = 0; e < p->ne; e++)
+        t[e] = (p->m[e][0] * (P[0] - p->mid[e][0])) + (p->m[e][1] * (P[1] - p->mid[e][1])) +
+               (p->m[e][2] * (P[2] - p->mid[e][2]));
+    iL[0] = iL[1] = iL[2] = iL[3] = 0.;
+    for (int e = 0; e < p->ne; e++) {
+        int e1 = (e + 1) % p->ne;
+        double rM = r[e], rP = r[e1];
+        double sM = dot3(D[e], p->u[e]), sP = dot3(D[e1], p->u[e]);
+        double val;
+        int have = 0;
+        if (count == 1 && line == e) {
+            if (fabs(dist / sM) < 1.E-6) {
+                val = log(rP + sP) - log(log_arg_taylor(sM, dist));
+                iL[0] += p->m[e][0] * val;
+                iL[1] += p->m[e][1] * val;
+                iL[2] += p->m[e][2] * val;
+                iL[3] += t[e] * val;
+            }
+        }
+        if (line != e) {
+            double nom, den;
+            if ((rM + sM) > (rP - sP)) {
+                nom = rP + sP;
+                den = rM + sM;
+            } else {
+                nom = rM - sM;
+                den = rP - sP;
+            }
+            val = log(nom) - log(den);
+            have = 1;
+        }
+        if (have) {
+            iL[0] += p->m[e][0] * val;
+            iL[1] += p->m[e][1] * val;
+            iL[2] += p->m[e][2] * val;
+            iL[3] += t[e] * val;
+        }
+    }
+}
+
+/* point-on-edge detection: ...TriangleIntegrator.cc:667-737 (fMinDistanceToSideLine 1e-14,
+   fToleranceLambda 1e-15) */
+static int edge_check(const ko_poly* p, const double P[3], int* line, double* distmin)
+{
+    int count = 0;
+    *line = 99;
+    *distmin = 0.;
+    for (int e = 0; e < p->ne; e++) {
+        double D[3] = {p->V[e][0] - P[0], p->V[e][1] - P[1], p->V[e][2] - P[2]};
+        double cv[3];
+        cross3(p->sv[e], D, cv);
+        double dl = sqrt(SQ(cv[0]) + SQ(cv[1]) + SQ(cv[2])) * p->invlen[e];
+        double lam = ((-D[0] * p->sv[e][0]) + (-D[1] * p->sv[e][1]) + (-D[2] * p->sv[e][2])) * SQ(p->invlen[e]);
+        if (dl < 1.E-14 && lam >= -1.E-15 && lam <= (1. + 1.E-15)) {
+            *distmin = dl;
+            count++;
+            *line = e;
+        }
+    }
+    return count;
+}
+
+static void rwg_eval(const ko_poly* p, int shape, const double* d, const double P[3], double out[4])
+{
+    int line;
+    double dist;
+    int count = edge_check(p, P, &line, &dist);
+    if (count == 2) {
+        /* field point on a vertex: average of points moved by +-1e-7 along the normal (:730-771) */
+        double acc[4] = {0., 0., 0., 0.};
+        for (int s = 0; s < 2; s++) {
+            double sg = s == 0 ? 1. : -1.;
+            double Q[3] = {P[0] + sg * 1.E-7 * p->n[0], P[1] + sg * 1.E-7 * p->n[1], P[2] + sg * 1.E-7 * p->n[2]};
+            double h = (p->n[0] * (Q[0] - p->c[0])) + (p->n[1] * (Q[1] - p->c[1])) + (p->n[2] * (Q[2] - p->c[2]));
+            double om = shape == KO_TRIANGLE ? ko_solid_angle_tri(d, Q) : ko_solid_angle_rect(d, Q);
+            double iL[4];
+            iql(p, Q, 9, 9, 9., iL);
+            for (int k = 0; k < 3; k++) acc[k] += (p->n[k] * om + iL[k]);
+            acc[3] += (-h * om - iL[3]);
+        }
+        for (int k = 0; k < 4; k++) out[k] = KO_K * 0.5 * acc[k];
+        return;
+    }
+    double h = (p->n[0] * (P[0] - p->c[0])) + (p->n[1] * (P[1] - p->c[1])) + (p->n[2] * (P[2] - p->c[2]));
+    double om = shape == KO_TRIANGLE ? ko_solid_angle_tri(d, P) : ko_solid_angle_rect(d, P);
+    double iL[4];
+    iql(p, P, count, line, dist, iL);
+    for (int k = 0; k < 3; k++) out[k] = KO_K * (p->n[k] * om + iL[k]);
+    out[3] = KO_K * ((-h * om) - iL[3]);
+}
+
+void ko_tri_rwg(const double* d, const double P[3], double out[4])
+{
+    ko_poly p;
+    poly_tri(d, &p);
+    rwg_eval(&p, KO_TRIANGLE, d, P, out);
+}
+
+void ko_rect_rwg(const double* d, const double P[3], double out[4])
+{
+    ko_poly p;
+    poly_rect(d, &p);
+    rwg_eval(&p, KO_RECTANGLE, d, P, out);
+}
+
+/* ---------------------------------------------------- triangle numeric dispatch */
+/* KElectrostaticCubatureTriangleIntegrator::ElectricFieldAndPotential (:595-665):
+   size = mean side, dr = |centroid - P| / size; cut-offs 131.4 / 30.4 / 3.42 (.hh:51-53) */
+int ko_tri_numeric(const double* d, const double P[3], double out[4])
+{
+    double A[3], B[3], C[3];
+    tri_vertices(d, A, B, C);
+    double l12 = sqrt(SQ(C[0] - B[0]) + SQ(C[1] - B[1]) + SQ(C[2] - B[2]));
+    double size = (d[0] + d[1] + l12) / 3.;
+    double c[3];
+    ko_centroid(KO_TRIANGLE, d, c);
+    double dv[3] = {c[0] - P[0], c[1] - P[1], c[2] - P[2]};
+    double dr = sqrt(SQ(dv[0]) + SQ(dv[1]) + SQ(dv[2])) * (1. / size);
+    int n = dr > 131.4 ? 7 : dr > 30.4 ? 12 : dr > 3.42 ? 33 : 0;
+    if (n)
+        ko_tri_cubature(n, d, P, out);
+    else
+        ko_tri_rwg(d, P, out);
+    return n;
+}
+
+/* ------------------------------------------------------ rectangle cubature */
+/* Numeric/include/KElectrostaticCubatureRectangleIntegrator.hh:64-160 and
+   Numeric/src/...RectangleIntegrator.cc:43-290: nodes Q = centre + x*(A/2)*N1 + y*(B/2)*N2
+   on [-1,1]^2 (Stroud 5-2 7-point, Tyler 12-point, 33-point degree-15 rule); weights sum 1. */
+int ko_rect_nodes(int n, const double* d, double* Q, double* w)
+{
+    double x[33], y[33];
+    int m = 0;
+    if (n == 7) {
+        const double t2 = sqrt((7. - sqrt(24.)) / 15.), s2 = sqrt((7. + sqrt(24.)) / 15.), r2 = sqrt(7. / 15.);
+        const double xs[7] = {0., r2, -r2, s2, -s2, t2, -t2};
+        const double ys[7] = {0., r2, -r2, -t2, t2, -s2, s2};
+        const double ws[7] = {2. / 7., 25. / 168., 25. / 168., 5. / 48., 5. / 48., 5. / 48., 5. / 48.};
+        for (m = 0; m < 7; m++) {
+            x[m] = xs[m];
+            y[m] = ys[m];
+            w[m] = ws[m];
+        }
+    } else if (n == 12) {
+        const double r = sqrt(6. / 7.), s = sqrt((114. - 3. * sqrt(583.)) / 287.),
+                     t = sqrt((114. + 3. * sqrt(583.)) / 287.);
+        const double xs[12] = {r, -r, 0., 0., s, s, -s, -s, t, t, -t, -t};
+        const double ys[12] = {0., 0., r, -r, s, -s, s, -s, t, -t, t, -t};
+        const double B1 = 49 / 810., B2 = (178981. + 2769. * sqrt(583)) / 1888920.,
+                     B3 = (178981. - 2769. * sqrt(583)) / 1888920.;
+        for (m = 0; m < 12; m++) {
+            x[m] = xs[m];
+            y[m] = ys[m];
+            w[m] = m < 4 ? B1 : m < 8 ? B2 : B3;
+        }
+    } else if (n == 33) {
+        static const double X[9] = {0.00000000000000000000, 0.77880971155441942252, 0.95729769978630736566,
+                                    0.13818345986246535375, 0.94132722587292523695, 0.47580862521827590507,
+                                    0.75580535657208143627, 0.69625007849174941396, 0.34271655604040678941};
+        static const double Y[9] = {0.00000000000000000000, 0.98348668243987226379, 0.85955600564163892859,
+                                    0.95892517028753485754, 0.39073621612946100068, 0.85007667369974857597,
+                                    0.64782163718701073204, 0.70741508996444936217e-1, 0.40930456169403884330};
+        static const double W[9] = {0.30038211543122536139 / 4.,  0.29991838864499131666e-1 / 4.,
+                                    0.38174421317083669640e-1 / 4., 0.60424923817749980681e-1 / 4.,
+                                    0.77492738533105339358e-1 / 4., 0.11884466730059560108 / 4.,
+                                    0.12976355037000271129 / 4.,  0.21334158145718938943 / 4.,
+                                    0.25687074948196783651 / 4.};
+        x[0] = X[0];
+        y[0] = Y[0];
+        w[0] = W[0];
+        m = 1;
+        /* each orbit: (X,Y), (-X,-Y), (-Y,X), (Y,-X) (.cc:250-257) */
+        for (int o = 1; o < 9; o++) {
+            double xo[4] = {X[o], -X[o], -Y[o], Y[o]}, yo[4] = {Y[o], -Y[o], X[o], -X[o]};
+            for (int q = 0; q < 4; q++) {
+                x[m] = xo[q];
+                y[m] = yo[q];
+                w[m] = W[o];
+                m++;
+            }
+        }
+    } else {
+        return 0;
+    }
+    const double a2 = 0.5 * d[0], b2 = 0.5 * d[1];
+    double cen[3];
+    for (int k = 0; k < 3; k++) cen[k] = d[2 + k] + a2 * d[5 + k] + b2 * d[8 + k];
+    for (int i = 0; i < m; i++)
+        for (int k = 0; k < 3; k++) Q[3 * i + k] = cen[k] + (x[i] * a2) * d[5 + k] + (y[i] * b2) * d[8 + k];
+    return m;
+}
+
+void ko_rect_cubature(int n, const double* d, const double P[3], double out[4])
+{
+    double Q[99], w[33];
+    int m = ko_rect_nodes(n, d, Q, w);
+    node_sum(m, Q, w, d[0] * d[1] * KO_K, P, out);
+}
+
+/* KElectrostaticCubatureRectangleIntegrator::ElectricFieldAndPotential: size=(A+B)/2,
+   centre P0 + 0.5*(A N1 + B N2), cut-offs 196.2 / 44.6 / 4.6 (.hh:52-54) */
+int ko_rect_numeric(const double* d, const double P[3], double out[4])
+{
+    double size = 0.5 * (d[0] + d[1]);
+    double c[3];
+    for (int k = 0; k < 3; k++) c[k] = d[2 + k] + 0.5 * (d[0] * d[5 + k] + d[1] * d[8 + k]);
+    double dv[3] = {c[0] - P[0], c[1] - P[1], c[2] - P[2]};
+    double dr = sqrt(SQ(dv[0]) + SQ(dv[1]) + SQ(dv[2])) * (1. / size);
+    int n = dr > 196.2 ? 7 : dr > 44.6 ? 12 : dr > 4.6 ? 33 : 0;
+    if (n)
+        ko_rect_cubature(n, d, P, out);
+    else
+        ko_rect_rwg(d, P, out);
+    return n;
+}
+
+/* ------------------------------------------------------------ line segment */
+/* Gauss-Legendre half-tables: Numeric/include/KElectrostaticQuadratureLineSegmentIntegrator.hh:59-83 */
+static const double GLx4[2] = {0.339981043584856, 0.861136311594053};
+static const double GLw4[2] = {0.652145154862546, 0.347854845137454};
+static const double GLx16[8] = {0.09501250983763744, 0.28160355077925891, 0.45801677765722739,
+                                0.61787624440264375, 0.75540440835500303, 0.86563120238783174,
+                                0.94457502307323258, 0.98940093499164993};
+static const double GLw16[8] = {0.189450610455068496, 0.182603415044923589, 0.169156519395002532,
+                                0.149595988816576731, 0.124628971255533872, 0.095158511682492785,
+                                0.062253523938647892, 0.027152459411754095};
+
+/* ElectricFieldAndPotential_nNodes (...LineSegmentIntegrator.cc:126-191): symmetric node pairs
+   about the midpoint, prefactor d/(4 eps0) */
+void ko_line_quadrature(int nn, const double* d, const double P[3], double out[4])
+{
+    const double* xs = nn == 4 ? GLx4 : GLx16;
+    const double* ws = nn == 4 ? GLw4 : GLw16;
+    int half = nn / 2;
+    double c[3] = {0.5 * (d[0] + d[3]), 0.5 * (d[1] + d[4]), 0.5 * (d[2] + d[5])};
+    double L = sqrt(SQ(d[3] - d[0]) + SQ(d[4] - d[1]) + SQ(d[5] - d[2]));
+    double hl = 0.5 * L, iu = 1. / L;
+    double u[3] = {iu * (d[3] - d[0]), iu * (d[4] - d[1]), iu * (d[5] - d[2])};
+    double sf[3] = {0., 0., 0.}, sp = 0.;
+    for (int i = 0; i < half; i++) {
+        double wf = ws[i] * hl, x = xs[i] * hl;
+        double pd[3], nd[3], pm = 0., nm = 0.;
+        for (int k = 0; k < 3; k++) {
+            pd[k] = P[k] - (c[k] + (u[k] * x));
+            pm += SQ(pd[k]);
+            nd[k] = P[k] - (c[k] - (u[k] * x));
+            nm += SQ(nd[k]);
+        }
+        pm = 1. / sqrt(pm);
+        nm = 1. / sqrt(nm);
+        sp += wf * (pm + nm);
+        pm = pm * pm * pm;
+        nm = nm * nm * nm;
+        for (int k = 0; k < 3; k++) sf[k] += wf * (pm * pd[k] + nm * nd[k]);
+    }
+    double pre = 0.25 * d[6] / KO_EPS0;
+    for (int k = 0; k < 3; k++) out[k] = pre * sf[k];
+    out[3] = pre * sp;
+}
+
+/* KElectrostaticAnalyticLineSegmentIntegrator::Potential / ElectricField (.cc:15-83):
+   ln((Da+Db+L)/(Da+Db-L)) with the inside-the-wire substitution */
+void ko_line_analytic(const double* d, const double P[3], double out[4])
+{
+    const double* p0 = d;
+    const double* p1 = d + 3;
+    double dia = d[6];
+    double l[3] = {p1[0] - p0[0], p1[1] - p0[1], p1[2] - p0[2]};
+    double L = norm3(l);
+    double a[3] = {p0[0] - P[0], p0[1] - P[1], p0[2] - P[2]};
+    double b[3] = {p1[0] - P[0], p1[1] - P[1], p1[2] - P[2]};
+    double Da = norm3(a), Db = norm3(b);
+    /* potential */
+    double DaP = Da, DbP = Db, DaE = Da, DbE = Db;
+    if (!((Da + Db) > (L + dia))) {
+        double u[3] = {l[0] / L, l[1] / L, l[2] / L};
+        double pp[3] = {P[0] - p0[0], P[1] - p0[1], P[2] - p0[2]};
+        double p = dot3(pp, u);
+        if (!(p < (-dia * .5) || p > (L + dia * .5))) {
+            double pf[3] = {p0[0] + p * u[0], p0[1] + p * u[1], p0[2] + p * u[2]};
+            double dd[3] = {P[0] - pf[0], P[1] - pf[1], P[2] - pf[2]};
+            if (!(norm3(dd) >= dia * .5)) {
+                double qa[3] = {p0[0] - pf[0], p0[1] - pf[1], p0[2] - pf[2]};
+                double qb[3] = {p1[0] - pf[0], p1[1] - pf[1], p1[2] - pf[2]};
+                double qa2 = SQ(qa[0]) + SQ(qa[1]) + SQ(qa[2]);
+                double qb2 = SQ(qb[0]) + SQ(qb[1]) + SQ(qb[2]);
+                DaP = sqrt(qa2 + dia * dia * .25);
+                DbP = sqrt(qb2 + dia * dia * .25);
+                DaE = norm3(qa);
+                DbE = norm3(qb);
+            }
+        }
+    }
+    double ln = log((DaP + DbP + L) / (DaP + DbP - L));
+    out[3] = dia / (4. * KO_EPS0) * ln;
+    double den = (DaE * (DaE + DbE + L) * (DaE + DbE - L) * DbE) * 4. * KO_EPS0;
+    den = -1. / den;
+    for (int k = 0; k < 3; k++)
+        out[k] = (2. * L * (p1[k] * DaE - P[k] * (DaE + DbE) + p0[k] * DbE)) * dia * den;
+}
+
+/* KElectrostaticQuadratureLineSegmentIntegrator::ElectricFieldAndPotential (.cc:266-310):
+   dr = |midpoint - P|/L; > 27.7 -> 4 nodes, > 2.0 -> 16 nodes, else analytic */
+int ko_line_numeric(const double* d, const double P[3], double out[4])
+{
+    double L = sqrt(((d[3] - d[0]) * (d[3] - d[0])) + ((d[4] - d[1]) * (d[4] - d[1])) +
+                    ((d[5] - d[2]) * (d[5] - d[2])));
+    double c[3] = {(d[0] + d[3]) * 0.5, (d[1] + d[4]) * 0.5, (d[2] + d[5]) * 0.5};
+    double dv[3] = {c[0] - P[0], c[1] - P[1], c[2] - P[2]};
+    double dr = sqrt(SQ(dv[0]) + SQ(dv[1]) + SQ(dv[2])) * (1. / L);
+    if (dr > 27.7) {
+        ko_line_quadrature(4, d, P, out);
+        return 4;
+    }
+    if (dr > 2.) {
+        ko_line_quadrature(16, d, P, out);
+        return 16;
+    }
+    ko_line_analytic(d, P, out);
+    return 0;
+}
+
+/* ------------------------------------------------------------ policies */
+/* KElectrostaticBoundaryIntegratorFactory.cc:29-88: numeric = cubature/RWG dispatch (default),
+   rwg = closed form everywhere (line: analytic). */
+int ko_element_eval(int policy, int shape, const double* d, const double P[3], double out[4])
+{
+    if (shape == KO_TRIANGLE) {
+        if (policy == KO_RWG) {
+            ko_tri_rwg(d, P, out);
+            return 0;
+        }
+        return ko_tri_numeric(d, P, out);
+    }
+    if (shape == KO_RECTANGLE) {
+        if (policy == KO_RWG) {
+            ko_rect_rwg(d, P, out);
+            return 0;
+        }
+        return ko_rect_numeric(d, P, out);
+    }
+    if (policy == KO_RWG) {
+        ko_line_analytic(d, P, out);
+        return 0;
+    }
+    return ko_line_numeric(d, P, out);
+}
+
+/* ------------------------------------------------------------ collocation */
+/* KElectrostaticBoundaryIntegrator::ComputeBoundaryIntegral (Core/include/...hh:260-280) with
+   the boundary visitor prefactor (1+flux)/(1-flux) for Neumann targets (Core/src/...cc:49-62) */
+double ko_matrix_element(int policy, int64_t i, int64_t j, const int32_t* shape, const double* recs,
+                         const int32_t* bc, const double* flux)
+{
+    const double* di = recs + KO_REC * i;
+    const double* dj = recs + KO_REC * j;
+    double ci[3], out[4];
+    ko_centroid(shape[i], di, ci);
+    if (bc == NULL || bc[i] == KO_DIRICHLET) {
+        ko_element_eval(policy, shape[j], dj, ci, out);
+        return out[3];
+    }
+    double cj[3];
+    ko_centroid(shape[j], dj, cj);
+    double dd[3] = {cj[0] - ci[0], cj[1] - ci[1], cj[2] - ci[2]};
+    if (norm3(dd) >= 1.e-12) {
+        double n[3];
+        ko_normal(shape[i], di, n);
+        ko_element_eval(policy, shape[j], dj, ci, out);
+        return out[0] * n[0] + out[1] * n[1] + out[2] * n[2];
+    }
+    double f = flux ? flux[i] : 0.;
+    return ((1. + f) / (1. - f)) / (2. * KO_EPS0);
+}
+
+void ko_assemble(int policy, int64_t n, const int32_t* shape, const double* recs, const int32_t* bc,
+                 const double* flux, double* A)
+{
+    for (int64_t i = 0; i < n; i++)
+        for (int64_t j = 0; j < n; j++) A[i * n + j] = ko_matrix_element(policy, i, j, shape, recs, bc, flux);
+}
+
+void ko_rhs(int64_t n, const int32_t* bc, const double* value, double* b)
+{
+    for (int64_t i = 0; i < n; i++) b[i] = (bc == NULL || bc[i] == KO_DIRICHLET) ? value[i] : 0.;
+}
+
+/* ------------------------------------------------------------ field */
+/* KIntegratingFieldSolver<...,ElectrostaticSingleThread>::ElectricFieldAndPotential
+   (KElectrostaticIntegratingFieldSolver.hh:303-339): Kahan sums in container order */
+void ko_field(int policy, int64_t n, const int32_t* shape, const double* recs, const double* sigma, int64_t m,
+              const double* P, double* phi, double* E)
+{
+    for (int64_t p = 0; p < m; p++) {
+        double s[4] = {0., 0., 0., 0.}, c[4] = {0., 0., 0., 0.};
+        for (int64_t j = 0; j < n; j++) {
+            double out[4];
+            ko_element_eval(policy, shape[j], recs + KO_REC * j, P + 3 * p, out);
+            for (int k = 0; k < 4; k++) {
+                double y = out[k] * sigma[j] - c[k];
+                double t = s[k] + y;
+                c[k] = (t - s[k]) - y;
+                s[k] = t;
+            }
+        }
+        if (phi) phi[p] = s[3];
+        if (E)
+            for (int k = 0; k < 3; k++) E[3 * p + k] = s[k];
+    }
+}
+
+/* ------------------------------------------------------------ solvers */
+static double dotn(int64_t n, const double* a, const double* b)
+{
+    double r = 0.;
+    for (int64_t i = 0; i < n; i++) r += a[i] * b[i];
+    return r;
+}
+
+static void matvec(int64_t n, const double* A, const double* x, double* y)
+{
+    for (int64_t i = 0; i < n; i++) {
+        double s = 0.;
+        for (int64_t j = 0; j < n; j++) s += A[i * n + j] * x[j];
+        y[i] = s;
+    }
+}
+
+/* GMRES: KGeneralizedMinimalResidual.hh:178-337 driven by KSimpleIterativeKrylovSolver::Solve
+   (:140-194).  Convergence: |p_{j+1}| / ||b|| <= tol.  The small least-squares system H y = p is
+   upper triangular after the Givens rotations; it is solved by back substitution (the
+   reference uses an SVD, which gives the same y to rounding for a non-singular H).
+   Restart: x <- x0 + V y (the reference's UpdateSolution writes V y alone). */
+int ko_gmres(int64_t n, const double* A, const double* b, double* x, double tol, int maxit, int restart,
+             double* resid, double* hist)
+{
+    int kmax = (restart > 0 && restart < maxit) ? restart : maxit;
+    double* V = (double*)malloc(sizeof(double) * (size_t)n * (size_t)(kmax + 1));
+    double* H = (double*)calloc((size_t)(kmax + 1) * (size_t)(kmax + 1), sizeof(double));
+    double* cs = (double*)malloc(sizeof(double) * (size_t)(kmax + 1));
+    double* sn = (double*)malloc(sizeof(double) * (size_t)(kmax + 1));
+    double* pv = (double*)malloc(sizeof(double) * (size_t)(kmax + 2));
+    double* w = (double*)malloc(sizeof(double) * (size_t)n);
+    double* y = (double*)malloc(sizeof(double) * (size_t)(kmax + 1));
+    double bnorm = sqrt(dotn(n, b, b));
+    int it = 0;
+    double rn = 0.;
+    int done = 0;
+    while (!done) {
+        /* ResetAndInitialize (:194-212) */
+        matvec(n, A, x, w);
+        for (int64_t i = 0; i < n; i++) w[i] = b[i] - w[i];
+        double nr = sqrt(dotn(n, w, w));
+        double inr = 1.0 / nr;
+        for (int64_t i = 0; i < n; i++) V[i] = w[i] * inr;
+        pv[0] = nr;
+        int j = 0;
+        for (;;) {
+            /* AugmentKrylovSubspace (:214-271): MGS then Givens */
+            double* vj1 = V + (size_t)(j + 1) * n;
+            matvec(n, A, V + (size_t)j * n, w);
+            double* h = H + (size_t)j * (kmax + 1);
+            for (int r = 0; r <= j; r++) {
+                double hh = dotn(n, w, V + (size_t)r * n);
+                h[r] = hh;
+                const double* vr = V + (size_t)r * n;
+                for (int64_t i = 0; i < n; i++) w[i] -= hh * vr[i];
+            }
+            double beta = sqrt(dotn(n, w, w));
+            double ib = 1.0 / beta;
+            for (int64_t i = 0; i < n; i++) vj1[i] = w[i] * ib;
+            for (int r = 0; r < j; r++) {
+                double a = h[r], bb = h[r + 1];
+                h[r] = cs[r] * a + sn[r] * bb;
+                h[r + 1] = -1.0 * sn[r] * a + cs[r] * bb;
+            }
+            double g = sqrt(h[j] * h[j] + beta * beta);
+            cs[j] = h[j] / g;
+            sn[j] = beta / g;
+            h[j] = g;
+            pv[j + 1] = -1.0 * sn[j] * pv[j];
+            pv[j] = cs[j] * pv[j];
+            rn = fabs(pv[j + 1]);
+            j++;
+            it++;
+            if (hist) hist[it - 1] = rn / bnorm;
+            int conv = (rn / bnorm) <= tol || it >= maxit;
+            if (conv || j >= kmax) {
+                /* UpdateSolution (:273-337): solve the triangular system */
+                for (int r = j - 1; r >= 0; r--) {
+                    double s = pv[r];
+                    for (int q = r + 1; q < j; q++) s -= H[(size_t)q * (kmax + 1) + r] * y[q];
+                    y[r] = s / H[(size_t)r * (kmax + 1) + r];
+                }
+                for (int r = 0; r < j; r++) {
+                    const double* vr = V + (size_t)r * n;
+                    for (int64_t i = 0; i < n; i++) x[i] += y[r] * vr[i];
+                }
+                if (conv) done = 1;
+                break;
+            }
+        }
+    }
+    if (resid) *resid = rn / bnorm;
+    free(V);
+    free(H);
+    free(cs);
+    free(sn);
+    free(pv);
+    free(w);
+    free(y);
+    return it;
+}
+
+/* Robin Hood: KRobinHood.hh:62-104 with KRobinHood_SingleThread.hh:53-120.
+   Every check_interval sub-iterations ||r||_inf/||b||_inf is compared with tol. */
+int64_t ko_robin_hood(int64_t n, const double* A, const double* b, double* x, double tol, int64_t check_interval,
+                      int64_t max_sub, double* resid)
+{
+    if (check_interval <= 0) check_interval = n;
+    double* bi = (double*)calloc((size_t)n, sizeof(double));
+    double* r = (double*)calloc((size_t)n, sizeof(double));
+    double xinf = 0., binf = 0.;
+    for (int64_t i = 0; i < n; i++) {
+        if (fabs(x[i]) > xinf) xinf = fabs(x[i]);
+        if (fabs(b[i]) > binf) binf = fabs(b[i]);
+    }
+    if (xinf > 1.e-16) matvec(n, A, x, bi);
+    double rnorm = 1.; /* KIterativeSolver: fResidualNorm(1.) */
+    int64_t sub = 0, total = 0;
+    while (rnorm > tol && total < max_sub) {
+        sub++;
+        total++;
+        for (int64_t i = 0; i < n; i++) r[i] = b[i] - bi[i];
+        if (sub % check_interval == 0) {
+            sub = 0;
+            double ri = 0.;
+            for (int64_t i = 0; i < n; i++)
+                if (fabs(r[i]) > ri) ri = fabs(r[i]);
+            rnorm = ri / binf;
+        }
+        int64_t im = 0;
+        double mx = fabs(r[0]);
+        for (int64_t i = 0; i < n; i++)
+            if (fabs(r[i]) > mx) {
+                mx = fabs(r[i]);
+                im = i;
+            }
+        double corr = (b[im] - bi[im]) / A[im * n + im];
+        x[im] += corr;
+        for (int64_t i = 0; i < n; i++) bi[i] += A[i * n + im] * corr;
+    }
+    if (resid) *resid = rnorm;
+    free(bi);
+    free(r);
+    return total;
+}
+
+/* KGaussianElimination::Solve (LinearAlgebra/Solvers/include/KGaussianElimination.hh:24-58):
+   Gauss-Jordan elimination on the augmented matrix without pivoting, x_i = a_in / a_ii */
+int ko_gauss(int64_t n, const double* A, const double* b, double* x)
+{
+    int64_t w = n + 1;
+    double* M = (double*)malloc(sizeof(double) * (size_t)n * (size_t)w);
+    for (int64_t i = 0; i < n; i++) {
+        for (int64_t j = 0; j < n; j++) M[i * w + j] = A[i * n + j];
+        M[i * w + n] = b[i];
+    }
+    for (int64_t col = 0; col < n; col++) {
+        for (int64_t row = 0; row < n; row++) {
+            if (row == col) continue;
+            double f = M[row * w + col] / M[col * w + col];
+            for (int64_t k = 0; k < w; k++) M[row * w + k] = M[row * w + k] - f * M[col * w + k];
+        }
+    }
+    for (int64_t i = 0; i < n; i++) x[i] = M[i * w + n] / M[i * w + i];
+    free(M);
+    return 0;
+}
