@@ -1,0 +1,187 @@
+#!/usr/bin/env python3
+"""Benchmark of the electrostatic BEM hot path on one MI355X.
+
+Workload (the benchmark configuration with a 20,000-triangle mesh and 1 M field points):
+two coaxial cylinders (r = 0.5 m and 1.0 m, length 4 m) with annular end caps, 20,000
+triangles with 1 % seeded tangential jitter; inner electrode -18.6 kV, outer electrode and
+caps 0 V.  Dense collocation matrix assembly (FP64 and FP32 storage), GMRES to 1e-10 relative
+residual in FP64 and with FP32-matrix inner solves + FP64 refinement, then potential + field
+at 1,000,000 points uniform in volume in the annular gap 0.5 < r < 1.0, |z| < 2.
+
+A step is one field-evaluation pass over the 1 M points with the FP64 charge densities
+(inputs resident in HBM); `value` is point-element pairs per second of that pass.  The solve
+(assembly + GMRES) is timed separately and reported beside it.
+
+Prints one JSON line.  --gpus N > 1 is refused: the library runs on one GPU.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+# FLOPs per (point, element) pair for field + potential, by regime (add/sub/mul = 1, FMA = 2,
+# sqrt/div/rsqrt/log/atan2 = 1): cubature 14 + 21 n_q; closed form 260 (convention);
+# segments 14 + 21 n_nodes, analytic 100.  Census order: triangle 7/12/33/closed,
+# rectangle 7/12/33/closed, segment 4/16/analytic.
+FLOPS_PER_PAIR = np.array([161, 266, 707, 260, 161, 266, 707, 260, 98, 350, 100], dtype=np.float64)
+PEAK_FP64_TFLOPS = 78.6  # MI355X FP64 vector peak
+PEAK_HBM_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--points", type=int, default=1_000_000)
+    ap.add_argument("--cpu-points", type=int, default=1200,
+                    help="points of the bounded CPU-baseline sample (x 20,000 elements)")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-solve", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    from kassiopeia_amd import geometry as G
+    from kassiopeia_amd import kemfield as K
+
+    mesh = G.coax_mesh()
+    pts = G.points_annulus(args.points, 0.5, 1.0, 2.0, seed=2024)
+    N, M = mesh.n, pts.shape[0]
+    log(f"mesh {N} triangles, {M} points")
+
+    ctx = K.Context(0)
+    # a dedicated (default-priority) stream shared by torch and the library, so that the HIP
+    # events below bracket exactly the library's kernels
+    stream = torch.cuda.Stream()
+    torch.cuda.set_stream(stream)
+    ctx.set_stream(stream.cuda_stream)
+    cont = K.KSurfaceContainer(ctx, mesh)
+    b = cont.rhs()
+
+    out = {}
+    # ---------------------------------------------------------------- solve
+    if not args.no_solve:
+        t0 = time.perf_counter()
+        A = K.KBoundaryIntegralMatrix(cont, fp64=True, fp32=True)
+        torch.cuda.synchronize()
+        t_asm = time.perf_counter() - t0
+        x64, st64, _ = A.solve(b, tolerance=1e-10)
+        xmx, stmx, _ = A.solve(b, tolerance=1e-10, precision="mixed")
+        x64, st64, _ = A.solve(b, tolerance=1e-10)  # second run: warm
+        xmx, stmx, _ = A.solve(b, tolerance=1e-10, precision="mixed")
+        A.close()
+        sig_err = float(np.linalg.norm(xmx - x64) / np.linalg.norm(x64))
+        log(f"assembly {t_asm*1e3:.1f} ms; GMRES fp64 {st64.iterations} it {st64.seconds*1e3:.1f} ms "
+            f"res {st64.true_residual:.2e}; mixed {stmx.iterations} it / {stmx.refinements} ref "
+            f"{stmx.seconds*1e3:.1f} ms res {stmx.true_residual:.2e}; sigma rel diff {sig_err:.2e}")
+        out.update(assembly_ms=round(t_asm * 1e3, 3), solve_ms_fp64=round(st64.seconds * 1e3, 3),
+                   solve_ms_mixed=round(stmx.seconds * 1e3, 3), gmres_iterations_fp64=st64.iterations,
+                   gmres_iterations_mixed=stmx.iterations, refinements_mixed=stmx.refinements,
+                   residual_fp64=st64.true_residual, residual_mixed=stmx.true_residual,
+                   sigma_mixed_vs_fp64=sig_err)
+        sigma = x64
+    else:
+        sigma = np.random.default_rng(3).normal(size=N) * 1e-7
+
+    # ---------------------------------------------------------------- field pass
+    dev = torch.device("cuda:0")
+    X = [torch.from_numpy(np.ascontiguousarray(pts[:, k])).to(dev) for k in range(3)]
+    S = torch.from_numpy(sigma).to(dev)
+    O = [torch.empty(M, dtype=torch.float64, device=dev) for _ in range(4)]
+    lib = ctx.lib
+
+    def ptr(t):
+        return t.data_ptr()
+
+    def run():
+        rc = lib.kem_field_eval_device(ctx.h, cont.h, ptr(S), M, ptr(X[0]), ptr(X[1]), ptr(X[2]), 0, ptr(O[0]),
+                                       ptr(O[1]), ptr(O[2]), ptr(O[3]))
+        if rc != 0:
+            raise RuntimeError(lib.kem_last_error(ctx.h).decode())
+
+    import ctypes
+
+    counts = (ctypes.c_int64 * 11)()
+    ctx.check(lib.kem_field_census(ctx.h, cont.h, M, ptr(X[0]), ptr(X[1]), ptr(X[2]), counts))
+    counts = np.array(counts[:], dtype=np.float64)
+    flops = float((counts * FLOPS_PER_PAIR).sum())
+    log(f"regime census (tri 7/12/33/closed): {counts[:4].astype(int).tolist()}")
+
+    torch.cuda.synchronize()
+    for _ in range(args.warmup):
+        run()
+    torch.cuda.synchronize()
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    ev0.record(stream)
+    for _ in range(args.steps):
+        run()
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    ms = ev0.elapsed_time(ev1) / args.steps
+    pairs = float(N) * M
+    value = pairs / (ms * 1e-3)
+    achieved = flops / (ms * 1e-3) / 1e12
+    log(f"field pass {ms:.2f} ms -> {value:.4e} pairs/s, {achieved:.2f} TFLOP/s FP64")
+    phi = O[0].cpu().numpy()
+    E = torch.stack(O[1:4], dim=1).cpu().numpy()
+
+    # ---------------------------------------------------------------- CPU baseline + parity
+    cpu = None
+    if not args.no_cpu_baseline and args.cpu_points > 0:
+        from oracle import oracle as OR
+
+        k = min(args.cpu_points, M)
+        t0 = time.perf_counter()
+        phr, Er = OR.field(mesh.shapes, mesh.recs, sigma, pts[:k])
+        tc = time.perf_counter() - t0
+        cpu = {"value": k * N / tc, "unit": "point-element pairs/s", "cores": 1, "kind": "port",
+               "sample": f"{k} points x {N} triangles of the same workload, FP64 oracle, 1 thread, {tc:.1f} s"}
+        ep = float(np.max(np.abs(phi[:k] - phr) / np.abs(phr)))
+        ee = float(np.max(np.linalg.norm(E[:k] - Er, axis=1) / np.linalg.norm(Er, axis=1)))
+        out.update(parity_phi_vs_oracle=ep, parity_E_vs_oracle=ee)
+        log(f"cpu baseline {cpu['value']:.3e} pairs/s; parity phi {ep:.2e} E {ee:.2e}")
+
+    line = {
+        "metric": "point-element field evals/s + solve time, mixed vs FP64 at 1e-6 rel. error",
+        "value": value,
+        "unit": "point-element pairs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"coaxial cylinders + annular caps, {N} triangles, GMRES 1e-10, "
+                               f"{M} field points in the gap (FP64 plain path)",
+                   "elements": N, "points": M},
+        # the kernel is FP64 VALU-bound; MI355X's FP64 vector and FP64 matrix peaks coincide (78.6)
+        "roofline": {"bound": "mfma", "compute": "fp64 vector ALU", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
+                     "kernel": "k_field_fp64", "flops_per_launch": flops,
+                     "pairs_by_regime": counts.astype(int).tolist()},
+        "cpu_baseline": cpu,
+    }
+    line.update(out)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

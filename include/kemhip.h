@@ -1,0 +1,158 @@
+/*
+ * kemhip.h -- C-ABI of the MI355X (gfx950) electrostatic boundary-element library.
+ *
+ * The drop-in boundary for KEMField's electrostatic BEM hot path.  Plain pointers and sizes
+ * only; no exceptions cross this boundary: every call returns KEM_OK (0), a positive warning
+ * or a negative KEM_E_* code, and kem_last_error() holds the message.  Each context owns (or
+ * borrows) one HIP stream; calls on one context are not thread-safe, separate contexts are
+ * independent.
+ *
+ * Each entry point names the KEMField interface it replaces (paths relative to
+ * KEMField/Source of KATRIN-Experiment/Kassiopeia).
+ *
+ * Element records ("shape data", 11 doubles each, KSurfaceContainer global order):
+ *   triangle / rectangle  [A, B, P0x, P0y, P0z, N1x, N1y, N1z, N2x, N2y, N2z]
+ *       (KTriangle / KRectangle; layout of Numeric/src/KElectrostaticCubatureTriangleIntegrator.cc:457-467)
+ *   line segment          [P0x, P0y, P0z, P1x, P1y, P1z, diameter, 0, 0, 0, 0]
+ *       (KLineSegment; Numeric/src/KElectrostaticQuadratureLineSegmentIntegrator.cc:197-203)
+ * Shape types: 0 triangle, 1 rectangle, 2 line segment.  Boundary types: 0 Dirichlet,
+ * 1 Neumann (KBoundary.hh:82-264).
+ */
+#ifndef KEMHIP_H
+#define KEMHIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes */
+#define KEM_OK 0
+#define KEM_W_NOT_CONVERGED 1 /* iteration or time limit reached; x holds the best iterate */
+#define KEM_E_ARG (-1)        /* invalid argument (null pointer, size, index) */
+#define KEM_E_HIP (-2)        /* HIP runtime error (allocation, launch, no device) */
+#define KEM_E_NAME (-3)       /* unknown integrator / solver name (KEMSimpleException) */
+#define KEM_E_GEOM (-4)       /* degenerate or non-finite element */
+#define KEM_E_STATE (-5)      /* object used with the wrong context / missing storage */
+
+/* ---- integrator policies (KElectrostaticBoundaryIntegratorFactory.cc:29-88) */
+#define KEM_POLICY_NUMERIC 0 /* default: cubature 7/12/33 by distance ratio, closed form near */
+#define KEM_POLICY_RWG 1     /* closed form at every distance (segments: analytic) */
+
+/* ---- precision modes */
+#define KEM_FP64 0  /* plain path: FP64 everywhere */
+#define KEM_MIXED 1 /* FP32 far-field cubature / FP32 matrix; FP64 near field, refinement, sums */
+
+/* ---- matrix storage flags for kem_matrix_assemble */
+#define KEM_STORE_FP64 1
+#define KEM_STORE_FP32 2
+
+/* ---- solvers (KKrylovSolverFactory.hh:40-199, KRobinHood.hh, KGaussianElimination.hh) */
+#define KEM_SOLVER_GMRES 0
+#define KEM_SOLVER_ROBIN_HOOD 1
+
+typedef struct kem_ctx kem_ctx;
+typedef struct kem_mesh kem_mesh;
+typedef struct kem_matrix kem_matrix;
+
+typedef struct kem_solve_opts {
+    int method;             /* KEM_SOLVER_* */
+    int precision;          /* KEM_FP64, or KEM_MIXED: FP32-matrix inner solves + FP64 refinement */
+    double tolerance;       /* relative: ||r||/||b|| (GMRES), ||r||_inf/||b||_inf (Robin Hood) */
+    int max_iterations;     /* GMRES iterations (all restarts), Robin Hood norm checks */
+    int restart;            /* GMRES iterations between restarts; 0 = never */
+    int64_t check_interval; /* Robin Hood sub-iterations between norm checks; 0 = N */
+    double time_limit_s;    /* wall-clock limit (KTimeTerminator); <= 0 = none */
+    double inner_tolerance; /* KEM_MIXED: relative tolerance of each FP32 inner solve */
+} kem_solve_opts;
+
+typedef struct kem_solve_stats {
+    int iterations;         /* GMRES iterations (sum over refinement steps), RH norm checks */
+    int64_t sub_iterations; /* Robin Hood element corrections */
+    int refinements;        /* KEM_MIXED: FP64 refinement steps */
+    double residual;        /* the solver's convergence measure at exit */
+    double true_residual;   /* ||b - A x|| / ||b|| recomputed with the FP64 matrix (if stored) */
+    double seconds;         /* wall time of the solve */
+} kem_solve_stats;
+
+/* ---- library / context */
+int kem_version(void);
+/* create a context on a HIP device with a stream of its own (default priority) */
+int kem_ctx_create(int device, kem_ctx** out);
+void kem_ctx_destroy(kem_ctx* ctx);
+/* run the context's work on a caller-owned HIP stream (hipStream_t); NULL restores its own */
+int kem_ctx_set_stream(kem_ctx* ctx, void* stream);
+void* kem_ctx_stream(kem_ctx* ctx);
+int kem_ctx_sync(kem_ctx* ctx);
+const char* kem_last_error(const kem_ctx* ctx);
+/* integrator policy by name: "numeric" | "default" | "rwg"  (Factory::Make(name), unknown -> KEM_E_NAME) */
+int kem_set_integrator(kem_ctx* ctx, const char* name);
+
+/* ---- surfaces: KSurfaceContainer of KSurface<KElectrostaticBasis, boundary, shape>
+   (Surfaces/src/KSurfaceContainer.cc:36-92).  Host arrays are copied.  bc_value: Dirichlet
+   voltage (KDirichletBoundary); bc_flux: Neumann normal boundary flux (KNeumannBoundary).
+   bc_type / bc_value / bc_flux may be NULL (all Dirichlet at 0 V). */
+int kem_mesh_create(kem_ctx* ctx, int64_t n, const int32_t* shape_type, const double* shape_data,
+                    const int32_t* bc_type, const double* bc_value, const double* bc_flux, kem_mesh** out);
+void kem_mesh_destroy(kem_mesh* mesh);
+int64_t kem_mesh_size(const kem_mesh* mesh);
+/* boundary values b_i (KBoundaryIntegralVector::operator(), BoundaryIntegrals/Core/include/
+   KBoundaryIntegralVector.hh:60-75): V_i for Dirichlet rows, 0 for Neumann rows */
+int kem_mesh_rhs(const kem_mesh* mesh, double* b);
+/* replace the Dirichlet values (e.g. another electrode voltage set) */
+int kem_mesh_set_values(kem_mesh* mesh, const double* bc_value);
+
+/* ---- element integrals: KElectrostaticBoundaryIntegrator::ElectricFieldAndPotential(shape, P)
+   (Electrostatic/Core/src/KElectrostaticBoundaryIntegrator.cc:101-312) for unit charge density.
+   Batched: pair k evaluates element elem[k] at point P[3k..3k+2] (host arrays);
+   out[4k..4k+3] = (Ex, Ey, Ez, phi). */
+int kem_element_eval(kem_ctx* ctx, const kem_mesh* mesh, int64_t npairs, const int64_t* elem, const double* P,
+                     double* out);
+
+/* ---- direct field: KIntegratingFieldSolver<KElectrostaticBoundaryIntegrator, Kind>::
+   ElectricFieldAndPotential (FieldSolvers/Integrating/include/KElectrostaticIntegratingFieldSolver.hh:303-339),
+   fused (one pass) and batched over points.  Host arrays, synchronous: P is npts x 3 (AoS),
+   sigma has mesh-size entries; phi (npts) and E (npts x 3) may be NULL to skip. */
+int kem_field_eval(kem_ctx* ctx, const kem_mesh* mesh, const double* sigma, int64_t npts, const double* P,
+                   int precision, double* phi, double* E);
+/* Same on device arrays (SoA point streams), asynchronous on the context's stream. */
+int kem_field_eval_device(kem_ctx* ctx, const kem_mesh* mesh, const double* d_sigma, int64_t npts,
+                          const double* d_x, const double* d_y, const double* d_z, int precision, double* d_phi,
+                          double* d_ex, double* d_ey, double* d_ez);
+
+/* Instrumentation (no reference counterpart): number of (point, element) pairs of a field
+   evaluation per quadrature regime of the numeric policy, counts[11] = triangle 7/12/33-point/
+   closed form, rectangle 7/12/33/closed form, segment 4-node/16-node/analytic.  Device points. */
+int kem_field_census(kem_ctx* ctx, const kem_mesh* mesh, int64_t npts, const double* d_x, const double* d_y,
+                     const double* d_z, int64_t* counts);
+
+/* ---- dense influence matrix: KBoundaryIntegralMatrix<KElectrostaticBoundaryIntegrator, true>
+   (BoundaryIntegrals/Core/include/KBoundaryIntegralMatrix.hh:16-99) as a device-resident
+   KSquareMatrix<double> (LinearAlgebra/Core/include/KMatrix.hh:9-111).  A(i, j) is the
+   collocation integral of source j at target i (Dirichlet: potential at the centroid of i;
+   Neumann: E.n_i, diagonal (1+f)/(1-f)/(2 eps0)).  storage: KEM_STORE_FP64 and/or
+   KEM_STORE_FP32. */
+int kem_matrix_assemble(kem_ctx* ctx, const kem_mesh* mesh, int storage, kem_matrix** out);
+void kem_matrix_destroy(kem_matrix* A);
+int64_t kem_matrix_dimension(const kem_matrix* A);
+/* KSquareMatrix::operator()(i, j) */
+int kem_matrix_get(kem_matrix* A, int64_t i, int64_t j, double* out);
+/* copy the whole FP64 matrix to host, row-major (n x n) */
+int kem_matrix_copy(kem_matrix* A, double* host_rowmajor);
+/* KSquareMatrix::Multiply / MultiplyTranspose (KMatrix.hh:92-111): y = A x or y = A^T x, host vectors */
+int kem_matrix_multiply(kem_matrix* A, int transpose, const double* x, double* y);
+
+/* ---- charge-density solve: KKrylovChargeDensitySolver / KRobinHoodChargeDensitySolver
+   (Interface/ChargeDensitySolvers/Electric/src/KKrylovChargeDensitySolver.cc:42-78,
+   .../KRobinHoodChargeDensitySolver.cc:83-212) over KSimpleIterativeKrylovSolver<GMRES>
+   (LinearAlgebra/Solvers/include/KSimpleIterativeKrylovSolver.hh:140-194) and KRobinHood
+   (KRobinHood.hh:62-104).  x holds the start vector on entry (KBoundaryIntegralSolutionVector)
+   and the solution on exit.  Returns KEM_OK, or KEM_W_NOT_CONVERGED at the iteration/time limit. */
+void kem_solve_opts_default(kem_solve_opts* opts);
+int kem_solve(kem_matrix* A, const kem_solve_opts* opts, const double* b, double* x, kem_solve_stats* stats);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,717 @@
+// kem_capi.cpp -- C-ABI of the MI355X electrostatic BEM library (include/kemhip.h).
+//
+// Host orchestration in C++: context/stream ownership, mesh upload, matrix storage and the
+// iterative solvers (GMRES, Robin Hood, FP32-matrix solves with FP64 iterative refinement),
+// all driving the HIP kernels of kem_kernels.hip on device-resident data.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/kemhip.h"
+#include "kem_kernels.h"
+#include "kem_layout.h"
+#include "kem_prep.h"
+
+struct kem_ctx {
+    int device = 0;
+    hipStream_t own = nullptr;
+    hipStream_t stream = nullptr;
+    int policy = KEM_POLICY_NUMERIC;
+    std::string err;
+    double* ws = nullptr;  // field partial sums / gemv partials
+    int64_t ws_doubles = 0;
+    double* scal = nullptr;  // small device scalars (dot products, Hessenberg column)
+    int64_t scal_doubles = 0;
+};
+
+struct kem_mesh {
+    kem_ctx* ctx = nullptr;
+    int64_t n = 0;
+    double* geom = nullptr;
+    double* nodes = nullptr;
+    double* targets = nullptr;
+    std::vector<int32_t> shape, bc;
+    std::vector<double> value, flux;
+    bool has_neumann = false;
+};
+
+struct kem_matrix {
+    kem_ctx* ctx = nullptr;
+    int64_t n = 0;
+    double* A64 = nullptr;
+    float* A32 = nullptr;
+};
+
+namespace {
+
+int fail(kem_ctx* ctx, int code, const std::string& msg)
+{
+    if (ctx) ctx->err = msg;
+    return code;
+}
+
+#define KEM_CK(ctx, expr)                                                                               \
+    do {                                                                                                \
+        hipError_t e_ = (expr);                                                                         \
+        if (e_ != hipSuccess)                                                                           \
+            return fail((ctx), KEM_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));           \
+    } while (0)
+
+template <typename T>
+hipError_t dalloc(T** p, int64_t count)
+{
+    *p = nullptr;
+    if (count <= 0) return hipSuccess;
+    return hipMalloc((void**)p, sizeof(T) * (size_t)count);
+}
+
+template <typename T>
+void dfree(T*& p)
+{
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
+hipError_t ensure_ws(kem_ctx* c, int64_t doubles)
+{
+    if (c->ws_doubles >= doubles) return hipSuccess;
+    dfree(c->ws);
+    c->ws_doubles = 0;
+    hipError_t e = dalloc(&c->ws, doubles);
+    if (e == hipSuccess) c->ws_doubles = doubles;
+    return e;
+}
+
+hipError_t ensure_scal(kem_ctx* c, int64_t doubles)
+{
+    if (c->scal_doubles >= doubles) return hipSuccess;
+    dfree(c->scal);
+    c->scal_doubles = 0;
+    hipError_t e = dalloc(&c->scal, doubles);
+    if (e == hipSuccess) c->scal_doubles = doubles;
+    return e;
+}
+
+// RAII device vector for solver temporaries
+struct DVec {
+    double* p = nullptr;
+    ~DVec() { dfree(p); }
+};
+
+double now_s()
+{
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+}  // namespace
+
+extern "C" {
+
+int kem_version(void) { return 1; }
+
+int kem_ctx_create(int device, kem_ctx** out)
+{
+    if (!out) return KEM_E_ARG;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return KEM_E_HIP;
+    if (device < 0 || device >= ndev) return KEM_E_ARG;
+    if (hipSetDevice(device) != hipSuccess) return KEM_E_HIP;
+    kem_ctx* c = new kem_ctx();
+    c->device = device;
+    if (hipStreamCreateWithFlags(&c->own, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return KEM_E_HIP;
+    }
+    c->stream = c->own;
+    *out = c;
+    return KEM_OK;
+}
+
+void kem_ctx_destroy(kem_ctx* c)
+{
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    dfree(c->ws);
+    dfree(c->scal);
+    if (c->own) (void)hipStreamDestroy(c->own);
+    delete c;
+}
+
+int kem_ctx_set_stream(kem_ctx* c, void* stream)
+{
+    if (!c) return KEM_E_ARG;
+    c->stream = stream ? (hipStream_t)stream : c->own;
+    return KEM_OK;
+}
+
+void* kem_ctx_stream(kem_ctx* c) { return c ? (void*)c->stream : nullptr; }
+
+int kem_ctx_sync(kem_ctx* c)
+{
+    if (!c) return KEM_E_ARG;
+    KEM_CK(c, hipSetDevice(c->device));
+    KEM_CK(c, hipStreamSynchronize(c->stream));
+    return KEM_OK;
+}
+
+const char* kem_last_error(const kem_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int kem_set_integrator(kem_ctx* c, const char* name)
+{
+    if (!c || !name) return KEM_E_ARG;
+    std::string s(name);
+    if (s == "numeric" || s == "default")
+        c->policy = KEM_POLICY_NUMERIC;
+    else if (s == "rwg")
+        c->policy = KEM_POLICY_RWG;
+    else
+        return fail(c, KEM_E_NAME, "unknown integrator '" + s + "' (known: numeric, default, rwg)");
+    return KEM_OK;
+}
+
+// ------------------------------------------------------------------ mesh
+int kem_mesh_create(kem_ctx* c, int64_t n, const int32_t* shape, const double* data, const int32_t* bc,
+                    const double* value, const double* flux, kem_mesh** out)
+{
+    if (!c || !out) return KEM_E_ARG;
+    *out = nullptr;
+    if (n <= 0 || !shape || !data) return fail(c, KEM_E_ARG, "mesh: empty or null arrays");
+    KEM_CK(c, hipSetDevice(c->device));
+    std::vector<double> g((size_t)n * KG_STRIDE), nd((size_t)n * KEM_NODE_STRIDE), t((size_t)n * KT_STRIDE);
+    kem_mesh* m = new kem_mesh();
+    m->ctx = c;
+    m->n = n;
+    m->shape.assign(shape, shape + n);
+    m->bc.assign((size_t)n, 0);
+    m->value.assign((size_t)n, 0.);
+    m->flux.assign((size_t)n, 0.);
+    for (int64_t i = 0; i < n; i++) {
+        if (bc) m->bc[i] = bc[i];
+        if (value) m->value[i] = value[i];
+        if (flux) m->flux[i] = flux[i];
+        if (m->bc[i] != 0 && m->bc[i] != 1) {
+            delete m;
+            return fail(c, KEM_E_ARG, "mesh: boundary type must be 0 (Dirichlet) or 1 (Neumann)");
+        }
+        if (m->bc[i] == 1) m->has_neumann = true;
+        if (kem::prep_element(shape[i], data + 11 * i, g.data() + (size_t)i * KG_STRIDE,
+                              nd.data() + (size_t)i * KEM_NODE_STRIDE) != 0) {
+            delete m;
+            return fail(c, KEM_E_GEOM, "mesh: degenerate or non-finite element " + std::to_string(i));
+        }
+        kem::prep_target(shape[i], data + 11 * i, m->bc[i], m->flux[i], t.data() + (size_t)i * KT_STRIDE);
+    }
+    hipError_t e = dalloc(&m->geom, (int64_t)g.size());
+    if (e == hipSuccess) e = dalloc(&m->nodes, (int64_t)nd.size());
+    if (e == hipSuccess) e = dalloc(&m->targets, (int64_t)t.size());
+    if (e == hipSuccess) e = hipMemcpy(m->geom, g.data(), sizeof(double) * g.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(m->nodes, nd.data(), sizeof(double) * nd.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(m->targets, t.data(), sizeof(double) * t.size(), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        kem_mesh_destroy(m);
+        return fail(c, KEM_E_HIP, std::string("mesh upload: ") + hipGetErrorString(e));
+    }
+    *out = m;
+    return KEM_OK;
+}
+
+void kem_mesh_destroy(kem_mesh* m)
+{
+    if (!m) return;
+    if (m->ctx) (void)hipSetDevice(m->ctx->device);
+    dfree(m->geom);
+    dfree(m->nodes);
+    dfree(m->targets);
+    delete m;
+}
+
+int64_t kem_mesh_size(const kem_mesh* m) { return m ? m->n : 0; }
+
+int kem_mesh_rhs(const kem_mesh* m, double* b)
+{
+    if (!m || !b) return KEM_E_ARG;
+    for (int64_t i = 0; i < m->n; i++) b[i] = m->bc[i] == 0 ? m->value[i] : 0.;
+    return KEM_OK;
+}
+
+int kem_mesh_set_values(kem_mesh* m, const double* value)
+{
+    if (!m || !value) return KEM_E_ARG;
+    m->value.assign(value, value + m->n);
+    return KEM_OK;
+}
+
+// ------------------------------------------------------------------ evaluation
+int kem_element_eval(kem_ctx* c, const kem_mesh* m, int64_t np, const int64_t* elem, const double* P, double* out)
+{
+    if (!c || !m || (np > 0 && (!elem || !P || !out))) return KEM_E_ARG;
+    if (m->ctx != c) return fail(c, KEM_E_STATE, "mesh belongs to another context");
+    if (np <= 0) return KEM_OK;
+    for (int64_t k = 0; k < np; k++)
+        if (elem[k] < 0 || elem[k] >= m->n) return fail(c, KEM_E_ARG, "element index out of range");
+    KEM_CK(c, hipSetDevice(c->device));
+    int64_t* de = nullptr;
+    double *dP = nullptr, *dO = nullptr;
+    hipError_t e = dalloc(&de, np);
+    if (e == hipSuccess) e = dalloc(&dP, 3 * np);
+    if (e == hipSuccess) e = dalloc(&dO, 4 * np);
+    if (e == hipSuccess) e = hipMemcpyAsync(de, elem, sizeof(int64_t) * np, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(dP, P, sizeof(double) * 3 * np, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = kem::launch_pairs(c->stream, c->policy, m->geom, m->nodes, np, de, dP, dO);
+    if (e == hipSuccess) e = hipMemcpyAsync(out, dO, sizeof(double) * 4 * np, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    dfree(de);
+    dfree(dP);
+    dfree(dO);
+    if (e != hipSuccess) return fail(c, KEM_E_HIP, std::string("element_eval: ") + hipGetErrorString(e));
+    return KEM_OK;
+}
+
+int kem_field_eval_device(kem_ctx* c, const kem_mesh* m, const double* sigma, int64_t npts, const double* x,
+                          const double* y, const double* z, int precision, double* phi, double* ex, double* ey,
+                          double* ez)
+{
+    if (!c || !m || !sigma || (npts > 0 && (!x || !y || !z))) return KEM_E_ARG;
+    if (m->ctx != c) return fail(c, KEM_E_STATE, "mesh belongs to another context");
+    if (precision != KEM_FP64) return fail(c, KEM_E_ARG, "field: precision mode not available");
+    if (npts <= 0) return KEM_OK;
+    KEM_CK(c, hipSetDevice(c->device));
+    const int64_t need = 4 * (int64_t)(2048 + 256) * 256;
+    KEM_CK(c, ensure_ws(c, need));
+    KEM_CK(c, kem::launch_field_fp64(c->stream, c->policy, m->geom, m->nodes, sigma, m->n, x, y, z, npts, phi, ex,
+                                     ey, ez, c->ws, c->ws_doubles));
+    return KEM_OK;
+}
+
+int kem_field_census(kem_ctx* c, const kem_mesh* m, int64_t npts, const double* x, const double* y, const double* z,
+                     int64_t* counts)
+{
+    if (!c || !m || !counts || (npts > 0 && (!x || !y || !z))) return KEM_E_ARG;
+    KEM_CK(c, hipSetDevice(c->device));
+    unsigned long long* d = nullptr;
+    KEM_CK(c, dalloc(&d, 12));
+    hipError_t e = hipMemsetAsync(d, 0, sizeof(unsigned long long) * 12, c->stream);
+    if (e == hipSuccess) e = kem::launch_census(c->stream, m->geom, m->n, x, y, z, npts, d);
+    unsigned long long h[12];
+    if (e == hipSuccess) e = hipMemcpyAsync(h, d, sizeof(h), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    dfree(d);
+    if (e != hipSuccess) return fail(c, KEM_E_HIP, std::string("census: ") + hipGetErrorString(e));
+    for (int k = 0; k < 11; k++) counts[k] = (int64_t)h[k];
+    return KEM_OK;
+}
+
+int kem_field_eval(kem_ctx* c, const kem_mesh* m, const double* sigma, int64_t npts, const double* P,
+                   int precision, double* phi, double* E)
+{
+    if (!c || !m || !sigma || (npts > 0 && !P)) return KEM_E_ARG;
+    if (npts <= 0) return KEM_OK;
+    KEM_CK(c, hipSetDevice(c->device));
+    std::vector<double> soa((size_t)npts * 3);
+    for (int64_t i = 0; i < npts; i++)
+        for (int k = 0; k < 3; k++) soa[(size_t)k * npts + i] = P[3 * i + k];
+    double *dS = nullptr, *dP = nullptr, *dO = nullptr;
+    hipError_t e = dalloc(&dS, m->n);
+    if (e == hipSuccess) e = dalloc(&dP, 3 * npts);
+    if (e == hipSuccess) e = dalloc(&dO, 4 * npts);
+    if (e == hipSuccess) e = hipMemcpyAsync(dS, sigma, sizeof(double) * m->n, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(dP, soa.data(), sizeof(double) * 3 * npts, hipMemcpyHostToDevice, c->stream);
+    int rc = KEM_OK;
+    if (e == hipSuccess) {
+        rc = kem_field_eval_device(c, m, dS, npts, dP, dP + npts, dP + 2 * npts, precision, dO, dO + npts,
+                                   dO + 2 * npts, dO + 3 * npts);
+        if (rc == KEM_OK) {
+            std::vector<double> o((size_t)npts * 4);
+            e = hipMemcpyAsync(o.data(), dO, sizeof(double) * 4 * npts, hipMemcpyDeviceToHost, c->stream);
+            if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+            if (e == hipSuccess) {
+                // device layout: phi, Ex, Ey, Ez planes of npts each
+                for (int64_t i = 0; i < npts; i++) {
+                    if (phi) phi[i] = o[(size_t)i];
+                    if (E)
+                        for (int k = 0; k < 3; k++) E[3 * i + k] = o[(size_t)(k + 1) * npts + i];
+                }
+            }
+        }
+    }
+    dfree(dS);
+    dfree(dP);
+    dfree(dO);
+    if (rc != KEM_OK) return rc;
+    if (e != hipSuccess) return fail(c, KEM_E_HIP, std::string("field_eval: ") + hipGetErrorString(e));
+    return KEM_OK;
+}
+
+// ------------------------------------------------------------------ matrix
+int kem_matrix_assemble(kem_ctx* c, const kem_mesh* m, int storage, kem_matrix** out)
+{
+    if (!c || !m || !out) return KEM_E_ARG;
+    *out = nullptr;
+    if (m->ctx != c) return fail(c, KEM_E_STATE, "mesh belongs to another context");
+    if (!(storage & (KEM_STORE_FP64 | KEM_STORE_FP32))) return fail(c, KEM_E_ARG, "matrix: no storage selected");
+    KEM_CK(c, hipSetDevice(c->device));
+    kem_matrix* A = new kem_matrix();
+    A->ctx = c;
+    A->n = m->n;
+    const int64_t nn = m->n * m->n;
+    hipError_t e = hipSuccess;
+    if (storage & KEM_STORE_FP64) e = dalloc(&A->A64, nn);
+    if (e == hipSuccess && (storage & KEM_STORE_FP32)) e = dalloc(&A->A32, nn);
+    if (e == hipSuccess)
+        e = kem::launch_assemble(c->stream, c->policy, m->has_neumann, m->geom, m->nodes, m->targets, m->n, A->A64,
+                                 A->A32);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) {
+        kem_matrix_destroy(A);
+        return fail(c, KEM_E_HIP, std::string("assemble: ") + hipGetErrorString(e));
+    }
+    *out = A;
+    return KEM_OK;
+}
+
+void kem_matrix_destroy(kem_matrix* A)
+{
+    if (!A) return;
+    if (A->ctx) (void)hipSetDevice(A->ctx->device);
+    dfree(A->A64);
+    dfree(A->A32);
+    delete A;
+}
+
+int64_t kem_matrix_dimension(const kem_matrix* A) { return A ? A->n : 0; }
+
+int kem_matrix_get(kem_matrix* A, int64_t i, int64_t j, double* out)
+{
+    if (!A || !out) return KEM_E_ARG;
+    kem_ctx* c = A->ctx;
+    if (i < 0 || j < 0 || i >= A->n || j >= A->n) return fail(c, KEM_E_ARG, "matrix index out of range");
+    KEM_CK(c, hipSetDevice(c->device));
+    const int64_t off = j * A->n + i;
+    if (A->A64) {
+        KEM_CK(c, hipMemcpy(out, A->A64 + off, sizeof(double), hipMemcpyDeviceToHost));
+    } else {
+        float f;
+        KEM_CK(c, hipMemcpy(&f, A->A32 + off, sizeof(float), hipMemcpyDeviceToHost));
+        *out = f;
+    }
+    return KEM_OK;
+}
+
+int kem_matrix_copy(kem_matrix* A, double* host)
+{
+    if (!A || !host) return KEM_E_ARG;
+    kem_ctx* c = A->ctx;
+    if (!A->A64) return fail(c, KEM_E_STATE, "matrix has no FP64 storage");
+    KEM_CK(c, hipSetDevice(c->device));
+    const int64_t n = A->n;
+    std::vector<double> cm((size_t)(n * n));
+    KEM_CK(c, hipMemcpy(cm.data(), A->A64, sizeof(double) * n * n, hipMemcpyDeviceToHost));
+    for (int64_t i = 0; i < n; i++)
+        for (int64_t j = 0; j < n; j++) host[i * n + j] = cm[(size_t)(j * n + i)];
+    return KEM_OK;
+}
+
+namespace {
+
+// y = A x (or A^T x) on device with the matrix's FP64 storage (or FP32 when use32)
+hipError_t matvec_dev(kem_matrix* A, bool use32, int transpose, const double* x, double* y)
+{
+    kem_ctx* c = A->ctx;
+    const int64_t n = A->n;
+    if (transpose) {
+        if (use32) return kem::launch_gemv_t<float>(c->stream, A->A32, n, n, n, x, y);
+        return kem::launch_gemv_t<double>(c->stream, A->A64, n, n, n, x, y);
+    }
+    hipError_t e = ensure_ws(c, (int64_t)kem::gemv_splits(n, n) * n);
+    if (e != hipSuccess) return e;
+    if (use32) return kem::launch_gemv<float>(c->stream, A->A32, n, n, n, x, 1.0, 0.0, y, c->ws);
+    return kem::launch_gemv<double>(c->stream, A->A64, n, n, n, x, 1.0, 0.0, y, c->ws);
+}
+
+}  // namespace
+
+int kem_matrix_multiply(kem_matrix* A, int transpose, const double* x, double* y)
+{
+    if (!A || !x || !y) return KEM_E_ARG;
+    kem_ctx* c = A->ctx;
+    KEM_CK(c, hipSetDevice(c->device));
+    const int64_t n = A->n;
+    DVec dx, dy;
+    KEM_CK(c, dalloc(&dx.p, n));
+    KEM_CK(c, dalloc(&dy.p, n));
+    KEM_CK(c, hipMemcpyAsync(dx.p, x, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+    KEM_CK(c, matvec_dev(A, A->A64 == nullptr, transpose, dx.p, dy.p));
+    KEM_CK(c, hipMemcpyAsync(y, dy.p, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+    KEM_CK(c, hipStreamSynchronize(c->stream));
+    return KEM_OK;
+}
+
+// ------------------------------------------------------------------ solvers
+void kem_solve_opts_default(kem_solve_opts* o)
+{
+    if (!o) return;
+    o->method = KEM_SOLVER_GMRES;
+    o->precision = KEM_FP64;
+    o->tolerance = 1e-8;  // KIterativeSolver default (KIterativeSolver.hh:14)
+    o->max_iterations = 1 << 30;
+    o->restart = 0;
+    o->check_interval = 0;
+    o->time_limit_s = 0.;
+    o->inner_tolerance = 1e-5;
+}
+
+namespace {
+
+struct SolveCtl {
+    double t0 = 0., limit = 0.;
+    bool expired() const { return limit > 0. && now_s() - t0 > limit; }
+};
+
+// GMRES with Givens rotations and a triangular least-squares solve
+// (KGeneralizedMinimalResidual.hh:178-337), convergence |p_{j+1}| / ||b|| <= tol
+// (KSimpleIterativeKrylovSolver.hh:84-103).  The Arnoldi step orthogonalises with classical
+// Gram-Schmidt plus one reorthogonalisation pass (CGS2): two GEMV-T / GEMV pairs over the
+// device-resident basis per iteration instead of 2(j+1) dependent dot/axpy launches, and
+// orthogonality to working precision (the reference's MGS loses it on BEM matrices, so its
+// iteration counts near the tolerance are rounding-dependent).  Device vectors d_b, d_x
+// (x holds x0).  On restart (or at the basis cap) x <- x0 + V y.
+int gmres_dev(kem_matrix* A, bool use32, const double* d_b, double* d_x, double tol, int maxit, int restart,
+              const SolveCtl& ctl, int* iters, double* resid)
+{
+    kem_ctx* c = A->ctx;
+    hipStream_t st = c->stream;
+    const int64_t n = A->n;
+    int kmax = (int)std::min<int64_t>(std::min<int64_t>(maxit, 1000), n + 1);
+    if (restart > 0) kmax = std::min(kmax, restart);
+    if (kmax < 1) kmax = 1;
+    DVec V, w;
+    KEM_CK(c, dalloc(&V.p, (int64_t)(kmax + 1) * n));
+    KEM_CK(c, dalloc(&w.p, n));
+    KEM_CK(c, ensure_scal(c, 256 + 2 * ((int64_t)kmax + 8)));
+    KEM_CK(c, ensure_ws(c, (int64_t)std::max(kem::gemv_splits(n, kmax + 1), kem::gemv_splits(n, n)) * n));
+    double* d_part = c->scal;                 // 256 dot partials
+    double* d_h1 = c->scal + 256;             // first-pass projections + ||w|| (kmax + 2)
+    double* d_h2 = d_h1 + (kmax + 8);         // second-pass projections
+    const int ld = kmax + 1;
+    std::vector<double> H((size_t)ld * ld), cs(kmax + 1), sn(kmax + 1), p(kmax + 2), y(kmax + 1), h2(kmax + 1);
+    double bnorm;
+    KEM_CK(c, kem::launch_dot(st, d_b, d_b, n, d_part, d_h1, true));
+    KEM_CK(c, hipMemcpyAsync(&bnorm, d_h1, sizeof(double), hipMemcpyDeviceToHost, st));
+    KEM_CK(c, hipStreamSynchronize(st));
+    if (bnorm == 0.) bnorm = 1.;
+    int it = 0;
+    double rn = 0.;
+    int status = KEM_OK;
+    for (;;) {
+        // r = b - A x ; v0 = r / ||r||
+        KEM_CK(c, matvec_dev(A, use32, 0, d_x, w.p));
+        KEM_CK(c, kem::launch_axpby(st, 1.0, d_b, -1.0, w.p, w.p, n));
+        KEM_CK(c, kem::launch_dot(st, w.p, w.p, n, d_part, d_h1, true));
+        KEM_CK(c, kem::launch_div_dev(st, d_h1, w.p, V.p, n));
+        KEM_CK(c, hipMemcpyAsync(&p[0], d_h1, sizeof(double), hipMemcpyDeviceToHost, st));
+        KEM_CK(c, hipStreamSynchronize(st));
+        rn = p[0];
+        if (!(rn > 0.)) break;  // x is already exact
+        int j = 0;
+        bool conv = false, stop = false;
+        for (;;) {
+            double* vj = V.p + (int64_t)j * n;
+            double* vj1 = V.p + (int64_t)(j + 1) * n;
+            KEM_CK(c, matvec_dev(A, use32, 0, vj, w.p));
+            // CGS2: h1 = V^T w, w -= V h1; h2 = V^T w, w -= V h2; h = h1 + h2
+            KEM_CK(c, kem::launch_gemv_t<double>(st, V.p, n, n, j + 1, w.p, d_h1));
+            KEM_CK(c, kem::launch_gemv<double>(st, V.p, n, n, j + 1, d_h1, -1.0, 1.0, w.p, c->ws));
+            KEM_CK(c, kem::launch_gemv_t<double>(st, V.p, n, n, j + 1, w.p, d_h2));
+            KEM_CK(c, kem::launch_gemv<double>(st, V.p, n, n, j + 1, d_h2, -1.0, 1.0, w.p, c->ws));
+            KEM_CK(c, kem::launch_dot(st, w.p, w.p, n, d_part, d_h1 + j + 1, true));
+            KEM_CK(c, kem::launch_div_dev(st, d_h1 + j + 1, w.p, vj1, n));
+            double* h = H.data() + (size_t)j * ld;
+            KEM_CK(c, hipMemcpyAsync(h, d_h1, sizeof(double) * (j + 2), hipMemcpyDeviceToHost, st));
+            KEM_CK(c, hipMemcpyAsync(h2.data(), d_h2, sizeof(double) * (j + 1), hipMemcpyDeviceToHost, st));
+            KEM_CK(c, hipStreamSynchronize(st));
+            for (int r = 0; r <= j; r++) h[r] += h2[r];
+            const double beta = h[j + 1];
+            for (int r = 0; r < j; r++) {
+                const double a = h[r], bb = h[r + 1];
+                h[r] = cs[r] * a + sn[r] * bb;
+                h[r + 1] = -1.0 * sn[r] * a + cs[r] * bb;
+            }
+            const double gam = std::sqrt(h[j] * h[j] + beta * beta);
+            cs[j] = h[j] / gam;
+            sn[j] = beta / gam;
+            h[j] = gam;
+            p[j + 1] = -1.0 * sn[j] * p[j];
+            p[j] = cs[j] * p[j];
+            rn = std::fabs(p[j + 1]);
+            j++;
+            it++;
+            conv = (rn / bnorm) <= tol;
+            stop = conv || it >= maxit || ctl.expired() || !(beta > 0.);
+            if (stop || j >= kmax) break;
+        }
+        // x += V y with H y = p (upper triangular)
+        for (int r = j - 1; r >= 0; r--) {
+            double s = p[r];
+            for (int q = r + 1; q < j; q++) s -= H[(size_t)q * ld + r] * y[q];
+            y[r] = s / H[(size_t)r * ld + r];
+        }
+        KEM_CK(c, hipMemcpyAsync(d_h1, y.data(), sizeof(double) * j, hipMemcpyHostToDevice, st));
+        KEM_CK(c, kem::launch_gemv<double>(st, V.p, n, n, j, d_h1, 1.0, 1.0, d_x, c->ws));
+        KEM_CK(c, hipStreamSynchronize(st));
+        if (conv) break;
+        if (stop) {
+            status = KEM_W_NOT_CONVERGED;
+            break;
+        }
+    }
+    *iters = it;
+    *resid = rn / bnorm;
+    return status;
+}
+
+// ||b - A64 x|| / ||b|| on device
+int true_residual(kem_matrix* A, const double* d_b, const double* d_x, double* d_tmp, double* out)
+{
+    kem_ctx* c = A->ctx;
+    hipStream_t st = c->stream;
+    const int64_t n = A->n;
+    KEM_CK(c, ensure_scal(c, 260));
+    KEM_CK(c, matvec_dev(A, A->A64 == nullptr, 0, d_x, d_tmp));
+    KEM_CK(c, kem::launch_axpby(st, 1.0, d_b, -1.0, d_tmp, d_tmp, n));
+    KEM_CK(c, kem::launch_dot(st, d_tmp, d_tmp, n, c->scal, c->scal + 256, true));
+    KEM_CK(c, kem::launch_dot(st, d_b, d_b, n, c->scal, c->scal + 257, true));
+    double v[2];
+    KEM_CK(c, hipMemcpyAsync(v, c->scal + 256, sizeof(double) * 2, hipMemcpyDeviceToHost, st));
+    KEM_CK(c, hipStreamSynchronize(st));
+    *out = v[0] / (v[1] > 0. ? v[1] : 1.);
+    return KEM_OK;
+}
+
+int robin_hood_dev(kem_matrix* A, const double* d_b, double* d_x, const std::vector<double>& hb,
+                   const std::vector<double>& hx, const kem_solve_opts* o, const SolveCtl& ctl, kem_solve_stats* s)
+{
+    kem_ctx* c = A->ctx;
+    hipStream_t st = c->stream;
+    const int64_t n = A->n;
+    if (!A->A64) return fail(c, KEM_E_STATE, "Robin Hood needs FP64 matrix storage");
+    DVec bi;
+    KEM_CK(c, dalloc(&bi.p, n));
+    KEM_CK(c, hipMemsetAsync(bi.p, 0, sizeof(double) * n, st));
+    double xinf = 0., binf = 0.;
+    for (int64_t i = 0; i < n; i++) {
+        xinf = std::max(xinf, std::fabs(hx[i]));
+        binf = std::max(binf, std::fabs(hb[i]));
+    }
+    if (xinf > 1.e-16) KEM_CK(c, matvec_dev(A, false, 0, d_x, bi.p));
+    if (binf == 0.) binf = 1.;
+    const int64_t interval = o->check_interval > 0 ? o->check_interval : n;
+    KEM_CK(c, ensure_scal(c, 260));
+    double rnorm = 1.;  // KIterativeSolver::fResidualNorm starts at 1
+    int checks = 0;
+    int status = KEM_OK;
+    while (rnorm > o->tolerance) {
+        if (checks >= o->max_iterations || ctl.expired()) {
+            status = KEM_W_NOT_CONVERGED;
+            break;
+        }
+        KEM_CK(c, kem::launch_robin_hood(st, A->A64, d_b, d_x, bi.p, n, interval, c->scal + 256));
+        double v;
+        KEM_CK(c, hipMemcpyAsync(&v, c->scal + 256, sizeof(double), hipMemcpyDeviceToHost, st));
+        KEM_CK(c, hipStreamSynchronize(st));
+        rnorm = v / binf;
+        checks++;
+    }
+    s->iterations = checks;
+    s->sub_iterations = (int64_t)checks * interval;
+    s->residual = rnorm;
+    return status;
+}
+
+}  // namespace
+
+int kem_solve(kem_matrix* A, const kem_solve_opts* opts, const double* b, double* x, kem_solve_stats* stats)
+{
+    if (!A || !b || !x) return KEM_E_ARG;
+    kem_ctx* c = A->ctx;
+    kem_solve_opts o;
+    kem_solve_opts_default(&o);
+    if (opts) o = *opts;
+    kem_solve_stats s{};
+    if (o.max_iterations <= 0) return fail(c, KEM_E_ARG, "max_iterations must be positive");
+    if (!(o.tolerance > 0.)) return fail(c, KEM_E_ARG, "tolerance must be positive");
+    KEM_CK(c, hipSetDevice(c->device));
+    const int64_t n = A->n;
+    SolveCtl ctl;
+    ctl.t0 = now_s();
+    ctl.limit = o.time_limit_s;
+    DVec db, dx, dr, dd;
+    KEM_CK(c, dalloc(&db.p, n));
+    KEM_CK(c, dalloc(&dx.p, n));
+    KEM_CK(c, hipMemcpyAsync(db.p, b, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+    KEM_CK(c, hipMemcpyAsync(dx.p, x, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+    int status = KEM_OK;
+    if (o.method == KEM_SOLVER_ROBIN_HOOD) {
+        std::vector<double> hb(b, b + n), hx(x, x + n);
+        status = robin_hood_dev(A, db.p, dx.p, hb, hx, &o, ctl, &s);
+    } else if (o.method == KEM_SOLVER_GMRES) {
+        if (o.precision == KEM_FP64) {
+            if (!A->A64) return fail(c, KEM_E_STATE, "FP64 solve needs FP64 matrix storage");
+            status = gmres_dev(A, false, db.p, dx.p, o.tolerance, o.max_iterations, o.restart, ctl, &s.iterations,
+                               &s.residual);
+        } else {
+            // FP32-matrix GMRES inner solves, FP64 residual refinement
+            if (!A->A32) return fail(c, KEM_E_STATE, "mixed solve needs FP32 matrix storage");
+            if (!A->A64) return fail(c, KEM_E_STATE, "mixed solve needs FP64 matrix storage for the residual");
+            KEM_CK(c, dalloc(&dr.p, n));
+            KEM_CK(c, dalloc(&dd.p, n));
+            double rel = 0.;
+            for (int ref = 0; ref < 50; ref++) {
+                int rc = true_residual(A, db.p, dx.p, dr.p, &rel);
+                if (rc != KEM_OK) return rc;
+                // dr holds r = b - A64 x
+                if (rel <= o.tolerance) break;
+                if (s.iterations >= o.max_iterations || ctl.expired()) {
+                    status = KEM_W_NOT_CONVERGED;
+                    break;
+                }
+                KEM_CK(c, hipMemsetAsync(dd.p, 0, sizeof(double) * n, c->stream));
+                int its = 0;
+                double res = 0.;
+                const double inner = std::max(o.inner_tolerance, 0.5 * o.tolerance / std::max(rel, 1e-300));
+                rc = gmres_dev(A, true, dr.p, dd.p, std::min(inner, 0.5), o.max_iterations - s.iterations, 0, ctl,
+                               &its, &res);
+                if (rc < 0) return rc;
+                s.iterations += its;
+                s.refinements++;
+                KEM_CK(c, kem::launch_axpby(c->stream, 1.0, dx.p, 1.0, dd.p, dx.p, n));
+            }
+            s.residual = rel;
+            if (rel > o.tolerance) status = KEM_W_NOT_CONVERGED;
+        }
+    } else {
+        return fail(c, KEM_E_NAME, "unknown solver method");
+    }
+    if (status < 0) return status;
+    KEM_CK(c, hipMemcpyAsync(x, dx.p, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+    if (A->A64) {
+        DVec tmp;
+        KEM_CK(c, dalloc(&tmp.p, n));
+        int rc = true_residual(A, db.p, dx.p, tmp.p, &s.true_residual);
+        if (rc != KEM_OK) return rc;
+    }
+    KEM_CK(c, hipStreamSynchronize(c->stream));
+    s.seconds = now_s() - ctl.t0;
+    if (stats) *stats = s;
+    return status;
+}
+
+}  // extern "C"

@@ -1,0 +1,54 @@
+// kem_kernels.h -- host launchers of the HIP kernels (kem_kernels.hip).
+#ifndef KEM_KERNELS_H
+#define KEM_KERNELS_H
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace kem {
+
+// phi/E (any may be null) at npts points (SoA) from nelem elements with charge densities sigma.
+// ws: scratch of ws_doubles doubles for element-range partial sums.
+hipError_t launch_field_fp64(hipStream_t st, int policy, const double* geom, const double* nodes,
+                             const double* sigma, int64_t nelem, const double* px, const double* py, const double* pz,
+                             int64_t npts, double* phi, double* ex, double* ey, double* ez, double* ws,
+                             int64_t ws_doubles);
+
+// pairs per quadrature regime (11 counters, see k_census)
+hipError_t launch_census(hipStream_t st, const double* geom, int64_t nelem, const double* px, const double* py,
+                         const double* pz, int64_t npts, unsigned long long* counts);
+
+hipError_t launch_pairs(hipStream_t st, int policy, const double* geom, const double* nodes, int64_t npairs,
+                        const int64_t* elem, const double* P, double* out);
+
+hipError_t launch_assemble(hipStream_t st, int policy, bool field, const double* geom, const double* nodes,
+                           const double* targets, int64_t n, double* A64, float* A32);
+
+int gemv_splits(int64_t nrows, int64_t ncols);
+// y = alpha * A x + beta * y, A column-major nrows x ncols (leading dimension lda);
+// part: scratch of gemv_splits(nrows, ncols) * nrows doubles
+template <typename T>
+hipError_t launch_gemv(hipStream_t st, const T* A, int64_t lda, int64_t nrows, int64_t ncols, const double* x,
+                       double alpha, double beta, double* y, double* part);
+// y = A^T x
+template <typename T>
+hipError_t launch_gemv_t(hipStream_t st, const T* A, int64_t lda, int64_t nrows, int64_t ncols, const double* x,
+                         double* y);
+
+// *out = a.b (or its square root); part: scratch of 256 doubles
+hipError_t launch_dot(hipStream_t st, const double* a, const double* b, int64_t n, double* part, double* out,
+                      bool want_sqrt);
+// y += sign * (*alpha) * x
+hipError_t launch_axpy_dev(hipStream_t st, const double* alpha, double sign, const double* x, double* y, int64_t n);
+// y = x * (1 / *d)
+hipError_t launch_div_dev(hipStream_t st, const double* d, const double* x, double* y, int64_t n);
+// z = a x + b y
+hipError_t launch_axpby(hipStream_t st, double a, const double* x, double b, const double* y, double* z, int64_t n);
+
+hipError_t launch_robin_hood(hipStream_t st, const double* A, const double* b, double* x, double* bi, int64_t n,
+                             int64_t steps, double* rnorm);
+
+}  // namespace kem
+
+#endif

@@ -1,0 +1,519 @@
+// kem_kernels.hip -- HIP kernels of the electrostatic BEM path for gfx950 (MI355X).
+//
+// Field evaluation and matrix assembly map one field point / collocation target to each lane
+// and stream the element records wave-uniformly (scalar loads), so a 64-lane wavefront
+// evaluates one element against 64 points per step; lanes diverge only on the quadrature
+// rule.  Dense linear algebra is HBM-bound streaming over column-major storage.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "kem_device.hpp"
+#include "kem_kernels.h"
+
+namespace kem {
+
+using dev::F4;
+
+namespace {
+
+constexpr int kBlock = 256;
+
+// ------------------------------------------------------------------ field evaluation
+// One lane per field point.  Elements [j0, j1) are summed in container order with Kahan
+// compensation per component (KElectrostaticIntegratingFieldSolver.hh:303-339).  With
+// nsplit > 1 each blockIdx.y handles one contiguous element range and writes its partial
+// sums; k_field_combine adds the partials in range order.
+template <int POLICY>
+__global__ __launch_bounds__(kBlock) void k_field_fp64(const double* __restrict__ geom,
+                                                       const double* __restrict__ nodes,
+                                                       const double* __restrict__ sigma, int64_t nelem,
+                                                       int64_t per_split, const double* __restrict__ px,
+                                                       const double* __restrict__ py, const double* __restrict__ pz,
+                                                       int64_t npts, double* __restrict__ out0,
+                                                       double* __restrict__ out1, double* __restrict__ out2,
+                                                       double* __restrict__ out3, int64_t out_stride)
+{
+    int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const bool live = i < npts;
+    if (!live) i = npts - 1;
+    const double x = px[i], y = py[i], z = pz[i];
+    const int64_t j0 = (int64_t)blockIdx.y * per_split;
+    int64_t j1 = j0 + per_split;
+    if (j1 > nelem) j1 = nelem;
+    double s0 = 0., s1 = 0., s2 = 0., s3 = 0.;
+    double c0 = 0., c1 = 0., c2 = 0., c3 = 0.;
+    for (int64_t j = j0; j < j1; j++) {
+        const F4 v = dev::element_eval<true>(geom + j * KG_STRIDE, nodes + j * KEM_NODE_STRIDE, POLICY, x, y, z);
+        const double sj = sigma[j];
+        double yk, t;
+        yk = v.x * sj - c0;
+        t = s0 + yk;
+        c0 = (t - s0) - yk;
+        s0 = t;
+        yk = v.y * sj - c1;
+        t = s1 + yk;
+        c1 = (t - s1) - yk;
+        s1 = t;
+        yk = v.z * sj - c2;
+        t = s2 + yk;
+        c2 = (t - s2) - yk;
+        s2 = t;
+        yk = v.p * sj - c3;
+        t = s3 + yk;
+        c3 = (t - s3) - yk;
+        s3 = t;
+    }
+    if (live) {
+        const int64_t o = (int64_t)blockIdx.y * out_stride + i;
+        if (out0) out0[o] = s0;
+        if (out1) out1[o] = s1;
+        if (out2) out2[o] = s2;
+        if (out3) out3[o] = s3;
+    }
+}
+
+// Kahan-sum the per-range partials (ws[c][s][i], c = Ex, Ey, Ez, phi) in range order.
+__global__ __launch_bounds__(kBlock) void k_field_combine(const double* __restrict__ ws, int nsplit, int64_t npts,
+                                                          double* __restrict__ phi, double* __restrict__ ex,
+                                                          double* __restrict__ ey, double* __restrict__ ez)
+{
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= npts) return;
+    double* outs[4] = {ex, ey, ez, phi};
+    for (int c = 0; c < 4; c++) {
+        if (!outs[c]) continue;
+        const double* p = ws + (int64_t)c * nsplit * npts;
+        double s = 0., comp = 0.;
+        for (int k = 0; k < nsplit; k++) {
+            const double yk = p[(int64_t)k * npts + i] - comp;
+            const double t = s + yk;
+            comp = (t - s) - yk;
+            s = t;
+        }
+        outs[c][i] = s;
+    }
+}
+
+// ------------------------------------------------------------------ element pairs
+__global__ __launch_bounds__(kBlock) void k_pairs(const double* __restrict__ geom, const double* __restrict__ nodes,
+                                                  int policy, int64_t npairs, const int64_t* __restrict__ elem,
+                                                  const double* __restrict__ P, double* __restrict__ out)
+{
+    const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (k >= npairs) return;
+    const int64_t j = elem[k];
+    const F4 v = dev::element_eval<true>(geom + j * KG_STRIDE, nodes + j * KEM_NODE_STRIDE, policy, P[3 * k],
+                                         P[3 * k + 1], P[3 * k + 2]);
+    out[4 * k + 0] = v.x;
+    out[4 * k + 1] = v.y;
+    out[4 * k + 2] = v.z;
+    out[4 * k + 3] = v.p;
+}
+
+// ------------------------------------------------------------------ assembly
+// Lane = collocation target i (rows), wave-uniform source j (columns).  Column-major storage:
+// A(i, j) at A[j*n + i], so the 64 lanes of a wavefront write 64 consecutive entries.
+// KElectrostaticBoundaryIntegrator::ComputeBoundaryIntegral (Core/include/...hh:260-280).
+template <bool FIELD>
+__global__ __launch_bounds__(kBlock) void k_assemble(const double* __restrict__ geom,
+                                                     const double* __restrict__ nodes,
+                                                     const double* __restrict__ targets, int policy, int64_t n,
+                                                     int64_t per_split, double* __restrict__ A64,
+                                                     float* __restrict__ A32)
+{
+    int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const bool live = i < n;
+    if (!live) i = n - 1;
+    const double* t = targets + i * KT_STRIDE;
+    const double cx = t[KT_CX], cy = t[KT_CY], cz = t[KT_CZ];
+    const double nx = t[KT_NX], ny = t[KT_NY], nz = t[KT_NZ];
+    const bool dirichlet = t[KT_BC] == 0.;
+    const double f = t[KT_FLUX];
+    const double diag = ((1. + f) / (1. - f)) / (2. * dev::kEps0);
+    const int64_t j0 = (int64_t)blockIdx.y * per_split;
+    int64_t j1 = j0 + per_split;
+    if (j1 > n) j1 = n;
+    for (int64_t j = j0; j < j1; j++) {
+        const F4 v = dev::element_eval<FIELD>(geom + j * KG_STRIDE, nodes + j * KEM_NODE_STRIDE, policy, cx, cy, cz);
+        double a;
+        if (!FIELD || dirichlet) {
+            a = v.p;
+        } else {
+            const double* tj = targets + j * KT_STRIDE;
+            const double dx = tj[KT_CX] - cx, dy = tj[KT_CY] - cy, dz = tj[KT_CZ] - cz;
+            a = (sqrt(dx * dx + dy * dy + dz * dz) >= 1.e-12) ? (v.x * nx + v.y * ny + v.z * nz) : diag;
+        }
+        if (live) {
+            if (A64) A64[j * n + i] = a;
+            if (A32) A32[j * n + i] = (float)a;
+        }
+    }
+}
+
+// ------------------------------------------------------------------ dense linear algebra
+// y_part[s][i] = sum_{j in split s} A[j*lda + i] * x[j]   (column-major, rows i)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_gemv_part(const T* __restrict__ A, int64_t lda, int64_t nrows,
+                                                      int64_t ncols, int64_t per_split, const double* __restrict__ x,
+                                                      double* __restrict__ part)
+{
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const int64_t j0 = (int64_t)blockIdx.y * per_split;
+    int64_t j1 = j0 + per_split;
+    if (j1 > ncols) j1 = ncols;
+    if (i >= nrows) return;
+    double a0 = 0., a1 = 0., a2 = 0., a3 = 0.;
+    int64_t j = j0;
+    for (; j + 4 <= j1; j += 4) {
+        a0 += (double)A[(j + 0) * lda + i] * x[j + 0];
+        a1 += (double)A[(j + 1) * lda + i] * x[j + 1];
+        a2 += (double)A[(j + 2) * lda + i] * x[j + 2];
+        a3 += (double)A[(j + 3) * lda + i] * x[j + 3];
+    }
+    for (; j < j1; j++) a0 += (double)A[j * lda + i] * x[j];
+    part[(int64_t)blockIdx.y * nrows + i] = (a0 + a1) + (a2 + a3);
+}
+
+// y[i] = alpha * sum_s part[s][i] + beta * y[i]
+__global__ __launch_bounds__(kBlock) void k_gemv_reduce(const double* __restrict__ part, int nsplit, int64_t nrows,
+                                                        double alpha, double beta, double* __restrict__ y)
+{
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= nrows) return;
+    double s = 0.;
+    for (int k = 0; k < nsplit; k++) s += part[(int64_t)k * nrows + i];
+    y[i] = (beta == 0. ? 0. : beta * y[i]) + alpha * s;
+}
+
+__device__ __forceinline__ double block_sum(double v, double* sh)
+{
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) sh[w] = v;
+    __syncthreads();
+    double r = 0.;
+    if (threadIdx.x == 0)
+        for (int k = 0; k < (int)(blockDim.x >> 6); k++) r += sh[k];
+    __syncthreads();
+    return r;
+}
+
+// y[j] = sum_i A[j*lda + i] * x[i]  (one block per column; y = A^T x)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_gemv_t(const T* __restrict__ A, int64_t lda, int64_t nrows,
+                                                   const double* __restrict__ x, double* __restrict__ y)
+{
+    __shared__ double sh[kBlock / 64];
+    const int64_t j = blockIdx.x;
+    const T* col = A + j * lda;
+    double s = 0.;
+    for (int64_t i = threadIdx.x; i < nrows; i += kBlock) s += (double)col[i] * x[i];
+    s = block_sum(s, sh);
+    if (threadIdx.x == 0) y[j] = s;
+}
+
+// partial dot products: part[b] = sum over the block's grid-stride slice
+__global__ __launch_bounds__(kBlock) void k_dot_part(const double* __restrict__ a, const double* __restrict__ b,
+                                                     int64_t n, double* __restrict__ part)
+{
+    __shared__ double sh[kBlock / 64];
+    double s = 0.;
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
+        s += a[i] * b[i];
+    s = block_sum(s, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+// out = sum part (sqrt if want_sqrt), single block
+__global__ __launch_bounds__(kBlock) void k_dot_final(const double* __restrict__ part, int nparts, int want_sqrt,
+                                                      double* __restrict__ out)
+{
+    __shared__ double sh[kBlock / 64];
+    double s = 0.;
+    for (int k = threadIdx.x; k < nparts; k += kBlock) s += part[k];
+    s = block_sum(s, sh);
+    if (threadIdx.x == 0) *out = want_sqrt ? sqrt(s) : s;
+}
+
+// y += sign * (*alpha) * x
+__global__ __launch_bounds__(kBlock) void k_axpy_dev(const double* __restrict__ alpha, double sign,
+                                                     const double* __restrict__ x, double* __restrict__ y, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) y[i] -= (-sign * (*alpha)) * x[i];
+}
+
+// y = x / (*d)
+__global__ __launch_bounds__(kBlock) void k_div_dev(const double* __restrict__ d, const double* __restrict__ x,
+                                                    double* __restrict__ y, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) y[i] = x[i] * (1.0 / (*d));
+}
+
+// z = a*x + b*y
+__global__ __launch_bounds__(kBlock) void k_axpby(double a, const double* __restrict__ x, double b,
+                                                  const double* __restrict__ y, double* __restrict__ z, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) z[i] = a * x[i] + b * y[i];
+}
+
+// ------------------------------------------------------------------ Robin Hood
+// `steps` Southwell sub-iterations of KRobinHood (KRobinHood.hh:76-97 with
+// KRobinHood_SingleThread.hh:78-108) in one workgroup: r = b - b_iter, i = first argmax |r_i|,
+// x_i += r_i / A_ii, b_iter += A(:, i) r_i / A_ii.  At the last step ||r||_inf (before its
+// update, as the reference checks it) is written to *rnorm.
+constexpr int kRHBlock = 1024;
+__global__ __launch_bounds__(kRHBlock) void k_robin_hood(const double* __restrict__ A, const double* __restrict__ b,
+                                                         double* __restrict__ x, double* __restrict__ bi, int64_t n,
+                                                         int64_t steps, double* __restrict__ rnorm)
+{
+    __shared__ double sv[kRHBlock / 64];
+    __shared__ int64_t si[kRHBlock / 64];
+    __shared__ double s_corr;
+    __shared__ int64_t s_idx;
+    for (int64_t s = 0; s < steps; s++) {
+        double best = -1.;
+        int64_t bidx = n;
+        for (int64_t i = threadIdx.x; i < n; i += kRHBlock) {
+            const double r = fabs(b[i] - bi[i]);
+            if (r > best) {
+                best = r;
+                bidx = i;
+            }
+        }
+        for (int o = 32; o > 0; o >>= 1) {
+            const double ov = __shfl_down(best, o, 64);
+            const int64_t oi = __shfl_down(bidx, o, 64);
+            if (ov > best || (ov == best && oi < bidx)) {
+                best = ov;
+                bidx = oi;
+            }
+        }
+        const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+        if (lane == 0) {
+            sv[w] = best;
+            si[w] = bidx;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double bv = sv[0];
+            int64_t bx = si[0];
+            for (int k = 1; k < kRHBlock / 64; k++)
+                if (sv[k] > bv || (sv[k] == bv && si[k] < bx)) {
+                    bv = sv[k];
+                    bx = si[k];
+                }
+            // no lane found a residual > -1 (NaN residuals): the reference starts from index 0
+            if (bx >= n) bx = 0;
+            if (s == steps - 1) *rnorm = bv;
+            const double corr = (b[bx] - bi[bx]) / A[bx * n + bx];
+            x[bx] += corr;
+            s_corr = corr;
+            s_idx = bx;
+        }
+        __syncthreads();
+        const double corr = s_corr;
+        const double* col = A + s_idx * n;
+        for (int64_t i = threadIdx.x; i < n; i += kRHBlock) bi[i] += col[i] * corr;
+        __syncthreads();
+    }
+}
+
+// ------------------------------------------------------------------ regime census
+// Counts the (point, element) pairs of a field evaluation by quadrature regime (numeric
+// policy): counts[0..3] triangle 7/12/33-point/closed form, [4..7] rectangle, [8..10] segment
+// 4-node/16-node/analytic.  Used to price a launch in FLOPs.
+__global__ __launch_bounds__(kBlock) void k_census(const double* __restrict__ geom, int64_t nelem, int64_t per_split,
+                                                   const double* __restrict__ px, const double* __restrict__ py,
+                                                   const double* __restrict__ pz, int64_t npts,
+                                                   unsigned long long* __restrict__ counts)
+{
+    __shared__ unsigned long long sc[12];
+    if (threadIdx.x < 12) sc[threadIdx.x] = 0;
+    __syncthreads();
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    unsigned long long c[12] = {0};
+    if (i < npts) {
+        const double x = px[i], y = py[i], z = pz[i];
+        const int64_t j0 = (int64_t)blockIdx.y * per_split;
+        int64_t j1 = j0 + per_split;
+        if (j1 > nelem) j1 = nelem;
+        for (int64_t j = j0; j < j1; j++) {
+            const double* g = geom + j * KG_STRIDE;
+            const int type = (int)g[KG_TYPE];
+            const double dr = dev::distance_ratio(g, x, y, z);
+            int k;
+            if (type == KEM_TRIANGLE)
+                k = dr > dev::kTriCub12 ? 0 : dr > dev::kTriCub33 ? 1 : dr > dev::kTriRWG ? 2 : 3;
+            else if (type == KEM_RECTANGLE)
+                k = dr > dev::kRectCub12 ? 4 : dr > dev::kRectCub33 ? 5 : dr > dev::kRectRWG ? 6 : 7;
+            else
+                k = dr > dev::kLineQuad16 ? 8 : dr > dev::kLineAna ? 9 : 10;
+            c[k]++;
+        }
+    }
+    for (int k = 0; k < 11; k++)
+        if (c[k]) atomicAdd(&sc[k], c[k]);
+    __syncthreads();
+    if (threadIdx.x < 11 && sc[threadIdx.x]) atomicAdd(&counts[threadIdx.x], sc[threadIdx.x]);
+}
+
+inline unsigned nblocks(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+
+}  // namespace
+
+// ====================================================================== launchers
+hipError_t launch_field_fp64(hipStream_t st, int policy, const double* geom, const double* nodes,
+                             const double* sigma, int64_t nelem, const double* px, const double* py, const double* pz,
+                             int64_t npts, double* phi, double* ex, double* ey, double* ez, double* ws,
+                             int64_t ws_doubles)
+{
+    if (npts <= 0) return hipSuccess;
+    const unsigned gx = nblocks(npts);
+    // enough blocks to fill 256 CUs several times over; each range >= 32 elements
+    int64_t nsplit = (2048 + gx - 1) / gx;
+    if (nsplit > (nelem + 31) / 32) nsplit = (nelem + 31) / 32;
+    if (nsplit < 1) nsplit = 1;
+    while (nsplit > 1 && 4 * nsplit * npts > ws_doubles) nsplit--;
+    const int64_t per = (nelem + nsplit - 1) / nsplit;
+    nsplit = (nelem + per - 1) / per;
+    dim3 grid(gx, (unsigned)nsplit);
+    if (nsplit == 1) {
+        if (policy == 0)
+            k_field_fp64<0><<<grid, kBlock, 0, st>>>(geom, nodes, sigma, nelem, per, px, py, pz, npts, ex, ey, ez,
+                                                     phi, 0);
+        else
+            k_field_fp64<1><<<grid, kBlock, 0, st>>>(geom, nodes, sigma, nelem, per, px, py, pz, npts, ex, ey, ez,
+                                                     phi, 0);
+        return hipGetLastError();
+    }
+    double* w0 = ws;
+    double* w1 = ws + nsplit * npts;
+    double* w2 = ws + 2 * nsplit * npts;
+    double* w3 = ws + 3 * nsplit * npts;
+    if (policy == 0)
+        k_field_fp64<0><<<grid, kBlock, 0, st>>>(geom, nodes, sigma, nelem, per, px, py, pz, npts, w0, w1, w2, w3,
+                                                 npts);
+    else
+        k_field_fp64<1><<<grid, kBlock, 0, st>>>(geom, nodes, sigma, nelem, per, px, py, pz, npts, w0, w1, w2, w3,
+                                                 npts);
+    k_field_combine<<<gx, kBlock, 0, st>>>(ws, (int)nsplit, npts, phi, ex, ey, ez);
+    return hipGetLastError();
+}
+
+hipError_t launch_census(hipStream_t st, const double* geom, int64_t nelem, const double* px, const double* py,
+                         const double* pz, int64_t npts, unsigned long long* counts)
+{
+    if (npts <= 0 || nelem <= 0) return hipSuccess;
+    const int64_t per = 4096;
+    dim3 grid(nblocks(npts), (unsigned)((nelem + per - 1) / per));
+    k_census<<<grid, kBlock, 0, st>>>(geom, nelem, per, px, py, pz, npts, counts);
+    return hipGetLastError();
+}
+
+hipError_t launch_pairs(hipStream_t st, int policy, const double* geom, const double* nodes, int64_t npairs,
+                        const int64_t* elem, const double* P, double* out)
+{
+    if (npairs <= 0) return hipSuccess;
+    k_pairs<<<nblocks(npairs), kBlock, 0, st>>>(geom, nodes, policy, npairs, elem, P, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_assemble(hipStream_t st, int policy, bool field, const double* geom, const double* nodes,
+                           const double* targets, int64_t n, double* A64, float* A32)
+{
+    if (n <= 0) return hipSuccess;
+    const unsigned gx = nblocks(n);
+    int64_t nsplit = (4096 + gx - 1) / gx;
+    if (nsplit > n) nsplit = n;
+    const int64_t per = (n + nsplit - 1) / nsplit;
+    nsplit = (n + per - 1) / per;
+    dim3 grid(gx, (unsigned)nsplit);
+    if (field)
+        k_assemble<true><<<grid, kBlock, 0, st>>>(geom, nodes, targets, policy, n, per, A64, A32);
+    else
+        k_assemble<false><<<grid, kBlock, 0, st>>>(geom, nodes, targets, policy, n, per, A64, A32);
+    return hipGetLastError();
+}
+
+int gemv_splits(int64_t nrows, int64_t ncols)
+{
+    const int64_t gx = (nrows + kBlock - 1) / kBlock;
+    int64_t s = (2048 + gx - 1) / gx;
+    if (s > (ncols + 15) / 16) s = (ncols + 15) / 16;
+    if (s < 1) s = 1;
+    if (s > 512) s = 512;
+    return (int)s;
+}
+
+template <typename T>
+hipError_t launch_gemv(hipStream_t st, const T* A, int64_t lda, int64_t nrows, int64_t ncols, const double* x,
+                       double alpha, double beta, double* y, double* part)
+{
+    int nsplit = gemv_splits(nrows, ncols);
+    const int64_t per = (ncols + nsplit - 1) / nsplit;
+    nsplit = (int)((ncols + per - 1) / per);
+    dim3 grid(nblocks(nrows), (unsigned)nsplit);
+    k_gemv_part<T><<<grid, kBlock, 0, st>>>(A, lda, nrows, ncols, per, x, part);
+    k_gemv_reduce<<<nblocks(nrows), kBlock, 0, st>>>(part, nsplit, nrows, alpha, beta, y);
+    return hipGetLastError();
+}
+template hipError_t launch_gemv<double>(hipStream_t, const double*, int64_t, int64_t, int64_t, const double*, double,
+                                        double, double*, double*);
+template hipError_t launch_gemv<float>(hipStream_t, const float*, int64_t, int64_t, int64_t, const double*, double,
+                                       double, double*, double*);
+
+template <typename T>
+hipError_t launch_gemv_t(hipStream_t st, const T* A, int64_t lda, int64_t nrows, int64_t ncols, const double* x,
+                         double* y)
+{
+    k_gemv_t<T><<<(unsigned)ncols, kBlock, 0, st>>>(A, lda, nrows, x, y);
+    return hipGetLastError();
+}
+template hipError_t launch_gemv_t<double>(hipStream_t, const double*, int64_t, int64_t, int64_t, const double*,
+                                          double*);
+template hipError_t launch_gemv_t<float>(hipStream_t, const float*, int64_t, int64_t, int64_t, const double*,
+                                         double*);
+
+constexpr int kDotParts = 256;
+
+hipError_t launch_dot(hipStream_t st, const double* a, const double* b, int64_t n, double* part, double* out,
+                      bool want_sqrt)
+{
+    int nb = (int)((n + kBlock - 1) / kBlock);
+    if (nb > kDotParts) nb = kDotParts;
+    if (nb < 1) nb = 1;
+    k_dot_part<<<nb, kBlock, 0, st>>>(a, b, n, part);
+    k_dot_final<<<1, kBlock, 0, st>>>(part, nb, want_sqrt ? 1 : 0, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_axpy_dev(hipStream_t st, const double* alpha, double sign, const double* x, double* y, int64_t n)
+{
+    k_axpy_dev<<<nblocks(n), kBlock, 0, st>>>(alpha, sign, x, y, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_div_dev(hipStream_t st, const double* d, const double* x, double* y, int64_t n)
+{
+    k_div_dev<<<nblocks(n), kBlock, 0, st>>>(d, x, y, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_axpby(hipStream_t st, double a, const double* x, double b, const double* y, double* z, int64_t n)
+{
+    k_axpby<<<nblocks(n), kBlock, 0, st>>>(a, x, b, y, z, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_robin_hood(hipStream_t st, const double* A, const double* b, double* x, double* bi, int64_t n,
+                             int64_t steps, double* rnorm)
+{
+    k_robin_hood<<<1, kRHBlock, 0, st>>>(A, b, x, bi, n, steps, rnorm);
+    return hipGetLastError();
+}
+
+}  // namespace kem

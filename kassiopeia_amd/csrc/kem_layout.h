@@ -1,0 +1,60 @@
+/*
+ * kem_layout.h -- device-resident element record layout (shared by host preparation and
+ * HIP kernels).
+ *
+ * Each boundary element is expanded once, on the host, from KEMField's 11-double shape
+ * record (KElectrostaticCubatureTriangleIntegrator.cc:457-467: [A, B, P0, N1, N2]; line
+ * segments [P0, P1, diameter], KElectrostaticQuadratureLineSegmentIntegrator.cc:197-203)
+ * into a fixed-stride FP64 record holding every per-element quantity the integrals need:
+ * dispatch centroid and 1/size, cubature prefactor, plane normal, vertices, and the
+ * per-edge quantities of the closed-form (solid angle + edge logarithm) near field.  The
+ * kernels read these records with wave-uniform loads; nothing per element is recomputed
+ * per (point, element) pair except what depends on the point.
+ *
+ * Cubature nodes are held in a second array, KEM_NODE_SLOTS nodes of 4 doubles (x, y, z, w)
+ * per element: triangle/rectangle 7-point at slot 0, 12-point at slot 7, 33-point at slot 19;
+ * line segments 4-node Gauss-Legendre at slot 0 and 16-node at slot 4 (positions of both
+ * members of each symmetric pair, weight * half-length).
+ */
+#ifndef KEM_LAYOUT_H
+#define KEM_LAYOUT_H
+
+enum kem_shape_type { KEM_TRIANGLE = 0, KEM_RECTANGLE = 1, KEM_LINESEGMENT = 2 };
+
+enum kem_geom_field {
+    KG_CX = 0, /* dispatch centre (triangle centroid, rectangle centre, segment midpoint) */
+    KG_CY = 1,
+    KG_CZ = 2,
+    KG_INVSIZE = 3, /* 1 / size: mean side (triangle), (A+B)/2 (rectangle), length (segment) */
+    KG_PREF = 4,    /* area/(4 pi eps0) (triangle, rectangle); d/(4 eps0) (segment) */
+    KG_TYPE = 5,    /* shape type as a double */
+    KG_NX = 6,      /* unit plane normal */
+    KG_NY = 7,
+    KG_NZ = 8,
+    KG_HX = 9, /* centroid used for the height h = n.(P - c) of the closed form */
+    KG_HY = 10,
+    KG_HZ = 11,
+    KG_NE = 12, /* number of edges (3, 4) */
+    KG_A = 13,  /* side A (rectangle in-plane test); segment: length L */
+    KG_B = 14,  /* side B; segment: diameter */
+    KG_V = 16,  /* vertices V0..V3 (segment: P0 at V0, P1 at V1) */
+    KG_U = 28,  /* unit edge directions u_e (segment: unit axis at u_0) */
+    KG_MID = 40, /* edge midpoints */
+    KG_M = 52,   /* in-plane outward edge normals m_e = u_e x n */
+    KG_SV = 64,  /* full edge vectors (point-on-edge detection) */
+    KG_IL = 76,  /* 1 / edge length */
+    KG_STRIDE = 80
+};
+
+#define KEM_NODE_SLOTS 52
+#define KEM_NODE_STRIDE (4 * KEM_NODE_SLOTS)
+#define KEM_N7 0
+#define KEM_N12 7
+#define KEM_N33 19
+#define KEM_L4 0
+#define KEM_L16 4
+
+/* collocation target record (one per element): centre, unit normal, boundary type, flux */
+enum kem_target_field { KT_CX = 0, KT_CY, KT_CZ, KT_NX, KT_NY, KT_NZ, KT_BC, KT_FLUX, KT_STRIDE = 8 };
+
+#endif

@@ -1,0 +1,337 @@
+// kem_prep.cpp -- host expansion of KEMField shape records into device element records.
+//
+// Runs once per mesh upload.  Every quantity is computed with the expression (and order of
+// operations) the reference integrators use for it, so that the per-pair device code
+// only adds the point-dependent arithmetic.  Reference files (KEMField/Source/...):
+//   Surfaces/src/KTriangle.cc:18-28, Surfaces/include/KTriangle.hh:27-34, KRectangle.hh:27-34
+//   BoundaryIntegrals/Electrostatic/Numeric/{include,src}/KElectrostaticCubatureTriangleIntegrator.*
+//   BoundaryIntegrals/Electrostatic/Numeric/{include,src}/KElectrostaticCubatureRectangleIntegrator.*
+//   BoundaryIntegrals/Electrostatic/Numeric/{include,src}/KElectrostaticQuadratureLineSegmentIntegrator.*
+//   BoundaryIntegrals/Electrostatic/RWG/src/KElectrostaticRWG{Triangle,Rectangle}Integrator.cc
+#include "kem_prep.h"
+
+#include <cmath>
+#include <cstring>
+
+namespace kem {
+
+namespace {
+
+inline double sq(double x) { return x * x; }
+
+// Published cubature rules (normalised weights, sum 1):
+//  triangle 7-point Radon / Stroud (degree 5), 12-point Gatermann 1988 (degree 7),
+//  33-point Papanicolopulos 2011 (degree 12): KElectrostaticCubatureTriangleIntegrator.hh:56-258
+//  rectangle Stroud 5-2 7-point, Tyler 12-point, 33-point (degree 13 on [-1,1]^2):
+//  KElectrostaticCubatureRectangleIntegrator.hh:64-160, .cc:43-290
+const double kT12a[4] = {0.6238226509439084e-1, 0.5522545665692000e-1, 0.3432430294509488e-1, 0.5158423343536001};
+const double kT12b[4] = {0.6751786707392436e-1, 0.3215024938520156, 0.6609491961867980, 0.2777161669764050};
+const double kT12c[4] = {0.8700998678316848, 0.6232720494910644, 0.3047265008681072, 0.2064414986699949};
+const double kT12w[4] = {0.2651702815743450e-1, 0.4388140871444811e-1, 0.2877504278497528e-1,
+                         0.6749318700980879e-1};
+const double kT33a[8] = {4.570749859701478e-01, 1.197767026828138e-01, 2.359249810891690e-02, 7.814843446812914e-01,
+                         9.507072731273288e-01, 1.162960196779266e-01, 2.303415635526714e-02, 2.138249025617059e-02};
+const double kT33b[8] = {2.714625070149261e-01, 4.401116486585931e-01, 4.882037509455416e-01, 1.092578276593543e-01,
+                         2.464636343633559e-02, 2.554542286385173e-01, 2.916556797383410e-01, 1.272797172335894e-01};
+const double kT33w[8] = {6.254121319590276e-02, 4.991833492806094e-02, 2.426683808145203e-02, 2.848605206887754e-02,
+                         7.931642509973638e-03, 4.322736365941421e-02, 2.178358503860756e-02, 1.508367757651144e-02};
+const double kR33X[9] = {0.00000000000000000000, 0.77880971155441942252, 0.95729769978630736566,
+                         0.13818345986246535375, 0.94132722587292523695, 0.47580862521827590507,
+                         0.75580535657208143627, 0.69625007849174941396, 0.34271655604040678941};
+const double kR33Y[9] = {0.00000000000000000000, 0.98348668243987226379, 0.85955600564163892859,
+                         0.95892517028753485754, 0.39073621612946100068, 0.85007667369974857597,
+                         0.64782163718701073204, 0.70741508996444936217e-1, 0.40930456169403884330};
+const double kR33W[9] = {0.30038211543122536139 / 4.,   0.29991838864499131666e-1 / 4., 0.38174421317083669640e-1 / 4.,
+                         0.60424923817749980681e-1 / 4., 0.77492738533105339358e-1 / 4., 0.11884466730059560108 / 4.,
+                         0.12976355037000271129 / 4.,   0.21334158145718938943 / 4.,   0.25687074948196783651 / 4.};
+// Gauss-Legendre half tables (positive nodes): ...QuadratureLineSegmentIntegrator.hh:59-83
+const double kGL4x[2] = {0.339981043584856, 0.861136311594053};
+const double kGL4w[2] = {0.652145154862546, 0.347854845137454};
+const double kGL16x[8] = {0.09501250983763744, 0.28160355077925891, 0.45801677765722739, 0.61787624440264375,
+                          0.75540440835500303, 0.86563120238783174, 0.94457502307323258, 0.98940093499164993};
+const double kGL16w[8] = {0.189450610455068496, 0.182603415044923589, 0.169156519395002532, 0.149595988816576731,
+                          0.124628971255533872, 0.095158511682492785, 0.062253523938647892, 0.027152459411754095};
+
+inline void put_node(double* nodes, int slot, const double q[3], double w)
+{
+    double* p = nodes + 4 * slot;
+    p[0] = q[0];
+    p[1] = q[1];
+    p[2] = q[2];
+    p[3] = w;
+}
+
+// Q = a*A + b*B + c*C component-wise (GaussPoints_Tri*P)
+inline void bary(const double A[3], const double B[3], const double C[3], double a, double b, double c, double Q[3])
+{
+    for (int k = 0; k < 3; k++) Q[k] = a * A[k] + b * B[k] + c * C[k];
+}
+
+void triangle_nodes(const double* d, double* nodes)
+{
+    double A[3], B[3], C[3], Q[3];
+    for (int k = 0; k < 3; k++) {
+        A[k] = d[2 + k];
+        B[k] = d[2 + k] + (d[0] * d[5 + k]);
+        C[k] = d[2 + k] + (d[1] * d[8 + k]);
+    }
+    // 7-point: centroid, then two orbits (alpha,beta,beta) in the order (a,b,c),(b,a,c),(c,b,a)
+    const double t = 1. / 3., q15 = std::sqrt(15), r = (1. + q15) / 7., s = (1. - q15) / 7.;
+    const double a7[3] = {t, t + (2. * t * s), t + (2. * t * r)};
+    const double b7[3] = {t, t - (t * s), t - (t * r)};
+    const double w7[3] = {9 / 40., (155. + q15) / 1200., (155. - q15) / 1200.};
+    int slot = KEM_N7;
+    bary(A, B, C, a7[0], b7[0], b7[0], Q);
+    put_node(nodes, slot++, Q, w7[0]);
+    for (int o = 1; o < 3; o++) {
+        bary(A, B, C, a7[o], b7[o], b7[o], Q);
+        put_node(nodes, slot++, Q, w7[o]);
+        bary(A, B, C, b7[o], a7[o], b7[o], Q);
+        put_node(nodes, slot++, Q, w7[o]);
+        bary(A, B, C, b7[o], b7[o], a7[o], Q);
+        put_node(nodes, slot++, Q, w7[o]);
+    }
+    // 12-point: cyclic orbits (a,b,c),(b,c,a),(c,a,b), weights doubled
+    slot = KEM_N12;
+    for (int o = 0; o < 4; o++) {
+        double a = kT12a[o], b = kT12b[o], c = kT12c[o];
+        bary(A, B, C, a, b, c, Q);
+        put_node(nodes, slot++, Q, kT12w[o] * 2.);
+        bary(A, B, C, b, c, a, Q);
+        put_node(nodes, slot++, Q, kT12w[o] * 2.);
+        bary(A, B, C, c, a, b, Q);
+        put_node(nodes, slot++, Q, kT12w[o] * 2.);
+    }
+    // 33-point: five 3-orbits and three 6-orbits
+    slot = KEM_N33;
+    for (int o = 0; o < 8; o++) {
+        double a = kT33a[o], b = kT33b[o], c = 1. - kT33a[o] - kT33b[o];
+        double perm[6][3] = {{a, b, c}, {b, a, c}, {c, b, a}, {a, c, b}, {c, a, b}, {b, c, a}};
+        int np = o < 5 ? 3 : 6;
+        for (int p = 0; p < np; p++) {
+            bary(A, B, C, perm[p][0], perm[p][1], perm[p][2], Q);
+            put_node(nodes, slot++, Q, kT33w[o]);
+        }
+    }
+}
+
+void rectangle_rule(const double* d, int slot, int n, const double* x, const double* y, const double* w, double* nodes)
+{
+    const double a2 = 0.5 * d[0], b2 = 0.5 * d[1];
+    double cen[3], Q[3];
+    for (int k = 0; k < 3; k++) cen[k] = d[2 + k] + a2 * d[5 + k] + b2 * d[8 + k];
+    for (int i = 0; i < n; i++) {
+        for (int k = 0; k < 3; k++) Q[k] = cen[k] + (x[i] * a2) * d[5 + k] + (y[i] * b2) * d[8 + k];
+        put_node(nodes, slot + i, Q, w[i]);
+    }
+}
+
+void rectangle_nodes(const double* d, double* nodes)
+{
+    {
+        const double t2 = std::sqrt((7. - std::sqrt(24.)) / 15.), s2 = std::sqrt((7. + std::sqrt(24.)) / 15.),
+                     r2 = std::sqrt(7. / 15.);
+        const double x[7] = {0., r2, -r2, s2, -s2, t2, -t2};
+        const double y[7] = {0., r2, -r2, -t2, t2, -s2, s2};
+        const double w[7] = {2. / 7., 25. / 168., 25. / 168., 5. / 48., 5. / 48., 5. / 48., 5. / 48.};
+        rectangle_rule(d, KEM_N7, 7, x, y, w, nodes);
+    }
+    {
+        const double r = std::sqrt(6. / 7.), s = std::sqrt((114. - 3. * std::sqrt(583.)) / 287.),
+                     t = std::sqrt((114. + 3. * std::sqrt(583.)) / 287.);
+        const double x[12] = {r, -r, 0., 0., s, s, -s, -s, t, t, -t, -t};
+        const double y[12] = {0., 0., r, -r, s, -s, s, -s, t, -t, t, -t};
+        const double B1 = 49 / 810., B2 = (178981. + 2769. * std::sqrt(583)) / 1888920.,
+                     B3 = (178981. - 2769. * std::sqrt(583)) / 1888920.;
+        const double w[12] = {B1, B1, B1, B1, B2, B2, B2, B2, B3, B3, B3, B3};
+        rectangle_rule(d, KEM_N12, 12, x, y, w, nodes);
+    }
+    {
+        double x[33], y[33], w[33];
+        x[0] = kR33X[0];
+        y[0] = kR33Y[0];
+        w[0] = kR33W[0];
+        int m = 1;
+        for (int o = 1; o < 9; o++) {
+            const double xo[4] = {kR33X[o], -kR33X[o], -kR33Y[o], kR33Y[o]};
+            const double yo[4] = {kR33Y[o], -kR33Y[o], kR33X[o], -kR33X[o]};
+            for (int q = 0; q < 4; q++, m++) {
+                x[m] = xo[q];
+                y[m] = yo[q];
+                w[m] = kR33W[o];
+            }
+        }
+        rectangle_rule(d, KEM_N33, 33, x, y, w, nodes);
+    }
+}
+
+void line_nodes(const double* d, double* nodes)
+{
+    const double c[3] = {0.5 * (d[0] + d[3]), 0.5 * (d[1] + d[4]), 0.5 * (d[2] + d[5])};
+    const double L = std::sqrt(sq(d[3] - d[0]) + sq(d[4] - d[1]) + sq(d[5] - d[2]));
+    const double hl = 0.5 * L, iu = 1. / L;
+    const double u[3] = {iu * (d[3] - d[0]), iu * (d[4] - d[1]), iu * (d[5] - d[2])};
+    auto rule = [&](int slot, int half, const double* xs, const double* ws) {
+        for (int i = 0; i < half; i++) {
+            double x = xs[i] * hl, wf = ws[i] * hl, qp[3], qm[3];
+            for (int k = 0; k < 3; k++) {
+                qp[k] = c[k] + (u[k] * x);
+                qm[k] = c[k] - (u[k] * x);
+            }
+            put_node(nodes, slot + 2 * i, qp, wf);
+            put_node(nodes, slot + 2 * i + 1, qm, wf);
+        }
+    };
+    rule(KEM_L4, 2, kGL4x, kGL4w);
+    rule(KEM_L16, 8, kGL16x, kGL16w);
+}
+
+bool finite_all(const double* d, int n)
+{
+    for (int k = 0; k < n; k++)
+        if (!std::isfinite(d[k])) return false;
+    return true;
+}
+
+}  // namespace
+
+int prep_element(int type, const double* d, double* g, double* nodes)
+{
+    std::memset(g, 0, sizeof(double) * KG_STRIDE);
+    std::memset(nodes, 0, sizeof(double) * KEM_NODE_STRIDE);
+    g[KG_TYPE] = type;
+    if (type == KEM_LINESEGMENT) {
+        if (!finite_all(d, 7)) return -1;
+        const double L = std::sqrt(((d[3] - d[0]) * (d[3] - d[0])) + ((d[4] - d[1]) * (d[4] - d[1])) +
+                                   ((d[5] - d[2]) * (d[5] - d[2])));
+        if (!(L > 0.) || !(d[6] > 0.)) return -1;
+        for (int k = 0; k < 3; k++) {
+            g[KG_CX + k] = (d[k] + d[3 + k]) * 0.5;
+            g[KG_V + k] = d[k];
+            g[KG_V + 3 + k] = d[3 + k];
+            g[KG_U + k] = (d[3 + k] - d[k]) / L;
+        }
+        g[KG_INVSIZE] = 1. / L;
+        g[KG_PREF] = 0.25 * d[6] / kEps0;
+        g[KG_A] = L;
+        g[KG_B] = d[6];
+        line_nodes(d, nodes);
+        return 0;
+    }
+    if (type != KEM_TRIANGLE && type != KEM_RECTANGLE) return -1;
+    if (!finite_all(d, 11) || !(d[0] > 0.) || !(d[1] > 0.)) return -1;
+    // unit normal n = unit(N1 x N2), normalised by multiplying with the inverse magnitude
+    double n[3] = {d[6] * d[10] - d[7] * d[9], d[7] * d[8] - d[5] * d[10], d[5] * d[9] - d[6] * d[8]};
+    const double nn = std::sqrt(sq(n[0]) + sq(n[1]) + sq(n[2]));
+    if (!(nn > 1e-12)) return -1;
+    const double inn = 1. / nn;
+    for (int k = 0; k < 3; k++) g[KG_NX + k] = n[k] * inn;
+    g[KG_A] = d[0];
+    g[KG_B] = d[1];
+    if (type == KEM_TRIANGLE) {
+        double V[3][3];
+        for (int k = 0; k < 3; k++) {
+            V[0][k] = d[2 + k];
+            V[1][k] = d[2 + k] + (d[0] * d[5 + k]);
+            V[2][k] = d[2 + k] + (d[1] * d[8 + k]);
+        }
+        const double l12 = std::sqrt(sq(V[2][0] - V[1][0]) + sq(V[2][1] - V[1][1]) + sq(V[2][2] - V[1][2]));
+        if (!(l12 > 0.)) return -1;
+        for (int k = 0; k < 3; k++) {
+            // centroid (KTriangle::Centroid; also the dispatch centre and the centre of h)
+            g[KG_CX + k] = d[2 + k] + (d[0] * d[5 + k] + d[1] * d[8 + k]) / 3.;
+            g[KG_HX + k] = g[KG_CX + k];
+        }
+        g[KG_INVSIZE] = 1. / ((d[0] + d[1] + l12) / 3.);
+        g[KG_PREF] = 0.5 * d[0] * d[1] * nn * kOneOverFourPiEps0;
+        g[KG_NE] = 3;
+        const double e12[3] = {V[2][0] - V[1][0], V[2][1] - V[1][1], V[2][2] - V[1][2]};
+        const double i12 = 1. / l12;
+        const double hl[3] = {0.5 * d[0], 0.5 * l12, 0.5 * d[1]};
+        for (int k = 0; k < 3; k++) {
+            for (int v = 0; v < 3; v++) g[KG_V + 3 * v + k] = V[v][k];
+            g[KG_U + 0 + k] = d[5 + k];
+            g[KG_U + 3 + k] = i12 * e12[k];
+            g[KG_U + 6 + k] = -d[8 + k];
+            g[KG_SV + 0 + k] = d[0] * d[5 + k];
+            g[KG_SV + 3 + k] = e12[k];
+            g[KG_SV + 6 + k] = (-1) * d[1] * d[8 + k];
+        }
+        g[KG_IL + 0] = 1. / d[0];
+        g[KG_IL + 1] = 1. / l12;
+        g[KG_IL + 2] = 1. / d[1];
+        for (int e = 0; e < 3; e++)
+            for (int k = 0; k < 3; k++) g[KG_MID + 3 * e + k] = hl[e] * g[KG_U + 3 * e + k] + V[e][k];
+        triangle_nodes(d, nodes);
+    } else {
+        double V[4][3];
+        for (int k = 0; k < 3; k++) {
+            V[0][k] = d[2 + k];
+            V[1][k] = d[2 + k] + (d[0] * d[5 + k]);
+            V[2][k] = d[2 + k] + (d[0] * d[5 + k]) + (d[1] * d[8 + k]);
+            V[3][k] = d[2 + k] + (d[1] * d[8 + k]);
+            // cubature / dispatch centre (...RectangleIntegrator.cc:470-473)
+            g[KG_CX + k] = d[2 + k] + 0.5 * (d[0] * d[5 + k] + d[1] * d[8 + k]);
+            // closed-form centre (RWG rectangle .cc:1212-1214, KSolidAngle.hh:188-190)
+            g[KG_HX + k] = d[2 + k] + 0.5 * (d[0] * d[5 + k]) + 0.5 * (d[1] * d[8 + k]);
+        }
+        g[KG_INVSIZE] = 1. / (0.5 * (d[0] + d[1]));
+        g[KG_PREF] = d[0] * d[1] * kOneOverFourPiEps0;
+        g[KG_NE] = 4;
+        for (int k = 0; k < 3; k++) {
+            for (int v = 0; v < 4; v++) g[KG_V + 3 * v + k] = V[v][k];
+            g[KG_U + 0 + k] = d[5 + k];
+            g[KG_U + 3 + k] = d[8 + k];
+            g[KG_U + 6 + k] = -d[5 + k];
+            g[KG_U + 9 + k] = -d[8 + k];
+            g[KG_MID + 0 + k] = (0.5 * d[0] * d[5 + k]) + V[0][k];
+            g[KG_MID + 3 + k] = (0.5 * d[1] * d[8 + k]) + V[1][k];
+            g[KG_MID + 6 + k] = (-0.5 * d[0] * d[5 + k]) + V[2][k];
+            g[KG_MID + 9 + k] = (-0.5 * d[1] * d[8 + k]) + V[3][k];
+            g[KG_SV + 0 + k] = d[0] * d[5 + k];
+            g[KG_SV + 3 + k] = d[1] * d[8 + k];
+            g[KG_SV + 6 + k] = -d[0] * d[5 + k];
+            g[KG_SV + 9 + k] = -d[1] * d[8 + k];
+        }
+        g[KG_IL + 0] = 1. / d[0];
+        g[KG_IL + 1] = 1. / d[1];
+        g[KG_IL + 2] = 1. / d[0];
+        g[KG_IL + 3] = 1. / d[1];
+        rectangle_nodes(d, nodes);
+    }
+    // in-plane outward edge normals m_e = u_e x n
+    const int ne = (int)g[KG_NE];
+    for (int e = 0; e < ne; e++) {
+        const double* u = g + KG_U + 3 * e;
+        const double* nv = g + KG_NX;
+        double* m = g + KG_M + 3 * e;
+        m[0] = (u[1] * nv[2]) - (u[2] * nv[1]);
+        m[1] = (u[2] * nv[0]) - (u[0] * nv[2]);
+        m[2] = (u[0] * nv[1]) - (u[1] * nv[0]);
+    }
+    return 0;
+}
+
+void prep_target(int type, const double* d, int bc, double flux, double* t)
+{
+    for (int k = 0; k < 3; k++) {
+        if (type == KEM_TRIANGLE)
+            t[KT_CX + k] = d[2 + k] + (d[0] * d[5 + k] + d[1] * d[8 + k]) / 3.;  // KTriangle.hh:31-34
+        else if (type == KEM_RECTANGLE)
+            t[KT_CX + k] = d[2 + k] + d[0] * d[5 + k] * .5 + d[1] * d[8 + k] * .5;  // KRectangle.hh:31-34
+        else
+            t[KT_CX + k] = (d[k] + d[3 + k]) * .5;  // KLineSegment.hh:29-32
+    }
+    if (type == KEM_LINESEGMENT) {
+        t[KT_NX] = t[KT_NY] = t[KT_NZ] = 0.;
+    } else {
+        // KTriangle::fN3 = unit(N1 x N2) (KThreeVector::Unit divides by the magnitude)
+        double n[3] = {d[6] * d[10] - d[7] * d[9], d[7] * d[8] - d[5] * d[10], d[5] * d[9] - d[6] * d[8]};
+        const double m = std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+        for (int k = 0; k < 3; k++) t[KT_NX + k] = n[k] / m;
+    }
+    t[KT_BC] = bc;
+    t[KT_FLUX] = flux;
+}
+
+}  // namespace kem

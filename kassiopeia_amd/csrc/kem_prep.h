@@ -1,0 +1,24 @@
+// kem_prep.h -- host-side element preparation and physical constants.
+#ifndef KEM_PREP_H
+#define KEM_PREP_H
+
+#include "kem_layout.h"
+
+namespace kem {
+
+// Kommon/Base/Utility/KConst_2021.h:37 (pi), :139-146 (eps0); KEMField/Source/Core/src/KEMConstants.cc:9-12
+constexpr double kPi = 3.141592653589793238462643383279502884;
+constexpr double kEps0 = 8.8541878128E-12;
+constexpr double kOneOverFourPiEps0 = 1. / (4. * kPi * kEps0);
+
+// Expand one KEMField shape record into the device record g[KG_STRIDE] and the node table
+// nodes[KEM_NODE_STRIDE].  Returns 0, or -1 for a degenerate / non-finite element.
+int prep_element(int type, const double* shape_data, double* g, double* nodes);
+
+// Collocation target record t[KT_STRIDE]: the shape's Centroid(), unit normal, boundary type
+// and Neumann flux.
+void prep_target(int type, const double* shape_data, int bc, double flux, double* t);
+
+}  // namespace kem
+
+#endif

@@ -861,6 +861,15 @@ static void matvec(int64_t n, const double* A, const double* x, double* y)
 int ko_gmres(int64_t n, const double* A, const double* b, double* x, double tol, int maxit, int restart,
              double* resid, double* hist)
 {
+    return ko_gmres_ortho(n, A, b, x, tol, maxit, restart, 0, resid, hist);
+}
+
+/* ortho = 0: modified Gram-Schmidt in the reference's order (KGeneralizedMinimalResidual.hh:225-236);
+   ortho = 1: classical Gram-Schmidt with one reorthogonalisation pass (CGS2), the scheme of the
+   HIP path. */
+int ko_gmres_ortho(int64_t n, const double* A, const double* b, double* x, double tol, int maxit, int restart,
+                   int ortho, double* resid, double* hist)
+{
     int kmax = (restart > 0 && restart < maxit) ? restart : maxit;
     double* V = (double*)malloc(sizeof(double) * (size_t)n * (size_t)(kmax + 1));
     double* H = (double*)calloc((size_t)(kmax + 1) * (size_t)(kmax + 1), sizeof(double));
@@ -887,11 +896,25 @@ int ko_gmres(int64_t n, const double* A, const double* b, double* x, double tol,
             double* vj1 = V + (size_t)(j + 1) * n;
             matvec(n, A, V + (size_t)j * n, w);
             double* h = H + (size_t)j * (kmax + 1);
-            for (int r = 0; r <= j; r++) {
-                double hh = dotn(n, w, V + (size_t)r * n);
-                h[r] = hh;
-                const double* vr = V + (size_t)r * n;
-                for (int64_t i = 0; i < n; i++) w[i] -= hh * vr[i];
+            if (ortho == 0) {
+                for (int r = 0; r <= j; r++) {
+                    double hh = dotn(n, w, V + (size_t)r * n);
+                    h[r] = hh;
+                    const double* vr = V + (size_t)r * n;
+                    for (int64_t i = 0; i < n; i++) w[i] -= hh * vr[i];
+                }
+            } else {
+                for (int r = 0; r <= j; r++) h[r] = 0.;
+                for (int pass = 0; pass < 2; pass++) {
+                    double hp[kmax + 1];
+                    for (int r = 0; r <= j; r++) hp[r] = dotn(n, w, V + (size_t)r * n);
+                    for (int64_t i = 0; i < n; i++) {
+                        double s = 0.;
+                        for (int r = 0; r <= j; r++) s += V[(size_t)r * n + i] * hp[r];
+                        w[i] -= s;
+                    }
+                    for (int r = 0; r <= j; r++) h[r] += hp[r];
+                }
             }
             double beta = sqrt(dotn(n, w, w));
             double ib = 1.0 / beta;

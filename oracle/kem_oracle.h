@@ -98,6 +98,9 @@ void ko_field(int policy, int64_t n, const int32_t* shape, const double* recs, c
    KSimpleIterativeKrylovSolver.hh:140-194.  A row-major.  Returns iterations; *resid = final |p_{j+1}|/||b||. */
 int ko_gmres(int64_t n, const double* A, const double* b, double* x, double tol, int maxit, int restart,
              double* resid, double* hist);
+/* same with the orthogonalisation selectable: 0 = MGS (reference), 1 = CGS2 (the HIP path's) */
+int ko_gmres_ortho(int64_t n, const double* A, const double* b, double* x, double tol, int maxit, int restart,
+                   int ortho, double* resid, double* hist);
 /* Robin Hood: KRobinHood.hh:62-104, KRobinHood_SingleThread.hh:53-120.  Returns sub-iterations. */
 int64_t ko_robin_hood(int64_t n, const double* A, const double* b, double* x, double tol, int64_t check_interval,
                       int64_t max_sub, double* resid);

@@ -57,6 +57,7 @@ def _load():
         "ko_rhs": (None, [i64, P, P, P]),
         "ko_field": (None, [ctypes.c_int, i64, P, P, P, i64, P, P, P]),
         "ko_gmres": (ctypes.c_int, [i64, P, P, P, d, ctypes.c_int, ctypes.c_int, P, P]),
+        "ko_gmres_ortho": (ctypes.c_int, [i64, P, P, P, d, ctypes.c_int, ctypes.c_int, ctypes.c_int, P, P]),
         "ko_robin_hood": (i64, [i64, P, P, P, d, i64, i64, P]),
         "ko_gauss": (ctypes.c_int, [i64, P, P, P]),
     }
@@ -224,14 +225,16 @@ def field(shapes, recs, sigma, points, policy=NUMERIC):
     return phi, E
 
 
-def gmres(A, b, tol=1e-10, maxit=10000, restart=0, x0=None):
+def gmres(A, b, tol=1e-10, maxit=10000, restart=0, x0=None, ortho="mgs"):
+    """ortho: "mgs" (the reference's modified Gram-Schmidt) or "cgs2" (the HIP path's)."""
     A = _f64(A)
     b = _f64(b)
     n = len(b)
     x = np.zeros(n) if x0 is None else _f64(x0).copy()
     resid = np.zeros(1)
     hist = np.zeros(maxit)
-    it = lib().ko_gmres(n, _p(A), _p(b), _p(x), tol, maxit, restart, _p(resid), _p(hist))
+    it = lib().ko_gmres_ortho(n, _p(A), _p(b), _p(x), tol, maxit, restart, {"mgs": 0, "cgs2": 1}[ortho], _p(resid),
+                              _p(hist))
     return x, it, resid[0], hist[:it]
 
 
