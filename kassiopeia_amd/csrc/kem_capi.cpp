@@ -28,6 +28,8 @@ struct kem_ctx {
     int64_t ws_doubles = 0;
     double* scal = nullptr;  // small device scalars (dot products, Hessenberg column)
     int64_t scal_doubles = 0;
+    void* sort_ws = nullptr;  // point ordering workspace (grow-only)
+    size_t sort_ws_bytes = 0;
 };
 
 struct kem_mesh {
@@ -104,6 +106,9 @@ struct DVec {
     ~DVec() { dfree(p); }
 };
 
+// point counts from which field evaluation visits points in Morton order
+constexpr int64_t kSortMinPoints = 4096;
+
 double now_s()
 {
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
@@ -141,6 +146,7 @@ void kem_ctx_destroy(kem_ctx* c)
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     dfree(c->ws);
     dfree(c->scal);
+    if (c->sort_ws) (void)hipFree(c->sort_ws);
     if (c->own) (void)hipStreamDestroy(c->own);
     delete c;
 }
@@ -286,6 +292,20 @@ int kem_field_eval_device(kem_ctx* c, const kem_mesh* m, const double* sigma, in
     KEM_CK(c, hipSetDevice(c->device));
     const int64_t need = 4 * (int64_t)(2048 + 256) * 256;
     KEM_CK(c, ensure_ws(c, need));
+    if (npts >= kSortMinPoints && npts <= INT32_MAX) {
+        // visit the points in Morton order so the lanes of a wavefront share quadrature rules
+        const size_t bytes = kem::sort_workspace_bytes(npts);
+        if (c->sort_ws_bytes < bytes) {
+            if (c->sort_ws) (void)hipFree(c->sort_ws);
+            c->sort_ws = nullptr;
+            c->sort_ws_bytes = 0;
+            KEM_CK(c, hipMalloc(&c->sort_ws, bytes));
+            c->sort_ws_bytes = bytes;
+        }
+        KEM_CK(c, kem::launch_field_sorted(c->stream, c->policy, m->geom, m->nodes, sigma, m->n, x, y, z, npts, phi,
+                                           ex, ey, ez, c->sort_ws, c->sort_ws_bytes, c->ws, c->ws_doubles));
+        return KEM_OK;
+    }
     KEM_CK(c, kem::launch_field_fp64(c->stream, c->policy, m->geom, m->nodes, sigma, m->n, x, y, z, npts, phi, ex,
                                      ey, ez, c->ws, c->ws_doubles));
     return KEM_OK;

@@ -15,6 +15,14 @@ hipError_t launch_field_fp64(hipStream_t st, int policy, const double* geom, con
                              int64_t npts, double* phi, double* ex, double* ey, double* ez, double* ws,
                              int64_t ws_doubles);
 
+// Same as launch_field_fp64 with the points visited in Morton order (kem_sort.hip): sort
+// keys, gather, evaluate, scatter back.  sort_ws: sort_workspace_bytes(npts) bytes.
+size_t sort_workspace_bytes(int64_t npts);
+hipError_t launch_field_sorted(hipStream_t st, int policy, const double* geom, const double* nodes,
+                               const double* sigma, int64_t nelem, const double* px, const double* py,
+                               const double* pz, int64_t npts, double* phi, double* ex, double* ey, double* ez,
+                               void* sort_ws, size_t sort_ws_bytes, double* ws, int64_t ws_doubles);
+
 // pairs per quadrature regime (11 counters, see k_census)
 hipError_t launch_census(hipStream_t st, const double* geom, int64_t nelem, const double* px, const double* py,
                          const double* pz, int64_t npts, unsigned long long* counts);

@@ -21,18 +21,19 @@ constexpr int kBlock = 256;
 
 // ------------------------------------------------------------------ field evaluation
 // One lane per field point.  Elements [j0, j1) are summed in container order with Kahan
-// compensation per component (KElectrostaticIntegratingFieldSolver.hh:303-339).  With
-// nsplit > 1 each blockIdx.y handles one contiguous element range and writes its partial
-// sums; k_field_combine adds the partials in range order.
-template <int POLICY>
-__global__ __launch_bounds__(kBlock) void k_field_fp64(const double* __restrict__ geom,
-                                                       const double* __restrict__ nodes,
-                                                       const double* __restrict__ sigma, int64_t nelem,
-                                                       int64_t per_split, const double* __restrict__ px,
-                                                       const double* __restrict__ py, const double* __restrict__ pz,
-                                                       int64_t npts, double* __restrict__ out0,
-                                                       double* __restrict__ out1, double* __restrict__ out2,
-                                                       double* __restrict__ out3, int64_t out_stride)
+// compensation per component (KElectrostaticIntegratingFieldSolver.hh:303-339).  The far pass
+// evaluates the cubature / Gauss-Legendre pairs; the near pass (k_field_near) then adds the
+// closed-form pairs of the same element range to the same sums.  Keeping the closed forms out
+// of this kernel keeps its register use low enough for high occupancy.  With nsplit > 1 each
+// blockIdx.y handles one contiguous element range and writes its partial sums;
+// k_field_combine adds the partials in range order.
+__global__ __launch_bounds__(kBlock) void k_field_far(const double* __restrict__ geom, const double* __restrict__ nodes,
+                                                      const double* __restrict__ sigma, int64_t nelem,
+                                                      int64_t per_split, const double* __restrict__ px,
+                                                      const double* __restrict__ py, const double* __restrict__ pz,
+                                                      int64_t npts, double* __restrict__ out0, double* __restrict__ out1,
+                                                      double* __restrict__ out2, double* __restrict__ out3,
+                                                      int64_t out_stride)
 {
     int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     const bool live = i < npts;
@@ -44,32 +45,72 @@ __global__ __launch_bounds__(kBlock) void k_field_fp64(const double* __restrict_
     double s0 = 0., s1 = 0., s2 = 0., s3 = 0.;
     double c0 = 0., c1 = 0., c2 = 0., c3 = 0.;
     for (int64_t j = j0; j < j1; j++) {
-        const F4 v = dev::element_eval<true>(geom + j * KG_STRIDE, nodes + j * KEM_NODE_STRIDE, POLICY, x, y, z);
-        const double sj = sigma[j];
-        double yk, t;
-        yk = v.x * sj - c0;
-        t = s0 + yk;
-        c0 = (t - s0) - yk;
-        s0 = t;
-        yk = v.y * sj - c1;
-        t = s1 + yk;
-        c1 = (t - s1) - yk;
-        s1 = t;
-        yk = v.z * sj - c2;
-        t = s2 + yk;
-        c2 = (t - s2) - yk;
-        s2 = t;
-        yk = v.p * sj - c3;
-        t = s3 + yk;
-        c3 = (t - s3) - yk;
-        s3 = t;
+        const double* g = geom + j * KG_STRIDE;
+        const int type = (int)g[KG_TYPE];
+        const double dr = dev::distance_ratio(g, x, y, z);
+        if (!dev::is_near(type, dr, 0)) {
+            const F4 v = dev::element_far(g, nodes + j * KEM_NODE_STRIDE, type, dr, x, y, z);
+            const double sj = sigma[j];
+            dev::kahan_add(v.x, sj, s0, c0);
+            dev::kahan_add(v.y, sj, s1, c1);
+            dev::kahan_add(v.z, sj, s2, c2);
+            dev::kahan_add(v.p, sj, s3, c3);
+        }
     }
     if (live) {
         const int64_t o = (int64_t)blockIdx.y * out_stride + i;
-        if (out0) out0[o] = s0;
-        if (out1) out1[o] = s1;
-        if (out2) out2[o] = s2;
-        if (out3) out3[o] = s3;
+        out0[o] = s0;
+        out1[o] = s1;
+        out2[o] = s2;
+        out3[o] = s3;
+    }
+}
+
+// Near pass: closed-form pairs of the element range, added to the far pass's sums
+// (accumulate = 1) or from zero (accumulate = 0, the rwg policy).
+__global__ __launch_bounds__(kBlock) void k_field_near(const double* __restrict__ geom,
+                                                       const double* __restrict__ sigma, int64_t nelem,
+                                                       int64_t per_split, const double* __restrict__ px,
+                                                       const double* __restrict__ py, const double* __restrict__ pz,
+                                                       int64_t npts, int policy, int accumulate,
+                                                       double* __restrict__ out0, double* __restrict__ out1,
+                                                       double* __restrict__ out2, double* __restrict__ out3,
+                                                       int64_t out_stride)
+{
+    int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const bool live = i < npts;
+    if (!live) i = npts - 1;
+    const double x = px[i], y = py[i], z = pz[i];
+    const int64_t j0 = (int64_t)blockIdx.y * per_split;
+    int64_t j1 = j0 + per_split;
+    if (j1 > nelem) j1 = nelem;
+    const int64_t o = (int64_t)blockIdx.y * out_stride + i;
+    double s0 = 0., s1 = 0., s2 = 0., s3 = 0.;
+    if (accumulate) {
+        s0 = out0[o];
+        s1 = out1[o];
+        s2 = out2[o];
+        s3 = out3[o];
+    }
+    double c0 = 0., c1 = 0., c2 = 0., c3 = 0.;
+    for (int64_t j = j0; j < j1; j++) {
+        const double* g = geom + j * KG_STRIDE;
+        const int type = (int)g[KG_TYPE];
+        const double dr = dev::distance_ratio(g, x, y, z);
+        if (dev::is_near(type, dr, policy)) {
+            const F4 v = dev::element_near(g, type, x, y, z);
+            const double sj = sigma[j];
+            dev::kahan_add(v.x, sj, s0, c0);
+            dev::kahan_add(v.y, sj, s1, c1);
+            dev::kahan_add(v.z, sj, s2, c2);
+            dev::kahan_add(v.p, sj, s3, c3);
+        }
+    }
+    if (live) {
+        out0[o] = s0;
+        out1[o] = s1;
+        out2[o] = s2;
+        out3[o] = s3;
     }
 }
 
@@ -381,26 +422,20 @@ hipError_t launch_field_fp64(hipStream_t st, int policy, const double* geom, con
     const int64_t per = (nelem + nsplit - 1) / nsplit;
     nsplit = (nelem + per - 1) / per;
     dim3 grid(gx, (unsigned)nsplit);
-    if (nsplit == 1) {
-        if (policy == 0)
-            k_field_fp64<0><<<grid, kBlock, 0, st>>>(geom, nodes, sigma, nelem, per, px, py, pz, npts, ex, ey, ez,
-                                                     phi, 0);
-        else
-            k_field_fp64<1><<<grid, kBlock, 0, st>>>(geom, nodes, sigma, nelem, per, px, py, pz, npts, ex, ey, ez,
-                                                     phi, 0);
-        return hipGetLastError();
+    double *o0 = ex, *o1 = ey, *o2 = ez, *o3 = phi;
+    int64_t stride = 0;
+    if (nsplit > 1) {
+        o0 = ws;
+        o1 = ws + nsplit * npts;
+        o2 = ws + 2 * nsplit * npts;
+        o3 = ws + 3 * nsplit * npts;
+        stride = npts;
     }
-    double* w0 = ws;
-    double* w1 = ws + nsplit * npts;
-    double* w2 = ws + 2 * nsplit * npts;
-    double* w3 = ws + 3 * nsplit * npts;
     if (policy == 0)
-        k_field_fp64<0><<<grid, kBlock, 0, st>>>(geom, nodes, sigma, nelem, per, px, py, pz, npts, w0, w1, w2, w3,
-                                                 npts);
-    else
-        k_field_fp64<1><<<grid, kBlock, 0, st>>>(geom, nodes, sigma, nelem, per, px, py, pz, npts, w0, w1, w2, w3,
-                                                 npts);
-    k_field_combine<<<gx, kBlock, 0, st>>>(ws, (int)nsplit, npts, phi, ex, ey, ez);
+        k_field_far<<<grid, kBlock, 0, st>>>(geom, nodes, sigma, nelem, per, px, py, pz, npts, o0, o1, o2, o3, stride);
+    k_field_near<<<grid, kBlock, 0, st>>>(geom, sigma, nelem, per, px, py, pz, npts, policy, policy == 0 ? 1 : 0, o0,
+                                          o1, o2, o3, stride);
+    if (nsplit > 1) k_field_combine<<<gx, kBlock, 0, st>>>(ws, (int)nsplit, npts, phi, ex, ey, ez);
     return hipGetLastError();
 }
 

@@ -236,6 +236,53 @@ def test_field_matches_oracle(ctx, oracle, cube6):
     assert (np.abs(phi2 - phi[:3]) / np.abs(phi[:3])).max() < 1e-14
 
 
+def test_field_sorted_path(ctx, oracle, cube6):
+    """From 4096 points on, points are visited in Morton order; results come back in the
+    caller's order and equal the unsorted evaluation and the oracle."""
+    from kassiopeia_amd import kemfield as K
+
+    m, Aref = cube6
+    sigma = np.linalg.solve(Aref, oracle.rhs(m.bc_value))
+    pts = G.points_shell(9000, 0.55, 2.0, 17)
+    cont = K.KSurfaceContainer(ctx, m)
+    ctx.set_integrator("numeric")
+    fs = K.KIntegratingElectrostaticFieldSolver(cont, sigma)
+    E, phi = fs.ElectricFieldAndPotential(pts)
+    parts = [fs.ElectricFieldAndPotential(pts[k:k + 3000]) for k in range(0, 9000, 3000)]
+    E2 = np.concatenate([p[0] for p in parts])
+    phi2 = np.concatenate([p[1] for p in parts])
+    assert (np.abs(phi - phi2) / np.abs(phi2)).max() < 1e-13
+    assert (np.linalg.norm(E - E2, axis=1) / np.linalg.norm(E2, axis=1)).max() < 1e-12
+    sel = np.arange(0, 9000, 45)
+    phr, Er = oracle.field(m.shapes, m.recs, sigma, pts[sel])
+    assert (np.abs(phi[sel] - phr) / np.abs(phr)).max() < TOL
+    assert max(relerr(E[k], Er[i]) for i, k in enumerate(sel)) < 1e-11
+
+
+@pytest.mark.parametrize("policy", ["numeric", "rwg"])
+def test_field_mixed_shapes(ctx, oracle, policy):
+    """Triangles, rectangles and thin wires in one container (group-major order)."""
+    from kassiopeia_amd import kemfield as K
+
+    tri = G.cube_mesh(2)
+    rect = G.cube_rect_mesh(2, edge=0.6)
+    rng = np.random.default_rng(9)
+    p0 = rng.normal(size=(6, 3)) * 0.3
+    wires = G.Mesh(np.full(6, 2, np.int32), G.line_records(p0, p0 + rng.normal(size=(6, 3)) * 0.2, 1e-3),
+                   np.zeros(6, np.int32), np.ones(6), np.zeros(6), np.zeros(6, np.int32))
+    m = G.concat(tri, rect, wires)
+    sigma = rng.normal(size=m.n)
+    pts = np.concatenate([G.points_shell(150, 0.05, 3.0, 4), m.centroids()[::7] + 1e-3])
+    cont = K.KSurfaceContainer(ctx, m)
+    ctx.set_integrator(policy)
+    E, phi = K.KIntegratingElectrostaticFieldSolver(cont, sigma).ElectricFieldAndPotential(pts)
+    ctx.set_integrator("numeric")
+    phr, Er = oracle.field(m.shapes, m.recs, sigma, pts, policy=0 if policy == "numeric" else 1)
+    tol = TOL if policy == "numeric" else 1e-10
+    assert (np.abs(phi - phr) / np.abs(phr)).max() < tol
+    assert max(relerr(E[k], Er[k]) for k in range(len(pts))) < 10 * tol
+
+
 def test_field_boundary_field_facade(ctx):
     from kassiopeia_amd import kemfield as K
 
