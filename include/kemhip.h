@@ -86,6 +86,16 @@ int kem_ctx_set_stream(kem_ctx* ctx, void* stream);
 void* kem_ctx_stream(kem_ctx* ctx);
 int kem_ctx_sync(kem_ctx* ctx);
 const char* kem_last_error(const kem_ctx* ctx);
+/* KEM_MIXED field evaluation: far pairs with distance ratio <= ratio keep FP64 cubature, the
+   rest run in FP32 (default 0: every far pair in FP32).  No reference counterpart: the
+   reference has no mixed-precision mode. */
+int kem_ctx_set_mixed_fp64_ratio(kem_ctx* ctx, double ratio);
+/* Instrumentation, no reference counterpart: with timing on, each field evaluation records HIP
+   events on the context's stream around its far-field kernel (up to 256 evaluations between
+   queries); kem_ctx_kernel_time waits for them and returns the summed kernel time and the
+   number of evaluations timed, then starts a new count. */
+int kem_ctx_set_kernel_timing(kem_ctx* ctx, int on);
+int kem_ctx_kernel_time(kem_ctx* ctx, double* total_ms, int* launches);
 /* integrator policy by name: "numeric" | "default" | "rwg"  (Factory::Make(name), unknown -> KEM_E_NAME) */
 int kem_set_integrator(kem_ctx* ctx, const char* name);
 
@@ -116,16 +126,20 @@ int kem_element_eval(kem_ctx* ctx, const kem_mesh* mesh, int64_t npairs, const i
    sigma has mesh-size entries; phi (npts) and E (npts x 3) may be NULL to skip. */
 int kem_field_eval(kem_ctx* ctx, const kem_mesh* mesh, const double* sigma, int64_t npts, const double* P,
                    int precision, double* phi, double* E);
-/* Same on device arrays (SoA point streams), asynchronous on the context's stream. */
+/* Same on device arrays (SoA point streams), asynchronous on the context's stream.
+   precision KEM_MIXED: far-field cubature in FP32 (node positions relative to the element
+   centre, point-to-centre offset rounded once from FP64, accuracy-targeted rule ladder),
+   closed-form near field and all sums over elements in FP64. */
 int kem_field_eval_device(kem_ctx* ctx, const kem_mesh* mesh, const double* d_sigma, int64_t npts,
                           const double* d_x, const double* d_y, const double* d_z, int precision, double* d_phi,
                           double* d_ex, double* d_ey, double* d_ez);
 
 /* Instrumentation (no reference counterpart): number of (point, element) pairs of a field
-   evaluation per quadrature regime of the numeric policy, counts[11] = triangle 7/12/33-point/
-   closed form, rectangle 7/12/33/closed form, segment 4-node/16-node/analytic.  Device points. */
+   evaluation per quadrature regime of the numeric policy at the given precision (KEM_MIXED
+   uses the FP32 far-field rule ladder), counts[11] = triangle 7/12/33-point/closed form,
+   rectangle 7/12/33/closed form, segment 4-node/16-node/analytic.  Device points. */
 int kem_field_census(kem_ctx* ctx, const kem_mesh* mesh, int64_t npts, const double* d_x, const double* d_y,
-                     const double* d_z, int64_t* counts);
+                     const double* d_z, int precision, int64_t* counts);
 
 /* ---- dense influence matrix: KBoundaryIntegralMatrix<KElectrostaticBoundaryIntegrator, true>
    (BoundaryIntegrals/Core/include/KBoundaryIntegralMatrix.hh:16-99) as a device-resident

@@ -57,6 +57,9 @@ SIGNATURES = {
     "kem_ctx_destroy": (None, [_P]),
     "kem_ctx_set_stream": (_INT, [_P, _P]),
     "kem_ctx_stream": (_P, [_P]),
+    "kem_ctx_set_mixed_fp64_ratio": (_INT, [_P, _D]),
+    "kem_ctx_set_kernel_timing": (_INT, [_P, _INT]),
+    "kem_ctx_kernel_time": (_INT, [_P, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)]),
     "kem_ctx_sync": (_INT, [_P]),
     "kem_last_error": (ctypes.c_char_p, [_P]),
     "kem_set_integrator": (_INT, [_P, ctypes.c_char_p]),
@@ -68,7 +71,7 @@ SIGNATURES = {
     "kem_element_eval": (_INT, [_P, _P, _I64, _P, _P, _P]),
     "kem_field_eval": (_INT, [_P, _P, _P, _I64, _P, _INT, _P, _P]),
     "kem_field_eval_device": (_INT, [_P, _P, _P, _I64, _P, _P, _P, _INT, _P, _P, _P, _P]),
-    "kem_field_census": (_INT, [_P, _P, _I64, _P, _P, _P, _P]),
+    "kem_field_census": (_INT, [_P, _P, _I64, _P, _P, _P, _INT, _P]),
     "kem_matrix_assemble": (_INT, [_P, _P, _INT, ctypes.POINTER(_P)]),
     "kem_matrix_destroy": (None, [_P]),
     "kem_matrix_dimension": (_I64, [_P]),

@@ -18,11 +18,16 @@
 #include "kem_layout.h"
 #include "kem_prep.h"
 
+// mixed precision: far pairs up to this distance ratio keep FP64 cubature; 0 = every far pair
+// in FP32 (the FP32 rule ladder alone keeps the global field error near 3e-8)
+constexpr double kMixedFp64RatioDefault = 0.0;
+
 struct kem_ctx {
     int device = 0;
     hipStream_t own = nullptr;
     hipStream_t stream = nullptr;
     int policy = KEM_POLICY_NUMERIC;
+    double mixed_fp64_ratio = kMixedFp64RatioDefault;
     std::string err;
     double* ws = nullptr;  // field partial sums / gemv partials
     int64_t ws_doubles = 0;
@@ -30,6 +35,11 @@ struct kem_ctx {
     int64_t scal_doubles = 0;
     void* sort_ws = nullptr;  // point ordering workspace (grow-only)
     size_t sort_ws_bytes = 0;
+    // far-kernel timing: one event pair per field evaluation since the last query
+    static constexpr int kTimedMax = 256;
+    bool timing = false;
+    hipEvent_t ev[2 * kTimedMax] = {};
+    int ev_used = 0;
 };
 
 struct kem_mesh {
@@ -38,6 +48,9 @@ struct kem_mesh {
     double* geom = nullptr;
     double* nodes = nullptr;
     double* targets = nullptr;
+    float* nodes32 = nullptr;  // mixed-precision far-field node table
+    double* clusters = nullptr;  // near-pass element clusters (centre, radius)
+    int32_t* order = nullptr;    // elements in cluster order
     std::vector<int32_t> shape, bc;
     std::vector<double> value, flux;
     bool has_neumann = false;
@@ -147,6 +160,8 @@ void kem_ctx_destroy(kem_ctx* c)
     dfree(c->ws);
     dfree(c->scal);
     if (c->sort_ws) (void)hipFree(c->sort_ws);
+    for (hipEvent_t& e : c->ev)
+        if (e) (void)hipEventDestroy(e);
     if (c->own) (void)hipStreamDestroy(c->own);
     delete c;
 }
@@ -159,6 +174,42 @@ int kem_ctx_set_stream(kem_ctx* c, void* stream)
 }
 
 void* kem_ctx_stream(kem_ctx* c) { return c ? (void*)c->stream : nullptr; }
+
+int kem_ctx_set_mixed_fp64_ratio(kem_ctx* c, double ratio)
+{
+    if (!c || !(ratio >= 0.)) return KEM_E_ARG;
+    c->mixed_fp64_ratio = ratio;
+    return KEM_OK;
+}
+
+int kem_ctx_set_kernel_timing(kem_ctx* c, int on)
+{
+    if (!c) return KEM_E_ARG;
+    KEM_CK(c, hipSetDevice(c->device));
+    if (on)
+        for (hipEvent_t& e : c->ev)
+            if (!e) KEM_CK(c, hipEventCreate(&e));
+    c->timing = on != 0;
+    c->ev_used = 0;
+    return KEM_OK;
+}
+
+int kem_ctx_kernel_time(kem_ctx* c, double* total_ms, int* launches)
+{
+    if (!c || !total_ms || !launches) return KEM_E_ARG;
+    KEM_CK(c, hipSetDevice(c->device));
+    double t = 0.;
+    for (int k = 0; k < c->ev_used; k++) {
+        KEM_CK(c, hipEventSynchronize(c->ev[2 * k + 1]));
+        float ms = 0.f;
+        KEM_CK(c, hipEventElapsedTime(&ms, c->ev[2 * k], c->ev[2 * k + 1]));
+        t += ms;
+    }
+    *total_ms = t;
+    *launches = c->ev_used;
+    c->ev_used = 0;
+    return KEM_OK;
+}
 
 int kem_ctx_sync(kem_ctx* c)
 {
@@ -192,6 +243,7 @@ int kem_mesh_create(kem_ctx* c, int64_t n, const int32_t* shape, const double* d
     if (n <= 0 || !shape || !data) return fail(c, KEM_E_ARG, "mesh: empty or null arrays");
     KEM_CK(c, hipSetDevice(c->device));
     std::vector<double> g((size_t)n * KG_STRIDE), nd((size_t)n * KEM_NODE_STRIDE), t((size_t)n * KT_STRIDE);
+    std::vector<float> nd32((size_t)n * KEM_NODE_STRIDE);
     kem_mesh* m = new kem_mesh();
     m->ctx = c;
     m->n = n;
@@ -214,10 +266,23 @@ int kem_mesh_create(kem_ctx* c, int64_t n, const int32_t* shape, const double* d
             return fail(c, KEM_E_GEOM, "mesh: degenerate or non-finite element " + std::to_string(i));
         }
         kem::prep_target(shape[i], data + 11 * i, m->bc[i], m->flux[i], t.data() + (size_t)i * KT_STRIDE);
+        kem::prep_nodes32(g.data() + (size_t)i * KG_STRIDE, nd.data() + (size_t)i * KEM_NODE_STRIDE,
+                          nd32.data() + (size_t)i * KEM_NODE_STRIDE);
     }
+    std::vector<int32_t> order((size_t)n);
+    std::vector<double> clusters((size_t)(4 * ((n + KEM_CLUSTER - 1) / KEM_CLUSTER)));
+    kem::build_clusters(n, g.data(), order.data(), clusters.data());
     hipError_t e = dalloc(&m->geom, (int64_t)g.size());
+    if (e == hipSuccess) e = dalloc(&m->clusters, (int64_t)clusters.size());
+    if (e == hipSuccess) e = dalloc(&m->order, n);
+    if (e == hipSuccess)
+        e = hipMemcpy(m->clusters, clusters.data(), sizeof(double) * clusters.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(m->order, order.data(), sizeof(int32_t) * order.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = dalloc(&m->nodes, (int64_t)nd.size());
     if (e == hipSuccess) e = dalloc(&m->targets, (int64_t)t.size());
+    if (e == hipSuccess) e = dalloc(&m->nodes32, (int64_t)nd32.size());
+    if (e == hipSuccess)
+        e = hipMemcpy(m->nodes32, nd32.data(), sizeof(float) * nd32.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(m->geom, g.data(), sizeof(double) * g.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(m->nodes, nd.data(), sizeof(double) * nd.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(m->targets, t.data(), sizeof(double) * t.size(), hipMemcpyHostToDevice);
@@ -236,6 +301,9 @@ void kem_mesh_destroy(kem_mesh* m)
     dfree(m->geom);
     dfree(m->nodes);
     dfree(m->targets);
+    dfree(m->nodes32);
+    dfree(m->clusters);
+    dfree(m->order);
     delete m;
 }
 
@@ -287,9 +355,16 @@ int kem_field_eval_device(kem_ctx* c, const kem_mesh* m, const double* sigma, in
 {
     if (!c || !m || !sigma || (npts > 0 && (!x || !y || !z))) return KEM_E_ARG;
     if (m->ctx != c) return fail(c, KEM_E_STATE, "mesh belongs to another context");
-    if (precision != KEM_FP64) return fail(c, KEM_E_ARG, "field: precision mode not available");
+    if (precision != KEM_FP64 && precision != KEM_MIXED) return fail(c, KEM_E_ARG, "field: unknown precision mode");
+    const float* n32 = precision == KEM_MIXED ? m->nodes32 : nullptr;
     if (npts <= 0) return KEM_OK;
     KEM_CK(c, hipSetDevice(c->device));
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    if (c->timing && c->ev_used < kem_ctx::kTimedMax) {
+        t0 = c->ev[2 * c->ev_used];
+        t1 = c->ev[2 * c->ev_used + 1];
+        c->ev_used++;
+    }
     const int64_t need = 4 * (int64_t)(2048 + 256) * 256;
     KEM_CK(c, ensure_ws(c, need));
     if (npts >= kSortMinPoints && npts <= INT32_MAX) {
@@ -302,24 +377,26 @@ int kem_field_eval_device(kem_ctx* c, const kem_mesh* m, const double* sigma, in
             KEM_CK(c, hipMalloc(&c->sort_ws, bytes));
             c->sort_ws_bytes = bytes;
         }
-        KEM_CK(c, kem::launch_field_sorted(c->stream, c->policy, m->geom, m->nodes, sigma, m->n, x, y, z, npts, phi,
-                                           ex, ey, ez, c->sort_ws, c->sort_ws_bytes, c->ws, c->ws_doubles));
+        KEM_CK(c, kem::launch_field_sorted(c->stream, c->policy, m->geom, m->nodes, n32, c->mixed_fp64_ratio, sigma, m->n,
+                                           m->clusters, m->order, x, y, z, npts, phi, ex, ey, ez, c->sort_ws, c->sort_ws_bytes, c->ws, c->ws_doubles, t0, t1));
         return KEM_OK;
     }
-    KEM_CK(c, kem::launch_field_fp64(c->stream, c->policy, m->geom, m->nodes, sigma, m->n, x, y, z, npts, phi, ex,
-                                     ey, ez, c->ws, c->ws_doubles));
+    KEM_CK(c, kem::launch_field(c->stream, c->policy, m->geom, m->nodes, n32, c->mixed_fp64_ratio, sigma, m->n,
+                                m->clusters, m->order, x, y, z, npts, phi, ex, ey, ez, c->ws, c->ws_doubles, t0, t1));
     return KEM_OK;
 }
 
 int kem_field_census(kem_ctx* c, const kem_mesh* m, int64_t npts, const double* x, const double* y, const double* z,
-                     int64_t* counts)
+                     int precision, int64_t* counts)
 {
     if (!c || !m || !counts || (npts > 0 && (!x || !y || !z))) return KEM_E_ARG;
     KEM_CK(c, hipSetDevice(c->device));
     unsigned long long* d = nullptr;
     KEM_CK(c, dalloc(&d, 12));
     hipError_t e = hipMemsetAsync(d, 0, sizeof(unsigned long long) * 12, c->stream);
-    if (e == hipSuccess) e = kem::launch_census(c->stream, m->geom, m->n, x, y, z, npts, d);
+    if (e == hipSuccess)
+        e = kem::launch_census(c->stream, m->geom, m->n, x, y, z, npts, precision == KEM_MIXED ? 1 : 0,
+                               c->mixed_fp64_ratio, d);
     unsigned long long h[12];
     if (e == hipSuccess) e = hipMemcpyAsync(h, d, sizeof(h), hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);

@@ -320,7 +320,9 @@ __device__ __forceinline__ F4 closed_form(const double* __restrict__ g, double p
 __device__ __forceinline__ double distance_ratio(const double* __restrict__ g, double px, double py, double pz)
 {
     const double dx = g[KG_CX] - px, dy = g[KG_CY] - py, dz = g[KG_CZ] - pz;
-    return sqrt(dx * dx + dy * dy + dz * dz) * g[KG_INVSIZE];
+    // explicit FMAs: every kernel classifies a pair with identical bits, so the far and near
+    // passes partition the pairs exactly
+    return sqrt(__builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx))) * g[KG_INVSIZE];
 }
 
 // One (element, point) pair under the "numeric" policy (default) or the closed form everywhere
@@ -355,36 +357,38 @@ __device__ __forceinline__ F4 element_eval(const double* __restrict__ g, const d
     return line_analytic(g, px, py, pz);
 }
 
-// Far/near split of the numeric policy for the field kernels: a pair is "near" when the
-// dispatch above sends it to a closed form (triangle dr <= 3.42, rectangle dr <= 4.6, segment
-// dr <= 2; every pair under the rwg policy).  The far pass evaluates only cubature / Gauss-
-// Legendre pairs and stays register-lean; the near pass evaluates only closed-form pairs.
-__device__ __forceinline__ bool is_near(int type, double dr, int policy)
+// Squared distance |centre - P|^2 with explicit FMAs: every field kernel classifies a pair
+// from these identical bits, so the far and near passes partition the pairs exactly.
+__device__ __forceinline__ double dist2(const double* __restrict__ g, double px, double py, double pz)
 {
-    if (policy != 0) return true;
-    if (type == KEM_TRIANGLE) return !(dr > kTriRWG);
-    if (type == KEM_RECTANGLE) return !(dr > kRectRWG);
-    return !(dr > kLineAna);
+    const double dx = g[KG_CX] - px, dy = g[KG_CY] - py, dz = g[KG_CZ] - pz;
+    return __builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx));
 }
 
-// Far pass of one pair (zero for near pairs).
+// Far/near split of the numeric policy for the field kernels: a pair is "near" when the
+// dispatch sends it to a closed form (dr <= 3.42 triangle, 4.6 rectangle, 2 segment; every
+// pair under the rwg policy).  The thresholds are compared as squared distances against
+// (dr_cut * size)^2 precomputed per element (kem_layout.h), so no square root is taken per
+// pair; this agrees with the reference's `dr > cut` except for distances within an ulp of a
+// cut-off.  The far pass evaluates only cubature / Gauss-Legendre pairs and stays
+// register-lean; the near pass evaluates only closed-form pairs.
+__device__ __forceinline__ bool is_near2(const double* __restrict__ g, double r2, int policy)
+{
+    return policy != 0 || !(r2 > g[KG_NEAR2]);
+}
+
+// Far pass of one pair (reference ladder, FP64).  Called only for pairs outside the near set.
 __device__ __forceinline__ F4 element_far(const double* __restrict__ g, const double* __restrict__ nd, int type,
-                                          double dr, double px, double py, double pz)
+                                          double r2, double px, double py, double pz)
 {
     const double pref = g[KG_PREF];
-    if (type == KEM_TRIANGLE) {
-        if (dr > kTriCub12) return cubature<7, true>(nd + 4 * KEM_N7, pref, px, py, pz);
-        if (dr > kTriCub33) return cubature<12, true>(nd + 4 * KEM_N12, pref, px, py, pz);
-        if (dr > kTriRWG) return cubature<33, true>(nd + 4 * KEM_N33, pref, px, py, pz);
-    } else if (type == KEM_RECTANGLE) {
-        if (dr > kRectCub12) return cubature<7, true>(nd + 4 * KEM_N7, pref, px, py, pz);
-        if (dr > kRectCub33) return cubature<12, true>(nd + 4 * KEM_N12, pref, px, py, pz);
-        if (dr > kRectRWG) return cubature<33, true>(nd + 4 * KEM_N33, pref, px, py, pz);
-    } else {
-        if (dr > kLineQuad16) return line_quadrature<2, true>(nd + 4 * KEM_L4, pref, px, py, pz);
-        if (dr > kLineAna) return line_quadrature<8, true>(nd + 4 * KEM_L16, pref, px, py, pz);
+    if (type == KEM_LINESEGMENT) {
+        if (r2 > g[KG_REF7_2]) return line_quadrature<2, true>(nd + 4 * KEM_L4, pref, px, py, pz);
+        return line_quadrature<8, true>(nd + 4 * KEM_L16, pref, px, py, pz);
     }
-    return F4{0., 0., 0., 0.};
+    if (r2 > g[KG_REF7_2]) return cubature<7, true>(nd + 4 * KEM_N7, pref, px, py, pz);
+    if (r2 > g[KG_REF12_2]) return cubature<12, true>(nd + 4 * KEM_N12, pref, px, py, pz);
+    return cubature<33, true>(nd + 4 * KEM_N33, pref, px, py, pz);
 }
 
 // Near pass of one pair (closed forms).
@@ -393,6 +397,55 @@ __device__ __forceinline__ F4 element_near(const double* __restrict__ g, int typ
     if (type == KEM_TRIANGLE) return closed_form<3, true>(g, px, py, pz);
     if (type == KEM_RECTANGLE) return closed_form<4, true>(g, px, py, pz);
     return line_analytic(g, px, py, pz);
+}
+
+// ---------------------------------------------------------------- mixed precision (FP32 far field)
+// Rule ladder for the FP32 far field (thresholds KG_MIX7_2 / KG_MIX12_2 set in kem_prep.cpp).
+// The reference's cut-offs keep every pair at machine precision; at FP32 accuracy the
+// truncation error only has to stay well below FP32 rounding (~1e-7).  Measured max relative
+// truncation error (potential and field, 23 triangles / 20 rectangles of aspect <= 4, random
+// directions, against the closed form): triangle 7-point 1.1e-8 at dr = 8, 12-point 8.4e-9 at
+// dr = 4, 33-point < 1e-10 at dr >= 3.42; rectangle 7-point 1e-8 at dr = 12, 12-point 4.4e-9
+// at dr = 6.  Near pairs (the same closed-form set as the FP64 path) stay FP64.
+
+// FP32 node sums relative to the element centre: (dx, dy, dz) = P - c in FP32 (computed in
+// FP64 and rounded once), nodes q = Q - c in FP32, so P - Q = (P - c) - q carries the relative
+// accuracy of FP32 in |P - Q| instead of losing digits to the absolute coordinates.
+template <int NQ>
+__device__ __forceinline__ void cubature32(const float* __restrict__ nd, float dx, float dy, float dz, float& s0,
+                                           float& s1, float& s2, float& s3)
+{
+    s0 = s1 = s2 = s3 = 0.f;
+#pragma unroll
+    for (int g = 0; g < NQ; g++) {
+        const float ex = dx - nd[4 * g + 0];
+        const float ey = dy - nd[4 * g + 1];
+        const float ez = dz - nd[4 * g + 2];
+        const float w = nd[4 * g + 3];
+        const float ir = __builtin_amdgcn_rsqf(ex * ex + ey * ey + ez * ez);
+        const float wi = w * ir;
+        const float f = wi * ir * ir;
+        s0 += ex * f;
+        s1 += ey * f;
+        s2 += ez * f;
+        s3 += wi;
+    }
+}
+
+// Far pass of one pair in FP32 without the prefactor (the caller scales by pref * sigma in
+// FP64).  Called only for pairs outside the near set; r2 = dist2 (FP64).
+__device__ __forceinline__ void element_far32(const double* __restrict__ g, const float* __restrict__ nd, int type,
+                                              double r2, float dx, float dy, float dz, float& s0, float& s1, float& s2,
+                                              float& s3)
+{
+    if (type == KEM_LINESEGMENT) {
+        // node pairs stored consecutively, weights already scaled by the half length
+        if (r2 > g[KG_MIX7_2]) return cubature32<4>(nd + 4 * KEM_L4, dx, dy, dz, s0, s1, s2, s3);
+        return cubature32<16>(nd + 4 * KEM_L16, dx, dy, dz, s0, s1, s2, s3);
+    }
+    if (r2 > g[KG_MIX7_2]) return cubature32<7>(nd + 4 * KEM_N7, dx, dy, dz, s0, s1, s2, s3);
+    if (r2 > g[KG_MIX12_2]) return cubature32<12>(nd + 4 * KEM_N12, dx, dy, dz, s0, s1, s2, s3);
+    return cubature32<33>(nd + 4 * KEM_N33, dx, dy, dz, s0, s1, s2, s3);
 }
 
 // Kahan accumulation of sigma * v into (s, c), component-wise, as the reference sums

@@ -22,11 +22,10 @@ constexpr int kBlock = 256;
 // ------------------------------------------------------------------ field evaluation
 // One lane per field point.  Elements [j0, j1) are summed in container order with Kahan
 // compensation per component (KElectrostaticIntegratingFieldSolver.hh:303-339).  The far pass
-// evaluates the cubature / Gauss-Legendre pairs; the near pass (k_field_near) then adds the
-// closed-form pairs of the same element range to the same sums.  Keeping the closed forms out
-// of this kernel keeps its register use low enough for high occupancy.  With nsplit > 1 each
-// blockIdx.y handles one contiguous element range and writes its partial sums;
-// k_field_combine adds the partials in range order.
+// evaluates the cubature / Gauss-Legendre pairs; the near pass then adds the closed-form pairs
+// to the same sums.  Keeping the closed forms out of this kernel keeps its register use low
+// enough for full occupancy.  With nsplit > 1 each blockIdx.y handles one contiguous element
+// range and writes its partial sums; k_field_combine adds the partials in range order.
 __global__ __launch_bounds__(kBlock) void k_field_far(const double* __restrict__ geom, const double* __restrict__ nodes,
                                                       const double* __restrict__ sigma, int64_t nelem,
                                                       int64_t per_split, const double* __restrict__ px,
@@ -46,10 +45,9 @@ __global__ __launch_bounds__(kBlock) void k_field_far(const double* __restrict__
     double c0 = 0., c1 = 0., c2 = 0., c3 = 0.;
     for (int64_t j = j0; j < j1; j++) {
         const double* g = geom + j * KG_STRIDE;
-        const int type = (int)g[KG_TYPE];
-        const double dr = dev::distance_ratio(g, x, y, z);
-        if (!dev::is_near(type, dr, 0)) {
-            const F4 v = dev::element_far(g, nodes + j * KEM_NODE_STRIDE, type, dr, x, y, z);
+        const double r2 = dev::dist2(g, x, y, z);
+        if (!dev::is_near2(g, r2, 0)) {
+            const F4 v = dev::element_far(g, nodes + j * KEM_NODE_STRIDE, (int)g[KG_TYPE], r2, x, y, z);
             const double sj = sigma[j];
             dev::kahan_add(v.x, sj, s0, c0);
             dev::kahan_add(v.y, sj, s1, c1);
@@ -66,8 +64,63 @@ __global__ __launch_bounds__(kBlock) void k_field_far(const double* __restrict__
     }
 }
 
-// Near pass: closed-form pairs of the element range, added to the far pass's sums
-// (accumulate = 1) or from zero (accumulate = 0, the rwg policy).
+// Mixed-precision far pass: the same pairs as k_field_far.  Pairs beyond the distance ratio
+// fp64_ratio run with FP32 node sums (dev::element_far32, accuracy-targeted rule ladder); the
+// band between the closed-form limit and fp64_ratio (empty by default) keeps the FP64 rules.
+// All sums over elements are FP64.
+__global__ __launch_bounds__(kBlock) void k_field_far32(const double* __restrict__ geom,
+                                                        const double* __restrict__ nodes,
+                                                        const float* __restrict__ nodes32, double fp64_ratio,
+                                                        const double* __restrict__ sigma, int64_t nelem,
+                                                        int64_t per_split, const double* __restrict__ px,
+                                                        const double* __restrict__ py, const double* __restrict__ pz,
+                                                        int64_t npts, double* __restrict__ out0,
+                                                        double* __restrict__ out1, double* __restrict__ out2,
+                                                        double* __restrict__ out3, int64_t out_stride)
+{
+    int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const bool live = i < npts;
+    if (!live) i = npts - 1;
+    const double x = px[i], y = py[i], z = pz[i];
+    const int64_t j0 = (int64_t)blockIdx.y * per_split;
+    int64_t j1 = j0 + per_split;
+    if (j1 > nelem) j1 = nelem;
+    const double band2 = fp64_ratio * fp64_ratio;
+    double a0 = 0., a1 = 0., a2 = 0., a3 = 0.;
+    for (int64_t j = j0; j < j1; j++) {
+        const double* g = geom + j * KG_STRIDE;
+        const double r2 = dev::dist2(g, x, y, z);
+        if (dev::is_near2(g, r2, 0)) continue;
+        const int type = (int)g[KG_TYPE];
+        if (band2 == 0. || r2 * (g[KG_INVSIZE] * g[KG_INVSIZE]) > band2) {
+            const float dx = (float)(x - g[KG_CX]), dy = (float)(y - g[KG_CY]), dz = (float)(z - g[KG_CZ]);
+            float s0, s1, s2, s3;
+            dev::element_far32(g, nodes32 + j * KEM_NODE_STRIDE, type, r2, dx, dy, dz, s0, s1, s2, s3);
+            const double wj = g[KG_PREF] * sigma[j];
+            a0 += (double)s0 * wj;
+            a1 += (double)s1 * wj;
+            a2 += (double)s2 * wj;
+            a3 += (double)s3 * wj;
+        } else {
+            const F4 v = dev::element_far(g, nodes + j * KEM_NODE_STRIDE, type, r2, x, y, z);
+            const double sj = sigma[j];
+            a0 += v.x * sj;
+            a1 += v.y * sj;
+            a2 += v.z * sj;
+            a3 += v.p * sj;
+        }
+    }
+    if (live) {
+        const int64_t o = (int64_t)blockIdx.y * out_stride + i;
+        out0[o] = a0;
+        out1[o] = a1;
+        out2[o] = a2;
+        out3[o] = a3;
+    }
+}
+
+// Near pass over all elements of a range (every pair under the rwg policy): closed-form
+// pairs added to the far pass's sums (accumulate = 1) or from zero (accumulate = 0).
 __global__ __launch_bounds__(kBlock) void k_field_near(const double* __restrict__ geom,
                                                        const double* __restrict__ sigma, int64_t nelem,
                                                        int64_t per_split, const double* __restrict__ px,
@@ -95,10 +148,8 @@ __global__ __launch_bounds__(kBlock) void k_field_near(const double* __restrict_
     double c0 = 0., c1 = 0., c2 = 0., c3 = 0.;
     for (int64_t j = j0; j < j1; j++) {
         const double* g = geom + j * KG_STRIDE;
-        const int type = (int)g[KG_TYPE];
-        const double dr = dev::distance_ratio(g, x, y, z);
-        if (dev::is_near(type, dr, policy)) {
-            const F4 v = dev::element_near(g, type, x, y, z);
+        if (dev::is_near2(g, dev::dist2(g, x, y, z), policy)) {
+            const F4 v = dev::element_near(g, (int)g[KG_TYPE], x, y, z);
             const double sj = sigma[j];
             dev::kahan_add(v.x, sj, s0, c0);
             dev::kahan_add(v.y, sj, s1, c1);
@@ -111,6 +162,71 @@ __global__ __launch_bounds__(kBlock) void k_field_near(const double* __restrict_
         out1[o] = s1;
         out2[o] = s2;
         out3[o] = s3;
+    }
+}
+
+__device__ __forceinline__ double wave_min(double v)
+{
+    for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ __forceinline__ double wave_max(double v)
+{
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// Near pass with cluster culling (numeric policy).  Each wavefront bounds its 64 points by the
+// sphere around their bounding box; a cluster of KEM_CLUSTER elements (Morton order) whose
+// sphere of near distances cannot reach it is skipped as a whole.  The pairs evaluated are
+// exactly the near set (the same dist2 test as the far pass); culling only removes clusters
+// proven to contain none.  Adds to the final sums (after the far pass and any combine).
+__global__ __launch_bounds__(kBlock) void k_field_near_culled(const double* __restrict__ geom,
+                                                              const double* __restrict__ sigma,
+                                                              const double* __restrict__ clusters, int64_t ncl,
+                                                              const int32_t* __restrict__ order, int64_t nelem,
+                                                              const double* __restrict__ px,
+                                                              const double* __restrict__ py,
+                                                              const double* __restrict__ pz, int64_t npts,
+                                                              double* __restrict__ out0, double* __restrict__ out1,
+                                                              double* __restrict__ out2, double* __restrict__ out3)
+{
+    int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const bool live = i < npts;
+    if (!live) i = npts - 1;
+    const double x = px[i], y = py[i], z = pz[i];
+    // bounding sphere of the wavefront's points
+    const double lx = wave_min(x), ly = wave_min(y), lz = wave_min(z);
+    const double hx = wave_max(x), hy = wave_max(y), hz = wave_max(z);
+    const double wx = 0.5 * (lx + hx), wy = 0.5 * (ly + hy), wz = 0.5 * (lz + hz);
+    const double wr = 0.5 * sqrt((hx - lx) * (hx - lx) + (hy - ly) * (hy - ly) + (hz - lz) * (hz - lz));
+    double s0 = out0[i], s1 = out1[i], s2 = out2[i], s3 = out3[i];
+    double c0 = 0., c1 = 0., c2 = 0., c3 = 0.;
+    for (int64_t k = 0; k < ncl; k++) {
+        const double* cl = clusters + 4 * k;
+        const double dx = cl[0] - wx, dy = cl[1] - wy, dz = cl[2] - wz;
+        const double reach = (cl[3] + wr) * (1. + 1e-9) + 1e-12;
+        if (dx * dx + dy * dy + dz * dz > reach * reach) continue;
+        int64_t e1 = (k + 1) * KEM_CLUSTER;
+        if (e1 > nelem) e1 = nelem;
+        for (int64_t e = k * KEM_CLUSTER; e < e1; e++) {
+            const int64_t j = order[e];
+            const double* g = geom + j * KG_STRIDE;
+            if (dev::is_near2(g, dev::dist2(g, x, y, z), 0)) {
+                const F4 v = dev::element_near(g, (int)g[KG_TYPE], x, y, z);
+                const double sj = sigma[j];
+                dev::kahan_add(v.x, sj, s0, c0);
+                dev::kahan_add(v.y, sj, s1, c1);
+                dev::kahan_add(v.z, sj, s2, c2);
+                dev::kahan_add(v.p, sj, s3, c3);
+            }
+        }
+    }
+    if (live) {
+        out0[i] = s0;
+        out1[i] = s1;
+        out2[i] = s2;
+        out3[i] = s3;
     }
 }
 
@@ -364,13 +480,14 @@ __global__ __launch_bounds__(kRHBlock) void k_robin_hood(const double* __restric
 }
 
 // ------------------------------------------------------------------ regime census
-// Counts the (point, element) pairs of a field evaluation by quadrature regime (numeric
-// policy): counts[0..3] triangle 7/12/33-point/closed form, [4..7] rectangle, [8..10] segment
-// 4-node/16-node/analytic.  Used to price a launch in FLOPs.
+// Counts the (point, element) pairs of a field evaluation by quadrature regime with the field
+// kernels' own classification (numeric policy; mixed = 1: the FP32 ladder beyond distance
+// ratio fp64_ratio): counts[0..3] triangle 7/12/33-point/closed form, [4..7] rectangle,
+// [8..10] segment 4-node/16-node/analytic.  Used to price a launch in FLOPs.
 __global__ __launch_bounds__(kBlock) void k_census(const double* __restrict__ geom, int64_t nelem, int64_t per_split,
                                                    const double* __restrict__ px, const double* __restrict__ py,
-                                                   const double* __restrict__ pz, int64_t npts,
-                                                   unsigned long long* __restrict__ counts)
+                                                   const double* __restrict__ pz, int64_t npts, int mixed,
+                                                   double fp64_ratio, unsigned long long* __restrict__ counts)
 {
     __shared__ unsigned long long sc[12];
     if (threadIdx.x < 12) sc[threadIdx.x] = 0;
@@ -385,15 +502,21 @@ __global__ __launch_bounds__(kBlock) void k_census(const double* __restrict__ ge
         for (int64_t j = j0; j < j1; j++) {
             const double* g = geom + j * KG_STRIDE;
             const int type = (int)g[KG_TYPE];
-            const double dr = dev::distance_ratio(g, x, y, z);
+            const double r2 = dev::dist2(g, x, y, z);
+            const int base = type == KEM_TRIANGLE ? 0 : type == KEM_RECTANGLE ? 4 : 8;
             int k;
-            if (type == KEM_TRIANGLE)
-                k = dr > dev::kTriCub12 ? 0 : dr > dev::kTriCub33 ? 1 : dr > dev::kTriRWG ? 2 : 3;
-            else if (type == KEM_RECTANGLE)
-                k = dr > dev::kRectCub12 ? 4 : dr > dev::kRectCub33 ? 5 : dr > dev::kRectRWG ? 6 : 7;
-            else
-                k = dr > dev::kLineQuad16 ? 8 : dr > dev::kLineAna ? 9 : 10;
-            c[k]++;
+            if (dev::is_near2(g, r2, 0)) {
+                k = type == KEM_LINESEGMENT ? 2 : 3;
+            } else {
+                const bool f32 = mixed && r2 * (g[KG_INVSIZE] * g[KG_INVSIZE]) > fp64_ratio * fp64_ratio;
+                const double t7 = f32 ? g[KG_MIX7_2] : g[KG_REF7_2];
+                const double t12 = f32 ? g[KG_MIX12_2] : g[KG_REF12_2];
+                if (type == KEM_LINESEGMENT)
+                    k = r2 > t7 ? 0 : 1;
+                else
+                    k = r2 > t7 ? 0 : r2 > t12 ? 1 : 2;
+            }
+            c[base + k]++;
         }
     }
     for (int k = 0; k < 11; k++)
@@ -407,10 +530,11 @@ inline unsigned nblocks(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock
 }  // namespace
 
 // ====================================================================== launchers
-hipError_t launch_field_fp64(hipStream_t st, int policy, const double* geom, const double* nodes,
-                             const double* sigma, int64_t nelem, const double* px, const double* py, const double* pz,
-                             int64_t npts, double* phi, double* ex, double* ey, double* ez, double* ws,
-                             int64_t ws_doubles)
+hipError_t launch_field(hipStream_t st, int policy, const double* geom, const double* nodes, const float* nodes32,
+                        double fp64_ratio, const double* sigma, int64_t nelem, const double* clusters,
+                        const int32_t* order, const double* px, const double* py, const double* pz, int64_t npts,
+                        double* phi, double* ex, double* ey, double* ez, double* ws, int64_t ws_doubles,
+                        hipEvent_t far_begin, hipEvent_t far_end)
 {
     if (npts <= 0) return hipSuccess;
     const unsigned gx = nblocks(npts);
@@ -431,21 +555,33 @@ hipError_t launch_field_fp64(hipStream_t st, int policy, const double* geom, con
         o3 = ws + 3 * nsplit * npts;
         stride = npts;
     }
-    if (policy == 0)
+    const bool culled = policy == 0 && clusters && order;
+    if (far_begin) (void)hipEventRecord(far_begin, st);
+    if (policy == 0 && nodes32)
+        k_field_far32<<<grid, kBlock, 0, st>>>(geom, nodes, nodes32, fp64_ratio, sigma, nelem, per, px, py, pz, npts, o0,
+                                               o1, o2, o3, stride);
+    else if (policy == 0)
         k_field_far<<<grid, kBlock, 0, st>>>(geom, nodes, sigma, nelem, per, px, py, pz, npts, o0, o1, o2, o3, stride);
-    k_field_near<<<grid, kBlock, 0, st>>>(geom, sigma, nelem, per, px, py, pz, npts, policy, policy == 0 ? 1 : 0, o0,
-                                          o1, o2, o3, stride);
+    if (far_end) (void)hipEventRecord(far_end, st);
+    if (!culled)
+        k_field_near<<<grid, kBlock, 0, st>>>(geom, sigma, nelem, per, px, py, pz, npts, policy, policy == 0 ? 1 : 0, o0,
+                                              o1, o2, o3, stride);
     if (nsplit > 1) k_field_combine<<<gx, kBlock, 0, st>>>(ws, (int)nsplit, npts, phi, ex, ey, ez);
+    if (culled) {
+        const int64_t ncl = (nelem + KEM_CLUSTER - 1) / KEM_CLUSTER;
+        k_field_near_culled<<<gx, kBlock, 0, st>>>(geom, sigma, clusters, ncl, order, nelem, px, py, pz, npts, ex, ey,
+                                                   ez, phi);
+    }
     return hipGetLastError();
 }
 
 hipError_t launch_census(hipStream_t st, const double* geom, int64_t nelem, const double* px, const double* py,
-                         const double* pz, int64_t npts, unsigned long long* counts)
+                         const double* pz, int64_t npts, int mixed, double fp64_ratio, unsigned long long* counts)
 {
     if (npts <= 0 || nelem <= 0) return hipSuccess;
     const int64_t per = 4096;
     dim3 grid(nblocks(npts), (unsigned)((nelem + per - 1) / per));
-    k_census<<<grid, kBlock, 0, st>>>(geom, nelem, per, px, py, pz, npts, counts);
+    k_census<<<grid, kBlock, 0, st>>>(geom, nelem, per, px, py, pz, npts, mixed, fp64_ratio, counts);
     return hipGetLastError();
 }
 

@@ -43,7 +43,15 @@ enum kem_geom_field {
     KG_M = 52,   /* in-plane outward edge normals m_e = u_e x n */
     KG_SV = 64,  /* full edge vectors (point-on-edge detection) */
     KG_IL = 76,  /* 1 / edge length */
-    KG_STRIDE = 80
+    /* squared distance thresholds |P - centre|^2 = (dr * size)^2 of the rule ladders:
+       near set (closed form), reference ladder (7- vs 12-point, 12- vs 33-point; segments
+       4- vs 16-node), mixed-precision FP32 ladder (7- vs 12-point, 12- vs 33-point) */
+    KG_NEAR2 = 80,
+    KG_REF7_2 = 81,
+    KG_REF12_2 = 82,
+    KG_MIX7_2 = 83,
+    KG_MIX12_2 = 84,
+    KG_STRIDE = 88
 };
 
 #define KEM_NODE_SLOTS 52
@@ -53,6 +61,12 @@ enum kem_geom_field {
 #define KEM_N33 19
 #define KEM_L4 0
 #define KEM_L16 4
+
+/* element clusters for the near pass: elements in Morton order of their centres, KEM_CLUSTER
+   per cluster; each cluster keeps (centre x, y, z, radius) with radius = max over its elements
+   of |c_j - centre| + near distance of j, so a point farther than radius from the centre has no
+   near pair in the cluster */
+#define KEM_CLUSTER 32
 
 /* collocation target record (one per element): centre, unit normal, boundary type, flux */
 enum kem_target_field { KT_CX = 0, KT_CY, KT_CZ, KT_NX, KT_NY, KT_NZ, KT_BC, KT_FLUX, KT_STRIDE = 8 };

@@ -10,8 +10,11 @@
 //   BoundaryIntegrals/Electrostatic/RWG/src/KElectrostaticRWG{Triangle,Rectangle}Integrator.cc
 #include "kem_prep.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <utility>
+#include <vector>
 
 namespace kem {
 
@@ -193,6 +196,34 @@ bool finite_all(const double* d, int n)
     return true;
 }
 
+// squared-distance thresholds of the rule ladders (see kem_layout.h); the distance-ratio
+// cut-offs are the reference's (KElectrostaticCubature*Integrator.hh, ...LineSegmentIntegrator.hh)
+// and the FP32 ladder's (kem_device.hpp)
+void set_thresholds(int type, double* g)
+{
+    const double size = 1. / g[KG_INVSIZE];
+    auto t2 = [&](double dr) { return (dr * size) * (dr * size); };
+    if (type == KEM_TRIANGLE) {
+        g[KG_NEAR2] = t2(3.42);
+        g[KG_REF7_2] = t2(131.4);
+        g[KG_REF12_2] = t2(30.4);
+        g[KG_MIX7_2] = t2(8.0);
+        g[KG_MIX12_2] = t2(4.0);
+    } else if (type == KEM_RECTANGLE) {
+        g[KG_NEAR2] = t2(4.6);
+        g[KG_REF7_2] = t2(196.2);
+        g[KG_REF12_2] = t2(44.6);
+        g[KG_MIX7_2] = t2(12.0);
+        g[KG_MIX12_2] = t2(6.0);
+    } else {
+        g[KG_NEAR2] = t2(2.0);
+        g[KG_REF7_2] = t2(27.7);
+        g[KG_REF12_2] = t2(27.7);
+        g[KG_MIX7_2] = t2(27.7);
+        g[KG_MIX12_2] = t2(27.7);
+    }
+}
+
 }  // namespace
 
 int prep_element(int type, const double* d, double* g, double* nodes)
@@ -215,6 +246,7 @@ int prep_element(int type, const double* d, double* g, double* nodes)
         g[KG_PREF] = 0.25 * d[6] / kEps0;
         g[KG_A] = L;
         g[KG_B] = d[6];
+        set_thresholds(type, g);
         line_nodes(d, nodes);
         return 0;
     }
@@ -299,6 +331,7 @@ int prep_element(int type, const double* d, double* g, double* nodes)
         g[KG_IL + 3] = 1. / d[1];
         rectangle_nodes(d, nodes);
     }
+    set_thresholds(type, g);
     // in-plane outward edge normals m_e = u_e x n
     const int ne = (int)g[KG_NE];
     for (int e = 0; e < ne; e++) {
@@ -310,6 +343,22 @@ int prep_element(int type, const double* d, double* g, double* nodes)
         m[2] = (u[0] * nv[1]) - (u[1] * nv[0]);
     }
     return 0;
+}
+
+void prep_nodes32(const double* g, const double* nodes, float* nodes32)
+{
+    for (int s = 0; s < KEM_NODE_SLOTS; s++) {
+        const double* q = nodes + 4 * s;
+        float* o = nodes32 + 4 * s;
+        if (q[3] == 0.) {
+            o[0] = o[1] = o[2] = o[3] = 0.f;
+            continue;
+        }
+        o[0] = (float)(q[0] - g[KG_CX]);
+        o[1] = (float)(q[1] - g[KG_CY]);
+        o[2] = (float)(q[2] - g[KG_CZ]);
+        o[3] = (float)q[3];
+    }
 }
 
 void prep_target(int type, const double* d, int bc, double flux, double* t)
@@ -332,6 +381,66 @@ void prep_target(int type, const double* d, int bc, double flux, double* t)
     }
     t[KT_BC] = bc;
     t[KT_FLUX] = flux;
+}
+
+namespace {
+uint64_t spread21(uint64_t v)
+{
+    v &= 0x1fffff;
+    v = (v | v << 32) & 0x1f00000000ffffULL;
+    v = (v | v << 16) & 0x1f0000ff0000ffULL;
+    v = (v | v << 8) & 0x100f00f00f00f00fULL;
+    v = (v | v << 4) & 0x10c30c30c30c30c3ULL;
+    v = (v | v << 2) & 0x1249249249249249ULL;
+    return v;
+}
+}  // namespace
+
+void build_clusters(int64_t n, const double* g, int32_t* order, double* clusters)
+{
+    double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int64_t j = 0; j < n; j++)
+        for (int k = 0; k < 3; k++) {
+            lo[k] = std::min(lo[k], g[j * KG_STRIDE + KG_CX + k]);
+            hi[k] = std::max(hi[k], g[j * KG_STRIDE + KG_CX + k]);
+        }
+    double ext = 0.;
+    for (int k = 0; k < 3; k++) ext = std::max(ext, hi[k] - lo[k]);
+    const double scale = ext > 0. ? 2097151. / ext : 0.;
+    std::vector<std::pair<uint64_t, int32_t>> key((size_t)n);
+    for (int64_t j = 0; j < n; j++) {
+        uint64_t m = 0;
+        for (int k = 0; k < 3; k++) {
+            double q = (g[j * KG_STRIDE + KG_CX + k] - lo[k]) * scale;
+            q = std::min(std::max(q, 0.), 2097151.);
+            m |= spread21((uint64_t)q) << k;
+        }
+        key[(size_t)j] = {m, (int32_t)j};
+    }
+    std::sort(key.begin(), key.end());
+    for (int64_t j = 0; j < n; j++) order[j] = key[(size_t)j].second;
+    const int64_t ncl = (n + KEM_CLUSTER - 1) / KEM_CLUSTER;
+    for (int64_t c = 0; c < ncl; c++) {
+        const int64_t e0 = c * KEM_CLUSTER, e1 = std::min(n, e0 + KEM_CLUSTER);
+        double blo[3] = {INFINITY, INFINITY, INFINITY}, bhi[3] = {-INFINITY, -INFINITY, -INFINITY};
+        for (int64_t e = e0; e < e1; e++)
+            for (int k = 0; k < 3; k++) {
+                blo[k] = std::min(blo[k], g[order[e] * (int64_t)KG_STRIDE + KG_CX + k]);
+                bhi[k] = std::max(bhi[k], g[order[e] * (int64_t)KG_STRIDE + KG_CX + k]);
+            }
+        double ctr[3];
+        for (int k = 0; k < 3; k++) ctr[k] = 0.5 * (blo[k] + bhi[k]);
+        double r = 0.;
+        for (int64_t e = e0; e < e1; e++) {
+            const double* ge = g + order[e] * (int64_t)KG_STRIDE;
+            const double d = std::sqrt(sq(ge[KG_CX] - ctr[0]) + sq(ge[KG_CY] - ctr[1]) + sq(ge[KG_CZ] - ctr[2]));
+            r = std::max(r, d + std::sqrt(ge[KG_NEAR2]));
+        }
+        clusters[4 * c + 0] = ctr[0];
+        clusters[4 * c + 1] = ctr[1];
+        clusters[4 * c + 2] = ctr[2];
+        clusters[4 * c + 3] = r * (1. + 1e-12);
+    }
 }
 
 }  // namespace kem

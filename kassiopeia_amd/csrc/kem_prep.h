@@ -4,6 +4,8 @@
 
 #include "kem_layout.h"
 
+#include <cstdint>
+
 namespace kem {
 
 // Kommon/Base/Utility/KConst_2021.h:37 (pi), :139-146 (eps0); KEMField/Source/Core/src/KEMConstants.cc:9-12
@@ -15,9 +17,19 @@ constexpr double kOneOverFourPiEps0 = 1. / (4. * kPi * kEps0);
 // nodes[KEM_NODE_STRIDE].  Returns 0, or -1 for a degenerate / non-finite element.
 int prep_element(int type, const double* shape_data, double* g, double* nodes);
 
+// FP32 node table for the mixed-precision far field: node positions relative to the
+// element's dispatch centre (KG_CX..KG_CZ of g) and weights, KEM_NODE_STRIDE floats.
+void prep_nodes32(const double* g, const double* nodes, float* nodes32);
+
 // Collocation target record t[KT_STRIDE]: the shape's Centroid(), unit normal, boundary type
 // and Neumann flux.
 void prep_target(int type, const double* shape_data, int bc, double flux, double* t);
+
+// Element clusters for the near pass: order[n] lists the elements in Morton order of their
+// dispatch centres; clusters[4 * ceil(n / KEM_CLUSTER)] holds, per run of KEM_CLUSTER
+// consecutive entries of order, the centre of their bounding box and a radius R such that
+// every point within the near distance of a member lies within R of the centre.
+void build_clusters(int64_t n, const double* g, int32_t* order, double* clusters);
 
 }  // namespace kem
 

@@ -283,6 +283,46 @@ def test_field_mixed_shapes(ctx, oracle, policy):
     assert max(relerr(E[k], Er[k]) for k in range(len(pts))) < 10 * tol
 
 
+@pytest.fixture(scope="module")
+def coax_small(ctx):
+    """1,280-triangle version of the benchmark's coaxial geometry (-18.6 kV / 0 V), solved in FP64."""
+    from kassiopeia_amd import kemfield as K
+
+    m = G.coax_mesh(nphi_in=12, nphi_out=24, nz=16, ncap_r=2, ncap_phi=16)
+    ctx.set_integrator("numeric")
+    cont = K.KSurfaceContainer(ctx, m)
+    A = K.KBoundaryIntegralMatrix(cont)
+    x, st, rc = A.solve(cont.rhs(), tolerance=1e-10)
+    assert rc == 0
+    return m, cont, x
+
+
+@pytest.mark.parametrize("npts", [3000, 20000])
+def test_field_mixed_precision(ctx, oracle, coax_small, npts):
+    """FP32 far field + FP64 near field and sums stays within 1e-6 of the FP64 path, for the
+    unsorted (< 4096 points) and the Morton-ordered path.  Bar (tolerance 1e-6): field error
+    norm-wise over the point set and its largest deviation relative to the RMS field; potential
+    relative to the electrode voltage scale.  The per-point relative field error is bounded
+    more loosely (it is amplified where |E| nearly vanishes).  The FP64 path itself is checked
+    against the oracle on a subset."""
+    from kassiopeia_amd import kemfield as K
+
+    m, cont, x = coax_small
+    pts = G.points_annulus(npts, 0.5, 1.0, 2.0, 31)
+    E64, p64 = K.KIntegratingElectrostaticFieldSolver(cont, x, "fp64").ElectricFieldAndPotential(pts)
+    Em, pm = K.KIntegratingElectrostaticFieldSolver(cont, x, "mixed").ElectricFieldAndPotential(pts)
+    dE = np.linalg.norm(Em - E64, axis=1)
+    assert np.linalg.norm(Em - E64) / np.linalg.norm(E64) < 1e-6
+    assert dE.max() / np.sqrt((E64**2).sum(1).mean()) < 1e-6
+    assert (dE / np.linalg.norm(E64, axis=1)).max() < 1e-5
+    ep = np.abs(pm - p64) / 18600.0
+    assert ep.max() < 1e-6, ep.max()
+    assert dE.max() > 0  # the mixed path is a different computation
+    sel = np.arange(0, npts, npts // 40)
+    phr, Er = oracle.field(m.shapes, m.recs, x, pts[sel])
+    assert (np.abs(p64[sel] - phr) / 18600.0).max() < 1e-12
+
+
 def test_field_boundary_field_facade(ctx):
     from kassiopeia_amd import kemfield as K
 
@@ -324,3 +364,21 @@ def test_solver_iteration_limit(ctx, cube6):
     x, st, rc = A.solve(cont.rhs(), tolerance=1e-14, max_iterations=3)
     assert rc == 1 and st.iterations == 3  # KEM_W_NOT_CONVERGED, best iterate returned
     assert np.isfinite(x).all()
+
+
+def test_field_mixed_fp64_band(ctx, coax_small):
+    """With the FP64 band covering every far pair the mixed kernel runs the FP64 rules (plain
+    FP64 sums instead of Kahan) and agrees with the FP64 path to rounding."""
+    from kassiopeia_amd import kemfield as K
+
+    m, cont, x = coax_small
+    pts = G.points_annulus(5000, 0.5, 1.0, 2.0, 7)
+    E64, p64 = K.KIntegratingElectrostaticFieldSolver(cont, x, "fp64").ElectricFieldAndPotential(pts)
+    ctx.check(ctx.lib.kem_ctx_set_mixed_fp64_ratio(ctx.h, 1e300))
+    try:
+        Eb, pb = K.KIntegratingElectrostaticFieldSolver(cont, x, "mixed").ElectricFieldAndPotential(pts)
+    finally:
+        ctx.check(ctx.lib.kem_ctx_set_mixed_fp64_ratio(ctx.h, 0.0))
+    assert np.linalg.norm(Eb - E64) / np.linalg.norm(E64) < 1e-13
+    assert (np.abs(pb - p64) / 18600.0).max() < 1e-13
+    assert ctx.lib.kem_ctx_set_mixed_fp64_ratio(ctx.h, -1.0) != 0
