@@ -39,6 +39,8 @@ extern "C" {
 /* ---- integrator policies (KElectrostaticBoundaryIntegratorFactory.cc:29-88) */
 #define KEM_POLICY_NUMERIC 0 /* default: cubature 7/12/33 by distance ratio, closed form near */
 #define KEM_POLICY_RWG 1     /* closed form at every distance (segments: analytic) */
+#define KEM_POLICY_ANALYTIC 2  /* Analytic triangle/rectangle/segment integrators */
+#define KEM_POLICY_REFERENCE 3 /* nested 32-node Gauss-Legendre (segments: 8 x 32 nodes) */
 
 /* ---- precision modes */
 #define KEM_FP64 0  /* plain path: FP64 everywhere */
@@ -96,7 +98,8 @@ int kem_ctx_set_mixed_fp64_ratio(kem_ctx* ctx, double ratio);
    number of evaluations timed, then starts a new count. */
 int kem_ctx_set_kernel_timing(kem_ctx* ctx, int on);
 int kem_ctx_kernel_time(kem_ctx* ctx, double* total_ms, int* launches);
-/* integrator policy by name: "numeric" | "default" | "rwg"  (Factory::Make(name), unknown -> KEM_E_NAME) */
+/* integrator policy by name: "numeric" | "default" | "analytic" | "rwg" | "reference"
+   (KElectrostaticBoundaryIntegratorFactory::Make(name), .cc:78-88; unknown -> KEM_E_NAME) */
 int kem_set_integrator(kem_ctx* ctx, const char* name);
 
 /* ---- surfaces: KSurfaceContainer of KSurface<KElectrostaticBasis, boundary, shape>

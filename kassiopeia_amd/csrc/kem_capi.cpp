@@ -49,6 +49,7 @@ struct kem_mesh {
     double* nodes = nullptr;
     double* targets = nullptr;
     float* nodes32 = nullptr;  // mixed-precision far-field node table
+    double* recs = nullptr;      // raw shape records (analytic / reference policies)
     double* clusters = nullptr;  // near-pass element clusters (centre, radius)
     int32_t* order = nullptr;    // elements in cluster order
     std::vector<int32_t> shape, bc;
@@ -229,8 +230,12 @@ int kem_set_integrator(kem_ctx* c, const char* name)
         c->policy = KEM_POLICY_NUMERIC;
     else if (s == "rwg")
         c->policy = KEM_POLICY_RWG;
+    else if (s == "analytic")
+        c->policy = KEM_POLICY_ANALYTIC;
+    else if (s == "reference")
+        c->policy = KEM_POLICY_REFERENCE;
     else
-        return fail(c, KEM_E_NAME, "unknown integrator '" + s + "' (known: numeric, default, rwg)");
+        return fail(c, KEM_E_NAME, "unknown integrator '" + s + "' (known: numeric, analytic, rwg, reference, default)");
     return KEM_OK;
 }
 
@@ -272,7 +277,12 @@ int kem_mesh_create(kem_ctx* c, int64_t n, const int32_t* shape, const double* d
     std::vector<int32_t> order((size_t)n);
     std::vector<double> clusters((size_t)(4 * ((n + KEM_CLUSTER - 1) / KEM_CLUSTER)));
     kem::build_clusters(n, g.data(), order.data(), clusters.data());
+    std::vector<double> recs((size_t)n * KEM_REC, 0.);
+    for (int64_t i = 0; i < n; i++)
+        for (int k = 0; k < 11; k++) recs[(size_t)i * KEM_REC + k] = data[11 * i + k];
     hipError_t e = dalloc(&m->geom, (int64_t)g.size());
+    if (e == hipSuccess) e = dalloc(&m->recs, (int64_t)recs.size());
+    if (e == hipSuccess) e = hipMemcpy(m->recs, recs.data(), sizeof(double) * recs.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = dalloc(&m->clusters, (int64_t)clusters.size());
     if (e == hipSuccess) e = dalloc(&m->order, n);
     if (e == hipSuccess)
@@ -304,6 +314,7 @@ void kem_mesh_destroy(kem_mesh* m)
     dfree(m->nodes32);
     dfree(m->clusters);
     dfree(m->order);
+    dfree(m->recs);
     delete m;
 }
 
@@ -339,7 +350,7 @@ int kem_element_eval(kem_ctx* c, const kem_mesh* m, int64_t np, const int64_t* e
     if (e == hipSuccess) e = dalloc(&dO, 4 * np);
     if (e == hipSuccess) e = hipMemcpyAsync(de, elem, sizeof(int64_t) * np, hipMemcpyHostToDevice, c->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(dP, P, sizeof(double) * 3 * np, hipMemcpyHostToDevice, c->stream);
-    if (e == hipSuccess) e = kem::launch_pairs(c->stream, c->policy, m->geom, m->nodes, np, de, dP, dO);
+    if (e == hipSuccess) e = kem::launch_pairs(c->stream, c->policy, m->geom, m->nodes, m->recs, np, de, dP, dO);
     if (e == hipSuccess) e = hipMemcpyAsync(out, dO, sizeof(double) * 4 * np, hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     dfree(de);
@@ -377,11 +388,11 @@ int kem_field_eval_device(kem_ctx* c, const kem_mesh* m, const double* sigma, in
             KEM_CK(c, hipMalloc(&c->sort_ws, bytes));
             c->sort_ws_bytes = bytes;
         }
-        KEM_CK(c, kem::launch_field_sorted(c->stream, c->policy, m->geom, m->nodes, n32, c->mixed_fp64_ratio, sigma, m->n,
+        KEM_CK(c, kem::launch_field_sorted(c->stream, c->policy, m->geom, m->nodes, m->recs, n32, c->mixed_fp64_ratio, sigma, m->n,
                                            m->clusters, m->order, x, y, z, npts, phi, ex, ey, ez, c->sort_ws, c->sort_ws_bytes, c->ws, c->ws_doubles, t0, t1));
         return KEM_OK;
     }
-    KEM_CK(c, kem::launch_field(c->stream, c->policy, m->geom, m->nodes, n32, c->mixed_fp64_ratio, sigma, m->n,
+    KEM_CK(c, kem::launch_field(c->stream, c->policy, m->geom, m->nodes, m->recs, n32, c->mixed_fp64_ratio, sigma, m->n,
                                 m->clusters, m->order, x, y, z, npts, phi, ex, ey, ez, c->ws, c->ws_doubles, t0, t1));
     return KEM_OK;
 }
@@ -464,7 +475,7 @@ int kem_matrix_assemble(kem_ctx* c, const kem_mesh* m, int storage, kem_matrix**
     if (storage & KEM_STORE_FP64) e = dalloc(&A->A64, nn);
     if (e == hipSuccess && (storage & KEM_STORE_FP32)) e = dalloc(&A->A32, nn);
     if (e == hipSuccess)
-        e = kem::launch_assemble(c->stream, c->policy, m->has_neumann, m->geom, m->nodes, m->targets, m->n, A->A64,
+        e = kem::launch_assemble(c->stream, c->policy, m->has_neumann, m->geom, m->nodes, m->recs, m->targets, m->n, A->A64,
                                  A->A32);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) {

@@ -14,7 +14,8 @@ namespace kem {
 // the culled near pass under the numeric policy; null scans every element.  ws: scratch of
 // ws_doubles doubles for element-range partial sums.  far_begin / far_end (nullable): events
 // recorded on st around the far-field kernel.
-hipError_t launch_field(hipStream_t st, int policy, const double* geom, const double* nodes, const float* nodes32,
+hipError_t launch_field(hipStream_t st, int policy, const double* geom, const double* nodes, const double* recs,
+                        const float* nodes32,
                         double fp64_ratio, const double* sigma, int64_t nelem, const double* clusters,
                         const int32_t* order, const double* px, const double* py, const double* pz, int64_t npts,
                         double* phi, double* ex, double* ey, double* ez, double* ws, int64_t ws_doubles,
@@ -24,7 +25,7 @@ hipError_t launch_field(hipStream_t st, int policy, const double* geom, const do
 // keys, gather, evaluate, scatter back.  sort_ws: sort_workspace_bytes(npts) bytes.
 size_t sort_workspace_bytes(int64_t npts);
 hipError_t launch_field_sorted(hipStream_t st, int policy, const double* geom, const double* nodes,
-                               const float* nodes32, double fp64_ratio, const double* sigma, int64_t nelem,
+                               const double* recs, const float* nodes32, double fp64_ratio, const double* sigma, int64_t nelem,
                                const double* clusters, const int32_t* order, const double* px, const double* py,
                                const double* pz, int64_t npts, double* phi, double* ex, double* ey, double* ez,
                                void* sort_ws, size_t sort_ws_bytes, double* ws, int64_t ws_doubles,
@@ -34,11 +35,11 @@ hipError_t launch_field_sorted(hipStream_t st, int policy, const double* geom, c
 hipError_t launch_census(hipStream_t st, const double* geom, int64_t nelem, const double* px, const double* py,
                          const double* pz, int64_t npts, int mixed, double fp64_ratio, unsigned long long* counts);
 
-hipError_t launch_pairs(hipStream_t st, int policy, const double* geom, const double* nodes, int64_t npairs,
-                        const int64_t* elem, const double* P, double* out);
+hipError_t launch_pairs(hipStream_t st, int policy, const double* geom, const double* nodes, const double* recs,
+                        int64_t npairs, const int64_t* elem, const double* P, double* out);
 
 hipError_t launch_assemble(hipStream_t st, int policy, bool field, const double* geom, const double* nodes,
-                           const double* targets, int64_t n, double* A64, float* A32);
+                           const double* recs, const double* targets, int64_t n, double* A64, float* A32);
 
 int gemv_splits(int64_t nrows, int64_t ncols);
 // y = alpha * A x + beta * y, A column-major nrows x ncols (leading dimension lda);

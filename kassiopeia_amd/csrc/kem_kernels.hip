@@ -10,6 +10,7 @@
 
 #include "kem_device.hpp"
 #include "kem_kernels.h"
+#include "kem_policy.hpp"
 
 namespace kem {
 
@@ -119,9 +120,12 @@ __global__ __launch_bounds__(kBlock) void k_field_far32(const double* __restrict
     }
 }
 
-// Near pass over all elements of a range (every pair under the rwg policy): closed-form
-// pairs added to the far pass's sums (accumulate = 1) or from zero (accumulate = 0).
+// Near pass over all elements of a range: closed-form pairs added to the far pass's sums
+// (numeric policy, accumulate = 1), or every pair from zero (rwg policy; PC = 1: the analytic
+// and reference policies, from the raw shape records).
+template <int PC>
 __global__ __launch_bounds__(kBlock) void k_field_near(const double* __restrict__ geom,
+                                                       const double* __restrict__ recs,
                                                        const double* __restrict__ sigma, int64_t nelem,
                                                        int64_t per_split, const double* __restrict__ px,
                                                        const double* __restrict__ py, const double* __restrict__ pz,
@@ -148,8 +152,9 @@ __global__ __launch_bounds__(kBlock) void k_field_near(const double* __restrict_
     double c0 = 0., c1 = 0., c2 = 0., c3 = 0.;
     for (int64_t j = j0; j < j1; j++) {
         const double* g = geom + j * KG_STRIDE;
-        if (dev::is_near2(g, dev::dist2(g, x, y, z), policy)) {
-            const F4 v = dev::element_near(g, (int)g[KG_TYPE], x, y, z);
+        if (PC == 1 || dev::is_near2(g, dev::dist2(g, x, y, z), policy)) {
+            const F4 v = PC == 1 ? dev::element_policy(g, recs + j * KEM_REC, policy, x, y, z)
+                                 : dev::element_near(g, (int)g[KG_TYPE], x, y, z);
             const double sj = sigma[j];
             dev::kahan_add(v.x, sj, s0, c0);
             dev::kahan_add(v.y, sj, s1, c1);
@@ -253,15 +258,19 @@ __global__ __launch_bounds__(kBlock) void k_field_combine(const double* __restri
 }
 
 // ------------------------------------------------------------------ element pairs
+template <int PC>
 __global__ __launch_bounds__(kBlock) void k_pairs(const double* __restrict__ geom, const double* __restrict__ nodes,
-                                                  int policy, int64_t npairs, const int64_t* __restrict__ elem,
-                                                  const double* __restrict__ P, double* __restrict__ out)
+                                                  const double* __restrict__ recs, int policy, int64_t npairs,
+                                                  const int64_t* __restrict__ elem, const double* __restrict__ P,
+                                                  double* __restrict__ out)
 {
     const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (k >= npairs) return;
     const int64_t j = elem[k];
-    const F4 v = dev::element_eval<true>(geom + j * KG_STRIDE, nodes + j * KEM_NODE_STRIDE, policy, P[3 * k],
-                                         P[3 * k + 1], P[3 * k + 2]);
+    const F4 v = PC == 1 ? dev::element_policy(geom + j * KG_STRIDE, recs + j * KEM_REC, policy, P[3 * k],
+                                               P[3 * k + 1], P[3 * k + 2])
+                         : dev::element_eval<true>(geom + j * KG_STRIDE, nodes + j * KEM_NODE_STRIDE, policy,
+                                                   P[3 * k], P[3 * k + 1], P[3 * k + 2]);
     out[4 * k + 0] = v.x;
     out[4 * k + 1] = v.y;
     out[4 * k + 2] = v.z;
@@ -272,9 +281,10 @@ __global__ __launch_bounds__(kBlock) void k_pairs(const double* __restrict__ geo
 // Lane = collocation target i (rows), wave-uniform source j (columns).  Column-major storage:
 // A(i, j) at A[j*n + i], so the 64 lanes of a wavefront write 64 consecutive entries.
 // KElectrostaticBoundaryIntegrator::ComputeBoundaryIntegral (Core/include/...hh:260-280).
-template <bool FIELD>
+template <bool FIELD, int PC>
 __global__ __launch_bounds__(kBlock) void k_assemble(const double* __restrict__ geom,
                                                      const double* __restrict__ nodes,
+                                                     const double* __restrict__ recs,
                                                      const double* __restrict__ targets, int policy, int64_t n,
                                                      int64_t per_split, double* __restrict__ A64,
                                                      float* __restrict__ A32)
@@ -292,7 +302,9 @@ __global__ __launch_bounds__(kBlock) void k_assemble(const double* __restrict__ 
     int64_t j1 = j0 + per_split;
     if (j1 > n) j1 = n;
     for (int64_t j = j0; j < j1; j++) {
-        const F4 v = dev::element_eval<FIELD>(geom + j * KG_STRIDE, nodes + j * KEM_NODE_STRIDE, policy, cx, cy, cz);
+        const F4 v = PC == 1 ? dev::element_policy(geom + j * KG_STRIDE, recs + j * KEM_REC, policy, cx, cy, cz)
+                             : dev::element_eval<FIELD>(geom + j * KG_STRIDE, nodes + j * KEM_NODE_STRIDE, policy,
+                                                        cx, cy, cz);
         double a;
         if (!FIELD || dirichlet) {
             a = v.p;
@@ -530,7 +542,8 @@ inline unsigned nblocks(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock
 }  // namespace
 
 // ====================================================================== launchers
-hipError_t launch_field(hipStream_t st, int policy, const double* geom, const double* nodes, const float* nodes32,
+hipError_t launch_field(hipStream_t st, int policy, const double* geom, const double* nodes, const double* recs,
+                        const float* nodes32,
                         double fp64_ratio, const double* sigma, int64_t nelem, const double* clusters,
                         const int32_t* order, const double* px, const double* py, const double* pz, int64_t npts,
                         double* phi, double* ex, double* ey, double* ez, double* ws, int64_t ws_doubles,
@@ -564,8 +577,14 @@ hipError_t launch_field(hipStream_t st, int policy, const double* geom, const do
         k_field_far<<<grid, kBlock, 0, st>>>(geom, nodes, sigma, nelem, per, px, py, pz, npts, o0, o1, o2, o3, stride);
     if (far_end) (void)hipEventRecord(far_end, st);
     if (!culled)
-        k_field_near<<<grid, kBlock, 0, st>>>(geom, sigma, nelem, per, px, py, pz, npts, policy, policy == 0 ? 1 : 0, o0,
-                                              o1, o2, o3, stride);
+    {
+        if (policy >= 2)
+            k_field_near<1><<<grid, kBlock, 0, st>>>(geom, recs, sigma, nelem, per, px, py, pz, npts, policy, 0, o0, o1,
+                                                     o2, o3, stride);
+        else
+            k_field_near<0><<<grid, kBlock, 0, st>>>(geom, recs, sigma, nelem, per, px, py, pz, npts, policy,
+                                                     policy == 0 ? 1 : 0, o0, o1, o2, o3, stride);
+    }
     if (nsplit > 1) k_field_combine<<<gx, kBlock, 0, st>>>(ws, (int)nsplit, npts, phi, ex, ey, ez);
     if (culled) {
         const int64_t ncl = (nelem + KEM_CLUSTER - 1) / KEM_CLUSTER;
@@ -585,16 +604,19 @@ hipError_t launch_census(hipStream_t st, const double* geom, int64_t nelem, cons
     return hipGetLastError();
 }
 
-hipError_t launch_pairs(hipStream_t st, int policy, const double* geom, const double* nodes, int64_t npairs,
-                        const int64_t* elem, const double* P, double* out)
+hipError_t launch_pairs(hipStream_t st, int policy, const double* geom, const double* nodes, const double* recs,
+                        int64_t npairs, const int64_t* elem, const double* P, double* out)
 {
     if (npairs <= 0) return hipSuccess;
-    k_pairs<<<nblocks(npairs), kBlock, 0, st>>>(geom, nodes, policy, npairs, elem, P, out);
+    if (policy >= 2)
+        k_pairs<1><<<nblocks(npairs), kBlock, 0, st>>>(geom, nodes, recs, policy, npairs, elem, P, out);
+    else
+        k_pairs<0><<<nblocks(npairs), kBlock, 0, st>>>(geom, nodes, recs, policy, npairs, elem, P, out);
     return hipGetLastError();
 }
 
 hipError_t launch_assemble(hipStream_t st, int policy, bool field, const double* geom, const double* nodes,
-                           const double* targets, int64_t n, double* A64, float* A32)
+                           const double* recs, const double* targets, int64_t n, double* A64, float* A32)
 {
     if (n <= 0) return hipSuccess;
     const unsigned gx = nblocks(n);
@@ -603,10 +625,16 @@ hipError_t launch_assemble(hipStream_t st, int policy, bool field, const double*
     const int64_t per = (n + nsplit - 1) / nsplit;
     nsplit = (n + per - 1) / per;
     dim3 grid(gx, (unsigned)nsplit);
-    if (field)
-        k_assemble<true><<<grid, kBlock, 0, st>>>(geom, nodes, targets, policy, n, per, A64, A32);
-    else
-        k_assemble<false><<<grid, kBlock, 0, st>>>(geom, nodes, targets, policy, n, per, A64, A32);
+    if (policy >= 2) {
+        if (field)
+            k_assemble<true, 1><<<grid, kBlock, 0, st>>>(geom, nodes, recs, targets, policy, n, per, A64, A32);
+        else
+            k_assemble<false, 1><<<grid, kBlock, 0, st>>>(geom, nodes, recs, targets, policy, n, per, A64, A32);
+    } else if (field) {
+        k_assemble<true, 0><<<grid, kBlock, 0, st>>>(geom, nodes, recs, targets, policy, n, per, A64, A32);
+    } else {
+        k_assemble<false, 0><<<grid, kBlock, 0, st>>>(geom, nodes, recs, targets, policy, n, per, A64, A32);
+    }
     return hipGetLastError();
 }
 

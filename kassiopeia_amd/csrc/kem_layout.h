@@ -54,6 +54,9 @@ enum kem_geom_field {
     KG_STRIDE = 88
 };
 
+/* raw shape record kept for the analytic / reference policies (11 doubles, padded) */
+#define KEM_REC 12
+
 #define KEM_NODE_SLOTS 52
 #define KEM_NODE_STRIDE (4 * KEM_NODE_SLOTS)
 #define KEM_N7 0

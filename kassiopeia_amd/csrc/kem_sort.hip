@@ -137,7 +137,7 @@ size_t sort_workspace_bytes(int64_t n)
 }
 
 hipError_t launch_field_sorted(hipStream_t st, int policy, const double* geom, const double* nodes,
-                               const float* nodes32, double fp64_ratio, const double* sigma, int64_t nelem,
+                               const double* recs, const float* nodes32, double fp64_ratio, const double* sigma, int64_t nelem,
                                const double* clusters, const int32_t* order, const double* px, const double* py,
                                const double* pz, int64_t npts, double* phi, double* ex, double* ey, double* ez,
                                void* sort_ws, size_t sort_ws_bytes, double* ws, int64_t ws_doubles,
@@ -171,7 +171,7 @@ hipError_t launch_field_sorted(hipStream_t st, int policy, const double* geom, c
     hipError_t e = hipcub::DeviceRadixSort::SortPairs(tmp, tb, k0, k1, i0, i1, (int)npts, 0, 63, st);
     if (e != hipSuccess) return e;
     k_gather3<<<nblocks(npts), kBlock, 0, st>>>(i1, px, py, pz, npts, sx, sy, sz);
-    e = launch_field(st, policy, geom, nodes, nodes32, fp64_ratio, sigma, nelem, clusters, order, sx, sy, sz, npts, o[0],
+    e = launch_field(st, policy, geom, nodes, recs, nodes32, fp64_ratio, sigma, nelem, clusters, order, sx, sy, sz, npts, o[0],
                      o[1], o[2], o[3], ws, ws_doubles, far_begin, far_end);
     if (e != hipSuccess) return e;
     k_scatter4<<<nblocks(npts), kBlock, 0, st>>>(i1, o[0], o[1], o[2], o[3], npts, phi, ex, ey, ez);

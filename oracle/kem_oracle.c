@@ -747,11 +747,530 @@ int ko_line_numeric(const double* d, const double P[3], double out[4])
     return 0;
 }
 
+/* ------------------------------------------------------------ analytic policy */
+/* KElectrostaticAnalyticTriangleIntegrator.cc (Analytic/src): the triangle is put in a local
+   frame with the field point on the z axis; the potential and the field are sums of the
+   one-dimensional primitives below.  Branch structure, thresholds and expression order follow
+   the cited lines. */
+#define SGN(v) (((v) > 0.) - ((v) < 0.))
+static const double KO_PI_2 = 3.141592653589793238462643383279502884 / 2.;
+
+/* F1 :209-212 */
+static double at_F1(double a, double b, double u) { return u * asinh((a + b * u) / sqrt(u * u + 1.)); }
+
+/* shared g1/g2 of I3 and I3p with the negative-argument patch :216-227, :243-253 */
+static void at_g12(double a, double b, double u1, double u2, double* g1, double* g2)
+{
+    *g1 = (sqrt(b * b + 1.) * sqrt(a * a + 2 * a * b * u1 + (b * b + 1.) * u1 * u1 + 1.) + b * (a + b * u1) + u1);
+    *g2 = (sqrt(b * b + 1.) * sqrt(a * a + 2 * a * b * u2 + (b * b + 1.) * u2 * u2 + 1.) + b * (a + b * u2) + u2);
+    if (*g1 <= 0) *g1 = -(1 + a * a + b * b) / (2 * (b * b + 1) * u1);
+    if (*g2 <= 0) *g2 = -(1 + a * a + b * b) / (2 * (b * b + 1) * u2);
+}
+
+/* I3 :214-241 */
+static double at_I3(double a, double b, double u1, double u2)
+{
+    double g1, g2;
+    at_g12(a, b, u1, u2, &g1, &g2);
+    if (fabs(g1) < 1.e-12 && fabs(a) < 1.e-14) g1 = 1.e-12;
+    if (fabs(g2) < 1.e-12 && fabs(a) < 1.e-14) g2 = 1.e-12;
+    return a / sqrt(b * b + 1.) * log(fabs(g2 / g1));
+}
+
+/* I3p :243-256 */
+static double at_I3p(double a, double b, double u1, double u2)
+{
+    double g1, g2;
+    at_g12(a, b, u1, u2, &g1, &g2);
+    return 1. / sqrt(b * b + 1.) * log(fabs(g2 / g1));
+}
+
+/* I4 in the substituted variable t :258-275 */
+static double at_I4t(double alpha, double gamma, double q2, double prefac, double t1, double t2)
+{
+    double q = sqrt(q2);
+    double g1 = sqrt(gamma * t1 * t1 + alpha);
+    double g2 = sqrt(gamma * t2 * t2 + alpha);
+    if (t1 > 1.e15 || t2 > 1.e15) {
+        if (t2 < 1.e15) return (prefac * 1. / q * (atan(g2 / q) - KO_PI_2));
+        if (t1 < 1.e15) return (prefac * 1. / q * (KO_PI_2 - atan(g1 / q)));
+        return 0.;
+    }
+    return prefac * 1. / q * atan(q * (g2 - g1) / (q2 + g1 * g2));
+}
+
+/* I4 :277-333.  The reference nudges u1, u2 or a by 1e-14 and re-enters; the loop below takes
+   the same steps (bounded: each nudge leaves the 1e-14 window within two steps). */
+static double at_I4(double a, double b, double u1, double u2)
+{
+    for (int it = 0; it < 64; it++) {
+        if (fabs(u1 - b / a) < 1.e-14) {
+            u1 = (u2 > b / a) ? u1 + 1.e-14 : u1 - 1.e-14;
+            continue;
+        }
+        if (fabs(u2 - b / a) < 1.e-14) {
+            u2 = (u1 > b / a) ? u2 + 1.e-14 : u2 - 1.e-14;
+            continue;
+        }
+        if (fabs(a) < 1.e-14) {
+            a = (a > 0) ? a + 1.e-14 : a - 1.e-14;
+            continue;
+        }
+        break;
+    }
+    double alpha = ((double)1.) + (a * a) / (b * b);
+    double gamma = (a * a + b * b) * (a * a + b * b + ((double)1.)) / (b * b);
+    double q2 = (a * a + b * b) * (a * a + b * b) / (b * b);
+    double prefac = (a * a / b + b);
+    double t1 = (a * u1 != b) ? (b * u1 + a) / (a * u1 - b) : 1.e15;
+    double t2 = (a * u2 != b) ? (b * u2 + a) / (a * u2 - b) : 1.e15;
+    double sign = 1.;
+    if (a < 0.) sign = -sign;
+    if (b < 0.) sign = -sign;
+    if (u1 > b / a) sign = -sign;
+    if (SGN(u1 - b / a) != SGN(u2 - b / a))
+        return sign * (at_I4t(alpha, gamma, q2, prefac, t1, 1.e16) + at_I4t(alpha, gamma, q2, prefac, t2, 1.e16));
+    return sign * at_I4t(alpha, gamma, q2, prefac, t1, t2);
+}
+
+/* I4_2 in t :335-346 */
+static double at_I42t(double alpha, double gamma, double prefac, double t1, double t2)
+{
+    double g1 = sqrt(alpha * t1 * t1 + gamma);
+    double g2 = sqrt(alpha * t2 * t2 + gamma);
+    if (gamma - alpha <= 0) return prefac * (g2 - g1) / ((alpha - gamma) + g1 * g2);
+    double q = sqrt(gamma - alpha);
+    return prefac * 1. / q * atanh(q * (g2 - g1) / ((alpha - gamma) + g1 * g2));
+}
+
+/* I4_2 :348-377 */
+static double at_I42(double a, double b, double u1, double u2)
+{
+    double alpha = ((double)1.) + (a * a) / (b * b);
+    double gamma = (a * a + b * b) * (a * a + b * b + 1) / (b * b);
+    double lambda = -a / b;
+    double prefac = (a * a / b + b);
+    double t1 = (b * u1 != -a) ? (b - a * u1) / (a + b * u1) : 1.e15;
+    double t2 = (b * u2 != -a) ? (b - a * u2) / (a + b * u2) : 1.e15;
+    if (SGN(u1 - lambda) != SGN(u2 - lambda)) {
+        if (u1 > lambda)
+            return (at_I42t(alpha, gamma, prefac, fabs(t1), 1.e15) + at_I42t(alpha, gamma, prefac, fabs(t2), 1.e15));
+        return (at_I42t(alpha, gamma, prefac, 1.e15, fabs(t1)) + at_I42t(alpha, gamma, prefac, 1.e15, fabs(t2)));
+    }
+    if (u1 > lambda) return at_I42t(alpha, gamma, prefac, t1, t2);
+    return at_I42t(alpha, gamma, prefac, t2, t1);
+}
+
+/* I1 :379-382 */
+static double at_I1(double a, double b, double u1, double u2)
+{
+    return at_F1(a, b, u2) - at_F1(a, b, u1) + at_I3(a, b, u1, u2) - at_I4(a, b, u1, u2);
+}
+
+/* I6 :384-389 */
+static double at_I6(double x, double u1, double u2)
+{
+    if (fabs(x) < 1.e-15) return 0;
+    return x * log((sqrt(u2 * u2 + x * x + 1.) + u2) / (sqrt(u1 * u1 + x * x + 1.) + u1));
+}
+
+/* I7 :391-408 */
+static double at_I7(double x, double u1, double u2)
+{
+    double t1 = (fabs(u1) > 1.e-16) ? 1. / u1 : 1.e16;
+    double t2 = (fabs(u2) > 1.e-16) ? 1. / u2 : 1.e16;
+    double g1 = sqrt(1. + t1 * t1 * (1. + x * x));
+    double g2 = sqrt(1. + t2 * t2 * (1. + x * x));
+    return atan(x * (g2 - g1) / (x * x + g2 * g1));
+}
+
+/* I2 :410-427 */
+static double at_I2(double x, double u1, double u2)
+{
+    double f = (at_F1(x, 0., u2) - at_F1(x, 0., u1));
+    if (SGN(u1) != SGN(u2)) {
+        if (u1 <= 0.) return f + at_I6(x, u1, u2) + at_I7(x, 0., fabs(u1)) + at_I7(x, 0., fabs(u2));
+        return f + at_I6(x, u1, u2) + at_I7(x, fabs(u1), 0.) + at_I7(x, fabs(u2), 0.);
+    }
+    if (u1 <= 0.) return f + at_I6(x, u1, u2) + at_I7(x, u2, u1);
+    return f + at_I6(x, u1, u2) + at_I7(x, u1, u2);
+}
+
+/* J2 :429-438 */
+static double at_J2(double a, double u1, double u2)
+{
+    if (a == 0.) return 0.;
+    double g1 = sqrt(u1 * u1 + a * a + 1.);
+    double g2 = sqrt(u2 * u2 + a * a + 1.);
+    return a / (2. * fabs(a)) * log(((g2 - fabs(a)) * (g1 + fabs(a))) / ((g2 + fabs(a)) * (g1 - fabs(a))));
+}
+
+/* Local_Ez's per-edge term :473-487 */
+static double at_ez_edge(double a, double b, double u0, double u1)
+{
+    if (fabs(b) > 1.e-14) return at_I4(a, b, u0, u1);
+    if (SGN(u0) != SGN(u1)) {
+        if (u0 <= 0.) return -(at_I7(a, 0., fabs(u0)) + at_I7(a, 0., fabs(u1)));
+        return -(at_I7(a, fabs(u0), 0.) + at_I7(a, fabs(u1), 0.));
+    }
+    if (u0 <= 0.) return at_I7(a, u0, u1);
+    return at_I7(a, u1, u0);
+}
+
+/* Local_Ey's per-edge term :452-462 */
+static double at_ey_edge(double a, double b, double u0, double u1)
+{
+    if (fabs(b) > 1.e-14) return at_I42(a, b, u0, u1) + b * at_I3p(a, b, u0, u1);
+    return at_J2(a, u0, u1);
+}
+
+/* Potential_noZ's one-edge term :178-208 (ylim: 1e-14 for the upper edge, 5e-14 for the lower) */
+static double at_noz_edge(double a, double b, double y, double ylim)
+{
+    double logArg = (1. + b * b) * y + a * b + sqrt(1. + b * b) * sqrt((1. + b * b) * y * y + 2 * a * b * y + a * a);
+    if (logArg > 1.e-14) {
+        if (fabs(y) > ylim) return y * asinh((a + b * y) / fabs(y)) + a / sqrt(1. + b * b) * log(logArg);
+        return a / sqrt(1. + b * b) * log(logArg);
+    }
+    if (fabs(y) > ylim) return y * asinh(y * b / fabs(y));
+    return 0.;
+}
+
+static double at_noz(double a2, double b2, double a1, double b1, double y)
+{
+    return at_noz_edge(a2, b2, y, 1.e-14) - at_noz_edge(a1, b1, y, 5.e-14);
+}
+
+/* local frame of Potential :13-70 / ElectricField :84-136 */
+typedef struct {
+    double N2p[3], a[2], b[2], u[2], z, zsign;
+} at_frame;
+
+static void at_local(const double* d, const double P[3], at_frame* f)
+{
+    const double* N1 = d + 5;
+    const double* N2 = d + 8;
+    double N3[3];
+    ko_normal(KO_TRIANGLE, d, N3);
+    double p[3] = {d[2] - P[0], d[3] - P[1], d[4] - P[2]};
+    double n1dotn2 = dot3(N1, N2);
+    double t[3] = {N2[0] - n1dotn2 * N1[0], N2[1] - n1dotn2 * N1[1], N2[2] - n1dotn2 * N1[2]};
+    double tm = norm3(t);
+    for (int k = 0; k < 3; k++) f->N2p[k] = tm > 0. ? t[k] / tm : t[k];
+    double n2dotn2p = dot3(N2, f->N2p);
+    double x0 = dot3(p, N1), y0 = dot3(p, f->N2p), z0 = -dot3(p, N3);
+    double x1 = x0 + d[0];
+    double x2 = x0 + d[1] * n1dotn2;
+    double y1;
+    if (z0 < 0) {
+        y0 = -y0;
+        y1 = y0 - d[1] * n2dotn2p;
+        f->zsign = -1;
+    } else {
+        y1 = y0 + d[1] * n2dotn2p;
+        f->zsign = 1;
+    }
+    double z = fabs(z0);
+    f->z = z;
+    if (z > 1.e-14) {
+        f->u[0] = y0 / z;
+        if (f->u[0] == 0.) f->u[0] = 1.e-18;
+        f->u[1] = y1 / z;
+        if (f->u[1] == 0.) f->u[1] = 1.e-18;
+        f->a[0] = (x0 * y1 - x2 * y0) / (z * (y1 - y0));
+        f->a[1] = (x1 * y1 - x2 * y0) / (z * (y1 - y0));
+    } else {
+        f->u[0] = y0;
+        if (f->u[0] == 0.) f->u[0] = 1.e-18;
+        f->u[1] = y1;
+        if (f->u[1] == 0.) f->u[1] = 1.e-18;
+        f->a[0] = (x0 * y1 - x2 * y0) / (y1 - y0);
+        f->a[1] = (x1 * y1 - x2 * y0) / (y1 - y0);
+    }
+    f->b[0] = (x2 - x0) / (y1 - y0);
+    f->b[1] = (x2 - x1) / (y1 - y0);
+}
+
+/* Potential :13-83 and ElectricField :85-174 */
+void ko_tri_analytic(const double* d, const double P[3], double out[4])
+{
+    at_frame f;
+    at_local(d, P, &f);
+    double I;
+    if (f.z > 1.e-14) {
+        if (fabs(f.b[0]) < 1.e-13)
+            I = f.z * (at_I1(f.a[1], f.b[1], f.u[0], f.u[1]) - at_I2(f.a[0], f.u[0], f.u[1]));
+        else if (fabs(f.b[1]) < 1.e-13)
+            I = f.z * (at_I2(f.a[1], f.u[0], f.u[1]) - at_I1(f.a[0], f.b[0], f.u[0], f.u[1]));
+        else
+            I = f.z * (at_I1(f.a[1], f.b[1], f.u[0], f.u[1]) - at_I1(f.a[0], f.b[0], f.u[0], f.u[1]));
+    } else {
+        I = (at_noz(f.a[1], f.b[1], f.a[0], f.b[0], f.u[1]) - at_noz(f.a[1], f.b[1], f.a[0], f.b[0], f.u[0]));
+    }
+    out[3] = fabs(I) / (4. * KO_PI * KO_EPS0);
+
+    double c[3];
+    ko_centroid(KO_TRIANGLE, d, c);
+    double dc[3] = {c[0] - P[0], c[1] - P[1], c[2] - P[2]};
+    double dist = norm3(dc);
+    double prefac = 1. / (4. * KO_PI * KO_EPS0);
+    double loc[3];
+    loc[0] = f.zsign * prefac * (at_I3p(f.a[1], f.b[1], f.u[0], f.u[1]) - at_I3p(f.a[0], f.b[0], f.u[0], f.u[1]));
+    loc[1] = prefac * (at_ey_edge(f.a[0], f.b[0], f.u[0], f.u[1]) - at_ey_edge(f.a[1], f.b[1], f.u[0], f.u[1]));
+    if (f.z > 1.e-14)
+        loc[2] = prefac * (at_ez_edge(f.a[1], f.b[1], f.u[0], f.u[1]) - at_ez_edge(f.a[0], f.b[0], f.u[0], f.u[1]));
+    else
+        loc[2] = (dist < 1.e-12) ? 1. / (2. * KO_EPS0) : 0.;
+    double N3[3];
+    ko_normal(KO_TRIANGLE, d, N3);
+    for (int j = 0; j < 3; j++) out[j] = (d[5 + j] * loc[0] + f.N2p[j] * loc[1] + N3[j] * loc[2]);
+}
+
+/* KElectrostaticAnalyticRectangleIntegrator.cc: Integral_ln :128-150 */
+static double ar_ln(double x, double y, double w)
+{
+    double r = sqrt(fabs(x * x + y * y + w * w));
+    double r0 = sqrt(fabs(y * y + w * w));
+    double xa = fabs(x);
+    double c1 = (xa < 1.e-10) ? 0. : xa * log(fabs(y + r) + 1.e-12);
+    double c2 = (fabs(y) < 1.e-12) ? 0. : y * log(fabs((xa + r) / r0) + 1.e-12);
+    double c3 = (fabs(w) < 1.e-12) ? 0. : w * (atan(xa / w) + atan(y * w / (x * x + w * w + xa * r)) - atan(y / w));
+    double ret = c1 + c2 - xa + c3;
+    return (x < 0.) ? -ret : ret;
+}
+
+/* EFieldLocalXY :152-180 */
+static double ar_exy(double x1, double x2, double y1, double y2, double z)
+{
+    double a1 = y2 + sqrt(y2 * y2 + x1 * x1 + z * z);
+    double a2 = y2 + sqrt(y2 * y2 + x2 * x2 + z * z);
+    double a3 = y1 + sqrt(y1 * y1 + x1 * x1 + z * z);
+    double a4 = y1 + sqrt(y1 * y1 + x2 * x2 + z * z);
+    if (fabs(z) < 1.e-14) {
+        if (fabs(x1) < 1.e-14) {
+            a1 = fabs(y1);
+            a3 = fabs(y2);
+        }
+        if (fabs(x2) < 1.e-14) {
+            a2 = fabs(y1);
+            a4 = fabs(y2);
+        }
+    }
+    if (fabs(a1 - a3) < 1.e-14) a1 = a3 = 1.;
+    if (fabs(a2 - a4) < 1.e-14) a2 = a4 = 1.;
+    return log((a2 * a3) / (a1 * a4));
+}
+
+/* EFieldLocalZ :182-205 */
+static double ar_ez(double x1, double x2, double y1, double y2, double z)
+{
+    double t1 = (fabs(y1) > 1.e-15 ? z / y1 : 1.e15);
+    double t2 = (fabs(y2) > 1.e-15 ? z / y2 : 1.e15);
+    double g1 = sqrt(((x2 * x2 + z * z) * t1 * t1 + z * z) / (x2 * x2));
+    double g2 = sqrt(((x1 * x1 + z * z) * t1 * t1 + z * z) / (x1 * x1));
+    double g3 = sqrt(((x2 * x2 + z * z) * t2 * t2 + z * z) / (x2 * x2));
+    double g4 = sqrt(((x1 * x1 + z * z) * t2 * t2 + z * z) / (x1 * x1));
+    if (x2 < 0.) g1 = -g1;
+    if (x1 < 0.) g2 = -g2;
+    if (x2 < 0.) g3 = -g3;
+    if (x1 < 0.) g4 = -g4;
+    return atan(g3) - atan(g4) - atan(g1) + atan(g2);
+}
+
+/* Potential :25-43 and ElectricField :45-121 */
+void ko_rect_analytic(const double* d, const double P[3], double out[4])
+{
+    const double* N1 = d + 5;
+    const double* N2 = d + 8;
+    double N3[3];
+    ko_normal(KO_RECTANGLE, d, N3);
+    double p[3] = {P[0] - d[2], P[1] - d[3], P[2] - d[4]};
+    double uP = dot3(p, N1), vP = dot3(p, N2), w = dot3(p, N3);
+    double xmin = -uP, xmax = -uP + d[0], ymin = -vP, ymax = -vP + d[1];
+    double I = (ar_ln(xmax, ymax, w) - ar_ln(xmin, ymax, w) - ar_ln(xmax, ymin, w) + ar_ln(xmin, ymin, w));
+    out[3] = I / (4. * KO_PI * KO_EPS0);
+    double prefac = 1. / (4. * KO_PI * KO_EPS0);
+    double loc[3];
+    loc[0] = prefac * ar_exy(xmin, xmax, ymin, ymax, w);
+    loc[1] = prefac * ar_exy(ymin, ymax, xmin, xmax, w);
+    double tmin = w / ymin, tmax = w / ymax;
+    double sign_z = (w < 0) ? -1. : 1.;
+    if (fabs(w) < 1.e-13)
+        loc[2] = (xmin < 0. && xmax > 0. && ymin < 0. && ymax > 0.) ? 1.0 / (2. * KO_EPS0) : 0.;
+    else if (SGN(tmin) != SGN(tmax))
+        loc[2] = prefac * sign_z * fabs(ar_ez(xmin, xmax, 0, fabs(ymin), w) + ar_ez(xmin, xmax, 0, fabs(ymax), w));
+    else
+        loc[2] = prefac * sign_z * fabs(ar_ez(xmin, xmax, ymax, ymin, w));
+    for (int i = 0; i < 3; i++) out[i] = (N1[i] * loc[0] + N2[i] * loc[1] + N3[i] * loc[2]);
+}
+
+/* ------------------------------------------------------------ reference policy */
+/* Gauss-Legendre 32-point rule, positive nodes ascending (mirror nodes -x, same weights);
+   KGaussLegendreQuadrature.cc (Math/src) applies them as sum_j w_j f(B + A x_j) followed by
+   sum_j w_j f(B - A x_j), times A = (b - a)/2, B = (b + a)/2 */
+static const double GL32x[16] = {
+    0.0483076656877383162348, 0.144471961582796493485, 0.239287362252137074545, 0.33186860228212764978,
+    0.421351276130635345364,  0.506899908932229390024, 0.587715757240762329041, 0.663044266930215200975,
+    0.732182118740289680387,  0.794483795967942406963, 0.849367613732569970134, 0.896321155766052123965,
+    0.934906075937739689171,  0.964762255587506430774, 0.9856115115452683354,   0.997263861849481563545};
+static const double GL32w[16] = {
+    0.0965400885147278005668, 0.0956387200792748594191, 0.0938443990808045656392, 0.0911738786957638847129,
+    0.0876520930044038111428, 0.0833119242269467552222, 0.0781938957870703064717, 0.0723457941088485062254,
+    0.0658222227763618468377, 0.0586840934785355471453, 0.0509980592623761761962, 0.0428358980222266806569,
+    0.0342738629130214331027, 0.0253920653092620594558, 0.0162743947309056706052, 0.00701861000947009660041};
+
+/* point charge kernel for the nested rules: (E, phi) of unit charge at Q, times 1/(4 pi eps0);
+   triF (BiQuadratureTriangleIntegrator.cc:51-70) computes C = 1/R, C^3; rectF
+   (BiQuadratureRectangleIntegrator.cc:33-56) 1/R and R^3 */
+static void bq_point(int rect, const double Q[3], const double P[3], double v[4])
+{
+    double QP[3] = {P[0] - Q[0], P[1] - Q[1], P[2] - Q[2]};
+    double R = norm3(QP);
+    if (rect) {
+        double R3 = R * R * R;
+        for (int j = 0; j < 3; j++) v[j] = KO_K / R3 * QP[j];
+        v[3] = KO_K / R;
+    } else {
+        double C = 1. / R, C3 = C * C * C;
+        for (int j = 0; j < 3; j++) v[j] = KO_K * C3 * QP[j];
+        v[3] = KO_K * C;
+    }
+}
+
+/* KElectrostaticBiQuadratureTriangleIntegrator.cc:24-150: outer 32-node rule over the height
+   y in [0, h] of vertex P1 above side P2-P0, inner rule along the chord parallel to P2->P0 */
+void ko_tri_biquad(const double* d, const double P[3], double out[4])
+{
+    double A[3], B[3], C[3];
+    for (int k = 0; k < 3; k++) {
+        A[k] = d[2 + k];
+        B[k] = d[2 + k] + d[5 + k] * d[0];
+        C[k] = d[2 + k] + d[8 + k] * d[1];
+    }
+    double a[3], b[3], c[3];
+    for (int k = 0; k < 3; k++) {
+        a[k] = A[k] - C[k];
+        b[k] = B[k] - C[k];
+        c[k] = B[k] - A[k];
+    }
+    double la = norm3(a), lb = norm3(b);
+    double ua[3];
+    for (int k = 0; k < 3; k++) ua[k] = (1. / la) * a[k];
+    double cosg = dot3(a, b) / (la * lb);
+    double D[3], DB[3];
+    for (int k = 0; k < 3; k++) {
+        D[k] = C[k] + (lb * cosg) * ua[k];
+        DB[k] = D[k] - B[k];
+    }
+    double h = norm3(DB);
+    double so[4] = {0., 0., 0., 0.};
+    double Ao = (h - 0.) / 2., Bo = (h + 0.) / 2.;
+    for (int half = 0; half < 2; half++)
+        for (int jo = 0; jo < 16; jo++) {
+            double y = half == 0 ? Bo + Ao * GL32x[jo] : Bo - Ao * GL32x[jo];
+            double E[3], F[3], EF[3];
+            for (int k = 0; k < 3; k++) {
+                E[k] = B[k] - ((y / h) * b[k]);
+                F[k] = B[k] - ((y / h) * c[k]);
+                EF[k] = E[k] - F[k];
+            }
+            double L = norm3(EF);
+            double Ai = (L - 0.) / 2., Bi = (L + 0.) / 2.;
+            double si[4] = {0., 0., 0., 0.};
+            for (int hi = 0; hi < 2; hi++)
+                for (int ji = 0; ji < 16; ji++) {
+                    double x = hi == 0 ? Bi + Ai * GL32x[ji] : Bi - Ai * GL32x[ji];
+                    double Q[3] = {E[0] + x * ua[0], E[1] + x * ua[1], E[2] + x * ua[2]};
+                    double v[4];
+                    bq_point(0, Q, P, v);
+                    for (int k = 0; k < 4; k++) si[k] += GL32w[ji] * v[k];
+                }
+            for (int k = 0; k < 4; k++) so[k] += GL32w[jo] * (Ai * si[k]);
+        }
+    for (int k = 0; k < 4; k++) out[k] = Ao * so[k];
+}
+
+/* KElectrostaticBiQuadratureRectangleIntegrator.cc:18-104: outer 32-node rule over x in
+   [0, A] along N1, inner over y in [0, B] along N2 */
+void ko_rect_biquad(const double* d, const double P[3], double out[4])
+{
+    double so[4] = {0., 0., 0., 0.};
+    double Ao = (d[0] - 0.) / 2., Bo = (d[0] + 0.) / 2.;
+    double Ai = (d[1] - 0.) / 2., Bi = (d[1] + 0.) / 2.;
+    for (int half = 0; half < 2; half++)
+        for (int jo = 0; jo < 16; jo++) {
+            double x = half == 0 ? Bo + Ao * GL32x[jo] : Bo - Ao * GL32x[jo];
+            double si[4] = {0., 0., 0., 0.};
+            for (int hi = 0; hi < 2; hi++)
+                for (int ji = 0; ji < 16; ji++) {
+                    double y = hi == 0 ? Bi + Ai * GL32x[ji] : Bi - Ai * GL32x[ji];
+                    double Q[3];
+                    for (int k = 0; k < 3; k++) Q[k] = d[2 + k] + x * d[5 + k] + y * d[8 + k];
+                    double v[4];
+                    bq_point(1, Q, P, v);
+                    for (int k = 0; k < 4; k++) si[k] += GL32w[ji] * v[k];
+                }
+            for (int k = 0; k < 4; k++) so[k] += GL32w[jo] * (Ai * si[k]);
+        }
+    for (int k = 0; k < 4; k++) out[k] = Ao * so[k];
+}
+
+/* KElectrostatic256NodeQuadratureLineSegmentIntegrator.cc:192-270 (ElectricFieldAndPotential):
+   the segment in 8 equal parts, the 32-node rule on each */
+void ko_line_256(const double* d, const double P[3], double out[4])
+{
+    double L = sqrt(SQ(d[3] - d[0]) + SQ(d[4] - d[1]) + SQ(d[5] - d[2]));
+    double pu = 1. / L;
+    double u[3] = {pu * (d[3] - d[0]), pu * (d[4] - d[1]), pu * (d[5] - d[2])};
+    double seg = L / 8, hs = 0.5 * seg;
+    double prefac = 0.25 * d[6] / KO_EPS0;
+    double sf[3] = {0., 0., 0.}, sp = 0.;
+    for (int k = 0; k < 8; k++) {
+        double ctr[3];
+        for (int j = 0; j < 3; j++) {
+            double s0 = d[j] + (k * seg * u[j]);
+            double s1 = d[j] + ((k + 1) * seg * u[j]);
+            ctr[j] = 0.5 * (s0 + s1);
+        }
+        for (int i = 0; i < 16; i++) {
+            double wf = GL32w[i] * hs, nd = GL32x[i] * hs;
+            double pd[3], ndir[3], pm = 0., nm = 0.;
+            for (int j = 0; j < 3; j++) {
+                pd[j] = P[j] - (ctr[j] + (u[j] * nd));
+                pm += SQ(pd[j]);
+                ndir[j] = P[j] - (ctr[j] - (u[j] * nd));
+                nm += SQ(ndir[j]);
+            }
+            pm = 1. / sqrt(pm);
+            nm = 1. / sqrt(nm);
+            sp += wf * (pm + nm);
+            pm = pm * pm * pm;
+            nm = nm * nm * nm;
+            for (int j = 0; j < 3; j++) sf[j] += wf * (pm * pd[j] + nm * ndir[j]);
+        }
+    }
+    for (int j = 0; j < 3; j++) out[j] = prefac * sf[j];
+    out[3] = prefac * sp;
+}
+
 /* ------------------------------------------------------------ policies */
 /* KElectrostaticBoundaryIntegratorFactory.cc:29-88: numeric = cubature/RWG dispatch (default),
-   rwg = closed form everywhere (line: analytic). */
+   rwg = closed form everywhere (line: analytic), analytic = the Analytic integrators,
+   reference = nested 32-node Gauss-Legendre (line: 256 nodes). */
 int ko_element_eval(int policy, int shape, const double* d, const double P[3], double out[4])
 {
+    if (policy == KO_ANALYTIC) {
+        if (shape == KO_TRIANGLE) ko_tri_analytic(d, P, out);
+        else if (shape == KO_RECTANGLE) ko_rect_analytic(d, P, out);
+        else ko_line_analytic(d, P, out);
+        return 0;
+    }
+    if (policy == KO_REFERENCE) {
+        if (shape == KO_TRIANGLE) ko_tri_biquad(d, P, out);
+        else if (shape == KO_RECTANGLE) ko_rect_biquad(d, P, out);
+        else ko_line_256(d, P, out);
+        return 0;
+    }
     if (shape == KO_TRIANGLE) {
         if (policy == KO_RWG) {
             ko_tri_rwg(d, P, out);

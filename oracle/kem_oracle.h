@@ -33,7 +33,7 @@ extern "C" {
 #define KO_REC 11
 
 enum ko_shape { KO_TRIANGLE = 0, KO_RECTANGLE = 1, KO_LINESEGMENT = 2 };
-enum ko_policy { KO_NUMERIC = 0, KO_RWG = 1, KO_ANALYTIC = 2 };
+enum ko_policy { KO_NUMERIC = 0, KO_RWG = 1, KO_ANALYTIC = 2, KO_REFERENCE = 3 };
 enum ko_bc { KO_DIRICHLET = 0, KO_NEUMANN = 1 };
 
 /* constants: KEMField/Source/Core/src/KEMConstants.cc:9-12 over Kommon KConst_2021.h:37,139-146 */
@@ -70,6 +70,15 @@ void ko_rect_rwg(const double* rec, const double P[3], double out[4]);
 void ko_line_quadrature(int nnodes, const double* rec, const double P[3], double out[4]);
 void ko_line_analytic(const double* rec, const double P[3], double out[4]);
 
+/* "analytic" policy: Analytic/src/KElectrostaticAnalyticTriangleIntegrator.cc:13-174 (helpers
+   :176-506), Analytic/src/KElectrostaticAnalyticRectangleIntegrator.cc:25-205 */
+void ko_tri_analytic(const double* rec, const double P[3], double out[4]);
+void ko_rect_analytic(const double* rec, const double P[3], double out[4]);
+/* "reference" policy: Reference/src/KElectrostaticBiQuadratureTriangleIntegrator.cc:24-150,
+   ...BiQuadratureRectangleIntegrator.cc:18-104, ...256NodeQuadratureLineSegmentIntegrator.cc:192-270 */
+void ko_tri_biquad(const double* rec, const double P[3], double out[4]);
+void ko_rect_biquad(const double* rec, const double P[3], double out[4]);
+void ko_line_256(const double* rec, const double P[3], double out[4]);
 /* the "numeric" dispatch (default policy): cubature 7/12/33 vs closed form by distance ratio.
    Returns the regime used: 7, 12, 33 (cubature), 0 (closed form), 4/16 for line nodes. */
 int ko_tri_numeric(const double* rec, const double P[3], double out[4]);

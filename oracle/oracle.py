@@ -14,7 +14,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = os.path.join(_HERE, "libkem_oracle.so")
 
 TRIANGLE, RECTANGLE, LINESEGMENT = 0, 1, 2
-NUMERIC, RWG = 0, 1
+NUMERIC, RWG, ANALYTIC, REFERENCE = 0, 1, 2, 3
 DIRICHLET, NEUMANN = 0, 1
 REC = 11
 
@@ -48,6 +48,11 @@ def _load():
         "ko_rect_rwg": (None, [P, P, P]),
         "ko_line_quadrature": (None, [ctypes.c_int, P, P, P]),
         "ko_line_analytic": (None, [P, P, P]),
+        "ko_tri_analytic": (None, [P, P, P]),
+        "ko_rect_analytic": (None, [P, P, P]),
+        "ko_tri_biquad": (None, [P, P, P]),
+        "ko_rect_biquad": (None, [P, P, P]),
+        "ko_line_256": (None, [P, P, P]),
         "ko_tri_numeric": (ctypes.c_int, [P, P, P]),
         "ko_rect_numeric": (ctypes.c_int, [P, P, P]),
         "ko_line_numeric": (ctypes.c_int, [P, P, P]),
@@ -177,6 +182,26 @@ def line_quadrature(nn, rec, P):
 
 def line_analytic(rec, P):
     return _eval4(lib().ko_line_analytic, _p(_recs(rec)), _p(_f64(P)))
+
+
+def tri_analytic(rec, P):
+    return _eval4(lib().ko_tri_analytic, _p(_recs(rec)), _p(_f64(P)))
+
+
+def rect_analytic(rec, P):
+    return _eval4(lib().ko_rect_analytic, _p(_recs(rec)), _p(_f64(P)))
+
+
+def tri_biquad(rec, P):
+    return _eval4(lib().ko_tri_biquad, _p(_recs(rec)), _p(_f64(P)))
+
+
+def rect_biquad(rec, P):
+    return _eval4(lib().ko_rect_biquad, _p(_recs(rec)), _p(_f64(P)))
+
+
+def line_256(rec, P):
+    return _eval4(lib().ko_line_256, _p(_recs(rec)), _p(_f64(P)))
 
 
 def element_eval(shape, rec, P, policy=NUMERIC):

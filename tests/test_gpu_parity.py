@@ -53,8 +53,11 @@ def random_elements(rng, n, shape):
     return np.array(recs)
 
 
+POLICIES = {"numeric": 0, "rwg": 1, "analytic": 2, "reference": 3}
+
+
 @pytest.mark.parametrize("shape", [0, 1, 2])
-@pytest.mark.parametrize("policy", ["numeric", "rwg"])
+@pytest.mark.parametrize("policy", ["numeric", "rwg", "analytic", "reference"])
 def test_element_integrals_all_regimes(ctx, oracle, shape, policy):
     from kassiopeia_amd import kemfield as K
 
@@ -75,14 +78,15 @@ def test_element_integrals_all_regimes(ctx, oracle, shape, policy):
             pts.append(cen[j] + d * dr * scale)
             drs.append(dr)
     E, phi = integ.ElectricFieldAndPotential(elems, pts)
-    pol = 0 if policy == "numeric" else 1
+    pol = POLICIES[policy]
     worst_p = worst_e = 0.0
     for k, (j, P) in enumerate(zip(elems, pts)):
         ref, reg = oracle.element_eval(shape, recs[j], P, policy=pol)
         regs.append(reg)
-        # the closed form cancels like eps * dr^2 far from the element (the numeric policy
-        # switches to cubature there); its parity bound scales accordingly
-        tol = TOL if (policy == "numeric" or shape == 2) else max(TOL, 1e-14 * drs[k] ** 2)
+        # the closed forms (rwg, analytic) cancel like eps * dr^2 far from the element (the
+        # numeric policy switches to cubature there); their parity bound scales accordingly
+        closed = policy in ("rwg", "analytic") and shape != 2
+        tol = max(TOL, 1e-14 * drs[k] ** 2) if closed else TOL
         worst_p = max(worst_p, abs(phi[k] - ref[3]) / abs(ref[3]) / tol)
         worst_e = max(worst_e, relerr(E[k], ref[:3]) / tol)
     assert worst_p < 1.0, worst_p
@@ -234,6 +238,36 @@ def test_field_matches_oracle(ctx, oracle, cube6):
     # split over element ranges (few points) and one range (many points) agree
     E2, phi2 = K.KIntegratingElectrostaticFieldSolver(cont, sigma).ElectricFieldAndPotential(pts[:3])
     assert (np.abs(phi2 - phi[:3]) / np.abs(phi[:3])).max() < 1e-14
+
+
+@pytest.mark.parametrize("policy", ["analytic", "reference"])
+def test_policy_assembly_and_field(ctx, oracle, policy):
+    """Collocation matrix and field under the analytic / reference integrators
+    (KElectrostaticBoundaryIntegratorFactory::MakeAnalytic / MakeReference) against the oracle,
+    with a rectangle and a segment in the mesh so every integrator of the policy runs."""
+    from kassiopeia_amd import kemfield as K
+
+    wire = G.line_records([[-0.2, -0.2, 1.2]], [[0.3, 0.2, 1.6]], 2e-3)
+    line = G.Mesh(np.full(1, 2, np.int32), wire, np.zeros(1, np.int32), np.ones(1), np.zeros(1),
+                  np.zeros(1, np.int32))
+    m = G.concat(G.cube_mesh(2), G.cube_rect_mesh(1, edge=1.5), line)
+    pol = POLICIES[policy]
+    Aref = oracle.assemble(m.shapes, m.recs, policy=pol)
+    ctx.set_integrator(policy)
+    try:
+        cont = K.KSurfaceContainer(ctx, m)
+        A = K.KBoundaryIntegralMatrix(cont).to_numpy()
+        assert np.abs(A - Aref).max() / np.abs(Aref).max() < TOL
+        diag = np.abs(np.diag(Aref))
+        assert (np.abs(np.diag(A) - np.diag(Aref)) / diag).max() < TOL
+        sigma = np.linspace(1.0, 2.0, m.n) * 1e-11
+        pts = np.concatenate([G.points_shell(40, 0.9, 2.0, 9), [[0.5, 0.1, 0.2], [0.1, 0.2, 0.5]]])
+        E, phi = K.KIntegratingElectrostaticFieldSolver(cont, sigma).ElectricFieldAndPotential(pts)
+        phr, Er = oracle.field(m.shapes, m.recs, sigma, pts, policy=pol)
+        assert (np.abs(phi - phr) / np.abs(phr)).max() < TOL
+        assert max(relerr(E[k], Er[k]) for k in range(len(pts))) < 1e-11
+    finally:
+        ctx.set_integrator("numeric")
 
 
 def test_field_sorted_path(ctx, oracle, cube6):
