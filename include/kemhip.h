@@ -53,6 +53,7 @@ extern "C" {
 /* ---- solvers (KKrylovSolverFactory.hh:40-199, KRobinHood.hh, KGaussianElimination.hh) */
 #define KEM_SOLVER_GMRES 0
 #define KEM_SOLVER_ROBIN_HOOD 1
+#define KEM_SOLVER_GAUSS 2 /* Gauss-Jordan without pivoting, KGaussianElimination.hh:24-58 (n <= 32768) */
 
 typedef struct kem_ctx kem_ctx;
 typedef struct kem_mesh kem_mesh;
@@ -168,6 +169,20 @@ int kem_matrix_multiply(kem_matrix* A, int transpose, const double* x, double* y
    and the solution on exit.  Returns KEM_OK, or KEM_W_NOT_CONVERGED at the iteration/time limit. */
 void kem_solve_opts_default(kem_solve_opts* opts);
 int kem_solve(kem_matrix* A, const kem_solve_opts* opts, const double* b, double* x, kem_solve_stats* stats);
+
+/* Multi-right-hand-side solve (electrode voltage sets).  The reference solves one right-hand
+   side per KChargeDensitySolver call (KKrylovChargeDensitySolver.cc:42-78); this runs GMRES
+   for nrhs of them in lockstep, each with the reference's convergence rule
+   (KSimpleIterativeKrylovSolver.hh:84-103), sharing one block matrix product per iteration
+   (KEM_MIXED: FP32 matrix on MFMA, FP64 refinement against the FP64 matrix).  B, X: host,
+   column-major n x nrhs (X holds the initial guesses and receives the solutions).
+   residuals (optional, nrhs): final ||b - A x|| / ||b|| per right-hand side (FP64 matrix).
+   stats: iterations / refinements = maximum over groups of 64, residual = worst. */
+int kem_solve_batch(kem_matrix* A, const kem_solve_opts* opts, int64_t nrhs, const double* B, double* X,
+                    double* residuals, kem_solve_stats* stats);
+/* Y = A X for nrhs vectors (host, column-major n x nrhs): KSquareMatrix::Multiply
+   (KMatrix.hh:92-111) over a block; KEM_MIXED uses the FP32 matrix on MFMA. */
+int kem_matrix_multiply_batch(kem_matrix* A, int precision, int64_t nrhs, const double* X, double* Y);
 
 #ifdef __cplusplus
 }

@@ -13,7 +13,7 @@ KEM_W_NOT_CONVERGED = 1
 KEM_E_ARG, KEM_E_HIP, KEM_E_NAME, KEM_E_GEOM, KEM_E_STATE = -1, -2, -3, -4, -5
 KEM_FP64, KEM_MIXED = 0, 1
 KEM_STORE_FP64, KEM_STORE_FP32 = 1, 2
-KEM_SOLVER_GMRES, KEM_SOLVER_ROBIN_HOOD = 0, 1
+KEM_SOLVER_GMRES, KEM_SOLVER_ROBIN_HOOD, KEM_SOLVER_GAUSS = 0, 1, 2
 
 
 class KemError(RuntimeError):
@@ -80,6 +80,8 @@ SIGNATURES = {
     "kem_matrix_multiply": (_INT, [_P, _INT, _P, _P]),
     "kem_solve_opts_default": (None, [ctypes.POINTER(SolveOpts)]),
     "kem_solve": (_INT, [_P, ctypes.POINTER(SolveOpts), _P, _P, ctypes.POINTER(SolveStats)]),
+    "kem_solve_batch": (_INT, [_P, ctypes.POINTER(SolveOpts), _I64, _P, _P, _P, ctypes.POINTER(SolveStats)]),
+    "kem_matrix_multiply_batch": (_INT, [_P, _INT, _I64, _P, _P]),
 }
 
 _lib = None

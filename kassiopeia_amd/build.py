@@ -25,6 +25,7 @@ COMMON = ["-O3", "-fPIC", "-std=c++17", "-Wall", f"--offload-arch={ARCH}", "-mun
 SOURCES = [
     ("kem_kernels.hip", True),
     ("kem_sort.hip", True),
+    ("kem_batch.hip", True),
     ("kem_capi.cpp", False),
     ("kem_prep.cpp", False),
 ]

@@ -1,0 +1,345 @@
+// kem_batch.hip -- multi-right-hand-side kernels: block matrix products on MFMA (FP32 matrix)
+// and on the FP64 vector ALU, and the batched Krylov-basis operations of lockstep GMRES.
+//
+// Layouts.  The matrix is column-major, A(i, j) at A[j*n + i].  Block operands fed to the
+// products are row-major "transposed" panels Vt[j*KP + c] (row j of the n x KP panel holds one
+// entry per right-hand side c, zero-padded to KP columns).  Per-right-hand-side vectors are
+// column-major: W[c*ldw + i].  The Krylov bases are V[(r*(kmax+1) + q)*n + i] (basis vector q of
+// right-hand side r).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+
+#include "kem_kernels.h"
+
+namespace kem {
+namespace {
+
+constexpr int kBlock = 256;
+typedef float f16v __attribute__((ext_vector_type(16)));
+
+inline unsigned nblocks(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+
+// ------------------------------------------------------------------ FP32 MFMA block product
+// part[s][i][c] = sum_{j in split s} A32(i, j) * Vt[j][c]   (c < KP = 32*NT)
+// One wavefront computes a 64-row x KP tile with v_mfma_f32_32x32x2f32: operand A is 32 rows x
+// 2 columns of the matrix (lane l holds row l%32 of column l/32: two 128-byte runs per load),
+// operand B is 2 rows x 32 columns of the panel (lane l holds panel row l/32, column l%32), and
+// accumulator register v of lane l holds tile row 8*(v/4) + 4*(l/32) + v%4, column l%32 (lane
+// layout measured on gfx950).  Four wavefronts per workgroup take consecutive 64-row tiles of the
+// same element range; blockIdx.y splits the columns of A so that the grid fills the chip.
+template <int NT>
+__global__ __launch_bounds__(kBlock) void k_bmm32(const float* __restrict__ A, int64_t n,
+                                                  const float* __restrict__ Vt, int64_t per_split,
+                                                  float* __restrict__ part)
+{
+    constexpr int KP = 32 * NT;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int64_t i0 = ((int64_t)blockIdx.x * 4 + wave) * 64;
+    if (i0 >= n) return;  // whole wavefront beyond the last row
+    const int64_t j0 = (int64_t)blockIdx.y * per_split;
+    const int64_t j1 = std::min<int64_t>(n, j0 + per_split);
+    const int lr = lane & 31, lk = lane >> 5;
+    const int64_t ra = std::min<int64_t>(i0 + lr, n - 1), rb = std::min<int64_t>(i0 + 32 + lr, n - 1);
+    f16v acc0[NT], acc1[NT];
+    for (int t = 0; t < NT; t++)
+        for (int v = 0; v < 16; v++) {
+            acc0[t][v] = 0.f;
+            acc1[t][v] = 0.f;
+        }
+    for (int64_t j = j0; j < j1; j += 2) {
+        const int64_t jj = j + lk;
+        const bool ok = jj < j1;
+        const int64_t js = ok ? jj : j;  // j < j1: a valid column
+        float a0 = A[js * n + ra], a1 = A[js * n + rb];
+        float b[NT];
+        for (int t = 0; t < NT; t++) b[t] = Vt[js * KP + 32 * t + lr];
+        if (!ok) {
+            a0 = a1 = 0.f;
+            for (int t = 0; t < NT; t++) b[t] = 0.f;
+        }
+        for (int t = 0; t < NT; t++) {
+            acc0[t] = __builtin_amdgcn_mfma_f32_32x32x2f32(a0, b[t], acc0[t], 0, 0, 0);
+            acc1[t] = __builtin_amdgcn_mfma_f32_32x32x2f32(a1, b[t], acc1[t], 0, 0, 0);
+        }
+    }
+    float* out = part + (int64_t)blockIdx.y * n * KP;
+    for (int t = 0; t < NT; t++)
+        for (int v = 0; v < 16; v++) {
+            const int64_t r = 8 * (v >> 2) + 4 * lk + (v & 3);
+            if (i0 + r < n) out[(i0 + r) * KP + 32 * t + lr] = acc0[t][v];
+            if (i0 + 32 + r < n) out[(i0 + 32 + r) * KP + 32 * t + lr] = acc1[t][v];
+        }
+}
+
+// W[c*ldw + i] = sum_s part[s][i][c] for c < ncols: 64-row tiles summed in FP64 (splits in
+// order) and transposed through LDS so that both the reads and the writes are coalesced.
+template <int KP>
+__global__ __launch_bounds__(kBlock) void k_bmm32_reduce(const float* __restrict__ part, int nsplit, int64_t n,
+                                                         int ncols, double* __restrict__ W, int64_t ldw)
+{
+    __shared__ double sh[KP][65];
+    const int64_t i0 = (int64_t)blockIdx.x * 64;
+    for (int e = threadIdx.x; e < 64 * KP; e += kBlock) {
+        const int ii = e / KP, c = e % KP;
+        const int64_t i = i0 + ii;
+        double s = 0.;
+        if (i < n)
+            for (int k = 0; k < nsplit; k++) s += (double)part[((int64_t)k * n + i) * KP + c];
+        sh[c][ii] = s;
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < 64 * KP; e += kBlock) {
+        const int c = e / 64, ii = e % 64;
+        const int64_t i = i0 + ii;
+        if (c < ncols && i < n) W[c * ldw + i] = sh[c][ii];
+    }
+}
+
+// ------------------------------------------------------------------ FP64 block product (VALU)
+// part[s][c][i] = sum_{j in split s} A64(i, j) * Xt[j][c], KB right-hand sides per lane; the
+// panel row Xt[j][:] is wave-uniform (scalar loads).
+template <int KB>
+__global__ __launch_bounds__(kBlock) void k_bmm64(const double* __restrict__ A, int64_t n,
+                                                  const double* __restrict__ Xt, int64_t per_split,
+                                                  double* __restrict__ part)
+{
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const int64_t ic = std::min<int64_t>(i, n - 1);
+    const int64_t j0 = (int64_t)blockIdx.y * per_split;
+    const int64_t j1 = std::min<int64_t>(n, j0 + per_split);
+    double acc[KB];
+    for (int c = 0; c < KB; c++) acc[c] = 0.;
+    for (int64_t j = j0; j < j1; j++) {
+        const double a = A[j * n + ic];
+        const double* x = Xt + j * KB;
+        for (int c = 0; c < KB; c++) acc[c] += a * x[c];
+    }
+    if (i < n) {
+        double* out = part + (int64_t)blockIdx.y * KB * n;
+        for (int c = 0; c < KB; c++) out[c * n + i] = acc[c];
+    }
+}
+
+// W[c*ldw + i] = sum_s part[s][c][i] for c < ncols
+__global__ __launch_bounds__(kBlock) void k_bmm64_reduce(const double* __restrict__ part, int nsplit, int kb,
+                                                         int64_t n, int ncols, double* __restrict__ W, int64_t ldw)
+{
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const int c = blockIdx.y;
+    if (i >= n || c >= ncols) return;
+    double s = 0.;
+    for (int k = 0; k < nsplit; k++) s += part[((int64_t)k * kb + c) * n + i];
+    W[c * ldw + i] = s;
+}
+
+// Vt[i][c] = X[c*ldx + i] (c < ncols), 0 for the padding columns; T = float or double
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_panel(const double* __restrict__ X, int64_t ldx, int64_t n, int ncols,
+                                                  int kp, T* __restrict__ Vt)
+{
+    const int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (e >= n * kp) return;
+    const int64_t i = e / kp;
+    const int c = (int)(e % kp);
+    Vt[e] = c < ncols ? (T)X[c * ldx + i] : (T)0;
+}
+
+// ------------------------------------------------------------------ batched Krylov operations
+__device__ __forceinline__ double block_sum(double v, double* sh)
+{
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) sh[w] = v;
+    __syncthreads();
+    double r = 0.;
+    if (threadIdx.x == 0)
+        for (int k = 0; k < (int)(blockDim.x >> 6); k++) r += sh[k];
+    __syncthreads();
+    return r;
+}
+
+// h[r*ldh + q] = <V_r q, W_r> for q <= j (block per (q, r))
+__global__ __launch_bounds__(kBlock) void k_bproj(const double* __restrict__ V, int64_t vstride,
+                                                  const double* __restrict__ W, int64_t n, double* __restrict__ h,
+                                                  int ldh)
+{
+    __shared__ double sh[kBlock / 64];
+    const int q = blockIdx.x, r = blockIdx.y;
+    const double* v = V + r * vstride + (int64_t)q * n;
+    const double* w = W + (int64_t)r * n;
+    double s = 0.;
+    for (int64_t i = threadIdx.x; i < n; i += kBlock) s += v[i] * w[i];
+    s = block_sum(s, sh);
+    if (threadIdx.x == 0) h[r * ldh + q] = s;
+}
+
+// Y_r += sign * sum_{q < cnt_r} V_r q * h[r*ldh + q]   (grid: rows x r)
+__global__ __launch_bounds__(kBlock) void k_bupdate(const double* __restrict__ V, int64_t vstride,
+                                                    const double* __restrict__ h, int ldh,
+                                                    const int* __restrict__ cnt, double sign, int64_t n,
+                                                    double* __restrict__ Y)
+{
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const int r = blockIdx.y;
+    if (i >= n) return;
+    const double* v = V + r * vstride + i;
+    const double* hr = h + r * ldh;
+    const int m = cnt[r];
+    double s = 0.;
+    for (int q = 0; q < m; q++) s += v[(int64_t)q * n] * hr[q];
+    Y[(int64_t)r * n + i] += sign * s;
+}
+
+// nrm[r] = ||W_r|| (block per r)
+__global__ __launch_bounds__(kBlock) void k_bnorm(const double* __restrict__ W, int64_t n, double* __restrict__ nrm,
+                                                  int stride)
+{
+    __shared__ double sh[kBlock / 64];
+    const int r = blockIdx.x;
+    const double* w = W + (int64_t)r * n;
+    double s = 0.;
+    for (int64_t i = threadIdx.x; i < n; i += kBlock) s += w[i] * w[i];
+    s = block_sum(s, sh);
+    if (threadIdx.x == 0) nrm[r * stride] = sqrt(s);
+}
+
+// V_r q = W_r * (1 / nrm_r); panel column r = V_r q for active r, 0 otherwise
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_bnormalize(const double* __restrict__ W, const double* __restrict__ nrm,
+                                                       int stride, const int* __restrict__ active, int64_t n,
+                                                       double* __restrict__ V, int64_t vstride, int q, int kp,
+                                                       T* __restrict__ Vt)
+{
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const int r = blockIdx.y;
+    if (i >= n) return;
+    const double d = nrm[r * stride];
+    const double v = active[r] ? W[(int64_t)r * n + i] * (1.0 / d) : 0.;
+    V[r * vstride + (int64_t)q * n + i] = v;
+    Vt[i * kp + r] = (T)v;
+}
+
+// Z_r = a X_r + b Y_r over all r (column-major, ld n), elementwise
+__global__ __launch_bounds__(kBlock) void k_baxpby(double a, const double* __restrict__ X, double b,
+                                                   const double* __restrict__ Y, double* __restrict__ Z, int64_t len)
+{
+    const int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (e < len) Z[e] = a * X[e] + b * Y[e];
+}
+
+int bmm_splits(int64_t n, int64_t tiles)
+{
+    int64_t s = (2560 + tiles - 1) / std::max<int64_t>(tiles, 1);
+    s = std::min<int64_t>(s, std::max<int64_t>(1, n / 256));
+    return (int)std::max<int64_t>(1, std::min<int64_t>(s, 64));
+}
+
+}  // namespace
+
+size_t bmm_workspace_bytes(int64_t n, int nrhs, bool fp32)
+{
+    const int kp = nrhs <= 32 ? 32 : 64;
+    if (fp32) {
+        const int64_t tiles = (n + 63) / 64;
+        const int s = bmm_splits(n, tiles);
+        return sizeof(float) * ((size_t)s * n * kp + (size_t)n * kp) + 256;
+    }
+    const int kb = nrhs <= 16 ? 16 : 64;
+    const int64_t blocks = (n + kBlock - 1) / kBlock;
+    const int s = bmm_splits(n, blocks * 4);
+    return sizeof(double) * ((size_t)s * n * kb + (size_t)n * kb) + 256;
+}
+
+hipError_t launch_bmm(hipStream_t st, const float* A32, const double* A64, int64_t n, const double* X, int64_t ldx,
+                      int nrhs, double* Y, int64_t ldy, void* ws, size_t ws_bytes, bool x_is_panel)
+{
+    if (n <= 0 || nrhs <= 0) return hipSuccess;
+    if (nrhs > 64 || ws_bytes < bmm_workspace_bytes(n, nrhs, A32 != nullptr)) return hipErrorInvalidValue;
+    if (A32) {
+        const int nt = nrhs <= 32 ? 1 : 2, kp = 32 * nt;
+        const int64_t tiles = (n + 63) / 64;
+        const int s = bmm_splits(n, tiles);
+        int64_t per = (n + s - 1) / s;
+        per += per & 1;
+        const int ns = (int)((n + per - 1) / per);
+        float* part = (float*)ws;
+        float* Vt = part + (size_t)s * n * kp;
+        const float* P = x_is_panel ? (const float*)X : Vt;
+        if (!x_is_panel)
+            k_panel<float><<<nblocks(n * kp), kBlock, 0, st>>>(X, ldx, n, nrhs, kp, Vt);
+        dim3 grid((unsigned)((tiles + 3) / 4), (unsigned)ns);
+        if (nt == 1)
+            k_bmm32<1><<<grid, kBlock, 0, st>>>(A32, n, P, per, part);
+        else
+            k_bmm32<2><<<grid, kBlock, 0, st>>>(A32, n, P, per, part);
+        const unsigned gr = (unsigned)((n + 63) / 64);
+        if (kp == 32)
+            k_bmm32_reduce<32><<<gr, kBlock, 0, st>>>(part, ns, n, nrhs, Y, ldy);
+        else
+            k_bmm32_reduce<64><<<gr, kBlock, 0, st>>>(part, ns, n, nrhs, Y, ldy);
+        return hipGetLastError();
+    }
+    const int kb = nrhs <= 16 ? 16 : 64;
+    const int64_t blocks = (n + kBlock - 1) / kBlock;
+    const int s = bmm_splits(n, blocks * 4);
+    const int64_t per = (n + s - 1) / s;
+    const int ns = (int)((n + per - 1) / per);
+    double* part = (double*)ws;
+    double* Xt = part + (size_t)s * n * kb;
+    const double* P = x_is_panel ? X : Xt;
+    if (!x_is_panel) k_panel<double><<<nblocks(n * kb), kBlock, 0, st>>>(X, ldx, n, nrhs, kb, Xt);
+    dim3 grid((unsigned)blocks, (unsigned)ns);
+    if (kb == 16)
+        k_bmm64<16><<<grid, kBlock, 0, st>>>(A64, n, P, per, part);
+    else
+        k_bmm64<64><<<grid, kBlock, 0, st>>>(A64, n, P, per, part);
+    k_bmm64_reduce<<<dim3((unsigned)blocks, (unsigned)nrhs), kBlock, 0, st>>>(part, ns, kb, n, nrhs, Y, ldy);
+    return hipGetLastError();
+}
+
+hipError_t launch_bproj(hipStream_t st, const double* V, int64_t vstride, const double* W, int64_t n, int nq,
+                        int nrhs, double* h, int ldh)
+{
+    if (nq <= 0 || nrhs <= 0) return hipSuccess;
+    k_bproj<<<dim3((unsigned)nq, (unsigned)nrhs), kBlock, 0, st>>>(V, vstride, W, n, h, ldh);
+    return hipGetLastError();
+}
+
+hipError_t launch_bupdate(hipStream_t st, const double* V, int64_t vstride, const double* h, int ldh, const int* cnt,
+                          double sign, int64_t n, int nrhs, double* Y)
+{
+    if (nrhs <= 0) return hipSuccess;
+    k_bupdate<<<dim3(nblocks(n), (unsigned)nrhs), kBlock, 0, st>>>(V, vstride, h, ldh, cnt, sign, n, Y);
+    return hipGetLastError();
+}
+
+hipError_t launch_bnorm(hipStream_t st, const double* W, int64_t n, int nrhs, double* nrm, int stride)
+{
+    if (nrhs <= 0) return hipSuccess;
+    k_bnorm<<<(unsigned)nrhs, kBlock, 0, st>>>(W, n, nrm, stride);
+    return hipGetLastError();
+}
+
+hipError_t launch_bnormalize(hipStream_t st, const double* W, const double* nrm, int stride, const int* active,
+                             int64_t n, int nrhs, double* V, int64_t vstride, int q, int kp, float* Vt32,
+                             double* Vt64)
+{
+    if (nrhs <= 0) return hipSuccess;
+    dim3 grid(nblocks(n), (unsigned)nrhs);
+    if (Vt32)
+        k_bnormalize<float><<<grid, kBlock, 0, st>>>(W, nrm, stride, active, n, V, vstride, q, kp, Vt32);
+    else
+        k_bnormalize<double><<<grid, kBlock, 0, st>>>(W, nrm, stride, active, n, V, vstride, q, kp, Vt64);
+    return hipGetLastError();
+}
+
+hipError_t launch_baxpby(hipStream_t st, double a, const double* X, double b, const double* Y, double* Z, int64_t len)
+{
+    if (len <= 0) return hipSuccess;
+    k_baxpby<<<nblocks(len), kBlock, 0, st>>>(a, X, b, Y, Z, len);
+    return hipGetLastError();
+}
+
+}  // namespace kem

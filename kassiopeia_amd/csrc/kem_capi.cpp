@@ -805,6 +805,18 @@ int kem_solve(kem_matrix* A, const kem_solve_opts* opts, const double* b, double
             s.residual = rel;
             if (rel > o.tolerance) status = KEM_W_NOT_CONVERGED;
         }
+    } else if (o.method == KEM_SOLVER_GAUSS) {
+        // dense direct solve on a copy of the FP64 matrix (KGaussianEliminationChargeDensitySolver)
+        if (!A->A64) return fail(c, KEM_E_STATE, "Gaussian elimination needs FP64 matrix storage");
+        if (n > 32768) return fail(c, KEM_E_ARG, "Gaussian elimination is limited to n <= 32768");
+        DVec M, f;
+        KEM_CK(c, dalloc(&M.p, n * (n + 1)));
+        KEM_CK(c, dalloc(&f.p, n));
+        KEM_CK(c, hipMemcpyAsync(M.p, A->A64, sizeof(double) * n * n, hipMemcpyDeviceToDevice, c->stream));
+        KEM_CK(c, hipMemcpyAsync(M.p + n * n, db.p, sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));
+        KEM_CK(c, kem::launch_gauss_jordan(c->stream, M.p, n, f.p, dx.p));
+        KEM_CK(c, hipStreamSynchronize(c->stream));  // before M and f are released
+        s.iterations = 1;
     } else {
         return fail(c, KEM_E_NAME, "unknown solver method");
     }
@@ -817,6 +829,358 @@ int kem_solve(kem_matrix* A, const kem_solve_opts* opts, const double* b, double
         if (rc != KEM_OK) return rc;
     }
     KEM_CK(c, hipStreamSynchronize(c->stream));
+    s.seconds = now_s() - ctl.t0;
+    if (stats) *stats = s;
+    return status;
+}
+
+
+// ------------------------------------------------------------------ multi-RHS
+namespace {
+
+// panel width of the block product for nrhs right-hand sides
+int panel_cols(int nrhs, bool fp32) { return fp32 ? (nrhs <= 32 ? 32 : 64) : (nrhs <= 16 ? 16 : 64); }
+
+// Y = A X for up to 64 right-hand sides (device, column-major with leading dimension n)
+int block_product(kem_matrix* A, bool use32, int nrhs, const double* d_X, double* d_Y, void** ws, size_t* ws_bytes)
+{
+    kem_ctx* c = A->ctx;
+    const size_t need = kem::bmm_workspace_bytes(A->n, nrhs, use32);
+    if (*ws_bytes < need) {
+        if (*ws) (void)hipFree(*ws);
+        *ws = nullptr;
+        *ws_bytes = 0;
+        KEM_CK(c, hipMalloc(ws, need));
+        *ws_bytes = need;
+    }
+    KEM_CK(c, kem::launch_bmm(c->stream, use32 ? A->A32 : nullptr, use32 ? nullptr : A->A64, A->n, d_X, A->n, nrhs,
+                              d_Y, A->n, *ws, *ws_bytes, false));
+    return KEM_OK;
+}
+
+struct DBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    ~DBuf()
+    {
+        if (p) (void)hipFree(p);
+    }
+};
+
+// Lockstep GMRES over nrhs <= 64 right-hand sides: each keeps its own Krylov space, Hessenberg
+// matrix and Givens rotations (the single-RHS algorithm of gmres_dev, CGS2, x <- x0 + V y), and
+// one block product A [v_1 .. v_k] per iteration serves all of them (FP32 matrix: MFMA).  A
+// right-hand side leaves the lockstep when it converges (|p_{j+1}| <= tol_r ||b_r||) or breaks
+// down.  d_B, d_X: n x nrhs column-major; X holds x0 on entry.
+int gmres_batch(kem_matrix* A, bool use32, int nrhs, const double* d_B, double* d_X, const std::vector<double>& tol,
+                int maxit, int restart, const SolveCtl& ctl, int* iters, std::vector<double>& resid,
+                std::vector<int>& conv_out)
+{
+    kem_ctx* c = A->ctx;
+    hipStream_t st = c->stream;
+    const int64_t n = A->n;
+    int kmax = (int)std::min<int64_t>(std::min<int64_t>(maxit, 300), n + 1);
+    if (restart > 0) kmax = std::min(kmax, restart);
+    if (kmax < 1) kmax = 1;
+    const int ldh = kmax + 2;
+    const int kp = panel_cols(nrhs, use32);
+    const int64_t vstride = (int64_t)(kmax + 1) * n;
+    DVec V, W, h1, h2, Y;
+    DBuf panel, ws, ints;
+    KEM_CK(c, dalloc(&V.p, vstride * nrhs));
+    KEM_CK(c, dalloc(&W.p, n * nrhs));
+    KEM_CK(c, dalloc(&h1.p, (int64_t)ldh * nrhs));
+    KEM_CK(c, dalloc(&h2.p, (int64_t)ldh * nrhs));
+    KEM_CK(c, dalloc(&Y.p, (int64_t)ldh * nrhs));
+    KEM_CK(c, hipMalloc(&panel.p, (use32 ? sizeof(float) : sizeof(double)) * (size_t)n * kp));
+    KEM_CK(c, hipMemsetAsync(panel.p, 0, (use32 ? sizeof(float) : sizeof(double)) * (size_t)n * kp, st));
+    KEM_CK(c, hipMalloc(&ints.p, sizeof(int) * 2 * nrhs));
+    int* d_cnt = (int*)ints.p;
+    int* d_active = d_cnt + nrhs;
+    float* P32 = use32 ? (float*)panel.p : nullptr;
+    double* P64 = use32 ? nullptr : (double*)panel.p;
+    const size_t wsb = kem::bmm_workspace_bytes(n, nrhs, use32);
+    KEM_CK(c, hipMalloc(&ws.p, wsb));
+    ws.bytes = wsb;
+
+    std::vector<double> bnorm(nrhs), H((size_t)nrhs * ldh * ldh), cs((size_t)nrhs * ldh), sn((size_t)nrhs * ldh),
+        p((size_t)nrhs * ldh), y((size_t)nrhs * ldh), hh1((size_t)nrhs * ldh), hh2((size_t)nrhs * ldh);
+    std::vector<int> active(nrhs), jr(nrhs), cnt(nrhs), done(nrhs, 0);
+    KEM_CK(c, kem::launch_bnorm(st, d_B, n, nrhs, h1.p, ldh));
+    KEM_CK(c, hipMemcpy2DAsync(bnorm.data(), sizeof(double), h1.p, sizeof(double) * ldh, sizeof(double), nrhs,
+                               hipMemcpyDeviceToHost, st));
+    KEM_CK(c, hipStreamSynchronize(st));
+    for (auto& b : bnorm)
+        if (b == 0.) b = 1.;
+    resid.assign(nrhs, 0.);
+    int it = 0, status = KEM_OK;
+    for (;;) {
+        // W = B - A X; V_0 = W / ||W||
+        int rc = block_product(A, use32, nrhs, d_X, W.p, &ws.p, &ws.bytes);
+        if (rc != KEM_OK) return rc;
+        KEM_CK(c, kem::launch_baxpby(st, 1.0, d_B, -1.0, W.p, W.p, n * nrhs));
+        KEM_CK(c, kem::launch_bnorm(st, W.p, n, nrhs, h1.p, ldh));
+        KEM_CK(c, hipMemcpy2DAsync(hh1.data(), sizeof(double), h1.p, sizeof(double) * ldh, sizeof(double), nrhs,
+                                   hipMemcpyDeviceToHost, st));
+        KEM_CK(c, hipStreamSynchronize(st));
+        int nact = 0;
+        for (int r = 0; r < nrhs; r++) {
+            p[(size_t)r * ldh] = hh1[r];
+            resid[r] = hh1[r] / bnorm[r];
+            active[r] = !done[r] && hh1[r] > 0. && resid[r] > tol[r];
+            if (!active[r]) done[r] = 1;
+            jr[r] = 0;
+            nact += active[r];
+        }
+        if (nact == 0) break;
+        KEM_CK(c, hipMemcpyAsync(d_active, active.data(), sizeof(int) * nrhs, hipMemcpyHostToDevice, st));
+        KEM_CK(c, kem::launch_bnormalize(st, W.p, h1.p, ldh, d_active, n, nrhs, V.p, vstride, 0, kp, P32, P64));
+        int j = 0;
+        bool stop = false;
+        for (;;) {
+            KEM_CK(c, kem::launch_bmm(st, use32 ? A->A32 : nullptr, use32 ? nullptr : A->A64, n,
+                                      use32 ? (const double*)P32 : P64, n, nrhs, W.p, n, ws.p, ws.bytes, true));
+            for (int r = 0; r < nrhs; r++) cnt[r] = j + 1;
+            KEM_CK(c, hipMemcpyAsync(d_cnt, cnt.data(), sizeof(int) * nrhs, hipMemcpyHostToDevice, st));
+            KEM_CK(c, kem::launch_bproj(st, V.p, vstride, W.p, n, j + 1, nrhs, h1.p, ldh));
+            KEM_CK(c, kem::launch_bupdate(st, V.p, vstride, h1.p, ldh, d_cnt, -1.0, n, nrhs, W.p));
+            KEM_CK(c, kem::launch_bproj(st, V.p, vstride, W.p, n, j + 1, nrhs, h2.p, ldh));
+            KEM_CK(c, kem::launch_bupdate(st, V.p, vstride, h2.p, ldh, d_cnt, -1.0, n, nrhs, W.p));
+            KEM_CK(c, kem::launch_bnorm(st, W.p, n, nrhs, h1.p + j + 1, ldh));
+            KEM_CK(c, kem::launch_bnormalize(st, W.p, h1.p + j + 1, ldh, d_active, n, nrhs, V.p, vstride, j + 1, kp,
+                                             P32, P64));
+            KEM_CK(c, hipMemcpyAsync(hh1.data(), h1.p, sizeof(double) * ldh * nrhs, hipMemcpyDeviceToHost, st));
+            KEM_CK(c, hipMemcpyAsync(hh2.data(), h2.p, sizeof(double) * ldh * nrhs, hipMemcpyDeviceToHost, st));
+            KEM_CK(c, hipStreamSynchronize(st));
+            it++;
+            bool changed = false;
+            nact = 0;
+            for (int r = 0; r < nrhs; r++) {
+                if (!active[r]) continue;
+                double* h = H.data() + ((size_t)r * ldh + j) * ldh;
+                const double* a1 = hh1.data() + (size_t)r * ldh;
+                const double* a2 = hh2.data() + (size_t)r * ldh;
+                for (int q = 0; q <= j; q++) h[q] = a1[q] + a2[q];
+                const double beta = a1[j + 1];
+                h[j + 1] = beta;
+                double* csr = cs.data() + (size_t)r * ldh;
+                double* snr = sn.data() + (size_t)r * ldh;
+                double* pr = p.data() + (size_t)r * ldh;
+                for (int q = 0; q < j; q++) {
+                    const double a = h[q], bb = h[q + 1];
+                    h[q] = csr[q] * a + snr[q] * bb;
+                    h[q + 1] = -1.0 * snr[q] * a + csr[q] * bb;
+                }
+                const double gam = std::sqrt(h[j] * h[j] + beta * beta);
+                csr[j] = h[j] / gam;
+                snr[j] = beta / gam;
+                h[j] = gam;
+                pr[j + 1] = -1.0 * snr[j] * pr[j];
+                pr[j] = csr[j] * pr[j];
+                jr[r] = j + 1;
+                resid[r] = std::fabs(pr[j + 1]) / bnorm[r];
+                if (resid[r] <= tol[r] || !(beta > 0.)) {
+                    active[r] = 0;
+                    changed = true;
+                    if (resid[r] <= tol[r]) done[r] = 1;
+                }
+                nact += active[r];
+            }
+            j++;
+            stop = it >= maxit || ctl.expired();
+            if (nact == 0 || stop || j >= kmax) break;
+            if (changed) KEM_CK(c, hipMemcpyAsync(d_active, active.data(), sizeof(int) * nrhs, hipMemcpyHostToDevice, st));
+        }
+        // x_r += V_r y_r with H_r y_r = p_r (upper triangular, jr[r] columns)
+        for (int r = 0; r < nrhs; r++) {
+            const int m = jr[r];
+            const double* Hr = H.data() + (size_t)r * ldh * ldh;
+            double* yr = y.data() + (size_t)r * ldh;
+            const double* pr = p.data() + (size_t)r * ldh;
+            for (int q = m - 1; q >= 0; q--) {
+                double s = pr[q];
+                for (int u = q + 1; u < m; u++) s -= Hr[(size_t)u * ldh + q] * yr[u];
+                yr[q] = s / Hr[(size_t)q * ldh + q];
+            }
+            cnt[r] = m;
+        }
+        KEM_CK(c, hipMemcpyAsync(Y.p, y.data(), sizeof(double) * ldh * nrhs, hipMemcpyHostToDevice, st));
+        KEM_CK(c, hipMemcpyAsync(d_cnt, cnt.data(), sizeof(int) * nrhs, hipMemcpyHostToDevice, st));
+        KEM_CK(c, kem::launch_bupdate(st, V.p, vstride, Y.p, ldh, d_cnt, 1.0, n, nrhs, d_X));
+        KEM_CK(c, hipStreamSynchronize(st));
+        bool all = true;
+        for (int r = 0; r < nrhs; r++) all = all && done[r];
+        if (all) break;
+        if (stop) {
+            status = KEM_W_NOT_CONVERGED;
+            break;
+        }
+    }
+    conv_out.assign(done.begin(), done.end());
+    *iters = it;
+    return status;
+}
+
+// ||b_r - A64 x_r|| / ||b_r|| for all r; R receives the residual vectors
+int block_residual(kem_matrix* A, int nrhs, const double* d_B, const double* d_X, double* d_R, std::vector<double>& rel,
+                   void** ws, size_t* wsb)
+{
+    kem_ctx* c = A->ctx;
+    hipStream_t st = c->stream;
+    const int64_t n = A->n;
+    int rc = block_product(A, A->A64 == nullptr, nrhs, d_X, d_R, ws, wsb);
+    if (rc != KEM_OK) return rc;
+    KEM_CK(c, kem::launch_baxpby(st, 1.0, d_B, -1.0, d_R, d_R, n * nrhs));
+    KEM_CK(c, ensure_scal(c, 2 * (int64_t)nrhs));
+    KEM_CK(c, kem::launch_bnorm(st, d_R, n, nrhs, c->scal, 1));
+    KEM_CK(c, kem::launch_bnorm(st, d_B, n, nrhs, c->scal + nrhs, 1));
+    std::vector<double> v(2 * (size_t)nrhs);
+    KEM_CK(c, hipMemcpyAsync(v.data(), c->scal, sizeof(double) * 2 * nrhs, hipMemcpyDeviceToHost, st));
+    KEM_CK(c, hipStreamSynchronize(st));
+    rel.resize(nrhs);
+    for (int r = 0; r < nrhs; r++) rel[r] = v[r] / (v[nrhs + r] > 0. ? v[nrhs + r] : 1.);
+    return KEM_OK;
+}
+
+// one group of <= 64 right-hand sides (device B, X)
+int solve_group(kem_matrix* A, const kem_solve_opts& o, int nrhs, const double* d_B, double* d_X,
+                const SolveCtl& ctl, kem_solve_stats* s, std::vector<double>& rel)
+{
+    kem_ctx* c = A->ctx;
+    hipStream_t st = c->stream;
+    const int64_t n = A->n;
+    int its = 0;
+    std::vector<int> conv;
+    if (o.precision == KEM_FP64) {
+        if (!A->A64) return fail(c, KEM_E_STATE, "FP64 solve needs FP64 matrix storage");
+        std::vector<double> tol(nrhs, o.tolerance);
+        int status = gmres_batch(A, false, nrhs, d_B, d_X, tol, o.max_iterations, o.restart, ctl, &its, rel, conv);
+        if (status < 0) return status;
+        s->iterations = std::max(s->iterations, its);
+        return status;
+    }
+    if (!A->A32) return fail(c, KEM_E_STATE, "mixed solve needs FP32 matrix storage");
+    if (!A->A64) return fail(c, KEM_E_STATE, "mixed solve needs FP64 matrix storage for the residual");
+    DVec R, D;
+    DBuf ws;
+    KEM_CK(c, dalloc(&R.p, n * nrhs));
+    KEM_CK(c, dalloc(&D.p, n * nrhs));
+    int status = KEM_OK, total = 0, refinements = 0;
+    for (int ref = 0; ref < 50; ref++) {
+        int rc = block_residual(A, nrhs, d_B, d_X, R.p, rel, &ws.p, &ws.bytes);
+        if (rc != KEM_OK) return rc;
+        double worst = 0.;
+        std::vector<double> tol(nrhs);
+        for (int r = 0; r < nrhs; r++) {
+            worst = std::max(worst, rel[r]);
+            // inner tolerance relative to the residual right-hand side; converged ones are skipped
+            tol[r] = rel[r] <= o.tolerance ? 2.0
+                                            : std::min(0.5, std::max(o.inner_tolerance, 0.5 * o.tolerance / rel[r]));
+        }
+        if (worst <= o.tolerance) break;
+        if (total >= o.max_iterations || ctl.expired()) {
+            status = KEM_W_NOT_CONVERGED;
+            break;
+        }
+        KEM_CK(c, hipMemsetAsync(D.p, 0, sizeof(double) * n * nrhs, st));
+        std::vector<double> inner;
+        rc = gmres_batch(A, true, nrhs, R.p, D.p, tol, o.max_iterations - total, 0, ctl, &its, inner, conv);
+        if (rc < 0) return rc;
+        total += its;
+        refinements++;
+        KEM_CK(c, kem::launch_baxpby(st, 1.0, d_X, 1.0, D.p, d_X, n * nrhs));
+    }
+    s->iterations = std::max(s->iterations, total);
+    s->refinements = std::max(s->refinements, refinements);
+    for (double v : rel)
+        if (v > o.tolerance) status = KEM_W_NOT_CONVERGED;
+    return status;
+}
+
+}  // namespace
+
+int kem_matrix_multiply_batch(kem_matrix* A, int precision, int64_t nrhs, const double* X, double* Y)
+{
+    if (!A || nrhs < 0 || (nrhs > 0 && (!X || !Y))) return KEM_E_ARG;
+    kem_ctx* c = A->ctx;
+    if (precision != KEM_FP64 && precision != KEM_MIXED) return fail(c, KEM_E_ARG, "unknown precision mode");
+    const bool use32 = precision == KEM_MIXED;
+    if (use32 && !A->A32) return fail(c, KEM_E_STATE, "FP32 product needs FP32 matrix storage");
+    if (!use32 && !A->A64) return fail(c, KEM_E_STATE, "FP64 product needs FP64 matrix storage");
+    if (nrhs == 0) return KEM_OK;
+    KEM_CK(c, hipSetDevice(c->device));
+    const int64_t n = A->n;
+    const int g = (int)std::min<int64_t>(nrhs, 64);
+    DVec dX, dY;
+    DBuf ws;
+    KEM_CK(c, dalloc(&dX.p, n * g));
+    KEM_CK(c, dalloc(&dY.p, n * g));
+    for (int64_t r0 = 0; r0 < nrhs; r0 += 64) {
+        const int k = (int)std::min<int64_t>(64, nrhs - r0);
+        KEM_CK(c, hipMemcpyAsync(dX.p, X + r0 * n, sizeof(double) * n * k, hipMemcpyHostToDevice, c->stream));
+        int rc = block_product(A, use32, k, dX.p, dY.p, &ws.p, &ws.bytes);
+        if (rc != KEM_OK) return rc;
+        KEM_CK(c, hipMemcpyAsync(Y + r0 * n, dY.p, sizeof(double) * n * k, hipMemcpyDeviceToHost, c->stream));
+    }
+    KEM_CK(c, hipStreamSynchronize(c->stream));
+    return KEM_OK;
+}
+
+int kem_solve_batch(kem_matrix* A, const kem_solve_opts* opts, int64_t nrhs, const double* B, double* X,
+                    double* residuals, kem_solve_stats* stats)
+{
+    if (!A || nrhs < 0 || (nrhs > 0 && (!B || !X))) return KEM_E_ARG;
+    kem_ctx* c = A->ctx;
+    kem_solve_opts o;
+    kem_solve_opts_default(&o);
+    if (opts) o = *opts;
+    if (o.method != KEM_SOLVER_GMRES) return fail(c, KEM_E_ARG, "batched solve supports GMRES only");
+    if (o.max_iterations <= 0) return fail(c, KEM_E_ARG, "max_iterations must be positive");
+    if (!(o.tolerance > 0.)) return fail(c, KEM_E_ARG, "tolerance must be positive");
+    if (o.precision != KEM_FP64 && o.precision != KEM_MIXED) return fail(c, KEM_E_ARG, "unknown precision mode");
+    kem_solve_stats s{};
+    if (nrhs == 0) {
+        if (stats) *stats = s;
+        return KEM_OK;
+    }
+    KEM_CK(c, hipSetDevice(c->device));
+    const int64_t n = A->n;
+    SolveCtl ctl;
+    ctl.t0 = now_s();
+    ctl.limit = o.time_limit_s;
+    const int g = (int)std::min<int64_t>(nrhs, 64);
+    DVec dB, dX, dT;
+    KEM_CK(c, dalloc(&dB.p, n * g));
+    KEM_CK(c, dalloc(&dX.p, n * g));
+    int status = KEM_OK;
+    double worst = 0., worst_true = 0.;
+    for (int64_t r0 = 0; r0 < nrhs; r0 += 64) {
+        const int k = (int)std::min<int64_t>(64, nrhs - r0);
+        KEM_CK(c, hipMemcpyAsync(dB.p, B + r0 * n, sizeof(double) * n * k, hipMemcpyHostToDevice, c->stream));
+        KEM_CK(c, hipMemcpyAsync(dX.p, X + r0 * n, sizeof(double) * n * k, hipMemcpyHostToDevice, c->stream));
+        std::vector<double> rel;
+        int rc = solve_group(A, o, k, dB.p, dX.p, ctl, &s, rel);
+        if (rc < 0) return rc;
+        if (rc == KEM_W_NOT_CONVERGED) status = rc;
+        KEM_CK(c, hipMemcpyAsync(X + r0 * n, dX.p, sizeof(double) * n * k, hipMemcpyDeviceToHost, c->stream));
+        if (A->A64) {
+            if (!dT.p) KEM_CK(c, dalloc(&dT.p, n * g));
+            DBuf ws;
+            std::vector<double> tr;
+            rc = block_residual(A, k, dB.p, dX.p, dT.p, tr, &ws.p, &ws.bytes);
+            if (rc != KEM_OK) return rc;
+            for (int r = 0; r < k; r++) {
+                worst_true = std::max(worst_true, tr[r]);
+                if (residuals) residuals[r0 + r] = tr[r];
+            }
+        } else if (residuals) {
+            for (int r = 0; r < k; r++) residuals[r0 + r] = rel[r];
+        }
+        for (double v : rel) worst = std::max(worst, v);
+    }
+    KEM_CK(c, hipStreamSynchronize(c->stream));
+    s.residual = worst;
+    s.true_residual = worst_true;
     s.seconds = now_s() - ctl.t0;
     if (stats) *stats = s;
     return status;

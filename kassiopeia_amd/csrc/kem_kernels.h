@@ -62,8 +62,31 @@ hipError_t launch_div_dev(hipStream_t st, const double* d, const double* x, doub
 // z = a x + b y
 hipError_t launch_axpby(hipStream_t st, double a, const double* x, double b, const double* y, double* z, int64_t n);
 
+// Gauss-Jordan without pivoting on M = [A | b] (column-major, n x (n+1), overwritten);
+// f: scratch of n doubles; x: n doubles
+hipError_t launch_gauss_jordan(hipStream_t st, double* M, int64_t n, double* f, double* x);
+
 hipError_t launch_robin_hood(hipStream_t st, const double* A, const double* b, double* x, double* bi, int64_t n,
                              int64_t steps, double* rnorm);
+
+// ---- multi-right-hand-side kernels (kem_batch.hip)
+// Workspace of launch_bmm for nrhs <= 64 right-hand sides (FP32 MFMA path or FP64 path).
+size_t bmm_workspace_bytes(int64_t n, int nrhs, bool fp32);
+// Y[c*ldy + i] = sum_j A(i, j) X(j, c) for c < nrhs <= 64: A32 != null selects the FP32 MFMA
+// product (X rounded to FP32), else the FP64 product with A64.  x_is_panel: X already is the
+// row-major panel (FP32: n x (nrhs <= 32 ? 32 : 64) floats; FP64: n x (nrhs <= 16 ? 16 : 64)).
+hipError_t launch_bmm(hipStream_t st, const float* A32, const double* A64, int64_t n, const double* X, int64_t ldx,
+                      int nrhs, double* Y, int64_t ldy, void* ws, size_t ws_bytes, bool x_is_panel);
+// batched Krylov-basis operations, V[(r*(kmax+1) + q)*n + i] with vstride = (kmax+1)*n
+hipError_t launch_bproj(hipStream_t st, const double* V, int64_t vstride, const double* W, int64_t n, int nq,
+                        int nrhs, double* h, int ldh);
+hipError_t launch_bupdate(hipStream_t st, const double* V, int64_t vstride, const double* h, int ldh, const int* cnt,
+                          double sign, int64_t n, int nrhs, double* Y);
+hipError_t launch_bnorm(hipStream_t st, const double* W, int64_t n, int nrhs, double* nrm, int stride);
+hipError_t launch_bnormalize(hipStream_t st, const double* W, const double* nrm, int stride, const int* active,
+                             int64_t n, int nrhs, double* V, int64_t vstride, int q, int kp, float* Vt32,
+                             double* Vt64);
+hipError_t launch_baxpby(hipStream_t st, double a, const double* X, double b, const double* Y, double* Z, int64_t len);
 
 }  // namespace kem
 

@@ -6,6 +6,7 @@
 // rule.  Dense linear algebra is HBM-bound streaming over column-major storage.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 
 #include "kem_device.hpp"
@@ -491,6 +492,39 @@ __global__ __launch_bounds__(kRHBlock) void k_robin_hood(const double* __restric
     }
 }
 
+// ------------------------------------------------------------------ Gauss-Jordan
+// KGaussianElimination::Solve (LinearAlgebra/Solvers/include/KGaussianElimination.hh:24-58):
+// no pivoting; for each column col every row != col of the augmented matrix [A | b] becomes
+// a[row][:] - (a[row][col] / a[col][col]) * a[col][:] over all n+1 columns; then
+// x[row] = a[row][n] / a[row][row].  Within one column step the rows are independent (the
+// pivot row is not modified), so the step is one factor kernel and one update kernel over the
+// whole matrix.  Column-major M (entry (row, j) at j*n + row), lanes along rows.  Built
+// without FMA contraction so each entry sees the reference's roundings.
+__global__ __launch_bounds__(kBlock) void k_gj_factors(const double* __restrict__ M, int64_t n, int64_t col,
+                                                       double* __restrict__ f)
+{
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    f[i] = (i == col) ? 0. : M[col * n + i] / M[col * n + col];
+}
+
+__global__ __launch_bounds__(kBlock) void k_gj_update(double* __restrict__ M, int64_t n, int64_t col,
+                                                      const double* __restrict__ f)
+{
+    _Pragma("clang fp contract(off)");
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n || i == col) return;
+    const double fi = f[i];
+    for (int64_t j = blockIdx.y; j <= n; j += gridDim.y) M[j * n + i] = M[j * n + i] - fi * M[j * n + col];
+}
+
+__global__ __launch_bounds__(kBlock) void k_gj_solution(const double* __restrict__ M, int64_t n,
+                                                        double* __restrict__ x)
+{
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) x[i] = M[n * n + i] / M[i * n + i];
+}
+
 // ------------------------------------------------------------------ regime census
 // Counts the (point, element) pairs of a field evaluation by quadrature regime with the field
 // kernels' own classification (numeric policy; mixed = 1: the FP32 ladder beyond distance
@@ -705,6 +739,19 @@ hipError_t launch_div_dev(hipStream_t st, const double* d, const double* x, doub
 hipError_t launch_axpby(hipStream_t st, double a, const double* x, double b, const double* y, double* z, int64_t n)
 {
     k_axpby<<<nblocks(n), kBlock, 0, st>>>(a, x, b, y, z, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_gauss_jordan(hipStream_t st, double* M, int64_t n, double* f, double* x)
+{
+    if (n <= 0) return hipSuccess;
+    const unsigned gx = nblocks(n);
+    const unsigned gy = (unsigned)std::min<int64_t>(n + 1, 1024);
+    for (int64_t col = 0; col < n; col++) {
+        k_gj_factors<<<gx, kBlock, 0, st>>>(M, n, col, f);
+        k_gj_update<<<dim3(gx, gy), kBlock, 0, st>>>(M, n, col, f);
+    }
+    k_gj_solution<<<gx, kBlock, 0, st>>>(M, n, x);
     return hipGetLastError();
 }
 

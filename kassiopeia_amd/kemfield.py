@@ -177,6 +177,35 @@ class KBoundaryIntegralMatrix:
         self.ctx.check(self.ctx.lib.kem_matrix_multiply(self.h, 0, _ptr(x), _ptr(y)))
         return y
 
+    def MultiplyBatch(self, X, precision="fp64"):
+        """Y = A X for the columns of X (n x k); precision "mixed" uses the FP32 matrix (MFMA)."""
+        X = np.asfortranarray(np.asarray(X, dtype=np.float64).reshape(self.n, -1))
+        Y = np.zeros_like(X, order="F")
+        prec = {"fp64": _lib.KEM_FP64, "mixed": _lib.KEM_MIXED}[precision]
+        self.ctx.check(self.ctx.lib.kem_matrix_multiply_batch(self.h, prec, X.shape[1], _ptr(X), _ptr(Y)))
+        return Y
+
+    def solve_batch(self, B, X0=None, tolerance=1e-8, max_iterations=1 << 30, restart=0, precision="fp64",
+                    time_limit_s=0.0, inner_tolerance=1e-5):
+        """Solve A X = B for the columns of B (n x k) by lockstep GMRES; returns (X, residuals,
+        stats, rc) with the per-column relative residuals ||b - A x|| / ||b||."""
+        o = _lib.SolveOpts()
+        self.ctx.lib.kem_solve_opts_default(ctypes.byref(o))
+        o.method = _lib.KEM_SOLVER_GMRES
+        o.precision = {"fp64": _lib.KEM_FP64, "mixed": _lib.KEM_MIXED}[precision]
+        o.tolerance = tolerance
+        o.max_iterations = int(min(max_iterations, (1 << 31) - 1))
+        o.restart = restart
+        o.time_limit_s = time_limit_s
+        o.inner_tolerance = inner_tolerance
+        B = np.asfortranarray(np.asarray(B, dtype=np.float64).reshape(self.n, -1))
+        X = np.zeros_like(B, order="F") if X0 is None else np.asfortranarray(np.array(X0, dtype=np.float64))
+        res = np.zeros(B.shape[1])
+        st = _lib.SolveStats()
+        rc = self.ctx.check(self.ctx.lib.kem_solve_batch(self.h, ctypes.byref(o), B.shape[1], _ptr(B), _ptr(X),
+                                                         _ptr(res), ctypes.byref(st)))
+        return X, res, st, rc
+
     def MultiplyTranspose(self, x):
         x = _f64(x)
         y = np.zeros(self.n)
@@ -187,7 +216,8 @@ class KBoundaryIntegralMatrix:
               check_interval=0, precision="fp64", time_limit_s=0.0, inner_tolerance=1e-5):
         o = _lib.SolveOpts()
         self.ctx.lib.kem_solve_opts_default(ctypes.byref(o))
-        methods = {"gmres": _lib.KEM_SOLVER_GMRES, "robin_hood": _lib.KEM_SOLVER_ROBIN_HOOD}
+        methods = {"gmres": _lib.KEM_SOLVER_GMRES, "robin_hood": _lib.KEM_SOLVER_ROBIN_HOOD,
+                   "gauss": _lib.KEM_SOLVER_GAUSS}
         if method not in methods:
             raise KEMSimpleException(f"unknown solver '{method}'")
         o.method = methods[method]
@@ -240,6 +270,13 @@ class KRobinHoodChargeDensitySolver:
 
     def solve(self, matrix, b):
         return matrix.solve(b, **self.kw)
+
+
+class KGaussianEliminationChargeDensitySolver:
+    """Direct solve (gaussian_elimination_bem_solver; KGaussianEliminationChargeDensitySolver.cc:29-46)."""
+
+    def solve(self, matrix, b):
+        return matrix.solve(b, method="gauss")
 
 
 class KIntegratingElectrostaticFieldSolver:

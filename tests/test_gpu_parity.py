@@ -210,6 +210,23 @@ def test_robin_hood_matches_oracle(ctx, oracle, cube6):
     assert st.residual <= 1e-8
 
 
+def test_gauss_jordan_matches_oracle(ctx, oracle, cube6):
+    """KGaussianElimination (no pivoting) on the device matrix: the oracle's Gauss-Jordan on the
+    same matrix entries gives the same x to rounding, and x solves the system."""
+    from kassiopeia_amd import kemfield as K
+
+    m, Aref = cube6
+    cont = K.KSurfaceContainer(ctx, m)
+    A = K.KBoundaryIntegralMatrix(cont)
+    b = cont.rhs()
+    x, st, rc = K.KGaussianEliminationChargeDensitySolver().solve(A, b)
+    assert rc == 0
+    xo = oracle.gauss(A.to_numpy(), b)
+    assert relerr(x, xo) < 1e-14
+    assert np.linalg.norm(Aref @ x - b) / np.linalg.norm(b) < 1e-12
+    assert st.true_residual < 1e-12
+
+
 def test_mixed_solve_refines_to_fp64(ctx, oracle, cube6):
     from kassiopeia_amd import kemfield as K
 
