@@ -9,12 +9,17 @@ residual in FP64 and with FP32-matrix inner solves + FP64 refinement, then poten
 at 1,000,000 points uniform in volume in the annular gap 0.5 < r < 1.0, |z| < 2.
 
 A step is one field-evaluation pass over the 1 M points with the FP64 charge densities
-(inputs resident in HBM); `value` is point-element pairs per second of that pass.  The solve
-(assembly + GMRES) is timed separately and reported beside it.
+(inputs resident in HBM); `value` is point-element pairs per second of that pass (the
+mixed-precision pass when its error against the FP64 pass is within 1e-6).  The solve
+(assembly + GMRES) is timed separately and reported beside it, as is the multi-right-hand-side
+case on the same mesh: 16 axial ring groups, 64 voltage sets U[-1000, 0] V, lockstep GMRES
+on the FP32 matrix (MFMA block products) with FP64 refinement, and the field of those 64
+solutions over 4 M points in uneven batches (sizes log-uniform in 1k-256k) in one pass.
 
 Prints one JSON line.  --gpus N > 1 is refused: the library runs on one GPU.
 """
 import argparse
+import ctypes
 import json
 import os
 import sys
@@ -49,13 +54,14 @@ def main():
                     help="points of the bounded CPU-baseline sample (x 20,000 elements)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-solve", action="store_true")
+    ap.add_argument("--no-batch", action="store_true", help="skip the multi-right-hand-side section")
+    ap.add_argument("--batch-points", type=int, default=4_000_000)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported", file=sys.stderr)
         sys.exit(2)
 
-    import torch
-
+    from kassiopeia_amd import _lib
     from kassiopeia_amd import geometry as G
     from kassiopeia_amd import kemfield as K
 
@@ -65,26 +71,25 @@ def main():
     log(f"mesh {N} triangles, {M} points")
 
     ctx = K.Context(0)
-    # a dedicated (default-priority) stream shared by torch and the library, so that the HIP
-    # events below bracket exactly the library's kernels
-    stream = torch.cuda.Stream()
-    torch.cuda.set_stream(stream)
-    ctx.set_stream(stream.cuda_stream)
+    lib = ctx.lib
     cont = K.KSurfaceContainer(ctx, mesh)
     b = cont.rhs()
 
+    def sync():
+        ctx.check(lib.kem_ctx_sync(ctx.h))
+
     out = {}
+    A = None
     # ---------------------------------------------------------------- solve
     if not args.no_solve:
         t0 = time.perf_counter()
         A = K.KBoundaryIntegralMatrix(cont, fp64=True, fp32=True)
-        torch.cuda.synchronize()
+        sync()
         t_asm = time.perf_counter() - t0
         x64, st64, _ = A.solve(b, tolerance=1e-10)
         xmx, stmx, _ = A.solve(b, tolerance=1e-10, precision="mixed")
         x64, st64, _ = A.solve(b, tolerance=1e-10)  # second run: warm
         xmx, stmx, _ = A.solve(b, tolerance=1e-10, precision="mixed")
-        A.close()
         sig_err = float(np.linalg.norm(xmx - x64) / np.linalg.norm(x64))
         log(f"assembly {t_asm*1e3:.1f} ms; GMRES fp64 {st64.iterations} it {st64.seconds*1e3:.1f} ms "
             f"res {st64.true_residual:.2e}; mixed {stmx.iterations} it / {stmx.refinements} ref "
@@ -99,51 +104,41 @@ def main():
         sigma = np.random.default_rng(3).normal(size=N) * 1e-7
 
     # ---------------------------------------------------------------- field pass
-    dev = torch.device("cuda:0")
-    X = [torch.from_numpy(np.ascontiguousarray(pts[:, k])).to(dev) for k in range(3)]
-    S = torch.from_numpy(sigma).to(dev)
-    O = [torch.empty(M, dtype=torch.float64, device=dev) for _ in range(4)]
-    lib = ctx.lib
-
-    def ptr(t):
-        return t.data_ptr()
+    X = [K.DeviceArray(ctx, pts[:, k]) for k in range(3)]
+    S = K.DeviceArray(ctx, sigma)
+    O = [K.DeviceArray.empty(ctx, M) for _ in range(4)]
+    Om = [K.DeviceArray.empty(ctx, M) for _ in range(4)]
 
     def run(precision, outs):
-        rc = lib.kem_field_eval_device(ctx.h, cont.h, ptr(S), M, ptr(X[0]), ptr(X[1]), ptr(X[2]), precision,
-                                       ptr(outs[0]), ptr(outs[1]), ptr(outs[2]), ptr(outs[3]))
-        if rc != 0:
-            raise RuntimeError(lib.kem_last_error(ctx.h).decode())
-
-    import ctypes
+        ctx.check(lib.kem_field_eval_device(ctx.h, cont.h, S.ptr, M, X[0].ptr, X[1].ptr, X[2].ptr, precision,
+                                            outs[0].ptr, outs[1].ptr, outs[2].ptr, outs[3].ptr))
 
     def census(precision):
         c = (ctypes.c_int64 * 11)()
-        ctx.check(lib.kem_field_census(ctx.h, cont.h, M, ptr(X[0]), ptr(X[1]), ptr(X[2]), precision, c))
+        ctx.check(lib.kem_field_census(ctx.h, cont.h, M, X[0].ptr, X[1].ptr, X[2].ptr, precision, c))
         return np.array(c[:], dtype=np.float64)
 
     def timed(precision, outs):
-        torch.cuda.synchronize()
+        """(pass ms, far-kernel ms): steps back-to-back on the library's stream between two
+        stream synchronisations; the far kernel from HIP events the library records around it."""
         for _ in range(args.warmup):
             run(precision, outs)
-        torch.cuda.synchronize()
+        sync()
         ctx.check(lib.kem_ctx_set_kernel_timing(ctx.h, 1))
-        ev0 = torch.cuda.Event(enable_timing=True)
-        ev1 = torch.cuda.Event(enable_timing=True)
-        ev0.record(stream)
+        t0 = time.perf_counter()
         for _ in range(args.steps):
             run(precision, outs)
-        ev1.record(stream)
-        torch.cuda.synchronize()
+        sync()
+        t = time.perf_counter() - t0
         kt, kn = ctypes.c_double(), ctypes.c_int()
         ctx.check(lib.kem_ctx_kernel_time(ctx.h, ctypes.byref(kt), ctypes.byref(kn)))
         ctx.check(lib.kem_ctx_set_kernel_timing(ctx.h, 0))
-        return ev0.elapsed_time(ev1) / args.steps, kt.value / max(kn.value, 1)
+        return t * 1e3 / args.steps, kt.value / max(kn.value, 1)
 
     pairs = float(N) * M
-    c64 = census(0)
-    cmx = census(1)
+    c64 = census(_lib.KEM_FP64)
+    cmx = census(_lib.KEM_MIXED)
     flops64 = float((c64 * FLOPS_PER_PAIR).sum())
-    flopsmx = float((cmx * FLOPS_PER_PAIR).sum())
     # the far-field kernel evaluates every pair but the closed-form ones
     far = np.ones(11)
     far[[3, 7, 10]] = 0.
@@ -151,22 +146,24 @@ def main():
     farflopsmx = float((cmx * FLOPS_PER_PAIR * far).sum())
     log(f"regime census fp64 (tri 7/12/33/closed): {c64[:4].astype(int).tolist()}; "
         f"mixed: {cmx[:4].astype(int).tolist()}")
-    ms64, kms64 = timed(0, O)
-    Om = [torch.empty(M, dtype=torch.float64, device=dev) for _ in range(4)]
-    msmx, kmsmx = timed(1, Om)
+    ms64, kms64 = timed(_lib.KEM_FP64, O)
+    msmx, kmsmx = timed(_lib.KEM_MIXED, Om)
     # error of the mixed result against the FP64 path: norm-wise over the whole point set, the
     # largest deviation relative to the RMS field, and the potential relative to the electrode
     # voltage scale; the per-point relative field error is reported beside them (it is
     # amplified where |E| nearly vanishes)
-    dE = torch.stack([Om[k] - O[k] for k in (1, 2, 3)], 1)
-    E64 = torch.stack([O[k] for k in (1, 2, 3)], 1)
-    Ed, En = dE.norm(dim=1), E64.norm(dim=1)
-    err_E_l2 = float(dE.norm() / E64.norm())
-    err_E_rms = float(Ed.max() / E64.pow(2).sum(1).mean().sqrt())
+    phi = O[0].get()
+    E = np.stack([O[k].get() for k in (1, 2, 3)], 1)
+    phim = Om[0].get()
+    Emx = np.stack([Om[k].get() for k in (1, 2, 3)], 1)
+    dE = Emx - E
+    Ed, En = np.linalg.norm(dE, axis=1), np.linalg.norm(E, axis=1)
+    err_E_l2 = float(np.linalg.norm(dE) / np.linalg.norm(E))
+    err_E_rms = float(Ed.max() / np.sqrt((E ** 2).sum(1).mean()))
     errE = Ed / En
     vscale = float(np.abs(mesh.bc_value).max()) or 1.0
-    err_p_max = float(((Om[0] - O[0]).abs() / vscale).max())
-    err_E_max, err_E_p99 = float(errE.max()), float(torch.quantile(errE[:200000], 0.99))
+    err_p_max = float((np.abs(phim - phi) / vscale).max())
+    err_E_max, err_E_p99 = float(errE.max()), float(np.quantile(errE, 0.99))
     log(f"field pass fp64 {ms64:.2f} ms (far kernel {kms64:.2f} ms, {pairs / ms64 * 1e3:.4e} pairs/s, "
         f"{flops64 / ms64 / 1e9:.2f} TFLOP/s); mixed {msmx:.2f} ms (far kernel {kmsmx:.2f} ms, "
         f"{pairs / msmx * 1e3:.4e} pairs/s), speed-up {ms64 / msmx:.2f}x; mixed vs fp64: "
@@ -185,8 +182,56 @@ def main():
                mixed_err_E_pointwise_max=err_E_max, mixed_err_E_pointwise_p99=err_E_p99,
                mixed_within_1e6=mixed_ok, fp64_pass_tflops=flops64 / ms64 / 1e9,
                fp64_far_kernel_frac=farflops64 / kms64 / 1e9 / PEAK_FP64_TFLOPS)
-    phi = O[0].cpu().numpy()
-    E = torch.stack(O[1:4], dim=1).cpu().numpy()
+    for arr in X + O + Om + [S]:
+        arr.close()
+
+    # ---------------------------------------------------------------- multi-RHS (64 voltage sets)
+    if not args.no_batch and A is not None:
+        groups = G.coax_ring_groups(mesh, 16)
+        V = np.random.default_rng(64).uniform(-1000.0, 0.0, size=(16, 64))
+        B = V[groups, :]
+        Xb, res, stb, rcb = A.solve_batch(B, tolerance=1e-10, precision="mixed")  # warm
+        Xb, res, stb, rcb = A.solve_batch(B, tolerance=1e-10, precision="mixed")
+        x1, st1, _ = A.solve(B[:, 0], tolerance=1e-10, precision="mixed")
+        log(f"64 voltage sets: lockstep mixed GMRES {stb.seconds*1e3:.1f} ms ({stb.iterations} it, "
+            f"{stb.refinements} ref, worst residual {res.max():.2e}); one set alone {st1.seconds*1e3:.1f} ms")
+        out.update(batch64_solve_ms=round(stb.seconds * 1e3, 3), batch64_iterations=stb.iterations,
+                   batch64_refinements=stb.refinements, batch64_worst_residual=float(res.max()),
+                   single_rhs_mixed_solve_ms=round(st1.seconds * 1e3, 3),
+                   batch64_speedup_vs_64_single=round(64 * st1.seconds / stb.seconds, 2))
+        # uneven batches: sizes log-uniform in [1k, 256k] until 4 M points, batch b uses set b % 64
+        rng = np.random.default_rng(4096)
+        sizes = []
+        while sum(sizes) < args.batch_points:
+            sizes.append(int(np.exp(rng.uniform(np.log(1000), np.log(256000)))))
+        sizes[-1] -= sum(sizes) - args.batch_points
+        Pb = G.points_annulus(args.batch_points, 0.5, 1.0, 2.0, seed=77)
+        sidx = np.repeat(np.arange(len(sizes)) % 64, sizes).astype(np.int32)
+        Sd = K.DeviceArray(ctx, np.asfortranarray(Xb).T)
+        Id = K.DeviceArray(ctx, sidx)
+        Xd = [K.DeviceArray(ctx, Pb[:, k]) for k in range(3)]
+        Od = [K.DeviceArray.empty(ctx, args.batch_points) for _ in range(4)]
+
+        def run_sets(prec):
+            ctx.check(lib.kem_field_eval_sets(ctx.h, cont.h, 64, Sd.ptr, Id.ptr, args.batch_points, Xd[0].ptr,
+                                              Xd[1].ptr, Xd[2].ptr, prec, Od[0].ptr, Od[1].ptr, Od[2].ptr,
+                                              Od[3].ptr))
+
+        run_sets(_lib.KEM_MIXED)
+        sync()
+        t0 = time.perf_counter()
+        run_sets(_lib.KEM_MIXED)
+        sync()
+        tb = time.perf_counter() - t0
+        bpairs = float(N) * args.batch_points
+        log(f"uneven batches: {len(sizes)} batches ({min(sizes)}..{max(sizes)} points), 64 sets, "
+            f"{args.batch_points} points: mixed {tb*1e3:.1f} ms = {bpairs / tb:.4e} pairs/s")
+        out.update(batches_count=len(sizes), batches_points=args.batch_points,
+                   batches_field_ms_mixed=round(tb * 1e3, 3), batches_pairs_per_s_mixed=bpairs / tb)
+        for arr in Xd + Od + [Sd, Id]:
+            arr.close()
+    if A is not None:
+        A.close()
 
     # ---------------------------------------------------------------- CPU baseline + parity
     cpu = None

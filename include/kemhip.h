@@ -98,6 +98,13 @@ int kem_ctx_set_mixed_fp64_ratio(kem_ctx* ctx, double ratio);
    queries); kem_ctx_kernel_time waits for them and returns the summed kernel time and the
    number of evaluations timed, then starts a new count. */
 int kem_ctx_set_kernel_timing(kem_ctx* ctx, int on);
+/* Device memory on the context's GPU for callers without an allocator of their own (no
+   reference counterpart): allocate / free (waits for the stream first), and copies that wait
+   for the stream to finish. */
+int kem_dev_alloc(kem_ctx* ctx, int64_t bytes, void** ptr);
+int kem_dev_free(kem_ctx* ctx, void* ptr);
+int kem_dev_upload(kem_ctx* ctx, void* dst, const void* src, int64_t bytes);
+int kem_dev_download(kem_ctx* ctx, void* dst, const void* src, int64_t bytes);
 int kem_ctx_kernel_time(kem_ctx* ctx, double* total_ms, int* launches);
 /* integrator policy by name: "numeric" | "default" | "analytic" | "rwg" | "reference"
    (KElectrostaticBoundaryIntegratorFactory::Make(name), .cc:78-88; unknown -> KEM_E_NAME) */
@@ -137,6 +144,14 @@ int kem_field_eval(kem_ctx* ctx, const kem_mesh* mesh, const double* sigma, int6
 int kem_field_eval_device(kem_ctx* ctx, const kem_mesh* mesh, const double* d_sigma, int64_t npts,
                           const double* d_x, const double* d_y, const double* d_z, int precision, double* d_phi,
                           double* d_ex, double* d_ey, double* d_ez);
+/* Field of several charge-density sets in one pass (right-hand sides of kem_solve_batch, or
+   uneven point batches each with its own solution): point i uses sigma[set[i]*n + j]
+   (device arrays: sigma n x nsets column-major, set[npts] in [0, nsets), nsets <= 64).  No
+   reference counterpart (one KIntegratingFieldSolver per solution there).  Validates the set
+   indices (waits for the stream), then runs asynchronously like kem_field_eval_device. */
+int kem_field_eval_sets(kem_ctx* ctx, const kem_mesh* mesh, int64_t nsets, const double* sigma, const int32_t* set,
+                        int64_t npts, const double* x, const double* y, const double* z, int precision, double* phi,
+                        double* ex, double* ey, double* ez);
 
 /* Instrumentation (no reference counterpart): number of (point, element) pairs of a field
    evaluation per quadrature regime of the numeric policy at the given precision (KEM_MIXED

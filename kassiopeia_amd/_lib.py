@@ -58,6 +58,10 @@ SIGNATURES = {
     "kem_ctx_set_stream": (_INT, [_P, _P]),
     "kem_ctx_stream": (_P, [_P]),
     "kem_ctx_set_mixed_fp64_ratio": (_INT, [_P, _D]),
+    "kem_dev_alloc": (_INT, [_P, _I64, ctypes.POINTER(_P)]),
+    "kem_dev_free": (_INT, [_P, _P]),
+    "kem_dev_upload": (_INT, [_P, _P, _P, _I64]),
+    "kem_dev_download": (_INT, [_P, _P, _P, _I64]),
     "kem_ctx_set_kernel_timing": (_INT, [_P, _INT]),
     "kem_ctx_kernel_time": (_INT, [_P, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)]),
     "kem_ctx_sync": (_INT, [_P]),
@@ -71,6 +75,7 @@ SIGNATURES = {
     "kem_element_eval": (_INT, [_P, _P, _I64, _P, _P, _P]),
     "kem_field_eval": (_INT, [_P, _P, _P, _I64, _P, _INT, _P, _P]),
     "kem_field_eval_device": (_INT, [_P, _P, _P, _I64, _P, _P, _P, _INT, _P, _P, _P, _P]),
+    "kem_field_eval_sets": (_INT, [_P, _P, _I64, _P, _P, _I64, _P, _P, _P, _INT, _P, _P, _P, _P]),
     "kem_field_census": (_INT, [_P, _P, _I64, _P, _P, _P, _INT, _P]),
     "kem_matrix_assemble": (_INT, [_P, _P, _INT, ctypes.POINTER(_P)]),
     "kem_matrix_destroy": (None, [_P]),

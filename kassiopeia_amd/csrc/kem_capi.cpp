@@ -212,6 +212,46 @@ int kem_ctx_kernel_time(kem_ctx* c, double* total_ms, int* launches)
     return KEM_OK;
 }
 
+int kem_dev_alloc(kem_ctx* c, int64_t bytes, void** p)
+{
+    if (!c || !p || bytes < 0) return KEM_E_ARG;
+    *p = nullptr;
+    if (bytes == 0) return KEM_OK;
+    KEM_CK(c, hipSetDevice(c->device));
+    KEM_CK(c, hipMalloc(p, (size_t)bytes));
+    return KEM_OK;
+}
+
+int kem_dev_free(kem_ctx* c, void* p)
+{
+    if (!c) return KEM_E_ARG;
+    if (!p) return KEM_OK;
+    KEM_CK(c, hipSetDevice(c->device));
+    KEM_CK(c, hipStreamSynchronize(c->stream));
+    KEM_CK(c, hipFree(p));
+    return KEM_OK;
+}
+
+int kem_dev_upload(kem_ctx* c, void* dst, const void* src, int64_t bytes)
+{
+    if (!c || (bytes > 0 && (!dst || !src)) || bytes < 0) return KEM_E_ARG;
+    if (bytes == 0) return KEM_OK;
+    KEM_CK(c, hipSetDevice(c->device));
+    KEM_CK(c, hipMemcpyAsync(dst, src, (size_t)bytes, hipMemcpyHostToDevice, c->stream));
+    KEM_CK(c, hipStreamSynchronize(c->stream));
+    return KEM_OK;
+}
+
+int kem_dev_download(kem_ctx* c, void* dst, const void* src, int64_t bytes)
+{
+    if (!c || (bytes > 0 && (!dst || !src)) || bytes < 0) return KEM_E_ARG;
+    if (bytes == 0) return KEM_OK;
+    KEM_CK(c, hipSetDevice(c->device));
+    KEM_CK(c, hipMemcpyAsync(dst, src, (size_t)bytes, hipMemcpyDeviceToHost, c->stream));
+    KEM_CK(c, hipStreamSynchronize(c->stream));
+    return KEM_OK;
+}
+
 int kem_ctx_sync(kem_ctx* c)
 {
     if (!c) return KEM_E_ARG;
@@ -360,16 +400,12 @@ int kem_element_eval(kem_ctx* c, const kem_mesh* m, int64_t np, const int64_t* e
     return KEM_OK;
 }
 
-int kem_field_eval_device(kem_ctx* c, const kem_mesh* m, const double* sigma, int64_t npts, const double* x,
-                          const double* y, const double* z, int precision, double* phi, double* ex, double* ey,
-                          double* ez)
+namespace {
+// field evaluation on device arrays; pset (nullable): per-point charge-density set
+int field_eval(kem_ctx* c, const kem_mesh* m, const double* sigma, const int32_t* pset, int64_t npts, const double* x,
+               const double* y, const double* z, int precision, double* phi, double* ex, double* ey, double* ez)
 {
-    if (!c || !m || !sigma || (npts > 0 && (!x || !y || !z))) return KEM_E_ARG;
-    if (m->ctx != c) return fail(c, KEM_E_STATE, "mesh belongs to another context");
-    if (precision != KEM_FP64 && precision != KEM_MIXED) return fail(c, KEM_E_ARG, "field: unknown precision mode");
     const float* n32 = precision == KEM_MIXED ? m->nodes32 : nullptr;
-    if (npts <= 0) return KEM_OK;
-    KEM_CK(c, hipSetDevice(c->device));
     hipEvent_t t0 = nullptr, t1 = nullptr;
     if (c->timing && c->ev_used < kem_ctx::kTimedMax) {
         t0 = c->ev[2 * c->ev_used];
@@ -388,13 +424,49 @@ int kem_field_eval_device(kem_ctx* c, const kem_mesh* m, const double* sigma, in
             KEM_CK(c, hipMalloc(&c->sort_ws, bytes));
             c->sort_ws_bytes = bytes;
         }
-        KEM_CK(c, kem::launch_field_sorted(c->stream, c->policy, m->geom, m->nodes, m->recs, n32, c->mixed_fp64_ratio, sigma, m->n,
-                                           m->clusters, m->order, x, y, z, npts, phi, ex, ey, ez, c->sort_ws, c->sort_ws_bytes, c->ws, c->ws_doubles, t0, t1));
+        KEM_CK(c, kem::launch_field_sorted(c->stream, c->policy, m->geom, m->nodes, m->recs, n32, c->mixed_fp64_ratio,
+                                           sigma, pset, m->n, m->clusters, m->order, x, y, z, npts, phi, ex, ey, ez,
+                                           c->sort_ws, c->sort_ws_bytes, c->ws, c->ws_doubles, t0, t1));
         return KEM_OK;
     }
-    KEM_CK(c, kem::launch_field(c->stream, c->policy, m->geom, m->nodes, m->recs, n32, c->mixed_fp64_ratio, sigma, m->n,
-                                m->clusters, m->order, x, y, z, npts, phi, ex, ey, ez, c->ws, c->ws_doubles, t0, t1));
+    KEM_CK(c, kem::launch_field(c->stream, c->policy, m->geom, m->nodes, m->recs, n32, c->mixed_fp64_ratio, sigma,
+                                pset, m->n, m->clusters, m->order, x, y, z, npts, phi, ex, ey, ez, c->ws,
+                                c->ws_doubles, t0, t1));
     return KEM_OK;
+}
+}  // namespace
+
+int kem_field_eval_device(kem_ctx* c, const kem_mesh* m, const double* sigma, int64_t npts, const double* x,
+                          const double* y, const double* z, int precision, double* phi, double* ex, double* ey,
+                          double* ez)
+{
+    if (!c || !m || !sigma || (npts > 0 && (!x || !y || !z))) return KEM_E_ARG;
+    if (m->ctx != c) return fail(c, KEM_E_STATE, "mesh belongs to another context");
+    if (precision != KEM_FP64 && precision != KEM_MIXED) return fail(c, KEM_E_ARG, "field: unknown precision mode");
+    if (npts <= 0) return KEM_OK;
+    KEM_CK(c, hipSetDevice(c->device));
+    return field_eval(c, m, sigma, nullptr, npts, x, y, z, precision, phi, ex, ey, ez);
+}
+
+int kem_field_eval_sets(kem_ctx* c, const kem_mesh* m, int64_t nsets, const double* sigma, const int32_t* set,
+                        int64_t npts, const double* x, const double* y, const double* z, int precision, double* phi,
+                        double* ex, double* ey, double* ez)
+{
+    if (!c || !m || !sigma || nsets <= 0 || (npts > 0 && (!x || !y || !z || !set))) return KEM_E_ARG;
+    if (m->ctx != c) return fail(c, KEM_E_STATE, "mesh belongs to another context");
+    if (precision != KEM_FP64 && precision != KEM_MIXED) return fail(c, KEM_E_ARG, "field: unknown precision mode");
+    if (nsets > (1 << 6)) return fail(c, KEM_E_ARG, "field: at most 64 charge-density sets per call");
+    if (npts <= 0) return KEM_OK;
+    KEM_CK(c, hipSetDevice(c->device));
+    KEM_CK(c, ensure_scal(c, 1));
+    int* bad = (int*)c->scal;
+    int hbad = 0;
+    KEM_CK(c, hipMemsetAsync(bad, 0, sizeof(int), c->stream));
+    KEM_CK(c, kem::launch_check_sets(c->stream, set, npts, nsets, bad));
+    KEM_CK(c, hipMemcpyAsync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    KEM_CK(c, hipStreamSynchronize(c->stream));
+    if (hbad) return fail(c, KEM_E_ARG, "field: set index outside [0, nsets)");
+    return field_eval(c, m, sigma, set, npts, x, y, z, precision, phi, ex, ey, ez);
 }
 
 int kem_field_census(kem_ctx* c, const kem_mesh* m, int64_t npts, const double* x, const double* y, const double* z,

@@ -28,8 +28,10 @@ constexpr int kBlock = 256;
 // to the same sums.  Keeping the closed forms out of this kernel keeps its register use low
 // enough for full occupancy.  With nsplit > 1 each blockIdx.y handles one contiguous element
 // range and writes its partial sums; k_field_combine adds the partials in range order.
+template <bool MULTI>
 __global__ __launch_bounds__(kBlock) void k_field_far(const double* __restrict__ geom, const double* __restrict__ nodes,
-                                                      const double* __restrict__ sigma, int64_t nelem,
+                                                      const double* __restrict__ sigma,
+                                                      const int32_t* __restrict__ pset, int64_t nelem,
                                                       int64_t per_split, const double* __restrict__ px,
                                                       const double* __restrict__ py, const double* __restrict__ pz,
                                                       int64_t npts, double* __restrict__ out0, double* __restrict__ out1,
@@ -40,6 +42,7 @@ __global__ __launch_bounds__(kBlock) void k_field_far(const double* __restrict__
     const bool live = i < npts;
     if (!live) i = npts - 1;
     const double x = px[i], y = py[i], z = pz[i];
+    const double* __restrict__ sg = MULTI ? sigma + (int64_t)pset[i] * nelem : sigma;
     const int64_t j0 = (int64_t)blockIdx.y * per_split;
     int64_t j1 = j0 + per_split;
     if (j1 > nelem) j1 = nelem;
@@ -50,7 +53,7 @@ __global__ __launch_bounds__(kBlock) void k_field_far(const double* __restrict__
         const double r2 = dev::dist2(g, x, y, z);
         if (!dev::is_near2(g, r2, 0)) {
             const F4 v = dev::element_far(g, nodes + j * KEM_NODE_STRIDE, (int)g[KG_TYPE], r2, x, y, z);
-            const double sj = sigma[j];
+            const double sj = sg[j];
             dev::kahan_add(v.x, sj, s0, c0);
             dev::kahan_add(v.y, sj, s1, c1);
             dev::kahan_add(v.z, sj, s2, c2);
@@ -70,10 +73,12 @@ __global__ __launch_bounds__(kBlock) void k_field_far(const double* __restrict__
 // fp64_ratio run with FP32 node sums (dev::element_far32, accuracy-targeted rule ladder); the
 // band between the closed-form limit and fp64_ratio (empty by default) keeps the FP64 rules.
 // All sums over elements are FP64.
+template <bool MULTI>
 __global__ __launch_bounds__(kBlock) void k_field_far32(const double* __restrict__ geom,
                                                         const double* __restrict__ nodes,
                                                         const float* __restrict__ nodes32, double fp64_ratio,
-                                                        const double* __restrict__ sigma, int64_t nelem,
+                                                        const double* __restrict__ sigma,
+                                                        const int32_t* __restrict__ pset, int64_t nelem,
                                                         int64_t per_split, const double* __restrict__ px,
                                                         const double* __restrict__ py, const double* __restrict__ pz,
                                                         int64_t npts, double* __restrict__ out0,
@@ -84,6 +89,7 @@ __global__ __launch_bounds__(kBlock) void k_field_far32(const double* __restrict
     const bool live = i < npts;
     if (!live) i = npts - 1;
     const double x = px[i], y = py[i], z = pz[i];
+    const double* __restrict__ sg = MULTI ? sigma + (int64_t)pset[i] * nelem : sigma;
     const int64_t j0 = (int64_t)blockIdx.y * per_split;
     int64_t j1 = j0 + per_split;
     if (j1 > nelem) j1 = nelem;
@@ -98,14 +104,14 @@ __global__ __launch_bounds__(kBlock) void k_field_far32(const double* __restrict
             const float dx = (float)(x - g[KG_CX]), dy = (float)(y - g[KG_CY]), dz = (float)(z - g[KG_CZ]);
             float s0, s1, s2, s3;
             dev::element_far32(g, nodes32 + j * KEM_NODE_STRIDE, type, r2, dx, dy, dz, s0, s1, s2, s3);
-            const double wj = g[KG_PREF] * sigma[j];
+            const double wj = g[KG_PREF] * sg[j];
             a0 += (double)s0 * wj;
             a1 += (double)s1 * wj;
             a2 += (double)s2 * wj;
             a3 += (double)s3 * wj;
         } else {
             const F4 v = dev::element_far(g, nodes + j * KEM_NODE_STRIDE, type, r2, x, y, z);
-            const double sj = sigma[j];
+            const double sj = sg[j];
             a0 += v.x * sj;
             a1 += v.y * sj;
             a2 += v.z * sj;
@@ -124,10 +130,11 @@ __global__ __launch_bounds__(kBlock) void k_field_far32(const double* __restrict
 // Near pass over all elements of a range: closed-form pairs added to the far pass's sums
 // (numeric policy, accumulate = 1), or every pair from zero (rwg policy; PC = 1: the analytic
 // and reference policies, from the raw shape records).
-template <int PC>
+template <int PC, bool MULTI>
 __global__ __launch_bounds__(kBlock) void k_field_near(const double* __restrict__ geom,
                                                        const double* __restrict__ recs,
-                                                       const double* __restrict__ sigma, int64_t nelem,
+                                                       const double* __restrict__ sigma,
+                                                       const int32_t* __restrict__ pset, int64_t nelem,
                                                        int64_t per_split, const double* __restrict__ px,
                                                        const double* __restrict__ py, const double* __restrict__ pz,
                                                        int64_t npts, int policy, int accumulate,
@@ -139,6 +146,7 @@ __global__ __launch_bounds__(kBlock) void k_field_near(const double* __restrict_
     const bool live = i < npts;
     if (!live) i = npts - 1;
     const double x = px[i], y = py[i], z = pz[i];
+    const double* __restrict__ sg = MULTI ? sigma + (int64_t)pset[i] * nelem : sigma;
     const int64_t j0 = (int64_t)blockIdx.y * per_split;
     int64_t j1 = j0 + per_split;
     if (j1 > nelem) j1 = nelem;
@@ -156,7 +164,7 @@ __global__ __launch_bounds__(kBlock) void k_field_near(const double* __restrict_
         if (PC == 1 || dev::is_near2(g, dev::dist2(g, x, y, z), policy)) {
             const F4 v = PC == 1 ? dev::element_policy(g, recs + j * KEM_REC, policy, x, y, z)
                                  : dev::element_near(g, (int)g[KG_TYPE], x, y, z);
-            const double sj = sigma[j];
+            const double sj = sg[j];
             dev::kahan_add(v.x, sj, s0, c0);
             dev::kahan_add(v.y, sj, s1, c1);
             dev::kahan_add(v.z, sj, s2, c2);
@@ -187,8 +195,10 @@ __device__ __forceinline__ double wave_max(double v)
 // sphere of near distances cannot reach it is skipped as a whole.  The pairs evaluated are
 // exactly the near set (the same dist2 test as the far pass); culling only removes clusters
 // proven to contain none.  Adds to the final sums (after the far pass and any combine).
+template <bool MULTI>
 __global__ __launch_bounds__(kBlock) void k_field_near_culled(const double* __restrict__ geom,
                                                               const double* __restrict__ sigma,
+                                                              const int32_t* __restrict__ pset,
                                                               const double* __restrict__ clusters, int64_t ncl,
                                                               const int32_t* __restrict__ order, int64_t nelem,
                                                               const double* __restrict__ px,
@@ -201,6 +211,7 @@ __global__ __launch_bounds__(kBlock) void k_field_near_culled(const double* __re
     const bool live = i < npts;
     if (!live) i = npts - 1;
     const double x = px[i], y = py[i], z = pz[i];
+    const double* __restrict__ sg = MULTI ? sigma + (int64_t)pset[i] * nelem : sigma;
     // bounding sphere of the wavefront's points
     const double lx = wave_min(x), ly = wave_min(y), lz = wave_min(z);
     const double hx = wave_max(x), hy = wave_max(y), hz = wave_max(z);
@@ -220,7 +231,7 @@ __global__ __launch_bounds__(kBlock) void k_field_near_culled(const double* __re
             const double* g = geom + j * KG_STRIDE;
             if (dev::is_near2(g, dev::dist2(g, x, y, z), 0)) {
                 const F4 v = dev::element_near(g, (int)g[KG_TYPE], x, y, z);
-                const double sj = sigma[j];
+                const double sj = sg[j];
                 dev::kahan_add(v.x, sj, s0, c0);
                 dev::kahan_add(v.y, sj, s1, c1);
                 dev::kahan_add(v.z, sj, s2, c2);
@@ -525,6 +536,15 @@ __global__ __launch_bounds__(kBlock) void k_gj_solution(const double* __restrict
     if (i < n) x[i] = M[n * n + i] / M[i * n + i];
 }
 
+// ------------------------------------------------------------------ set-index validation
+// *bad = 1 if any pset[i] lies outside [0, nsets) (benign race: every writer stores 1)
+__global__ __launch_bounds__(kBlock) void k_check_sets(const int32_t* __restrict__ pset, int64_t n, int64_t nsets,
+                                                       int* __restrict__ bad)
+{
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i < n && (pset[i] < 0 || pset[i] >= nsets)) *bad = 1;
+}
+
 // ------------------------------------------------------------------ regime census
 // Counts the (point, element) pairs of a field evaluation by quadrature regime with the field
 // kernels' own classification (numeric policy; mixed = 1: the FP32 ladder beyond distance
@@ -576,14 +596,14 @@ inline unsigned nblocks(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock
 }  // namespace
 
 // ====================================================================== launchers
-hipError_t launch_field(hipStream_t st, int policy, const double* geom, const double* nodes, const double* recs,
-                        const float* nodes32,
-                        double fp64_ratio, const double* sigma, int64_t nelem, const double* clusters,
-                        const int32_t* order, const double* px, const double* py, const double* pz, int64_t npts,
-                        double* phi, double* ex, double* ey, double* ez, double* ws, int64_t ws_doubles,
-                        hipEvent_t far_begin, hipEvent_t far_end)
+namespace {
+template <bool MULTI>
+void field_passes(hipStream_t st, int policy, const double* geom, const double* nodes, const double* recs,
+                  const float* nodes32, double fp64_ratio, const double* sigma, const int32_t* pset, int64_t nelem,
+                  const double* clusters, const int32_t* order, const double* px, const double* py,
+                  const double* pz, int64_t npts, double* phi, double* ex, double* ey, double* ez, double* ws,
+                  int64_t ws_doubles, hipEvent_t far_begin, hipEvent_t far_end)
 {
-    if (npts <= 0) return hipSuccess;
     const unsigned gx = nblocks(npts);
     // enough blocks to fill 256 CUs several times over; each range >= 32 elements
     int64_t nsplit = (2048 + gx - 1) / gx;
@@ -605,26 +625,49 @@ hipError_t launch_field(hipStream_t st, int policy, const double* geom, const do
     const bool culled = policy == 0 && clusters && order;
     if (far_begin) (void)hipEventRecord(far_begin, st);
     if (policy == 0 && nodes32)
-        k_field_far32<<<grid, kBlock, 0, st>>>(geom, nodes, nodes32, fp64_ratio, sigma, nelem, per, px, py, pz, npts, o0,
-                                               o1, o2, o3, stride);
+        k_field_far32<MULTI><<<grid, kBlock, 0, st>>>(geom, nodes, nodes32, fp64_ratio, sigma, pset, nelem, per, px, py,
+                                                      pz, npts, o0, o1, o2, o3, stride);
     else if (policy == 0)
-        k_field_far<<<grid, kBlock, 0, st>>>(geom, nodes, sigma, nelem, per, px, py, pz, npts, o0, o1, o2, o3, stride);
+        k_field_far<MULTI><<<grid, kBlock, 0, st>>>(geom, nodes, sigma, pset, nelem, per, px, py, pz, npts, o0, o1, o2,
+                                                    o3, stride);
     if (far_end) (void)hipEventRecord(far_end, st);
-    if (!culled)
-    {
+    if (!culled) {
         if (policy >= 2)
-            k_field_near<1><<<grid, kBlock, 0, st>>>(geom, recs, sigma, nelem, per, px, py, pz, npts, policy, 0, o0, o1,
-                                                     o2, o3, stride);
+            k_field_near<1, MULTI><<<grid, kBlock, 0, st>>>(geom, recs, sigma, pset, nelem, per, px, py, pz, npts,
+                                                            policy, 0, o0, o1, o2, o3, stride);
         else
-            k_field_near<0><<<grid, kBlock, 0, st>>>(geom, recs, sigma, nelem, per, px, py, pz, npts, policy,
-                                                     policy == 0 ? 1 : 0, o0, o1, o2, o3, stride);
+            k_field_near<0, MULTI><<<grid, kBlock, 0, st>>>(geom, recs, sigma, pset, nelem, per, px, py, pz, npts,
+                                                            policy, policy == 0 ? 1 : 0, o0, o1, o2, o3, stride);
     }
     if (nsplit > 1) k_field_combine<<<gx, kBlock, 0, st>>>(ws, (int)nsplit, npts, phi, ex, ey, ez);
     if (culled) {
         const int64_t ncl = (nelem + KEM_CLUSTER - 1) / KEM_CLUSTER;
-        k_field_near_culled<<<gx, kBlock, 0, st>>>(geom, sigma, clusters, ncl, order, nelem, px, py, pz, npts, ex, ey,
-                                                   ez, phi);
+        k_field_near_culled<MULTI><<<gx, kBlock, 0, st>>>(geom, sigma, pset, clusters, ncl, order, nelem, px, py, pz,
+                                                          npts, ex, ey, ez, phi);
     }
+}
+}  // namespace
+
+hipError_t launch_field(hipStream_t st, int policy, const double* geom, const double* nodes, const double* recs,
+                        const float* nodes32, double fp64_ratio, const double* sigma, const int32_t* pset,
+                        int64_t nelem, const double* clusters, const int32_t* order, const double* px,
+                        const double* py, const double* pz, int64_t npts, double* phi, double* ex, double* ey,
+                        double* ez, double* ws, int64_t ws_doubles, hipEvent_t far_begin, hipEvent_t far_end)
+{
+    if (npts <= 0) return hipSuccess;
+    if (pset)
+        field_passes<true>(st, policy, geom, nodes, recs, nodes32, fp64_ratio, sigma, pset, nelem, clusters, order, px,
+                           py, pz, npts, phi, ex, ey, ez, ws, ws_doubles, far_begin, far_end);
+    else
+        field_passes<false>(st, policy, geom, nodes, recs, nodes32, fp64_ratio, sigma, nullptr, nelem, clusters, order,
+                            px, py, pz, npts, phi, ex, ey, ez, ws, ws_doubles, far_begin, far_end);
+    return hipGetLastError();
+}
+
+hipError_t launch_check_sets(hipStream_t st, const int32_t* pset, int64_t n, int64_t nsets, int* bad)
+{
+    if (n <= 0) return hipSuccess;
+    k_check_sets<<<nblocks(n), kBlock, 0, st>>>(pset, n, nsets, bad);
     return hipGetLastError();
 }
 

@@ -272,6 +272,61 @@ class KRobinHoodChargeDensitySolver:
         return matrix.solve(b, **self.kw)
 
 
+class DeviceArray:
+    """A float64 / int32 array in the context's device memory (kem_dev_alloc), for the entry
+    points that take device pointers."""
+
+    def __init__(self, ctx: "Context", host: np.ndarray):
+        self.ctx = ctx
+        self.host = np.ascontiguousarray(host)
+        self.ptr = _P()
+        ctx.check(ctx.lib.kem_dev_alloc(ctx.h, self.host.nbytes, ctypes.byref(self.ptr)))
+        ctx.check(ctx.lib.kem_dev_upload(ctx.h, self.ptr, self.host.ctypes.data_as(_P), self.host.nbytes))
+
+    @classmethod
+    def empty(cls, ctx: "Context", shape, dtype=np.float64):
+        return cls(ctx, np.zeros(shape, dtype=dtype))
+
+    def get(self) -> np.ndarray:
+        out = np.empty_like(self.host)
+        self.ctx.check(self.ctx.lib.kem_dev_download(self.ctx.h, out.ctypes.data_as(_P), self.ptr, out.nbytes))
+        return out
+
+    def upload(self, host: np.ndarray):
+        host = np.ascontiguousarray(host, dtype=self.host.dtype)
+        assert host.nbytes == self.host.nbytes
+        self.ctx.check(self.ctx.lib.kem_dev_upload(self.ctx.h, self.ptr, host.ctypes.data_as(_P), host.nbytes))
+
+    def close(self):
+        if self.ptr:
+            self.ctx.lib.kem_dev_free(self.ctx.h, self.ptr)
+            self.ptr = _P()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def field_sets(container: "KSurfaceContainer", sigma_sets, set_index, points, precision="fp64"):
+    """Field of several charge-density sets in one pass (kem_field_eval_sets): point i uses
+    column set_index[i] of sigma_sets (n x k)."""
+    ctx = container.ctx
+    S = DeviceArray(ctx, np.asfortranarray(np.asarray(sigma_sets, dtype=np.float64)).T)
+    idx = DeviceArray(ctx, np.asarray(set_index, dtype=np.int32))
+    P = _f64(points, (-1, 3))
+    X = [DeviceArray(ctx, P[:, k]) for k in range(3)]
+    m = P.shape[0]
+    O = [DeviceArray.empty(ctx, m) for _ in range(4)]
+    prec = {"fp64": _lib.KEM_FP64, "mixed": _lib.KEM_MIXED}[precision]
+    ctx.check(ctx.lib.kem_field_eval_sets(ctx.h, container.h, S.host.shape[0], S.ptr, idx.ptr, m, X[0].ptr,
+                                          X[1].ptr, X[2].ptr, prec, O[0].ptr, O[1].ptr, O[2].ptr, O[3].ptr))
+    ctx.check(ctx.lib.kem_ctx_sync(ctx.h))
+    E = np.stack([O[k].get() for k in (1, 2, 3)], 1)
+    return E, O[0].get()
+
+
 class KGaussianEliminationChargeDensitySolver:
     """Direct solve (gaussian_elimination_bem_solver; KGaussianEliminationChargeDensitySolver.cc:29-46)."""
 

@@ -100,3 +100,39 @@ def test_solve_batch_errors(coax):
     m, cont, A, Ah = coax
     with pytest.raises(_lib.KemError):
         A.solve_batch(np.ones((m.n, 2)), tolerance=-1.0)
+
+
+@pytest.mark.parametrize("sizes", [[700, 1500, 123, 900], [1000, 5000, 123, 7000, 2500, 64]])
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_field_sets_uneven_batches(coax, sizes, precision):
+    """Uneven point batches, each with its own charge-density set, in one pass (unsorted below
+    4096 points, Morton-sorted by (set, position) above): every batch equals the single-set
+    evaluation of its points."""
+    from kassiopeia_amd import kemfield as K
+
+    m, cont, A, Ah = coax
+    groups = G.coax_ring_groups(m, 16)
+    V = np.random.default_rng(3).uniform(-1000.0, 0.0, size=(16, 3))
+    S = np.linalg.solve(Ah, V[groups, :])
+    pts = G.points_annulus(sum(sizes), 0.5, 1.0, 2.0, 77)
+    bset = np.arange(len(sizes)) % 3
+    set_index = np.repeat(bset, sizes).astype(np.int32)
+    E, phi = K.field_sets(cont, S, set_index, pts, precision)
+    off = np.concatenate([[0], np.cumsum(sizes)])
+    for b in range(len(sizes)):
+        sl = slice(off[b], off[b + 1])
+        Eb, pb = K.KIntegratingElectrostaticFieldSolver(cont, S[:, bset[b]], precision).ElectricFieldAndPotential(
+            pts[sl])
+        tol = 1e-13 if precision == "fp64" else 1e-7
+        assert np.abs(phi[sl] - pb).max() <= tol * np.abs(pb).max()
+        assert np.abs(E[sl] - Eb).max() <= tol * np.abs(Eb).max()
+
+
+def test_field_sets_rejects_bad_index(coax):
+    from kassiopeia_amd import _lib
+    from kassiopeia_amd import kemfield as K
+
+    m, cont, A, Ah = coax
+    S = np.ones((m.n, 2)) * 1e-9
+    with pytest.raises(_lib.KemError):
+        K.field_sets(cont, S, np.array([0, 1, 2], np.int32), np.ones((3, 3)) * 0.7)
