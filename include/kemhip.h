@@ -195,6 +195,24 @@ int kem_solve(kem_matrix* A, const kem_solve_opts* opts, const double* b, double
    stats: iterations / refinements = maximum over groups of 64, residual = worst. */
 int kem_solve_batch(kem_matrix* A, const kem_solve_opts* opts, int64_t nrhs, const double* B, double* X,
                     double* residuals, kem_solve_stats* stats);
+/* ---- adjoint (no reference counterpart: KEMField has no derivatives).  Gradients of a loss
+   L(phi) of the potentials phi_k = sum_j G(P_k; T_j) sigma_j with respect to the charge
+   densities and the triangle vertices, under the numeric policy with each pair's quadrature
+   regime frozen.  Vertex gradients are per element and corner: dvert[j*9 + 3m + c] =
+   d/d(v_m)_c with v_0 = P0, v_1 = P0 + A N1, v_2 = P0 + B N2 of element j's record.
+   Triangles only (KEM_E_GEOM otherwise). */
+/* A^T lambda = b by GMRES (FP64, or KEM_MIXED: FP32 A^T inner solves + FP64 refinement). */
+int kem_solve_transposed(kem_matrix* A, const kem_solve_opts* opts, const double* b, double* x,
+                         kem_solve_stats* stats);
+/* gphi[k] = dL/dphi_k at npts points P (AoS): dsigma[j] = sum_k gphi_k G(P_k; T_j) and dvert
+   (n x 9) = sum_k gphi_k sigma_j dG(P_k; T_j)/dvertices (host arrays, synchronous). */
+int kem_field_vjp(kem_ctx* ctx, const kem_mesh* mesh, const double* sigma, int64_t npts, const double* P,
+                  const double* gphi, double* dsigma, double* dvert);
+/* dvert (n x 9) = -sum_ij lambda_i sigma_j dA_ij/dvertices for the Dirichlet collocation
+   matrix A_ij = G(c_i; T_j), c_i = centroid of T_i (the term lambda^T (dA) sigma of the adjoint
+   of A sigma = b).  KEM_E_STATE for meshes with Neumann rows. */
+int kem_assemble_vjp(kem_ctx* ctx, const kem_mesh* mesh, const double* lambda, const double* sigma, double* dvert);
+
 /* Y = A X for nrhs vectors (host, column-major n x nrhs): KSquareMatrix::Multiply
    (KMatrix.hh:92-111) over a block; KEM_MIXED uses the FP32 matrix on MFMA. */
 int kem_matrix_multiply_batch(kem_matrix* A, int precision, int64_t nrhs, const double* X, double* Y);

@@ -85,6 +85,9 @@ SIGNATURES = {
     "kem_matrix_multiply": (_INT, [_P, _INT, _P, _P]),
     "kem_solve_opts_default": (None, [ctypes.POINTER(SolveOpts)]),
     "kem_solve": (_INT, [_P, ctypes.POINTER(SolveOpts), _P, _P, ctypes.POINTER(SolveStats)]),
+    "kem_solve_transposed": (_INT, [_P, ctypes.POINTER(SolveOpts), _P, _P, ctypes.POINTER(SolveStats)]),
+    "kem_field_vjp": (_INT, [_P, _P, _P, _I64, _P, _P, _P, _P]),
+    "kem_assemble_vjp": (_INT, [_P, _P, _P, _P, _P]),
     "kem_solve_batch": (_INT, [_P, ctypes.POINTER(SolveOpts), _I64, _P, _P, _P, ctypes.POINTER(SolveStats)]),
     "kem_matrix_multiply_batch": (_INT, [_P, _INT, _I64, _P, _P]),
 }

@@ -26,6 +26,7 @@ SOURCES = [
     ("kem_kernels.hip", True),
     ("kem_sort.hip", True),
     ("kem_batch.hip", True),
+    ("kem_adjoint.hip", True),
     ("kem_capi.cpp", False),
     ("kem_prep.cpp", False),
 ]

@@ -50,6 +50,8 @@ struct kem_mesh {
     double* targets = nullptr;
     float* nodes32 = nullptr;  // mixed-precision far-field node table
     double* recs = nullptr;      // raw shape records (analytic / reference policies)
+    double* rule = nullptr;      // triangle rule barycentrics and weights (adjoint)
+    bool all_triangles = true;
     double* clusters = nullptr;  // near-pass element clusters (centre, radius)
     int32_t* order = nullptr;    // elements in cluster order
     std::vector<int32_t> shape, bc;
@@ -62,6 +64,7 @@ struct kem_matrix {
     int64_t n = 0;
     double* A64 = nullptr;
     float* A32 = nullptr;
+    int trans = 0;  // solves run on A^T (kem_solve_transposed)
 };
 
 namespace {
@@ -112,6 +115,14 @@ hipError_t ensure_scal(kem_ctx* c, int64_t doubles)
     hipError_t e = dalloc(&c->scal, doubles);
     if (e == hipSuccess) c->scal_doubles = doubles;
     return e;
+}
+
+// make the context's device current and drop a stale error another caller left behind, so
+// that the hipGetLastError() after each launch reports this call's own launches
+hipError_t enter(kem_ctx* c)
+{
+    (void)hipGetLastError();
+    return hipSetDevice(c->device);
 }
 
 // RAII device vector for solver temporaries
@@ -186,7 +197,7 @@ int kem_ctx_set_mixed_fp64_ratio(kem_ctx* c, double ratio)
 int kem_ctx_set_kernel_timing(kem_ctx* c, int on)
 {
     if (!c) return KEM_E_ARG;
-    KEM_CK(c, hipSetDevice(c->device));
+    KEM_CK(c, enter(c));
     if (on)
         for (hipEvent_t& e : c->ev)
             if (!e) KEM_CK(c, hipEventCreate(&e));
@@ -198,7 +209,7 @@ int kem_ctx_set_kernel_timing(kem_ctx* c, int on)
 int kem_ctx_kernel_time(kem_ctx* c, double* total_ms, int* launches)
 {
     if (!c || !total_ms || !launches) return KEM_E_ARG;
-    KEM_CK(c, hipSetDevice(c->device));
+    KEM_CK(c, enter(c));
     double t = 0.;
     for (int k = 0; k < c->ev_used; k++) {
         KEM_CK(c, hipEventSynchronize(c->ev[2 * k + 1]));
@@ -217,7 +228,7 @@ int kem_dev_alloc(kem_ctx* c, int64_t bytes, void** p)
     if (!c || !p || bytes < 0) return KEM_E_ARG;
     *p = nullptr;
     if (bytes == 0) return KEM_OK;
-    KEM_CK(c, hipSetDevice(c->device));
+    KEM_CK(c, enter(c));
     KEM_CK(c, hipMalloc(p, (size_t)bytes));
     return KEM_OK;
 }
@@ -226,7 +237,7 @@ int kem_dev_free(kem_ctx* c, void* p)
 {
     if (!c) return KEM_E_ARG;
     if (!p) return KEM_OK;
-    KEM_CK(c, hipSetDevice(c->device));
+    KEM_CK(c, enter(c));
     KEM_CK(c, hipStreamSynchronize(c->stream));
     KEM_CK(c, hipFree(p));
     return KEM_OK;
@@ -236,7 +247,7 @@ int kem_dev_upload(kem_ctx* c, void* dst, const void* src, int64_t bytes)
 {
     if (!c || (bytes > 0 && (!dst || !src)) || bytes < 0) return KEM_E_ARG;
     if (bytes == 0) return KEM_OK;
-    KEM_CK(c, hipSetDevice(c->device));
+    KEM_CK(c, enter(c));
     KEM_CK(c, hipMemcpyAsync(dst, src, (size_t)bytes, hipMemcpyHostToDevice, c->stream));
     KEM_CK(c, hipStreamSynchronize(c->stream));
     return KEM_OK;
@@ -246,7 +257,7 @@ int kem_dev_download(kem_ctx* c, void* dst, const void* src, int64_t bytes)
 {
     if (!c || (bytes > 0 && (!dst || !src)) || bytes < 0) return KEM_E_ARG;
     if (bytes == 0) return KEM_OK;
-    KEM_CK(c, hipSetDevice(c->device));
+    KEM_CK(c, enter(c));
     KEM_CK(c, hipMemcpyAsync(dst, src, (size_t)bytes, hipMemcpyDeviceToHost, c->stream));
     KEM_CK(c, hipStreamSynchronize(c->stream));
     return KEM_OK;
@@ -255,7 +266,7 @@ int kem_dev_download(kem_ctx* c, void* dst, const void* src, int64_t bytes)
 int kem_ctx_sync(kem_ctx* c)
 {
     if (!c) return KEM_E_ARG;
-    KEM_CK(c, hipSetDevice(c->device));
+    KEM_CK(c, enter(c));
     KEM_CK(c, hipStreamSynchronize(c->stream));
     return KEM_OK;
 }
@@ -286,7 +297,7 @@ int kem_mesh_create(kem_ctx* c, int64_t n, const int32_t* shape, const double* d
     if (!c || !out) return KEM_E_ARG;
     *out = nullptr;
     if (n <= 0 || !shape || !data) return fail(c, KEM_E_ARG, "mesh: empty or null arrays");
-    KEM_CK(c, hipSetDevice(c->device));
+    KEM_CK(c, enter(c));
     std::vector<double> g((size_t)n * KG_STRIDE), nd((size_t)n * KEM_NODE_STRIDE), t((size_t)n * KT_STRIDE);
     std::vector<float> nd32((size_t)n * KEM_NODE_STRIDE);
     kem_mesh* m = new kem_mesh();
@@ -320,7 +331,12 @@ int kem_mesh_create(kem_ctx* c, int64_t n, const int32_t* shape, const double* d
     std::vector<double> recs((size_t)n * KEM_REC, 0.);
     for (int64_t i = 0; i < n; i++)
         for (int k = 0; k < 11; k++) recs[(size_t)i * KEM_REC + k] = data[11 * i + k];
+    std::vector<double> rule(4 * KEM_NODE_SLOTS, 0.);
+    kem::triangle_rule_table(rule.data());
+    for (int64_t i = 0; i < n; i++) m->all_triangles = m->all_triangles && shape[i] == KEM_TRIANGLE;
     hipError_t e = dalloc(&m->geom, (int64_t)g.size());
+    if (e == hipSuccess) e = dalloc(&m->rule, (int64_t)rule.size());
+    if (e == hipSuccess) e = hipMemcpy(m->rule, rule.data(), sizeof(double) * rule.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = dalloc(&m->recs, (int64_t)recs.size());
     if (e == hipSuccess) e = hipMemcpy(m->recs, recs.data(), sizeof(double) * recs.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = dalloc(&m->clusters, (int64_t)clusters.size());
@@ -355,6 +371,7 @@ void kem_mesh_destroy(kem_mesh* m)
     dfree(m->clusters);
     dfree(m->order);
     dfree(m->recs);
+    dfree(m->rule);
     delete m;
 }
 
@@ -382,7 +399,7 @@ int kem_element_eval(kem_ctx* c, const kem_mesh* m, int64_t np, const int64_t* e
     if (np <= 0) return KEM_OK;
     for (int64_t k = 0; k < np; k++)
         if (elem[k] < 0 || elem[k] >= m->n) return fail(c, KEM_E_ARG, "element index out of range");
-    KEM_CK(c, hipSetDevice(c->device));
+    KEM_CK(c, enter(c));
     int64_t* de = nullptr;
     double *dP = nullptr, *dO = nullptr;
     hipError_t e = dalloc(&de, np);
@@ -444,7 +461,7 @@ int kem_field_eval_device(kem_ctx* c, const kem_mesh* m, const double* sigma, in
     if (m->ctx != c) return fail(c, KEM_E_STATE, "mesh belongs to another context");
     if (precision != KEM_FP64 && precision != KEM_MIXED) return fail(c, KEM_E_ARG, "field: unknown precision mode");
     if (npts <= 0) return KEM_OK;
-    KEM_CK(c, hipSetDevice(c->device));
+    KEM_CK(c, enter(c));
     return field_eval(c, m, sigma, nullptr, npts, x, y, z, precision, phi, ex, ey, ez);
 }
 
@@ -457,7 +474,7 @@ int kem_field_eval_sets(kem_ctx* c, const kem_mesh* m, int64_t nsets, const doub
     if (precision != KEM_FP64 && precision != KEM_MIXED) return fail(c, KEM_E_ARG, "field: unknown precision mode");
     if (nsets > (1 << 6)) return fail(c, KEM_E_ARG, "field: at most 64 charge-density sets per call");
     if (npts <= 0) return KEM_OK;
-    KEM_CK(c, hipSetDevice(c->device));
+    KEM_CK(c, enter(c));
     KEM_CK(c, ensure_scal(c, 1));
     int* bad = (int*)c->scal;
     int hbad = 0;
@@ -473,7 +490,7 @@ int kem_field_census(kem_ctx* c, const kem_mesh* m, int64_t npts, const double* 
                      int precision, int64_t* counts)
 {
     if (!c || !m || !counts || (npts > 0 && (!x || !y || !z))) return KEM_E_ARG;
-    KEM_CK(c, hipSetDevice(c->device));
+    KEM_CK(c, enter(c));
     unsigned long long* d = nullptr;
     KEM_CK(c, dalloc(&d, 12));
     hipError_t e = hipMemsetAsync(d, 0, sizeof(unsigned long long) * 12, c->stream);
@@ -494,7 +511,7 @@ int kem_field_eval(kem_ctx* c, const kem_mesh* m, const double* sigma, int64_t n
 {
     if (!c || !m || !sigma || (npts > 0 && !P)) return KEM_E_ARG;
     if (npts <= 0) return KEM_OK;
-    KEM_CK(c, hipSetDevice(c->device));
+    KEM_CK(c, enter(c));
     std::vector<double> soa((size_t)npts * 3);
     for (int64_t i = 0; i < npts; i++)
         for (int k = 0; k < 3; k++) soa[(size_t)k * npts + i] = P[3 * i + k];
@@ -538,7 +555,7 @@ int kem_matrix_assemble(kem_ctx* c, const kem_mesh* m, int storage, kem_matrix**
     *out = nullptr;
     if (m->ctx != c) return fail(c, KEM_E_STATE, "mesh belongs to another context");
     if (!(storage & (KEM_STORE_FP64 | KEM_STORE_FP32))) return fail(c, KEM_E_ARG, "matrix: no storage selected");
-    KEM_CK(c, hipSetDevice(c->device));
+    KEM_CK(c, enter(c));
     kem_matrix* A = new kem_matrix();
     A->ctx = c;
     A->n = m->n;
@@ -574,7 +591,7 @@ int kem_matrix_get(kem_matrix* A, int64_t i, int64_t j, double* out)
     if (!A || !out) return KEM_E_ARG;
     kem_ctx* c = A->ctx;
     if (i < 0 || j < 0 || i >= A->n || j >= A->n) return fail(c, KEM_E_ARG, "matrix index out of range");
-    KEM_CK(c, hipSetDevice(c->device));
+    KEM_CK(c, enter(c));
     const int64_t off = j * A->n + i;
     if (A->A64) {
         KEM_CK(c, hipMemcpy(out, A->A64 + off, sizeof(double), hipMemcpyDeviceToHost));
@@ -591,7 +608,7 @@ int kem_matrix_copy(kem_matrix* A, double* host)
     if (!A || !host) return KEM_E_ARG;
     kem_ctx* c = A->ctx;
     if (!A->A64) return fail(c, KEM_E_STATE, "matrix has no FP64 storage");
-    KEM_CK(c, hipSetDevice(c->device));
+    KEM_CK(c, enter(c));
     const int64_t n = A->n;
     std::vector<double> cm((size_t)(n * n));
     KEM_CK(c, hipMemcpy(cm.data(), A->A64, sizeof(double) * n * n, hipMemcpyDeviceToHost));
@@ -623,7 +640,7 @@ int kem_matrix_multiply(kem_matrix* A, int transpose, const double* x, double* y
 {
     if (!A || !x || !y) return KEM_E_ARG;
     kem_ctx* c = A->ctx;
-    KEM_CK(c, hipSetDevice(c->device));
+    KEM_CK(c, enter(c));
     const int64_t n = A->n;
     DVec dx, dy;
     KEM_CK(c, dalloc(&dx.p, n));
@@ -693,7 +710,7 @@ int gmres_dev(kem_matrix* A, bool use32, const double* d_b, double* d_x, double 
     int status = KEM_OK;
     for (;;) {
         // r = b - A x ; v0 = r / ||r||
-        KEM_CK(c, matvec_dev(A, use32, 0, d_x, w.p));
+        KEM_CK(c, matvec_dev(A, use32, A->trans, d_x, w.p));
         KEM_CK(c, kem::launch_axpby(st, 1.0, d_b, -1.0, w.p, w.p, n));
         KEM_CK(c, kem::launch_dot(st, w.p, w.p, n, d_part, d_h1, true));
         KEM_CK(c, kem::launch_div_dev(st, d_h1, w.p, V.p, n));
@@ -706,7 +723,7 @@ int gmres_dev(kem_matrix* A, bool use32, const double* d_b, double* d_x, double 
         for (;;) {
             double* vj = V.p + (int64_t)j * n;
             double* vj1 = V.p + (int64_t)(j + 1) * n;
-            KEM_CK(c, matvec_dev(A, use32, 0, vj, w.p));
+            KEM_CK(c, matvec_dev(A, use32, A->trans, vj, w.p));
             // CGS2: h1 = V^T w, w -= V h1; h2 = V^T w, w -= V h2; h = h1 + h2
             KEM_CK(c, kem::launch_gemv_t<double>(st, V.p, n, n, j + 1, w.p, d_h1));
             KEM_CK(c, kem::launch_gemv<double>(st, V.p, n, n, j + 1, d_h1, -1.0, 1.0, w.p, c->ws));
@@ -765,7 +782,7 @@ int true_residual(kem_matrix* A, const double* d_b, const double* d_x, double* d
     hipStream_t st = c->stream;
     const int64_t n = A->n;
     KEM_CK(c, ensure_scal(c, 260));
-    KEM_CK(c, matvec_dev(A, A->A64 == nullptr, 0, d_x, d_tmp));
+    KEM_CK(c, matvec_dev(A, A->A64 == nullptr, A->trans, d_x, d_tmp));
     KEM_CK(c, kem::launch_axpby(st, 1.0, d_b, -1.0, d_tmp, d_tmp, n));
     KEM_CK(c, kem::launch_dot(st, d_tmp, d_tmp, n, c->scal, c->scal + 256, true));
     KEM_CK(c, kem::launch_dot(st, d_b, d_b, n, c->scal, c->scal + 257, true));
@@ -828,7 +845,7 @@ int kem_solve(kem_matrix* A, const kem_solve_opts* opts, const double* b, double
     kem_solve_stats s{};
     if (o.max_iterations <= 0) return fail(c, KEM_E_ARG, "max_iterations must be positive");
     if (!(o.tolerance > 0.)) return fail(c, KEM_E_ARG, "tolerance must be positive");
-    KEM_CK(c, hipSetDevice(c->device));
+    KEM_CK(c, enter(c));
     const int64_t n = A->n;
     SolveCtl ctl;
     ctl.t0 = now_s();
@@ -906,6 +923,83 @@ int kem_solve(kem_matrix* A, const kem_solve_opts* opts, const double* b, double
     return status;
 }
 
+
+// ------------------------------------------------------------------ adjoint
+int kem_solve_transposed(kem_matrix* A, const kem_solve_opts* opts, const double* b, double* x,
+                         kem_solve_stats* stats)
+{
+    if (!A) return KEM_E_ARG;
+    kem_solve_opts o;
+    kem_solve_opts_default(&o);
+    if (opts) o = *opts;
+    if (o.method != KEM_SOLVER_GMRES) return fail(A->ctx, KEM_E_ARG, "transposed solve supports GMRES only");
+    A->trans = 1;
+    const int rc = kem_solve(A, &o, b, x, stats);
+    A->trans = 0;
+    return rc;
+}
+
+namespace {
+int vjp_checks(kem_ctx* c, const kem_mesh* m)
+{
+    if (m->ctx != c) return fail(c, KEM_E_STATE, "mesh belongs to another context");
+    if (c->policy != KEM_POLICY_NUMERIC) return fail(c, KEM_E_STATE, "derivatives need the numeric integrator policy");
+    if (!m->all_triangles) return fail(c, KEM_E_GEOM, "derivatives are implemented for triangles only");
+    return KEM_OK;
+}
+}  // namespace
+
+int kem_field_vjp(kem_ctx* c, const kem_mesh* m, const double* sigma, int64_t npts, const double* P,
+                  const double* gphi, double* dsigma, double* dvert)
+{
+    if (!c || !m || !sigma || !dsigma || !dvert || npts < 0 || (npts > 0 && (!P || !gphi))) return KEM_E_ARG;
+    int rc = vjp_checks(c, m);
+    if (rc != KEM_OK) return rc;
+    KEM_CK(c, enter(c));
+    const int64_t n = m->n;
+    DVec ds, dP, dg, o1, o2, ws;
+    const size_t wsd = kem::vjp_workspace_doubles(n, npts);
+    KEM_CK(c, dalloc(&ds.p, n));
+    KEM_CK(c, dalloc(&dP.p, 3 * std::max<int64_t>(npts, 1)));
+    KEM_CK(c, dalloc(&dg.p, std::max<int64_t>(npts, 1)));
+    KEM_CK(c, dalloc(&o1.p, n));
+    KEM_CK(c, dalloc(&o2.p, 9 * n));
+    KEM_CK(c, dalloc(&ws.p, (int64_t)wsd));
+    KEM_CK(c, hipMemcpyAsync(ds.p, sigma, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+    if (npts > 0) {
+        KEM_CK(c, hipMemcpyAsync(dP.p, P, sizeof(double) * 3 * npts, hipMemcpyHostToDevice, c->stream));
+        KEM_CK(c, hipMemcpyAsync(dg.p, gphi, sizeof(double) * npts, hipMemcpyHostToDevice, c->stream));
+    }
+    KEM_CK(c, kem::launch_field_vjp(c->stream, m->geom, ds.p, m->rule, n, dP.p, dg.p, npts, o1.p, o2.p, ws.p, wsd));
+    KEM_CK(c, hipMemcpyAsync(dsigma, o1.p, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+    KEM_CK(c, hipMemcpyAsync(dvert, o2.p, sizeof(double) * 9 * n, hipMemcpyDeviceToHost, c->stream));
+    KEM_CK(c, hipStreamSynchronize(c->stream));
+    return KEM_OK;
+}
+
+int kem_assemble_vjp(kem_ctx* c, const kem_mesh* m, const double* lambda, const double* sigma, double* dvert)
+{
+    if (!c || !m || !lambda || !sigma || !dvert) return KEM_E_ARG;
+    int rc = vjp_checks(c, m);
+    if (rc != KEM_OK) return rc;
+    if (m->has_neumann) return fail(c, KEM_E_STATE, "matrix derivatives are implemented for Dirichlet rows only");
+    KEM_CK(c, enter(c));
+    const int64_t n = m->n;
+    std::vector<double> neg(lambda, lambda + n);
+    for (auto& v : neg) v = -v;
+    DVec dl, du, o2, ws;
+    const size_t wsd = kem::vjp_workspace_doubles(n, 0);
+    KEM_CK(c, dalloc(&dl.p, n));
+    KEM_CK(c, dalloc(&du.p, n));
+    KEM_CK(c, dalloc(&o2.p, 9 * n));
+    KEM_CK(c, dalloc(&ws.p, (int64_t)wsd));
+    KEM_CK(c, hipMemcpyAsync(dl.p, neg.data(), sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+    KEM_CK(c, hipMemcpyAsync(du.p, sigma, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+    KEM_CK(c, kem::launch_matrix_vjp(c->stream, m->geom, m->targets, m->rule, n, dl.p, du.p, o2.p, ws.p, wsd));
+    KEM_CK(c, hipMemcpyAsync(dvert, o2.p, sizeof(double) * 9 * n, hipMemcpyDeviceToHost, c->stream));
+    KEM_CK(c, hipStreamSynchronize(c->stream));
+    return KEM_OK;
+}
 
 // ------------------------------------------------------------------ multi-RHS
 namespace {
@@ -1180,7 +1274,7 @@ int kem_matrix_multiply_batch(kem_matrix* A, int precision, int64_t nrhs, const 
     if (use32 && !A->A32) return fail(c, KEM_E_STATE, "FP32 product needs FP32 matrix storage");
     if (!use32 && !A->A64) return fail(c, KEM_E_STATE, "FP64 product needs FP64 matrix storage");
     if (nrhs == 0) return KEM_OK;
-    KEM_CK(c, hipSetDevice(c->device));
+    KEM_CK(c, enter(c));
     const int64_t n = A->n;
     const int g = (int)std::min<int64_t>(nrhs, 64);
     DVec dX, dY;
@@ -1215,7 +1309,7 @@ int kem_solve_batch(kem_matrix* A, const kem_solve_opts* opts, int64_t nrhs, con
         if (stats) *stats = s;
         return KEM_OK;
     }
-    KEM_CK(c, hipSetDevice(c->device));
+    KEM_CK(c, enter(c));
     const int64_t n = A->n;
     SolveCtl ctl;
     ctl.t0 = now_s();

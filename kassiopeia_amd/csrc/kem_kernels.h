@@ -93,6 +93,17 @@ hipError_t launch_bnormalize(hipStream_t st, const double* W, const double* nrm,
                              double* Vt64);
 hipError_t launch_baxpby(hipStream_t st, double a, const double* X, double b, const double* Y, double* Z, int64_t len);
 
+// ---- adjoint (kem_adjoint.hip): vertex derivatives under the numeric policy, triangles only
+size_t vjp_workspace_doubles(int64_t n, int64_t npts);
+// dsig[j] = sum_k gk[k] G(P_k; T_j);  dvert[j*9 + 3m + c] = sum_k gk[k] sigma_j dG(P_k; T_j)/d(v_m)_c
+hipError_t launch_field_vjp(hipStream_t st, const double* geom, const double* sigma, const double* rule, int64_t n,
+                            const double* pts, const double* gk, int64_t npts, double* dsig, double* dvert,
+                            double* ws, size_t ws_doubles);
+// dvert = d/dV sum_ij lw_i A_ij u_j (Dirichlet collocation matrix, centroids c_i move with T_i)
+hipError_t launch_matrix_vjp(hipStream_t st, const double* geom, const double* targets, const double* rule,
+                             int64_t n, const double* lw, const double* u, double* dvert, double* ws,
+                             size_t ws_doubles);
+
 }  // namespace kem
 
 #endif

@@ -70,51 +70,23 @@ inline void bary(const double A[3], const double B[3], const double C[3], double
     for (int k = 0; k < 3; k++) Q[k] = a * A[k] + b * B[k] + c * C[k];
 }
 
+}  // namespace
+void triangle_rule_table(double* t);
+namespace {
+
 void triangle_nodes(const double* d, double* nodes)
 {
-    double A[3], B[3], C[3], Q[3];
+    double A[3], B[3], C[3], Q[3], t[4 * KEM_NODE_SLOTS];
     for (int k = 0; k < 3; k++) {
         A[k] = d[2 + k];
         B[k] = d[2 + k] + (d[0] * d[5 + k]);
         C[k] = d[2 + k] + (d[1] * d[8 + k]);
     }
-    // 7-point: centroid, then two orbits (alpha,beta,beta) in the order (a,b,c),(b,a,c),(c,b,a)
-    const double t = 1. / 3., q15 = std::sqrt(15), r = (1. + q15) / 7., s = (1. - q15) / 7.;
-    const double a7[3] = {t, t + (2. * t * s), t + (2. * t * r)};
-    const double b7[3] = {t, t - (t * s), t - (t * r)};
-    const double w7[3] = {9 / 40., (155. + q15) / 1200., (155. - q15) / 1200.};
-    int slot = KEM_N7;
-    bary(A, B, C, a7[0], b7[0], b7[0], Q);
-    put_node(nodes, slot++, Q, w7[0]);
-    for (int o = 1; o < 3; o++) {
-        bary(A, B, C, a7[o], b7[o], b7[o], Q);
-        put_node(nodes, slot++, Q, w7[o]);
-        bary(A, B, C, b7[o], a7[o], b7[o], Q);
-        put_node(nodes, slot++, Q, w7[o]);
-        bary(A, B, C, b7[o], b7[o], a7[o], Q);
-        put_node(nodes, slot++, Q, w7[o]);
-    }
-    // 12-point: cyclic orbits (a,b,c),(b,c,a),(c,a,b), weights doubled
-    slot = KEM_N12;
-    for (int o = 0; o < 4; o++) {
-        double a = kT12a[o], b = kT12b[o], c = kT12c[o];
-        bary(A, B, C, a, b, c, Q);
-        put_node(nodes, slot++, Q, kT12w[o] * 2.);
-        bary(A, B, C, b, c, a, Q);
-        put_node(nodes, slot++, Q, kT12w[o] * 2.);
-        bary(A, B, C, c, a, b, Q);
-        put_node(nodes, slot++, Q, kT12w[o] * 2.);
-    }
-    // 33-point: five 3-orbits and three 6-orbits
-    slot = KEM_N33;
-    for (int o = 0; o < 8; o++) {
-        double a = kT33a[o], b = kT33b[o], c = 1. - kT33a[o] - kT33b[o];
-        double perm[6][3] = {{a, b, c}, {b, a, c}, {c, b, a}, {a, c, b}, {c, a, b}, {b, c, a}};
-        int np = o < 5 ? 3 : 6;
-        for (int p = 0; p < np; p++) {
-            bary(A, B, C, perm[p][0], perm[p][1], perm[p][2], Q);
-            put_node(nodes, slot++, Q, kT33w[o]);
-        }
+    triangle_rule_table(t);
+    for (int slot = 0; slot < KEM_N33 + 33; slot++) {
+        const double* r = t + 4 * slot;
+        bary(A, B, C, r[0], r[1], r[2], Q);
+        put_node(nodes, slot, Q, r[3]);
     }
 }
 
@@ -225,6 +197,44 @@ void set_thresholds(int type, double* g)
 }
 
 }  // namespace
+
+void triangle_rule_table(double* t)
+{
+    auto put = [&](int slot, double a, double b, double c, double w) {
+        t[4 * slot + 0] = a;
+        t[4 * slot + 1] = b;
+        t[4 * slot + 2] = c;
+        t[4 * slot + 3] = w;
+    };
+    // 7-point: centroid, then two orbits (alpha,beta,beta) in the order (a,b,c),(b,a,c),(c,b,a)
+    const double th = 1. / 3., q15 = std::sqrt(15), r = (1. + q15) / 7., s = (1. - q15) / 7.;
+    const double a7[3] = {th, th + (2. * th * s), th + (2. * th * r)};
+    const double b7[3] = {th, th - (th * s), th - (th * r)};
+    const double w7[3] = {9 / 40., (155. + q15) / 1200., (155. - q15) / 1200.};
+    int slot = KEM_N7;
+    put(slot++, a7[0], b7[0], b7[0], w7[0]);
+    for (int o = 1; o < 3; o++) {
+        put(slot++, a7[o], b7[o], b7[o], w7[o]);
+        put(slot++, b7[o], a7[o], b7[o], w7[o]);
+        put(slot++, b7[o], b7[o], a7[o], w7[o]);
+    }
+    // 12-point: cyclic orbits (a,b,c),(b,c,a),(c,a,b), weights doubled
+    slot = KEM_N12;
+    for (int o = 0; o < 4; o++) {
+        const double a = kT12a[o], b = kT12b[o], c = kT12c[o];
+        put(slot++, a, b, c, kT12w[o] * 2.);
+        put(slot++, b, c, a, kT12w[o] * 2.);
+        put(slot++, c, a, b, kT12w[o] * 2.);
+    }
+    // 33-point: five 3-orbits and three 6-orbits
+    slot = KEM_N33;
+    for (int o = 0; o < 8; o++) {
+        const double a = kT33a[o], b = kT33b[o], c = 1. - kT33a[o] - kT33b[o];
+        const double perm[6][3] = {{a, b, c}, {b, a, c}, {c, b, a}, {a, c, b}, {c, a, b}, {b, c, a}};
+        const int np = o < 5 ? 3 : 6;
+        for (int p = 0; p < np; p++) put(slot++, perm[p][0], perm[p][1], perm[p][2], kT33w[o]);
+    }
+}
 
 int prep_element(int type, const double* d, double* g, double* nodes)
 {

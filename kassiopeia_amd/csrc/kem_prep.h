@@ -17,6 +17,11 @@ constexpr double kOneOverFourPiEps0 = 1. / (4. * kPi * kEps0);
 // nodes[KEM_NODE_STRIDE].  Returns 0, or -1 for a degenerate / non-finite element.
 int prep_element(int type, const double* shape_data, double* g, double* nodes);
 
+// Barycentric coordinates and weights of the triangle rules in node-slot order (7-, 12-,
+// 33-point at KEM_N7 / KEM_N12 / KEM_N33): t[4*slot .. 4*slot+3] = (a, b, c, w), the node
+// being a*P0 + b*P1 + c*P2 (GaussPoints_Tri7P/12P/33P, KElectrostaticCubatureTriangleIntegrator.cc:34-270).
+void triangle_rule_table(double* t);
+
 // FP32 node table for the mixed-precision far field: node positions relative to the
 // element's dispatch centre (KG_CX..KG_CZ of g) and weights, KEM_NODE_STRIDE floats.
 void prep_nodes32(const double* g, const double* nodes, float* nodes32);

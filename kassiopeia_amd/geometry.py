@@ -281,6 +281,38 @@ def ring_stack_mesh(nrings: int = 8, r_min: float = 0.2, r_max: float = 1.0, dz:
     return concat(*parts)
 
 
+def ring_stack_indexed(nrings: int = 8, r_min: float = 0.2, r_max: float = 1.0, dz: float = 0.25,
+                       width: float = 0.1, n_target: int = 5000):
+    """The ring stack of ring_stack_mesh as a shared-vertex mesh: (X (nv, 3) vertex positions,
+    tri (ne, 3) vertex indices in SetValues order (p0, p1, p2), electrode (ne,) ring index).
+    Each annulus is an (nr+1) x nphi vertex grid wrapping around in phi, two triangles per cell
+    with normals along +z."""
+    radii = np.linspace(r_min, r_max, nrings)
+    areas = np.pi * ((radii + width) ** 2 - radii ** 2)
+    per = np.maximum(2 * 8 * 2, np.round(n_target * areas / areas.sum())).astype(int)
+    zs = (np.arange(nrings) - (nrings - 1) / 2.0) * dz
+    X, tris, elec = [], [], []
+    base = 0
+    for k in range(nrings):
+        nr = 2
+        nphi = max(8, int(round(per[k] / (2 * nr))))
+        r = np.linspace(radii[k], radii[k] + width, nr + 1)
+        ph = 2.0 * np.pi * np.arange(nphi) / nphi
+        R, PH = np.meshgrid(r, ph, indexing="ij")
+        X.append(np.stack([R * np.cos(PH), R * np.sin(PH), np.full_like(R, zs[k])], -1).reshape(-1, 3))
+        ir, ip = np.meshgrid(np.arange(nr), np.arange(nphi), indexing="ij")
+        ir, ip = ir.reshape(-1), ip.reshape(-1)
+        a = base + ir * nphi + ip
+        b = base + (ir + 1) * nphi + ip
+        c = base + (ir + 1) * nphi + (ip + 1) % nphi
+        d = base + ir * nphi + (ip + 1) % nphi
+        t = np.stack([np.stack([a, b, c], 1), np.stack([c, d, a], 1)], 1).reshape(-1, 3)
+        tris.append(t)
+        elec.append(np.full(t.shape[0], k))
+        base += (nr + 1) * nphi
+    return np.concatenate(X), np.concatenate(tris), np.concatenate(elec)
+
+
 def ring_bore_mesh(nrings: int = 32, r_ring: float = 1.9, ring_width: float = 0.05,
                    z_span: float = 8.0, r_outer: float = 2.0, n_target: int = 100000, volts=None,
                    seed: int = 99) -> Mesh:

@@ -17,6 +17,7 @@ exception type for them); HIP failures raise ``KemError``.
 from __future__ import annotations
 
 import ctypes
+import weakref
 
 import numpy as np
 
@@ -50,6 +51,12 @@ class Context:
             raise _lib.KemError(rc, "kem_ctx_create failed (no HIP device?)")
         self.h = h
         self.device = device
+        # objects holding library handles that depend on this context; closed before it
+        self._deps = weakref.WeakSet()
+
+    def track(self, obj):
+        self._deps.add(obj)
+        return obj
 
     def check(self, rc):
         if rc < 0:
@@ -70,6 +77,8 @@ class Context:
 
     def close(self):
         if getattr(self, "h", None):
+            for d in list(self._deps):
+                d.close()
             self.lib.kem_ctx_destroy(self.h)
             self.h = None
 
@@ -95,6 +104,7 @@ class KSurfaceContainer:
         ctx.check(ctx.lib.kem_mesh_create(ctx.h, mesh.n, _ptr(shapes), _ptr(recs), _ptr(bc), _ptr(val), _ptr(flux),
                                           ctypes.byref(h)))
         self.h = h
+        ctx.track(self)
 
     def size(self):
         return int(self.ctx.lib.kem_mesh_size(self.h))
@@ -110,9 +120,9 @@ class KSurfaceContainer:
         self.mesh = self.mesh.__class__(**{**self.mesh.__dict__, "bc_value": v})
 
     def close(self):
-        if getattr(self, "h", None):
+        if getattr(self, "h", None) and self.ctx.h:
             self.ctx.lib.kem_mesh_destroy(self.h)
-            self.h = None
+        self.h = None
 
     def __del__(self):
         try:
@@ -157,6 +167,7 @@ class KBoundaryIntegralMatrix:
         self.ctx.check(self.ctx.lib.kem_matrix_assemble(self.ctx.h, container.h, storage, ctypes.byref(h)))
         self.h = h
         self.n = container.mesh.n
+        self.ctx.track(self)
 
     def Dimension(self):
         return int(self.ctx.lib.kem_matrix_dimension(self.h))
@@ -235,9 +246,9 @@ class KBoundaryIntegralMatrix:
         return x, st, rc
 
     def close(self):
-        if getattr(self, "h", None):
+        if getattr(self, "h", None) and self.ctx.h:
             self.ctx.lib.kem_matrix_destroy(self.h)
-            self.h = None
+        self.h = None
 
     def __del__(self):
         try:
@@ -281,6 +292,7 @@ class DeviceArray:
         self.host = np.ascontiguousarray(host)
         self.ptr = _P()
         ctx.check(ctx.lib.kem_dev_alloc(ctx.h, self.host.nbytes, ctypes.byref(self.ptr)))
+        ctx.track(self)
         ctx.check(ctx.lib.kem_dev_upload(ctx.h, self.ptr, self.host.ctypes.data_as(_P), self.host.nbytes))
 
     @classmethod
@@ -298,9 +310,9 @@ class DeviceArray:
         self.ctx.check(self.ctx.lib.kem_dev_upload(self.ctx.h, self.ptr, host.ctypes.data_as(_P), host.nbytes))
 
     def close(self):
-        if self.ptr:
+        if self.ptr and self.ctx.h:
             self.ctx.lib.kem_dev_free(self.ctx.h, self.ptr)
-            self.ptr = _P()
+        self.ptr = _P()
 
     def __del__(self):
         try:
@@ -320,11 +332,15 @@ def field_sets(container: "KSurfaceContainer", sigma_sets, set_index, points, pr
     m = P.shape[0]
     O = [DeviceArray.empty(ctx, m) for _ in range(4)]
     prec = {"fp64": _lib.KEM_FP64, "mixed": _lib.KEM_MIXED}[precision]
-    ctx.check(ctx.lib.kem_field_eval_sets(ctx.h, container.h, S.host.shape[0], S.ptr, idx.ptr, m, X[0].ptr,
-                                          X[1].ptr, X[2].ptr, prec, O[0].ptr, O[1].ptr, O[2].ptr, O[3].ptr))
-    ctx.check(ctx.lib.kem_ctx_sync(ctx.h))
-    E = np.stack([O[k].get() for k in (1, 2, 3)], 1)
-    return E, O[0].get()
+    try:
+        ctx.check(ctx.lib.kem_field_eval_sets(ctx.h, container.h, S.host.shape[0], S.ptr, idx.ptr, m, X[0].ptr,
+                                              X[1].ptr, X[2].ptr, prec, O[0].ptr, O[1].ptr, O[2].ptr, O[3].ptr))
+        ctx.check(ctx.lib.kem_ctx_sync(ctx.h))
+        E = np.stack([O[k].get() for k in (1, 2, 3)], 1)
+        return E, O[0].get()
+    finally:
+        for a in [S, idx] + X + O:
+            a.close()
 
 
 class KGaussianEliminationChargeDensitySolver:
