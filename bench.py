@@ -33,8 +33,8 @@ sys.path.insert(0, ROOT)
 # FLOPs per (point, element) pair for field + potential, by regime (add/sub/mul = 1, FMA = 2,
 # sqrt/div/rsqrt/log/atan2 = 1): cubature 14 + 21 n_q; closed form 260 (convention);
 # segments 14 + 21 n_nodes, analytic 100.  Census order: triangle 7/12/33/closed,
-# rectangle 7/12/33/closed, segment 4/16/analytic.
-FLOPS_PER_PAIR = np.array([161, 266, 707, 260, 161, 266, 707, 260, 98, 350, 100], dtype=np.float64)
+# rectangle 7/12/33/closed, segment 4/16/analytic, triangle 6-point (FP32 ladder).
+FLOPS_PER_PAIR = np.array([161, 266, 707, 260, 161, 266, 707, 260, 98, 350, 100, 140], dtype=np.float64)
 PEAK_FP64_TFLOPS = 78.6  # MI355X FP64 vector peak
 PEAK_FP32_TFLOPS = 157.3  # MI355X FP32 vector peak (packed)
 PEAK_HBM_GBS = 8000.0
@@ -50,16 +50,22 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--points", type=int, default=1_000_000)
-    ap.add_argument("--cpu-points", type=int, default=1200,
-                    help="points of the bounded CPU-baseline sample (x 20,000 elements)")
+    ap.add_argument("--cpu-points", type=int, default=0,
+                    help="points of the bounded CPU-baseline sample (x 20,000 elements); 0: 1500 per thread")
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="OpenMP threads of the CPU baseline; 0: the host's cores, at most 16")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the multi-right-hand-side section")
     ap.add_argument("--batch-points", type=int, default=4_000_000)
+    ap.add_argument("--workload", choices=["coax20k", "large"], default="coax20k",
+                    help="large: 100,000-triangle ring bore, 16 voltage sets, 10 M points, adjoint")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "large":
+        return run_large(args)
 
     from kassiopeia_amd import _lib
     from kassiopeia_amd import geometry as G
@@ -114,7 +120,7 @@ def main():
                                             outs[0].ptr, outs[1].ptr, outs[2].ptr, outs[3].ptr))
 
     def census(precision):
-        c = (ctypes.c_int64 * 11)()
+        c = (ctypes.c_int64 * 12)()
         ctx.check(lib.kem_field_census(ctx.h, cont.h, M, X[0].ptr, X[1].ptr, X[2].ptr, precision, c))
         return np.array(c[:], dtype=np.float64)
 
@@ -140,12 +146,12 @@ def main():
     cmx = census(_lib.KEM_MIXED)
     flops64 = float((c64 * FLOPS_PER_PAIR).sum())
     # the far-field kernel evaluates every pair but the closed-form ones
-    far = np.ones(11)
+    far = np.ones(12)
     far[[3, 7, 10]] = 0.
     farflops64 = float((c64 * FLOPS_PER_PAIR * far).sum())
     farflopsmx = float((cmx * FLOPS_PER_PAIR * far).sum())
     log(f"regime census fp64 (tri 7/12/33/closed): {c64[:4].astype(int).tolist()}; "
-        f"mixed: {cmx[:4].astype(int).tolist()}")
+        f"mixed: {cmx[:4].astype(int).tolist()} + 6-point {int(cmx[11])}")
     ms64, kms64 = timed(_lib.KEM_FP64, O)
     msmx, kmsmx = timed(_lib.KEM_MIXED, Om)
     # error of the mixed result against the FP64 path: norm-wise over the whole point set, the
@@ -235,19 +241,24 @@ def main():
 
     # ---------------------------------------------------------------- CPU baseline + parity
     cpu = None
-    if not args.no_cpu_baseline and args.cpu_points > 0:
+    if not args.no_cpu_baseline:
         from oracle import oracle as OR
 
-        k = min(args.cpu_points, M)
+        threads = args.cpu_threads or max(1, min(16, os.cpu_count() or 1))
+        k = min(args.cpu_points or 1500 * threads, M)
         t0 = time.perf_counter()
-        phr, Er = OR.field(mesh.shapes, mesh.recs, sigma, pts[:k])
+        phr, Er = OR.field(mesh.shapes, mesh.recs, sigma, pts[:k], threads=threads)
         tc = time.perf_counter() - t0
-        cpu = {"value": k * N / tc, "unit": "point-element pairs/s", "cores": 1, "kind": "port",
-               "sample": f"{k} points x {N} triangles of the same workload, FP64 oracle, 1 thread, {tc:.1f} s"}
-        ep = float(np.max(np.abs(phi[:k] - phr) / np.abs(phr)))
-        ee = float(np.max(np.linalg.norm(E[:k] - Er, axis=1) / np.linalg.norm(Er, axis=1)))
-        out.update(parity_phi_vs_oracle=ep, parity_E_vs_oracle=ee)
-        log(f"cpu baseline {cpu['value']:.3e} pairs/s; parity phi {ep:.2e} E {ee:.2e}")
+        cpu = {"value": k * N / tc, "unit": "point-element pairs/s", "cores": threads, "kind": "port",
+               "sample": f"{k} points x {N} triangles of the same workload, FP64 oracle (the reference's "
+                         f"integrators restated in C), {threads} OpenMP threads, {tc:.1f} s"}
+        # normalised by the largest |phi| and |E| of the sample (pointwise ratios are amplified
+        # where phi or E nearly vanish)
+        ep = float(np.max(np.abs(phi[:k] - phr)) / np.max(np.abs(phr)))
+        ee = float(np.max(np.linalg.norm(E[:k] - Er, axis=1)) / np.max(np.linalg.norm(Er, axis=1)))
+        out.update(parity_phi_vs_oracle=ep, parity_E_vs_oracle=ee,
+                   gpu_mixed_vs_cpu_speedup=(pairs / msmx * 1e3) / cpu["value"])
+        log(f"cpu baseline {cpu['value']:.3e} pairs/s on {threads} threads; parity phi {ep:.2e} E {ee:.2e}")
 
     line = {
         "metric": "point-element field evals/s + solve time, mixed vs FP64 at 1e-6 rel. error",
@@ -282,6 +293,127 @@ def main():
         "cpu_baseline": cpu,
     }
     line.update(out)
+    print(json.dumps(line), flush=True)
+
+
+def run_large(args):
+    """100,000-triangle case: 32 coaxial ring electrodes (r = 1.9 m) inside a grounded r = 2 m
+    cylinder, voltages U[-20, 0] kV; FP64 pipeline (FP64 matrix, FP64 GMRES, FP64 field) against
+    the mixed pipeline (FP32 + FP64 matrices, FP32 GMRES + FP64 refinement, FP32 far field) on
+    10 M points uniform in the bore (r < 1.8 m, |z| < 3.5 m); 16 voltage sets in lockstep; the
+    voltage + vertex adjoint of a 10k-point axial potential loss."""
+    from kassiopeia_amd import _lib
+    from kassiopeia_amd import geometry as G
+    from kassiopeia_amd import kemfield as K
+
+    rng = np.random.default_rng(20)
+    volts = rng.uniform(-20000.0, 0.0, 32)
+    mesh = G.ring_bore_mesh(volts=volts)
+    N = mesh.n
+    M = 10_000_000 if args.points == 1_000_000 else args.points
+    pts = G.points_annulus(M, 0.0, 1.8, 3.5, seed=55)
+    log(f"large: mesh {N} triangles, {M} points")
+    ctx = K.Context(0)
+    lib = ctx.lib
+    cont = K.KSurfaceContainer(ctx, mesh)
+    b = cont.rhs()
+
+    def sync():
+        ctx.check(lib.kem_ctx_sync(ctx.h))
+
+    out = {}
+    X = [K.DeviceArray(ctx, pts[:, k]) for k in range(3)]
+    O = [K.DeviceArray.empty(ctx, M) for _ in range(4)]
+
+    def field(sig, prec):
+        S = K.DeviceArray(ctx, sig)
+        sync()
+        t0 = time.perf_counter()
+        ctx.check(lib.kem_field_eval_device(ctx.h, cont.h, S.ptr, M, X[0].ptr, X[1].ptr, X[2].ptr, prec, O[0].ptr,
+                                            O[1].ptr, O[2].ptr, O[3].ptr))
+        sync()
+        t = time.perf_counter() - t0
+        S.close()
+        return t, O[0].get(), np.stack([O[k].get() for k in (1, 2, 3)], 1)
+
+    # FP64 pipeline
+    t0 = time.perf_counter()
+    A = K.KBoundaryIntegralMatrix(cont, fp64=True, fp32=False)
+    sync()
+    t_asm64 = time.perf_counter() - t0
+    x64, st64, _ = A.solve(b, tolerance=1e-10)
+    A.close()
+    log(f"FP64: assembly {t_asm64:.2f} s, GMRES {st64.iterations} it {st64.seconds:.2f} s")
+    t_f64, phi64, E64 = field(x64, _lib.KEM_FP64)
+    log(f"FP64 field {t_f64:.2f} s = {N * M / t_f64:.3e} pairs/s")
+    # mixed pipeline
+    t0 = time.perf_counter()
+    A = K.KBoundaryIntegralMatrix(cont, fp64=True, fp32=True)
+    sync()
+    t_asm = time.perf_counter() - t0
+    xmx, stmx, _ = A.solve(b, tolerance=1e-10, precision="mixed")
+    log(f"mixed: assembly {t_asm:.2f} s, GMRES {stmx.iterations} it / {stmx.refinements} ref {stmx.seconds:.2f} s")
+    t_fmx, phimx, Emx = field(xmx, _lib.KEM_MIXED)
+    log(f"mixed field {t_fmx:.2f} s = {N * M / t_fmx:.3e} pairs/s")
+    dE = Emx - E64
+    err_E_l2 = float(np.linalg.norm(dE) / np.linalg.norm(E64))
+    err_E_rms = float(np.linalg.norm(dE, axis=1).max() / np.sqrt((E64 ** 2).sum(1).mean()))
+    err_p = float(np.abs(phimx - phi64).max() / np.abs(volts).max())
+    t_plain = t_asm64 + st64.seconds + t_f64
+    t_mixed = t_asm + stmx.seconds + t_fmx
+    log(f"mixed vs FP64: |dE| l2 {err_E_l2:.2e}, max|dE|/rms|E| {err_E_rms:.2e}, |dphi|/V {err_p:.2e}; "
+        f"pipeline {t_plain:.2f} s -> {t_mixed:.2f} s = {t_plain / t_mixed:.2f}x")
+    # 16 voltage sets
+    groups = mesh.group
+    V16 = rng.uniform(-20000.0, 0.0, size=(33, 16))
+    V16[32, :] = 0.0
+    Xb, res, stb, _ = A.solve_batch(V16[groups, :], tolerance=1e-10, precision="mixed")
+    log(f"16 voltage sets: {stb.seconds:.2f} s ({stb.iterations} it, {stb.refinements} ref, worst {res.max():.1e})")
+    # adjoint of L = mean (phi(axis) - target)^2 over 10k axial points (element-corner gradients)
+    zax = np.linspace(-3.5, 3.5, 10000)
+    pax = np.ascontiguousarray(np.stack([0 * zax, 0 * zax, zax], 1))
+    ctx.set_integrator("numeric")
+    Eax, phax = K.KIntegratingElectrostaticFieldSolver(cont, x64).ElectricFieldAndPotential(pax)
+    g = 2.0 * (phax - (-10000.0)) / phax.size
+    sync()
+    t0 = time.perf_counter()
+    dsig, dvf = np.zeros(N), np.zeros((N, 9))
+    ctx.check(lib.kem_field_vjp(ctx.h, cont.h, K._ptr(x64), pax.shape[0], K._ptr(pax), K._ptr(g), K._ptr(dsig),
+                                K._ptr(dvf)))
+    t_fvjp = time.perf_counter() - t0
+    o = _lib.SolveOpts()
+    lib.kem_solve_opts_default(ctypes.byref(o))
+    o.tolerance, o.precision = 1e-10, _lib.KEM_MIXED
+    lam = np.zeros(N)
+    sta = _lib.SolveStats()
+    ctx.check(lib.kem_solve_transposed(A.h, ctypes.byref(o), K._ptr(dsig), K._ptr(lam), ctypes.byref(sta)))
+    t0 = time.perf_counter()
+    dvm = np.zeros((N, 9))
+    ctx.check(lib.kem_assemble_vjp(ctx.h, cont.h, K._ptr(lam), K._ptr(x64), K._ptr(dvm)))
+    t_mvjp = time.perf_counter() - t0
+    dV = np.bincount(groups, weights=lam, minlength=33)
+    log(f"adjoint: field VJP {t_fvjp:.2f} s, transposed solve {sta.seconds:.2f} s ({sta.iterations} it), "
+        f"matrix VJP {t_mvjp:.2f} s; |dL/dV| max {np.abs(dV).max():.3e}")
+    A.close()
+    for arr in X + O:
+        arr.close()
+    value = N * M / t_fmx
+    line = {
+        "metric": "point-element field evals/s + solve time, mixed vs FP64 at 1e-6 rel. error",
+        "value": value, "unit": "point-element pairs/s", "n_gpus": 1, "steps": 1, "warmup": 0,
+        "ms_per_step": t_fmx * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32+f64", "data": "synthetic",
+        "config": {"workload": f"32 ring electrodes + grounded cylinder, {N} triangles, 16 voltage sets, "
+                               f"{M} points in the bore; value = mixed-precision field pass", "elements": N,
+                   "points": M},
+        "assembly_s_fp64_only": t_asm64, "assembly_s_fp64_fp32": t_asm, "solve_s_fp64": st64.seconds,
+        "solve_s_mixed": stmx.seconds, "gmres_iterations_fp64": st64.iterations,
+        "gmres_iterations_mixed": stmx.iterations, "field_s_fp64": t_f64, "field_s_mixed": t_fmx,
+        "pipeline_s_fp64": t_plain, "pipeline_s_mixed": t_mixed, "pipeline_speedup": t_plain / t_mixed,
+        "mixed_err_E_l2": err_E_l2, "mixed_err_E_max_over_rms": err_E_rms, "mixed_err_phi_max": err_p,
+        "batch16_solve_s": stb.seconds, "batch16_worst_residual": float(res.max()),
+        "adjoint_field_vjp_s": t_fvjp, "adjoint_solve_s": sta.seconds, "adjoint_matrix_vjp_s": t_mvjp,
+    }
     print(json.dumps(line), flush=True)
 
 

@@ -155,8 +155,9 @@ int kem_field_eval_sets(kem_ctx* ctx, const kem_mesh* mesh, int64_t nsets, const
 
 /* Instrumentation (no reference counterpart): number of (point, element) pairs of a field
    evaluation per quadrature regime of the numeric policy at the given precision (KEM_MIXED
-   uses the FP32 far-field rule ladder), counts[11] = triangle 7/12/33-point/closed form,
-   rectangle 7/12/33/closed form, segment 4-node/16-node/analytic.  Device points. */
+   uses the FP32 far-field rule ladder), counts[12] = triangle 7/12/33-point/closed form,
+   rectangle 7/12/33/closed form, segment 4-node/16-node/analytic, triangle 6-point (FP32
+   ladder only).  Device points. */
 int kem_field_census(kem_ctx* ctx, const kem_mesh* mesh, int64_t npts, const double* d_x, const double* d_y,
                      const double* d_z, int precision, int64_t* counts);
 

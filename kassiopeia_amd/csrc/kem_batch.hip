@@ -48,7 +48,25 @@ __global__ __launch_bounds__(kBlock) void k_bmm32(const float* __restrict__ A, i
             acc0[t][v] = 0.f;
             acc1[t][v] = 0.f;
         }
-    for (int64_t j = j0; j < j1; j += 2) {
+    // KB column pairs are loaded before their MFMAs so that the loads of a batch are in flight
+    // together (the kernel is otherwise bound by one memory latency per column pair)
+    constexpr int KB = 8;
+    int64_t j = j0;
+    for (; j + 2 * KB <= j1; j += 2 * KB) {
+        float a0[KB], a1[KB], b[KB][NT];
+        for (int q = 0; q < KB; q++) {
+            const int64_t js = j + 2 * q + lk;
+            a0[q] = A[js * n + ra];
+            a1[q] = A[js * n + rb];
+            for (int t = 0; t < NT; t++) b[q][t] = Vt[js * KP + 32 * t + lr];
+        }
+        for (int q = 0; q < KB; q++)
+            for (int t = 0; t < NT; t++) {
+                acc0[t] = __builtin_amdgcn_mfma_f32_32x32x2f32(a0[q], b[q][t], acc0[t], 0, 0, 0);
+                acc1[t] = __builtin_amdgcn_mfma_f32_32x32x2f32(a1[q], b[q][t], acc1[t], 0, 0, 0);
+            }
+    }
+    for (; j < j1; j += 2) {
         const int64_t jj = j + lk;
         const bool ok = jj < j1;
         const int64_t js = ok ? jj : j;  // j < j1: a valid column

@@ -299,7 +299,7 @@ int kem_mesh_create(kem_ctx* c, int64_t n, const int32_t* shape, const double* d
     if (n <= 0 || !shape || !data) return fail(c, KEM_E_ARG, "mesh: empty or null arrays");
     KEM_CK(c, enter(c));
     std::vector<double> g((size_t)n * KG_STRIDE), nd((size_t)n * KEM_NODE_STRIDE), t((size_t)n * KT_STRIDE);
-    std::vector<float> nd32((size_t)n * KEM_NODE_STRIDE);
+    std::vector<float> nd32((size_t)n * KEM_NODE32_STRIDE);
     kem_mesh* m = new kem_mesh();
     m->ctx = c;
     m->n = n;
@@ -323,7 +323,7 @@ int kem_mesh_create(kem_ctx* c, int64_t n, const int32_t* shape, const double* d
         }
         kem::prep_target(shape[i], data + 11 * i, m->bc[i], m->flux[i], t.data() + (size_t)i * KT_STRIDE);
         kem::prep_nodes32(g.data() + (size_t)i * KG_STRIDE, nd.data() + (size_t)i * KEM_NODE_STRIDE,
-                          nd32.data() + (size_t)i * KEM_NODE_STRIDE);
+                          nd32.data() + (size_t)i * KEM_NODE32_STRIDE);
     }
     std::vector<int32_t> order((size_t)n);
     std::vector<double> clusters((size_t)(4 * ((n + KEM_CLUSTER - 1) / KEM_CLUSTER)));
@@ -502,7 +502,7 @@ int kem_field_census(kem_ctx* c, const kem_mesh* m, int64_t npts, const double* 
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     dfree(d);
     if (e != hipSuccess) return fail(c, KEM_E_HIP, std::string("census: ") + hipGetErrorString(e));
-    for (int k = 0; k < 11; k++) counts[k] = (int64_t)h[k];
+    for (int k = 0; k < 12; k++) counts[k] = (int64_t)h[k];
     return KEM_OK;
 }
 

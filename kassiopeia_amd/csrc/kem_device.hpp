@@ -404,9 +404,10 @@ __device__ __forceinline__ F4 element_near(const double* __restrict__ g, int typ
 // The reference's cut-offs keep every pair at machine precision; at FP32 accuracy the
 // truncation error only has to stay well below FP32 rounding (~1e-7).  Measured max relative
 // truncation error (potential and field, 23 triangles / 20 rectangles of aspect <= 4, random
-// directions, against the closed form): triangle 7-point 1.1e-8 at dr = 8, 12-point 8.4e-9 at
-// dr = 4, 33-point < 1e-10 at dr >= 3.42; rectangle 7-point 1e-8 at dr = 12, 12-point 4.4e-9
-// at dr = 6.  Near pairs (the same closed-form set as the FP64 path) stay FP64.
+// directions, against the closed form): triangle 6-point (degree 4) 1.5e-8 at dr = 12,
+// 7-point 1.1e-8 at dr = 8, 12-point 8.4e-9 at dr = 4, 33-point < 1e-10 at dr >= 3.42;
+// rectangle 7-point 1e-8 at dr = 12, 12-point 4.4e-9 at dr = 6.  Near pairs (the same
+// closed-form set as the FP64 path) stay FP64.
 
 // FP32 node sums relative to the element centre: (dx, dy, dz) = P - c in FP32 (computed in
 // FP64 and rounded once), nodes q = Q - c in FP32, so P - Q = (P - c) - q carries the relative
@@ -443,6 +444,7 @@ __device__ __forceinline__ void element_far32(const double* __restrict__ g, cons
         if (r2 > g[KG_MIX7_2]) return cubature32<4>(nd + 4 * KEM_L4, dx, dy, dz, s0, s1, s2, s3);
         return cubature32<16>(nd + 4 * KEM_L16, dx, dy, dz, s0, s1, s2, s3);
     }
+    if (r2 > g[KG_MIX6_2]) return cubature32<6>(nd + 4 * KEM_N6, dx, dy, dz, s0, s1, s2, s3);
     if (r2 > g[KG_MIX7_2]) return cubature32<7>(nd + 4 * KEM_N7, dx, dy, dz, s0, s1, s2, s3);
     if (r2 > g[KG_MIX12_2]) return cubature32<12>(nd + 4 * KEM_N12, dx, dy, dz, s0, s1, s2, s3);
     return cubature32<33>(nd + 4 * KEM_N33, dx, dy, dz, s0, s1, s2, s3);

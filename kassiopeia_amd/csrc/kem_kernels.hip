@@ -103,7 +103,7 @@ __global__ __launch_bounds__(kBlock) void k_field_far32(const double* __restrict
         if (band2 == 0. || r2 * (g[KG_INVSIZE] * g[KG_INVSIZE]) > band2) {
             const float dx = (float)(x - g[KG_CX]), dy = (float)(y - g[KG_CY]), dz = (float)(z - g[KG_CZ]);
             float s0, s1, s2, s3;
-            dev::element_far32(g, nodes32 + j * KEM_NODE_STRIDE, type, r2, dx, dy, dz, s0, s1, s2, s3);
+            dev::element_far32(g, nodes32 + j * KEM_NODE32_STRIDE, type, r2, dx, dy, dz, s0, s1, s2, s3);
             const double wj = g[KG_PREF] * sg[j];
             a0 += (double)s0 * wj;
             a1 += (double)s1 * wj;
@@ -549,7 +549,8 @@ __global__ __launch_bounds__(kBlock) void k_check_sets(const int32_t* __restrict
 // Counts the (point, element) pairs of a field evaluation by quadrature regime with the field
 // kernels' own classification (numeric policy; mixed = 1: the FP32 ladder beyond distance
 // ratio fp64_ratio): counts[0..3] triangle 7/12/33-point/closed form, [4..7] rectangle,
-// [8..10] segment 4-node/16-node/analytic.  Used to price a launch in FLOPs.
+// [8..10] segment 4-node/16-node/analytic, [11] triangle 6-point (FP32 ladder only).  Used to
+// price a launch in FLOPs.
 __global__ __launch_bounds__(kBlock) void k_census(const double* __restrict__ geom, int64_t nelem, int64_t per_split,
                                                    const double* __restrict__ px, const double* __restrict__ py,
                                                    const double* __restrict__ pz, int64_t npts, int mixed,
@@ -579,16 +580,18 @@ __global__ __launch_bounds__(kBlock) void k_census(const double* __restrict__ ge
                 const double t12 = f32 ? g[KG_MIX12_2] : g[KG_REF12_2];
                 if (type == KEM_LINESEGMENT)
                     k = r2 > t7 ? 0 : 1;
+                else if (f32 && r2 > g[KG_MIX6_2])
+                    k = 11 - base;
                 else
                     k = r2 > t7 ? 0 : r2 > t12 ? 1 : 2;
             }
             c[base + k]++;
         }
     }
-    for (int k = 0; k < 11; k++)
+    for (int k = 0; k < 12; k++)
         if (c[k]) atomicAdd(&sc[k], c[k]);
     __syncthreads();
-    if (threadIdx.x < 11 && sc[threadIdx.x]) atomicAdd(&counts[threadIdx.x], sc[threadIdx.x]);
+    if (threadIdx.x < 12 && sc[threadIdx.x]) atomicAdd(&counts[threadIdx.x], sc[threadIdx.x]);
 }
 
 inline unsigned nblocks(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }

@@ -51,6 +51,7 @@ enum kem_geom_field {
     KG_REF12_2 = 82,
     KG_MIX7_2 = 83,
     KG_MIX12_2 = 84,
+    KG_MIX6_2 = 85, /* mixed ladder: triangle 6-point rule beyond this (no other shape) */
     KG_STRIDE = 88
 };
 
@@ -64,6 +65,10 @@ enum kem_geom_field {
 #define KEM_N33 19
 #define KEM_L4 0
 #define KEM_L16 4
+/* the FP32 node table (mixed far field) holds 6 more slots: a degree-4 6-point triangle rule */
+#define KEM_N6 52
+#define KEM_NODE32_SLOTS 58
+#define KEM_NODE32_STRIDE (4 * KEM_NODE32_SLOTS)
 
 /* element clusters for the near pass: elements in Morton order of their centres, KEM_CLUSTER
    per cluster; each cluster keeps (centre x, y, z, radius) with radius = max over its elements

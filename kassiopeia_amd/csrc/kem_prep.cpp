@@ -181,18 +181,21 @@ void set_thresholds(int type, double* g)
         g[KG_REF12_2] = t2(30.4);
         g[KG_MIX7_2] = t2(8.0);
         g[KG_MIX12_2] = t2(4.0);
+        g[KG_MIX6_2] = t2(12.0);
     } else if (type == KEM_RECTANGLE) {
         g[KG_NEAR2] = t2(4.6);
         g[KG_REF7_2] = t2(196.2);
         g[KG_REF12_2] = t2(44.6);
         g[KG_MIX7_2] = t2(12.0);
         g[KG_MIX12_2] = t2(6.0);
+        g[KG_MIX6_2] = INFINITY;
     } else {
         g[KG_NEAR2] = t2(2.0);
         g[KG_REF7_2] = t2(27.7);
         g[KG_REF12_2] = t2(27.7);
         g[KG_MIX7_2] = t2(27.7);
         g[KG_MIX12_2] = t2(27.7);
+        g[KG_MIX6_2] = INFINITY;
     }
 }
 
@@ -368,6 +371,26 @@ void prep_nodes32(const double* g, const double* nodes, float* nodes32)
         o[1] = (float)(q[1] - g[KG_CY]);
         o[2] = (float)(q[2] - g[KG_CZ]);
         o[3] = (float)q[3];
+    }
+    // degree-4 6-point rule (Dunavant 1985, positive weights) for the far end of the mixed
+    // ladder: orbits (1-2a, a, a), a = 0.445948490915965 (w 0.223381589678011) and
+    // a = 0.091576213509771 (w 0.109951743655322)
+    for (int s = KEM_N6; s < KEM_NODE32_SLOTS; s++)
+        for (int k = 0; k < 4; k++) nodes32[4 * s + k] = 0.f;
+    if ((int)g[KG_TYPE] != KEM_TRIANGLE) return;
+    const double a[2] = {0.445948490915965, 0.091576213509771};
+    const double w[2] = {0.223381589678011, 0.109951743655322};
+    int s = KEM_N6;
+    for (int o = 0; o < 2; o++) {
+        const double b[3][3] = {{1. - 2. * a[o], a[o], a[o]}, {a[o], 1. - 2. * a[o], a[o]}, {a[o], a[o], 1. - 2. * a[o]}};
+        for (int p = 0; p < 3; p++, s++) {
+            float* out = nodes32 + 4 * s;
+            for (int k = 0; k < 3; k++) {
+                const double q = b[p][0] * g[KG_V + k] + b[p][1] * g[KG_V + 3 + k] + b[p][2] * g[KG_V + 6 + k];
+                out[k] = (float)(q - g[KG_CX + k]);
+            }
+            out[3] = (float)w[o];
+        }
     }
 }
 

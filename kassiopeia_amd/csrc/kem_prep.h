@@ -23,7 +23,8 @@ int prep_element(int type, const double* shape_data, double* g, double* nodes);
 void triangle_rule_table(double* t);
 
 // FP32 node table for the mixed-precision far field: node positions relative to the
-// element's dispatch centre (KG_CX..KG_CZ of g) and weights, KEM_NODE_STRIDE floats.
+// element's dispatch centre (KG_CX..KG_CZ of g) and weights, KEM_NODE32_STRIDE floats (the
+// 52 slots of the FP64 table plus a 6-point triangle rule at KEM_N6).
 void prep_nodes32(const double* g, const double* nodes, float* nodes32);
 
 // Collocation target record t[KT_STRIDE]: the shape's Centroid(), unit normal, boundary type

@@ -1334,25 +1334,40 @@ void ko_rhs(int64_t n, const int32_t* bc, const double* value, double* b)
 /* ------------------------------------------------------------ field */
 /* KIntegratingFieldSolver<...,ElectrostaticSingleThread>::ElectricFieldAndPotential
    (KElectrostaticIntegratingFieldSolver.hh:303-339): Kahan sums in container order */
+static void field_point(int policy, int64_t n, const int32_t* shape, const double* recs, const double* sigma,
+                        const double* P, double* phi, double* E)
+{
+    double s[4] = {0., 0., 0., 0.}, c[4] = {0., 0., 0., 0.};
+    for (int64_t j = 0; j < n; j++) {
+        double out[4];
+        ko_element_eval(policy, shape[j], recs + KO_REC * j, P, out);
+        for (int k = 0; k < 4; k++) {
+            double y = out[k] * sigma[j] - c[k];
+            double t = s[k] + y;
+            c[k] = (t - s[k]) - y;
+            s[k] = t;
+        }
+    }
+    if (phi) *phi = s[3];
+    if (E)
+        for (int k = 0; k < 3; k++) E[k] = s[k];
+}
+
 void ko_field(int policy, int64_t n, const int32_t* shape, const double* recs, const double* sigma, int64_t m,
               const double* P, double* phi, double* E)
 {
-    for (int64_t p = 0; p < m; p++) {
-        double s[4] = {0., 0., 0., 0.}, c[4] = {0., 0., 0., 0.};
-        for (int64_t j = 0; j < n; j++) {
-            double out[4];
-            ko_element_eval(policy, shape[j], recs + KO_REC * j, P + 3 * p, out);
-            for (int k = 0; k < 4; k++) {
-                double y = out[k] * sigma[j] - c[k];
-                double t = s[k] + y;
-                c[k] = (t - s[k]) - y;
-                s[k] = t;
-            }
-        }
-        if (phi) phi[p] = s[3];
-        if (E)
-            for (int k = 0; k < 3; k++) E[3 * p + k] = s[k];
-    }
+    for (int64_t p = 0; p < m; p++) field_point(policy, n, shape, recs, sigma, P + 3 * p, phi ? phi + p : 0, E ? E + 3 * p : 0);
+}
+
+void ko_field_mt(int nthreads, int policy, int64_t n, const int32_t* shape, const double* recs, const double* sigma,
+                 int64_t m, const double* P, double* phi, double* E)
+{
+    if (m <= 0) return;
+    /* the first point serially: initialises the lazily built rule tables before the threads */
+    field_point(policy, n, shape, recs, sigma, P, phi ? phi : 0, E ? E : 0);
+#pragma omp parallel for num_threads(nthreads) schedule(dynamic, 8)
+    for (int64_t p = 1; p < m; p++)
+        field_point(policy, n, shape, recs, sigma, P + 3 * p, phi ? phi + p : 0, E ? E + 3 * p : 0);
 }
 
 /* ------------------------------------------------------------ solvers */

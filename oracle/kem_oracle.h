@@ -101,6 +101,9 @@ void ko_rhs(int64_t n, const int32_t* bc, const double* value, double* b);
    Points are AoS (m x 3); outputs phi[m], E[m*3]. */
 void ko_field(int policy, int64_t n, const int32_t* shape, const double* recs, const double* sigma, int64_t m,
               const double* P, double* phi, double* E);
+/* the same over nthreads OpenMP threads (points shared out; every point's sum as above) */
+void ko_field_mt(int nthreads, int policy, int64_t n, const int32_t* shape, const double* recs, const double* sigma,
+                 int64_t m, const double* P, double* phi, double* E);
 
 /* GMRES (unrestarted unless restart>0), MGS + Givens, relative tolerance ||r||/||b||:
    LinearAlgebra/Solvers/include/KGeneralizedMinimalResidual.hh:178-337,

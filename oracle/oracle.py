@@ -61,6 +61,7 @@ def _load():
         "ko_matrix_element": (d, [ctypes.c_int, i64, i64, P, P, P, P]),
         "ko_rhs": (None, [i64, P, P, P]),
         "ko_field": (None, [ctypes.c_int, i64, P, P, P, i64, P, P, P]),
+        "ko_field_mt": (None, [ctypes.c_int, ctypes.c_int, i64, P, P, P, i64, P, P, P]),
         "ko_gmres": (ctypes.c_int, [i64, P, P, P, d, ctypes.c_int, ctypes.c_int, P, P]),
         "ko_gmres_ortho": (ctypes.c_int, [i64, P, P, P, d, ctypes.c_int, ctypes.c_int, ctypes.c_int, P, P]),
         "ko_robin_hood": (i64, [i64, P, P, P, d, i64, i64, P]),
@@ -237,8 +238,9 @@ def rhs(values, bc=None):
     return b
 
 
-def field(shapes, recs, sigma, points, policy=NUMERIC):
-    """Kahan-summed (phi, E) at points (m x 3)."""
+def field(shapes, recs, sigma, points, policy=NUMERIC, threads=1):
+    """Kahan-summed (phi, E) at points (m x 3); threads > 1 shares the points out over OpenMP
+    threads (identical results)."""
     shapes = np.ascontiguousarray(shapes, dtype=np.int32)
     recs = _recs(recs)
     sigma = _f64(sigma)
@@ -246,7 +248,10 @@ def field(shapes, recs, sigma, points, policy=NUMERIC):
     m = pts.shape[0]
     phi = np.zeros(m)
     E = np.zeros((m, 3))
-    lib().ko_field(policy, len(shapes), _p(shapes), _p(recs), _p(sigma), m, _p(pts), _p(phi), _p(E))
+    if threads > 1:
+        lib().ko_field_mt(threads, policy, len(shapes), _p(shapes), _p(recs), _p(sigma), m, _p(pts), _p(phi), _p(E))
+    else:
+        lib().ko_field(policy, len(shapes), _p(shapes), _p(recs), _p(sigma), m, _p(pts), _p(phi), _p(E))
     return phi, E
 
 

@@ -171,3 +171,15 @@ def test_field_divergence_free(oracle, cube6):
         div += (Ep[0, k] - Em[0, k]) / (2 * h)
     _, E0 = oracle.field(m.shapes, m.recs, x, P[None])
     assert abs(div) < 1e-5 * np.linalg.norm(E0) / 0.1
+
+
+def test_field_threads_equal_serial(oracle):
+    """ko_field_mt shares points over OpenMP threads; every point's Kahan sum is unchanged."""
+    from kassiopeia_amd import geometry as G
+
+    m = G.cube_mesh(3)
+    s = np.random.default_rng(4).normal(size=m.n)
+    P = G.points_shell(300, 0.9, 2.0, 3)
+    a = oracle.field(m.shapes, m.recs, s, P)
+    b = oracle.field(m.shapes, m.recs, s, P, threads=4)
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
