@@ -47,6 +47,14 @@ extern "C" {
 #define KEM_MIXED 1 /* FP32 far-field cubature / FP32 matrix; FP64 near field, refinement, sums */
 
 /* ---- matrix storage flags for kem_matrix_assemble */
+/* shape types and boundary types of kem_mesh_create (KTriangle / KRectangle / KLineSegment;
+   KDirichletBoundary / KNeumannBoundary) */
+#define KEM_SHAPE_TRIANGLE 0
+#define KEM_SHAPE_RECTANGLE 1
+#define KEM_SHAPE_LINESEGMENT 2
+#define KEM_BC_DIRICHLET 0
+#define KEM_BC_NEUMANN 1
+
 #define KEM_STORE_FP64 1
 #define KEM_STORE_FP32 2
 
@@ -56,6 +64,7 @@ extern "C" {
 #define KEM_SOLVER_GAUSS 2 /* Gauss-Jordan without pivoting, KGaussianElimination.hh:24-58 (n <= 32768) */
 
 typedef struct kem_ctx kem_ctx;
+typedef struct kem_graph kem_graph;
 typedef struct kem_mesh kem_mesh;
 typedef struct kem_matrix kem_matrix;
 
@@ -98,6 +107,7 @@ int kem_ctx_set_mixed_fp64_ratio(kem_ctx* ctx, double ratio);
    queries); kem_ctx_kernel_time waits for them and returns the summed kernel time and the
    number of evaluations timed, then starts a new count. */
 int kem_ctx_set_kernel_timing(kem_ctx* ctx, int on);
+int kem_ctx_kernel_time(kem_ctx* ctx, double* total_ms, int* launches);
 /* Device memory on the context's GPU for callers without an allocator of their own (no
    reference counterpart): allocate / free (waits for the stream first), and copies that wait
    for the stream to finish. */
@@ -105,7 +115,6 @@ int kem_dev_alloc(kem_ctx* ctx, int64_t bytes, void** ptr);
 int kem_dev_free(kem_ctx* ctx, void* ptr);
 int kem_dev_upload(kem_ctx* ctx, void* dst, const void* src, int64_t bytes);
 int kem_dev_download(kem_ctx* ctx, void* dst, const void* src, int64_t bytes);
-int kem_ctx_kernel_time(kem_ctx* ctx, double* total_ms, int* launches);
 /* integrator policy by name: "numeric" | "default" | "analytic" | "rwg" | "reference"
    (KElectrostaticBoundaryIntegratorFactory::Make(name), .cc:78-88; unknown -> KEM_E_NAME) */
 int kem_set_integrator(kem_ctx* ctx, const char* name);
@@ -149,6 +158,16 @@ int kem_field_eval_device(kem_ctx* ctx, const kem_mesh* mesh, const double* d_si
    (device arrays: sigma n x nsets column-major, set[npts] in [0, nsets), nsets <= 64).  No
    reference counterpart (one KIntegratingFieldSolver per solution there).  Validates the set
    indices (waits for the stream), then runs asynchronously like kem_field_eval_device. */
+/* A field evaluation on fixed device buffers captured once as a HIP graph and replayed with
+   kem_graph_launch (asynchronous on the context's stream): for repeated evaluations at points
+   that change in place, e.g. the particles of a tracking step.  The buffers, sigma and the
+   point count are fixed at capture; their contents may change between launches.  No
+   reference counterpart.  The graph must be destroyed before its context and mesh. */
+int kem_field_graph_create(kem_ctx* ctx, const kem_mesh* mesh, const double* sigma, int64_t npts, const double* x,
+                           const double* y, const double* z, int precision, double* phi, double* ex, double* ey,
+                           double* ez, kem_graph** out);
+int kem_graph_launch(kem_graph* graph);
+void kem_graph_destroy(kem_graph* graph);
 int kem_field_eval_sets(kem_ctx* ctx, const kem_mesh* mesh, int64_t nsets, const double* sigma, const int32_t* set,
                         int64_t npts, const double* x, const double* y, const double* z, int precision, double* phi,
                         double* ex, double* ey, double* ez);

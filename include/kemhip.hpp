@@ -1,0 +1,333 @@
+/*
+ * kemhip.hpp -- C++ facade over the C-ABI (kemhip.h) with KEMField's class and method names,
+ * header-only, for host code that drives the MI355X library the way it drives KEMField.
+ *
+ * Mirrors (paths under KEMField/Source):
+ *   KSurfaceContainer                 Surfaces/include/KSurfaceContainer.hh
+ *   KElectrostaticBoundaryIntegrator  BoundaryIntegrals/Electrostatic/Core/include/KElectrostaticBoundaryIntegrator.hh:57-61
+ *   KElectrostaticBoundaryIntegratorFactory::Make  .../Core/src/KElectrostaticBoundaryIntegratorFactory.cc:78-88
+ *   KBoundaryIntegralMatrix (KSquareMatrix<double>)  BoundaryIntegrals/Core/include/KBoundaryIntegralMatrix.hh:16-99,
+ *                                     LinearAlgebra/Core/include/KMatrix.hh:15-111
+ *   KKrylovChargeDensitySolver / KRobinHoodChargeDensitySolver / KGaussianEliminationChargeDensitySolver
+ *                                     Interface/ChargeDensitySolvers/Electric/src/ (one .cc each)
+ *   KIntegratingElectrostaticFieldSolver  Interface/FieldSolvers/Electric/src/KIntegratingElectrostaticFieldSolver.cc:44-89
+ *   KElectrostaticBoundaryField       Interface/Fields/Electric/src/KElectrostaticBoundaryField.cc:16-199
+ *
+ * Errors: a negative C-ABI code throws kemhip::Error (unknown integrator / solver names throw
+ * kemhip::KEMSimpleException, as the reference's factory does); KEM_W_NOT_CONVERGED is
+ * returned in the solver statistics, not thrown (the reference warns).
+ */
+#ifndef KEMHIP_HPP
+#define KEMHIP_HPP
+
+#include <array>
+#include <cmath>
+#include <cstdint>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "kemhip.h"
+
+namespace kemhip {
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+struct KEMSimpleException : Error {
+    using Error::Error;
+};
+
+using Position = std::array<double, 3>;
+using FieldVector = std::array<double, 3>;
+
+class Context {
+  public:
+    explicit Context(int device = 0)
+    {
+        const int rc = kem_ctx_create(device, &h_);
+        if (rc != KEM_OK) throw Error(rc, "kem_ctx_create failed (no HIP device?)");
+    }
+    ~Context() { kem_ctx_destroy(h_); }
+    Context(const Context&) = delete;
+    Context& operator=(const Context&) = delete;
+    kem_ctx* get() const { return h_; }
+    int check(int rc) const
+    {
+        if (rc >= 0) return rc;
+        const std::string msg = kem_last_error(h_);
+        if (rc == KEM_E_NAME) throw KEMSimpleException(rc, msg);
+        throw Error(rc, msg);
+    }
+
+  private:
+    kem_ctx* h_ = nullptr;
+};
+
+// A flattened container: shape type per element, the reference's 11-double shape records
+// (triangles / rectangles [A, B, P0, N1, N2], segments [P0, P1, diameter]), boundary type,
+// value (Dirichlet potential) and flux (Neumann) in container order.
+struct SurfaceData {
+    std::vector<int32_t> shape;
+    std::vector<double> data;
+    std::vector<int32_t> bc;
+    std::vector<double> value, flux;
+    size_t size() const { return shape.size(); }
+    // KTriangle::SetValues(p0, p1, p2) (Surfaces/src/KTriangle.cc:18-28)
+    void AddTriangle(const Position& p0, const Position& p1, const Position& p2, double potential)
+    {
+        double n1[3], n2[3];
+        for (int k = 0; k < 3; k++) {
+            n1[k] = p1[k] - p0[k];
+            n2[k] = p2[k] - p0[k];
+        }
+        const double a = norm(n1), b = norm(n2);
+        shape.push_back(KEM_SHAPE_TRIANGLE);
+        data.insert(data.end(), {a, b, p0[0], p0[1], p0[2], n1[0] / a, n1[1] / a, n1[2] / a, n2[0] / b, n2[1] / b,
+                                 n2[2] / b});
+        bc.push_back(KEM_BC_DIRICHLET);
+        value.push_back(potential);
+        flux.push_back(0.);
+    }
+
+  private:
+    static double norm(const double* v) { return std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]); }
+};
+
+class KSurfaceContainer {
+  public:
+    KSurfaceContainer(const Context& ctx, const SurfaceData& s) : ctx_(ctx), n_((int64_t)s.size())
+    {
+        ctx.check(kem_mesh_create(ctx.get(), n_, s.shape.data(), s.data.data(), s.bc.data(), s.value.data(),
+                                  s.flux.data(), &h_));
+    }
+    ~KSurfaceContainer() { kem_mesh_destroy(h_); }
+    KSurfaceContainer(const KSurfaceContainer&) = delete;
+    KSurfaceContainer& operator=(const KSurfaceContainer&) = delete;
+    size_t size() const { return (size_t)n_; }
+    kem_mesh* get() const { return h_; }
+    const Context& context() const { return ctx_; }
+    // boundary values b_i (KBoundaryIntegralVector.hh:60-75)
+    std::vector<double> BoundaryValues() const
+    {
+        std::vector<double> b(size());
+        ctx_.check(kem_mesh_rhs(h_, b.data()));
+        return b;
+    }
+
+  private:
+    const Context& ctx_;
+    int64_t n_;
+    kem_mesh* h_ = nullptr;
+};
+
+class KElectrostaticBoundaryIntegrator {
+  public:
+    KElectrostaticBoundaryIntegrator(const Context& ctx, std::string name = "numeric") : ctx_(ctx), name_(std::move(name))
+    {
+        ctx_.check(kem_set_integrator(ctx_.get(), name_.c_str()));
+    }
+    const std::string& Name() const { return name_; }
+    void Select() const { ctx_.check(kem_set_integrator(ctx_.get(), name_.c_str())); }
+    // batched: out[k] = (E, phi) of element elem[k] at P[k] for unit charge density
+    void ElectricFieldAndPotential(const KSurfaceContainer& c, const std::vector<int64_t>& elem,
+                                   const std::vector<Position>& P, std::vector<std::pair<FieldVector, double>>& out) const
+    {
+        Select();
+        std::vector<double> buf(4 * elem.size());
+        ctx_.check(kem_element_eval(ctx_.get(), c.get(), (int64_t)elem.size(), elem.data(),
+                                    reinterpret_cast<const double*>(P.data()), buf.data()));
+        out.resize(elem.size());
+        for (size_t k = 0; k < elem.size(); k++) out[k] = {{buf[4 * k], buf[4 * k + 1], buf[4 * k + 2]}, buf[4 * k + 3]};
+    }
+    std::pair<FieldVector, double> ElectricFieldAndPotential(const KSurfaceContainer& c, int64_t j, const Position& P) const
+    {
+        std::vector<std::pair<FieldVector, double>> out;
+        ElectricFieldAndPotential(c, {j}, {P}, out);
+        return out[0];
+    }
+    double Potential(const KSurfaceContainer& c, int64_t j, const Position& P) const
+    {
+        return ElectricFieldAndPotential(c, j, P).second;
+    }
+    FieldVector ElectricField(const KSurfaceContainer& c, int64_t j, const Position& P) const
+    {
+        return ElectricFieldAndPotential(c, j, P).first;
+    }
+
+  private:
+    const Context& ctx_;
+    std::string name_;
+};
+
+struct KElectrostaticBoundaryIntegratorFactory {
+    static KElectrostaticBoundaryIntegrator Make(const Context& ctx, const std::string& name)
+    {
+        return KElectrostaticBoundaryIntegrator(ctx, name);
+    }
+    static KElectrostaticBoundaryIntegrator MakeDefault(const Context& ctx) { return Make(ctx, "default"); }
+};
+
+// KBoundaryIntegralMatrix as a KSquareMatrix<double>: device-resident, FP64 and/or FP32 storage
+class KBoundaryIntegralMatrix {
+  public:
+    KBoundaryIntegralMatrix(const KSurfaceContainer& c, const KElectrostaticBoundaryIntegrator& integrator,
+                            bool fp64 = true, bool fp32 = false)
+        : ctx_(c.context())
+    {
+        integrator.Select();
+        ctx_.check(kem_matrix_assemble(ctx_.get(), c.get(), (fp64 ? KEM_STORE_FP64 : 0) | (fp32 ? KEM_STORE_FP32 : 0),
+                                       &h_));
+    }
+    ~KBoundaryIntegralMatrix() { kem_matrix_destroy(h_); }
+    KBoundaryIntegralMatrix(const KBoundaryIntegralMatrix&) = delete;
+    KBoundaryIntegralMatrix& operator=(const KBoundaryIntegralMatrix&) = delete;
+    unsigned int Dimension() const { return (unsigned int)kem_matrix_dimension(h_); }
+    double operator()(unsigned int i, unsigned int j) const
+    {
+        double v = 0.;
+        ctx_.check(kem_matrix_get(h_, i, j, &v));
+        return v;
+    }
+    void Multiply(const std::vector<double>& x, std::vector<double>& y) const
+    {
+        y.resize(Dimension());
+        ctx_.check(kem_matrix_multiply(h_, 0, x.data(), y.data()));
+    }
+    void MultiplyTranspose(const std::vector<double>& x, std::vector<double>& y) const
+    {
+        y.resize(Dimension());
+        ctx_.check(kem_matrix_multiply(h_, 1, x.data(), y.data()));
+    }
+    kem_matrix* get() const { return h_; }
+    const Context& context() const { return ctx_; }
+
+  private:
+    const Context& ctx_;
+    kem_matrix* h_ = nullptr;
+};
+
+// Charge-density solvers: Solve(A, b, x) with x holding the initial guess (KChargeDensitySolver
+// writes sigma into the surfaces; here x is the caller's vector in container order)
+class KChargeDensitySolver {
+  public:
+    virtual ~KChargeDensitySolver() = default;
+    kem_solve_stats Solve(const KBoundaryIntegralMatrix& A, const std::vector<double>& b, std::vector<double>& x) const
+    {
+        x.resize(b.size(), 0.);
+        kem_solve_stats st{};
+        A.context().check(kem_solve(A.get(), &opts_, b.data(), x.data(), &st));
+        return st;
+    }
+
+  protected:
+    KChargeDensitySolver() { kem_solve_opts_default(&opts_); }
+    kem_solve_opts opts_;
+};
+
+class KKrylovChargeDensitySolver : public KChargeDensitySolver {
+  public:
+    KKrylovChargeDensitySolver() { opts_.method = KEM_SOLVER_GMRES; }
+    void SetTolerance(double t) { opts_.tolerance = t; }
+    void SetMaxIterations(int n) { opts_.max_iterations = n; }
+    void SetIterationsBetweenRestart(int n) { opts_.restart = n; }
+    void SetTimeLimitSeconds(double s) { opts_.time_limit_s = s; }
+    void SetMixedPrecision(bool on) { opts_.precision = on ? KEM_MIXED : KEM_FP64; }
+};
+
+class KRobinHoodChargeDensitySolver : public KChargeDensitySolver {
+  public:
+    KRobinHoodChargeDensitySolver() { opts_.method = KEM_SOLVER_ROBIN_HOOD; }
+    void SetTolerance(double t) { opts_.tolerance = t; }
+    void SetCheckSubInterval(int64_t n) { opts_.check_interval = n; }
+};
+
+class KGaussianEliminationChargeDensitySolver : public KChargeDensitySolver {
+  public:
+    KGaussianEliminationChargeDensitySolver() { opts_.method = KEM_SOLVER_GAUSS; }
+};
+
+// direct field of a solved container (fused, batched)
+class KIntegratingElectrostaticFieldSolver {
+  public:
+    KIntegratingElectrostaticFieldSolver(const KSurfaceContainer& c, std::vector<double> sigma, bool mixed = false)
+        : c_(c), sigma_(std::move(sigma)), precision_(mixed ? KEM_MIXED : KEM_FP64)
+    {
+    }
+    void ElectricFieldAndPotential(const std::vector<Position>& P, std::vector<FieldVector>& E,
+                                   std::vector<double>& phi) const
+    {
+        E.resize(P.size());
+        phi.resize(P.size());
+        c_.context().check(kem_field_eval(c_.context().get(), c_.get(), sigma_.data(), (int64_t)P.size(),
+                                          reinterpret_cast<const double*>(P.data()), precision_, phi.data(),
+                                          reinterpret_cast<double*>(E.data())));
+    }
+    std::pair<FieldVector, double> ElectricFieldAndPotential(const Position& P) const
+    {
+        std::vector<FieldVector> E;
+        std::vector<double> phi;
+        ElectricFieldAndPotential({P}, E, phi);
+        return {E[0], phi[0]};
+    }
+    double Potential(const Position& P) const { return ElectricFieldAndPotential(P).second; }
+    FieldVector ElectricField(const Position& P) const { return ElectricFieldAndPotential(P).first; }
+    const std::vector<double>& ChargeDensities() const { return sigma_; }
+
+  private:
+    const KSurfaceContainer& c_;
+    std::vector<double> sigma_;
+    int precision_;
+};
+
+// KElectrostaticBoundaryField: Initialize() solves for the charge densities, then the field
+// solver answers Potential / ElectricField / ElectricFieldAndPotential (the time argument is
+// accepted and ignored, as for the static field of the reference)
+class KElectrostaticBoundaryField {
+  public:
+    KElectrostaticBoundaryField(const KSurfaceContainer& c, KElectrostaticBoundaryIntegrator integrator,
+                                const KChargeDensitySolver& solver, bool mixed_field = false)
+        : c_(c), integrator_(std::move(integrator)), solver_(solver), mixed_(mixed_field)
+    {
+    }
+    kem_solve_stats Initialize()
+    {
+        KBoundaryIntegralMatrix A(c_, integrator_);
+        std::vector<double> x;
+        const kem_solve_stats st = solver_.Solve(A, c_.BoundaryValues(), x);
+        field_.reset(new KIntegratingElectrostaticFieldSolver(c_, std::move(x), mixed_));
+        return st;
+    }
+    double Potential(const Position& P, double /*t*/ = 0.) const { return solver().Potential(P); }
+    FieldVector ElectricField(const Position& P, double /*t*/ = 0.) const { return solver().ElectricField(P); }
+    std::pair<FieldVector, double> ElectricFieldAndPotential(const Position& P, double /*t*/ = 0.) const
+    {
+        return solver().ElectricFieldAndPotential(P);
+    }
+    // many points per call (the batched entry a tracking step can use)
+    void ElectricFieldAndPotential(const std::vector<Position>& P, std::vector<FieldVector>& E,
+                                   std::vector<double>& phi) const
+    {
+        solver().ElectricFieldAndPotential(P, E, phi);
+    }
+    const std::vector<double>& ChargeDensities() const { return solver().ChargeDensities(); }
+
+  private:
+    const KIntegratingElectrostaticFieldSolver& solver() const
+    {
+        if (!field_) throw Error(KEM_E_STATE, "KElectrostaticBoundaryField: Initialize() first");
+        return *field_;
+    }
+    const KSurfaceContainer& c_;
+    KElectrostaticBoundaryIntegrator integrator_;
+    const KChargeDensitySolver& solver_;
+    bool mixed_;
+    std::unique_ptr<KIntegratingElectrostaticFieldSolver> field_;
+};
+
+}  // namespace kemhip
+
+#endif

@@ -48,6 +48,12 @@ def build(verbose=False):
     tmp = OUT + ".tmp"
     subprocess.run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-fPIC", *objs, "-o", tmp], check=True)
     os.replace(tmp, OUT)
+    # the C++ facade example (include/kemhip.hpp), linked against the library just built
+    facade = os.path.join(BUILD, "facade_test")
+    subprocess.run([os.environ.get("CXX", "g++"), "-O2", "-std=c++17", "-Wall",
+                    "-I", os.path.join(os.path.dirname(HERE), "include"),
+                    os.path.join(os.path.dirname(HERE), "tests", "cpp", "facade_test.cpp"),
+                    "-L", HERE, "-lkemhip", "-Wl,-rpath,$ORIGIN/..", "-o", facade], check=True)
     if verbose:
         print("built", OUT)
     return OUT

@@ -59,6 +59,12 @@ struct kem_mesh {
     bool has_neumann = false;
 };
 
+struct kem_graph {
+    kem_ctx* ctx = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
 struct kem_matrix {
     kem_ctx* ctx = nullptr;
     int64_t n = 0;
@@ -463,6 +469,70 @@ int kem_field_eval_device(kem_ctx* c, const kem_mesh* m, const double* sigma, in
     if (npts <= 0) return KEM_OK;
     KEM_CK(c, enter(c));
     return field_eval(c, m, sigma, nullptr, npts, x, y, z, precision, phi, ex, ey, ez);
+}
+
+int kem_field_graph_create(kem_ctx* c, const kem_mesh* m, const double* sigma, int64_t npts, const double* x,
+                           const double* y, const double* z, int precision, double* phi, double* ex, double* ey,
+                           double* ez, kem_graph** out)
+{
+    if (!c || !m || !sigma || !out || npts <= 0 || !x || !y || !z) return KEM_E_ARG;
+    *out = nullptr;
+    if (m->ctx != c) return fail(c, KEM_E_STATE, "mesh belongs to another context");
+    if (precision != KEM_FP64 && precision != KEM_MIXED) return fail(c, KEM_E_ARG, "field: unknown precision mode");
+    KEM_CK(c, enter(c));
+    // workspaces reach their final size before the capture (no allocation inside a graph)
+    KEM_CK(c, ensure_ws(c, 4 * (int64_t)(2048 + 256) * 256));
+    if (npts >= kSortMinPoints && npts <= INT32_MAX) {
+        const size_t bytes = kem::sort_workspace_bytes(npts);
+        if (c->sort_ws_bytes < bytes) {
+            if (c->sort_ws) (void)hipFree(c->sort_ws);
+            c->sort_ws = nullptr;
+            c->sort_ws_bytes = 0;
+            KEM_CK(c, hipMalloc(&c->sort_ws, bytes));
+            c->sort_ws_bytes = bytes;
+        }
+    }
+    KEM_CK(c, hipStreamSynchronize(c->stream));
+    const bool timing = c->timing;
+    c->timing = false;
+    kem_graph* g = new kem_graph();
+    g->ctx = c;
+    hipError_t e = hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal);
+    int rc = KEM_OK;
+    if (e == hipSuccess) {
+        rc = field_eval(c, m, sigma, nullptr, npts, x, y, z, precision, phi, ex, ey, ez);
+        hipGraph_t graph = nullptr;
+        const hipError_t e2 = hipStreamEndCapture(c->stream, &graph);
+        g->graph = graph;
+        if (rc == KEM_OK && e2 != hipSuccess) e = e2;
+    }
+    c->timing = timing;
+    if (e == hipSuccess && rc == KEM_OK) e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);
+    if (e != hipSuccess || rc != KEM_OK) {
+        kem_graph_destroy(g);
+        if (rc != KEM_OK) return rc;
+        return fail(c, KEM_E_HIP, std::string("field graph capture: ") + hipGetErrorString(e));
+    }
+    *out = g;
+    return KEM_OK;
+}
+
+int kem_graph_launch(kem_graph* g)
+{
+    if (!g || !g->exec) return KEM_E_ARG;
+    kem_ctx* c = g->ctx;
+    KEM_CK(c, enter(c));
+    KEM_CK(c, hipGraphLaunch(g->exec, c->stream));
+    return KEM_OK;
+}
+
+void kem_graph_destroy(kem_graph* g)
+{
+    if (!g) return;
+    if (g->ctx) (void)hipSetDevice(g->ctx->device);
+    if (g->exec) (void)hipGraphExecDestroy(g->exec);
+    if (g->graph) (void)hipGraphDestroy(g->graph);
+    delete g;
 }
 
 int kem_field_eval_sets(kem_ctx* c, const kem_mesh* m, int64_t nsets, const double* sigma, const int32_t* set,

@@ -1,0 +1,110 @@
+// Drives the library through the C++ facade (include/kemhip.hpp) the way KEMField host code
+// drives KEMField: build a container, assemble, solve (GMRES, Gauss, Robin Hood), evaluate the
+// field.  Unit cube, 4 x 4 squares per face, two triangles per square, 1 V
+// (CubeCapacitanceTest.cc-style).  Prints "facade OK" and exits 0 when every check passes.
+#include <cmath>
+#include <cstdio>
+#include <vector>
+
+#include "kemhip.hpp"
+
+using namespace kemhip;
+
+static int fails = 0;
+static void check(bool ok, const char* what, double v)
+{
+    std::printf("%-48s %s (%.6e)\n", what, ok ? "ok" : "FAILED", v);
+    if (!ok) fails++;
+}
+
+int main()
+{
+    Context ctx(0);
+    SurfaceData s;
+    const int n = 4;
+    for (int axis = 0; axis < 3; axis++)
+        for (int sign = -1; sign <= 1; sign += 2)
+            for (int a = 0; a < n; a++)
+                for (int b = 0; b < n; b++) {
+                    const int u = (axis + 1) % 3, v = (axis + 2) % 3;
+                    auto P = [&](int ia, int ib) {
+                        Position p{};
+                        p[axis] = 0.5 * sign;
+                        p[u] = -0.5 + (double)ia / n;
+                        p[v] = -0.5 + (double)ib / n;
+                        return p;
+                    };
+                    // outward orientation: (p1 - p0) x (p2 - p0) along sign * e_axis
+                    if (sign > 0) {
+                        s.AddTriangle(P(a, b), P(a + 1, b), P(a + 1, b + 1), 1.);
+                        s.AddTriangle(P(a + 1, b + 1), P(a, b + 1), P(a, b), 1.);
+                    } else {
+                        s.AddTriangle(P(a, b), P(a, b + 1), P(a + 1, b + 1), 1.);
+                        s.AddTriangle(P(a + 1, b + 1), P(a + 1, b), P(a, b), 1.);
+                    }
+                }
+    KSurfaceContainer cont(ctx, s);
+    KElectrostaticBoundaryIntegrator integ = KElectrostaticBoundaryIntegratorFactory::MakeDefault(ctx);
+    KBoundaryIntegralMatrix A(cont, integ);
+    const std::vector<double> b = cont.BoundaryValues();
+    check(A.Dimension() == cont.size(), "KSquareMatrix::Dimension", A.Dimension());
+
+    KKrylovChargeDensitySolver gmres;
+    gmres.SetTolerance(1e-10);
+    std::vector<double> x;
+    kem_solve_stats st = gmres.Solve(A, b, x);
+    check(st.residual <= 1e-10, "GMRES converged to 1e-10", st.residual);
+    KGaussianEliminationChargeDensitySolver gauss;
+    std::vector<double> xg;
+    gauss.Solve(A, b, xg);
+    double d = 0., nx = 0.;
+    for (size_t i = 0; i < x.size(); i++) {
+        d += (x[i] - xg[i]) * (x[i] - xg[i]);
+        nx += x[i] * x[i];
+    }
+    check(std::sqrt(d / nx) < 1e-8, "Gauss-Jordan == GMRES", std::sqrt(d / nx));
+    KRobinHoodChargeDensitySolver rh;
+    rh.SetTolerance(1e-8);
+    std::vector<double> xr;
+    rh.Solve(A, b, xr);
+    d = 0.;
+    for (size_t i = 0; i < x.size(); i++) d += (x[i] - xr[i]) * (x[i] - xr[i]);
+    check(std::sqrt(d / nx) < 1e-5, "Robin Hood == GMRES", std::sqrt(d / nx));
+
+    // operator()(i, j) against Multiply with a unit vector
+    std::vector<double> e(A.Dimension(), 0.), col;
+    e[7] = 1.;
+    A.Multiply(e, col);
+    check(std::fabs(col[3] - A(3, 7)) <= 1e-15 * std::fabs(col[3]), "operator() == Multiply(e_j)", col[3]);
+
+    // capacitance Q / (4 pi eps0 V); area of each triangle = (1/n)^2 / 2
+    double Q = 0.;
+    for (double sig : x) Q += sig * 0.5 / (n * n);
+    const double C = Q / (4. * M_PI * 8.8541878128E-12);
+    check(std::fabs(C - 0.6533) < 2e-3, "capacitance / (4 pi eps0)", C);
+
+    KElectrostaticBoundaryField field(cont, integ, gmres);
+    field.Initialize();
+    const double phic = field.Potential({0.1, -0.05, 0.2});
+    check(std::fabs(phic - 1.) < 1e-2, "potential inside the conductor ~ 1 V", phic);
+    const Position far{0., 0., 20.};
+    const auto ep = field.ElectricFieldAndPotential(far);
+    check(std::fabs(ep.second - C / 20.) < 2e-3 * C / 20., "far potential ~ C/r (monopole)", ep.second);
+    check(std::fabs(ep.first[2] * 400. / C - 1.) < 1e-2, "far field ~ C/r^2", ep.first[2] * 400. / C);
+
+    KIntegratingElectrostaticFieldSolver mixed(cont, x, true);
+    const auto em = mixed.ElectricFieldAndPotential(far);
+    check(std::fabs(em.second - ep.second) < 1e-6 * std::fabs(ep.second), "mixed field == FP64 field (1e-6)",
+          std::fabs(em.second - ep.second) / std::fabs(ep.second));
+
+    bool threw = false;
+    try {
+        KElectrostaticBoundaryIntegratorFactory::Make(ctx, "no-such-integrator");
+    } catch (const KEMSimpleException&) {
+        threw = true;
+    }
+    check(threw, "unknown integrator throws KEMSimpleException", 0.);
+    integ.Select();
+    if (fails == 0) std::printf("facade OK\n");
+    return fails == 0 ? 0 : 1;
+}

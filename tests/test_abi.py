@@ -58,3 +58,18 @@ def test_gfx950_code_object_present():
 
     data = open(_lib.LIBPATH, "rb").read()
     assert b"gfx950" in data
+
+
+def test_cpp_facade_header_compiles(tmp_path):
+    """include/kemhip.hpp is self-contained C++17 over the C header."""
+    import shutil
+    import subprocess
+
+    gxx = shutil.which("g++")
+    if gxx is None:
+        pytest.skip("no g++")
+    src = tmp_path / "t.cpp"
+    src.write_text('#include "kemhip.hpp"\nint main() { return 0; }\n')
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    subprocess.run([gxx, "-std=c++17", "-Wall", "-Werror", "-fsyntax-only", "-I", os.path.join(root, "include"),
+                    str(src)], check=True)
