@@ -39,6 +39,13 @@ PEAK_FP64_TFLOPS = 78.6  # MI355X FP64 vector peak
 PEAK_FP32_TFLOPS = 157.3  # MI355X FP32 vector peak (packed)
 PEAK_HBM_GBS = 8000.0
 
+# HBM bytes per launch of the far-field kernels on the default workload (20,000 triangles,
+# 1,000,000 points): rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE of one dispatch, scaled by the
+# readings of a 1 GiB device copy (bench.py --copy-calibration: FETCH_SIZE x 1.99987, WRITE_SIZE
+# x 1.0).
+TRAFFIC_BYTES = {"k_field_far32c": (6108990.75 * 1.99987 + 31250.0) * 1024,
+                 "k_field_far": (183564770.6 * 1.99987 + 31250.0) * 1024}
+
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
@@ -60,10 +67,14 @@ def main():
     ap.add_argument("--batch-points", type=int, default=4_000_000)
     ap.add_argument("--workload", choices=["coax20k", "large"], default="coax20k",
                     help="large: 100,000-triangle ring bore, 16 voltage sets, 10 M points, adjoint")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only copy 1 GiB device to device (reference kernel for the rocprofv3 byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported", file=sys.stderr)
         sys.exit(2)
+    if args.copy_calibration:
+        return copy_calibration()
     if args.workload == "large":
         return run_large(args)
 
@@ -181,6 +192,8 @@ def main():
     kflops = farflopsmx if mixed_ok else farflops64
     value = pairs / (ms * 1e-3)
     achieved = kflops / (kms * 1e-3) / 1e12
+    kname = "k_field_far32c" if mixed_ok else "k_field_far"
+    default_workload = N == 20000 and M == 1_000_000
     out.update(field_ms_fp64=ms64, field_ms_mixed=msmx, far_kernel_ms_fp64=kms64, far_kernel_ms_mixed=kmsmx,
                pairs_per_s_fp64=pairs / ms64 * 1e3,
                pairs_per_s_mixed=pairs / msmx * 1e3, speedup_mixed_vs_fp64=ms64 / msmx,
@@ -286,14 +299,39 @@ def main():
         "roofline": {"bound": "mfma", "compute": "vector ALU (" + ("fp32" if mixed_ok else "fp64") + ")",
                      "achieved": achieved, "peak": PEAK_FP32_TFLOPS if mixed_ok else PEAK_FP64_TFLOPS,
                      "unit": "TFLOP/s",
-                     "frac": achieved / (PEAK_FP32_TFLOPS if mixed_ok else PEAK_FP64_TFLOPS), "traffic": None,
-                     "kernel": "k_field_far32" if mixed_ok else "k_field_far",
+                     "frac": achieved / (PEAK_FP32_TFLOPS if mixed_ok else PEAK_FP64_TFLOPS),
+                     "traffic": TRAFFIC_BYTES[kname] if default_workload else None,
+                     "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)",
+                     "kernel": kname,
                      "kernel_ms": kms, "flops_per_launch": kflops,
                      "pairs_by_regime": (cmx if mixed_ok else c64).astype(int).tolist()},
         "cpu_baseline": cpu,
     }
     line.update(out)
     print(json.dumps(line), flush=True)
+
+
+def copy_calibration():
+    """One device-to-device copy of 1 GiB (the HIP runtime's copy kernel): the reference
+    dispatch whose rocprofv3 FETCH_SIZE / WRITE_SIZE readings scale those of the field kernels
+    to bytes."""
+    from kassiopeia_amd import kemfield as K
+
+    ctx = K.Context(0)
+    n = 1 << 30
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    ctx.check(ctx.lib.kem_dev_alloc(ctx.h, n, ctypes.byref(a)))
+    ctx.check(ctx.lib.kem_dev_alloc(ctx.h, n, ctypes.byref(b)))
+    hip = ctypes.CDLL("libamdhip64.so.7")  # the runtime the library is bound to
+    hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    hip.hipMemcpy.restype = ctypes.c_int
+    rc = hip.hipMemcpy(b, a, n, 3)  # hipMemcpyDeviceToDevice
+    ctx.check(ctx.lib.kem_dev_free(ctx.h, a))
+    ctx.check(ctx.lib.kem_dev_free(ctx.h, b))
+    ctx.close()
+    if rc != 0:
+        raise RuntimeError(f"hipMemcpy failed ({rc})")
+    print(json.dumps({"copied_bytes": n}), flush=True)
 
 
 def run_large(args):

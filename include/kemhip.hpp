@@ -92,6 +92,40 @@ struct SurfaceData {
         value.push_back(potential);
         flux.push_back(0.);
     }
+    // KRectangle::SetValues(p0, p1, p2, p3) (Surfaces/src/KRectangle.cc:18-28): sides p1 - p0
+    // and p3 - p0 (p2, the corner opposite p0, is not used)
+    void AddRectangle(const Position& p0, const Position& p1, const Position& p3, double potential)
+    {
+        double n1[3], n2[3];
+        for (int k = 0; k < 3; k++) {
+            n1[k] = p1[k] - p0[k];
+            n2[k] = p3[k] - p0[k];
+        }
+        const double a = norm(n1), b = norm(n2);
+        shape.push_back(KEM_SHAPE_RECTANGLE);
+        data.insert(data.end(), {a, b, p0[0], p0[1], p0[2], n1[0] / a, n1[1] / a, n1[2] / a, n2[0] / b, n2[1] / b,
+                                 n2[2] / b});
+        bc.push_back(KEM_BC_DIRICHLET);
+        value.push_back(potential);
+        flux.push_back(0.);
+    }
+    // KLineSegment::SetValues(p0, p1, diameter) (Surfaces/include/KLineSegment.hh:23)
+    void AddLineSegment(const Position& p0, const Position& p1, double diameter, double potential)
+    {
+        shape.push_back(KEM_SHAPE_LINESEGMENT);
+        data.insert(data.end(), {p0[0], p0[1], p0[2], p1[0], p1[1], p1[2], diameter, 0., 0., 0., 0.});
+        bc.push_back(KEM_BC_DIRICHLET);
+        value.push_back(potential);
+        flux.push_back(0.);
+    }
+    // turns element i into a Neumann (dielectric interface) element with the given flux
+    // (KNeumannBoundary's normal boundary flux, the permittivity ratio of the interface)
+    void SetNeumann(size_t i, double normal_flux)
+    {
+        bc.at(i) = KEM_BC_NEUMANN;
+        value.at(i) = 0.;
+        flux.at(i) = normal_flux;
+    }
 
   private:
     static double norm(const double* v) { return std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]); }

@@ -54,6 +54,8 @@ struct kem_mesh {
     bool all_triangles = true;
     double* clusters = nullptr;  // near-pass element clusters (centre, radius)
     int32_t* order = nullptr;    // elements in cluster order
+    float* far6 = nullptr;       // mixed far field: 6-point records in cluster order
+    double* far6_cl = nullptr;   // and the clusters' 6-point reach
     std::vector<int32_t> shape, bc;
     std::vector<double> value, flux;
     bool has_neumann = false;
@@ -334,6 +336,9 @@ int kem_mesh_create(kem_ctx* c, int64_t n, const int32_t* shape, const double* d
     std::vector<int32_t> order((size_t)n);
     std::vector<double> clusters((size_t)(4 * ((n + KEM_CLUSTER - 1) / KEM_CLUSTER)));
     kem::build_clusters(n, g.data(), order.data(), clusters.data());
+    std::vector<float> far6((size_t)n * KEM_FAR6_STRIDE);
+    std::vector<double> far6_cl(clusters.size());
+    kem::build_far6(n, g.data(), nd32.data(), order.data(), clusters.data(), far6.data(), far6_cl.data());
     std::vector<double> recs((size_t)n * KEM_REC, 0.);
     for (int64_t i = 0; i < n; i++)
         for (int k = 0; k < 11; k++) recs[(size_t)i * KEM_REC + k] = data[11 * i + k];
@@ -350,6 +355,11 @@ int kem_mesh_create(kem_ctx* c, int64_t n, const int32_t* shape, const double* d
     if (e == hipSuccess)
         e = hipMemcpy(m->clusters, clusters.data(), sizeof(double) * clusters.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(m->order, order.data(), sizeof(int32_t) * order.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = dalloc(&m->far6, (int64_t)far6.size());
+    if (e == hipSuccess) e = hipMemcpy(m->far6, far6.data(), sizeof(float) * far6.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = dalloc(&m->far6_cl, (int64_t)far6_cl.size());
+    if (e == hipSuccess)
+        e = hipMemcpy(m->far6_cl, far6_cl.data(), sizeof(double) * far6_cl.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = dalloc(&m->nodes, (int64_t)nd.size());
     if (e == hipSuccess) e = dalloc(&m->targets, (int64_t)t.size());
     if (e == hipSuccess) e = dalloc(&m->nodes32, (int64_t)nd32.size());
@@ -376,6 +386,8 @@ void kem_mesh_destroy(kem_mesh* m)
     dfree(m->nodes32);
     dfree(m->clusters);
     dfree(m->order);
+    dfree(m->far6);
+    dfree(m->far6_cl);
     dfree(m->recs);
     dfree(m->rule);
     delete m;
@@ -425,18 +437,33 @@ int kem_element_eval(kem_ctx* c, const kem_mesh* m, int64_t np, const int64_t* e
 
 namespace {
 // field evaluation on device arrays; pset (nullable): per-point charge-density set
-int field_eval(kem_ctx* c, const kem_mesh* m, const double* sigma, const int32_t* pset, int64_t npts, const double* x,
-               const double* y, const double* z, int precision, double* phi, double* ex, double* ey, double* ez)
+void field_mesh(const kem_mesh* m, kem::FieldMesh& f)
 {
-    const float* n32 = precision == KEM_MIXED ? m->nodes32 : nullptr;
+    f.n = m->n;
+    f.geom = m->geom;
+    f.nodes = m->nodes;
+    f.recs = m->recs;
+    f.nodes32 = m->nodes32;
+    f.clusters = m->clusters;
+    f.order = m->order;
+    f.far6 = m->far6;
+    f.far6_cl = m->far6_cl;
+}
+
+int field_eval(kem_ctx* c, const kem_mesh* m, const double* sigma, const int32_t* pset, int64_t nsets, int64_t npts,
+               const double* x, const double* y, const double* z, int precision, double* phi, double* ex, double* ey,
+               double* ez)
+{
+    kem::FieldMesh fm;
+    field_mesh(m, fm);
+    const bool mixed = precision == KEM_MIXED;
     hipEvent_t t0 = nullptr, t1 = nullptr;
     if (c->timing && c->ev_used < kem_ctx::kTimedMax) {
         t0 = c->ev[2 * c->ev_used];
         t1 = c->ev[2 * c->ev_used + 1];
         c->ev_used++;
     }
-    const int64_t need = 4 * (int64_t)(2048 + 256) * 256;
-    KEM_CK(c, ensure_ws(c, need));
+    KEM_CK(c, ensure_ws(c, kem::field_workspace_doubles(m->n, nsets)));
     if (npts >= kSortMinPoints && npts <= INT32_MAX) {
         // visit the points in Morton order so the lanes of a wavefront share quadrature rules
         const size_t bytes = kem::sort_workspace_bytes(npts);
@@ -447,14 +474,13 @@ int field_eval(kem_ctx* c, const kem_mesh* m, const double* sigma, const int32_t
             KEM_CK(c, hipMalloc(&c->sort_ws, bytes));
             c->sort_ws_bytes = bytes;
         }
-        KEM_CK(c, kem::launch_field_sorted(c->stream, c->policy, m->geom, m->nodes, m->recs, n32, c->mixed_fp64_ratio,
-                                           sigma, pset, m->n, m->clusters, m->order, x, y, z, npts, phi, ex, ey, ez,
-                                           c->sort_ws, c->sort_ws_bytes, c->ws, c->ws_doubles, t0, t1));
+        KEM_CK(c, kem::launch_field_sorted(c->stream, c->policy, fm, mixed, c->mixed_fp64_ratio, sigma, pset, nsets,
+                                           x, y, z, npts, phi, ex, ey, ez, c->sort_ws, c->sort_ws_bytes, c->ws,
+                                           c->ws_doubles, t0, t1));
         return KEM_OK;
     }
-    KEM_CK(c, kem::launch_field(c->stream, c->policy, m->geom, m->nodes, m->recs, n32, c->mixed_fp64_ratio, sigma,
-                                pset, m->n, m->clusters, m->order, x, y, z, npts, phi, ex, ey, ez, c->ws,
-                                c->ws_doubles, t0, t1));
+    KEM_CK(c, kem::launch_field(c->stream, c->policy, fm, mixed, c->mixed_fp64_ratio, sigma, pset, nsets, x, y, z,
+                                npts, phi, ex, ey, ez, c->ws, c->ws_doubles, t0, t1));
     return KEM_OK;
 }
 }  // namespace
@@ -468,7 +494,7 @@ int kem_field_eval_device(kem_ctx* c, const kem_mesh* m, const double* sigma, in
     if (precision != KEM_FP64 && precision != KEM_MIXED) return fail(c, KEM_E_ARG, "field: unknown precision mode");
     if (npts <= 0) return KEM_OK;
     KEM_CK(c, enter(c));
-    return field_eval(c, m, sigma, nullptr, npts, x, y, z, precision, phi, ex, ey, ez);
+    return field_eval(c, m, sigma, nullptr, 1, npts, x, y, z, precision, phi, ex, ey, ez);
 }
 
 int kem_field_graph_create(kem_ctx* c, const kem_mesh* m, const double* sigma, int64_t npts, const double* x,
@@ -481,7 +507,7 @@ int kem_field_graph_create(kem_ctx* c, const kem_mesh* m, const double* sigma, i
     if (precision != KEM_FP64 && precision != KEM_MIXED) return fail(c, KEM_E_ARG, "field: unknown precision mode");
     KEM_CK(c, enter(c));
     // workspaces reach their final size before the capture (no allocation inside a graph)
-    KEM_CK(c, ensure_ws(c, 4 * (int64_t)(2048 + 256) * 256));
+    KEM_CK(c, ensure_ws(c, kem::field_workspace_doubles(m->n, 1)));
     if (npts >= kSortMinPoints && npts <= INT32_MAX) {
         const size_t bytes = kem::sort_workspace_bytes(npts);
         if (c->sort_ws_bytes < bytes) {
@@ -500,7 +526,7 @@ int kem_field_graph_create(kem_ctx* c, const kem_mesh* m, const double* sigma, i
     hipError_t e = hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal);
     int rc = KEM_OK;
     if (e == hipSuccess) {
-        rc = field_eval(c, m, sigma, nullptr, npts, x, y, z, precision, phi, ex, ey, ez);
+        rc = field_eval(c, m, sigma, nullptr, 1, npts, x, y, z, precision, phi, ex, ey, ez);
         hipGraph_t graph = nullptr;
         const hipError_t e2 = hipStreamEndCapture(c->stream, &graph);
         g->graph = graph;
@@ -553,7 +579,7 @@ int kem_field_eval_sets(kem_ctx* c, const kem_mesh* m, int64_t nsets, const doub
     KEM_CK(c, hipMemcpyAsync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost, c->stream));
     KEM_CK(c, hipStreamSynchronize(c->stream));
     if (hbad) return fail(c, KEM_E_ARG, "field: set index outside [0, nsets)");
-    return field_eval(c, m, sigma, set, npts, x, y, z, precision, phi, ex, ey, ez);
+    return field_eval(c, m, sigma, set, nsets, npts, x, y, z, precision, phi, ex, ey, ez);
 }
 
 int kem_field_census(kem_ctx* c, const kem_mesh* m, int64_t npts, const double* x, const double* y, const double* z,

@@ -8,30 +8,44 @@
 
 namespace kem {
 
-// phi/E at npts points (SoA) from nelem elements with charge densities sigma.  pset (nullable):
-// per-point index of the charge-density set, point i using sigma[pset[i]*nelem + j].  nodes32
-// != null selects the mixed-precision far field (FP32 node sums beyond distance ratio
-// fp64_ratio, FP64 below it, FP64 near field and accumulation).  clusters / order
-// (kem::build_clusters) select the culled near pass under the numeric policy; null scans every
-// element.  ws: scratch of ws_doubles doubles for element-range partial sums.  far_begin /
-// far_end (nullable): events recorded on st around the far-field kernel.
-hipError_t launch_field(hipStream_t st, int policy, const double* geom, const double* nodes, const double* recs,
-                        const float* nodes32, double fp64_ratio, const double* sigma, const int32_t* pset,
-                        int64_t nelem, const double* clusters, const int32_t* order, const double* px,
-                        const double* py, const double* pz, int64_t npts, double* phi, double* ex, double* ey,
-                        double* ez, double* ws, int64_t ws_doubles, hipEvent_t far_begin = nullptr,
-                        hipEvent_t far_end = nullptr);
+// Device arrays of a mesh that the field passes read.
+struct FieldMesh {
+    int64_t n = 0;
+    const double* geom = nullptr;
+    const double* nodes = nullptr;
+    const double* recs = nullptr;      // raw shape records (analytic / reference policies)
+    const float* nodes32 = nullptr;    // FP32 node table (mixed-precision far field)
+    const double* clusters = nullptr;  // element clusters (kem::build_clusters); null: element scan
+    const int32_t* order = nullptr;    // elements in cluster order
+    const float* far6 = nullptr;       // 6-point far records in cluster order (kem::build_far6)
+    const double* far6_cl = nullptr;   // and the clusters' 6-point reach
+};
 
-// Same as launch_field with the points visited in Morton order (kem_sort.hip): sort keys
-// (the set index in the top bits when pset is given, so wavefronts share one set), gather,
-// evaluate, scatter back.  sort_ws: sort_workspace_bytes(npts) bytes.
+// Doubles of the ws argument of launch_field for npts points and nsets charge-density sets.
+int64_t field_workspace_doubles(int64_t nelem, int64_t nsets);
+
+// phi/E at npts points (SoA) from the mesh's elements with charge densities sigma.  pset
+// (nullable): per-point index of the charge-density set, point i using
+// sigma[pset[i]*n + j], 0 <= pset[i] < nsets.  mixed selects the mixed-precision far field
+// (FP32 node sums beyond distance ratio fp64_ratio, FP64 below it, FP64 near field and
+// accumulation); with fp64_ratio = 0 and far6 given it runs over element clusters.  clusters /
+// order select the culled near pass under the numeric policy; null scans every element.
+// ws: scratch of ws_doubles >= field_workspace_doubles(n, nsets) doubles (element-range
+// partial sums, per-set element weights).  far_begin / far_end (nullable): events recorded on
+// st around the far-field pass.
+hipError_t launch_field(hipStream_t st, int policy, const FieldMesh& m, bool mixed, double fp64_ratio,
+                        const double* sigma, const int32_t* pset, int64_t nsets, const double* px, const double* py,
+                        const double* pz, int64_t npts, double* phi, double* ex, double* ey, double* ez, double* ws,
+                        int64_t ws_doubles, hipEvent_t far_begin = nullptr, hipEvent_t far_end = nullptr);
+
+// Same as launch_field with the points visited in Morton order (kem_sort.hip): sort keys,
+// gather (with the set indices), evaluate, scatter back.  sort_ws: sort_workspace_bytes(npts) bytes.
 size_t sort_workspace_bytes(int64_t npts);
-hipError_t launch_field_sorted(hipStream_t st, int policy, const double* geom, const double* nodes,
-                               const double* recs, const float* nodes32, double fp64_ratio, const double* sigma,
-                               const int32_t* pset, int64_t nelem, const double* clusters, const int32_t* order,
-                               const double* px, const double* py, const double* pz, int64_t npts, double* phi,
-                               double* ex, double* ey, double* ez, void* sort_ws, size_t sort_ws_bytes, double* ws,
-                               int64_t ws_doubles, hipEvent_t far_begin = nullptr, hipEvent_t far_end = nullptr);
+hipError_t launch_field_sorted(hipStream_t st, int policy, const FieldMesh& m, bool mixed, double fp64_ratio,
+                               const double* sigma, const int32_t* pset, int64_t nsets, const double* px,
+                               const double* py, const double* pz, int64_t npts, double* phi, double* ex, double* ey,
+                               double* ez, void* sort_ws, size_t sort_ws_bytes, double* ws, int64_t ws_doubles,
+                               hipEvent_t far_begin = nullptr, hipEvent_t far_end = nullptr);
 
 // *bad = 1 when a set index lies outside [0, nsets) (*bad must be 0 on entry)
 hipError_t launch_check_sets(hipStream_t st, const int32_t* pset, int64_t n, int64_t nsets, int* bad);

@@ -31,7 +31,7 @@ constexpr int kBlock = 256;
 template <bool MULTI>
 __global__ __launch_bounds__(kBlock) void k_field_far(const double* __restrict__ geom, const double* __restrict__ nodes,
                                                       const double* __restrict__ sigma,
-                                                      const int32_t* __restrict__ pset, int64_t nelem,
+                                                      const int32_t* __restrict__ pset, int64_t nsets, int64_t nelem,
                                                       int64_t per_split, const double* __restrict__ px,
                                                       const double* __restrict__ py, const double* __restrict__ pz,
                                                       int64_t npts, double* __restrict__ out0, double* __restrict__ out1,
@@ -42,7 +42,7 @@ __global__ __launch_bounds__(kBlock) void k_field_far(const double* __restrict__
     const bool live = i < npts;
     if (!live) i = npts - 1;
     const double x = px[i], y = py[i], z = pz[i];
-    const double* __restrict__ sg = MULTI ? sigma + (int64_t)pset[i] * nelem : sigma;
+    const double* __restrict__ sg = MULTI ? sigma + pset[i] : sigma;  // element-major sets
     const int64_t j0 = (int64_t)blockIdx.y * per_split;
     int64_t j1 = j0 + per_split;
     if (j1 > nelem) j1 = nelem;
@@ -53,7 +53,7 @@ __global__ __launch_bounds__(kBlock) void k_field_far(const double* __restrict__
         const double r2 = dev::dist2(g, x, y, z);
         if (!dev::is_near2(g, r2, 0)) {
             const F4 v = dev::element_far(g, nodes + j * KEM_NODE_STRIDE, (int)g[KG_TYPE], r2, x, y, z);
-            const double sj = sg[j];
+            const double sj = sg[MULTI ? j * nsets : j];
             dev::kahan_add(v.x, sj, s0, c0);
             dev::kahan_add(v.y, sj, s1, c1);
             dev::kahan_add(v.z, sj, s2, c2);
@@ -78,7 +78,7 @@ __global__ __launch_bounds__(kBlock) void k_field_far32(const double* __restrict
                                                         const double* __restrict__ nodes,
                                                         const float* __restrict__ nodes32, double fp64_ratio,
                                                         const double* __restrict__ sigma,
-                                                        const int32_t* __restrict__ pset, int64_t nelem,
+                                                        const int32_t* __restrict__ pset, int64_t nsets, int64_t nelem,
                                                         int64_t per_split, const double* __restrict__ px,
                                                         const double* __restrict__ py, const double* __restrict__ pz,
                                                         int64_t npts, double* __restrict__ out0,
@@ -89,7 +89,7 @@ __global__ __launch_bounds__(kBlock) void k_field_far32(const double* __restrict
     const bool live = i < npts;
     if (!live) i = npts - 1;
     const double x = px[i], y = py[i], z = pz[i];
-    const double* __restrict__ sg = MULTI ? sigma + (int64_t)pset[i] * nelem : sigma;
+    const double* __restrict__ sg = MULTI ? sigma + pset[i] : sigma;  // element-major sets
     const int64_t j0 = (int64_t)blockIdx.y * per_split;
     int64_t j1 = j0 + per_split;
     if (j1 > nelem) j1 = nelem;
@@ -104,14 +104,14 @@ __global__ __launch_bounds__(kBlock) void k_field_far32(const double* __restrict
             const float dx = (float)(x - g[KG_CX]), dy = (float)(y - g[KG_CY]), dz = (float)(z - g[KG_CZ]);
             float s0, s1, s2, s3;
             dev::element_far32(g, nodes32 + j * KEM_NODE32_STRIDE, type, r2, dx, dy, dz, s0, s1, s2, s3);
-            const double wj = g[KG_PREF] * sg[j];
+            const double wj = g[KG_PREF] * sg[MULTI ? j * nsets : j];
             a0 += (double)s0 * wj;
             a1 += (double)s1 * wj;
             a2 += (double)s2 * wj;
             a3 += (double)s3 * wj;
         } else {
             const F4 v = dev::element_far(g, nodes + j * KEM_NODE_STRIDE, type, r2, x, y, z);
-            const double sj = sg[j];
+            const double sj = sg[MULTI ? j * nsets : j];
             a0 += v.x * sj;
             a1 += v.y * sj;
             a2 += v.z * sj;
@@ -134,7 +134,7 @@ template <int PC, bool MULTI>
 __global__ __launch_bounds__(kBlock) void k_field_near(const double* __restrict__ geom,
                                                        const double* __restrict__ recs,
                                                        const double* __restrict__ sigma,
-                                                       const int32_t* __restrict__ pset, int64_t nelem,
+                                                       const int32_t* __restrict__ pset, int64_t nsets, int64_t nelem,
                                                        int64_t per_split, const double* __restrict__ px,
                                                        const double* __restrict__ py, const double* __restrict__ pz,
                                                        int64_t npts, int policy, int accumulate,
@@ -146,7 +146,7 @@ __global__ __launch_bounds__(kBlock) void k_field_near(const double* __restrict_
     const bool live = i < npts;
     if (!live) i = npts - 1;
     const double x = px[i], y = py[i], z = pz[i];
-    const double* __restrict__ sg = MULTI ? sigma + (int64_t)pset[i] * nelem : sigma;
+    const double* __restrict__ sg = MULTI ? sigma + pset[i] : sigma;  // element-major sets
     const int64_t j0 = (int64_t)blockIdx.y * per_split;
     int64_t j1 = j0 + per_split;
     if (j1 > nelem) j1 = nelem;
@@ -164,7 +164,7 @@ __global__ __launch_bounds__(kBlock) void k_field_near(const double* __restrict_
         if (PC == 1 || dev::is_near2(g, dev::dist2(g, x, y, z), policy)) {
             const F4 v = PC == 1 ? dev::element_policy(g, recs + j * KEM_REC, policy, x, y, z)
                                  : dev::element_near(g, (int)g[KG_TYPE], x, y, z);
-            const double sj = sg[j];
+            const double sj = sg[MULTI ? j * nsets : j];
             dev::kahan_add(v.x, sj, s0, c0);
             dev::kahan_add(v.y, sj, s1, c1);
             dev::kahan_add(v.z, sj, s2, c2);
@@ -198,7 +198,7 @@ __device__ __forceinline__ double wave_max(double v)
 template <bool MULTI>
 __global__ __launch_bounds__(kBlock) void k_field_near_culled(const double* __restrict__ geom,
                                                               const double* __restrict__ sigma,
-                                                              const int32_t* __restrict__ pset,
+                                                              const int32_t* __restrict__ pset, int64_t nsets,
                                                               const double* __restrict__ clusters, int64_t ncl,
                                                               const int32_t* __restrict__ order, int64_t nelem,
                                                               const double* __restrict__ px,
@@ -211,7 +211,7 @@ __global__ __launch_bounds__(kBlock) void k_field_near_culled(const double* __re
     const bool live = i < npts;
     if (!live) i = npts - 1;
     const double x = px[i], y = py[i], z = pz[i];
-    const double* __restrict__ sg = MULTI ? sigma + (int64_t)pset[i] * nelem : sigma;
+    const double* __restrict__ sg = MULTI ? sigma + pset[i] : sigma;  // element-major sets
     // bounding sphere of the wavefront's points
     const double lx = wave_min(x), ly = wave_min(y), lz = wave_min(z);
     const double hx = wave_max(x), hy = wave_max(y), hz = wave_max(z);
@@ -231,7 +231,7 @@ __global__ __launch_bounds__(kBlock) void k_field_near_culled(const double* __re
             const double* g = geom + j * KG_STRIDE;
             if (dev::is_near2(g, dev::dist2(g, x, y, z), 0)) {
                 const F4 v = dev::element_near(g, (int)g[KG_TYPE], x, y, z);
-                const double sj = sg[j];
+                const double sj = sg[MULTI ? j * nsets : j];
                 dev::kahan_add(v.x, sj, s0, c0);
                 dev::kahan_add(v.y, sj, s1, c1);
                 dev::kahan_add(v.z, sj, s2, c2);
@@ -244,6 +244,151 @@ __global__ __launch_bounds__(kBlock) void k_field_near_culled(const double* __re
         out1[i] = s1;
         out2[i] = s2;
         out3[i] = s3;
+    }
+}
+
+// A value every lane holds alike, moved to scalar registers so branches on it are uniform.
+__device__ __forceinline__ double uniform(double v)
+{
+    const int lo = __builtin_amdgcn_readfirstlane(__double2loint(v));
+    const int hi = __builtin_amdgcn_readfirstlane(__double2hiint(v));
+    return __hiloint2double(hi, lo);
+}
+
+typedef float v16f __attribute__((ext_vector_type(16)));
+
+// the double whose low and high words are the bits of lo and hi
+__device__ __forceinline__ double f2d(float lo, float hi)
+{
+    return __hiloint2double(__float_as_int(hi), __float_as_int(lo));
+}
+
+// weights of the degree-4 6-point rule's two orbits as the FP32 node table stores them
+constexpr float kW6a = (float)0.223381589678011, kW6b = (float)0.109951743655322;
+
+// one FP32 quadrature node at offset (ex, ey, ez) from the point, weight w (dev::cubature32)
+__device__ __forceinline__ void node32(float ex, float ey, float ez, float w, float& s0, float& s1, float& s2,
+                                       float& s3)
+{
+    const float ir = __builtin_amdgcn_rsqf(ex * ex + ey * ey + ez * ez);
+    const float wi = w * ir;
+    const float f = wi * ir * ir;
+    s0 += ex * f;
+    s1 += ey * f;
+    s2 += ez * f;
+    s3 += wi;
+}
+
+// Per-call element weights of the far pass, sets element-major (entry j*nsets + s for
+// element j, set s; sigma is set-major, sigma[s*nelem + j]):
+//   wk[k*nsets + s] = prefactor * sigma of the k-th element in cluster order (order != null),
+//   wk[j*nsets + s] = sigma (order == null, nsets > 1: the transposed sets).
+__global__ __launch_bounds__(kBlock) void k_far_weights(const double* __restrict__ geom,
+                                                        const int32_t* __restrict__ order,
+                                                        const double* __restrict__ sigma, int64_t nelem,
+                                                        int64_t nsets, double* __restrict__ wk)
+{
+    const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (t >= nsets * nelem) return;
+    const int64_t k = t / nsets, s = t - k * nsets;
+    if (order) {
+        const int64_t j = order[k];
+        wk[t] = geom[j * KG_STRIDE + KG_PREF] * sigma[s * nelem + j];
+    } else {
+        wk[t] = sigma[s * nelem + k];
+    }
+}
+
+// Mixed-precision far pass over element clusters.  Each wavefront bounds its 64 points by a
+// sphere; an element whose 6-point reach (KEM_FAR6_REACH) lies beyond that sphere is a 6-point
+// pair for every lane -- the per-pair ladder of k_field_far32 would pick the same rule -- so
+// the wavefront evaluates it from the element's far record without per-lane regime tests; a
+// cluster beyond its own reach skips even the per-element test.  Every other element takes
+// k_field_far32's per-pair path.  Each pair's FP32 result equals k_field_far32's (same nodes,
+// same rounding of P - centre); the FP64 sums run over elements in cluster order.  Elements
+// [c0*32, c1*32) of the cluster order per blockIdx.y; wk from k_far_weights.
+template <bool MULTI>
+__global__ __launch_bounds__(kBlock) void k_field_far32c(const double* __restrict__ geom,
+                                                         const float* __restrict__ nodes32,
+                                                         const float* __restrict__ far6,
+                                                         const double* __restrict__ far6_cl,
+                                                         const int32_t* __restrict__ order,
+                                                         const double* __restrict__ wk,
+                                                         const int32_t* __restrict__ pset, int64_t nsets, int64_t nelem,
+                                                         int64_t ncl, int64_t cl_per_split,
+                                                         const double* __restrict__ px,
+                                                         const double* __restrict__ py,
+                                                         const double* __restrict__ pz, int64_t npts,
+                                                         double* __restrict__ out0, double* __restrict__ out1,
+                                                         double* __restrict__ out2, double* __restrict__ out3,
+                                                         int64_t out_stride)
+{
+    int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const bool live = i < npts;
+    if (!live) i = npts - 1;
+    const double x = px[i], y = py[i], z = pz[i];
+    const double* __restrict__ wl = MULTI ? wk + pset[i] : wk;  // element-major sets
+    // bounding sphere of the wavefront's points, radius padded against rounding
+    const double lx = wave_min(x), ly = wave_min(y), lz = wave_min(z);
+    const double hx = wave_max(x), hy = wave_max(y), hz = wave_max(z);
+    const double wx = uniform(0.5 * (lx + hx)), wy = uniform(0.5 * (ly + hy)), wz = uniform(0.5 * (lz + hz));
+    const double wr =
+        uniform(0.5 * sqrt((hx - lx) * (hx - lx) + (hy - ly) * (hy - ly) + (hz - lz) * (hz - lz)) * (1. + 1e-9) +
+                1e-12);
+    const int64_t c0 = (int64_t)blockIdx.y * cl_per_split;
+    int64_t c1 = c0 + cl_per_split;
+    if (c1 > ncl) c1 = ncl;
+    double a0 = 0., a1 = 0., a2 = 0., a3 = 0.;
+    for (int64_t c = c0; c < c1; c++) {
+        const double* cl = far6_cl + 4 * c;
+        const double qx = wx - cl[0], qy = wy - cl[1], qz = wz - cl[2];
+        const double tc = cl[3] + wr;
+        const bool all6 = qx * qx + qy * qy + qz * qz > tc * tc;
+        int64_t k1 = (c + 1) * KEM_CLUSTER;
+        if (k1 > nelem) k1 = nelem;
+        for (int64_t k = c * KEM_CLUSTER; k < k1; k++) {
+            // the whole far record in two 16-dword loads: nodes 0-3, then nodes 4-5, centre, reach
+            const v16f* rv = reinterpret_cast<const v16f*>(far6 + k * KEM_FAR6_STRIDE);
+            const v16f A = rv[0], B = rv[1];
+            const double wj = wl[MULTI ? k * nsets : k];
+            const double cx = f2d(B[8], B[9]), cy = f2d(B[10], B[11]), cz = f2d(B[12], B[13]);
+            bool six = all6;
+            if (!six) {
+                const double ux = wx - cx, uy = wy - cy, uz = wz - cz;
+                const double te = f2d(B[14], B[15]) + wr;
+                six = ux * ux + uy * uy + uz * uz > te * te;
+            }
+            float s0, s1, s2, s3;
+            if (six) {
+                const float dx = (float)(x - cx), dy = (float)(y - cy), dz = (float)(z - cz);
+                s0 = s1 = s2 = s3 = 0.f;
+                node32(dx - A[0], dy - A[1], dz - A[2], kW6a, s0, s1, s2, s3);
+                node32(dx - A[4], dy - A[5], dz - A[6], kW6a, s0, s1, s2, s3);
+                node32(dx - A[8], dy - A[9], dz - A[10], kW6a, s0, s1, s2, s3);
+                node32(dx - A[12], dy - A[13], dz - A[14], kW6b, s0, s1, s2, s3);
+                node32(dx - B[0], dy - B[1], dz - B[2], kW6b, s0, s1, s2, s3);
+                node32(dx - B[4], dy - B[5], dz - B[6], kW6b, s0, s1, s2, s3);
+            } else {
+                const int64_t j = order[k];
+                const double* g = geom + j * KG_STRIDE;
+                const double r2 = dev::dist2(g, x, y, z);
+                if (dev::is_near2(g, r2, 0)) continue;
+                const float dx = (float)(x - g[KG_CX]), dy = (float)(y - g[KG_CY]), dz = (float)(z - g[KG_CZ]);
+                dev::element_far32(g, nodes32 + j * KEM_NODE32_STRIDE, (int)g[KG_TYPE], r2, dx, dy, dz, s0, s1, s2,
+                                   s3);
+            }
+            a0 += (double)s0 * wj;
+            a1 += (double)s1 * wj;
+            a2 += (double)s2 * wj;
+            a3 += (double)s3 * wj;
+        }
+    }
+    if (live) {
+        const int64_t o = (int64_t)blockIdx.y * out_stride + i;
+        out0[o] = a0;
+        out1[o] = a1;
+        out2[o] = a2;
+        out3[o] = a3;
     }
 }
 
@@ -600,20 +745,39 @@ inline unsigned nblocks(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock
 
 // ====================================================================== launchers
 namespace {
+constexpr int64_t kPartialDoubles = 4 * (int64_t)(2048 + 256) * 256;
+
 template <bool MULTI>
-void field_passes(hipStream_t st, int policy, const double* geom, const double* nodes, const double* recs,
-                  const float* nodes32, double fp64_ratio, const double* sigma, const int32_t* pset, int64_t nelem,
-                  const double* clusters, const int32_t* order, const double* px, const double* py,
-                  const double* pz, int64_t npts, double* phi, double* ex, double* ey, double* ez, double* ws,
-                  int64_t ws_doubles, hipEvent_t far_begin, hipEvent_t far_end)
+void field_passes(hipStream_t st, int policy, const FieldMesh& m, bool mixed, double fp64_ratio, const double* sigma,
+                  const int32_t* pset, int64_t nsets, const double* px, const double* py, const double* pz,
+                  int64_t npts, double* phi, double* ex, double* ey, double* ez, double* ws, int64_t ws_doubles,
+                  hipEvent_t far_begin, hipEvent_t far_end)
 {
+    const int64_t nelem = m.n;
     const unsigned gx = nblocks(npts);
-    // enough blocks to fill 256 CUs several times over; each range >= 32 elements
+    // ws: the cluster pass's element weights, then (several sets) the sets transposed to
+    // element-major order, then the partial sums of split element ranges
+    const bool clustered = policy == 0 && mixed && m.nodes32 && fp64_ratio == 0. && m.far6 && m.far6_cl && m.order;
+    const int64_t nsig = (nsets * nelem + 31) / 32 * 32;
+    double* wk = ws;
+    if (clustered) {
+        ws += nsig;
+        ws_doubles -= nsig;
+    }
+    const double* sg = sigma;
+    if (MULTI) {
+        k_far_weights<<<nblocks(nsets * nelem), kBlock, 0, st>>>(m.geom, nullptr, sigma, nelem, nsets, ws);
+        sg = ws;
+        ws += nsig;
+        ws_doubles -= nsig;
+    }
+    // enough blocks to fill 256 CUs several times over; each range >= 32 elements (one cluster)
     int64_t nsplit = (2048 + gx - 1) / gx;
     if (nsplit > (nelem + 31) / 32) nsplit = (nelem + 31) / 32;
     if (nsplit < 1) nsplit = 1;
     while (nsplit > 1 && 4 * nsplit * npts > ws_doubles) nsplit--;
-    const int64_t per = (nelem + nsplit - 1) / nsplit;
+    int64_t per = (nelem + nsplit - 1) / nsplit;
+    if (clustered) per = (per + KEM_CLUSTER - 1) / KEM_CLUSTER * KEM_CLUSTER;
     nsplit = (nelem + per - 1) / per;
     dim3 grid(gx, (unsigned)nsplit);
     double *o0 = ex, *o1 = ey, *o2 = ez, *o3 = phi;
@@ -625,45 +789,57 @@ void field_passes(hipStream_t st, int policy, const double* geom, const double* 
         o3 = ws + 3 * nsplit * npts;
         stride = npts;
     }
-    const bool culled = policy == 0 && clusters && order;
+    const bool culled = policy == 0 && m.clusters && m.order;
     if (far_begin) (void)hipEventRecord(far_begin, st);
-    if (policy == 0 && nodes32)
-        k_field_far32<MULTI><<<grid, kBlock, 0, st>>>(geom, nodes, nodes32, fp64_ratio, sigma, pset, nelem, per, px, py,
-                                                      pz, npts, o0, o1, o2, o3, stride);
-    else if (policy == 0)
-        k_field_far<MULTI><<<grid, kBlock, 0, st>>>(geom, nodes, sigma, pset, nelem, per, px, py, pz, npts, o0, o1, o2,
-                                                    o3, stride);
+    if (clustered) {
+        const int64_t ncl = (nelem + KEM_CLUSTER - 1) / KEM_CLUSTER;
+        k_far_weights<<<nblocks(nsets * nelem), kBlock, 0, st>>>(m.geom, m.order, sigma, nelem, nsets, wk);
+        k_field_far32c<MULTI><<<grid, kBlock, 0, st>>>(m.geom, m.nodes32, m.far6, m.far6_cl, m.order, wk, pset, nsets,
+                                                       nelem, ncl, per / KEM_CLUSTER, px, py, pz, npts, o0, o1, o2,
+                                                       o3, stride);
+    } else if (policy == 0 && mixed && m.nodes32) {
+        k_field_far32<MULTI><<<grid, kBlock, 0, st>>>(m.geom, m.nodes, m.nodes32, fp64_ratio, sg, pset, nsets, nelem,
+                                                      per, px, py, pz, npts, o0, o1, o2, o3, stride);
+    } else if (policy == 0) {
+        k_field_far<MULTI><<<grid, kBlock, 0, st>>>(m.geom, m.nodes, sg, pset, nsets, nelem, per, px, py, pz, npts, o0,
+                                                    o1, o2, o3, stride);
+    }
     if (far_end) (void)hipEventRecord(far_end, st);
     if (!culled) {
         if (policy >= 2)
-            k_field_near<1, MULTI><<<grid, kBlock, 0, st>>>(geom, recs, sigma, pset, nelem, per, px, py, pz, npts,
-                                                            policy, 0, o0, o1, o2, o3, stride);
+            k_field_near<1, MULTI><<<grid, kBlock, 0, st>>>(m.geom, m.recs, sg, pset, nsets, nelem, per, px, py, pz,
+                                                            npts, policy, 0, o0, o1, o2, o3, stride);
         else
-            k_field_near<0, MULTI><<<grid, kBlock, 0, st>>>(geom, recs, sigma, pset, nelem, per, px, py, pz, npts,
-                                                            policy, policy == 0 ? 1 : 0, o0, o1, o2, o3, stride);
+            k_field_near<0, MULTI><<<grid, kBlock, 0, st>>>(m.geom, m.recs, sg, pset, nsets, nelem, per, px, py, pz,
+                                                            npts, policy, policy == 0 ? 1 : 0, o0, o1, o2, o3, stride);
     }
     if (nsplit > 1) k_field_combine<<<gx, kBlock, 0, st>>>(ws, (int)nsplit, npts, phi, ex, ey, ez);
     if (culled) {
         const int64_t ncl = (nelem + KEM_CLUSTER - 1) / KEM_CLUSTER;
-        k_field_near_culled<MULTI><<<gx, kBlock, 0, st>>>(geom, sigma, pset, clusters, ncl, order, nelem, px, py, pz,
-                                                          npts, ex, ey, ez, phi);
+        k_field_near_culled<MULTI><<<gx, kBlock, 0, st>>>(m.geom, sg, pset, nsets, m.clusters, ncl, m.order, nelem,
+                                                          px, py, pz, npts, ex, ey, ez, phi);
     }
 }
 }  // namespace
 
-hipError_t launch_field(hipStream_t st, int policy, const double* geom, const double* nodes, const double* recs,
-                        const float* nodes32, double fp64_ratio, const double* sigma, const int32_t* pset,
-                        int64_t nelem, const double* clusters, const int32_t* order, const double* px,
-                        const double* py, const double* pz, int64_t npts, double* phi, double* ex, double* ey,
-                        double* ez, double* ws, int64_t ws_doubles, hipEvent_t far_begin, hipEvent_t far_end)
+int64_t field_workspace_doubles(int64_t nelem, int64_t nsets)
+{
+    return kPartialDoubles + 2 * ((nsets * nelem + 31) / 32 * 32);
+}
+
+hipError_t launch_field(hipStream_t st, int policy, const FieldMesh& m, bool mixed, double fp64_ratio,
+                        const double* sigma, const int32_t* pset, int64_t nsets, const double* px, const double* py,
+                        const double* pz, int64_t npts, double* phi, double* ex, double* ey, double* ez, double* ws,
+                        int64_t ws_doubles, hipEvent_t far_begin, hipEvent_t far_end)
 {
     if (npts <= 0) return hipSuccess;
+    if (nsets < 1 || ws_doubles < field_workspace_doubles(m.n, nsets)) return hipErrorInvalidValue;
     if (pset)
-        field_passes<true>(st, policy, geom, nodes, recs, nodes32, fp64_ratio, sigma, pset, nelem, clusters, order, px,
-                           py, pz, npts, phi, ex, ey, ez, ws, ws_doubles, far_begin, far_end);
+        field_passes<true>(st, policy, m, mixed, fp64_ratio, sigma, pset, nsets, px, py, pz, npts, phi, ex, ey, ez, ws,
+                           ws_doubles, far_begin, far_end);
     else
-        field_passes<false>(st, policy, geom, nodes, recs, nodes32, fp64_ratio, sigma, nullptr, nelem, clusters, order,
-                            px, py, pz, npts, phi, ex, ey, ez, ws, ws_doubles, far_begin, far_end);
+        field_passes<false>(st, policy, m, mixed, fp64_ratio, sigma, nullptr, 1, px, py, pz, npts, phi, ex, ey, ez,
+                            ws, ws_doubles, far_begin, far_end);
     return hipGetLastError();
 }
 

@@ -76,6 +76,17 @@ enum kem_geom_field {
    near pair in the cluster */
 #define KEM_CLUSTER 32
 
+/* far records of the mixed far field, one per element in cluster order (32 floats = 128 B):
+   the 6-point rule's nodes as (x, y, z, w) FP32 offsets from the element centre (slots
+   KEM_N6.. of the FP32 node table), then the element centre and the 6-point reach
+   sqrt((12 size)^2) * (1 + 1e-9) as FP64 (infinite for rectangles and segments).  A point
+   farther than the reach from the centre is a 6-point pair.  Per cluster: centre x, y, z and
+   the largest |c_j - centre| + reach_j (FP64), so a point farther than that from the centre
+   is a 6-point pair with every element of the cluster. */
+#define KEM_FAR6_STRIDE 32
+#define KEM_FAR6_CENTRE 24
+#define KEM_FAR6_REACH 30
+
 /* collocation target record (one per element): centre, unit normal, boundary type, flux */
 enum kem_target_field { KT_CX = 0, KT_CY, KT_CZ, KT_NX, KT_NY, KT_NZ, KT_BC, KT_FLUX, KT_STRIDE = 8 };
 

@@ -476,4 +476,31 @@ void build_clusters(int64_t n, const double* g, int32_t* order, double* clusters
     }
 }
 
+void build_far6(int64_t n, const double* g, const float* nodes32, const int32_t* order, const double* clusters,
+                float* far6, double* far6_cl)
+{
+    const int64_t ncl = (n + KEM_CLUSTER - 1) / KEM_CLUSTER;
+    for (int64_t c = 0; c < ncl; c++) {
+        const int64_t e0 = c * KEM_CLUSTER, e1 = std::min(n, e0 + KEM_CLUSTER);
+        double reach = 0.;
+        for (int64_t e = e0; e < e1; e++) {
+            const int64_t j = order[e];
+            const double* ge = g + j * (int64_t)KG_STRIDE;
+            float* r = far6 + e * KEM_FAR6_STRIDE;
+            for (int k = 0; k < KEM_FAR6_STRIDE; k++) r[k] = 0.f;
+            for (int k = 0; k < 24; k++) r[k] = nodes32[j * KEM_NODE32_STRIDE + 4 * KEM_N6 + k];
+            // the centre exactly as the element record holds it, so P - centre rounds as in
+            // the per-element path
+            double ce[4] = {ge[KG_CX], ge[KG_CY], ge[KG_CZ], INFINITY};
+            if ((int)ge[KG_TYPE] == KEM_TRIANGLE) ce[3] = std::sqrt(ge[KG_MIX6_2]) * (1. + 1e-9);
+            std::memcpy(r + KEM_FAR6_CENTRE, ce, sizeof ce);
+            const double d = std::sqrt(sq(ce[0] - clusters[4 * c]) + sq(ce[1] - clusters[4 * c + 1]) +
+                                       sq(ce[2] - clusters[4 * c + 2]));
+            reach = std::max(reach, d + ce[3]);
+        }
+        for (int k = 0; k < 3; k++) far6_cl[4 * c + k] = clusters[4 * c + k];
+        far6_cl[4 * c + 3] = reach * (1. + 1e-12);
+    }
+}
+
 }  // namespace kem

@@ -62,11 +62,11 @@ __device__ __forceinline__ uint64_t spread3(uint64_t a)
     return v;
 }
 
-// 63-bit Morton key of each point in the box spanned by the block partials; with a set index
-// per point the key is (set << 57) | (Morton >> 6), so points sort by set first
+// 63-bit Morton key of each point in the box spanned by the block partials (points of several
+// charge-density sets sort together: the kernels read the sets element-major, so a wavefront
+// may mix sets)
 __global__ __launch_bounds__(kBlock) void k_morton(const double* __restrict__ x, const double* __restrict__ y,
-                                                   const double* __restrict__ z, const int32_t* __restrict__ pset,
-                                                   int64_t n, const double* __restrict__ part, int nparts,
+                                                   const double* __restrict__ z, int64_t n, const double* __restrict__ part, int nparts,
                                                    uint64_t* __restrict__ key, int32_t* __restrict__ idx)
 {
     const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
@@ -85,7 +85,7 @@ __global__ __launch_bounds__(kBlock) void k_morton(const double* __restrict__ x,
         q = q < 0. ? 0. : (q > 2097151.0 ? 2097151.0 : q);
         code |= spread3((uint64_t)q) << k;
     }
-    key[i] = pset ? ((uint64_t)(uint32_t)pset[i] << 57) | (code >> 6) : code;
+    key[i] = code;
     idx[i] = (int32_t)i;
 }
 
@@ -139,12 +139,11 @@ size_t sort_workspace_bytes(int64_t n)
            align256(cub_temp_bytes(n));
 }
 
-hipError_t launch_field_sorted(hipStream_t st, int policy, const double* geom, const double* nodes,
-                               const double* recs, const float* nodes32, double fp64_ratio, const double* sigma,
-                               const int32_t* pset, int64_t nelem, const double* clusters, const int32_t* order,
-                               const double* px, const double* py, const double* pz, int64_t npts, double* phi,
-                               double* ex, double* ey, double* ez, void* sort_ws, size_t sort_ws_bytes, double* ws,
-                               int64_t ws_doubles, hipEvent_t far_begin, hipEvent_t far_end)
+hipError_t launch_field_sorted(hipStream_t st, int policy, const FieldMesh& m, bool mixed, double fp64_ratio,
+                               const double* sigma, const int32_t* pset, int64_t nsets, const double* px,
+                               const double* py, const double* pz, int64_t npts, double* phi, double* ex, double* ey,
+                               double* ez, void* sort_ws, size_t sort_ws_bytes, double* ws, int64_t ws_doubles,
+                               hipEvent_t far_begin, hipEvent_t far_end)
 {
     if (npts <= 0) return hipSuccess;
     if (npts > INT32_MAX || sort_ws_bytes < sort_workspace_bytes(npts)) return hipErrorInvalidValue;
@@ -171,12 +170,12 @@ hipError_t launch_field_sorted(hipStream_t st, int policy, const double* geom, c
     int nb = (int)((npts + kBlock - 1) / kBlock);
     if (nb > kBoxParts) nb = kBoxParts;
     k_bbox_part<<<nb, kBlock, 0, st>>>(px, py, pz, npts, box);
-    k_morton<<<nblocks(npts), kBlock, 0, st>>>(px, py, pz, pset, npts, box, nb, k0, i0);
-    hipError_t e = hipcub::DeviceRadixSort::SortPairs(tmp, tb, k0, k1, i0, i1, (int)npts, 0, pset ? 64 : 63, st);
+    k_morton<<<nblocks(npts), kBlock, 0, st>>>(px, py, pz, npts, box, nb, k0, i0);
+    hipError_t e = hipcub::DeviceRadixSort::SortPairs(tmp, tb, k0, k1, i0, i1, (int)npts, 0, 63, st);
     if (e != hipSuccess) return e;
     k_gather3<<<nblocks(npts), kBlock, 0, st>>>(i1, px, py, pz, pset, npts, sx, sy, sz, ss);
-    e = launch_field(st, policy, geom, nodes, recs, nodes32, fp64_ratio, sigma, pset ? ss : nullptr, nelem, clusters,
-                     order, sx, sy, sz, npts, o[0], o[1], o[2], o[3], ws, ws_doubles, far_begin, far_end);
+    e = launch_field(st, policy, m, mixed, fp64_ratio, sigma, pset ? ss : nullptr, nsets, sx, sy, sz, npts, o[0], o[1],
+                     o[2], o[3], ws, ws_doubles, far_begin, far_end);
     if (e != hipSuccess) return e;
     k_scatter4<<<nblocks(npts), kBlock, 0, st>>>(i1, o[0], o[1], o[2], o[3], npts, phi, ex, ey, ez);
     return hipGetLastError();

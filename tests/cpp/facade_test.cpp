@@ -97,6 +97,36 @@ int main()
     check(std::fabs(em.second - ep.second) < 1e-6 * std::fabs(ep.second), "mixed field == FP64 field (1e-6)",
           std::fabs(em.second - ep.second) / std::fabs(ep.second));
 
+    // the same cube from rectangles, 6 x 6 per face (CubeCapacitanceTest.cc:106-116: the
+    // capacitance 0.6606785 within 1e-2)
+    SurfaceData r;
+    const int nr = 6;
+    for (int axis = 0; axis < 3; axis++)
+        for (int sign = -1; sign <= 1; sign += 2)
+            for (int a = 0; a < nr; a++)
+                for (int b = 0; b < nr; b++) {
+                    const int u = (axis + 1) % 3, v = (axis + 2) % 3;
+                    auto P = [&](int ia, int ib) {
+                        Position p{};
+                        p[axis] = 0.5 * sign;
+                        p[u] = -0.5 + (double)ia / nr;
+                        p[v] = -0.5 + (double)ib / nr;
+                        return p;
+                    };
+                    if (sign > 0)
+                        r.AddRectangle(P(a, b), P(a + 1, b), P(a, b + 1), 1.);
+                    else
+                        r.AddRectangle(P(a, b), P(a, b + 1), P(a + 1, b), 1.);
+                }
+    KSurfaceContainer rcont(ctx, r);
+    KBoundaryIntegralMatrix Ar(rcont, integ);
+    std::vector<double> xr2;
+    gmres.Solve(Ar, rcont.BoundaryValues(), xr2);
+    double Qr = 0.;
+    for (double sig : xr2) Qr += sig / (nr * nr);
+    const double Cr = Qr / (4. * M_PI * 8.8541878128E-12);
+    check(std::fabs(Cr - 0.6606785) < 1e-2, "rectangle cube capacitance (0.6606785, 1e-2)", Cr);
+
     bool threw = false;
     try {
         KElectrostaticBoundaryIntegratorFactory::Make(ctx, "no-such-integrator");

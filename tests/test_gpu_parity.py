@@ -433,3 +433,64 @@ def test_field_mixed_fp64_band(ctx, coax_small):
     assert np.linalg.norm(Eb - E64) / np.linalg.norm(E64) < 1e-13
     assert (np.abs(pb - p64) / 18600.0).max() < 1e-13
     assert ctx.lib.kem_ctx_set_mixed_fp64_ratio(ctx.h, -1.0) != 0
+
+
+def _mixed_both_ways(ctx, fn):
+    """fn() under the cluster far pass (FP64 band 0) and under the per-pair far pass (a band
+    that rounds to an empty one): the two choose the same rule for every pair."""
+    a = fn()
+    ctx.check(ctx.lib.kem_ctx_set_mixed_fp64_ratio(ctx.h, 1e-300))
+    try:
+        b = fn()
+    finally:
+        ctx.check(ctx.lib.kem_ctx_set_mixed_fp64_ratio(ctx.h, 0.0))
+    return a, b
+
+
+@pytest.mark.parametrize("npts", [700, 3000, 20000])
+def test_field_mixed_cluster_pass(ctx, coax_small, npts):
+    """The cluster far pass (6-point pairs decided per wavefront and element) reproduces the
+    per-pair far pass: the same rule for every pair, FP64 sums over elements in another order.
+    The wavefront-uniform 6-point code is a separate FP32 instruction sequence, so pair results
+    may differ in the last FP32 bit: bounds 1e-9 (field, norm-wise) and 2e-8 (potential / V),
+    where one element dropped or counted twice would show at 1e-4.  700 / 3000 points run
+    unsorted with element ranges split over blocks (rarely wavefront-uniform: equal to 1e-15),
+    20000 in Morton order."""
+    from kassiopeia_amd import kemfield as K
+
+    m, cont, x = coax_small
+    pts = G.points_annulus(npts, 0.5, 1.0, 2.0, 5)
+    (Ec, pc), (Ep, pp) = _mixed_both_ways(
+        ctx, lambda: K.KIntegratingElectrostaticFieldSolver(cont, x, "mixed").ElectricFieldAndPotential(pts))
+    assert np.linalg.norm(Ec - Ep) / np.linalg.norm(Ep) < 1e-9
+    assert (np.abs(pc - pp) / 18600.0).max() < 2e-8
+
+
+def test_field_mixed_cluster_pass_shapes_and_sets(ctx):
+    """Rectangles and wires (no 6-point rule: always the per-pair path) among triangles, points
+    far and near, and several charge-density sets in one pass (bounds as above)."""
+    from kassiopeia_amd import kemfield as K
+
+    tri = G.cube_mesh(4)
+    rect = G.cube_rect_mesh(3, edge=0.6)
+    rng = np.random.default_rng(19)
+    p0 = rng.normal(size=(10, 3)) * 0.3
+    wires = G.Mesh(np.full(10, 2, np.int32), G.line_records(p0, p0 + rng.normal(size=(10, 3)) * 0.2, 1e-3),
+                   np.zeros(10, np.int32), np.ones(10), np.zeros(10), np.zeros(10, np.int32))
+    m = G.concat(tri, rect, wires)
+    cont = K.KSurfaceContainer(ctx, m)
+    pts = np.concatenate([G.points_shell(6000, 0.05, 30.0, 8), m.centroids() + 1e-3])
+    sigma = rng.normal(size=m.n)
+    (Ec, pc), (Ep, pp) = _mixed_both_ways(
+        ctx, lambda: K.KIntegratingElectrostaticFieldSolver(cont, sigma, "mixed").ElectricFieldAndPotential(pts))
+    assert np.linalg.norm(Ec - Ep) / np.linalg.norm(Ep) < 1e-9
+    assert np.abs(pc - pp).max() / np.abs(pp).max() < 2e-8
+    S = rng.normal(size=(m.n, 3))
+    sid = rng.integers(0, 3, size=len(pts)).astype(np.int32)
+    (Es, ps), (Eq, pq) = _mixed_both_ways(ctx, lambda: K.field_sets(cont, S, sid, pts, "mixed"))
+    assert np.linalg.norm(Es - Eq) / np.linalg.norm(Eq) < 1e-9
+    assert np.abs(ps - pq).max() / np.abs(pq).max() < 2e-8
+    for k in range(3):  # each set against its own single-set pass
+        sel = sid == k
+        Ek, pk = K.KIntegratingElectrostaticFieldSolver(cont, S[:, k], "mixed").ElectricFieldAndPotential(pts[sel])
+        assert np.linalg.norm(Es[sel] - Ek) / np.linalg.norm(Ek) < 1e-9
