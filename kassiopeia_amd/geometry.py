@@ -123,6 +123,32 @@ def _mesh(recs, shape, volts, group, bc_type=None, flux=None) -> Mesh:
     )
 
 
+def read_triangle_file(path) -> Mesh:
+    """KEMField's triangle input cards (the sphericalCapacitorFiles format), read as
+    ReadInTriangles does (Applications/Calculation/ComputeSphericalCapacitor.cc:1403-1476):
+    '#' starts a comment, a one-token line is the count, every other line holds 4 integers
+    (index, group, unused, replicas) and 15 doubles d = P0 (3), N1 (3), N2 (3), A, B, potential,
+    permittivity above, permittivity below.  A triangle is Dirichlet at the potential when
+    |below / above| < 1e-10, else Neumann with normal boundary flux below / above.  The shape
+    record is taken as written ([A, B, P0, N1, N2], no renormalisation).  A path ending in .gz is
+    read through gzip."""
+    import gzip
+
+    opener = gzip.open if str(path).endswith(".gz") else open
+    rows = []
+    with opener(path, "rt") as f:
+        for line in f:
+            tok = line.split("#", 1)[0].split()
+            if len(tok) < 19:
+                continue
+            rows.append([float(v) for v in tok[4:19]])
+    d = np.array(rows, dtype=np.float64).reshape(-1, 15)
+    recs = np.concatenate([d[:, 9:11], d[:, 0:9]], axis=1)
+    neumann = np.abs(d[:, 13] / d[:, 12]) >= 1e-10
+    return _mesh(recs, TRIANGLE, np.where(neumann, 0.0, d[:, 11]), 0, bc_type=neumann.astype(np.int32),
+                 flux=np.where(neumann, d[:, 13] / d[:, 12], 0.0))
+
+
 def concat(*meshes: Mesh) -> Mesh:
     return Mesh(*[np.concatenate([getattr(m, f.name) for m in meshes])
                   for f in dataclasses.fields(Mesh)])
