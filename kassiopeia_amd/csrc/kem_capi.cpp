@@ -463,7 +463,7 @@ int field_eval(kem_ctx* c, const kem_mesh* m, const double* sigma, const int32_t
         t1 = c->ev[2 * c->ev_used + 1];
         c->ev_used++;
     }
-    KEM_CK(c, ensure_ws(c, kem::field_workspace_doubles(m->n, nsets)));
+    KEM_CK(c, ensure_ws(c, kem::field_workspace_doubles(c->policy, fm, nsets, npts)));
     if (npts >= kSortMinPoints && npts <= INT32_MAX) {
         // visit the points in Morton order so the lanes of a wavefront share quadrature rules
         const size_t bytes = kem::sort_workspace_bytes(npts);
@@ -507,7 +507,11 @@ int kem_field_graph_create(kem_ctx* c, const kem_mesh* m, const double* sigma, i
     if (precision != KEM_FP64 && precision != KEM_MIXED) return fail(c, KEM_E_ARG, "field: unknown precision mode");
     KEM_CK(c, enter(c));
     // workspaces reach their final size before the capture (no allocation inside a graph)
-    KEM_CK(c, ensure_ws(c, kem::field_workspace_doubles(m->n, 1)));
+    {
+        kem::FieldMesh fm;
+        field_mesh(m, fm);
+        KEM_CK(c, ensure_ws(c, kem::field_workspace_doubles(c->policy, fm, 1, npts)));
+    }
     if (npts >= kSortMinPoints && npts <= INT32_MAX) {
         const size_t bytes = kem::sort_workspace_bytes(npts);
         if (c->sort_ws_bytes < bytes) {

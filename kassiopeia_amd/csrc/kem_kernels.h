@@ -22,7 +22,7 @@ struct FieldMesh {
 };
 
 // Doubles of the ws argument of launch_field for npts points and nsets charge-density sets.
-int64_t field_workspace_doubles(int64_t nelem, int64_t nsets);
+int64_t field_workspace_doubles(int policy, const FieldMesh& m, int64_t nsets, int64_t npts);
 
 // phi/E at npts points (SoA) from the mesh's elements with charge densities sigma.  pset
 // (nullable): per-point index of the charge-density set, point i using
@@ -30,8 +30,8 @@ int64_t field_workspace_doubles(int64_t nelem, int64_t nsets);
 // (FP32 node sums beyond distance ratio fp64_ratio, FP64 below it, FP64 near field and
 // accumulation); with fp64_ratio = 0 and far6 given it runs over element clusters.  clusters /
 // order select the culled near pass under the numeric policy; null scans every element.
-// ws: scratch of ws_doubles >= field_workspace_doubles(n, nsets) doubles (element-range
-// partial sums, per-set element weights).  far_begin / far_end (nullable): events recorded on
+// ws: scratch of ws_doubles >= field_workspace_doubles(policy, m, nsets, npts) doubles
+// (element-range partial sums, per-set element weights).  far_begin / far_end (nullable): events recorded on
 // st around the far-field pass.
 hipError_t launch_field(hipStream_t st, int policy, const FieldMesh& m, bool mixed, double fp64_ratio,
                         const double* sigma, const int32_t* pset, int64_t nsets, const double* px, const double* py,

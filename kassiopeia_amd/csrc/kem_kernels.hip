@@ -22,6 +22,19 @@ namespace {
 constexpr int kBlock = 256;
 
 // ------------------------------------------------------------------ field evaluation
+// Point block and element range of a far-pass block.  xcd = 0: blockIdx.x and blockIdx.y.
+// xcd = 1: a 1-D grid with 8 element ranges per point block, block L taking range L % 8 of
+// point block L / 8.  The hardware hands consecutive blocks to the 8 XCDs in turn, so all the
+// blocks one XCD runs read the same eighth of the elements, which then stays in that XCD's L2.
+struct FarTile {
+    int64_t pb, split;
+};
+__device__ __forceinline__ FarTile far_tile(int xcd)
+{
+    if (xcd) return {(int64_t)(blockIdx.x >> 3), (int64_t)(blockIdx.x & 7)};
+    return {(int64_t)blockIdx.x, (int64_t)blockIdx.y};
+}
+
 // One lane per field point.  Elements [j0, j1) are summed in container order with Kahan
 // compensation per component (KElectrostaticIntegratingFieldSolver.hh:303-339).  The far pass
 // evaluates the cubature / Gauss-Legendre pairs; the near pass then adds the closed-form pairs
@@ -36,14 +49,15 @@ __global__ __launch_bounds__(kBlock) void k_field_far(const double* __restrict__
                                                       const double* __restrict__ py, const double* __restrict__ pz,
                                                       int64_t npts, double* __restrict__ out0, double* __restrict__ out1,
                                                       double* __restrict__ out2, double* __restrict__ out3,
-                                                      int64_t out_stride)
+                                                      int64_t out_stride, int xcd)
 {
-    int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const FarTile t = far_tile(xcd);
+    int64_t i = t.pb * kBlock + threadIdx.x;
     const bool live = i < npts;
     if (!live) i = npts - 1;
     const double x = px[i], y = py[i], z = pz[i];
     const double* __restrict__ sg = MULTI ? sigma + pset[i] : sigma;  // element-major sets
-    const int64_t j0 = (int64_t)blockIdx.y * per_split;
+    const int64_t j0 = t.split * per_split;
     int64_t j1 = j0 + per_split;
     if (j1 > nelem) j1 = nelem;
     double s0 = 0., s1 = 0., s2 = 0., s3 = 0.;
@@ -61,7 +75,7 @@ __global__ __launch_bounds__(kBlock) void k_field_far(const double* __restrict__
         }
     }
     if (live) {
-        const int64_t o = (int64_t)blockIdx.y * out_stride + i;
+        const int64_t o = t.split * out_stride + i;
         out0[o] = s0;
         out1[o] = s1;
         out2[o] = s2;
@@ -83,14 +97,15 @@ __global__ __launch_bounds__(kBlock) void k_field_far32(const double* __restrict
                                                         const double* __restrict__ py, const double* __restrict__ pz,
                                                         int64_t npts, double* __restrict__ out0,
                                                         double* __restrict__ out1, double* __restrict__ out2,
-                                                        double* __restrict__ out3, int64_t out_stride)
+                                                        double* __restrict__ out3, int64_t out_stride, int xcd)
 {
-    int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const FarTile t = far_tile(xcd);
+    int64_t i = t.pb * kBlock + threadIdx.x;
     const bool live = i < npts;
     if (!live) i = npts - 1;
     const double x = px[i], y = py[i], z = pz[i];
     const double* __restrict__ sg = MULTI ? sigma + pset[i] : sigma;  // element-major sets
-    const int64_t j0 = (int64_t)blockIdx.y * per_split;
+    const int64_t j0 = t.split * per_split;
     int64_t j1 = j0 + per_split;
     if (j1 > nelem) j1 = nelem;
     const double band2 = fp64_ratio * fp64_ratio;
@@ -119,7 +134,7 @@ __global__ __launch_bounds__(kBlock) void k_field_far32(const double* __restrict
         }
     }
     if (live) {
-        const int64_t o = (int64_t)blockIdx.y * out_stride + i;
+        const int64_t o = t.split * out_stride + i;
         out0[o] = a0;
         out1[o] = a1;
         out2[o] = a2;
@@ -321,9 +336,10 @@ __global__ __launch_bounds__(kBlock) void k_field_far32c(const double* __restric
                                                          const double* __restrict__ pz, int64_t npts,
                                                          double* __restrict__ out0, double* __restrict__ out1,
                                                          double* __restrict__ out2, double* __restrict__ out3,
-                                                         int64_t out_stride)
+                                                         int64_t out_stride, int xcd)
 {
-    int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const FarTile t = far_tile(xcd);
+    int64_t i = t.pb * kBlock + threadIdx.x;
     const bool live = i < npts;
     if (!live) i = npts - 1;
     const double x = px[i], y = py[i], z = pz[i];
@@ -335,7 +351,7 @@ __global__ __launch_bounds__(kBlock) void k_field_far32c(const double* __restric
     const double wr =
         uniform(0.5 * sqrt((hx - lx) * (hx - lx) + (hy - ly) * (hy - ly) + (hz - lz) * (hz - lz)) * (1. + 1e-9) +
                 1e-12);
-    const int64_t c0 = (int64_t)blockIdx.y * cl_per_split;
+    const int64_t c0 = t.split * cl_per_split;
     int64_t c1 = c0 + cl_per_split;
     if (c1 > ncl) c1 = ncl;
     double a0 = 0., a1 = 0., a2 = 0., a3 = 0.;
@@ -384,7 +400,7 @@ __global__ __launch_bounds__(kBlock) void k_field_far32c(const double* __restric
         }
     }
     if (live) {
-        const int64_t o = (int64_t)blockIdx.y * out_stride + i;
+        const int64_t o = t.split * out_stride + i;
         out0[o] = a0;
         out1[o] = a1;
         out2[o] = a2;
@@ -746,6 +762,46 @@ inline unsigned nblocks(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock
 // ====================================================================== launchers
 namespace {
 constexpr int64_t kPartialDoubles = 4 * (int64_t)(2048 + 256) * 256;
+// From this many point blocks on, the numeric-policy far pass of one charge-density set splits
+// the elements into one range per XCD (far_tile); it runs over chunks of kXcdChunk points with 8
+// partial sums each.  (Measured on 1 M points x 20,000 triangles: FP64 399 -> 361 ms, mixed
+// 68 -> 60 ms.  Passes over several sets keep the single element range: with the per-lane set
+// weights the split measured 23 % slower.)
+constexpr int64_t kXcdMinBlocks = 1024;
+constexpr int64_t kXcdChunk = (int64_t)1 << 20;
+constexpr int kXcds = 8;
+
+bool xcd_split(int policy, const FieldMesh& m, int64_t npts)
+{
+    return policy == 0 && m.clusters && m.order && (int64_t)nblocks(npts) >= kXcdMinBlocks;
+}
+
+int64_t partial_doubles(int policy, const FieldMesh& m, int64_t npts)
+{
+    if (xcd_split(policy, m, npts)) return std::max(kPartialDoubles, 4 * kXcds * std::min(npts, kXcdChunk));
+    return kPartialDoubles;
+}
+
+template <bool MULTI>
+void far_kernel(hipStream_t st, dim3 grid, int kind, const FieldMesh& m, double fp64_ratio, const double* wk,
+                const double* sg, const int32_t* pset, int64_t nsets, int64_t per, const double* px,
+                const double* py, const double* pz, int64_t npts, double* o0, double* o1, double* o2, double* o3,
+                int64_t stride, int xcd)
+{
+    const int64_t nelem = m.n;
+    if (kind == 2) {
+        const int64_t ncl = (nelem + KEM_CLUSTER - 1) / KEM_CLUSTER;
+        k_field_far32c<MULTI><<<grid, kBlock, 0, st>>>(m.geom, m.nodes32, m.far6, m.far6_cl, m.order, wk, pset, nsets,
+                                                       nelem, ncl, per / KEM_CLUSTER, px, py, pz, npts, o0, o1, o2,
+                                                       o3, stride, xcd);
+    } else if (kind == 1) {
+        k_field_far32<MULTI><<<grid, kBlock, 0, st>>>(m.geom, m.nodes, m.nodes32, fp64_ratio, sg, pset, nsets, nelem,
+                                                      per, px, py, pz, npts, o0, o1, o2, o3, stride, xcd);
+    } else {
+        k_field_far<MULTI><<<grid, kBlock, 0, st>>>(m.geom, m.nodes, sg, pset, nsets, nelem, per, px, py, pz, npts, o0,
+                                                    o1, o2, o3, stride, xcd);
+    }
+}
 
 template <bool MULTI>
 void field_passes(hipStream_t st, int policy, const FieldMesh& m, bool mixed, double fp64_ratio, const double* sigma,
@@ -758,6 +814,7 @@ void field_passes(hipStream_t st, int policy, const FieldMesh& m, bool mixed, do
     // ws: the cluster pass's element weights, then (several sets) the sets transposed to
     // element-major order, then the partial sums of split element ranges
     const bool clustered = policy == 0 && mixed && m.nodes32 && fp64_ratio == 0. && m.far6 && m.far6_cl && m.order;
+    const int kind = clustered ? 2 : (policy == 0 && mixed && m.nodes32) ? 1 : 0;
     const int64_t nsig = (nsets * nelem + 31) / 32 * 32;
     double* wk = ws;
     if (clustered) {
@@ -771,49 +828,60 @@ void field_passes(hipStream_t st, int policy, const FieldMesh& m, bool mixed, do
         ws += nsig;
         ws_doubles -= nsig;
     }
-    // enough blocks to fill 256 CUs several times over; each range >= 32 elements (one cluster)
-    int64_t nsplit = (2048 + gx - 1) / gx;
-    if (nsplit > (nelem + 31) / 32) nsplit = (nelem + 31) / 32;
-    if (nsplit < 1) nsplit = 1;
-    while (nsplit > 1 && 4 * nsplit * npts > ws_doubles) nsplit--;
-    int64_t per = (nelem + nsplit - 1) / nsplit;
-    if (clustered) per = (per + KEM_CLUSTER - 1) / KEM_CLUSTER * KEM_CLUSTER;
-    nsplit = (nelem + per - 1) / per;
-    dim3 grid(gx, (unsigned)nsplit);
-    double *o0 = ex, *o1 = ey, *o2 = ez, *o3 = phi;
-    int64_t stride = 0;
-    if (nsplit > 1) {
-        o0 = ws;
-        o1 = ws + nsplit * npts;
-        o2 = ws + 2 * nsplit * npts;
-        o3 = ws + 3 * nsplit * npts;
-        stride = npts;
-    }
     const bool culled = policy == 0 && m.clusters && m.order;
-    if (far_begin) (void)hipEventRecord(far_begin, st);
-    if (clustered) {
-        const int64_t ncl = (nelem + KEM_CLUSTER - 1) / KEM_CLUSTER;
-        k_far_weights<<<nblocks(nsets * nelem), kBlock, 0, st>>>(m.geom, m.order, sigma, nelem, nsets, wk);
-        k_field_far32c<MULTI><<<grid, kBlock, 0, st>>>(m.geom, m.nodes32, m.far6, m.far6_cl, m.order, wk, pset, nsets,
-                                                       nelem, ncl, per / KEM_CLUSTER, px, py, pz, npts, o0, o1, o2,
-                                                       o3, stride);
-    } else if (policy == 0 && mixed && m.nodes32) {
-        k_field_far32<MULTI><<<grid, kBlock, 0, st>>>(m.geom, m.nodes, m.nodes32, fp64_ratio, sg, pset, nsets, nelem,
-                                                      per, px, py, pz, npts, o0, o1, o2, o3, stride);
-    } else if (policy == 0) {
-        k_field_far<MULTI><<<grid, kBlock, 0, st>>>(m.geom, m.nodes, sg, pset, nsets, nelem, per, px, py, pz, npts, o0,
-                                                    o1, o2, o3, stride);
+    if (!MULTI && xcd_split(policy, m, npts)) {
+        // one element range per XCD, 8 partial sums per point, in chunks of points
+        int64_t per = (nelem + kXcds - 1) / kXcds;
+        if (clustered) per = (per + KEM_CLUSTER - 1) / KEM_CLUSTER * KEM_CLUSTER;
+        if (far_begin) (void)hipEventRecord(far_begin, st);
+        if (clustered)
+            k_far_weights<<<nblocks(nsets * nelem), kBlock, 0, st>>>(m.geom, m.order, sigma, nelem, nsets, wk);
+        for (int64_t p0 = 0; p0 < npts; p0 += kXcdChunk) {
+            const int64_t np = std::min(kXcdChunk, npts - p0);
+            const unsigned gc = nblocks(np);
+            double* q = ws;
+            far_kernel<MULTI>(st, dim3(gc * kXcds), kind, m, fp64_ratio, wk, sg, MULTI ? pset + p0 : nullptr, nsets,
+                              per, px + p0, py + p0, pz + p0, np, q, q + kXcds * np, q + 2 * kXcds * np,
+                              q + 3 * kXcds * np, np, 1);
+            k_field_combine<<<gc, kBlock, 0, st>>>(ws, kXcds, np, phi + p0, ex + p0, ey + p0, ez + p0);
+        }
+        if (far_end) (void)hipEventRecord(far_end, st);
+    } else {
+        // enough blocks to fill 256 CUs several times over; each range >= 32 elements (one cluster)
+        int64_t nsplit = (2048 + gx - 1) / gx;
+        if (nsplit > (nelem + 31) / 32) nsplit = (nelem + 31) / 32;
+        if (nsplit < 1) nsplit = 1;
+        while (nsplit > 1 && 4 * nsplit * npts > ws_doubles) nsplit--;
+        int64_t per = (nelem + nsplit - 1) / nsplit;
+        if (clustered) per = (per + KEM_CLUSTER - 1) / KEM_CLUSTER * KEM_CLUSTER;
+        nsplit = (nelem + per - 1) / per;
+        dim3 grid(gx, (unsigned)nsplit);
+        double *o0 = ex, *o1 = ey, *o2 = ez, *o3 = phi;
+        int64_t stride = 0;
+        if (nsplit > 1) {
+            o0 = ws;
+            o1 = ws + nsplit * npts;
+            o2 = ws + 2 * nsplit * npts;
+            o3 = ws + 3 * nsplit * npts;
+            stride = npts;
+        }
+        if (far_begin) (void)hipEventRecord(far_begin, st);
+        if (clustered)
+            k_far_weights<<<nblocks(nsets * nelem), kBlock, 0, st>>>(m.geom, m.order, sigma, nelem, nsets, wk);
+        if (policy == 0) far_kernel<MULTI>(st, grid, kind, m, fp64_ratio, wk, sg, pset, nsets, per, px, py, pz, npts,
+                                           o0, o1, o2, o3, stride, 0);
+        if (far_end) (void)hipEventRecord(far_end, st);
+        if (!culled) {
+            if (policy >= 2)
+                k_field_near<1, MULTI><<<grid, kBlock, 0, st>>>(m.geom, m.recs, sg, pset, nsets, nelem, per, px, py,
+                                                                pz, npts, policy, 0, o0, o1, o2, o3, stride);
+            else
+                k_field_near<0, MULTI><<<grid, kBlock, 0, st>>>(m.geom, m.recs, sg, pset, nsets, nelem, per, px, py,
+                                                                pz, npts, policy, policy == 0 ? 1 : 0, o0, o1, o2, o3,
+                                                                stride);
+        }
+        if (nsplit > 1) k_field_combine<<<gx, kBlock, 0, st>>>(ws, (int)nsplit, npts, phi, ex, ey, ez);
     }
-    if (far_end) (void)hipEventRecord(far_end, st);
-    if (!culled) {
-        if (policy >= 2)
-            k_field_near<1, MULTI><<<grid, kBlock, 0, st>>>(m.geom, m.recs, sg, pset, nsets, nelem, per, px, py, pz,
-                                                            npts, policy, 0, o0, o1, o2, o3, stride);
-        else
-            k_field_near<0, MULTI><<<grid, kBlock, 0, st>>>(m.geom, m.recs, sg, pset, nsets, nelem, per, px, py, pz,
-                                                            npts, policy, policy == 0 ? 1 : 0, o0, o1, o2, o3, stride);
-    }
-    if (nsplit > 1) k_field_combine<<<gx, kBlock, 0, st>>>(ws, (int)nsplit, npts, phi, ex, ey, ez);
     if (culled) {
         const int64_t ncl = (nelem + KEM_CLUSTER - 1) / KEM_CLUSTER;
         k_field_near_culled<MULTI><<<gx, kBlock, 0, st>>>(m.geom, sg, pset, nsets, m.clusters, ncl, m.order, nelem,
@@ -822,9 +890,9 @@ void field_passes(hipStream_t st, int policy, const FieldMesh& m, bool mixed, do
 }
 }  // namespace
 
-int64_t field_workspace_doubles(int64_t nelem, int64_t nsets)
+int64_t field_workspace_doubles(int policy, const FieldMesh& m, int64_t nsets, int64_t npts)
 {
-    return kPartialDoubles + 2 * ((nsets * nelem + 31) / 32 * 32);
+    return partial_doubles(policy, m, npts) + 2 * ((nsets * m.n + 31) / 32 * 32);
 }
 
 hipError_t launch_field(hipStream_t st, int policy, const FieldMesh& m, bool mixed, double fp64_ratio,
@@ -833,7 +901,7 @@ hipError_t launch_field(hipStream_t st, int policy, const FieldMesh& m, bool mix
                         int64_t ws_doubles, hipEvent_t far_begin, hipEvent_t far_end)
 {
     if (npts <= 0) return hipSuccess;
-    if (nsets < 1 || ws_doubles < field_workspace_doubles(m.n, nsets)) return hipErrorInvalidValue;
+    if (nsets < 1 || ws_doubles < field_workspace_doubles(policy, m, nsets, npts)) return hipErrorInvalidValue;
     if (pset)
         field_passes<true>(st, policy, m, mixed, fp64_ratio, sigma, pset, nsets, px, py, pz, npts, phi, ex, ey, ez, ws,
                            ws_doubles, far_begin, far_end);

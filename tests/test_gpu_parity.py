@@ -494,3 +494,46 @@ def test_field_mixed_cluster_pass_shapes_and_sets(ctx):
         sel = sid == k
         Ek, pk = K.KIntegratingElectrostaticFieldSolver(cont, S[:, k], "mixed").ElectricFieldAndPotential(pts[sel])
         assert np.linalg.norm(Es[sel] - Ek) / np.linalg.norm(Ek) < 1e-9
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_field_xcd_split_pass(ctx, oracle, coax_small, precision):
+    """From 262,144 points on, the far pass gives each XCD one eighth of the elements and runs
+    over chunks of 2^20 points (1,100,000 points: two chunks).  The same points evaluated in
+    calls of 50,000 (element ranges split over blocks) agree to summation order (FP64) or to
+    the last FP32 bit of some pairs (mixed, bound 1e-9 as for the cluster pass); FP64 matches
+    the oracle on a sample."""
+    from kassiopeia_amd import kemfield as K
+
+    m, cont, x = coax_small
+    pts = G.points_annulus(1_100_000, 0.5, 1.0, 2.0, 13)
+    solver = K.KIntegratingElectrostaticFieldSolver(cont, x, precision)
+    E, phi = solver.ElectricFieldAndPotential(pts)
+    sel = np.arange(0, len(pts), 22)  # 50,000 points
+    Es, ps = solver.ElectricFieldAndPotential(pts[sel])
+    tol = 1e-14 if precision == "fp64" else 1e-9
+    assert np.linalg.norm(E[sel] - Es) / np.linalg.norm(Es) < tol
+    assert (np.abs(phi[sel] - ps) / 18600.0).max() < (1e-14 if precision == "fp64" else 2e-8)
+    if precision == "fp64":
+        few = sel[::250]
+        phr, Er = oracle.field(m.shapes, m.recs, x, pts[few])
+        assert (np.abs(phi[few] - phr) / 18600.0).max() < 1e-12
+        assert max(relerr(E[k], Er[i]) for i, k in enumerate(few)) < 1e-11
+
+
+def test_field_sets_large_pass(ctx, coax_small):
+    """Several charge-density sets in one pass over 300,000 points (3 sets; passes over several
+    sets keep one element range per block even at this size) against each set's own pass."""
+    from kassiopeia_amd import kemfield as K
+
+    m, cont, x = coax_small
+    rng = np.random.default_rng(23)
+    S = np.stack([x, -0.5 * x, x * rng.uniform(0.5, 1.5, size=x.size)], 1)
+    pts = G.points_annulus(300_000, 0.5, 1.0, 2.0, 17)
+    sid = rng.integers(0, 3, size=len(pts)).astype(np.int32)
+    for prec, tol in (("fp64", 1e-14), ("mixed", 1e-9)):
+        Es, ps = K.field_sets(cont, S, sid, pts, prec)
+        for k in range(3):
+            sel = sid == k
+            Ek, pk = K.KIntegratingElectrostaticFieldSolver(cont, S[:, k], prec).ElectricFieldAndPotential(pts[sel])
+            assert np.linalg.norm(Es[sel] - Ek) / np.linalg.norm(Ek) < tol, (prec, k)
