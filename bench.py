@@ -43,8 +43,8 @@ PEAK_HBM_GBS = 8000.0
 # 1,000,000 points): rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE of one dispatch, scaled by the
 # readings of a 1 GiB device copy (bench.py --copy-calibration: FETCH_SIZE x 1.99987, WRITE_SIZE
 # x 1.0).
-TRAFFIC_BYTES = {"k_field_far32c": (6108990.75 * 1.99987 + 31250.0) * 1024,
-                 "k_field_far": (183564770.6 * 1.99987 + 31250.0) * 1024}
+TRAFFIC_BYTES = {"k_field_far32c": (147108.25 * 1.99987 + 250000.0) * 1024,
+                 "k_field_far": (1202322.8 * 1.99987 + 250000.0) * 1024}
 
 
 def log(*a):
