@@ -67,6 +67,8 @@ def main():
     ap.add_argument("--batch-points", type=int, default=4_000_000)
     ap.add_argument("--workload", choices=["coax20k", "large"], default="coax20k",
                     help="large: 100,000-triangle ring bore, 16 voltage sets, 10 M points, adjoint")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the library's default launch shapes instead of tuning them on the workload")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 1 GiB device to device (reference kernel for the rocprofv3 byte counters)")
     args = ap.parse_args()
@@ -119,6 +121,20 @@ def main():
         sigma = x64
     else:
         sigma = np.random.default_rng(3).normal(size=N) * 1e-7
+
+    # ---------------------------------------------------------------- launch shapes
+    tuning = None
+    if not args.no_autotune:
+        from kassiopeia_amd import autotune as T
+
+        t0 = time.perf_counter()
+        tuning = {"far_range_mixed": T.tune_field(cont, sigma, pts, "mixed", repeats=1),
+                  "far_range_fp64": T.tune_field(cont, sigma, pts, "fp64", repeats=1)}
+        if A is not None:
+            tuning["gemv_blocks"] = T.tune_matvec(A, repeats=5)
+            tuning["bmm_blocks"] = T.tune_block_product(A, nrhs=64, precision="mixed", repeats=3)
+        log(f"autotune ({time.perf_counter() - t0:.1f} s): " +
+            ", ".join(f"{k} {v['best']}" for k, v in tuning.items()))
 
     # ---------------------------------------------------------------- field pass
     X = [K.DeviceArray(ctx, pts[:, k]) for k in range(3)]
@@ -306,6 +322,7 @@ def main():
                      "kernel_ms": kms, "flops_per_launch": kflops,
                      "pairs_by_regime": (cmx if mixed_ok else c64).astype(int).tolist()},
         "cpu_baseline": cpu,
+        "tuning": None if tuning is None else {k: v["best"] for k, v in tuning.items()},
     }
     line.update(out)
     print(json.dumps(line), flush=True)
@@ -360,6 +377,16 @@ def run_large(args):
         ctx.check(lib.kem_ctx_sync(ctx.h))
 
     out = {}
+    tuning = None
+    if not args.no_autotune:
+        # element-range sizes of the far pass, tuned on the first 2 M points with provisional
+        # charge densities (the time per pair does not depend on their values)
+        from kassiopeia_amd import autotune as T
+
+        t0 = time.perf_counter()
+        sig0 = np.ones(N) * 1e-9
+        tuning = {p: T.tune_field(cont, sig0, pts[:2_000_000], p, repeats=1)["best"] for p in ("mixed", "fp64")}
+        log(f"autotune ({time.perf_counter() - t0:.1f} s): far range mixed {tuning['mixed']}, fp64 {tuning['fp64']}")
     X = [K.DeviceArray(ctx, pts[:, k]) for k in range(3)]
     O = [K.DeviceArray.empty(ctx, M) for _ in range(4)]
 
@@ -464,6 +491,7 @@ def run_large(args):
         "mixed_solve_only_err_E_max_over_rms": err_solve_rms,
         "batch16_solve_s": stb.seconds, "batch16_worst_residual": float(res.max()),
         "adjoint_field_vjp_s": t_fvjp, "adjoint_solve_s": sta.seconds, "adjoint_matrix_vjp_s": t_mvjp,
+        "tuning": tuning,
     }
     print(json.dumps(line), flush=True)
 

@@ -102,10 +102,24 @@ const char* kem_last_error(const kem_ctx* ctx);
    rest run in FP32 (default 0: every far pair in FP32).  No reference counterpart: the
    reference has no mixed-precision mode. */
 int kem_ctx_set_mixed_fp64_ratio(kem_ctx* ctx, double ratio);
+/* Launch tuning (no reference counterpart; kassiopeia_amd/autotune.py measures and sets them):
+   KEM_TUNE_FAR_RANGE_FP64 / _MIXED = elements per element range of the far pass over 262,144
+   or more points of one charge-density set; the elements split into 8 R ranges, R per XCD, so
+   each XCD's range stays in its L2 (0: no split; default 2560).  KEM_TUNE_GEMV_BLOCKS /
+   KEM_TUNE_BMM_BLOCKS = target block counts of the split-K matrix-vector product (default 2048)
+   and of the multi-RHS block product (default 2560); they set how many partial sums are
+   reduced. */
+#define KEM_TUNE_FAR_RANGE_FP64 1
+#define KEM_TUNE_FAR_RANGE_MIXED 2
+#define KEM_TUNE_GEMV_BLOCKS 3
+#define KEM_TUNE_BMM_BLOCKS 4
+int kem_ctx_set_tuning(kem_ctx* ctx, int param, int64_t value);
+int kem_ctx_get_tuning(const kem_ctx* ctx, int param, int64_t* value);
 /* Instrumentation, no reference counterpart: with timing on, each field evaluation records HIP
-   events on the context's stream around its far-field kernel (up to 256 evaluations between
-   queries); kem_ctx_kernel_time waits for them and returns the summed kernel time and the
-   number of evaluations timed, then starts a new count. */
+   events on the context's stream around its far-field pass, each kem_matrix_multiply around
+   its matrix-vector kernels and each 64-column block of kem_matrix_multiply_batch around its
+   block product (up to 256 timed regions between queries); kem_ctx_kernel_time waits for them
+   and returns the summed time and the number of regions timed, then starts a new count. */
 int kem_ctx_set_kernel_timing(kem_ctx* ctx, int on);
 int kem_ctx_kernel_time(kem_ctx* ctx, double* total_ms, int* launches);
 /* Device memory on the context's GPU for callers without an allocator of their own (no

@@ -14,6 +14,7 @@ KEM_E_ARG, KEM_E_HIP, KEM_E_NAME, KEM_E_GEOM, KEM_E_STATE = -1, -2, -3, -4, -5
 KEM_FP64, KEM_MIXED = 0, 1
 KEM_STORE_FP64, KEM_STORE_FP32 = 1, 2
 KEM_SOLVER_GMRES, KEM_SOLVER_ROBIN_HOOD, KEM_SOLVER_GAUSS = 0, 1, 2
+KEM_TUNE_FAR_RANGE_FP64, KEM_TUNE_FAR_RANGE_MIXED, KEM_TUNE_GEMV_BLOCKS, KEM_TUNE_BMM_BLOCKS = 1, 2, 3, 4
 
 
 class KemError(RuntimeError):
@@ -62,6 +63,8 @@ SIGNATURES = {
     "kem_dev_free": (_INT, [_P, _P]),
     "kem_dev_upload": (_INT, [_P, _P, _P, _I64]),
     "kem_dev_download": (_INT, [_P, _P, _P, _I64]),
+    "kem_ctx_set_tuning": (_INT, [_P, _INT, _I64]),
+    "kem_ctx_get_tuning": (_INT, [_P, _INT, ctypes.POINTER(_I64)]),
     "kem_ctx_set_kernel_timing": (_INT, [_P, _INT]),
     "kem_ctx_kernel_time": (_INT, [_P, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)]),
     "kem_ctx_sync": (_INT, [_P]),

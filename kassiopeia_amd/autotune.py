@@ -1,0 +1,120 @@
+"""Autotuner of the library's launch shapes, measured on the device on the caller's own problem.
+
+The C-ABI exposes four launch-shape knobs (``kem_ctx_set_tuning``):
+
+* ``KEM_TUNE_FAR_RANGE_FP64`` / ``KEM_TUNE_FAR_RANGE_MIXED``: elements per element range of the
+  far field pass over large point sets.  The elements split into 8 R ranges, R per XCD, so the
+  range an XCD works on stays in its L2; 0 disables the split.
+* ``KEM_TUNE_GEMV_BLOCKS``: target block count of the split-K matrix-vector product (GMRES,
+  Robin Hood residuals, ``KSquareMatrix::Multiply``).
+* ``KEM_TUNE_BMM_BLOCKS``: target block count of the multi-right-hand-side block product.
+
+Each knob is swept over its candidates; every candidate runs once untimed and then ``repeats``
+times, timed by the HIP events the library records around the kernels the knob shapes
+(``kem_ctx_set_kernel_timing``).  The fastest candidate stays set on the context.  The results
+are plain dicts that ``save`` / ``load`` keep as JSON and ``apply`` sets on another context.
+"""
+import ctypes
+import json
+
+import numpy as np
+
+from . import _lib
+from . import kemfield as K
+
+FAR_RANGE_CANDIDATES = (0, 640, 1280, 2560, 5120, 10240)
+GEMV_CANDIDATES = (512, 1024, 2048, 4096, 8192)
+BMM_CANDIDATES = (640, 1280, 2560, 5120, 10240)
+
+_KNOB = {"far_range_fp64": _lib.KEM_TUNE_FAR_RANGE_FP64, "far_range_mixed": _lib.KEM_TUNE_FAR_RANGE_MIXED,
+         "gemv_blocks": _lib.KEM_TUNE_GEMV_BLOCKS, "bmm_blocks": _lib.KEM_TUNE_BMM_BLOCKS}
+
+
+def get(ctx: K.Context, name: str) -> int:
+    v = ctypes.c_int64()
+    ctx.check(ctx.lib.kem_ctx_get_tuning(ctx.h, _KNOB[name], ctypes.byref(v)))
+    return int(v.value)
+
+
+def set_knob(ctx: K.Context, name: str, value: int):
+    ctx.check(ctx.lib.kem_ctx_set_tuning(ctx.h, _KNOB[name], int(value)))
+
+
+def _sweep(ctx, name, candidates, run, repeats):
+    """ms per timed region for each candidate; leaves the fastest set."""
+    old = get(ctx, name)
+    times = {}
+    try:
+        for v in candidates:
+            set_knob(ctx, name, v)
+            run()  # untimed: workspaces grow, caches warm
+            ctx.check(ctx.lib.kem_ctx_sync(ctx.h))
+            ctx.check(ctx.lib.kem_ctx_set_kernel_timing(ctx.h, 1))
+            for _ in range(repeats):
+                run()
+            t, n = ctypes.c_double(), ctypes.c_int()
+            ctx.check(ctx.lib.kem_ctx_kernel_time(ctx.h, ctypes.byref(t), ctypes.byref(n)))
+            ctx.check(ctx.lib.kem_ctx_set_kernel_timing(ctx.h, 0))
+            times[int(v)] = t.value / max(n.value, 1)
+    except BaseException:
+        set_knob(ctx, name, old)
+        raise
+    best = min(times, key=times.get)
+    set_knob(ctx, name, best)
+    return {"best": best, "ms": times}
+
+
+def tune_field(container: K.KSurfaceContainer, sigma, points, precision="mixed", repeats=2,
+               candidates=FAR_RANGE_CANDIDATES):
+    """Element-range size of the far pass for this container, charge densities and points
+    (points: (m, 3) array; the split applies from 262,144 points on)."""
+    ctx = container.ctx
+    P = np.asarray(points, dtype=np.float64).reshape(-1, 3)
+    m = P.shape[0]
+    X = [K.DeviceArray(ctx, np.ascontiguousarray(P[:, k])) for k in range(3)]
+    S = K.DeviceArray(ctx, np.asarray(sigma, dtype=np.float64))
+    O = [K.DeviceArray.empty(ctx, m) for _ in range(4)]
+    prec = {"fp64": _lib.KEM_FP64, "mixed": _lib.KEM_MIXED}[precision]
+
+    def run():
+        ctx.check(ctx.lib.kem_field_eval_device(ctx.h, container.h, S.ptr, m, X[0].ptr, X[1].ptr, X[2].ptr, prec,
+                                                O[0].ptr, O[1].ptr, O[2].ptr, O[3].ptr))
+
+    try:
+        return _sweep(ctx, "far_range_" + precision, candidates, run, repeats)
+    finally:
+        for a in X + O + [S]:
+            a.close()
+
+
+def tune_matvec(matrix: K.KBoundaryIntegralMatrix, repeats=5, candidates=GEMV_CANDIDATES):
+    """Split-K target of the matrix-vector product (FP64 storage if present, else FP32)."""
+    x = np.random.default_rng(0).normal(size=matrix.n)
+    return _sweep(matrix.ctx, "gemv_blocks", candidates, lambda: matrix.Multiply(x), repeats)
+
+
+def tune_block_product(matrix: K.KBoundaryIntegralMatrix, nrhs=64, precision="mixed", repeats=3,
+                       candidates=BMM_CANDIDATES):
+    """Split-K target of the block product with nrhs right-hand sides (MFMA for "mixed")."""
+    X = np.random.default_rng(1).normal(size=(matrix.n, nrhs))
+    return _sweep(matrix.ctx, "bmm_blocks", candidates, lambda: matrix.MultiplyBatch(X, precision), repeats)
+
+
+def current(ctx: K.Context) -> dict:
+    return {name: get(ctx, name) for name in _KNOB}
+
+
+def apply(ctx: K.Context, table: dict):
+    """Sets every knob of table ({name: value} or the results of the tune_* functions)."""
+    for name, v in table.items():
+        set_knob(ctx, name, v["best"] if isinstance(v, dict) else v)
+
+
+def save(table: dict, path: str):
+    with open(path, "w") as f:
+        json.dump({k: (v["best"] if isinstance(v, dict) else v) for k, v in table.items()}, f, indent=1)
+
+
+def load(path: str) -> dict:
+    with open(path) as f:
+        return {k: int(v) for k, v in json.load(f).items() if k in _KNOB}

@@ -249,7 +249,7 @@ __global__ __launch_bounds__(kBlock) void k_baxpby(double a, const double* __res
 
 int bmm_splits(int64_t n, int64_t tiles)
 {
-    int64_t s = (2560 + tiles - 1) / std::max<int64_t>(tiles, 1);
+    int64_t s = (launch_tuning().bmm_blocks + tiles - 1) / std::max<int64_t>(tiles, 1);
     s = std::min<int64_t>(s, std::max<int64_t>(1, n / 256));
     return (int)std::max<int64_t>(1, std::min<int64_t>(s, 64));
 }

@@ -21,6 +21,8 @@
 // mixed precision: far pairs up to this distance ratio keep FP64 cubature; 0 = every far pair
 // in FP32 (the FP32 rule ladder alone keeps the global field error near 3e-8)
 constexpr double kMixedFp64RatioDefault = 0.0;
+// elements per XCD range of the large single-set far passes (kem::launch_field); 0 = no split
+constexpr int64_t kFarRangeDefault = 2560;
 
 struct kem_ctx {
     int device = 0;
@@ -28,6 +30,8 @@ struct kem_ctx {
     hipStream_t stream = nullptr;
     int policy = KEM_POLICY_NUMERIC;
     double mixed_fp64_ratio = kMixedFp64RatioDefault;
+    int64_t far_range_fp64 = kFarRangeDefault, far_range_mixed = kFarRangeDefault;
+    kem::LaunchTuning tuning;
     std::string err;
     double* ws = nullptr;  // field partial sums / gemv partials
     int64_t ws_doubles = 0;
@@ -130,7 +134,19 @@ hipError_t ensure_scal(kem_ctx* c, int64_t doubles)
 hipError_t enter(kem_ctx* c)
 {
     (void)hipGetLastError();
+    kem::set_launch_tuning(c->tuning);
     return hipSetDevice(c->device);
+}
+
+// the next timing event pair when kernel timing is on (else null events)
+void timing_pair(kem_ctx* c, hipEvent_t* t0, hipEvent_t* t1)
+{
+    *t0 = *t1 = nullptr;
+    if (c->timing && c->ev_used < kem_ctx::kTimedMax) {
+        *t0 = c->ev[2 * c->ev_used];
+        *t1 = c->ev[2 * c->ev_used + 1];
+        c->ev_used++;
+    }
 }
 
 // RAII device vector for solver temporaries
@@ -200,6 +216,37 @@ int kem_ctx_set_mixed_fp64_ratio(kem_ctx* c, double ratio)
     if (!c || !(ratio >= 0.)) return KEM_E_ARG;
     c->mixed_fp64_ratio = ratio;
     return KEM_OK;
+}
+
+int kem_ctx_set_tuning(kem_ctx* c, int param, int64_t value)
+{
+    if (!c) return KEM_E_ARG;
+    switch (param) {
+    case KEM_TUNE_FAR_RANGE_FP64:
+    case KEM_TUNE_FAR_RANGE_MIXED:
+        if (value < 0 || value > ((int64_t)1 << 31)) return fail(c, KEM_E_ARG, "tuning: range size out of [0, 2^31]");
+        (param == KEM_TUNE_FAR_RANGE_FP64 ? c->far_range_fp64 : c->far_range_mixed) = value;
+        return KEM_OK;
+    case KEM_TUNE_GEMV_BLOCKS:
+    case KEM_TUNE_BMM_BLOCKS:
+        if (value < 1 || value > (1 << 20)) return fail(c, KEM_E_ARG, "tuning: block count out of [1, 2^20]");
+        (param == KEM_TUNE_GEMV_BLOCKS ? c->tuning.gemv_blocks : c->tuning.bmm_blocks) = value;
+        return KEM_OK;
+    default:
+        return fail(c, KEM_E_ARG, "tuning: unknown parameter");
+    }
+}
+
+int kem_ctx_get_tuning(const kem_ctx* c, int param, int64_t* value)
+{
+    if (!c || !value) return KEM_E_ARG;
+    switch (param) {
+    case KEM_TUNE_FAR_RANGE_FP64: *value = c->far_range_fp64; return KEM_OK;
+    case KEM_TUNE_FAR_RANGE_MIXED: *value = c->far_range_mixed; return KEM_OK;
+    case KEM_TUNE_GEMV_BLOCKS: *value = c->tuning.gemv_blocks; return KEM_OK;
+    case KEM_TUNE_BMM_BLOCKS: *value = c->tuning.bmm_blocks; return KEM_OK;
+    default: return KEM_E_ARG;
+    }
 }
 
 int kem_ctx_set_kernel_timing(kem_ctx* c, int on)
@@ -457,12 +504,9 @@ int field_eval(kem_ctx* c, const kem_mesh* m, const double* sigma, const int32_t
     kem::FieldMesh fm;
     field_mesh(m, fm);
     const bool mixed = precision == KEM_MIXED;
-    hipEvent_t t0 = nullptr, t1 = nullptr;
-    if (c->timing && c->ev_used < kem_ctx::kTimedMax) {
-        t0 = c->ev[2 * c->ev_used];
-        t1 = c->ev[2 * c->ev_used + 1];
-        c->ev_used++;
-    }
+    const int64_t range = mixed ? c->far_range_mixed : c->far_range_fp64;
+    hipEvent_t t0, t1;
+    timing_pair(c, &t0, &t1);
     KEM_CK(c, ensure_ws(c, kem::field_workspace_doubles(c->policy, fm, nsets, npts)));
     if (npts >= kSortMinPoints && npts <= INT32_MAX) {
         // visit the points in Morton order so the lanes of a wavefront share quadrature rules
@@ -476,11 +520,11 @@ int field_eval(kem_ctx* c, const kem_mesh* m, const double* sigma, const int32_t
         }
         KEM_CK(c, kem::launch_field_sorted(c->stream, c->policy, fm, mixed, c->mixed_fp64_ratio, sigma, pset, nsets,
                                            x, y, z, npts, phi, ex, ey, ez, c->sort_ws, c->sort_ws_bytes, c->ws,
-                                           c->ws_doubles, t0, t1));
+                                           c->ws_doubles, range, t0, t1));
         return KEM_OK;
     }
     KEM_CK(c, kem::launch_field(c->stream, c->policy, fm, mixed, c->mixed_fp64_ratio, sigma, pset, nsets, x, y, z,
-                                npts, phi, ex, ey, ez, c->ws, c->ws_doubles, t0, t1));
+                                npts, phi, ex, ey, ez, c->ws, c->ws_doubles, range, t0, t1));
     return KEM_OK;
 }
 }  // namespace
@@ -746,7 +790,12 @@ int kem_matrix_multiply(kem_matrix* A, int transpose, const double* x, double* y
     KEM_CK(c, dalloc(&dx.p, n));
     KEM_CK(c, dalloc(&dy.p, n));
     KEM_CK(c, hipMemcpyAsync(dx.p, x, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+    KEM_CK(c, ensure_ws(c, (int64_t)kem::gemv_splits(n, n) * n));
+    hipEvent_t t0, t1;
+    timing_pair(c, &t0, &t1);
+    if (t0) KEM_CK(c, hipEventRecord(t0, c->stream));
     KEM_CK(c, matvec_dev(A, A->A64 == nullptr, transpose, dx.p, dy.p));
+    if (t1) KEM_CK(c, hipEventRecord(t1, c->stream));
     KEM_CK(c, hipMemcpyAsync(y, dy.p, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
     KEM_CK(c, hipStreamSynchronize(c->stream));
     return KEM_OK;
@@ -1381,11 +1430,17 @@ int kem_matrix_multiply_batch(kem_matrix* A, int precision, int64_t nrhs, const 
     DBuf ws;
     KEM_CK(c, dalloc(&dX.p, n * g));
     KEM_CK(c, dalloc(&dY.p, n * g));
+    ws.bytes = kem::bmm_workspace_bytes(n, g, use32);
+    KEM_CK(c, hipMalloc(&ws.p, ws.bytes));
     for (int64_t r0 = 0; r0 < nrhs; r0 += 64) {
         const int k = (int)std::min<int64_t>(64, nrhs - r0);
         KEM_CK(c, hipMemcpyAsync(dX.p, X + r0 * n, sizeof(double) * n * k, hipMemcpyHostToDevice, c->stream));
+        hipEvent_t t0, t1;
+        timing_pair(c, &t0, &t1);
+        if (t0) KEM_CK(c, hipEventRecord(t0, c->stream));
         int rc = block_product(A, use32, k, dX.p, dY.p, &ws.p, &ws.bytes);
         if (rc != KEM_OK) return rc;
+        if (t1) KEM_CK(c, hipEventRecord(t1, c->stream));
         KEM_CK(c, hipMemcpyAsync(Y + r0 * n, dY.p, sizeof(double) * n * k, hipMemcpyDeviceToHost, c->stream));
     }
     KEM_CK(c, hipStreamSynchronize(c->stream));

@@ -30,13 +30,16 @@ int64_t field_workspace_doubles(int policy, const FieldMesh& m, int64_t nsets, i
 // (FP32 node sums beyond distance ratio fp64_ratio, FP64 below it, FP64 near field and
 // accumulation); with fp64_ratio = 0 and far6 given it runs over element clusters.  clusters /
 // order select the culled near pass under the numeric policy; null scans every element.
+// range_elems > 0: from 262,144 points on, a single-set numeric-policy far pass splits the
+// elements into 8 R ranges of at most range_elems elements, R per XCD (0: no split).
 // ws: scratch of ws_doubles >= field_workspace_doubles(policy, m, nsets, npts) doubles
 // (element-range partial sums, per-set element weights).  far_begin / far_end (nullable): events recorded on
 // st around the far-field pass.
 hipError_t launch_field(hipStream_t st, int policy, const FieldMesh& m, bool mixed, double fp64_ratio,
                         const double* sigma, const int32_t* pset, int64_t nsets, const double* px, const double* py,
                         const double* pz, int64_t npts, double* phi, double* ex, double* ey, double* ez, double* ws,
-                        int64_t ws_doubles, hipEvent_t far_begin = nullptr, hipEvent_t far_end = nullptr);
+                        int64_t ws_doubles, int64_t range_elems, hipEvent_t far_begin = nullptr,
+                        hipEvent_t far_end = nullptr);
 
 // Same as launch_field with the points visited in Morton order (kem_sort.hip): sort keys,
 // gather (with the set indices), evaluate, scatter back.  sort_ws: sort_workspace_bytes(npts) bytes.
@@ -45,7 +48,7 @@ hipError_t launch_field_sorted(hipStream_t st, int policy, const FieldMesh& m, b
                                const double* sigma, const int32_t* pset, int64_t nsets, const double* px,
                                const double* py, const double* pz, int64_t npts, double* phi, double* ex, double* ey,
                                double* ez, void* sort_ws, size_t sort_ws_bytes, double* ws, int64_t ws_doubles,
-                               hipEvent_t far_begin = nullptr, hipEvent_t far_end = nullptr);
+                               int64_t range_elems, hipEvent_t far_begin = nullptr, hipEvent_t far_end = nullptr);
 
 // *bad = 1 when a set index lies outside [0, nsets) (*bad must be 0 on entry)
 hipError_t launch_check_sets(hipStream_t st, const int32_t* pset, int64_t n, int64_t nsets, int* bad);
@@ -59,6 +62,15 @@ hipError_t launch_pairs(hipStream_t st, int policy, const double* geom, const do
 
 hipError_t launch_assemble(hipStream_t st, int policy, bool field, const double* geom, const double* nodes,
                            const double* recs, const double* targets, int64_t n, double* A64, float* A32);
+
+// Launch shapes of the calling thread, set from the context at every C-ABI entry point:
+// target block counts of the split-K matrix-vector product and of the FP32 / FP64 block product.
+struct LaunchTuning {
+    int64_t gemv_blocks = 2048;
+    int64_t bmm_blocks = 2560;
+};
+void set_launch_tuning(const LaunchTuning& t);
+const LaunchTuning& launch_tuning();
 
 int gemv_splits(int64_t nrows, int64_t ncols);
 // y = alpha * A x + beta * y, A column-major nrows x ncols (leading dimension lda);

@@ -22,16 +22,21 @@ namespace {
 constexpr int kBlock = 256;
 
 // ------------------------------------------------------------------ field evaluation
-// Point block and element range of a far-pass block.  xcd = 0: blockIdx.x and blockIdx.y.
-// xcd = 1: a 1-D grid with 8 element ranges per point block, block L taking range L % 8 of
-// point block L / 8.  The hardware hands consecutive blocks to the 8 XCDs in turn, so all the
-// blocks one XCD runs read the same eighth of the elements, which then stays in that XCD's L2.
+// Point block and element range of a far-pass block.  xcd_pb = 0: blockIdx.x and blockIdx.y.
+// xcd_pb > 0 (the number of point blocks): a 1-D grid of 8 R element ranges per point block;
+// block L takes point block (L / 8) % xcd_pb and element range L % 8 + 8 ((L / 8) / xcd_pb).
+// The hardware hands consecutive blocks to the 8 XCDs in turn, so XCD x runs the ranges
+// x, x + 8, x + 16, ... one after the other, each over all point blocks, and the range it
+// works on stays in its L2.
 struct FarTile {
     int64_t pb, split;
 };
-__device__ __forceinline__ FarTile far_tile(int xcd)
+__device__ __forceinline__ FarTile far_tile(int64_t xcd_pb)
 {
-    if (xcd) return {(int64_t)(blockIdx.x >> 3), (int64_t)(blockIdx.x & 7)};
+    if (xcd_pb) {
+        const int64_t q = (int64_t)(blockIdx.x >> 3);
+        return {q % xcd_pb, (int64_t)(blockIdx.x & 7) + 8 * (q / xcd_pb)};
+    }
     return {(int64_t)blockIdx.x, (int64_t)blockIdx.y};
 }
 
@@ -49,9 +54,9 @@ __global__ __launch_bounds__(kBlock) void k_field_far(const double* __restrict__
                                                       const double* __restrict__ py, const double* __restrict__ pz,
                                                       int64_t npts, double* __restrict__ out0, double* __restrict__ out1,
                                                       double* __restrict__ out2, double* __restrict__ out3,
-                                                      int64_t out_stride, int xcd)
+                                                      int64_t out_stride, int64_t xcd_pb)
 {
-    const FarTile t = far_tile(xcd);
+    const FarTile t = far_tile(xcd_pb);
     int64_t i = t.pb * kBlock + threadIdx.x;
     const bool live = i < npts;
     if (!live) i = npts - 1;
@@ -97,9 +102,9 @@ __global__ __launch_bounds__(kBlock) void k_field_far32(const double* __restrict
                                                         const double* __restrict__ py, const double* __restrict__ pz,
                                                         int64_t npts, double* __restrict__ out0,
                                                         double* __restrict__ out1, double* __restrict__ out2,
-                                                        double* __restrict__ out3, int64_t out_stride, int xcd)
+                                                        double* __restrict__ out3, int64_t out_stride, int64_t xcd_pb)
 {
-    const FarTile t = far_tile(xcd);
+    const FarTile t = far_tile(xcd_pb);
     int64_t i = t.pb * kBlock + threadIdx.x;
     const bool live = i < npts;
     if (!live) i = npts - 1;
@@ -336,9 +341,9 @@ __global__ __launch_bounds__(kBlock) void k_field_far32c(const double* __restric
                                                          const double* __restrict__ pz, int64_t npts,
                                                          double* __restrict__ out0, double* __restrict__ out1,
                                                          double* __restrict__ out2, double* __restrict__ out3,
-                                                         int64_t out_stride, int xcd)
+                                                         int64_t out_stride, int64_t xcd_pb)
 {
-    const FarTile t = far_tile(xcd);
+    const FarTile t = far_tile(xcd_pb);
     int64_t i = t.pb * kBlock + threadIdx.x;
     const bool live = i < npts;
     if (!live) i = npts - 1;
@@ -763,22 +768,23 @@ inline unsigned nblocks(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock
 namespace {
 constexpr int64_t kPartialDoubles = 4 * (int64_t)(2048 + 256) * 256;
 // From this many point blocks on, the numeric-policy far pass of one charge-density set splits
-// the elements into one range per XCD (far_tile); it runs over chunks of kXcdChunk points with 8
-// partial sums each.  (Measured on 1 M points x 20,000 triangles: FP64 399 -> 361 ms, mixed
-// 68 -> 60 ms.  Passes over several sets keep the single element range: with the per-lane set
-// weights the split measured 23 % slower.)
+// the elements into ranges of at most range_elems elements, a multiple of 8 of them (far_tile),
+// over chunks of points whose 8 R partial sums per point fit kXcdPartialDoubles.  (Measured on
+// 1 M points x 20,000 triangles, 8 ranges: FP64 399 -> 361 ms, mixed 68 -> 60 ms.  Passes over
+// several sets keep one element range per block: with the per-lane set weights 8 ranges
+// measured 23 % slower.)
 constexpr int64_t kXcdMinBlocks = 1024;
-constexpr int64_t kXcdChunk = (int64_t)1 << 20;
+constexpr int64_t kXcdPartialDoubles = (int64_t)32 << 20;
 constexpr int kXcds = 8;
 
-bool xcd_split(int policy, const FieldMesh& m, int64_t npts)
+bool xcd_split(int policy, const FieldMesh& m, int64_t npts, int64_t range_elems)
 {
-    return policy == 0 && m.clusters && m.order && (int64_t)nblocks(npts) >= kXcdMinBlocks;
+    return policy == 0 && m.clusters && m.order && range_elems > 0 && (int64_t)nblocks(npts) >= kXcdMinBlocks;
 }
 
 int64_t partial_doubles(int policy, const FieldMesh& m, int64_t npts)
 {
-    if (xcd_split(policy, m, npts)) return std::max(kPartialDoubles, 4 * kXcds * std::min(npts, kXcdChunk));
+    if (xcd_split(policy, m, npts, 1)) return std::max(kPartialDoubles, kXcdPartialDoubles);
     return kPartialDoubles;
 }
 
@@ -786,20 +792,20 @@ template <bool MULTI>
 void far_kernel(hipStream_t st, dim3 grid, int kind, const FieldMesh& m, double fp64_ratio, const double* wk,
                 const double* sg, const int32_t* pset, int64_t nsets, int64_t per, const double* px,
                 const double* py, const double* pz, int64_t npts, double* o0, double* o1, double* o2, double* o3,
-                int64_t stride, int xcd)
+                int64_t stride, int64_t xcd_pb)
 {
     const int64_t nelem = m.n;
     if (kind == 2) {
         const int64_t ncl = (nelem + KEM_CLUSTER - 1) / KEM_CLUSTER;
         k_field_far32c<MULTI><<<grid, kBlock, 0, st>>>(m.geom, m.nodes32, m.far6, m.far6_cl, m.order, wk, pset, nsets,
                                                        nelem, ncl, per / KEM_CLUSTER, px, py, pz, npts, o0, o1, o2,
-                                                       o3, stride, xcd);
+                                                       o3, stride, xcd_pb);
     } else if (kind == 1) {
         k_field_far32<MULTI><<<grid, kBlock, 0, st>>>(m.geom, m.nodes, m.nodes32, fp64_ratio, sg, pset, nsets, nelem,
-                                                      per, px, py, pz, npts, o0, o1, o2, o3, stride, xcd);
+                                                      per, px, py, pz, npts, o0, o1, o2, o3, stride, xcd_pb);
     } else {
         k_field_far<MULTI><<<grid, kBlock, 0, st>>>(m.geom, m.nodes, sg, pset, nsets, nelem, per, px, py, pz, npts, o0,
-                                                    o1, o2, o3, stride, xcd);
+                                                    o1, o2, o3, stride, xcd_pb);
     }
 }
 
@@ -807,7 +813,7 @@ template <bool MULTI>
 void field_passes(hipStream_t st, int policy, const FieldMesh& m, bool mixed, double fp64_ratio, const double* sigma,
                   const int32_t* pset, int64_t nsets, const double* px, const double* py, const double* pz,
                   int64_t npts, double* phi, double* ex, double* ey, double* ez, double* ws, int64_t ws_doubles,
-                  hipEvent_t far_begin, hipEvent_t far_end)
+                  int64_t range_elems, hipEvent_t far_begin, hipEvent_t far_end)
 {
     const int64_t nelem = m.n;
     const unsigned gx = nblocks(npts);
@@ -829,21 +835,27 @@ void field_passes(hipStream_t st, int policy, const FieldMesh& m, bool mixed, do
         ws_doubles -= nsig;
     }
     const bool culled = policy == 0 && m.clusters && m.order;
-    if (!MULTI && xcd_split(policy, m, npts)) {
-        // one element range per XCD, 8 partial sums per point, in chunks of points
-        int64_t per = (nelem + kXcds - 1) / kXcds;
+    if (!MULTI && xcd_split(policy, m, npts, range_elems)) {
+        // 8 R element ranges of at most range_elems elements, R per XCD; partial sums per
+        // range over chunks of points
+        const int64_t r = std::min<int64_t>(kXcdPartialDoubles / (4 * kXcds * kBlock),
+                                            std::max<int64_t>(1, ((nelem + kXcds - 1) / kXcds + range_elems - 1) /
+                                                                     range_elems));
+        const int64_t nr = kXcds * r;
+        int64_t per = (nelem + nr - 1) / nr;
         if (clustered) per = (per + KEM_CLUSTER - 1) / KEM_CLUSTER * KEM_CLUSTER;
+        const int64_t chunk = std::max<int64_t>(kBlock, kXcdPartialDoubles / (4 * nr) / kBlock * kBlock);
         if (far_begin) (void)hipEventRecord(far_begin, st);
         if (clustered)
             k_far_weights<<<nblocks(nsets * nelem), kBlock, 0, st>>>(m.geom, m.order, sigma, nelem, nsets, wk);
-        for (int64_t p0 = 0; p0 < npts; p0 += kXcdChunk) {
-            const int64_t np = std::min(kXcdChunk, npts - p0);
+        for (int64_t p0 = 0; p0 < npts; p0 += chunk) {
+            const int64_t np = std::min(chunk, npts - p0);
             const unsigned gc = nblocks(np);
             double* q = ws;
-            far_kernel<MULTI>(st, dim3(gc * kXcds), kind, m, fp64_ratio, wk, sg, MULTI ? pset + p0 : nullptr, nsets,
-                              per, px + p0, py + p0, pz + p0, np, q, q + kXcds * np, q + 2 * kXcds * np,
-                              q + 3 * kXcds * np, np, 1);
-            k_field_combine<<<gc, kBlock, 0, st>>>(ws, kXcds, np, phi + p0, ex + p0, ey + p0, ez + p0);
+            far_kernel<MULTI>(st, dim3((unsigned)(gc * nr)), kind, m, fp64_ratio, wk, sg, MULTI ? pset + p0 : nullptr,
+                              nsets, per, px + p0, py + p0, pz + p0, np, q, q + nr * np, q + 2 * nr * np,
+                              q + 3 * nr * np, np, (int64_t)gc);
+            k_field_combine<<<gc, kBlock, 0, st>>>(ws, (int)nr, np, phi + p0, ex + p0, ey + p0, ez + p0);
         }
         if (far_end) (void)hipEventRecord(far_end, st);
     } else {
@@ -898,16 +910,16 @@ int64_t field_workspace_doubles(int policy, const FieldMesh& m, int64_t nsets, i
 hipError_t launch_field(hipStream_t st, int policy, const FieldMesh& m, bool mixed, double fp64_ratio,
                         const double* sigma, const int32_t* pset, int64_t nsets, const double* px, const double* py,
                         const double* pz, int64_t npts, double* phi, double* ex, double* ey, double* ez, double* ws,
-                        int64_t ws_doubles, hipEvent_t far_begin, hipEvent_t far_end)
+                        int64_t ws_doubles, int64_t range_elems, hipEvent_t far_begin, hipEvent_t far_end)
 {
     if (npts <= 0) return hipSuccess;
     if (nsets < 1 || ws_doubles < field_workspace_doubles(policy, m, nsets, npts)) return hipErrorInvalidValue;
     if (pset)
         field_passes<true>(st, policy, m, mixed, fp64_ratio, sigma, pset, nsets, px, py, pz, npts, phi, ex, ey, ez, ws,
-                           ws_doubles, far_begin, far_end);
+                           ws_doubles, range_elems, far_begin, far_end);
     else
         field_passes<false>(st, policy, m, mixed, fp64_ratio, sigma, nullptr, 1, px, py, pz, npts, phi, ex, ey, ez,
-                            ws, ws_doubles, far_begin, far_end);
+                            ws, ws_doubles, range_elems, far_begin, far_end);
     return hipGetLastError();
 }
 
@@ -962,10 +974,17 @@ hipError_t launch_assemble(hipStream_t st, int policy, bool field, const double*
     return hipGetLastError();
 }
 
+namespace {
+thread_local LaunchTuning g_tuning;
+}
+
+void set_launch_tuning(const LaunchTuning& t) { g_tuning = t; }
+const LaunchTuning& launch_tuning() { return g_tuning; }
+
 int gemv_splits(int64_t nrows, int64_t ncols)
 {
     const int64_t gx = (nrows + kBlock - 1) / kBlock;
-    int64_t s = (2048 + gx - 1) / gx;
+    int64_t s = (g_tuning.gemv_blocks + gx - 1) / gx;
     if (s > (ncols + 15) / 16) s = (ncols + 15) / 16;
     if (s < 1) s = 1;
     if (s > 512) s = 512;

@@ -1,0 +1,87 @@
+"""The launch-shape autotuner (kassiopeia_amd/autotune.py) on the device: every knob's sweep
+returns a time per candidate and leaves the fastest set, results do not depend on the knobs
+beyond summation order, and tables round-trip through JSON."""
+import numpy as np
+import pytest
+
+from kassiopeia_amd import geometry as G
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def setup():
+    from kassiopeia_amd import kemfield as K
+
+    ctx = K.Context(0)
+    m = G.coax_mesh(nphi_in=12, nphi_out=24, nz=16, ncap_r=2, ncap_phi=16)
+    cont = K.KSurfaceContainer(ctx, m)
+    A = K.KBoundaryIntegralMatrix(cont, fp64=True, fp32=True)
+    x, st, rc = A.solve(cont.rhs(), tolerance=1e-10)
+    yield ctx, m, cont, A, x
+    A.close()
+    cont.close()
+    ctx.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_tune_field(setup, precision):
+    from kassiopeia_amd import autotune as T
+    from kassiopeia_amd import kemfield as K
+
+    ctx, m, cont, A, x = setup
+    pts = G.points_annulus(300_000, 0.5, 1.0, 2.0, 3)
+    res = T.tune_field(cont, x, pts, precision, repeats=1, candidates=(0, 64, 2560))
+    assert set(res["ms"]) == {0, 64, 2560} and all(t > 0 for t in res["ms"].values())
+    assert res["best"] == min(res["ms"], key=res["ms"].get) == T.get(ctx, "far_range_" + precision)
+    # the field does not depend on the split beyond summation order (and, mixed, the last FP32
+    # bit of some pairs)
+    out = {}
+    for v in (0, 64, 2560):
+        T.set_knob(ctx, "far_range_" + precision, v)
+        out[v] = K.KIntegratingElectrostaticFieldSolver(cont, x, precision).ElectricFieldAndPotential(pts)
+    tol = 1e-14 if precision == "fp64" else 1e-9
+    for v in (64, 2560):
+        assert np.linalg.norm(out[v][0] - out[0][0]) / np.linalg.norm(out[0][0]) < tol
+        assert np.abs(out[v][1] - out[0][1]).max() / 18600.0 < 10 * tol
+    T.set_knob(ctx, "far_range_" + precision, 2560)
+
+
+def test_tune_products(setup):
+    from kassiopeia_amd import autotune as T
+
+    ctx, m, cont, A, x = setup
+    v = np.random.default_rng(2).normal(size=A.n)
+    ref = A.Multiply(v)
+    r = T.tune_matvec(A, repeats=2, candidates=(256, 2048))
+    assert set(r["ms"]) == {256, 2048} and T.get(ctx, "gemv_blocks") == r["best"]
+    for c in (1, 256, 4096):
+        T.set_knob(ctx, "gemv_blocks", c)
+        assert np.linalg.norm(A.Multiply(v) - ref) <= 1e-14 * np.linalg.norm(ref)
+    X = np.random.default_rng(3).normal(size=(A.n, 40))
+    ref = A.MultiplyBatch(X, "mixed")
+    r = T.tune_block_product(A, nrhs=40, repeats=1, candidates=(320, 2560))
+    assert T.get(ctx, "bmm_blocks") == r["best"]
+    for c in (64, 10240):
+        T.set_knob(ctx, "bmm_blocks", c)
+        assert np.linalg.norm(A.MultiplyBatch(X, "mixed") - ref) <= 1e-6 * np.linalg.norm(ref)
+    T.apply(ctx, {"gemv_blocks": 2048, "bmm_blocks": 2560})
+
+
+def test_tuning_table_roundtrip_and_errors(setup, tmp_path):
+    from kassiopeia_amd import _lib
+    from kassiopeia_amd import autotune as T
+
+    ctx = setup[0]
+    table = {"far_range_fp64": 1280, "far_range_mixed": {"best": 5120, "ms": {}}, "gemv_blocks": 1024,
+             "bmm_blocks": 2560}
+    T.save(table, tmp_path / "t.json")
+    loaded = T.load(tmp_path / "t.json")
+    assert loaded == {"far_range_fp64": 1280, "far_range_mixed": 5120, "gemv_blocks": 1024, "bmm_blocks": 2560}
+    T.apply(ctx, loaded)
+    assert T.current(ctx) == loaded
+    with pytest.raises(_lib.KemError):
+        T.set_knob(ctx, "gemv_blocks", 0)
+    with pytest.raises(_lib.KemError):
+        ctx.check(ctx.lib.kem_ctx_set_tuning(ctx.h, 99, 1))
+    T.apply(ctx, {"far_range_fp64": 2560, "far_range_mixed": 2560, "gemv_blocks": 2048})
