@@ -774,7 +774,7 @@ constexpr int64_t kPartialDoubles = 4 * (int64_t)(2048 + 256) * 256;
 // several sets keep one element range per block: with the per-lane set weights 8 ranges
 // measured 23 % slower.)
 constexpr int64_t kXcdMinBlocks = 1024;
-constexpr int64_t kXcdPartialDoubles = (int64_t)32 << 20;
+constexpr int64_t kXcdPartialDoubles = (int64_t)128 << 20;
 constexpr int kXcds = 8;
 
 bool xcd_split(int policy, const FieldMesh& m, int64_t npts, int64_t range_elems)
