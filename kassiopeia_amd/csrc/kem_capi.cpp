@@ -763,19 +763,21 @@ int kem_matrix_copy(kem_matrix* A, double* host)
 
 namespace {
 
-// y = A x (or A^T x) on device with the matrix's FP64 storage (or FP32 when use32)
-hipError_t matvec_dev(kem_matrix* A, bool use32, int transpose, const double* x, double* y)
+// y = A x (or A^T x) on device with the matrix's FP64 storage (or FP32 when use32); skip:
+// optional device flag that turns the kernels into no-ops (a finished solve in a graph replay)
+hipError_t matvec_dev(kem_matrix* A, bool use32, int transpose, const double* x, double* y,
+                      const int* skip = nullptr)
 {
     kem_ctx* c = A->ctx;
     const int64_t n = A->n;
     if (transpose) {
-        if (use32) return kem::launch_gemv_t<float>(c->stream, A->A32, n, n, n, x, y);
-        return kem::launch_gemv_t<double>(c->stream, A->A64, n, n, n, x, y);
+        if (use32) return kem::launch_gemv_t<float>(c->stream, A->A32, n, n, n, x, y, skip);
+        return kem::launch_gemv_t<double>(c->stream, A->A64, n, n, n, x, y, skip);
     }
     hipError_t e = ensure_ws(c, (int64_t)kem::gemv_splits(n, n) * n);
     if (e != hipSuccess) return e;
-    if (use32) return kem::launch_gemv<float>(c->stream, A->A32, n, n, n, x, 1.0, 0.0, y, c->ws);
-    return kem::launch_gemv<double>(c->stream, A->A64, n, n, n, x, 1.0, 0.0, y, c->ws);
+    if (use32) return kem::launch_gemv<float>(c->stream, A->A32, n, n, n, x, 1.0, 0.0, y, c->ws, skip);
+    return kem::launch_gemv<double>(c->stream, A->A64, n, n, n, x, 1.0, 0.0, y, c->ws, skip);
 }
 
 }  // namespace
@@ -825,11 +827,17 @@ struct SolveCtl {
 // GMRES with Givens rotations and a triangular least-squares solve
 // (KGeneralizedMinimalResidual.hh:178-337), convergence |p_{j+1}| / ||b|| <= tol
 // (KSimpleIterativeKrylovSolver.hh:84-103).  The Arnoldi step orthogonalises with classical
-// Gram-Schmidt plus one reorthogonalisation pass (CGS2): two GEMV-T / GEMV pairs over the
+// Gram-Schmidt plus one reorthogonalisation pass (CGS2): two projection / update pairs over the
 // device-resident basis per iteration instead of 2(j+1) dependent dot/axpy launches, and
 // orthogonality to working precision (the reference's MGS loses it on BEM matrices, so its
-// iteration counts near the tolerance are rounding-dependent).  Device vectors d_b, d_x
-// (x holds x0).  On restart (or at the basis cap) x <- x0 + V y.
+// iteration counts near the tolerance are rounding-dependent).  One iteration -- the
+// matrix-vector product, CGS2, normalisation and the Givens rotation -- runs on device state
+// (kem::launch_arnoldi_step) and is captured once per solve as a HIP graph; the host replays it
+// kGmresReplays times between reads of the column index and stop flag, and a finished cycle
+// turns the remaining replays into no-ops.  Device vectors d_b, d_x (x holds x0).  On restart
+// (or at the basis cap) x <- x0 + V y.
+constexpr int kGmresReplays = 4;
+
 int gmres_dev(kem_matrix* A, bool use32, const double* d_b, double* d_x, double tol, int maxit, int restart,
               const SolveCtl& ctl, int* iters, double* resid)
 {
@@ -839,21 +847,75 @@ int gmres_dev(kem_matrix* A, bool use32, const double* d_b, double* d_x, double 
     int kmax = (int)std::min<int64_t>(std::min<int64_t>(maxit, 1000), n + 1);
     if (restart > 0) kmax = std::min(kmax, restart);
     if (kmax < 1) kmax = 1;
-    DVec V, w;
-    KEM_CK(c, dalloc(&V.p, (int64_t)(kmax + 1) * n));
-    KEM_CK(c, dalloc(&w.p, n));
-    KEM_CK(c, ensure_scal(c, 256 + 2 * ((int64_t)kmax + 8)));
-    KEM_CK(c, ensure_ws(c, (int64_t)std::max(kem::gemv_splits(n, kmax + 1), kem::gemv_splits(n, n)) * n));
-    double* d_part = c->scal;                 // 256 dot partials
-    double* d_h1 = c->scal + 256;             // first-pass projections + ||w|| (kmax + 2)
-    double* d_h2 = d_h1 + (kmax + 8);         // second-pass projections
     const int ld = kmax + 1;
-    std::vector<double> H((size_t)ld * ld), cs(kmax + 1), sn(kmax + 1), p(kmax + 2), y(kmax + 1), h2(kmax + 1);
+    DVec V, w, vcur, H, rot, hh, part, scal;
+    KEM_CK(c, dalloc(&V.p, (int64_t)ld * n));
+    KEM_CK(c, dalloc(&w.p, n));
+    KEM_CK(c, dalloc(&vcur.p, n));
+    KEM_CK(c, dalloc(&H.p, (int64_t)ld * ld));
+    KEM_CK(c, dalloc(&rot.p, 3 * (int64_t)(kmax + 2)));
+    KEM_CK(c, dalloc(&hh.p, 2 * (int64_t)(kmax + 8) + 8));
+    KEM_CK(c, dalloc(&part.p, 256 + (int64_t)kem::gemv_splits(n, ld) * n));
+    KEM_CK(c, dalloc(&scal.p, 8));
+    int* d_ints = nullptr;
+    KEM_CK(c, hipMalloc((void**)&d_ints, 4 * sizeof(int)));
+    struct IntBuf {
+        int* p;
+        ~IntBuf() { (void)hipFree(p); }
+    } ints_guard{d_ints};
+    KEM_CK(c, ensure_ws(c, (int64_t)kem::gemv_splits(n, n) * n));
+    kem::ArnoldiState a;
+    a.n = n;
+    a.kmax = kmax;
+    a.V = V.p;
+    a.w = w.p;
+    a.vcur = vcur.p;
+    a.H = H.p;
+    a.cs = rot.p;
+    a.sn = rot.p + (kmax + 2);
+    a.p = rot.p + 2 * (kmax + 2);
+    a.h1 = hh.p;
+    a.h2 = hh.p + (kmax + 8);
+    a.nrm = hh.p + 2 * (kmax + 8);
+    a.part = part.p;
+    a.ints = d_ints;
+    a.scal = scal.p;
     double bnorm;
-    KEM_CK(c, kem::launch_dot(st, d_b, d_b, n, d_part, d_h1, true));
-    KEM_CK(c, hipMemcpyAsync(&bnorm, d_h1, sizeof(double), hipMemcpyDeviceToHost, st));
+    KEM_CK(c, kem::launch_dot(st, d_b, d_b, n, part.p, scal.p + 2, true));
+    KEM_CK(c, hipMemcpyAsync(&bnorm, scal.p + 2, sizeof(double), hipMemcpyDeviceToHost, st));
     KEM_CK(c, hipStreamSynchronize(st));
     if (bnorm == 0.) bnorm = 1.;
+
+    // one iteration as a graph (launched directly if the stream cannot be captured)
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    struct GraphGuard {
+        hipGraph_t* g;
+        hipGraphExec_t* e;
+        ~GraphGuard()
+        {
+            if (*e) (void)hipGraphExecDestroy(*e);
+            if (*g) (void)hipGraphDestroy(*g);
+        }
+    } graph_guard{&graph, &exec};
+    auto launch_step = [&]() -> hipError_t {
+        hipError_t e = matvec_dev(A, use32, A->trans, vcur.p, w.p, d_ints + 1);
+        if (e == hipSuccess) e = kem::launch_arnoldi_step(st, a);
+        return e;
+    };
+    if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) == hipSuccess) {
+        const hipError_t e1 = launch_step();
+        const hipError_t e2 = hipStreamEndCapture(st, &graph);
+        if (e1 != hipSuccess || e2 != hipSuccess || hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess) {
+            if (exec) (void)hipGraphExecDestroy(exec);
+            if (graph) (void)hipGraphDestroy(graph);
+            exec = nullptr;
+            graph = nullptr;
+        }
+    }
+    (void)hipGetLastError();
+
+    std::vector<double> Hh((size_t)ld * ld), p(kmax + 2), y(kmax + 1);
     int it = 0;
     double rn = 0.;
     int status = KEM_OK;
@@ -861,60 +923,53 @@ int gmres_dev(kem_matrix* A, bool use32, const double* d_b, double* d_x, double 
         // r = b - A x ; v0 = r / ||r||
         KEM_CK(c, matvec_dev(A, use32, A->trans, d_x, w.p));
         KEM_CK(c, kem::launch_axpby(st, 1.0, d_b, -1.0, w.p, w.p, n));
-        KEM_CK(c, kem::launch_dot(st, w.p, w.p, n, d_part, d_h1, true));
-        KEM_CK(c, kem::launch_div_dev(st, d_h1, w.p, V.p, n));
-        KEM_CK(c, hipMemcpyAsync(&p[0], d_h1, sizeof(double), hipMemcpyDeviceToHost, st));
+        KEM_CK(c, kem::launch_dot(st, w.p, w.p, n, part.p, scal.p + 2, true));
+        KEM_CK(c, kem::launch_div_dev(st, scal.p + 2, w.p, V.p, n));
+        KEM_CK(c, hipMemcpyAsync(vcur.p, V.p, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+        KEM_CK(c, hipMemcpyAsync(&rn, scal.p + 2, sizeof(double), hipMemcpyDeviceToHost, st));
         KEM_CK(c, hipStreamSynchronize(st));
-        rn = p[0];
         if (!(rn > 0.)) break;  // x is already exact
-        int j = 0;
-        bool conv = false, stop = false;
+        const int jlimit = std::min(kmax, maxit - it);
+        int hints[4] = {0, 0, jlimit, 0};
+        const double hs[2] = {tol * bnorm, rn};
+        KEM_CK(c, hipMemcpyAsync(d_ints, hints, sizeof(hints), hipMemcpyHostToDevice, st));
+        KEM_CK(c, hipMemcpyAsync(scal.p, hs, sizeof(hs), hipMemcpyHostToDevice, st));
+        KEM_CK(c, hipMemcpyAsync(a.p, &rn, sizeof(double), hipMemcpyHostToDevice, st));
+        bool expired = false;
         for (;;) {
-            double* vj = V.p + (int64_t)j * n;
-            double* vj1 = V.p + (int64_t)(j + 1) * n;
-            KEM_CK(c, matvec_dev(A, use32, A->trans, vj, w.p));
-            // CGS2: h1 = V^T w, w -= V h1; h2 = V^T w, w -= V h2; h = h1 + h2
-            KEM_CK(c, kem::launch_gemv_t<double>(st, V.p, n, n, j + 1, w.p, d_h1));
-            KEM_CK(c, kem::launch_gemv<double>(st, V.p, n, n, j + 1, d_h1, -1.0, 1.0, w.p, c->ws));
-            KEM_CK(c, kem::launch_gemv_t<double>(st, V.p, n, n, j + 1, w.p, d_h2));
-            KEM_CK(c, kem::launch_gemv<double>(st, V.p, n, n, j + 1, d_h2, -1.0, 1.0, w.p, c->ws));
-            KEM_CK(c, kem::launch_dot(st, w.p, w.p, n, d_part, d_h1 + j + 1, true));
-            KEM_CK(c, kem::launch_div_dev(st, d_h1 + j + 1, w.p, vj1, n));
-            double* h = H.data() + (size_t)j * ld;
-            KEM_CK(c, hipMemcpyAsync(h, d_h1, sizeof(double) * (j + 2), hipMemcpyDeviceToHost, st));
-            KEM_CK(c, hipMemcpyAsync(h2.data(), d_h2, sizeof(double) * (j + 1), hipMemcpyDeviceToHost, st));
+            for (int k = 0; k < kGmresReplays; k++) KEM_CK(c, exec ? hipGraphLaunch(exec, st) : launch_step());
+            double last[2];
+            KEM_CK(c, hipMemcpyAsync(hints, d_ints, sizeof(hints), hipMemcpyDeviceToHost, st));
+            KEM_CK(c, hipMemcpyAsync(last, scal.p, sizeof(last), hipMemcpyDeviceToHost, st));
             KEM_CK(c, hipStreamSynchronize(st));
-            for (int r = 0; r <= j; r++) h[r] += h2[r];
-            const double beta = h[j + 1];
-            for (int r = 0; r < j; r++) {
-                const double a = h[r], bb = h[r + 1];
-                h[r] = cs[r] * a + sn[r] * bb;
-                h[r + 1] = -1.0 * sn[r] * a + cs[r] * bb;
+            rn = last[1];
+            if (hints[1]) break;
+            if (ctl.expired()) {
+                expired = true;
+                break;
             }
-            const double gam = std::sqrt(h[j] * h[j] + beta * beta);
-            cs[j] = h[j] / gam;
-            sn[j] = beta / gam;
-            h[j] = gam;
-            p[j + 1] = -1.0 * sn[j] * p[j];
-            p[j] = cs[j] * p[j];
-            rn = std::fabs(p[j + 1]);
-            j++;
-            it++;
-            conv = (rn / bnorm) <= tol;
-            stop = conv || it >= maxit || ctl.expired() || !(beta > 0.);
-            if (stop || j >= kmax) break;
         }
+        const int j = hints[0];
+        it += j;
+        const bool conv = hints[3] != 0;
         // x += V y with H y = p (upper triangular)
-        for (int r = j - 1; r >= 0; r--) {
-            double s = p[r];
-            for (int q = r + 1; q < j; q++) s -= H[(size_t)q * ld + r] * y[q];
-            y[r] = s / H[(size_t)r * ld + r];
+        if (j > 0) {
+            KEM_CK(c, hipMemcpyAsync(Hh.data(), H.p, sizeof(double) * (size_t)ld * j, hipMemcpyDeviceToHost, st));
+            KEM_CK(c, hipMemcpyAsync(p.data(), a.p, sizeof(double) * j, hipMemcpyDeviceToHost, st));
+            KEM_CK(c, hipStreamSynchronize(st));
+            for (int r = j - 1; r >= 0; r--) {
+                double s = p[r];
+                for (int q = r + 1; q < j; q++) s -= Hh[(size_t)q * ld + r] * y[q];
+                y[r] = s / Hh[(size_t)r * ld + r];
+            }
+            KEM_CK(c, hipMemcpyAsync(hh.p, y.data(), sizeof(double) * j, hipMemcpyHostToDevice, st));
+            KEM_CK(c, kem::launch_gemv<double>(st, V.p, n, n, j, hh.p, 1.0, 1.0, d_x, c->ws));
+            KEM_CK(c, hipStreamSynchronize(st));
         }
-        KEM_CK(c, hipMemcpyAsync(d_h1, y.data(), sizeof(double) * j, hipMemcpyHostToDevice, st));
-        KEM_CK(c, kem::launch_gemv<double>(st, V.p, n, n, j, d_h1, 1.0, 1.0, d_x, c->ws));
-        KEM_CK(c, hipStreamSynchronize(st));
         if (conv) break;
-        if (stop) {
+        // stopped without convergence: iteration limit, time limit or breakdown; otherwise the
+        // basis is full and the cycle restarts from the new x
+        if (expired || it >= maxit || j < jlimit) {
             status = KEM_W_NOT_CONVERGED;
             break;
         }

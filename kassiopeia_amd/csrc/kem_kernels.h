@@ -74,14 +74,43 @@ const LaunchTuning& launch_tuning();
 
 int gemv_splits(int64_t nrows, int64_t ncols);
 // y = alpha * A x + beta * y, A column-major nrows x ncols (leading dimension lda);
-// part: scratch of gemv_splits(nrows, ncols) * nrows doubles
+// part: scratch of gemv_splits(nrows, ncols) * nrows doubles.  skip (nullable): device flag;
+// the kernels do nothing while it is nonzero (a finished solve in a replayed graph).
 template <typename T>
 hipError_t launch_gemv(hipStream_t st, const T* A, int64_t lda, int64_t nrows, int64_t ncols, const double* x,
-                       double alpha, double beta, double* y, double* part);
+                       double alpha, double beta, double* y, double* part, const int* skip = nullptr);
 // y = A^T x
 template <typename T>
 hipError_t launch_gemv_t(hipStream_t st, const T* A, int64_t lda, int64_t nrows, int64_t ncols, const double* x,
-                         double* y);
+                         double* y, const int* skip = nullptr);
+
+// ---- GMRES Arnoldi step on device state (one step per graph replay)
+// State of one GMRES cycle: basis V (kmax+1 columns of n), Hessenberg H (column-major, leading
+// dimension ld = kmax+1), rotations cs / sn, rotated right-hand side p, projections h1 / h2,
+// norm scalar, the column index j and the stop flag done (ints), the cycle's column limit
+// jlimit, and the tolerance tol * ||b|| against which |p[j+1]| is tested.
+struct ArnoldiState {
+    int64_t n = 0;
+    int kmax = 0;
+    double* V = nullptr;
+    double* w = nullptr;
+    double* vcur = nullptr;  // the current basis vector (input of the matrix-vector product)
+    double* H = nullptr;
+    double* cs = nullptr;
+    double* sn = nullptr;
+    double* p = nullptr;
+    double* h1 = nullptr;
+    double* h2 = nullptr;
+    double* nrm = nullptr;
+    double* part = nullptr;  // 256 dot partials + gemv partials (gemv_splits(n, kmax + 1) * n)
+    int* ints = nullptr;     // [0] j, [1] done, [2] jlimit, [3] converged
+    double* scal = nullptr;  // [0] tol * ||b||, [1] |p[j+1]| after the last step
+};
+// After the matrix-vector product w = A vcur: CGS2 against columns 0..j of V, normalisation into
+// V[j+1] and vcur, Givens rotation of column j (KGeneralizedMinimalResidual.hh:178-271), j += 1;
+// done = 1 when |p[j+1]| <= tol * ||b||, j reaches jlimit or the new norm is 0.  Every kernel
+// returns at once while done is set.
+hipError_t launch_arnoldi_step(hipStream_t st, const ArnoldiState& a);
 
 // *out = a.b (or its square root); part: scratch of 256 doubles
 hipError_t launch_dot(hipStream_t st, const double* a, const double* b, int64_t n, double* part, double* out,

@@ -20,6 +20,7 @@ using dev::F4;
 namespace {
 
 constexpr int kBlock = 256;
+constexpr int kDotParts = 256;  // block partials of a dot product
 
 // ------------------------------------------------------------------ field evaluation
 // Point block and element range of a far-pass block.  xcd_pb = 0: blockIdx.x and blockIdx.y.
@@ -503,8 +504,9 @@ __global__ __launch_bounds__(kBlock) void k_assemble(const double* __restrict__ 
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_gemv_part(const T* __restrict__ A, int64_t lda, int64_t nrows,
                                                       int64_t ncols, int64_t per_split, const double* __restrict__ x,
-                                                      double* __restrict__ part)
+                                                      double* __restrict__ part, const int* __restrict__ skip)
 {
+    if (skip && *skip) return;
     const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     const int64_t j0 = (int64_t)blockIdx.y * per_split;
     int64_t j1 = j0 + per_split;
@@ -524,8 +526,10 @@ __global__ __launch_bounds__(kBlock) void k_gemv_part(const T* __restrict__ A, i
 
 // y[i] = alpha * sum_s part[s][i] + beta * y[i]
 __global__ __launch_bounds__(kBlock) void k_gemv_reduce(const double* __restrict__ part, int nsplit, int64_t nrows,
-                                                        double alpha, double beta, double* __restrict__ y)
+                                                        double alpha, double beta, double* __restrict__ y,
+                                                        const int* __restrict__ skip)
 {
+    if (skip && *skip) return;
     const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (i >= nrows) return;
     double s = 0.;
@@ -549,8 +553,10 @@ __device__ __forceinline__ double block_sum(double v, double* sh)
 // y[j] = sum_i A[j*lda + i] * x[i]  (one block per column; y = A^T x)
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_gemv_t(const T* __restrict__ A, int64_t lda, int64_t nrows,
-                                                   const double* __restrict__ x, double* __restrict__ y)
+                                                   const double* __restrict__ x, double* __restrict__ y,
+                                                   const int* __restrict__ skip)
 {
+    if (skip && *skip) return;
     __shared__ double sh[kBlock / 64];
     const int64_t j = blockIdx.x;
     const T* col = A + j * lda;
@@ -592,6 +598,28 @@ __global__ __launch_bounds__(kBlock) void k_axpy_dev(const double* __restrict__ 
 }
 
 // y = x / (*d)
+__global__ __launch_bounds__(kBlock) void k_dot_part_skip(const double* __restrict__ a, const double* __restrict__ b,
+                                                          int64_t n, double* __restrict__ part,
+                                                          const int* __restrict__ skip)
+{
+    if (*skip) return;
+    __shared__ double sh[kBlock / 64];
+    double s = 0.;
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
+        s += a[i] * b[i];
+    s = block_sum(s, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+__global__ __launch_bounds__(kBlock) void k_dot_final_skip(const double* __restrict__ part, int nparts,
+                                                           double* __restrict__ out, const int* __restrict__ skip)
+{
+    if (*skip) return;
+    __shared__ double sh[kBlock / 64];
+    double s = 0.;
+    for (int k = threadIdx.x; k < nparts; k += kBlock) s += part[k];
+    s = block_sum(s, sh);
+    if (threadIdx.x == 0) *out = sqrt(s);
+}
 __global__ __launch_bounds__(kBlock) void k_div_dev(const double* __restrict__ d, const double* __restrict__ x,
                                                     double* __restrict__ y, int64_t n)
 {
@@ -600,6 +628,92 @@ __global__ __launch_bounds__(kBlock) void k_div_dev(const double* __restrict__ d
 }
 
 // z = a*x + b*y
+// ---- GMRES Arnoldi step on device state (ints: [0] j, [1] done, [2] jlimit, [3] converged)
+// h[q] = V[:, q] . w for q <= j (one block per column of the basis's capacity)
+__global__ __launch_bounds__(kBlock) void k_arn_proj(const double* __restrict__ V, int64_t n,
+                                                     const double* __restrict__ w, double* __restrict__ h,
+                                                     const int* __restrict__ ints)
+{
+    if (ints[1]) return;
+    const int64_t q = blockIdx.x;
+    if (q > ints[0]) return;
+    __shared__ double sh[kBlock / 64];
+    const double* col = V + q * n;
+    double s = 0.;
+    for (int64_t i = threadIdx.x; i < n; i += kBlock) s += col[i] * w[i];
+    s = block_sum(s, sh);
+    if (threadIdx.x == 0) h[q] = s;
+}
+// part[k][i] = sum over the k-th column range (of columns 0..j) of V[i, q] h[q]
+__global__ __launch_bounds__(kBlock) void k_arn_update_part(const double* __restrict__ V, int64_t n,
+                                                            int64_t per_split, const double* __restrict__ h,
+                                                            double* __restrict__ part, const int* __restrict__ ints)
+{
+    if (ints[1]) return;
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const int64_t ncols = (int64_t)ints[0] + 1;
+    const int64_t j0 = (int64_t)blockIdx.y * per_split;
+    int64_t j1 = j0 + per_split;
+    if (j1 > ncols) j1 = ncols;
+    if (i >= n) return;
+    double a0 = 0., a1 = 0., a2 = 0., a3 = 0.;
+    int64_t q = j0;
+    for (; q + 4 <= j1; q += 4) {
+        a0 += V[(q + 0) * n + i] * h[q + 0];
+        a1 += V[(q + 1) * n + i] * h[q + 1];
+        a2 += V[(q + 2) * n + i] * h[q + 2];
+        a3 += V[(q + 3) * n + i] * h[q + 3];
+    }
+    for (; q < j1; q++) a0 += V[q * n + i] * h[q];
+    part[(int64_t)blockIdx.y * n + i] = (a0 + a1) + (a2 + a3);
+}
+// V[:, j+1] = vcur = w / nrm
+__global__ __launch_bounds__(kBlock) void k_arn_normalize(const double* __restrict__ w, const double* __restrict__ nrm,
+                                                          double* __restrict__ V, double* __restrict__ vcur,
+                                                          int64_t n, const int* __restrict__ ints)
+{
+    if (ints[1]) return;
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const double v = w[i] * (1.0 / (*nrm));
+    V[((int64_t)ints[0] + 1) * n + i] = v;
+    vcur[i] = v;
+}
+// Column j of the Hessenberg matrix from the two projection passes and the norm, rotated by
+// the previous Givens rotations; the new rotation zeroes H[j+1, j] and rotates p
+// (KGeneralizedMinimalResidual.hh:178-271).  Rounded as the host sequence it replaces: no
+// contraction.
+__global__ void k_arn_givens(const double* __restrict__ h1, const double* __restrict__ h2,
+                             const double* __restrict__ nrm, double* __restrict__ H, int ld, double* __restrict__ cs,
+                             double* __restrict__ sn, double* __restrict__ p, int* __restrict__ ints,
+                             double* __restrict__ scal)
+{
+    _Pragma("clang fp contract(off)");
+    if (threadIdx.x != 0 || ints[1]) return;
+    const int j = ints[0];
+    double* h = H + (int64_t)j * ld;
+    for (int r = 0; r <= j; r++) h[r] = h1[r] + h2[r];
+    const double beta = *nrm;
+    h[j + 1] = beta;
+    for (int r = 0; r < j; r++) {
+        const double a = h[r], bb = h[r + 1];
+        h[r] = cs[r] * a + sn[r] * bb;
+        h[r + 1] = -1.0 * sn[r] * a + cs[r] * bb;
+    }
+    const double gam = sqrt(h[j] * h[j] + beta * beta);
+    cs[j] = h[j] / gam;
+    sn[j] = beta / gam;
+    h[j] = gam;
+    p[j + 1] = -1.0 * sn[j] * p[j];
+    p[j] = cs[j] * p[j];
+    const double rn = fabs(p[j + 1]);
+    scal[1] = rn;
+    ints[0] = j + 1;
+    const bool conv = rn <= scal[0];
+    ints[3] = conv ? 1 : 0;
+    if (conv || j + 1 >= ints[2] || !(beta > 0.)) ints[1] = 1;
+}
+
 __global__ __launch_bounds__(kBlock) void k_axpby(double a, const double* __restrict__ x, double b,
                                                   const double* __restrict__ y, double* __restrict__ z, int64_t n)
 {
@@ -993,34 +1107,55 @@ int gemv_splits(int64_t nrows, int64_t ncols)
 
 template <typename T>
 hipError_t launch_gemv(hipStream_t st, const T* A, int64_t lda, int64_t nrows, int64_t ncols, const double* x,
-                       double alpha, double beta, double* y, double* part)
+                       double alpha, double beta, double* y, double* part, const int* skip)
 {
     int nsplit = gemv_splits(nrows, ncols);
     const int64_t per = (ncols + nsplit - 1) / nsplit;
     nsplit = (int)((ncols + per - 1) / per);
     dim3 grid(nblocks(nrows), (unsigned)nsplit);
-    k_gemv_part<T><<<grid, kBlock, 0, st>>>(A, lda, nrows, ncols, per, x, part);
-    k_gemv_reduce<<<nblocks(nrows), kBlock, 0, st>>>(part, nsplit, nrows, alpha, beta, y);
+    k_gemv_part<T><<<grid, kBlock, 0, st>>>(A, lda, nrows, ncols, per, x, part, skip);
+    k_gemv_reduce<<<nblocks(nrows), kBlock, 0, st>>>(part, nsplit, nrows, alpha, beta, y, skip);
     return hipGetLastError();
 }
 template hipError_t launch_gemv<double>(hipStream_t, const double*, int64_t, int64_t, int64_t, const double*, double,
-                                        double, double*, double*);
+                                        double, double*, double*, const int*);
 template hipError_t launch_gemv<float>(hipStream_t, const float*, int64_t, int64_t, int64_t, const double*, double,
-                                       double, double*, double*);
+                                       double, double*, double*, const int*);
 
 template <typename T>
 hipError_t launch_gemv_t(hipStream_t st, const T* A, int64_t lda, int64_t nrows, int64_t ncols, const double* x,
-                         double* y)
+                         double* y, const int* skip)
 {
-    k_gemv_t<T><<<(unsigned)ncols, kBlock, 0, st>>>(A, lda, nrows, x, y);
+    k_gemv_t<T><<<(unsigned)ncols, kBlock, 0, st>>>(A, lda, nrows, x, y, skip);
     return hipGetLastError();
 }
 template hipError_t launch_gemv_t<double>(hipStream_t, const double*, int64_t, int64_t, int64_t, const double*,
-                                          double*);
+                                          double*, const int*);
 template hipError_t launch_gemv_t<float>(hipStream_t, const float*, int64_t, int64_t, int64_t, const double*,
-                                         double*);
+                                         double*, const int*);
 
-constexpr int kDotParts = 256;
+hipError_t launch_arnoldi_step(hipStream_t st, const ArnoldiState& a)
+{
+    const int64_t n = a.n, ncap = (int64_t)a.kmax + 1;
+    int nsplit = gemv_splits(n, ncap);
+    const int64_t per = (ncap + nsplit - 1) / nsplit;
+    nsplit = (int)((ncap + per - 1) / per);
+    const int* ints = a.ints;
+    double* gpart = a.part + 256;
+    // CGS2: h1 = V^T w, w -= V h1; h2 = V^T w, w -= V h2 (columns 0..j)
+    for (double* h : {a.h1, a.h2}) {
+        k_arn_proj<<<(unsigned)ncap, kBlock, 0, st>>>(a.V, n, a.w, h, ints);
+        k_arn_update_part<<<dim3(nblocks(n), (unsigned)nsplit), kBlock, 0, st>>>(a.V, n, per, h, gpart, ints);
+        k_gemv_reduce<<<nblocks(n), kBlock, 0, st>>>(gpart, nsplit, n, -1.0, 1.0, a.w, ints + 1);
+    }
+    const int nb = (int)std::max<int64_t>(1, std::min<int64_t>(kDotParts, nblocks(n)));
+    k_dot_part_skip<<<nb, kBlock, 0, st>>>(a.w, a.w, n, a.part, ints + 1);
+    k_dot_final_skip<<<1, kBlock, 0, st>>>(a.part, nb, a.nrm, ints + 1);
+    k_arn_normalize<<<nblocks(n), kBlock, 0, st>>>(a.w, a.nrm, a.V, a.vcur, n, ints);
+    k_arn_givens<<<1, 64, 0, st>>>(a.h1, a.h2, a.nrm, a.H, a.kmax + 1, a.cs, a.sn, a.p, a.ints, a.scal);
+    return hipGetLastError();
+}
+
 
 hipError_t launch_dot(hipStream_t st, const double* a, const double* b, int64_t n, double* part, double* out,
                       bool want_sqrt)
