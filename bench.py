@@ -122,6 +122,37 @@ def main():
     else:
         sigma = np.random.default_rng(3).normal(size=N) * 1e-7
 
+    # ---------------------------------------------------------------- single-RHS matvec (HBM)
+    if A is not None:
+        v = np.random.default_rng(5).normal(size=N)
+        mv = {}
+        for name, Am in (("fp64", A),):
+            Am.Multiply(v)
+            ctx.check(lib.kem_ctx_set_kernel_timing(ctx.h, 1))
+            for _ in range(10):
+                Am.Multiply(v)
+            kt, kn = ctypes.c_double(), ctypes.c_int()
+            ctx.check(lib.kem_ctx_kernel_time(ctx.h, ctypes.byref(kt), ctypes.byref(kn)))
+            ctx.check(lib.kem_ctx_set_kernel_timing(ctx.h, 0))
+            ms_mv = kt.value / max(kn.value, 1)
+            mv[name] = (ms_mv, 8.0 * N * N / (ms_mv * 1e-3) / 1e9)
+        A32 = K.KBoundaryIntegralMatrix(cont, fp64=False, fp32=True)
+        A32.Multiply(v)
+        ctx.check(lib.kem_ctx_set_kernel_timing(ctx.h, 1))
+        for _ in range(10):
+            A32.Multiply(v)
+        kt, kn = ctypes.c_double(), ctypes.c_int()
+        ctx.check(lib.kem_ctx_kernel_time(ctx.h, ctypes.byref(kt), ctypes.byref(kn)))
+        ctx.check(lib.kem_ctx_set_kernel_timing(ctx.h, 0))
+        A32.close()
+        ms_mv = kt.value / max(kn.value, 1)
+        mv["fp32"] = (ms_mv, 4.0 * N * N / (ms_mv * 1e-3) / 1e9)
+        log("matvec (kernel time, HBM bytes of the matrix): " +
+            ", ".join(f"{k} {t:.3f} ms = {g:.0f} GB/s ({g / PEAK_HBM_GBS * 100:.0f} % of 8 TB/s)"
+                      for k, (t, g) in mv.items()))
+        out.update(matvec_ms_fp64=mv["fp64"][0], matvec_gbs_fp64=mv["fp64"][1], matvec_ms_fp32=mv["fp32"][0],
+                   matvec_gbs_fp32=mv["fp32"][1])
+
     # ---------------------------------------------------------------- launch shapes
     tuning = None
     if not args.no_autotune:
