@@ -32,8 +32,9 @@ inline unsigned nblocks(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock
 template <int NT>
 __global__ __launch_bounds__(kBlock) void k_bmm32(const float* __restrict__ A, int64_t n,
                                                   const float* __restrict__ Vt, int64_t per_split,
-                                                  float* __restrict__ part)
+                                                  float* __restrict__ part, const int* __restrict__ skip)
 {
+    if (skip && *skip) return;
     constexpr int KP = 32 * NT;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int64_t i0 = ((int64_t)blockIdx.x * 4 + wave) * 64;
@@ -95,8 +96,10 @@ __global__ __launch_bounds__(kBlock) void k_bmm32(const float* __restrict__ A, i
 // order) and transposed through LDS so that both the reads and the writes are coalesced.
 template <int KP>
 __global__ __launch_bounds__(kBlock) void k_bmm32_reduce(const float* __restrict__ part, int nsplit, int64_t n,
-                                                         int ncols, double* __restrict__ W, int64_t ldw)
+                                                         int ncols, double* __restrict__ W, int64_t ldw,
+                                                         const int* __restrict__ skip)
 {
+    if (skip && *skip) return;
     __shared__ double sh[KP][65];
     const int64_t i0 = (int64_t)blockIdx.x * 64;
     for (int e = threadIdx.x; e < 64 * KP; e += kBlock) {
@@ -121,8 +124,9 @@ __global__ __launch_bounds__(kBlock) void k_bmm32_reduce(const float* __restrict
 template <int KB>
 __global__ __launch_bounds__(kBlock) void k_bmm64(const double* __restrict__ A, int64_t n,
                                                   const double* __restrict__ Xt, int64_t per_split,
-                                                  double* __restrict__ part)
+                                                  double* __restrict__ part, const int* __restrict__ skip)
 {
+    if (skip && *skip) return;
     const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     const int64_t ic = std::min<int64_t>(i, n - 1);
     const int64_t j0 = (int64_t)blockIdx.y * per_split;
@@ -142,8 +146,10 @@ __global__ __launch_bounds__(kBlock) void k_bmm64(const double* __restrict__ A, 
 
 // W[c*ldw + i] = sum_s part[s][c][i] for c < ncols
 __global__ __launch_bounds__(kBlock) void k_bmm64_reduce(const double* __restrict__ part, int nsplit, int kb,
-                                                         int64_t n, int ncols, double* __restrict__ W, int64_t ldw)
+                                                         int64_t n, int ncols, double* __restrict__ W, int64_t ldw,
+                                                         const int* __restrict__ skip)
 {
+    if (skip && *skip) return;
     const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     const int c = blockIdx.y;
     if (i >= n || c >= ncols) return;
@@ -197,8 +203,9 @@ __global__ __launch_bounds__(kBlock) void k_bproj(const double* __restrict__ V, 
 __global__ __launch_bounds__(kBlock) void k_bupdate(const double* __restrict__ V, int64_t vstride,
                                                     const double* __restrict__ h, int ldh,
                                                     const int* __restrict__ cnt, double sign, int64_t n,
-                                                    double* __restrict__ Y)
+                                                    double* __restrict__ Y, const int* __restrict__ skip)
 {
+    if (skip && *skip) return;
     const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     const int r = blockIdx.y;
     if (i >= n) return;
@@ -212,8 +219,9 @@ __global__ __launch_bounds__(kBlock) void k_bupdate(const double* __restrict__ V
 
 // nrm[r] = ||W_r|| (block per r)
 __global__ __launch_bounds__(kBlock) void k_bnorm(const double* __restrict__ W, int64_t n, double* __restrict__ nrm,
-                                                  int stride)
+                                                  int stride, const int* __restrict__ skip)
 {
+    if (skip && *skip) return;
     __shared__ double sh[kBlock / 64];
     const int r = blockIdx.x;
     const double* w = W + (int64_t)r * n;
@@ -247,6 +255,123 @@ __global__ __launch_bounds__(kBlock) void k_baxpby(double a, const double* __res
     if (e < len) Z[e] = a * X[e] + b * Y[e];
 }
 
+// ---- one lockstep GMRES iteration on device state (ints: [0] j, [1] stop, [2] jlimit, [3] steps)
+// h[r*ldh + q] = V_r q . W_r for q <= j
+__global__ __launch_bounds__(kBlock) void k_bproj_idx(const double* __restrict__ V, int64_t vstride,
+                                                      const double* __restrict__ W, int64_t n, double* __restrict__ h,
+                                                      int ldh, const int* __restrict__ ints)
+{
+    if (ints[1]) return;
+    const int q = blockIdx.x, r = blockIdx.y;
+    if (q > ints[0]) return;
+    __shared__ double sh[kBlock / 64];
+    const double* v = V + r * vstride + (int64_t)q * n;
+    const double* w = W + (int64_t)r * n;
+    double s = 0.;
+    for (int64_t i = threadIdx.x; i < n; i += kBlock) s += v[i] * w[i];
+    s = block_sum(s, sh);
+    if (threadIdx.x == 0) h[r * ldh + q] = s;
+}
+
+// V_r (j+1) = W_r / nrm_r; panel column r = that for active r, 0 otherwise
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_bnormalize_idx(const double* __restrict__ W,
+                                                           const double* __restrict__ nrm,
+                                                           const int* __restrict__ active, int64_t n,
+                                                           double* __restrict__ V, int64_t vstride, int kp,
+                                                           T* __restrict__ Vt, const int* __restrict__ ints)
+{
+    if (ints[1]) return;
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const int r = blockIdx.y;
+    if (i >= n) return;
+    const double v = active[r] ? W[(int64_t)r * n + i] * (1.0 / nrm[r]) : 0.;
+    V[r * vstride + (int64_t)(ints[0] + 1) * n + i] = v;
+    Vt[i * kp + r] = (T)v;
+}
+
+// Column j of each active right-hand side's Hessenberg matrix, its Givens rotations and the
+// stop test (the single-RHS algorithm per column, KGeneralizedMinimalResidual.hh:178-271);
+// j += 1, cnt = j + 1 for the next step; stop when no column is active or j reaches jlimit.
+// One block of 64 threads, thread r = right-hand side r.  No contraction: the roundings of
+// the host sequence it replaces.
+__global__ __launch_bounds__(64) void k_bgivens(const double* __restrict__ h1, const double* __restrict__ h2,
+                                                const double* __restrict__ nrm, double* __restrict__ H, int ldh,
+                                                double* __restrict__ cs, double* __restrict__ sn,
+                                                double* __restrict__ p, const double* __restrict__ bnorm,
+                                                const double* __restrict__ tol, double* __restrict__ resid,
+                                                int* __restrict__ active, int* __restrict__ done,
+                                                int* __restrict__ jr, int* __restrict__ cnt,
+                                                int* __restrict__ ints, int nrhs)
+{
+    _Pragma("clang fp contract(off)");
+    if (ints[1]) return;
+    const int j = ints[0];
+    const int r = threadIdx.x;
+    __shared__ int nact;
+    if (r == 0) nact = 0;
+    __syncthreads();
+    if (r < nrhs && active[r]) {
+        double* h = H + ((int64_t)r * ldh + j) * ldh;
+        const double* a1 = h1 + (int64_t)r * ldh;
+        const double* a2 = h2 + (int64_t)r * ldh;
+        for (int q = 0; q <= j; q++) h[q] = a1[q] + a2[q];
+        const double beta = nrm[r];
+        h[j + 1] = beta;
+        double* csr = cs + (int64_t)r * ldh;
+        double* snr = sn + (int64_t)r * ldh;
+        double* pr = p + (int64_t)r * ldh;
+        for (int q = 0; q < j; q++) {
+            const double a = h[q], bb = h[q + 1];
+            h[q] = csr[q] * a + snr[q] * bb;
+            h[q + 1] = -1.0 * snr[q] * a + csr[q] * bb;
+        }
+        const double gam = sqrt(h[j] * h[j] + beta * beta);
+        csr[j] = h[j] / gam;
+        snr[j] = beta / gam;
+        h[j] = gam;
+        pr[j + 1] = -1.0 * snr[j] * pr[j];
+        pr[j] = csr[j] * pr[j];
+        jr[r] = j + 1;
+        const double rr = fabs(pr[j + 1]) / bnorm[r];
+        resid[r] = rr;
+        if (rr <= tol[r] || !(beta > 0.)) {
+            active[r] = 0;
+            if (rr <= tol[r]) done[r] = 1;
+        } else {
+            atomicAdd(&nact, 1);
+        }
+    }
+    __syncthreads();
+    if (r < nrhs) cnt[r] = j + 2;
+    if (r == 0) {
+        ints[0] = j + 1;
+        ints[3] += 1;
+        if (nact == 0 || j + 1 >= ints[2]) ints[1] = 1;
+    }
+}
+
+// y_r = H_r^{-1} p_r over the jr[r] leading columns (back substitution, host order);
+// cnt[r] = jr[r].  One thread per right-hand side.
+__global__ __launch_bounds__(64) void k_btrisolve(const double* __restrict__ H, int ldh, const double* __restrict__ p,
+                                                  const int* __restrict__ jr, double* __restrict__ Y,
+                                                  int* __restrict__ cnt, int nrhs)
+{
+    _Pragma("clang fp contract(off)");
+    const int r = blockIdx.x * 64 + threadIdx.x;
+    if (r >= nrhs) return;
+    const int m = jr[r];
+    const double* Hr = H + (int64_t)r * ldh * ldh;
+    const double* pr = p + (int64_t)r * ldh;
+    double* yr = Y + (int64_t)r * ldh;
+    for (int q = m - 1; q >= 0; q--) {
+        double s = pr[q];
+        for (int u = q + 1; u < m; u++) s -= Hr[(int64_t)u * ldh + q] * yr[u];
+        yr[q] = s / Hr[(int64_t)q * ldh + q];
+    }
+    cnt[r] = m;
+}
+
 int bmm_splits(int64_t n, int64_t tiles)
 {
     int64_t s = (launch_tuning().bmm_blocks + tiles - 1) / std::max<int64_t>(tiles, 1);
@@ -271,7 +396,7 @@ size_t bmm_workspace_bytes(int64_t n, int nrhs, bool fp32)
 }
 
 hipError_t launch_bmm(hipStream_t st, const float* A32, const double* A64, int64_t n, const double* X, int64_t ldx,
-                      int nrhs, double* Y, int64_t ldy, void* ws, size_t ws_bytes, bool x_is_panel)
+                      int nrhs, double* Y, int64_t ldy, void* ws, size_t ws_bytes, bool x_is_panel, const int* skip)
 {
     if (n <= 0 || nrhs <= 0) return hipSuccess;
     if (nrhs > 64 || ws_bytes < bmm_workspace_bytes(n, nrhs, A32 != nullptr)) return hipErrorInvalidValue;
@@ -289,14 +414,14 @@ hipError_t launch_bmm(hipStream_t st, const float* A32, const double* A64, int64
             k_panel<float><<<nblocks(n * kp), kBlock, 0, st>>>(X, ldx, n, nrhs, kp, Vt);
         dim3 grid((unsigned)((tiles + 3) / 4), (unsigned)ns);
         if (nt == 1)
-            k_bmm32<1><<<grid, kBlock, 0, st>>>(A32, n, P, per, part);
+            k_bmm32<1><<<grid, kBlock, 0, st>>>(A32, n, P, per, part, skip);
         else
-            k_bmm32<2><<<grid, kBlock, 0, st>>>(A32, n, P, per, part);
+            k_bmm32<2><<<grid, kBlock, 0, st>>>(A32, n, P, per, part, skip);
         const unsigned gr = (unsigned)((n + 63) / 64);
         if (kp == 32)
-            k_bmm32_reduce<32><<<gr, kBlock, 0, st>>>(part, ns, n, nrhs, Y, ldy);
+            k_bmm32_reduce<32><<<gr, kBlock, 0, st>>>(part, ns, n, nrhs, Y, ldy, skip);
         else
-            k_bmm32_reduce<64><<<gr, kBlock, 0, st>>>(part, ns, n, nrhs, Y, ldy);
+            k_bmm32_reduce<64><<<gr, kBlock, 0, st>>>(part, ns, n, nrhs, Y, ldy, skip);
         return hipGetLastError();
     }
     const int kb = nrhs <= 16 ? 16 : 64;
@@ -310,10 +435,10 @@ hipError_t launch_bmm(hipStream_t st, const float* A32, const double* A64, int64
     if (!x_is_panel) k_panel<double><<<nblocks(n * kb), kBlock, 0, st>>>(X, ldx, n, nrhs, kb, Xt);
     dim3 grid((unsigned)blocks, (unsigned)ns);
     if (kb == 16)
-        k_bmm64<16><<<grid, kBlock, 0, st>>>(A64, n, P, per, part);
+        k_bmm64<16><<<grid, kBlock, 0, st>>>(A64, n, P, per, part, skip);
     else
-        k_bmm64<64><<<grid, kBlock, 0, st>>>(A64, n, P, per, part);
-    k_bmm64_reduce<<<dim3((unsigned)blocks, (unsigned)nrhs), kBlock, 0, st>>>(part, ns, kb, n, nrhs, Y, ldy);
+        k_bmm64<64><<<grid, kBlock, 0, st>>>(A64, n, P, per, part, skip);
+    k_bmm64_reduce<<<dim3((unsigned)blocks, (unsigned)nrhs), kBlock, 0, st>>>(part, ns, kb, n, nrhs, Y, ldy, skip);
     return hipGetLastError();
 }
 
@@ -329,14 +454,14 @@ hipError_t launch_bupdate(hipStream_t st, const double* V, int64_t vstride, cons
                           double sign, int64_t n, int nrhs, double* Y)
 {
     if (nrhs <= 0) return hipSuccess;
-    k_bupdate<<<dim3(nblocks(n), (unsigned)nrhs), kBlock, 0, st>>>(V, vstride, h, ldh, cnt, sign, n, Y);
+    k_bupdate<<<dim3(nblocks(n), (unsigned)nrhs), kBlock, 0, st>>>(V, vstride, h, ldh, cnt, sign, n, Y, nullptr);
     return hipGetLastError();
 }
 
 hipError_t launch_bnorm(hipStream_t st, const double* W, int64_t n, int nrhs, double* nrm, int stride)
 {
     if (nrhs <= 0) return hipSuccess;
-    k_bnorm<<<(unsigned)nrhs, kBlock, 0, st>>>(W, n, nrm, stride);
+    k_bnorm<<<(unsigned)nrhs, kBlock, 0, st>>>(W, n, nrm, stride, nullptr);
     return hipGetLastError();
 }
 
@@ -350,6 +475,38 @@ hipError_t launch_bnormalize(hipStream_t st, const double* W, const double* nrm,
         k_bnormalize<float><<<grid, kBlock, 0, st>>>(W, nrm, stride, active, n, V, vstride, q, kp, Vt32);
     else
         k_bnormalize<double><<<grid, kBlock, 0, st>>>(W, nrm, stride, active, n, V, vstride, q, kp, Vt64);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_arnoldi_step(hipStream_t st, const BatchArnoldi& a)
+{
+    const int64_t n = a.n;
+    const int nrhs = a.nrhs;
+    int* stop = a.ints + 1;
+    hipError_t e = launch_bmm(st, a.use32 ? a.A32 : nullptr, a.use32 ? nullptr : a.A64, n,
+                              a.use32 ? (const double*)a.P32 : a.P64, n, nrhs, a.W, n, a.ws, a.ws_bytes, true, stop);
+    if (e != hipSuccess) return e;
+    for (double* h : {a.h1, a.h2}) {
+        k_bproj_idx<<<dim3((unsigned)(a.kmax + 1), (unsigned)nrhs), kBlock, 0, st>>>(a.V, a.vstride, a.W, n, h, a.ldh,
+                                                                                   a.ints);
+        k_bupdate<<<dim3(nblocks(n), (unsigned)nrhs), kBlock, 0, st>>>(a.V, a.vstride, h, a.ldh, a.cnt, -1.0, n, a.W,
+                                                                      stop);
+    }
+    k_bnorm<<<(unsigned)nrhs, kBlock, 0, st>>>(a.W, n, a.nrm, 1, stop);
+    if (a.use32)
+        k_bnormalize_idx<float><<<dim3(nblocks(n), (unsigned)nrhs), kBlock, 0, st>>>(a.W, a.nrm, a.active, n, a.V,
+                                                                                    a.vstride, a.kp, a.P32, a.ints);
+    else
+        k_bnormalize_idx<double><<<dim3(nblocks(n), (unsigned)nrhs), kBlock, 0, st>>>(a.W, a.nrm, a.active, n, a.V,
+                                                                                     a.vstride, a.kp, a.P64, a.ints);
+    k_bgivens<<<1, 64, 0, st>>>(a.h1, a.h2, a.nrm, a.H, a.ldh, a.cs, a.sn, a.p, a.bnorm, a.tol, a.resid, a.active,
+                                a.done, a.jr, a.cnt, a.ints, nrhs);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_trisolve(hipStream_t st, const BatchArnoldi& a, double* Y)
+{
+    k_btrisolve<<<(unsigned)((a.nrhs + 63) / 64), 64, 0, st>>>(a.H, a.ldh, a.p, a.jr, Y, a.cnt, a.nrhs);
     return hipGetLastError();
 }
 

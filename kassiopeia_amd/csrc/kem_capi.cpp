@@ -1255,33 +1255,90 @@ int gmres_batch(kem_matrix* A, bool use32, int nrhs, const double* d_B, double* 
     const int ldh = kmax + 2;
     const int kp = panel_cols(nrhs, use32);
     const int64_t vstride = (int64_t)(kmax + 1) * n;
-    DVec V, W, h1, h2, Y;
+    DVec V, W, h1, h2, Y, H, rot, vec;
     DBuf panel, ws, ints;
     KEM_CK(c, dalloc(&V.p, vstride * nrhs));
     KEM_CK(c, dalloc(&W.p, n * nrhs));
     KEM_CK(c, dalloc(&h1.p, (int64_t)ldh * nrhs));
     KEM_CK(c, dalloc(&h2.p, (int64_t)ldh * nrhs));
     KEM_CK(c, dalloc(&Y.p, (int64_t)ldh * nrhs));
+    KEM_CK(c, dalloc(&H.p, (int64_t)nrhs * ldh * ldh));
+    KEM_CK(c, dalloc(&rot.p, 3 * (int64_t)ldh * nrhs));
+    KEM_CK(c, dalloc(&vec.p, 4 * (int64_t)nrhs));
     KEM_CK(c, hipMalloc(&panel.p, (use32 ? sizeof(float) : sizeof(double)) * (size_t)n * kp));
     KEM_CK(c, hipMemsetAsync(panel.p, 0, (use32 ? sizeof(float) : sizeof(double)) * (size_t)n * kp, st));
-    KEM_CK(c, hipMalloc(&ints.p, sizeof(int) * 2 * nrhs));
-    int* d_cnt = (int*)ints.p;
-    int* d_active = d_cnt + nrhs;
-    float* P32 = use32 ? (float*)panel.p : nullptr;
-    double* P64 = use32 ? nullptr : (double*)panel.p;
+    KEM_CK(c, hipMalloc(&ints.p, sizeof(int) * (4 * (size_t)nrhs + 4)));
     const size_t wsb = kem::bmm_workspace_bytes(n, nrhs, use32);
     KEM_CK(c, hipMalloc(&ws.p, wsb));
     ws.bytes = wsb;
+    kem::BatchArnoldi a;
+    a.n = n;
+    a.nrhs = nrhs;
+    a.kmax = kmax;
+    a.ldh = ldh;
+    a.kp = kp;
+    a.use32 = use32;
+    a.A32 = A->A32;
+    a.A64 = A->A64;
+    a.V = V.p;
+    a.vstride = vstride;
+    a.W = W.p;
+    a.P32 = use32 ? (float*)panel.p : nullptr;
+    a.P64 = use32 ? nullptr : (double*)panel.p;
+    a.ws = ws.p;
+    a.ws_bytes = ws.bytes;
+    a.h1 = h1.p;
+    a.h2 = h2.p;
+    a.H = H.p;
+    a.cs = rot.p;
+    a.sn = rot.p + (int64_t)ldh * nrhs;
+    a.p = rot.p + 2 * (int64_t)ldh * nrhs;
+    a.nrm = vec.p;
+    a.bnorm = vec.p + nrhs;
+    a.tol = vec.p + 2 * nrhs;
+    a.resid = vec.p + 3 * nrhs;
+    a.active = (int*)ints.p;
+    a.done = a.active + nrhs;
+    a.jr = a.done + nrhs;
+    a.cnt = a.jr + nrhs;
+    a.ints = a.cnt + nrhs;
 
-    std::vector<double> bnorm(nrhs), H((size_t)nrhs * ldh * ldh), cs((size_t)nrhs * ldh), sn((size_t)nrhs * ldh),
-        p((size_t)nrhs * ldh), y((size_t)nrhs * ldh), hh1((size_t)nrhs * ldh), hh2((size_t)nrhs * ldh);
-    std::vector<int> active(nrhs), jr(nrhs), cnt(nrhs), done(nrhs, 0);
+    std::vector<double> bnorm(nrhs), hp((size_t)ldh * nrhs, 0.), hh1(nrhs);
+    std::vector<int> active(nrhs), done(nrhs, 0), ones(nrhs, 1);
     KEM_CK(c, kem::launch_bnorm(st, d_B, n, nrhs, h1.p, ldh));
     KEM_CK(c, hipMemcpy2DAsync(bnorm.data(), sizeof(double), h1.p, sizeof(double) * ldh, sizeof(double), nrhs,
                                hipMemcpyDeviceToHost, st));
     KEM_CK(c, hipStreamSynchronize(st));
     for (auto& b : bnorm)
         if (b == 0.) b = 1.;
+    KEM_CK(c, hipMemcpyAsync(a.bnorm, bnorm.data(), sizeof(double) * nrhs, hipMemcpyHostToDevice, st));
+    KEM_CK(c, hipMemcpyAsync(a.tol, tol.data(), sizeof(double) * nrhs, hipMemcpyHostToDevice, st));
+    KEM_CK(c, hipMemcpyAsync(a.done, done.data(), sizeof(int) * nrhs, hipMemcpyHostToDevice, st));
+
+    // one lockstep iteration as a graph (launched directly if the stream cannot be captured)
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    struct GraphGuard {
+        hipGraph_t* g;
+        hipGraphExec_t* e;
+        ~GraphGuard()
+        {
+            if (*e) (void)hipGraphExecDestroy(*e);
+            if (*g) (void)hipGraphDestroy(*g);
+        }
+    } graph_guard{&graph, &exec};
+    if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) == hipSuccess) {
+        const hipError_t e1 = kem::launch_batch_arnoldi_step(st, a);
+        const hipError_t e2 = hipStreamEndCapture(st, &graph);
+        if (e1 != hipSuccess || e2 != hipSuccess || hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess) {
+            if (exec) (void)hipGraphExecDestroy(exec);
+            if (graph) (void)hipGraphDestroy(graph);
+            exec = nullptr;
+            graph = nullptr;
+        }
+    }
+    (void)hipGetLastError();
+
     resid.assign(nrhs, 0.);
     int it = 0, status = KEM_OK;
     for (;;) {
@@ -1295,93 +1352,47 @@ int gmres_batch(kem_matrix* A, bool use32, int nrhs, const double* d_B, double* 
         KEM_CK(c, hipStreamSynchronize(st));
         int nact = 0;
         for (int r = 0; r < nrhs; r++) {
-            p[(size_t)r * ldh] = hh1[r];
+            hp[(size_t)r * ldh] = hh1[r];
             resid[r] = hh1[r] / bnorm[r];
             active[r] = !done[r] && hh1[r] > 0. && resid[r] > tol[r];
             if (!active[r]) done[r] = 1;
-            jr[r] = 0;
             nact += active[r];
         }
         if (nact == 0) break;
-        KEM_CK(c, hipMemcpyAsync(d_active, active.data(), sizeof(int) * nrhs, hipMemcpyHostToDevice, st));
-        KEM_CK(c, kem::launch_bnormalize(st, W.p, h1.p, ldh, d_active, n, nrhs, V.p, vstride, 0, kp, P32, P64));
-        int j = 0;
-        bool stop = false;
+        const int jlimit = std::min(kmax, maxit - it);
+        const int hints[4] = {0, 0, jlimit, 0};
+        KEM_CK(c, hipMemcpyAsync(a.active, active.data(), sizeof(int) * nrhs, hipMemcpyHostToDevice, st));
+        KEM_CK(c, hipMemcpyAsync(a.done, done.data(), sizeof(int) * nrhs, hipMemcpyHostToDevice, st));
+        KEM_CK(c, hipMemcpyAsync(a.cnt, ones.data(), sizeof(int) * nrhs, hipMemcpyHostToDevice, st));
+        KEM_CK(c, hipMemcpyAsync(a.ints, hints, sizeof(hints), hipMemcpyHostToDevice, st));
+        KEM_CK(c, hipMemcpyAsync(a.p, hp.data(), sizeof(double) * ldh * nrhs, hipMemcpyHostToDevice, st));
+        KEM_CK(c, hipMemcpyAsync(a.resid, resid.data(), sizeof(double) * nrhs, hipMemcpyHostToDevice, st));
+        KEM_CK(c, hipMemsetAsync(a.jr, 0, sizeof(int) * nrhs, st));
+        KEM_CK(c, kem::launch_bnormalize(st, W.p, h1.p, ldh, a.active, n, nrhs, V.p, vstride, 0, kp, a.P32, a.P64));
+        int got[4] = {0, 0, 0, 0};
+        bool expired = false;
         for (;;) {
-            KEM_CK(c, kem::launch_bmm(st, use32 ? A->A32 : nullptr, use32 ? nullptr : A->A64, n,
-                                      use32 ? (const double*)P32 : P64, n, nrhs, W.p, n, ws.p, ws.bytes, true));
-            for (int r = 0; r < nrhs; r++) cnt[r] = j + 1;
-            KEM_CK(c, hipMemcpyAsync(d_cnt, cnt.data(), sizeof(int) * nrhs, hipMemcpyHostToDevice, st));
-            KEM_CK(c, kem::launch_bproj(st, V.p, vstride, W.p, n, j + 1, nrhs, h1.p, ldh));
-            KEM_CK(c, kem::launch_bupdate(st, V.p, vstride, h1.p, ldh, d_cnt, -1.0, n, nrhs, W.p));
-            KEM_CK(c, kem::launch_bproj(st, V.p, vstride, W.p, n, j + 1, nrhs, h2.p, ldh));
-            KEM_CK(c, kem::launch_bupdate(st, V.p, vstride, h2.p, ldh, d_cnt, -1.0, n, nrhs, W.p));
-            KEM_CK(c, kem::launch_bnorm(st, W.p, n, nrhs, h1.p + j + 1, ldh));
-            KEM_CK(c, kem::launch_bnormalize(st, W.p, h1.p + j + 1, ldh, d_active, n, nrhs, V.p, vstride, j + 1, kp,
-                                             P32, P64));
-            KEM_CK(c, hipMemcpyAsync(hh1.data(), h1.p, sizeof(double) * ldh * nrhs, hipMemcpyDeviceToHost, st));
-            KEM_CK(c, hipMemcpyAsync(hh2.data(), h2.p, sizeof(double) * ldh * nrhs, hipMemcpyDeviceToHost, st));
+            for (int k = 0; k < kGmresReplays; k++)
+                KEM_CK(c, exec ? hipGraphLaunch(exec, st) : kem::launch_batch_arnoldi_step(st, a));
+            KEM_CK(c, hipMemcpyAsync(got, a.ints, sizeof(got), hipMemcpyDeviceToHost, st));
             KEM_CK(c, hipStreamSynchronize(st));
-            it++;
-            bool changed = false;
-            nact = 0;
-            for (int r = 0; r < nrhs; r++) {
-                if (!active[r]) continue;
-                double* h = H.data() + ((size_t)r * ldh + j) * ldh;
-                const double* a1 = hh1.data() + (size_t)r * ldh;
-                const double* a2 = hh2.data() + (size_t)r * ldh;
-                for (int q = 0; q <= j; q++) h[q] = a1[q] + a2[q];
-                const double beta = a1[j + 1];
-                h[j + 1] = beta;
-                double* csr = cs.data() + (size_t)r * ldh;
-                double* snr = sn.data() + (size_t)r * ldh;
-                double* pr = p.data() + (size_t)r * ldh;
-                for (int q = 0; q < j; q++) {
-                    const double a = h[q], bb = h[q + 1];
-                    h[q] = csr[q] * a + snr[q] * bb;
-                    h[q + 1] = -1.0 * snr[q] * a + csr[q] * bb;
-                }
-                const double gam = std::sqrt(h[j] * h[j] + beta * beta);
-                csr[j] = h[j] / gam;
-                snr[j] = beta / gam;
-                h[j] = gam;
-                pr[j + 1] = -1.0 * snr[j] * pr[j];
-                pr[j] = csr[j] * pr[j];
-                jr[r] = j + 1;
-                resid[r] = std::fabs(pr[j + 1]) / bnorm[r];
-                if (resid[r] <= tol[r] || !(beta > 0.)) {
-                    active[r] = 0;
-                    changed = true;
-                    if (resid[r] <= tol[r]) done[r] = 1;
-                }
-                nact += active[r];
+            if (got[1]) break;
+            if (ctl.expired()) {
+                expired = true;
+                break;
             }
-            j++;
-            stop = it >= maxit || ctl.expired();
-            if (nact == 0 || stop || j >= kmax) break;
-            if (changed) KEM_CK(c, hipMemcpyAsync(d_active, active.data(), sizeof(int) * nrhs, hipMemcpyHostToDevice, st));
         }
+        it += got[3];
         // x_r += V_r y_r with H_r y_r = p_r (upper triangular, jr[r] columns)
-        for (int r = 0; r < nrhs; r++) {
-            const int m = jr[r];
-            const double* Hr = H.data() + (size_t)r * ldh * ldh;
-            double* yr = y.data() + (size_t)r * ldh;
-            const double* pr = p.data() + (size_t)r * ldh;
-            for (int q = m - 1; q >= 0; q--) {
-                double s = pr[q];
-                for (int u = q + 1; u < m; u++) s -= Hr[(size_t)u * ldh + q] * yr[u];
-                yr[q] = s / Hr[(size_t)q * ldh + q];
-            }
-            cnt[r] = m;
-        }
-        KEM_CK(c, hipMemcpyAsync(Y.p, y.data(), sizeof(double) * ldh * nrhs, hipMemcpyHostToDevice, st));
-        KEM_CK(c, hipMemcpyAsync(d_cnt, cnt.data(), sizeof(int) * nrhs, hipMemcpyHostToDevice, st));
-        KEM_CK(c, kem::launch_bupdate(st, V.p, vstride, Y.p, ldh, d_cnt, 1.0, n, nrhs, d_X));
+        KEM_CK(c, kem::launch_batch_trisolve(st, a, Y.p));
+        KEM_CK(c, kem::launch_bupdate(st, V.p, vstride, Y.p, ldh, a.cnt, 1.0, n, nrhs, d_X));
+        KEM_CK(c, hipMemcpyAsync(done.data(), a.done, sizeof(int) * nrhs, hipMemcpyDeviceToHost, st));
+        KEM_CK(c, hipMemcpyAsync(resid.data(), a.resid, sizeof(double) * nrhs, hipMemcpyDeviceToHost, st));
         KEM_CK(c, hipStreamSynchronize(st));
         bool all = true;
         for (int r = 0; r < nrhs; r++) all = all && done[r];
         if (all) break;
-        if (stop) {
+        if (expired || it >= maxit) {
             status = KEM_W_NOT_CONVERGED;
             break;
         }

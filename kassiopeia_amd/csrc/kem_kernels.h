@@ -136,7 +136,8 @@ size_t bmm_workspace_bytes(int64_t n, int nrhs, bool fp32);
 // product (X rounded to FP32), else the FP64 product with A64.  x_is_panel: X already is the
 // row-major panel (FP32: n x (nrhs <= 32 ? 32 : 64) floats; FP64: n x (nrhs <= 16 ? 16 : 64)).
 hipError_t launch_bmm(hipStream_t st, const float* A32, const double* A64, int64_t n, const double* X, int64_t ldx,
-                      int nrhs, double* Y, int64_t ldy, void* ws, size_t ws_bytes, bool x_is_panel);
+                      int nrhs, double* Y, int64_t ldy, void* ws, size_t ws_bytes, bool x_is_panel,
+                      const int* skip = nullptr);
 // batched Krylov-basis operations, V[(r*(kmax+1) + q)*n + i] with vstride = (kmax+1)*n
 hipError_t launch_bproj(hipStream_t st, const double* V, int64_t vstride, const double* W, int64_t n, int nq,
                         int nrhs, double* h, int ldh);
@@ -147,6 +148,34 @@ hipError_t launch_bnormalize(hipStream_t st, const double* W, const double* nrm,
                              int64_t n, int nrhs, double* V, int64_t vstride, int q, int kp, float* Vt32,
                              double* Vt64);
 hipError_t launch_baxpby(hipStream_t st, double a, const double* X, double b, const double* Y, double* Z, int64_t len);
+
+// Lockstep GMRES over nrhs <= 64 right-hand sides on device state: after one step the
+// Hessenberg columns, rotations and residuals of every active right-hand side are updated on
+// the device (each with the single-RHS algorithm) and the step is captured as a HIP graph.
+struct BatchArnoldi {
+    int64_t n = 0;
+    int nrhs = 0, kmax = 0, ldh = 0, kp = 0;
+    bool use32 = false;
+    const float* A32 = nullptr;
+    const double* A64 = nullptr;
+    double* V = nullptr;  // nrhs bases of kmax + 1 columns, vstride apart
+    int64_t vstride = 0;
+    double* W = nullptr;  // n x nrhs work block
+    float* P32 = nullptr;  // row-major panel of the current basis vectors (n x kp)
+    double* P64 = nullptr;
+    void* ws = nullptr;  // block-product workspace
+    size_t ws_bytes = 0;
+    double *h1 = nullptr, *h2 = nullptr;  // projections, ldh x nrhs
+    double* nrm = nullptr;                // nrhs
+    double* H = nullptr;                  // nrhs x ldh x ldh
+    double *cs = nullptr, *sn = nullptr, *p = nullptr;  // ldh x nrhs
+    double *bnorm = nullptr, *tol = nullptr, *resid = nullptr;  // nrhs
+    int *active = nullptr, *done = nullptr, *jr = nullptr, *cnt = nullptr;  // nrhs
+    int* ints = nullptr;  // [0] j, [1] stop, [2] jlimit, [3] steps
+};
+hipError_t launch_batch_arnoldi_step(hipStream_t st, const BatchArnoldi& a);
+// Y (ldh x nrhs) = the least-squares coefficients of every right-hand side; cnt = jr
+hipError_t launch_batch_trisolve(hipStream_t st, const BatchArnoldi& a, double* Y);
 
 // ---- adjoint (kem_adjoint.hip): vertex derivatives under the numeric policy, triangles only
 size_t vjp_workspace_doubles(int64_t n, int64_t npts);
