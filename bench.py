@@ -159,7 +159,7 @@ def main():
         from kassiopeia_amd import autotune as T
 
         t0 = time.perf_counter()
-        tuning = {"far_range_mixed": T.tune_field(cont, sigma, pts, "mixed", repeats=1),
+        tuning = {"far_range_mixed": T.tune_field(cont, sigma, pts, "mixed", repeats=2),
                   "far_range_fp64": T.tune_field(cont, sigma, pts, "fp64", repeats=1)}
         if A is not None:
             tuning["gemv_blocks"] = T.tune_matvec(A, repeats=5)
@@ -240,7 +240,12 @@ def main():
     value = pairs / (ms * 1e-3)
     achieved = kflops / (kms * 1e-3) / 1e12
     kname = "k_field_far32c" if mixed_ok else "k_field_far"
-    default_workload = N == 20000 and M == 1_000_000
+    # the measured traffic holds for the profiled launch: 20,000 triangles, 1,000,000 points,
+    # one element range of 2,500 per XCD (element-range size >= 2,500)
+    from kassiopeia_amd import autotune as T
+
+    rng_used = T.get(ctx, "far_range_mixed" if mixed_ok else "far_range_fp64")
+    default_workload = N == 20000 and M == 1_000_000 and rng_used >= 2500
     out.update(field_ms_fp64=ms64, field_ms_mixed=msmx, far_kernel_ms_fp64=kms64, far_kernel_ms_mixed=kmsmx,
                pairs_per_s_fp64=pairs / ms64 * 1e3,
                pairs_per_s_mixed=pairs / msmx * 1e3, speedup_mixed_vs_fp64=ms64 / msmx,
@@ -437,13 +442,14 @@ def run_large(args):
     out = {}
     tuning = None
     if not args.no_autotune:
-        # element-range sizes of the far pass, tuned on the first 2 M points with provisional
+        # element-range sizes of the far pass, tuned on the whole point set with provisional
         # charge densities (the time per pair does not depend on their values)
         from kassiopeia_amd import autotune as T
 
         t0 = time.perf_counter()
         sig0 = np.ones(N) * 1e-9
-        tuning = {p: T.tune_field(cont, sig0, pts[:2_000_000], p, repeats=1)["best"] for p in ("mixed", "fp64")}
+        tuning = {p: T.tune_field(cont, sig0, pts, p, repeats=1, candidates=(0, 640, 2560, 10240))["best"]
+                  for p in ("mixed", "fp64")}
         log(f"autotune ({time.perf_counter() - t0:.1f} s): far range mixed {tuning['mixed']}, fp64 {tuning['fp64']}")
     X = [K.DeviceArray(ctx, pts[:, k]) for k in range(3)]
     O = [K.DeviceArray.empty(ctx, M) for _ in range(4)]
