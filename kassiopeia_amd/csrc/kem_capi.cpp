@@ -28,6 +28,7 @@ struct kem_ctx {
     int device = 0;
     hipStream_t own = nullptr;
     hipStream_t stream = nullptr;
+    kem::SideStream side;  // the near pass beside the far pass (own stream, fork/join events)
     int policy = KEM_POLICY_NUMERIC;
     double mixed_fp64_ratio = kMixedFp64RatioDefault;
     int64_t far_range_fp64 = kFarRangeDefault, far_range_mixed = kFarRangeDefault;
@@ -179,8 +180,11 @@ int kem_ctx_create(int device, kem_ctx** out)
     if (hipSetDevice(device) != hipSuccess) return KEM_E_HIP;
     kem_ctx* c = new kem_ctx();
     c->device = device;
-    if (hipStreamCreateWithFlags(&c->own, hipStreamNonBlocking) != hipSuccess) {
-        delete c;
+    if (hipStreamCreateWithFlags(&c->own, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&c->side.st, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&c->side.fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->side.join, hipEventDisableTiming) != hipSuccess) {
+        kem_ctx_destroy(c);
         return KEM_E_HIP;
     }
     c->stream = c->own;
@@ -198,6 +202,10 @@ void kem_ctx_destroy(kem_ctx* c)
     if (c->sort_ws) (void)hipFree(c->sort_ws);
     for (hipEvent_t& e : c->ev)
         if (e) (void)hipEventDestroy(e);
+    if (c->side.st) (void)hipStreamSynchronize(c->side.st);
+    if (c->side.fork) (void)hipEventDestroy(c->side.fork);
+    if (c->side.join) (void)hipEventDestroy(c->side.join);
+    if (c->side.st) (void)hipStreamDestroy(c->side.st);
     if (c->own) (void)hipStreamDestroy(c->own);
     delete c;
 }
@@ -497,10 +505,12 @@ void field_mesh(const kem_mesh* m, kem::FieldMesh& f)
     f.far6_cl = m->far6_cl;
 }
 
+// side: run the near pass on the context's side stream (not under graph capture)
 int field_eval(kem_ctx* c, const kem_mesh* m, const double* sigma, const int32_t* pset, int64_t nsets, int64_t npts,
                const double* x, const double* y, const double* z, int precision, double* phi, double* ex, double* ey,
-               double* ez)
+               double* ez, bool side = true)
 {
+    const kem::SideStream* ss = side ? &c->side : nullptr;
     kem::FieldMesh fm;
     field_mesh(m, fm);
     const bool mixed = precision == KEM_MIXED;
@@ -520,11 +530,11 @@ int field_eval(kem_ctx* c, const kem_mesh* m, const double* sigma, const int32_t
         }
         KEM_CK(c, kem::launch_field_sorted(c->stream, c->policy, fm, mixed, c->mixed_fp64_ratio, sigma, pset, nsets,
                                            x, y, z, npts, phi, ex, ey, ez, c->sort_ws, c->sort_ws_bytes, c->ws,
-                                           c->ws_doubles, range, t0, t1));
+                                           c->ws_doubles, range, t0, t1, ss));
         return KEM_OK;
     }
     KEM_CK(c, kem::launch_field(c->stream, c->policy, fm, mixed, c->mixed_fp64_ratio, sigma, pset, nsets, x, y, z,
-                                npts, phi, ex, ey, ez, c->ws, c->ws_doubles, range, t0, t1));
+                                npts, phi, ex, ey, ez, c->ws, c->ws_doubles, range, t0, t1, ss));
     return KEM_OK;
 }
 }  // namespace
@@ -574,7 +584,7 @@ int kem_field_graph_create(kem_ctx* c, const kem_mesh* m, const double* sigma, i
     hipError_t e = hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal);
     int rc = KEM_OK;
     if (e == hipSuccess) {
-        rc = field_eval(c, m, sigma, nullptr, 1, npts, x, y, z, precision, phi, ex, ey, ez);
+        rc = field_eval(c, m, sigma, nullptr, 1, npts, x, y, z, precision, phi, ex, ey, ez, false);
         hipGraph_t graph = nullptr;
         const hipError_t e2 = hipStreamEndCapture(c->stream, &graph);
         g->graph = graph;

@@ -21,6 +21,14 @@ struct FieldMesh {
     const double* far6_cl = nullptr;   // and the clusters' 6-point reach
 };
 
+// A second stream on which the culled near pass runs beside the far pass (the near pass waits
+// on latency, the far pass on the vector ALUs), with the two events that fork it from and join
+// it to the calling stream.  Default priority, created by the caller; null st: one stream.
+struct SideStream {
+    hipStream_t st = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
+};
+
 // Doubles of the ws argument of launch_field for npts points and nsets charge-density sets.
 int64_t field_workspace_doubles(int policy, const FieldMesh& m, int64_t nsets, int64_t npts);
 
@@ -34,12 +42,13 @@ int64_t field_workspace_doubles(int policy, const FieldMesh& m, int64_t nsets, i
 // elements into 8 R ranges of at most range_elems elements, R per XCD (0: no split).
 // ws: scratch of ws_doubles >= field_workspace_doubles(policy, m, nsets, npts) doubles
 // (element-range partial sums, per-set element weights).  far_begin / far_end (nullable): events recorded on
-// st around the far-field pass.
+// st around the far-field pass.  side (nullable): the culled near pass runs on side->st; the
+// sums are the same with and without it (the near pass fills its own partial slice).
 hipError_t launch_field(hipStream_t st, int policy, const FieldMesh& m, bool mixed, double fp64_ratio,
                         const double* sigma, const int32_t* pset, int64_t nsets, const double* px, const double* py,
                         const double* pz, int64_t npts, double* phi, double* ex, double* ey, double* ez, double* ws,
                         int64_t ws_doubles, int64_t range_elems, hipEvent_t far_begin = nullptr,
-                        hipEvent_t far_end = nullptr);
+                        hipEvent_t far_end = nullptr, const SideStream* side = nullptr);
 
 // Same as launch_field with the points visited in Morton order (kem_sort.hip): sort keys,
 // gather (with the set indices), evaluate, scatter back.  sort_ws: sort_workspace_bytes(npts) bytes.
@@ -48,7 +57,8 @@ hipError_t launch_field_sorted(hipStream_t st, int policy, const FieldMesh& m, b
                                const double* sigma, const int32_t* pset, int64_t nsets, const double* px,
                                const double* py, const double* pz, int64_t npts, double* phi, double* ex, double* ey,
                                double* ez, void* sort_ws, size_t sort_ws_bytes, double* ws, int64_t ws_doubles,
-                               int64_t range_elems, hipEvent_t far_begin = nullptr, hipEvent_t far_end = nullptr);
+                               int64_t range_elems, hipEvent_t far_begin = nullptr, hipEvent_t far_end = nullptr,
+                               const SideStream* side = nullptr);
 
 // *bad = 1 when a set index lies outside [0, nsets) (*bad must be 0 on entry)
 hipError_t launch_check_sets(hipStream_t st, const int32_t* pset, int64_t n, int64_t nsets, int* bad);

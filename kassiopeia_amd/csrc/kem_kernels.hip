@@ -215,7 +215,8 @@ __device__ __forceinline__ double wave_max(double v)
 // sphere around their bounding box; a cluster of KEM_CLUSTER elements (Morton order) whose
 // sphere of near distances cannot reach it is skipped as a whole.  The pairs evaluated are
 // exactly the near set (the same dist2 test as the far pass); culling only removes clusters
-// proven to contain none.  Adds to the final sums (after the far pass and any combine).
+// proven to contain none.  Writes its own sums (one slice of the partial-sum layout that
+// k_field_combine adds), so it may run beside the far pass.
 template <bool MULTI>
 __global__ __launch_bounds__(kBlock) void k_field_near_culled(const double* __restrict__ geom,
                                                               const double* __restrict__ sigma,
@@ -238,7 +239,7 @@ __global__ __launch_bounds__(kBlock) void k_field_near_culled(const double* __re
     const double hx = wave_max(x), hy = wave_max(y), hz = wave_max(z);
     const double wx = 0.5 * (lx + hx), wy = 0.5 * (ly + hy), wz = 0.5 * (lz + hz);
     const double wr = 0.5 * sqrt((hx - lx) * (hx - lx) + (hy - ly) * (hy - ly) + (hz - lz) * (hz - lz));
-    double s0 = out0[i], s1 = out1[i], s2 = out2[i], s3 = out3[i];
+    double s0 = 0., s1 = 0., s2 = 0., s3 = 0.;
     double c0 = 0., c1 = 0., c2 = 0., c3 = 0.;
     for (int64_t k = 0; k < ncl; k++) {
         const double* cl = clusters + 4 * k;
@@ -939,10 +940,41 @@ bool xcd_split(int policy, const FieldMesh& m, int64_t npts, int64_t range_elems
     return policy == 0 && m.clusters && m.order && range_elems > 0 && (int64_t)nblocks(npts) >= kXcdMinBlocks;
 }
 
+// the culled near pass writes one more slice of partial sums beside the far pass's
+bool culled_near(int policy, const FieldMesh& m) { return policy == 0 && m.clusters && m.order; }
+
 int64_t partial_doubles(int policy, const FieldMesh& m, int64_t npts)
 {
     if (xcd_split(policy, m, npts, 1)) return std::max(kPartialDoubles, kXcdPartialDoubles);
+    if (culled_near(policy, m)) return std::max(kPartialDoubles, 8 * npts);
     return kPartialDoubles;
+}
+
+// Culled near pass over points [p0, p0 + np) into slice `slice` of the partial layout q (ns
+// slices of np doubles per component).  With a side stream it runs there, forked from st
+// after the work queued so far; join_side() makes st wait for it.
+template <bool MULTI>
+void near_slice(hipStream_t st, const SideStream* side, const FieldMesh& m, const double* sg, const int32_t* pset,
+                int64_t nsets, const double* px, const double* py, const double* pz, int64_t np, double* q,
+                int64_t ns, int64_t slice)
+{
+    hipStream_t s = st;
+    if (side && side->st) {
+        (void)hipEventRecord(side->fork, st);
+        (void)hipStreamWaitEvent(side->st, side->fork, 0);
+        s = side->st;
+    }
+    const int64_t ncl = (m.n + KEM_CLUSTER - 1) / KEM_CLUSTER;
+    double* o = q + slice * np;
+    k_field_near_culled<MULTI><<<nblocks(np), kBlock, 0, s>>>(m.geom, sg, pset, nsets, m.clusters, ncl, m.order, m.n,
+                                                              px, py, pz, np, o, o + ns * np, o + 2 * ns * np,
+                                                              o + 3 * ns * np);
+    if (s != st) (void)hipEventRecord(side->join, s);
+}
+
+void join_side(hipStream_t st, const SideStream* side)
+{
+    if (side && side->st) (void)hipStreamWaitEvent(st, side->join, 0);
 }
 
 template <bool MULTI>
@@ -970,7 +1002,7 @@ template <bool MULTI>
 void field_passes(hipStream_t st, int policy, const FieldMesh& m, bool mixed, double fp64_ratio, const double* sigma,
                   const int32_t* pset, int64_t nsets, const double* px, const double* py, const double* pz,
                   int64_t npts, double* phi, double* ex, double* ey, double* ez, double* ws, int64_t ws_doubles,
-                  int64_t range_elems, hipEvent_t far_begin, hipEvent_t far_end)
+                  int64_t range_elems, hipEvent_t far_begin, hipEvent_t far_end, const SideStream* side)
 {
     const int64_t nelem = m.n;
     const unsigned gx = nblocks(npts);
@@ -991,17 +1023,19 @@ void field_passes(hipStream_t st, int policy, const FieldMesh& m, bool mixed, do
         ws += nsig;
         ws_doubles -= nsig;
     }
-    const bool culled = policy == 0 && m.clusters && m.order;
+    const bool culled = culled_near(policy, m);
+    const int64_t extra = culled ? 1 : 0;  // the near pass's slice, after the far pass's
     if (!MULTI && xcd_split(policy, m, npts, range_elems)) {
         // 8 R element ranges of at most range_elems elements, R per XCD; partial sums per
-        // range over chunks of points
-        const int64_t r = std::min<int64_t>(kXcdPartialDoubles / (4 * kXcds * kBlock),
+        // range (and of the near pass) over chunks of points
+        const int64_t r = std::min<int64_t>(kXcdPartialDoubles / (4 * kXcds * kBlock) - 1,
                                             std::max<int64_t>(1, ((nelem + kXcds - 1) / kXcds + range_elems - 1) /
                                                                      range_elems));
         const int64_t nr = kXcds * r;
+        const int64_t ns = nr + extra;
         int64_t per = (nelem + nr - 1) / nr;
         if (clustered) per = (per + KEM_CLUSTER - 1) / KEM_CLUSTER * KEM_CLUSTER;
-        const int64_t chunk = std::max<int64_t>(kBlock, kXcdPartialDoubles / (4 * nr) / kBlock * kBlock);
+        const int64_t chunk = std::max<int64_t>(kBlock, kXcdPartialDoubles / (4 * ns) / kBlock * kBlock);
         if (far_begin) (void)hipEventRecord(far_begin, st);
         if (clustered)
             k_far_weights<<<nblocks(nsets * nelem), kBlock, 0, st>>>(m.geom, m.order, sigma, nelem, nsets, wk);
@@ -1009,10 +1043,14 @@ void field_passes(hipStream_t st, int policy, const FieldMesh& m, bool mixed, do
             const int64_t np = std::min(chunk, npts - p0);
             const unsigned gc = nblocks(np);
             double* q = ws;
+            if (culled)
+                near_slice<MULTI>(st, side, m, sg, MULTI ? pset + p0 : nullptr, nsets, px + p0, py + p0, pz + p0, np,
+                                  q, ns, nr);
             far_kernel<MULTI>(st, dim3((unsigned)(gc * nr)), kind, m, fp64_ratio, wk, sg, MULTI ? pset + p0 : nullptr,
-                              nsets, per, px + p0, py + p0, pz + p0, np, q, q + nr * np, q + 2 * nr * np,
-                              q + 3 * nr * np, np, (int64_t)gc);
-            k_field_combine<<<gc, kBlock, 0, st>>>(ws, (int)nr, np, phi + p0, ex + p0, ey + p0, ez + p0);
+                              nsets, per, px + p0, py + p0, pz + p0, np, q, q + ns * np, q + 2 * ns * np,
+                              q + 3 * ns * np, np, (int64_t)gc);
+            if (culled) join_side(st, side);
+            k_field_combine<<<gc, kBlock, 0, st>>>(ws, (int)ns, np, phi + p0, ex + p0, ey + p0, ez + p0);
         }
         if (far_end) (void)hipEventRecord(far_end, st);
     } else {
@@ -1020,20 +1058,22 @@ void field_passes(hipStream_t st, int policy, const FieldMesh& m, bool mixed, do
         int64_t nsplit = (2048 + gx - 1) / gx;
         if (nsplit > (nelem + 31) / 32) nsplit = (nelem + 31) / 32;
         if (nsplit < 1) nsplit = 1;
-        while (nsplit > 1 && 4 * nsplit * npts > ws_doubles) nsplit--;
+        while (nsplit > 1 && 4 * (nsplit + extra) * npts > ws_doubles) nsplit--;
         int64_t per = (nelem + nsplit - 1) / nsplit;
         if (clustered) per = (per + KEM_CLUSTER - 1) / KEM_CLUSTER * KEM_CLUSTER;
         nsplit = (nelem + per - 1) / per;
+        const int64_t ns = nsplit + extra;
         dim3 grid(gx, (unsigned)nsplit);
         double *o0 = ex, *o1 = ey, *o2 = ez, *o3 = phi;
         int64_t stride = 0;
-        if (nsplit > 1) {
+        if (ns > 1) {
             o0 = ws;
-            o1 = ws + nsplit * npts;
-            o2 = ws + 2 * nsplit * npts;
-            o3 = ws + 3 * nsplit * npts;
+            o1 = ws + ns * npts;
+            o2 = ws + 2 * ns * npts;
+            o3 = ws + 3 * ns * npts;
             stride = npts;
         }
+        if (culled) near_slice<MULTI>(st, side, m, sg, pset, nsets, px, py, pz, npts, ws, ns, nsplit);
         if (far_begin) (void)hipEventRecord(far_begin, st);
         if (clustered)
             k_far_weights<<<nblocks(nsets * nelem), kBlock, 0, st>>>(m.geom, m.order, sigma, nelem, nsets, wk);
@@ -1049,12 +1089,8 @@ void field_passes(hipStream_t st, int policy, const FieldMesh& m, bool mixed, do
                                                                 pz, npts, policy, policy == 0 ? 1 : 0, o0, o1, o2, o3,
                                                                 stride);
         }
-        if (nsplit > 1) k_field_combine<<<gx, kBlock, 0, st>>>(ws, (int)nsplit, npts, phi, ex, ey, ez);
-    }
-    if (culled) {
-        const int64_t ncl = (nelem + KEM_CLUSTER - 1) / KEM_CLUSTER;
-        k_field_near_culled<MULTI><<<gx, kBlock, 0, st>>>(m.geom, sg, pset, nsets, m.clusters, ncl, m.order, nelem,
-                                                          px, py, pz, npts, ex, ey, ez, phi);
+        if (culled) join_side(st, side);
+        if (ns > 1) k_field_combine<<<gx, kBlock, 0, st>>>(ws, (int)ns, npts, phi, ex, ey, ez);
     }
 }
 }  // namespace
@@ -1067,16 +1103,18 @@ int64_t field_workspace_doubles(int policy, const FieldMesh& m, int64_t nsets, i
 hipError_t launch_field(hipStream_t st, int policy, const FieldMesh& m, bool mixed, double fp64_ratio,
                         const double* sigma, const int32_t* pset, int64_t nsets, const double* px, const double* py,
                         const double* pz, int64_t npts, double* phi, double* ex, double* ey, double* ez, double* ws,
-                        int64_t ws_doubles, int64_t range_elems, hipEvent_t far_begin, hipEvent_t far_end)
+                        int64_t ws_doubles, int64_t range_elems, hipEvent_t far_begin, hipEvent_t far_end,
+                        const SideStream* side)
 {
     if (npts <= 0) return hipSuccess;
     if (nsets < 1 || ws_doubles < field_workspace_doubles(policy, m, nsets, npts)) return hipErrorInvalidValue;
+    if (side && side->st && (!side->fork || !side->join)) return hipErrorInvalidValue;
     if (pset)
         field_passes<true>(st, policy, m, mixed, fp64_ratio, sigma, pset, nsets, px, py, pz, npts, phi, ex, ey, ez, ws,
-                           ws_doubles, range_elems, far_begin, far_end);
+                           ws_doubles, range_elems, far_begin, far_end, side);
     else
         field_passes<false>(st, policy, m, mixed, fp64_ratio, sigma, nullptr, 1, px, py, pz, npts, phi, ex, ey, ez,
-                            ws, ws_doubles, range_elems, far_begin, far_end);
+                            ws, ws_doubles, range_elems, far_begin, far_end, side);
     return hipGetLastError();
 }
 

@@ -143,7 +143,8 @@ hipError_t launch_field_sorted(hipStream_t st, int policy, const FieldMesh& m, b
                                const double* sigma, const int32_t* pset, int64_t nsets, const double* px,
                                const double* py, const double* pz, int64_t npts, double* phi, double* ex, double* ey,
                                double* ez, void* sort_ws, size_t sort_ws_bytes, double* ws, int64_t ws_doubles,
-                               int64_t range_elems, hipEvent_t far_begin, hipEvent_t far_end)
+                               int64_t range_elems, hipEvent_t far_begin, hipEvent_t far_end,
+                               const SideStream* side)
 {
     if (npts <= 0) return hipSuccess;
     if (npts > INT32_MAX || sort_ws_bytes < sort_workspace_bytes(npts)) return hipErrorInvalidValue;
@@ -175,7 +176,7 @@ hipError_t launch_field_sorted(hipStream_t st, int policy, const FieldMesh& m, b
     if (e != hipSuccess) return e;
     k_gather3<<<nblocks(npts), kBlock, 0, st>>>(i1, px, py, pz, pset, npts, sx, sy, sz, ss);
     e = launch_field(st, policy, m, mixed, fp64_ratio, sigma, pset ? ss : nullptr, nsets, sx, sy, sz, npts, o[0], o[1],
-                     o[2], o[3], ws, ws_doubles, range_elems, far_begin, far_end);
+                     o[2], o[3], ws, ws_doubles, range_elems, far_begin, far_end, side);
     if (e != hipSuccess) return e;
     k_scatter4<<<nblocks(npts), kBlock, 0, st>>>(i1, o[0], o[1], o[2], o[3], npts, phi, ex, ey, ez);
     return hipGetLastError();
