@@ -12,6 +12,7 @@
  *                                     Interface/ChargeDensitySolvers/Electric/src/ (one .cc each)
  *   KIntegratingElectrostaticFieldSolver  Interface/FieldSolvers/Electric/src/KIntegratingElectrostaticFieldSolver.cc:44-89
  *   KElectrostaticBoundaryField       Interface/Fields/Electric/src/KElectrostaticBoundaryField.cc:16-199
+ *   KSElectricKEMField                Kassiopeia/Fields/Source/KSElectricKEMField.cxx:44-72 (outside KEMField)
  *
  * Errors: a negative C-ABI code throws kemhip::Error (unknown integrator / solver names throw
  * kemhip::KEMSimpleException, as the reference's factory does); KEM_W_NOT_CONVERGED is
@@ -348,6 +349,8 @@ class KElectrostaticBoundaryField {
         solver().ElectricFieldAndPotential(P, E, phi);
     }
     const std::vector<double>& ChargeDensities() const { return solver().ChargeDensities(); }
+    const KSurfaceContainer& Container() const { return c_; }
+    bool MixedPrecision() const { return mixed_; }
 
   private:
     const KIntegratingElectrostaticFieldSolver& solver() const
@@ -360,6 +363,110 @@ class KElectrostaticBoundaryField {
     const KChargeDensitySolver& solver_;
     bool mixed_;
     std::unique_ptr<KIntegratingElectrostaticFieldSolver> field_;
+};
+
+// KSElectricKEMField (Kassiopeia/Fields/Source/KSElectricKEMField.cxx:44-72): Kassiopeia's
+// electric-field component over a KElectrostaticBoundaryField.  InitializeComponent() solves
+// (KElectrostaticBoundaryField::Initialize) and keeps the charge densities on the device; the
+// point and result buffers stay on the device between calls and grow to the largest batch, so
+// a tracking step that collects the positions of all its particles evaluates them in one call
+// (CalculateFieldAndPotential over a vector) instead of one launch and round trip per particle.
+// The time argument is accepted and ignored (static field).
+class KSElectricKEMField {
+  public:
+    explicit KSElectricKEMField(KElectrostaticBoundaryField& field) : field_(field) {}
+    ~KSElectricKEMField() { release(); }
+    KSElectricKEMField(const KSElectricKEMField&) = delete;
+    KSElectricKEMField& operator=(const KSElectricKEMField&) = delete;
+
+    void InitializeComponent()
+    {
+        field_.Initialize();
+        release();
+        const std::vector<double>& s = field_.ChargeDensities();
+        sigma_ = alloc((int64_t)s.size());
+        ctx().check(kem_dev_upload(ctx().get(), sigma_, s.data(), (int64_t)(sizeof(double) * s.size())));
+    }
+    void DeinitializeComponent() { release(); }
+
+    void CalculatePotential(const Position& P, const double& /*t*/, double& phi)
+    {
+        FieldVector E;
+        CalculateFieldAndPotential(P, 0., E, phi);
+    }
+    void CalculateField(const Position& P, const double& /*t*/, FieldVector& E)
+    {
+        double phi;
+        CalculateFieldAndPotential(P, 0., E, phi);
+    }
+    void CalculateFieldAndPotential(const Position& P, const double& /*t*/, FieldVector& E, double& phi)
+    {
+        std::vector<FieldVector> Ev;
+        std::vector<double> pv;
+        CalculateFieldAndPotential(std::vector<Position>{P}, 0., Ev, pv);
+        E = Ev[0];
+        phi = pv[0];
+    }
+    // all sample points of one step in one evaluation
+    void CalculateFieldAndPotential(const std::vector<Position>& P, const double& /*t*/, std::vector<FieldVector>& E,
+                                    std::vector<double>& phi)
+    {
+        if (!sigma_) throw Error(KEM_E_STATE, "KSElectricKEMField: InitializeComponent() first");
+        const int64_t n = (int64_t)P.size();
+        E.resize(P.size());
+        phi.resize(P.size());
+        if (n == 0) return;
+        reserve(n);
+        host_.resize((size_t)(4 * n));
+        for (int64_t k = 0; k < n; k++)
+            for (int c = 0; c < 3; c++) host_[(size_t)(c * n + k)] = P[(size_t)k][(size_t)c];
+        const int64_t bytes = (int64_t)sizeof(double) * n;
+        for (int c = 0; c < 3; c++)
+            ctx().check(kem_dev_upload(ctx().get(), x_[c], host_.data() + c * n, bytes));
+        ctx().check(kem_field_eval_device(ctx().get(), field_.Container().get(), sigma_, n, x_[0], x_[1], x_[2],
+                                          field_.MixedPrecision() ? KEM_MIXED : KEM_FP64, out_[3], out_[0], out_[1],
+                                          out_[2]));
+        for (int c = 0; c < 4; c++) ctx().check(kem_dev_download(ctx().get(), host_.data() + c * n, out_[c], bytes));
+        for (int64_t k = 0; k < n; k++) {
+            for (int c = 0; c < 3; c++) E[(size_t)k][(size_t)c] = host_[(size_t)(c * n + k)];
+            phi[(size_t)k] = host_[(size_t)(3 * n + k)];
+        }
+    }
+
+  private:
+    const Context& ctx() const { return field_.Container().context(); }
+    double* alloc(int64_t count)
+    {
+        void* p = nullptr;
+        ctx().check(kem_dev_alloc(ctx().get(), (int64_t)sizeof(double) * count, &p));
+        return (double*)p;
+    }
+    void reserve(int64_t n)
+    {
+        if (n <= cap_) return;
+        for (double*& p : x_) (void)kem_dev_free(ctx().get(), p), p = nullptr;
+        for (double*& p : out_) (void)kem_dev_free(ctx().get(), p), p = nullptr;
+        cap_ = 0;
+        for (double*& p : x_) p = alloc(n);
+        for (double*& p : out_) p = alloc(n);
+        cap_ = n;
+    }
+    void release()
+    {
+        if (sigma_) (void)kem_dev_free(ctx().get(), sigma_);
+        sigma_ = nullptr;
+        for (double*& p : x_)
+            if (p) (void)kem_dev_free(ctx().get(), p), p = nullptr;
+        for (double*& p : out_)
+            if (p) (void)kem_dev_free(ctx().get(), p), p = nullptr;
+        cap_ = 0;
+    }
+    KElectrostaticBoundaryField& field_;
+    double* sigma_ = nullptr;
+    double* x_[3] = {nullptr, nullptr, nullptr};
+    double* out_[4] = {nullptr, nullptr, nullptr, nullptr};  // Ex, Ey, Ez, phi
+    int64_t cap_ = 0;
+    std::vector<double> host_;
 };
 
 }  // namespace kemhip

@@ -1049,10 +1049,12 @@ void field_passes(hipStream_t st, int policy, const FieldMesh& m, bool mixed, do
             far_kernel<MULTI>(st, dim3((unsigned)(gc * nr)), kind, m, fp64_ratio, wk, sg, MULTI ? pset + p0 : nullptr,
                               nsets, per, px + p0, py + p0, pz + p0, np, q, q + ns * np, q + 2 * ns * np,
                               q + 3 * ns * np, np, (int64_t)gc);
+            // far_end right after the last far kernel: the events time the far pass (with the
+            // joins and combines of earlier chunks), not the wait for the near pass
+            if (far_end && p0 + np >= npts) (void)hipEventRecord(far_end, st);
             if (culled) join_side(st, side);
             k_field_combine<<<gc, kBlock, 0, st>>>(ws, (int)ns, np, phi + p0, ex + p0, ey + p0, ez + p0);
         }
-        if (far_end) (void)hipEventRecord(far_end, st);
     } else {
         // enough blocks to fill 256 CUs several times over; each range >= 32 elements (one cluster)
         int64_t nsplit = (2048 + gx - 1) / gx;

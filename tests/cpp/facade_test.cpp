@@ -97,6 +97,47 @@ int main()
     check(std::fabs(em.second - ep.second) < 1e-6 * std::fabs(ep.second), "mixed field == FP64 field (1e-6)",
           std::fabs(em.second - ep.second) / std::fabs(ep.second));
 
+    // Kassiopeia's field component: single-point calls and one batched call over many sample
+    // points give the field solver's answers
+    {
+        KSElectricKEMField ks(field);
+        ks.InitializeComponent();
+        std::vector<Position> pts;
+        for (int k = 0; k < 5000; k++) {
+            const double t = 0.37 * k, r = 2. + 3. * (k % 17) / 17.;
+            pts.push_back({r * std::cos(t) * std::sin(0.1 + 0.0006 * k), r * std::sin(t) * std::sin(0.1 + 0.0006 * k),
+                           r * std::cos(0.1 + 0.0006 * k)});
+        }
+        std::vector<FieldVector> Eb;
+        std::vector<double> pb;
+        ks.CalculateFieldAndPotential(pts, 0., Eb, pb);
+        double worst = 0.;
+        for (int k = 0; k < 5000; k += 499) {
+            const auto ref = field.ElectricFieldAndPotential(pts[(size_t)k]);
+            double phi1;
+            FieldVector E1;
+            ks.CalculatePotential(pts[(size_t)k], 0., phi1);
+            ks.CalculateField(pts[(size_t)k], 0., E1);
+            const double en = std::sqrt(ref.first[0] * ref.first[0] + ref.first[1] * ref.first[1] +
+                                        ref.first[2] * ref.first[2]);
+            for (int c = 0; c < 3; c++) {
+                worst = std::max(worst, std::fabs(Eb[(size_t)k][(size_t)c] - ref.first[(size_t)c]) / en);
+                worst = std::max(worst, std::fabs(E1[(size_t)c] - ref.first[(size_t)c]) / en);
+            }
+            worst = std::max(worst, std::fabs(pb[(size_t)k] - ref.second) / std::fabs(ref.second));
+            worst = std::max(worst, std::fabs(phi1 - ref.second) / std::fabs(ref.second));
+        }
+        check(worst < 1e-12, "KSElectricKEMField batch and single calls == field solver (1e-12)", worst);
+        ks.DeinitializeComponent();
+        bool threw_state = false;
+        try {
+            ks.CalculateFieldAndPotential(pts, 0., Eb, pb);
+        } catch (const Error&) {
+            threw_state = true;
+        }
+        check(threw_state, "KSElectricKEMField before InitializeComponent throws", 0.);
+    }
+
     // the same cube from rectangles, 6 x 6 per face (CubeCapacitanceTest.cc:106-116: the
     // capacitance 0.6606785 within 1e-2)
     SurfaceData r;
