@@ -47,11 +47,13 @@ extern "C" {
 #define KEM_MIXED 1 /* FP32 far-field cubature / FP32 matrix; FP64 near field, refinement, sums */
 
 /* ---- matrix storage flags for kem_matrix_assemble */
-/* shape types and boundary types of kem_mesh_create (KTriangle / KRectangle / KLineSegment;
-   KDirichletBoundary / KNeumannBoundary) */
+/* shape types and boundary types of kem_mesh_create (KTriangle / KRectangle / KLineSegment /
+   KConicSection / KRing; KDirichletBoundary / KNeumannBoundary) */
 #define KEM_SHAPE_TRIANGLE 0
 #define KEM_SHAPE_RECTANGLE 1
 #define KEM_SHAPE_LINESEGMENT 2
+#define KEM_SHAPE_CONICSECTION 3 /* [P0x,P0y,P0z,P1x,P1y,P1z]: radius = x, axial position = z */
+#define KEM_SHAPE_RING 4         /* [Px,Py,Pz]: radius = x > 0, axial position = z */
 #define KEM_BC_DIRICHLET 0
 #define KEM_BC_NEUMANN 1
 

@@ -119,6 +119,25 @@ struct SurfaceData {
         value.push_back(potential);
         flux.push_back(0.);
     }
+    // KConicSection::SetValues(r0, z0, r1, z1) (Surfaces/src/KConicSection.cc:13-22): the surface
+    // of revolution about z of the segment (r0, z0)-(r1, z1)
+    void AddConicSection(double r0, double z0, double r1, double z1, double potential)
+    {
+        shape.push_back(KEM_SHAPE_CONICSECTION);
+        data.insert(data.end(), {r0, 0., z0, r1, 0., z1, 0., 0., 0., 0., 0.});
+        bc.push_back(KEM_BC_DIRICHLET);
+        value.push_back(potential);
+        flux.push_back(0.);
+    }
+    // KRing::SetValues(r, z) (Surfaces/src/KRing.cc:14-18): a charged circle of radius r at height z
+    void AddRing(double r, double z, double potential)
+    {
+        shape.push_back(KEM_SHAPE_RING);
+        data.insert(data.end(), {r, 0., z, 0., 0., 0., 0., 0., 0., 0., 0.});
+        bc.push_back(KEM_BC_DIRICHLET);
+        value.push_back(potential);
+        flux.push_back(0.);
+    }
     // turns element i into a Neumann (dielectric interface) element with the given flux
     // (KNeumannBoundary's normal boundary flux, the permittivity ratio of the interface)
     void SetNeumann(size_t i, double normal_flux)

@@ -57,6 +57,7 @@ struct kem_mesh {
     double* recs = nullptr;      // raw shape records (analytic / reference policies)
     double* rule = nullptr;      // triangle rule barycentrics and weights (adjoint)
     bool all_triangles = true;
+    bool has_axisym = false;  // conic sections or rings
     double* clusters = nullptr;  // near-pass element clusters (centre, radius)
     int32_t* order = nullptr;    // elements in cluster order
     float* far6 = nullptr;       // mixed far field: 6-point records in cluster order
@@ -399,7 +400,10 @@ int kem_mesh_create(kem_ctx* c, int64_t n, const int32_t* shape, const double* d
         for (int k = 0; k < 11; k++) recs[(size_t)i * KEM_REC + k] = data[11 * i + k];
     std::vector<double> rule(4 * KEM_NODE_SLOTS, 0.);
     kem::triangle_rule_table(rule.data());
-    for (int64_t i = 0; i < n; i++) m->all_triangles = m->all_triangles && shape[i] == KEM_TRIANGLE;
+    for (int64_t i = 0; i < n; i++) {
+        m->all_triangles = m->all_triangles && shape[i] == KEM_TRIANGLE;
+        m->has_axisym = m->has_axisym || shape[i] == KEM_CONICSECTION || shape[i] == KEM_RING;
+    }
     hipError_t e = dalloc(&m->geom, (int64_t)g.size());
     if (e == hipSuccess) e = dalloc(&m->rule, (int64_t)rule.size());
     if (e == hipSuccess) e = hipMemcpy(m->rule, rule.data(), sizeof(double) * rule.size(), hipMemcpyHostToDevice);
@@ -480,7 +484,7 @@ int kem_element_eval(kem_ctx* c, const kem_mesh* m, int64_t np, const int64_t* e
     if (e == hipSuccess) e = dalloc(&dO, 4 * np);
     if (e == hipSuccess) e = hipMemcpyAsync(de, elem, sizeof(int64_t) * np, hipMemcpyHostToDevice, c->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(dP, P, sizeof(double) * 3 * np, hipMemcpyHostToDevice, c->stream);
-    if (e == hipSuccess) e = kem::launch_pairs(c->stream, c->policy, m->geom, m->nodes, m->recs, np, de, dP, dO);
+    if (e == hipSuccess) e = kem::launch_pairs(c->stream, c->policy, m->geom, m->nodes, m->recs, np, de, dP, dO, m->has_axisym);
     if (e == hipSuccess) e = hipMemcpyAsync(out, dO, sizeof(double) * 4 * np, hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     dfree(de);
@@ -503,6 +507,7 @@ void field_mesh(const kem_mesh* m, kem::FieldMesh& f)
     f.order = m->order;
     f.far6 = m->far6;
     f.far6_cl = m->far6_cl;
+    f.axisym = m->has_axisym;
 }
 
 // side: run the near pass on the context's side stream (not under graph capture)
@@ -719,7 +724,7 @@ int kem_matrix_assemble(kem_ctx* c, const kem_mesh* m, int storage, kem_matrix**
     if (e == hipSuccess && (storage & KEM_STORE_FP32)) e = dalloc(&A->A32, nn);
     if (e == hipSuccess)
         e = kem::launch_assemble(c->stream, c->policy, m->has_neumann, m->geom, m->nodes, m->recs, m->targets, m->n, A->A64,
-                                 A->A32);
+                                 A->A32, m->has_axisym);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) {
         kem_matrix_destroy(A);

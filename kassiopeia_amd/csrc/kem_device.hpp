@@ -40,6 +40,14 @@ struct F4 {
     double x, y, z, p;  // Ex, Ey, Ez, phi
 };
 
+}  // namespace dev
+}  // namespace kem
+
+#include "kem_axisym.hpp"
+
+namespace kem {
+namespace dev {
+
 __device__ __forceinline__ double rnorm(double x, double y, double z) { return sqrt(x * x + y * y + z * z); }
 
 // 1/sqrt(a) for a > 0: the hardware estimate (v_rsq_f64, |rel. error| <= 5.2e-8 measured on
@@ -397,6 +405,15 @@ __device__ __forceinline__ F4 element_near(const double* __restrict__ g, int typ
     if (type == KEM_TRIANGLE) return closed_form<3, true>(g, px, py, pz);
     if (type == KEM_RECTANGLE) return closed_form<4, true>(g, px, py, pz);
     return line_analytic(g, px, py, pz);
+}
+
+// Near pass of one pair when the mesh holds conic sections or rings (kernels instantiated for
+// such meshes only, so the triangle kernels keep their register budget).
+__device__ __forceinline__ F4 element_near_any(const double* __restrict__ g, int type, double px, double py,
+                                               double pz)
+{
+    if (type >= KEM_CONICSECTION) return axisym_eval<true>(g, type, px, py, pz);
+    return element_near(g, type, px, py, pz);
 }
 
 // ---------------------------------------------------------------- mixed precision (FP32 far field)

@@ -19,6 +19,7 @@ struct FieldMesh {
     const int32_t* order = nullptr;    // elements in cluster order
     const float* far6 = nullptr;       // 6-point far records in cluster order (kem::build_far6)
     const double* far6_cl = nullptr;   // and the clusters' 6-point reach
+    bool axisym = false;               // holds conic sections or rings (kernels with their integrators)
 };
 
 // A second stream on which the culled near pass runs beside the far pass (the near pass waits
@@ -68,10 +69,11 @@ hipError_t launch_census(hipStream_t st, const double* geom, int64_t nelem, cons
                          const double* pz, int64_t npts, int mixed, double fp64_ratio, unsigned long long* counts);
 
 hipError_t launch_pairs(hipStream_t st, int policy, const double* geom, const double* nodes, const double* recs,
-                        int64_t npairs, const int64_t* elem, const double* P, double* out);
+                        int64_t npairs, const int64_t* elem, const double* P, double* out, bool axisym = false);
 
 hipError_t launch_assemble(hipStream_t st, int policy, bool field, const double* geom, const double* nodes,
-                           const double* recs, const double* targets, int64_t n, double* A64, float* A32);
+                           const double* recs, const double* targets, int64_t n, double* A64, float* A32,
+                           bool axisym = false);
 
 // Launch shapes of the calling thread, set from the context at every C-ABI entry point:
 // target block counts of the split-K matrix-vector product and of the FP32 / FP64 block product.

@@ -150,9 +150,11 @@ __global__ __launch_bounds__(kBlock) void k_field_far32(const double* __restrict
 
 // Near pass over all elements of a range: closed-form pairs added to the far pass's sums
 // (numeric policy, accumulate = 1), or every pair from zero (rwg policy; PC = 1: the analytic
-// and reference policies, from the raw shape records).
+// and reference policies, from the raw shape records; PC = 2: meshes with conic sections or
+// rings, any policy).
 template <int PC, bool MULTI>
 __global__ __launch_bounds__(kBlock) void k_field_near(const double* __restrict__ geom,
+                                                       const double* __restrict__ nodes,
                                                        const double* __restrict__ recs,
                                                        const double* __restrict__ sigma,
                                                        const int32_t* __restrict__ pset, int64_t nsets, int64_t nelem,
@@ -183,8 +185,10 @@ __global__ __launch_bounds__(kBlock) void k_field_near(const double* __restrict_
     for (int64_t j = j0; j < j1; j++) {
         const double* g = geom + j * KG_STRIDE;
         if (PC == 1 || dev::is_near2(g, dev::dist2(g, x, y, z), policy)) {
-            const F4 v = PC == 1 ? dev::element_policy(g, recs + j * KEM_REC, policy, x, y, z)
-                                 : dev::element_near(g, (int)g[KG_TYPE], x, y, z);
+            const F4 v = PC == 1   ? dev::element_policy(g, recs + j * KEM_REC, policy, x, y, z)
+                         : PC == 2 ? dev::element_any<true>(g, nodes + j * KEM_NODE_STRIDE, recs + j * KEM_REC,
+                                                            policy, x, y, z)
+                                   : dev::element_near(g, (int)g[KG_TYPE], x, y, z);
             const double sj = sg[MULTI ? j * nsets : j];
             dev::kahan_add(v.x, sj, s0, c0);
             dev::kahan_add(v.y, sj, s1, c1);
@@ -217,7 +221,7 @@ __device__ __forceinline__ double wave_max(double v)
 // exactly the near set (the same dist2 test as the far pass); culling only removes clusters
 // proven to contain none.  Writes its own sums (one slice of the partial-sum layout that
 // k_field_combine adds), so it may run beside the far pass.
-template <bool MULTI>
+template <bool MULTI, bool AXI>
 __global__ __launch_bounds__(kBlock) void k_field_near_culled(const double* __restrict__ geom,
                                                               const double* __restrict__ sigma,
                                                               const int32_t* __restrict__ pset, int64_t nsets,
@@ -252,7 +256,8 @@ __global__ __launch_bounds__(kBlock) void k_field_near_culled(const double* __re
             const int64_t j = order[e];
             const double* g = geom + j * KG_STRIDE;
             if (dev::is_near2(g, dev::dist2(g, x, y, z), 0)) {
-                const F4 v = dev::element_near(g, (int)g[KG_TYPE], x, y, z);
+                const F4 v = AXI ? dev::element_near_any(g, (int)g[KG_TYPE], x, y, z)
+                                 : dev::element_near(g, (int)g[KG_TYPE], x, y, z);
                 const double sj = sg[MULTI ? j * nsets : j];
                 dev::kahan_add(v.x, sj, s0, c0);
                 dev::kahan_add(v.y, sj, s1, c1);
@@ -447,10 +452,12 @@ __global__ __launch_bounds__(kBlock) void k_pairs(const double* __restrict__ geo
     const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (k >= npairs) return;
     const int64_t j = elem[k];
-    const F4 v = PC == 1 ? dev::element_policy(geom + j * KG_STRIDE, recs + j * KEM_REC, policy, P[3 * k],
-                                               P[3 * k + 1], P[3 * k + 2])
-                         : dev::element_eval<true>(geom + j * KG_STRIDE, nodes + j * KEM_NODE_STRIDE, policy,
-                                                   P[3 * k], P[3 * k + 1], P[3 * k + 2]);
+    const F4 v = PC == 1   ? dev::element_policy(geom + j * KG_STRIDE, recs + j * KEM_REC, policy, P[3 * k],
+                                                 P[3 * k + 1], P[3 * k + 2])
+                 : PC == 2 ? dev::element_any<true>(geom + j * KG_STRIDE, nodes + j * KEM_NODE_STRIDE,
+                                                    recs + j * KEM_REC, policy, P[3 * k], P[3 * k + 1], P[3 * k + 2])
+                           : dev::element_eval<true>(geom + j * KG_STRIDE, nodes + j * KEM_NODE_STRIDE, policy,
+                                                     P[3 * k], P[3 * k + 1], P[3 * k + 2]);
     out[4 * k + 0] = v.x;
     out[4 * k + 1] = v.y;
     out[4 * k + 2] = v.z;
@@ -482,9 +489,11 @@ __global__ __launch_bounds__(kBlock) void k_assemble(const double* __restrict__ 
     int64_t j1 = j0 + per_split;
     if (j1 > n) j1 = n;
     for (int64_t j = j0; j < j1; j++) {
-        const F4 v = PC == 1 ? dev::element_policy(geom + j * KG_STRIDE, recs + j * KEM_REC, policy, cx, cy, cz)
-                             : dev::element_eval<FIELD>(geom + j * KG_STRIDE, nodes + j * KEM_NODE_STRIDE, policy,
-                                                        cx, cy, cz);
+        const F4 v = PC == 1   ? dev::element_policy(geom + j * KG_STRIDE, recs + j * KEM_REC, policy, cx, cy, cz)
+                     : PC == 2 ? dev::element_any<FIELD>(geom + j * KG_STRIDE, nodes + j * KEM_NODE_STRIDE,
+                                                         recs + j * KEM_REC, policy, cx, cy, cz)
+                               : dev::element_eval<FIELD>(geom + j * KG_STRIDE, nodes + j * KEM_NODE_STRIDE, policy,
+                                                          cx, cy, cz);
         double a;
         if (!FIELD || dirichlet) {
             a = v.p;
@@ -966,9 +975,14 @@ void near_slice(hipStream_t st, const SideStream* side, const FieldMesh& m, cons
     }
     const int64_t ncl = (m.n + KEM_CLUSTER - 1) / KEM_CLUSTER;
     double* o = q + slice * np;
-    k_field_near_culled<MULTI><<<nblocks(np), kBlock, 0, s>>>(m.geom, sg, pset, nsets, m.clusters, ncl, m.order, m.n,
-                                                              px, py, pz, np, o, o + ns * np, o + 2 * ns * np,
-                                                              o + 3 * ns * np);
+    if (m.axisym)
+        k_field_near_culled<MULTI, true><<<nblocks(np), kBlock, 0, s>>>(m.geom, sg, pset, nsets, m.clusters, ncl,
+                                                                        m.order, m.n, px, py, pz, np, o,
+                                                                        o + ns * np, o + 2 * ns * np, o + 3 * ns * np);
+    else
+        k_field_near_culled<MULTI, false><<<nblocks(np), kBlock, 0, s>>>(m.geom, sg, pset, nsets, m.clusters, ncl,
+                                                                         m.order, m.n, px, py, pz, np, o,
+                                                                         o + ns * np, o + 2 * ns * np, o + 3 * ns * np);
     if (s != st) (void)hipEventRecord(side->join, s);
 }
 
@@ -1083,13 +1097,17 @@ void field_passes(hipStream_t st, int policy, const FieldMesh& m, bool mixed, do
                                            o0, o1, o2, o3, stride, 0);
         if (far_end) (void)hipEventRecord(far_end, st);
         if (!culled) {
-            if (policy >= 2)
-                k_field_near<1, MULTI><<<grid, kBlock, 0, st>>>(m.geom, m.recs, sg, pset, nsets, nelem, per, px, py,
-                                                                pz, npts, policy, 0, o0, o1, o2, o3, stride);
+            if (m.axisym)
+                k_field_near<2, MULTI><<<grid, kBlock, 0, st>>>(m.geom, m.nodes, m.recs, sg, pset, nsets, nelem, per,
+                                                                px, py, pz, npts, policy, policy == 0 ? 1 : 0, o0, o1,
+                                                                o2, o3, stride);
+            else if (policy >= 2)
+                k_field_near<1, MULTI><<<grid, kBlock, 0, st>>>(m.geom, m.nodes, m.recs, sg, pset, nsets, nelem, per,
+                                                                px, py, pz, npts, policy, 0, o0, o1, o2, o3, stride);
             else
-                k_field_near<0, MULTI><<<grid, kBlock, 0, st>>>(m.geom, m.recs, sg, pset, nsets, nelem, per, px, py,
-                                                                pz, npts, policy, policy == 0 ? 1 : 0, o0, o1, o2, o3,
-                                                                stride);
+                k_field_near<0, MULTI><<<grid, kBlock, 0, st>>>(m.geom, m.nodes, m.recs, sg, pset, nsets, nelem, per,
+                                                                px, py, pz, npts, policy, policy == 0 ? 1 : 0, o0, o1,
+                                                                o2, o3, stride);
         }
         if (culled) join_side(st, side);
         if (ns > 1) k_field_combine<<<gx, kBlock, 0, st>>>(ws, (int)ns, npts, phi, ex, ey, ez);
@@ -1138,10 +1156,12 @@ hipError_t launch_census(hipStream_t st, const double* geom, int64_t nelem, cons
 }
 
 hipError_t launch_pairs(hipStream_t st, int policy, const double* geom, const double* nodes, const double* recs,
-                        int64_t npairs, const int64_t* elem, const double* P, double* out)
+                        int64_t npairs, const int64_t* elem, const double* P, double* out, bool axisym)
 {
     if (npairs <= 0) return hipSuccess;
-    if (policy >= 2)
+    if (axisym)
+        k_pairs<2><<<nblocks(npairs), kBlock, 0, st>>>(geom, nodes, recs, policy, npairs, elem, P, out);
+    else if (policy >= 2)
         k_pairs<1><<<nblocks(npairs), kBlock, 0, st>>>(geom, nodes, recs, policy, npairs, elem, P, out);
     else
         k_pairs<0><<<nblocks(npairs), kBlock, 0, st>>>(geom, nodes, recs, policy, npairs, elem, P, out);
@@ -1149,7 +1169,8 @@ hipError_t launch_pairs(hipStream_t st, int policy, const double* geom, const do
 }
 
 hipError_t launch_assemble(hipStream_t st, int policy, bool field, const double* geom, const double* nodes,
-                           const double* recs, const double* targets, int64_t n, double* A64, float* A32)
+                           const double* recs, const double* targets, int64_t n, double* A64, float* A32,
+                           bool axisym)
 {
     if (n <= 0) return hipSuccess;
     const unsigned gx = nblocks(n);
@@ -1158,7 +1179,12 @@ hipError_t launch_assemble(hipStream_t st, int policy, bool field, const double*
     const int64_t per = (n + nsplit - 1) / nsplit;
     nsplit = (n + per - 1) / per;
     dim3 grid(gx, (unsigned)nsplit);
-    if (policy >= 2) {
+    if (axisym) {
+        if (field)
+            k_assemble<true, 2><<<grid, kBlock, 0, st>>>(geom, nodes, recs, targets, policy, n, per, A64, A32);
+        else
+            k_assemble<false, 2><<<grid, kBlock, 0, st>>>(geom, nodes, recs, targets, policy, n, per, A64, A32);
+    } else if (policy >= 2) {
         if (field)
             k_assemble<true, 1><<<grid, kBlock, 0, st>>>(geom, nodes, recs, targets, policy, n, per, A64, A32);
         else

@@ -19,7 +19,9 @@
 #ifndef KEM_LAYOUT_H
 #define KEM_LAYOUT_H
 
-enum kem_shape_type { KEM_TRIANGLE = 0, KEM_RECTANGLE = 1, KEM_LINESEGMENT = 2 };
+/* conic sections (surface of revolution of P0-P1 about z) and rings (a charged circle) are the
+   reference's axisymmetric shapes, KConicSection.hh / KRing.hh */
+enum kem_shape_type { KEM_TRIANGLE = 0, KEM_RECTANGLE = 1, KEM_LINESEGMENT = 2, KEM_CONICSECTION = 3, KEM_RING = 4 };
 
 enum kem_geom_field {
     KG_CX = 0, /* dispatch centre (triangle centroid, rectangle centre, segment midpoint) */
@@ -35,9 +37,9 @@ enum kem_geom_field {
     KG_HY = 10,
     KG_HZ = 11,
     KG_NE = 12, /* number of edges (3, 4) */
-    KG_A = 13,  /* side A (rectangle in-plane test); segment: length L */
+    KG_A = 13,  /* side A (rectangle in-plane test); segment, conic section: length L */
     KG_B = 14,  /* side B; segment: diameter */
-    KG_V = 16,  /* vertices V0..V3 (segment: P0 at V0, P1 at V1) */
+    KG_V = 16,  /* vertices V0..V3 (segment, conic section: P0 at V0, P1 at V1; ring: P at V0) */
     KG_U = 28,  /* unit edge directions u_e (segment: unit axis at u_0) */
     KG_MID = 40, /* edge midpoints */
     KG_M = 52,   /* in-plane outward edge normals m_e = u_e x n */

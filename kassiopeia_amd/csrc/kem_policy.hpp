@@ -565,6 +565,18 @@ __device__ __noinline__ F4 element_policy(const double* __restrict__ g, const do
     return pol::line_256(d, px, py, pz);
 }
 
+// One pair of any shape under any policy, the axisymmetric shapes included (their analytic
+// integrators serve every policy): the kernels for meshes with conic sections or rings.
+template <bool FIELD>
+__device__ __forceinline__ F4 element_any(const double* __restrict__ g, const double* __restrict__ nd,
+                                          const double* __restrict__ d, int policy, double px, double py, double pz)
+{
+    const int type = (int)g[KG_TYPE];
+    if (type >= KEM_CONICSECTION) return axisym_eval<FIELD>(g, type, px, py, pz);
+    if (policy >= 2) return element_policy(g, d, policy, px, py, pz);
+    return element_eval<FIELD>(g, nd, policy, px, py, pz);
+}
+
 }  // namespace dev
 }  // namespace kem
 

@@ -161,6 +161,31 @@ void line_nodes(const double* d, double* nodes)
     rule(KEM_L16, 8, kGL16x, kGL16w);
 }
 
+// KConicSection::Normal (Surfaces/src/KConicSection.cc:59-104): in the x-z plane, -z / +z for a
+// flat section running out / in, +x / -x for a cylinder running up / down, else the unit
+// normal of the generating line; KRing::Normal (KRing.cc:40-43) is +x
+void axisym_normal(int type, const double* d, double n[3])
+{
+    n[1] = 0.;
+    if (type == KEM_RING) {
+        n[0] = 1.;
+        n[2] = 0.;
+        return;
+    }
+    const double r0 = d[0], z0 = d[2], r1 = d[3], z1 = d[5];
+    if (std::fabs(z0 - z1) < 1.e-10) {
+        n[0] = 0.;
+        n[2] = r0 < r1 ? -1. : 1.;
+    } else if (std::fabs(r0 - r1) < 1.e-10) {
+        n[2] = 0.;
+        n[0] = z0 < z1 ? 1. : -1.;
+    } else {
+        const double m = (z0 - z1) / (r1 - r0);
+        n[0] = m / std::sqrt(1. + m * m);
+        n[2] = 1. / std::sqrt(1. + m * m);
+    }
+}
+
 bool finite_all(const double* d, int n)
 {
     for (int k = 0; k < n; k++)
@@ -261,6 +286,34 @@ int prep_element(int type, const double* d, double* g, double* nodes)
         g[KG_B] = d[6];
         set_thresholds(type, g);
         line_nodes(d, nodes);
+        return 0;
+    }
+    if (type == KEM_CONICSECTION || type == KEM_RING) {
+        // axisymmetric shapes: evaluated by the analytic ring / conic-section integrators at every
+        // distance (the near pass), so every threshold is infinite; radius = x, axial position = z
+        const int nd = type == KEM_RING ? 3 : 6;
+        if (!finite_all(d, nd)) return -1;
+        double c[3], n[3];
+        for (int k = 0; k < nd; k++) g[KG_V + k] = d[k];
+        if (type == KEM_RING) {
+            if (!(d[0] > 0.)) return -1;  // the ring prefactor divides by the radius
+            for (int k = 0; k < 3; k++) c[k] = d[k];
+            g[KG_INVSIZE] = 1. / d[0];
+        } else {
+            const double L = std::sqrt(sq(d[0] - d[3]) + sq(d[1] - d[4]) + sq(d[2] - d[5]));
+            if (!(L > 0.)) return -1;
+            for (int k = 0; k < 3; k++) c[k] = (d[k] + d[3 + k]) * .5;
+            g[KG_A] = L;
+            g[KG_INVSIZE] = 1. / L;
+        }
+        axisym_normal(type, d, n);
+        for (int k = 0; k < 3; k++) {
+            g[KG_CX + k] = c[k];
+            g[KG_HX + k] = c[k];
+            g[KG_NX + k] = n[k];
+        }
+        g[KG_PREF] = 1.;
+        for (int k = KG_NEAR2; k <= KG_MIX6_2; k++) g[k] = INFINITY;
         return 0;
     }
     if (type != KEM_TRIANGLE && type != KEM_RECTANGLE) return -1;
@@ -396,6 +449,18 @@ void prep_nodes32(const double* g, const double* nodes, float* nodes32)
 
 void prep_target(int type, const double* d, int bc, double flux, double* t)
 {
+    if (type == KEM_CONICSECTION || type == KEM_RING) {
+        double n[3];
+        axisym_normal(type, d, n);
+        for (int k = 0; k < 3; k++) {
+            // KConicSection::Centroid (P0 + P1)/2, KRing::Centroid P (KConicSection.hh, KRing.hh)
+            t[KT_CX + k] = type == KEM_RING ? d[k] : (d[k] + d[3 + k]) * .5;
+            t[KT_NX + k] = n[k];
+        }
+        t[KT_BC] = bc;
+        t[KT_FLUX] = flux;
+        return;
+    }
     for (int k = 0; k < 3; k++) {
         if (type == KEM_TRIANGLE)
             t[KT_CX + k] = d[2 + k] + (d[0] * d[5 + k] + d[1] * d[8 + k]) / 3.;  // KTriangle.hh:31-34

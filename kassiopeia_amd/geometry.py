@@ -21,7 +21,7 @@ import dataclasses
 
 import numpy as np
 
-TRIANGLE, RECTANGLE, LINESEGMENT = 0, 1, 2
+TRIANGLE, RECTANGLE, LINESEGMENT, CONICSECTION, RING = 0, 1, 2, 3, 4
 DIRICHLET, NEUMANN = 0, 1
 REC = 11
 
@@ -53,6 +53,11 @@ class Mesh:
         if ln.any():
             d = r[ln, 0:3] - r[ln, 3:6]
             a[ln] = np.pi * r[ln, 6] * np.sqrt(d[:, 0] ** 2 + d[:, 1] ** 2 + d[:, 2] ** 2)
+        # KConicSection::Area (KConicSection.cc:24-28), KRing::Area (KRing.cc:20-23)
+        cs = self.shapes == CONICSECTION
+        a[cs] = np.pi * (r[cs, 0] + r[cs, 3]) * np.sqrt((r[cs, 0] - r[cs, 3]) ** 2 + (r[cs, 2] - r[cs, 5]) ** 2)
+        rg = self.shapes == RING
+        a[rg] = 2.0 * np.pi * r[rg, 0]
         return a
 
     def centroids(self) -> np.ndarray:
@@ -62,8 +67,10 @@ class Mesh:
         c[t] = r[t, 2:5] + (r[t, 0:1] * r[t, 5:8] + r[t, 1:2] * r[t, 8:11]) / 3.0
         q = self.shapes == RECTANGLE
         c[q] = r[q, 2:5] + r[q, 0:1] * r[q, 5:8] * 0.5 + r[q, 1:2] * r[q, 8:11] * 0.5
-        ln = self.shapes == LINESEGMENT
+        ln = (self.shapes == LINESEGMENT) | (self.shapes == CONICSECTION)
         c[ln] = (r[ln, 0:3] + r[ln, 3:6]) * 0.5
+        rg = self.shapes == RING
+        c[rg] = r[rg, 0:3]
         return c
 
     def subset(self, idx) -> "Mesh":
@@ -109,6 +116,44 @@ def line_records(p0, p1, diameter) -> np.ndarray:
     rec[:, 3:6] = p1
     rec[:, 6] = diameter
     return rec
+
+
+def conic_records(r0, z0, r1, z1) -> np.ndarray:
+    """KConicSection::SetValues(r0, z0, r1, z1) (KConicSection.cc:13-22): P0 = (r0, 0, z0),
+    P1 = (r1, 0, z1); the section is the surface of revolution of P0-P1 about the z axis."""
+    r0, z0, r1, z1 = (np.atleast_1d(np.asarray(v, dtype=np.float64)) for v in (r0, z0, r1, z1))
+    rec = np.zeros((r0.shape[0], REC))
+    rec[:, 0], rec[:, 2], rec[:, 3], rec[:, 5] = r0, z0, r1, z1
+    return rec
+
+
+def ring_records(r, z) -> np.ndarray:
+    """KRing::SetValues(r, z) (KRing.cc:14-18): P = (r, 0, z), a charged circle about the z axis."""
+    r, z = (np.atleast_1d(np.asarray(v, dtype=np.float64)) for v in (r, z))
+    rec = np.zeros((r.shape[0], REC))
+    rec[:, 0], rec[:, 2] = r, z
+    return rec
+
+
+def disk_mesh(scale: int = 128, power: float = 1.5, volts: float = 1.0) -> Mesh:
+    """Unit disk of `scale` conic sections from r = 1 inwards, graded towards the rim and the
+    centre as the reference's disk-capacitance test builds it (UnitTest/KEMField/Fields/src/
+    DiskCapacitanceTest.cc:63-86, 121-145)."""
+    n = 2 * scale
+    seg = np.zeros(n)
+    mid = 1.0
+    for i in range(n // 2):
+        a, b = (i / (n / 2.0)) ** power, ((i + 1) / (n / 2.0)) ** power
+        seg[i] = seg[n - (i + 1)] = mid * (b - a)
+    r = [1.0]
+    for i in range(scale):
+        r.append(r[-1] - seg[i])
+    r = np.array(r)
+    # the running subtraction ends a few ulps from 0 (-2.8e-16 for 128 sections); a negative
+    # radius makes the radial-field integrand of the reference's ring integrator the square root
+    # of a negative number, so the centre is closed at r = 0
+    r[-1] = 0.0
+    return _mesh(conic_records(r[:-1], 0.0, r[1:], 0.0), CONICSECTION, volts, 0)
 
 
 def _mesh(recs, shape, volts, group, bc_type=None, flux=None) -> Mesh:

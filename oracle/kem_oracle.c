@@ -57,9 +57,15 @@ void ko_set_from_points(const double p0[3], const double p1[3], const double p2[
 }
 
 /* unit normal N3 = unit(N1 x N2): KTriangle.cc:16,28 */
+double ko_axisym_area(int shape, const double* d);
+void ko_axisym_normal(int shape, const double* d, double n[3]);
+
 void ko_normal(int shape, const double* d, double n[3])
 {
-    (void)shape;
+    if (shape == KO_CONICSECTION || shape == KO_RING) {
+        ko_axisym_normal(shape, d, n);
+        return;
+    }
     double c[3];
     cross3(d + 5, d + 8, c);
     double m = norm3(c);
@@ -69,6 +75,7 @@ void ko_normal(int shape, const double* d, double n[3])
 /* KTriangle.hh:27-30, KRectangle.hh:27-30, KLineSegment.hh:25-28 */
 double ko_area(int shape, const double* d)
 {
+    if (shape == KO_CONICSECTION || shape == KO_RING) return ko_axisym_area(shape, d);
     if (shape == KO_TRIANGLE) {
         double c[3];
         cross3(d + 5, d + 8, c);
@@ -82,7 +89,11 @@ double ko_area(int shape, const double* d)
 /* KTriangle.hh:31-34, KRectangle.hh:31-34, KLineSegment.hh:29-32 */
 void ko_centroid(int shape, const double* d, double c[3])
 {
-    for (int k = 0; k < 3; k++) {
+    if (shape == KO_RING) { /* KRing.hh Centroid: P */
+        for (int k = 0; k < 3; k++) c[k] = d[k];
+        return;
+    }
+    for (int k = 0; k < 3; k++) { /* conic section: (P0 + P1)/2, KConicSection.hh */
         if (shape == KO_TRIANGLE)
             c[k] = d[2 + k] + (d[0] * d[5 + k] + d[1] * d[8 + k]) / 3.;
         else if (shape == KO_RECTANGLE)
@@ -1253,12 +1264,266 @@ void ko_line_256(const double* d, const double P[3], double out[4])
     out[3] = prefac * sp;
 }
 
+/* ------------------------------------------------------ axisymmetric elements */
+/* Complete elliptic integrals from Carlson's symmetric forms (B. C. Carlson, Numer. Algorithms
+   10 (1995) 13-26, algorithms 1 and 4: duplication to a relative spread below TOL, then the
+   fifth-order series):  K(k) = R_F(0, 1-k^2, 1),  E(k) = K - (k^2/3) R_D(0, 1-k^2, 1),
+   (E - K)/k^2 = -R_D(0, 1-k^2, 1)/3.  This is an independent method: the reference evaluates
+   Cody's Chebyshev fits (Math/src/KEllipticIntegrals.cc:14-99) and a power series
+   (:101-125); the values agree to rounding except where the reference regularises the
+   logarithm of K, log(eta + 1e-19) with eta = 1 - k^2 (:54), which ko_ell_K reproduces. */
+static double carlson_rf(double x, double y, double z)
+{
+    const double a0 = (x + y + z) / 3.;
+    double q = fmax(fabs(a0 - x), fmax(fabs(a0 - y), fabs(a0 - z))) * 400.; /* (3 * 1e-16)^(-1/6) ~ 389 */
+    double a = a0, f = 1.;
+    while (f * q >= fabs(a)) {
+        const double sx = sqrt(x), sy = sqrt(y), sz = sqrt(z);
+        const double lam = sx * sy + sx * sz + sy * sz;
+        x = .25 * (x + lam);
+        y = .25 * (y + lam);
+        z = .25 * (z + lam);
+        a = .25 * (a + lam);
+        f *= .25;
+    }
+    const double X = (a - x) / a, Y = (a - y) / a, Z = -(X + Y);
+    (void)z;
+    const double e2 = X * Y - Z * Z, e3 = X * Y * Z;
+    return (1. - e2 / 10. + e3 / 14. + e2 * e2 / 24. - 3. * e2 * e3 / 44.) / sqrt(a);
+}
+
+static double carlson_rd(double x, double y, double z)
+{
+    const double a0 = (x + y + 3. * z) / 5.;
+    double q = fmax(fabs(a0 - x), fmax(fabs(a0 - y), fabs(a0 - z))) * 600.; /* (1e-16 / 4)^(-1/6) ~ 541 */
+    double a = a0, f = 1., s = 0.;
+    while (f * q >= fabs(a)) {
+        const double sx = sqrt(x), sy = sqrt(y), sz = sqrt(z);
+        const double lam = sx * sy + sx * sz + sy * sz;
+        s += f / (sz * (z + lam));
+        x = .25 * (x + lam);
+        y = .25 * (y + lam);
+        z = .25 * (z + lam);
+        a = .25 * (a + lam);
+        f *= .25;
+    }
+    const double X = (a - x) / a, Y = (a - y) / a, Z = -(X + Y) / 3.;
+    const double e2 = X * Y - 6. * Z * Z, e3 = (3. * X * Y - 8. * Z * Z) * Z, e4 = 3. * (X * Y - Z * Z) * Z * Z,
+                 e5 = X * Y * Z * Z * Z;
+    const double ser = 1. - 3. * e2 / 14. + e3 / 6. + 9. * e2 * e2 / 88. - 3. * e4 / 22. - 9. * e2 * e3 / 52. +
+                       3. * e5 / 26.;
+    return f * ser / (a * sqrt(a)) + 3. * s;
+}
+
+/* the reference's regularisation of K: its fit is ln4 + A(eta) - (1/2 + B(eta)) log(eta + 1e-19)
+   where the exact K has log(eta); B(eta) = O(eta) makes the difference -log1p(1e-19/eta)/2 */
+static double ell_k_reg(double eta) { return eta > 0. ? -.5 * log1p(1e-19 / eta) : 0.; }
+
+double ko_ell_K(double k)
+{
+    const double eta = 1. - k * k; /* KEllipticIntegrals.cc:24 */
+    if (!(eta > 0.)) return 1.386294361119890618834464242916 - .5 * log(fabs(eta) + 1e-19);
+    return carlson_rf(0., eta, 1.) + ell_k_reg(eta);
+}
+
+double ko_ell_E(double k)
+{
+    const double eta = 1. - k * k;
+    if (!(eta > 0.)) return 1.;
+    return carlson_rf(0., eta, 1.) - (k * k / 3.) * carlson_rd(0., eta, 1.);
+}
+
+/* (E(k) - K(k))/k^2: the reference takes the difference above k^2 = 0.8 (and so inherits the
+   regularised K there) and a power series below (KEllipticIntegrals.cc:101-125) */
+double ko_ell_EK(double k)
+{
+    const double k2 = k * k, eta = 1. - k2;
+    if (!(eta > 0.)) return (1. - ko_ell_K(k)) / k2;
+    double ek = -carlson_rd(0., eta, 1.) / 3.;
+    if (k2 > 0.8) ek -= ell_k_reg(eta) / k2;
+    return ek;
+}
+
+/* KConicSection: P0, P1 (Surfaces/src/KConicSection.cc:7-11), radius = x, axial position = z
+   of each point (Surfaces/include/KConicSection.hh GetR0/GetZ0); KRing: P (KRing.cc:8-11). */
+double ko_axisym_area(int shape, const double* d)
+{
+    if (shape == KO_RING) return 2. * KO_PI * d[0]; /* KRing.cc:20-23 */
+    return KO_PI * (d[0] + d[3]) * sqrt((d[0] - d[3]) * (d[0] - d[3]) + (d[2] - d[5]) * (d[2] - d[5]));
+}
+
+/* KConicSection::Normal (KConicSection.cc:59-104); KRing::Normal (KRing.cc:40-43) */
+void ko_axisym_normal(int shape, const double* d, double n[3])
+{
+    n[1] = 0.;
+    if (shape == KO_RING) {
+        n[0] = 1.;
+        n[2] = 0.;
+        return;
+    }
+    const double r0 = d[0], z0 = d[2], r1 = d[3], z1 = d[5];
+    if (fabs(z0 - z1) < 1.e-10) {
+        n[0] = 0.;
+        n[2] = r0 < r1 ? -1. : 1.;
+    } else if (fabs(r0 - r1) < 1.e-10) {
+        n[2] = 0.;
+        n[0] = z0 < z1 ? 1. : -1.;
+    } else {
+        const double m = (z0 - z1) / (r1 - r0);
+        n[0] = m / sqrt(1. + m * m);
+        n[2] = 1. / sqrt(1. + m * m);
+    }
+}
+
+/* Unscaled integrands of a charged ring at axial position Z, radius R seen from (z, r)
+   (Analytic/src/KElectrostaticAnalyticRingIntegrator.cc:93-110 potential, :138-160 radial and
+   :185-205 axial field); the ring runs along the generating line of a conic section,
+   R and Z at distance p from its first point (par: z, r, zA, rA, zB, rB, L). */
+static void ring_integrands(double p, const double par[7], double f[3])
+{
+    const double Z = par[2] + p / par[6] * (par[4] - par[2]);
+    const double R = par[3] + p / par[6] * (par[5] - par[3]);
+    const double dz = par[0] - Z, dr = par[1] - R, sumr = R + par[1];
+    const double eta = (dr * dr + dz * dz) / (sumr * sumr + dz * dz);
+    const double S = sqrt(sumr * sumr + dz * dz);
+    const double kp = eta > 1. ? 0. : sqrt(1. - eta);
+    f[0] = R * ko_ell_K(kp) / S;
+    const double k = sqrt(1. - eta);
+    const double E = ko_ell_E(k);
+    f[1] = R / (S * S * S) * (-2. * R * ko_ell_EK(k) + dr / eta * E);
+    f[2] = dz / (S * S * S) * E / eta * R;
+}
+
+/* Gluck's composite rule on n+1 equidistant nodes of [a, b] with end corrections
+   (Math/src/KGaussianQuadrature.cc:18-88): the end weights are the reference's table.  They
+   are not fixed by polynomial exactness (other end weights are also exact to degree 5 and 9
+   for every n), and the conic-section results depend on the rule's error term, so the same
+   weights are used. */
+static const double GL5[6] = {51. / 160., 991. / 720., 59. / 90., 97. / 80., 1333. / 1440., 91. / 90.};
+static const double GL9[10] = {0.2803440531305107e0, 0.1648702325837748e1, -0.2027449845679092e0,
+                               0.2797927414021179e1, -0.9761199294532843e0, 0.2556499393738999e1,
+                               0.1451083002645404e0, 0.1311227127425048e1, 0.9324249063051143e0,
+                               0.1006631393298060e1};
+
+static void gluck_quad(double a, double b, const double par[7], int n, double ans[3])
+{
+    if (n < 12) n = 12;
+    const double h = (b - a) / n;
+    const int m = n < 20 ? 6 : 10;
+    const double* w = n < 20 ? GL5 : GL9;
+    ans[0] = ans[1] = ans[2] = 0.;
+    for (int i = 0; i <= n; i++) {
+        const double x = a + h * i;
+        const double wi = i < m ? w[i] : (i > n - m ? w[n - i] : 1.);
+        double f[3];
+        ring_integrands(x, par, f);
+        for (int j = 0; j < 3; j++) ans[j] += wi * f[j];
+    }
+    for (int j = 0; j < 3; j++) ans[j] *= h;
+}
+
+/* field and potential of a uniformly charged ring (unit charge: the reference's prefactor
+   1/(2 pi^2 eps0 R) times R K/S, KElectrostaticAnalyticRingIntegrator.cc:26-67) */
+void ko_ring(const double* d, const double P[3], double out[4])
+{
+    const double R = d[0], Z = d[2];
+    const double par[7] = {P[2], sqrt(P[0] * P[0] + P[1] * P[1]), Z, R, Z, R, 1.};
+    const double c = 1. / (par[3] * 2. * KO_PI * KO_PI * KO_EPS0);
+    double f[3];
+    ring_integrands(0., par, f);
+    out[3] = c * f[0];
+    out[2] = c * f[2];
+    const double er = c * f[1];
+    if (par[1] < 1.e-14) {
+        out[0] = out[1] = 0.;
+    } else {
+        out[0] = P[0] / par[1] * er;
+        out[1] = P[1] / par[1] * er;
+    }
+}
+
+/* field and potential of a conic section of unit surface charge density: the integral of
+   ring integrands over the generating line, 20 / 100 / 500 nodes by the distance measure
+   |Da + Db - L|/L, and near the line a geometric split towards the projected point with the
+   logarithmic remainder added in closed form (Analytic/src/
+   KElectrostaticAnalyticConicSectionIntegrator.cc:28-104 potential, :107-221 field) */
+void ko_conic(const double* d, const double P[3], double out[4])
+{
+    static const double ln4 = 1.386294361119890;
+    const double z = P[2], r = sqrt(P[0] * P[0] + P[1] * P[1]);
+    const double za = d[2], ra = d[0], zb = d[5], rb = d[3];
+    const double L = sqrt((d[0] - d[3]) * (d[0] - d[3]) + (d[1] - d[4]) * (d[1] - d[4]) + (d[2] - d[5]) * (d[2] - d[5]));
+    const double par[7] = {z, r, za, ra, zb, rb, L};
+    const double Da = sqrt((za - z) * (za - z) + (ra - r) * (ra - r));
+    const double Db = sqrt((zb - z) * (zb - z) + (rb - r) * (rb - r));
+    const double D = fabs(Da + Db - L) / L;
+    double q[3];
+    if (D >= 5.e-2)
+        gluck_quad(0., L, par, 20, q);
+    else if (D >= 5.e-3)
+        gluck_quad(0., L, par, 100, q);
+    else if (D >= 5.e-4)
+        gluck_quad(0., L, par, 500, q);
+    else {
+        const double u0 = (zb - za) / L, u1 = (rb - ra) / L;
+        const double pp = (z - za) * u0 + (r - ra) * u1;
+        q[0] = q[1] = q[2] = 0.;
+        if (pp < L) {
+            const double pmax = L, pmin = pp < 0. ? 0. : pp;
+            double b = pmax, a = pmin + (b - pmin) * 0.3, t[3];
+            for (int it = 1; it <= 35; it++) {
+                gluck_quad(a, b, par, 50, t);
+                for (int j = 0; j < 3; j++) q[j] += t[j];
+                if (fabs(a - pmin) / L < 1.e-8) break;
+                b = a;
+                a = pmin + (b - pmin) * 0.3;
+            }
+            const double h = fabs(a - pmin);
+            const double tail = h * (ln4 + log(2. * par[1] + 1.e-12) + 1. - log(h)) / 2.;
+            for (int j = 0; j < 3; j++) q[j] += tail;
+        }
+        if (pp > 0.) {
+            const double pmax = pp > L ? L : pp, pmin = 0.;
+            double b = pmin, a = pmax - (pmax - b) * 0.3, t[3];
+            for (int it = 1; it <= 35; it++) {
+                gluck_quad(b, a, par, 50, t);
+                for (int j = 0; j < 3; j++) q[j] += t[j];
+                if (fabs(pmax - a) / L < 1.e-8) break;
+                b = a;
+                a = pmax - (pmax - b) * 0.3;
+            }
+            const double h = fabs(pmax - a);
+            const double tail = h * (ln4 + log(2. * par[1] + 1.e-12) + 1. - log(h)) / 2.;
+            for (int j = 0; j < 3; j++) q[j] += tail;
+        }
+    }
+    const double c = 1. / (KO_PI * KO_EPS0);
+    out[3] = c * q[0];
+    const double er = c * q[1];
+    out[2] = c * q[2];
+    if (r < 1.e-14) {
+        out[0] = out[1] = 0.;
+    } else {
+        out[0] = P[0] / r * er;
+        out[1] = P[1] / r * er;
+    }
+}
+
 /* ------------------------------------------------------------ policies */
 /* KElectrostaticBoundaryIntegratorFactory.cc:29-88: numeric = cubature/RWG dispatch (default),
    rwg = closed form everywhere (line: analytic), analytic = the Analytic integrators,
    reference = nested 32-node Gauss-Legendre (line: 256 nodes). */
 int ko_element_eval(int policy, int shape, const double* d, const double P[3], double out[4])
 {
+    /* every policy takes the analytic conic-section and ring integrators (...Factory.cc:39-73) */
+    if (shape == KO_CONICSECTION) {
+        ko_conic(d, P, out);
+        return 0;
+    }
+    if (shape == KO_RING) {
+        ko_ring(d, P, out);
+        return 0;
+    }
     if (policy == KO_ANALYTIC) {
         if (shape == KO_TRIANGLE) ko_tri_analytic(d, P, out);
         else if (shape == KO_RECTANGLE) ko_rect_analytic(d, P, out);

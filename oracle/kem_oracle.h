@@ -15,6 +15,9 @@
  *     (the layout of KElectrostaticCubatureTriangleIntegrator.cc:457-467)
  *   line segment:        7 doubles [P0x, P0y, P0z, P1x, P1y, P1z, diameter]
  *     (KElectrostaticQuadratureLineSegmentIntegrator.cc:197-203)
+ *   conic section:       6 doubles [P0x, P0y, P0z, P1x, P1y, P1z] (radius = x, axis = z;
+ *     Surfaces/src/KConicSection.cc:7-22)
+ *   ring:                3 doubles [Px, Py, Pz] (radius = x, axis = z; Surfaces/src/KRing.cc:8-18)
  *   Records are stored with a stride of KO_REC doubles.
  *
  * Outputs of a single (element, point) evaluation are 4 doubles [Ex, Ey, Ez, phi] for a
@@ -32,7 +35,7 @@ extern "C" {
 
 #define KO_REC 11
 
-enum ko_shape { KO_TRIANGLE = 0, KO_RECTANGLE = 1, KO_LINESEGMENT = 2 };
+enum ko_shape { KO_TRIANGLE = 0, KO_RECTANGLE = 1, KO_LINESEGMENT = 2, KO_CONICSECTION = 3, KO_RING = 4 };
 enum ko_policy { KO_NUMERIC = 0, KO_RWG = 1, KO_ANALYTIC = 2, KO_REFERENCE = 3 };
 enum ko_bc { KO_DIRICHLET = 0, KO_NEUMANN = 1 };
 
@@ -84,6 +87,15 @@ void ko_line_256(const double* rec, const double P[3], double out[4]);
 int ko_tri_numeric(const double* rec, const double P[3], double out[4]);
 int ko_rect_numeric(const double* rec, const double P[3], double out[4]);
 int ko_line_numeric(const double* rec, const double P[3], double out[4]);
+
+/* axisymmetric elements (every policy): Analytic/src/KElectrostaticAnalyticRingIntegrator.cc,
+   ...ConicSectionIntegrator.cc; the complete elliptic integrals K, E and (E - K)/k^2 as the
+   reference evaluates them (Math/src/KEllipticIntegrals.cc) */
+double ko_ell_K(double k);
+double ko_ell_E(double k);
+double ko_ell_EK(double k);
+void ko_ring(const double* rec, const double P[3], double out[4]);
+void ko_conic(const double* rec, const double P[3], double out[4]);
 
 /* policy dispatch over shapes: KElectrostaticBoundaryIntegratorFactory.cc:29-88 */
 int ko_element_eval(int policy, int shape, const double* rec, const double P[3], double out[4]);

@@ -13,7 +13,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = os.path.join(_HERE, "libkem_oracle.so")
 
-TRIANGLE, RECTANGLE, LINESEGMENT = 0, 1, 2
+TRIANGLE, RECTANGLE, LINESEGMENT, CONICSECTION, RING = 0, 1, 2, 3, 4
 NUMERIC, RWG, ANALYTIC, REFERENCE = 0, 1, 2, 3
 DIRICHLET, NEUMANN = 0, 1
 REC = 11
@@ -66,6 +66,11 @@ def _load():
         "ko_gmres_ortho": (ctypes.c_int, [i64, P, P, P, d, ctypes.c_int, ctypes.c_int, ctypes.c_int, P, P]),
         "ko_robin_hood": (i64, [i64, P, P, P, d, i64, i64, P]),
         "ko_gauss": (ctypes.c_int, [i64, P, P, P]),
+        "ko_ell_K": (d, [d]),
+        "ko_ell_E": (d, [d]),
+        "ko_ell_EK": (d, [d]),
+        "ko_ring": (None, [P, P, P]),
+        "ko_conic": (None, [P, P, P]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -203,6 +208,26 @@ def rect_biquad(rec, P):
 
 def line_256(rec, P):
     return _eval4(lib().ko_line_256, _p(_recs(rec)), _p(_f64(P)))
+
+
+def ell_K(k):
+    return lib().ko_ell_K(float(k))
+
+
+def ell_E(k):
+    return lib().ko_ell_E(float(k))
+
+
+def ell_EK(k):
+    return lib().ko_ell_EK(float(k))
+
+
+def ring(rec, P):
+    return _eval4(lib().ko_ring, _p(_recs(rec)), _p(_f64(P)))
+
+
+def conic(rec, P):
+    return _eval4(lib().ko_conic, _p(_recs(rec)), _p(_f64(P)))
 
 
 def element_eval(shape, rec, P, policy=NUMERIC):
