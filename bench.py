@@ -160,7 +160,8 @@ def main():
 
         t0 = time.perf_counter()
         tuning = {"far_range_mixed": T.tune_field(cont, sigma, pts, "mixed", repeats=2),
-                  "far_range_fp64": T.tune_field(cont, sigma, pts, "fp64", repeats=1)}
+                  "far_range_fp64": T.tune_field(cont, sigma, pts, "fp64", repeats=1),
+                  "near_blocks": T.tune_near(cont, sigma, pts, repeats=2)}
         if A is not None:
             tuning["gemv_blocks"] = T.tune_matvec(A, repeats=5)
             tuning["bmm_blocks"] = T.tune_block_product(A, nrhs=64, precision="mixed", repeats=3)
@@ -537,6 +538,23 @@ def run_large(args):
     A.close()
     for arr in X + O:
         arr.close()
+    # CPU baseline: the FP64 oracle on all host cores over a sample of the same points; the
+    # CPU pipeline takes at least its field pass, so N M / rate bounds its time from below
+    cpu = None
+    if not args.no_cpu_baseline:
+        from oracle import oracle as OR
+
+        threads = args.cpu_threads or max(1, min(16, os.cpu_count() or 1))
+        k = min(args.cpu_points or 64 * threads, M)
+        t0 = time.perf_counter()
+        OR.field(mesh.shapes, mesh.recs, x64, pts[:k], threads=threads)
+        tc = time.perf_counter() - t0
+        rate = k * N / tc
+        cpu = {"value": rate, "unit": "point-element pairs/s", "cores": threads, "kind": "port",
+               "sample": f"{k} points x {N} triangles of the same workload, FP64 oracle, {threads} OpenMP "
+                         f"threads, {tc:.1f} s"}
+        log(f"cpu baseline {rate:.3e} pairs/s on {threads} threads: field pass alone {N * M / rate:.0f} s, "
+            f"{N * M / rate / t_mixed:.0f}x the mixed GPU pipeline")
     value = N * M / t_fmx
     line = {
         "metric": "point-element field evals/s + solve time, mixed vs FP64 at 1e-6 rel. error",
@@ -556,6 +574,9 @@ def run_large(args):
         "batch16_solve_s": stb.seconds, "batch16_worst_residual": float(res.max()),
         "adjoint_field_vjp_s": t_fvjp, "adjoint_solve_s": sta.seconds, "adjoint_matrix_vjp_s": t_mvjp,
         "tuning": tuning,
+        "cpu_baseline": cpu,
+        "cpu_field_s_estimate": None if cpu is None else N * M / cpu["value"],
+        "mixed_pipeline_vs_cpu_field_lower_bound": None if cpu is None else N * M / cpu["value"] / t_mixed,
     }
     print(json.dumps(line), flush=True)
 

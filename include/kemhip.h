@@ -115,6 +115,7 @@ int kem_ctx_set_mixed_fp64_ratio(kem_ctx* ctx, double ratio);
 #define KEM_TUNE_FAR_RANGE_MIXED 2
 #define KEM_TUNE_GEMV_BLOCKS 3
 #define KEM_TUNE_BMM_BLOCKS 4
+#define KEM_TUNE_NEAR_BLOCKS 5 /* blocks of the near pass beside a single-set mixed far pass (0: one per 256 points; default 384) */
 int kem_ctx_set_tuning(kem_ctx* ctx, int param, int64_t value);
 int kem_ctx_get_tuning(const kem_ctx* ctx, int param, int64_t* value);
 /* Instrumentation, no reference counterpart: with timing on, each field evaluation records HIP

@@ -8,6 +8,8 @@ The C-ABI exposes four launch-shape knobs (``kem_ctx_set_tuning``):
 * ``KEM_TUNE_GEMV_BLOCKS``: target block count of the split-K matrix-vector product (GMRES,
   Robin Hood residuals, ``KSquareMatrix::Multiply``).
 * ``KEM_TUNE_BMM_BLOCKS``: target block count of the multi-right-hand-side block product.
+* ``KEM_TUNE_NEAR_BLOCKS``: blocks of the near pass that runs beside the mixed far pass on a
+  second stream (0: one per 256 points).  Its timed region is the whole field pass.
 
 Each knob is swept over its candidates; every candidate runs once untimed and then ``repeats``
 times, timed by the HIP events the library records around the kernels the knob shapes
@@ -16,6 +18,7 @@ are plain dicts that ``save`` / ``load`` keep as JSON and ``apply`` sets on anot
 """
 import ctypes
 import json
+import time
 
 import numpy as np
 
@@ -25,9 +28,11 @@ from . import kemfield as K
 FAR_RANGE_CANDIDATES = (0, 640, 1280, 2560, 5120, 10240)
 GEMV_CANDIDATES = (512, 1024, 2048, 4096, 8192)
 BMM_CANDIDATES = (640, 1280, 2560, 5120, 10240)
+NEAR_CANDIDATES = (0, 128, 256, 384, 512)
 
 _KNOB = {"far_range_fp64": _lib.KEM_TUNE_FAR_RANGE_FP64, "far_range_mixed": _lib.KEM_TUNE_FAR_RANGE_MIXED,
-         "gemv_blocks": _lib.KEM_TUNE_GEMV_BLOCKS, "bmm_blocks": _lib.KEM_TUNE_BMM_BLOCKS}
+         "gemv_blocks": _lib.KEM_TUNE_GEMV_BLOCKS, "bmm_blocks": _lib.KEM_TUNE_BMM_BLOCKS,
+         "near_blocks": _lib.KEM_TUNE_NEAR_BLOCKS}
 
 
 def get(ctx: K.Context, name: str) -> int:
@@ -40,8 +45,9 @@ def set_knob(ctx: K.Context, name: str, value: int):
     ctx.check(ctx.lib.kem_ctx_set_tuning(ctx.h, _KNOB[name], int(value)))
 
 
-def _sweep(ctx, name, candidates, run, repeats):
-    """ms per timed region for each candidate; leaves the fastest set."""
+def _sweep(ctx, name, candidates, run, repeats, wall=False):
+    """ms per timed region for each candidate (wall: the whole call between two stream
+    synchronisations); leaves the fastest set."""
     old = get(ctx, name)
     times = {}
     try:
@@ -49,6 +55,13 @@ def _sweep(ctx, name, candidates, run, repeats):
             set_knob(ctx, name, v)
             run()  # untimed: workspaces grow, caches warm
             ctx.check(ctx.lib.kem_ctx_sync(ctx.h))
+            if wall:
+                t0 = time.perf_counter()
+                for _ in range(repeats):
+                    run()
+                ctx.check(ctx.lib.kem_ctx_sync(ctx.h))
+                times[int(v)] = (time.perf_counter() - t0) * 1e3 / repeats
+                continue
             ctx.check(ctx.lib.kem_ctx_set_kernel_timing(ctx.h, 1))
             for _ in range(repeats):
                 run()
@@ -82,6 +95,26 @@ def tune_field(container: K.KSurfaceContainer, sigma, points, precision="mixed",
 
     try:
         return _sweep(ctx, "far_range_" + precision, candidates, run, repeats)
+    finally:
+        for a in X + O + [S]:
+            a.close()
+
+
+def tune_near(container: K.KSurfaceContainer, sigma, points, repeats=3, candidates=NEAR_CANDIDATES):
+    """Grid of the near pass beside the mixed far pass, timed over the whole field pass."""
+    ctx = container.ctx
+    P = np.asarray(points, dtype=np.float64).reshape(-1, 3)
+    m = P.shape[0]
+    X = [K.DeviceArray(ctx, np.ascontiguousarray(P[:, k])) for k in range(3)]
+    S = K.DeviceArray(ctx, np.asarray(sigma, dtype=np.float64))
+    O = [K.DeviceArray.empty(ctx, m) for _ in range(4)]
+
+    def run():
+        ctx.check(ctx.lib.kem_field_eval_device(ctx.h, container.h, S.ptr, m, X[0].ptr, X[1].ptr, X[2].ptr,
+                                                _lib.KEM_MIXED, O[0].ptr, O[1].ptr, O[2].ptr, O[3].ptr))
+
+    try:
+        return _sweep(ctx, "near_blocks", candidates, run, repeats, wall=True)
     finally:
         for a in X + O + [S]:
             a.close()

@@ -241,6 +241,10 @@ int kem_ctx_set_tuning(kem_ctx* c, int param, int64_t value)
         if (value < 1 || value > (1 << 20)) return fail(c, KEM_E_ARG, "tuning: block count out of [1, 2^20]");
         (param == KEM_TUNE_GEMV_BLOCKS ? c->tuning.gemv_blocks : c->tuning.bmm_blocks) = value;
         return KEM_OK;
+    case KEM_TUNE_NEAR_BLOCKS:
+        if (value < 0 || value > (1 << 20)) return fail(c, KEM_E_ARG, "tuning: block count out of [0, 2^20]");
+        c->tuning.near_blocks = value;
+        return KEM_OK;
     default:
         return fail(c, KEM_E_ARG, "tuning: unknown parameter");
     }
@@ -254,6 +258,7 @@ int kem_ctx_get_tuning(const kem_ctx* c, int param, int64_t* value)
     case KEM_TUNE_FAR_RANGE_MIXED: *value = c->far_range_mixed; return KEM_OK;
     case KEM_TUNE_GEMV_BLOCKS: *value = c->tuning.gemv_blocks; return KEM_OK;
     case KEM_TUNE_BMM_BLOCKS: *value = c->tuning.bmm_blocks; return KEM_OK;
+    case KEM_TUNE_NEAR_BLOCKS: *value = c->tuning.near_blocks; return KEM_OK;
     default: return KEM_E_ARG;
     }
 }

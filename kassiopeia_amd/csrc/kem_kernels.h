@@ -80,6 +80,7 @@ hipError_t launch_assemble(hipStream_t st, int policy, bool field, const double*
 struct LaunchTuning {
     int64_t gemv_blocks = 2048;
     int64_t bmm_blocks = 2560;
+    int64_t near_blocks = 384;  // grid of the near pass beside the mixed far pass (0: one block per point block)
 };
 void set_launch_tuning(const LaunchTuning& t);
 const LaunchTuning& launch_tuning();

@@ -220,7 +220,9 @@ __device__ __forceinline__ double wave_max(double v)
 // sphere of near distances cannot reach it is skipped as a whole.  The pairs evaluated are
 // exactly the near set (the same dist2 test as the far pass); culling only removes clusters
 // proven to contain none.  Writes its own sums (one slice of the partial-sum layout that
-// k_field_combine adds), so it may run beside the far pass.
+// k_field_combine adds), so it may run beside the far pass.  Blocks stride over the point
+// blocks, so a small grid keeps the pass's footprint to a few wavefronts per SIMD beside the
+// far kernel.
 template <bool MULTI, bool AXI>
 __global__ __launch_bounds__(kBlock) void k_field_near_culled(const double* __restrict__ geom,
                                                               const double* __restrict__ sigma,
@@ -233,7 +235,8 @@ __global__ __launch_bounds__(kBlock) void k_field_near_culled(const double* __re
                                                               double* __restrict__ out0, double* __restrict__ out1,
                                                               double* __restrict__ out2, double* __restrict__ out3)
 {
-    int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  for (int64_t pb = blockIdx.x; pb * kBlock < npts; pb += gridDim.x) {
+    int64_t i = pb * kBlock + threadIdx.x;
     const bool live = i < npts;
     if (!live) i = npts - 1;
     const double x = px[i], y = py[i], z = pz[i];
@@ -272,6 +275,7 @@ __global__ __launch_bounds__(kBlock) void k_field_near_culled(const double* __re
         out2[i] = s2;
         out3[i] = s3;
     }
+  }
 }
 
 // A value every lane holds alike, moved to scalar registers so branches on it are uniform.
@@ -933,6 +937,7 @@ inline unsigned nblocks(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock
 
 // ====================================================================== launchers
 namespace {
+thread_local LaunchTuning g_tuning;
 constexpr int64_t kPartialDoubles = 4 * (int64_t)(2048 + 256) * 256;
 // From this many point blocks on, the numeric-policy far pass of one charge-density set splits
 // the elements into ranges of at most range_elems elements, a multiple of 8 of them (far_tile),
@@ -965,7 +970,7 @@ int64_t partial_doubles(int policy, const FieldMesh& m, int64_t npts)
 template <bool MULTI>
 void near_slice(hipStream_t st, const SideStream* side, const FieldMesh& m, const double* sg, const int32_t* pset,
                 int64_t nsets, const double* px, const double* py, const double* pz, int64_t np, double* q,
-                int64_t ns, int64_t slice)
+                int64_t ns, int64_t slice, bool mixed)
 {
     hipStream_t s = st;
     if (side && side->st) {
@@ -975,12 +980,21 @@ void near_slice(hipStream_t st, const SideStream* side, const FieldMesh& m, cons
     }
     const int64_t ncl = (m.n + KEM_CLUSTER - 1) / KEM_CLUSTER;
     double* o = q + slice * np;
+    // beside the mixed far pass a grid of g_tuning.near_blocks blocks (0: one per point block):
+    // a few wavefronts per SIMD, which the far kernel's blocks do not crowd out, so the pass ends
+    // with the far kernel instead of after it (measured on 1 M points x 20,000 triangles: 67.5 ms
+    // with one block per point block, 64.9 ms with 64 to 512 blocks, 69.3 ms with 768).  The FP64
+    // far pass runs long enough that a small grid only takes its occupancy (362 -> 367..402 ms),
+    // and so does the near pass over several sets (4 M points in uneven batches: 253 -> 261 ms).
+    unsigned gn = nblocks(np);
+    if (s != st && mixed && !MULTI && g_tuning.near_blocks > 0 && (int64_t)gn > g_tuning.near_blocks)
+        gn = (unsigned)g_tuning.near_blocks;
     if (m.axisym)
-        k_field_near_culled<MULTI, true><<<nblocks(np), kBlock, 0, s>>>(m.geom, sg, pset, nsets, m.clusters, ncl,
+        k_field_near_culled<MULTI, true><<<gn, kBlock, 0, s>>>(m.geom, sg, pset, nsets, m.clusters, ncl,
                                                                         m.order, m.n, px, py, pz, np, o,
                                                                         o + ns * np, o + 2 * ns * np, o + 3 * ns * np);
     else
-        k_field_near_culled<MULTI, false><<<nblocks(np), kBlock, 0, s>>>(m.geom, sg, pset, nsets, m.clusters, ncl,
+        k_field_near_culled<MULTI, false><<<gn, kBlock, 0, s>>>(m.geom, sg, pset, nsets, m.clusters, ncl,
                                                                          m.order, m.n, px, py, pz, np, o,
                                                                          o + ns * np, o + 2 * ns * np, o + 3 * ns * np);
     if (s != st) (void)hipEventRecord(side->join, s);
@@ -1059,7 +1073,7 @@ void field_passes(hipStream_t st, int policy, const FieldMesh& m, bool mixed, do
             double* q = ws;
             if (culled)
                 near_slice<MULTI>(st, side, m, sg, MULTI ? pset + p0 : nullptr, nsets, px + p0, py + p0, pz + p0, np,
-                                  q, ns, nr);
+                                  q, ns, nr, kind != 0);
             far_kernel<MULTI>(st, dim3((unsigned)(gc * nr)), kind, m, fp64_ratio, wk, sg, MULTI ? pset + p0 : nullptr,
                               nsets, per, px + p0, py + p0, pz + p0, np, q, q + ns * np, q + 2 * ns * np,
                               q + 3 * ns * np, np, (int64_t)gc);
@@ -1089,7 +1103,7 @@ void field_passes(hipStream_t st, int policy, const FieldMesh& m, bool mixed, do
             o3 = ws + 3 * ns * npts;
             stride = npts;
         }
-        if (culled) near_slice<MULTI>(st, side, m, sg, pset, nsets, px, py, pz, npts, ws, ns, nsplit);
+        if (culled) near_slice<MULTI>(st, side, m, sg, pset, nsets, px, py, pz, npts, ws, ns, nsplit, kind != 0);
         if (far_begin) (void)hipEventRecord(far_begin, st);
         if (clustered)
             k_far_weights<<<nblocks(nsets * nelem), kBlock, 0, st>>>(m.geom, m.order, sigma, nelem, nsets, wk);
@@ -1195,10 +1209,6 @@ hipError_t launch_assemble(hipStream_t st, int policy, bool field, const double*
         k_assemble<false, 0><<<grid, kBlock, 0, st>>>(geom, nodes, recs, targets, policy, n, per, A64, A32);
     }
     return hipGetLastError();
-}
-
-namespace {
-thread_local LaunchTuning g_tuning;
 }
 
 void set_launch_tuning(const LaunchTuning& t) { g_tuning = t; }

@@ -74,14 +74,36 @@ def test_tuning_table_roundtrip_and_errors(setup, tmp_path):
 
     ctx = setup[0]
     table = {"far_range_fp64": 1280, "far_range_mixed": {"best": 5120, "ms": {}}, "gemv_blocks": 1024,
-             "bmm_blocks": 2560}
+             "bmm_blocks": 2560, "near_blocks": 256}
     T.save(table, tmp_path / "t.json")
     loaded = T.load(tmp_path / "t.json")
-    assert loaded == {"far_range_fp64": 1280, "far_range_mixed": 5120, "gemv_blocks": 1024, "bmm_blocks": 2560}
+    assert loaded == {"far_range_fp64": 1280, "far_range_mixed": 5120, "gemv_blocks": 1024, "bmm_blocks": 2560,
+                      "near_blocks": 256}
     T.apply(ctx, loaded)
     assert T.current(ctx) == loaded
     with pytest.raises(_lib.KemError):
         T.set_knob(ctx, "gemv_blocks", 0)
     with pytest.raises(_lib.KemError):
         ctx.check(ctx.lib.kem_ctx_set_tuning(ctx.h, 99, 1))
-    T.apply(ctx, {"far_range_fp64": 2560, "far_range_mixed": 2560, "gemv_blocks": 2048})
+    with pytest.raises(_lib.KemError):
+        T.set_knob(ctx, "near_blocks", -1)
+    T.apply(ctx, {"far_range_fp64": 2560, "far_range_mixed": 2560, "gemv_blocks": 2048, "near_blocks": 384})
+
+
+def test_tune_near(setup):
+    """The near pass's grid beside the mixed far pass: a sweep over the whole field pass, and the
+    field is bit-identical for every grid (each point's near sum runs in the same order)."""
+    from kassiopeia_amd import autotune as T
+    from kassiopeia_amd import kemfield as K
+
+    ctx, m, cont, A, x = setup
+    pts = G.points_annulus(300_000, 0.5, 1.0, 2.0, 5)
+    r = T.tune_near(cont, x, pts, repeats=1, candidates=(0, 7, 384))
+    assert set(r["ms"]) == {0, 7, 384} and T.get(ctx, "near_blocks") == r["best"]
+    out = {}
+    for v in (0, 7, 384):
+        T.set_knob(ctx, "near_blocks", v)
+        out[v] = K.KIntegratingElectrostaticFieldSolver(cont, x, "mixed").ElectricFieldAndPotential(pts)
+    for v in (7, 384):
+        assert np.array_equal(out[v][0], out[0][0]) and np.array_equal(out[v][1], out[0][1])
+    T.set_knob(ctx, "near_blocks", 384)
