@@ -335,8 +335,10 @@ __global__ __launch_bounds__(kBlock) void k_far_weights(const double* __restrict
 // pair for every lane -- the per-pair ladder of k_field_far32 would pick the same rule -- so
 // the wavefront evaluates it from the element's far record without per-lane regime tests; a
 // cluster beyond its own reach skips even the per-element test.  Every other element takes
-// k_field_far32's per-pair path.  Each pair's FP32 result equals k_field_far32's (same nodes,
-// same rounding of P - centre); the FP64 sums run over elements in cluster order.  Elements
+// k_field_far32's per-pair path.  A 6-point pair evaluates P - node as (P - cluster centre)
+// rounded to FP32 once per cluster minus the node's FP32 offset from the cluster centre (the
+// per-pair path rounds P - element centre per pair); the FP64 sums run over elements in
+// cluster order.  Elements
 // [c0*32, c1*32) of the cluster order per blockIdx.y; wk from k_far_weights.
 template <bool MULTI>
 __global__ __launch_bounds__(kBlock) void k_field_far32c(const double* __restrict__ geom,
@@ -376,6 +378,9 @@ __global__ __launch_bounds__(kBlock) void k_field_far32c(const double* __restric
         const double qx = wx - cl[0], qy = wy - cl[1], qz = wz - cl[2];
         const double tc = cl[3] + wr;
         const bool all6 = qx * qx + qy * qy + qz * qz > tc * tc;
+        // P - cluster centre, rounded to FP32 once per cluster; the far records hold the nodes
+        // relative to the cluster centre
+        const float dx = (float)(x - cl[0]), dy = (float)(y - cl[1]), dz = (float)(z - cl[2]);
         int64_t k1 = (c + 1) * KEM_CLUSTER;
         if (k1 > nelem) k1 = nelem;
         for (int64_t k = c * KEM_CLUSTER; k < k1; k++) {
@@ -392,7 +397,6 @@ __global__ __launch_bounds__(kBlock) void k_field_far32c(const double* __restric
             }
             float s0, s1, s2, s3;
             if (six) {
-                const float dx = (float)(x - cx), dy = (float)(y - cy), dz = (float)(z - cz);
                 s0 = s1 = s2 = s3 = 0.f;
                 node32(dx - A[0], dy - A[1], dz - A[2], kW6a, s0, s1, s2, s3);
                 node32(dx - A[4], dy - A[5], dz - A[6], kW6a, s0, s1, s2, s3);
@@ -405,8 +409,8 @@ __global__ __launch_bounds__(kBlock) void k_field_far32c(const double* __restric
                 const double* g = geom + j * KG_STRIDE;
                 const double r2 = dev::dist2(g, x, y, z);
                 if (dev::is_near2(g, r2, 0)) continue;
-                const float dx = (float)(x - g[KG_CX]), dy = (float)(y - g[KG_CY]), dz = (float)(z - g[KG_CZ]);
-                dev::element_far32(g, nodes32 + j * KEM_NODE32_STRIDE, (int)g[KG_TYPE], r2, dx, dy, dz, s0, s1, s2,
+                const float ex = (float)(x - g[KG_CX]), ey = (float)(y - g[KG_CY]), ez = (float)(z - g[KG_CZ]);
+                dev::element_far32(g, nodes32 + j * KEM_NODE32_STRIDE, (int)g[KG_TYPE], r2, ex, ey, ez, s0, s1, s2,
                                    s3);
             }
             a0 += (double)s0 * wj;

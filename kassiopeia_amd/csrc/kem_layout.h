@@ -79,12 +79,13 @@ enum kem_geom_field {
 #define KEM_CLUSTER 32
 
 /* far records of the mixed far field, one per element in cluster order (32 floats = 128 B):
-   the 6-point rule's nodes as (x, y, z, w) FP32 offsets from the element centre (slots
-   KEM_N6.. of the FP32 node table), then the element centre and the 6-point reach
-   sqrt((12 size)^2) * (1 + 1e-9) as FP64 (infinite for rectangles and segments).  A point
-   farther than the reach from the centre is a 6-point pair.  Per cluster: centre x, y, z and
-   the largest |c_j - centre| + reach_j (FP64), so a point farther than that from the centre
-   is a 6-point pair with every element of the cluster. */
+   the 6-point rule's nodes as (x, y, z, w) FP32 offsets from the CLUSTER centre (the element's
+   node offsets, slots KEM_N6.. of the FP32 node table, plus element centre - cluster centre,
+   rounded once), then the element centre and the 6-point reach sqrt((12 size)^2) * (1 + 1e-9)
+   as FP64 (infinite for other shapes).  A point farther than the reach from the element
+   centre is a 6-point pair.  Per cluster: centre x, y, z and the largest |c_j - centre| +
+   reach_j (FP64), so a point farther than that from the centre is a 6-point pair with every
+   element of the cluster. */
 #define KEM_FAR6_STRIDE 32
 #define KEM_FAR6_CENTRE 24
 #define KEM_FAR6_REACH 30

@@ -553,9 +553,15 @@ void build_far6(int64_t n, const double* g, const float* nodes32, const int32_t*
             const double* ge = g + j * (int64_t)KG_STRIDE;
             float* r = far6 + e * KEM_FAR6_STRIDE;
             for (int k = 0; k < KEM_FAR6_STRIDE; k++) r[k] = 0.f;
-            for (int k = 0; k < 24; k++) r[k] = nodes32[j * KEM_NODE32_STRIDE + 4 * KEM_N6 + k];
-            // the centre exactly as the element record holds it, so P - centre rounds as in
-            // the per-element path
+            // the 6 nodes as offsets from the cluster centre: (element centre - cluster centre)
+            // in FP64 plus the element's FP32 node offset, rounded once
+            for (int q = 0; q < 6; q++) {
+                const float* nq = nodes32 + j * KEM_NODE32_STRIDE + 4 * (KEM_N6 + q);
+                for (int k = 0; k < 3; k++)
+                    r[4 * q + k] = (float)((ge[KG_CX + k] - clusters[4 * c + k]) + (double)nq[k]);
+                r[4 * q + 3] = nq[3];
+            }
+            // the element centre (for the per-element 6-point test) and its 6-point reach
             double ce[4] = {ge[KG_CX], ge[KG_CY], ge[KG_CZ], INFINITY};
             if ((int)ge[KG_TYPE] == KEM_TRIANGLE) ce[3] = std::sqrt(ge[KG_MIX6_2]) * (1. + 1e-9);
             std::memcpy(r + KEM_FAR6_CENTRE, ce, sizeof ce);

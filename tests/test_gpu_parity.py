@@ -451,19 +451,19 @@ def _mixed_both_ways(ctx, fn):
 def test_field_mixed_cluster_pass(ctx, coax_small, npts):
     """The cluster far pass (6-point pairs decided per wavefront and element) reproduces the
     per-pair far pass: the same rule for every pair, FP64 sums over elements in another order.
-    The wavefront-uniform 6-point code is a separate FP32 instruction sequence, so pair results
-    may differ in the last FP32 bit: bounds 1e-9 (field, norm-wise) and 2e-8 (potential / V),
-    where one element dropped or counted twice would show at 1e-4.  700 / 3000 points run
-    unsorted with element ranges split over blocks (rarely wavefront-uniform: equal to 1e-15),
-    20000 in Morton order."""
+    The wavefront-uniform 6-point code rounds P - node differently (P - cluster centre once per
+    cluster, nodes relative to the cluster centre) so pair results differ at FP32 rounding:
+    bounds 1e-7 (field, norm-wise; 2e-8 measured) and 1e-7 (potential / V), where one element
+    dropped or counted twice would show at 1e-4.  700 / 3000 points run unsorted with element
+    ranges split over blocks (rarely wavefront-uniform), 20000 in Morton order."""
     from kassiopeia_amd import kemfield as K
 
     m, cont, x = coax_small
     pts = G.points_annulus(npts, 0.5, 1.0, 2.0, 5)
     (Ec, pc), (Ep, pp) = _mixed_both_ways(
         ctx, lambda: K.KIntegratingElectrostaticFieldSolver(cont, x, "mixed").ElectricFieldAndPotential(pts))
-    assert np.linalg.norm(Ec - Ep) / np.linalg.norm(Ep) < 1e-9
-    assert (np.abs(pc - pp) / 18600.0).max() < 2e-8
+    assert np.linalg.norm(Ec - Ep) / np.linalg.norm(Ep) < 1e-7
+    assert (np.abs(pc - pp) / 18600.0).max() < 1e-7
 
 
 def test_field_mixed_cluster_pass_shapes_and_sets(ctx):
@@ -483,17 +483,17 @@ def test_field_mixed_cluster_pass_shapes_and_sets(ctx):
     sigma = rng.normal(size=m.n)
     (Ec, pc), (Ep, pp) = _mixed_both_ways(
         ctx, lambda: K.KIntegratingElectrostaticFieldSolver(cont, sigma, "mixed").ElectricFieldAndPotential(pts))
-    assert np.linalg.norm(Ec - Ep) / np.linalg.norm(Ep) < 1e-9
-    assert np.abs(pc - pp).max() / np.abs(pp).max() < 2e-8
+    assert np.linalg.norm(Ec - Ep) / np.linalg.norm(Ep) < 1e-7
+    assert np.abs(pc - pp).max() / np.abs(pp).max() < 1e-7
     S = rng.normal(size=(m.n, 3))
     sid = rng.integers(0, 3, size=len(pts)).astype(np.int32)
     (Es, ps), (Eq, pq) = _mixed_both_ways(ctx, lambda: K.field_sets(cont, S, sid, pts, "mixed"))
-    assert np.linalg.norm(Es - Eq) / np.linalg.norm(Eq) < 1e-9
-    assert np.abs(ps - pq).max() / np.abs(pq).max() < 2e-8
+    assert np.linalg.norm(Es - Eq) / np.linalg.norm(Eq) < 1e-7
+    assert np.abs(ps - pq).max() / np.abs(pq).max() < 1e-7
     for k in range(3):  # each set against its own single-set pass
         sel = sid == k
         Ek, pk = K.KIntegratingElectrostaticFieldSolver(cont, S[:, k], "mixed").ElectricFieldAndPotential(pts[sel])
-        assert np.linalg.norm(Es[sel] - Ek) / np.linalg.norm(Ek) < 1e-9
+        assert np.linalg.norm(Es[sel] - Ek) / np.linalg.norm(Ek) < 1e-7
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
@@ -501,8 +501,9 @@ def test_field_xcd_split_pass(ctx, oracle, coax_small, precision):
     """From 262,144 points on, the far pass gives each XCD one eighth of the elements and runs
     over chunks of 2^20 points (1,100,000 points: two chunks).  The same points evaluated in
     calls of 50,000 (element ranges split over blocks) agree to summation order (FP64) or to
-    the last FP32 bit of some pairs (mixed, bound 1e-9 as for the cluster pass); FP64 matches
-    the oracle on a sample."""
+    FP32 rounding where a pair takes the wavefront-uniform path in one call and the per-pair
+    path in the other (mixed, bound 1e-7 as for the cluster pass); FP64 matches the oracle on
+    a sample."""
     from kassiopeia_amd import kemfield as K
 
     m, cont, x = coax_small
@@ -511,9 +512,9 @@ def test_field_xcd_split_pass(ctx, oracle, coax_small, precision):
     E, phi = solver.ElectricFieldAndPotential(pts)
     sel = np.arange(0, len(pts), 22)  # 50,000 points
     Es, ps = solver.ElectricFieldAndPotential(pts[sel])
-    tol = 1e-14 if precision == "fp64" else 1e-9
+    tol = 1e-14 if precision == "fp64" else 1e-7
     assert np.linalg.norm(E[sel] - Es) / np.linalg.norm(Es) < tol
-    assert (np.abs(phi[sel] - ps) / 18600.0).max() < (1e-14 if precision == "fp64" else 2e-8)
+    assert (np.abs(phi[sel] - ps) / 18600.0).max() < (1e-14 if precision == "fp64" else 1e-7)
     if precision == "fp64":
         few = sel[::250]
         phr, Er = oracle.field(m.shapes, m.recs, x, pts[few])
@@ -531,7 +532,7 @@ def test_field_sets_large_pass(ctx, coax_small):
     S = np.stack([x, -0.5 * x, x * rng.uniform(0.5, 1.5, size=x.size)], 1)
     pts = G.points_annulus(300_000, 0.5, 1.0, 2.0, 17)
     sid = rng.integers(0, 3, size=len(pts)).astype(np.int32)
-    for prec, tol in (("fp64", 1e-14), ("mixed", 1e-9)):
+    for prec, tol in (("fp64", 1e-14), ("mixed", 1e-7)):
         Es, ps = K.field_sets(cont, S, sid, pts, prec)
         for k in range(3):
             sel = sid == k
