@@ -142,6 +142,17 @@ int kem_set_integrator(kem_ctx* ctx, const char* name);
    bc_type / bc_value / bc_flux may be NULL (all Dirichlet at 0 V). */
 int kem_mesh_create(kem_ctx* ctx, int64_t n, const int32_t* shape_type, const double* shape_data,
                     const int32_t* bc_type, const double* bc_value, const double* bc_flux, kem_mesh** out);
+/* Symmetry groups (KSymmetryGroup<Shape>, Surfaces/include/KSymmetryGroup.hh): ngroups
+   unknowns over group_start[ngroups] elements; group g holds the elements [group_start[g],
+   group_start[g+1]) of one shape type (its members, e.g. the reflections and rotations the
+   reference generated) and one boundary condition, bc_type / bc_value / bc_flux per group.  A
+   group's integral sums its members in order (Core/include/KElectrostaticElementIntegrator.hh:
+   52-79); its collocation point, normal and area are its first member's.  Matrices, solves and
+   fields then run over ngroups unknowns; element indices of kem_element_eval name groups.
+   The derivative entry points reject grouped meshes. */
+int kem_mesh_create_groups(kem_ctx* ctx, int64_t ngroups, const int64_t* group_start, const int32_t* shape_type,
+                           const double* shape_data, const int32_t* bc_type, const double* bc_value,
+                           const double* bc_flux, kem_mesh** out);
 void kem_mesh_destroy(kem_mesh* mesh);
 int64_t kem_mesh_size(const kem_mesh* mesh);
 /* boundary values b_i (KBoundaryIntegralVector::operator(), BoundaryIntegrals/Core/include/

@@ -72,6 +72,7 @@ SIGNATURES = {
     "kem_last_error": (ctypes.c_char_p, [_P]),
     "kem_set_integrator": (_INT, [_P, ctypes.c_char_p]),
     "kem_mesh_create": (_INT, [_P, _I64, _P, _P, _P, _P, _P, ctypes.POINTER(_P)]),
+    "kem_mesh_create_groups": (_INT, [_P, _I64, _P, _P, _P, _P, _P, _P, ctypes.POINTER(_P)]),
     "kem_mesh_destroy": (None, [_P]),
     "kem_mesh_size": (_I64, [_P]),
     "kem_mesh_rhs": (_INT, [_P, _P]),

@@ -38,6 +38,8 @@ struct kem_ctx {
     int64_t ws_doubles = 0;
     double* scal = nullptr;  // small device scalars (dot products, Hessenberg column)
     int64_t scal_doubles = 0;
+    double* gsig = nullptr;  // charge densities per member of symmetry groups (grow-only)
+    int64_t gsig_doubles = 0;
     void* sort_ws = nullptr;  // point ordering workspace (grow-only)
     size_t sort_ws_bytes = 0;
     // far-kernel timing: one event pair per field evaluation since the last query
@@ -49,7 +51,10 @@ struct kem_ctx {
 
 struct kem_mesh {
     kem_ctx* ctx = nullptr;
-    int64_t n = 0;
+    int64_t n = 0;   // unknowns: elements, or symmetry groups
+    int64_t nm = 0;  // elements the device arrays hold (the members of all groups)
+    int64_t* gstart = nullptr;  // symmetry groups: members [gstart[g], gstart[g+1]) of group g (null: none)
+    int32_t* gof = nullptr;     // and the group of each member
     double* geom = nullptr;
     double* nodes = nullptr;
     double* targets = nullptr;
@@ -118,6 +123,16 @@ hipError_t ensure_ws(kem_ctx* c, int64_t doubles)
     c->ws_doubles = 0;
     hipError_t e = dalloc(&c->ws, doubles);
     if (e == hipSuccess) c->ws_doubles = doubles;
+    return e;
+}
+
+hipError_t ensure_gsig(kem_ctx* c, int64_t doubles)
+{
+    if (c->gsig_doubles >= doubles) return hipSuccess;
+    dfree(c->gsig);
+    c->gsig_doubles = 0;
+    hipError_t e = dalloc(&c->gsig, doubles);
+    if (e == hipSuccess) c->gsig_doubles = doubles;
     return e;
 }
 
@@ -200,6 +215,7 @@ void kem_ctx_destroy(kem_ctx* c)
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     dfree(c->ws);
     dfree(c->scal);
+    dfree(c->gsig);
     if (c->sort_ws) (void)hipFree(c->sort_ws);
     for (hipEvent_t& e : c->ev)
         if (e) (void)hipEventDestroy(e);
@@ -360,19 +376,20 @@ int kem_set_integrator(kem_ctx* c, const char* name)
 }
 
 // ------------------------------------------------------------------ mesh
-int kem_mesh_create(kem_ctx* c, int64_t n, const int32_t* shape, const double* data, const int32_t* bc,
-                    const double* value, const double* flux, kem_mesh** out)
+namespace {
+// A mesh of n unknowns over nm elements: gs (nullable) gives the members [gs[g], gs[g+1]) of
+// each symmetry group; without it every element is its own unknown (nm = n).
+int mesh_build(kem_ctx* c, int64_t n, const int64_t* gs, int64_t nm, const int32_t* shape, const double* data,
+               const int32_t* bc, const double* value, const double* flux, kem_mesh** out)
 {
-    if (!c || !out) return KEM_E_ARG;
-    *out = nullptr;
-    if (n <= 0 || !shape || !data) return fail(c, KEM_E_ARG, "mesh: empty or null arrays");
     KEM_CK(c, enter(c));
-    std::vector<double> g((size_t)n * KG_STRIDE), nd((size_t)n * KEM_NODE_STRIDE), t((size_t)n * KT_STRIDE);
-    std::vector<float> nd32((size_t)n * KEM_NODE32_STRIDE);
+    std::vector<double> g((size_t)nm * KG_STRIDE), nd((size_t)nm * KEM_NODE_STRIDE), t((size_t)n * KT_STRIDE);
+    std::vector<float> nd32((size_t)nm * KEM_NODE32_STRIDE);
     kem_mesh* m = new kem_mesh();
     m->ctx = c;
     m->n = n;
-    m->shape.assign(shape, shape + n);
+    m->nm = nm;
+    m->shape.assign(shape, shape + nm);
     m->bc.assign((size_t)n, 0);
     m->value.assign((size_t)n, 0.);
     m->flux.assign((size_t)n, 0.);
@@ -385,27 +402,32 @@ int kem_mesh_create(kem_ctx* c, int64_t n, const int32_t* shape, const double* d
             return fail(c, KEM_E_ARG, "mesh: boundary type must be 0 (Dirichlet) or 1 (Neumann)");
         }
         if (m->bc[i] == 1) m->has_neumann = true;
+        // the collocation target of a group is its first member (KSymmetryGroup::Centroid /
+        // Normal, Surfaces/include/KSymmetryGroup.hh:48-59)
+        const int64_t f = gs ? gs[i] : i;
+        kem::prep_target(shape[f], data + 11 * f, m->bc[i], m->flux[i], t.data() + (size_t)i * KT_STRIDE);
+    }
+    for (int64_t i = 0; i < nm; i++) {
         if (kem::prep_element(shape[i], data + 11 * i, g.data() + (size_t)i * KG_STRIDE,
                               nd.data() + (size_t)i * KEM_NODE_STRIDE) != 0) {
             delete m;
             return fail(c, KEM_E_GEOM, "mesh: degenerate or non-finite element " + std::to_string(i));
         }
-        kem::prep_target(shape[i], data + 11 * i, m->bc[i], m->flux[i], t.data() + (size_t)i * KT_STRIDE);
         kem::prep_nodes32(g.data() + (size_t)i * KG_STRIDE, nd.data() + (size_t)i * KEM_NODE_STRIDE,
                           nd32.data() + (size_t)i * KEM_NODE32_STRIDE);
     }
-    std::vector<int32_t> order((size_t)n);
-    std::vector<double> clusters((size_t)(4 * ((n + KEM_CLUSTER - 1) / KEM_CLUSTER)));
-    kem::build_clusters(n, g.data(), order.data(), clusters.data());
-    std::vector<float> far6((size_t)n * KEM_FAR6_STRIDE);
+    std::vector<int32_t> order((size_t)nm);
+    std::vector<double> clusters((size_t)(4 * ((nm + KEM_CLUSTER - 1) / KEM_CLUSTER)));
+    kem::build_clusters(nm, g.data(), order.data(), clusters.data());
+    std::vector<float> far6((size_t)nm * KEM_FAR6_STRIDE);
     std::vector<double> far6_cl(clusters.size());
-    kem::build_far6(n, g.data(), nd32.data(), order.data(), clusters.data(), far6.data(), far6_cl.data());
-    std::vector<double> recs((size_t)n * KEM_REC, 0.);
-    for (int64_t i = 0; i < n; i++)
+    kem::build_far6(nm, g.data(), nd32.data(), order.data(), clusters.data(), far6.data(), far6_cl.data());
+    std::vector<double> recs((size_t)nm * KEM_REC, 0.);
+    for (int64_t i = 0; i < nm; i++)
         for (int k = 0; k < 11; k++) recs[(size_t)i * KEM_REC + k] = data[11 * i + k];
     std::vector<double> rule(4 * KEM_NODE_SLOTS, 0.);
     kem::triangle_rule_table(rule.data());
-    for (int64_t i = 0; i < n; i++) {
+    for (int64_t i = 0; i < nm; i++) {
         m->all_triangles = m->all_triangles && shape[i] == KEM_TRIANGLE;
         m->has_axisym = m->has_axisym || shape[i] == KEM_CONICSECTION || shape[i] == KEM_RING;
     }
@@ -415,7 +437,16 @@ int kem_mesh_create(kem_ctx* c, int64_t n, const int32_t* shape, const double* d
     if (e == hipSuccess) e = dalloc(&m->recs, (int64_t)recs.size());
     if (e == hipSuccess) e = hipMemcpy(m->recs, recs.data(), sizeof(double) * recs.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = dalloc(&m->clusters, (int64_t)clusters.size());
-    if (e == hipSuccess) e = dalloc(&m->order, n);
+    if (e == hipSuccess) e = dalloc(&m->order, nm);
+    if (e == hipSuccess && gs) {
+        std::vector<int32_t> gof((size_t)nm);
+        for (int64_t k = 0; k < n; k++)
+            for (int64_t i = gs[k]; i < gs[k + 1]; i++) gof[(size_t)i] = (int32_t)k;
+        e = dalloc(&m->gstart, n + 1);
+        if (e == hipSuccess) e = hipMemcpy(m->gstart, gs, sizeof(int64_t) * (n + 1), hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = dalloc(&m->gof, nm);
+        if (e == hipSuccess) e = hipMemcpy(m->gof, gof.data(), sizeof(int32_t) * nm, hipMemcpyHostToDevice);
+    }
     if (e == hipSuccess)
         e = hipMemcpy(m->clusters, clusters.data(), sizeof(double) * clusters.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(m->order, order.data(), sizeof(int32_t) * order.size(), hipMemcpyHostToDevice);
@@ -439,6 +470,34 @@ int kem_mesh_create(kem_ctx* c, int64_t n, const int32_t* shape, const double* d
     *out = m;
     return KEM_OK;
 }
+}  // namespace
+
+int kem_mesh_create(kem_ctx* c, int64_t n, const int32_t* shape, const double* data, const int32_t* bc,
+                    const double* value, const double* flux, kem_mesh** out)
+{
+    if (!c || !out) return KEM_E_ARG;
+    *out = nullptr;
+    if (n <= 0 || !shape || !data) return fail(c, KEM_E_ARG, "mesh: empty or null arrays");
+    return mesh_build(c, n, nullptr, n, shape, data, bc, value, flux, out);
+}
+
+int kem_mesh_create_groups(kem_ctx* c, int64_t ngroups, const int64_t* group_start, const int32_t* shape,
+                           const double* data, const int32_t* bc, const double* value, const double* flux,
+                           kem_mesh** out)
+{
+    if (!c || !out) return KEM_E_ARG;
+    *out = nullptr;
+    if (ngroups <= 0 || !group_start || !shape || !data) return fail(c, KEM_E_ARG, "mesh: empty or null arrays");
+    if (group_start[0] != 0) return fail(c, KEM_E_ARG, "mesh: group_start[0] must be 0");
+    for (int64_t k = 0; k < ngroups; k++) {
+        if (group_start[k + 1] <= group_start[k]) return fail(c, KEM_E_ARG, "mesh: every group needs a member");
+        for (int64_t i = group_start[k] + 1; i < group_start[k + 1]; i++)
+            if (shape[i] != shape[group_start[k]])
+                return fail(c, KEM_E_ARG, "mesh: the members of a group share one shape type");
+    }
+    if (group_start[ngroups] > INT32_MAX) return fail(c, KEM_E_ARG, "mesh: too many members");
+    return mesh_build(c, ngroups, group_start, group_start[ngroups], shape, data, bc, value, flux, out);
+}
 
 void kem_mesh_destroy(kem_mesh* m)
 {
@@ -454,6 +513,8 @@ void kem_mesh_destroy(kem_mesh* m)
     dfree(m->far6_cl);
     dfree(m->recs);
     dfree(m->rule);
+    dfree(m->gstart);
+    dfree(m->gof);
     delete m;
 }
 
@@ -489,7 +550,8 @@ int kem_element_eval(kem_ctx* c, const kem_mesh* m, int64_t np, const int64_t* e
     if (e == hipSuccess) e = dalloc(&dO, 4 * np);
     if (e == hipSuccess) e = hipMemcpyAsync(de, elem, sizeof(int64_t) * np, hipMemcpyHostToDevice, c->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(dP, P, sizeof(double) * 3 * np, hipMemcpyHostToDevice, c->stream);
-    if (e == hipSuccess) e = kem::launch_pairs(c->stream, c->policy, m->geom, m->nodes, m->recs, np, de, dP, dO, m->has_axisym);
+    if (e == hipSuccess) e = kem::launch_pairs(c->stream, c->policy, m->geom, m->nodes, m->recs, np, de, dP, dO, m->has_axisym,
+                                                m->gstart);
     if (e == hipSuccess) e = hipMemcpyAsync(out, dO, sizeof(double) * 4 * np, hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     dfree(de);
@@ -503,7 +565,7 @@ namespace {
 // field evaluation on device arrays; pset (nullable): per-point charge-density set
 void field_mesh(const kem_mesh* m, kem::FieldMesh& f)
 {
-    f.n = m->n;
+    f.n = m->nm;
     f.geom = m->geom;
     f.nodes = m->nodes;
     f.recs = m->recs;
@@ -525,6 +587,12 @@ int field_eval(kem_ctx* c, const kem_mesh* m, const double* sigma, const int32_t
     field_mesh(m, fm);
     const bool mixed = precision == KEM_MIXED;
     const int64_t range = mixed ? c->far_range_mixed : c->far_range_fp64;
+    if (m->gstart) {
+        // symmetry groups: every member carries its group's charge density
+        KEM_CK(c, ensure_gsig(c, nsets * m->nm));
+        KEM_CK(c, kem::launch_expand_groups(c->stream, m->gof, m->nm, m->n, nsets, sigma, c->gsig));
+        sigma = c->gsig;
+    }
     hipEvent_t t0, t1;
     timing_pair(c, &t0, &t1);
     KEM_CK(c, ensure_ws(c, kem::field_workspace_doubles(c->policy, fm, nsets, npts)));
@@ -575,6 +643,7 @@ int kem_field_graph_create(kem_ctx* c, const kem_mesh* m, const double* sigma, i
         kem::FieldMesh fm;
         field_mesh(m, fm);
         KEM_CK(c, ensure_ws(c, kem::field_workspace_doubles(c->policy, fm, 1, npts)));
+        if (m->gstart) KEM_CK(c, ensure_gsig(c, m->nm));
     }
     if (npts >= kSortMinPoints && npts <= INT32_MAX) {
         const size_t bytes = kem::sort_workspace_bytes(npts);
@@ -659,7 +728,7 @@ int kem_field_census(kem_ctx* c, const kem_mesh* m, int64_t npts, const double* 
     KEM_CK(c, dalloc(&d, 12));
     hipError_t e = hipMemsetAsync(d, 0, sizeof(unsigned long long) * 12, c->stream);
     if (e == hipSuccess)
-        e = kem::launch_census(c->stream, m->geom, m->n, x, y, z, npts, precision == KEM_MIXED ? 1 : 0,
+        e = kem::launch_census(c->stream, m->geom, m->nm, x, y, z, npts, precision == KEM_MIXED ? 1 : 0,
                                c->mixed_fp64_ratio, d);
     unsigned long long h[12];
     if (e == hipSuccess) e = hipMemcpyAsync(h, d, sizeof(h), hipMemcpyDeviceToHost, c->stream);
@@ -729,7 +798,7 @@ int kem_matrix_assemble(kem_ctx* c, const kem_mesh* m, int storage, kem_matrix**
     if (e == hipSuccess && (storage & KEM_STORE_FP32)) e = dalloc(&A->A32, nn);
     if (e == hipSuccess)
         e = kem::launch_assemble(c->stream, c->policy, m->has_neumann, m->geom, m->nodes, m->recs, m->targets, m->n, A->A64,
-                                 A->A32, m->has_axisym);
+                                 A->A32, m->has_axisym, m->gstart);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) {
         kem_matrix_destroy(A);
@@ -1168,7 +1237,8 @@ int vjp_checks(kem_ctx* c, const kem_mesh* m)
 {
     if (m->ctx != c) return fail(c, KEM_E_STATE, "mesh belongs to another context");
     if (c->policy != KEM_POLICY_NUMERIC) return fail(c, KEM_E_STATE, "derivatives need the numeric integrator policy");
-    if (!m->all_triangles) return fail(c, KEM_E_GEOM, "derivatives are implemented for triangles only");
+    if (!m->all_triangles || m->gstart)
+        return fail(c, KEM_E_GEOM, "derivatives are implemented for triangles without symmetry groups only");
     return KEM_OK;
 }
 }  // namespace

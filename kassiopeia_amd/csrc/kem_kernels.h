@@ -69,11 +69,15 @@ hipError_t launch_census(hipStream_t st, const double* geom, int64_t nelem, cons
                          const double* pz, int64_t npts, int mixed, double fp64_ratio, unsigned long long* counts);
 
 hipError_t launch_pairs(hipStream_t st, int policy, const double* geom, const double* nodes, const double* recs,
-                        int64_t npairs, const int64_t* elem, const double* P, double* out, bool axisym = false);
+                        int64_t npairs, const int64_t* elem, const double* P, double* out, bool axisym = false,
+                        const int64_t* gstart = nullptr);
 
 hipError_t launch_assemble(hipStream_t st, int policy, bool field, const double* geom, const double* nodes,
                            const double* recs, const double* targets, int64_t n, double* A64, float* A32,
-                           bool axisym = false);
+                           bool axisym = false, const int64_t* gstart = nullptr);
+// symmetry groups: out[s*nm + i] = sigma[s*n + gof[i]] (each member takes its group's density)
+hipError_t launch_expand_groups(hipStream_t st, const int32_t* gof, int64_t nm, int64_t n, int64_t nsets,
+                                const double* sigma, double* out);
 
 // Launch shapes of the calling thread, set from the context at every C-ABI entry point:
 // target block counts of the split-K matrix-vector product and of the FP32 / FP64 block product.

@@ -455,34 +455,54 @@ template <int PC>
 __global__ __launch_bounds__(kBlock) void k_pairs(const double* __restrict__ geom, const double* __restrict__ nodes,
                                                   const double* __restrict__ recs, int policy, int64_t npairs,
                                                   const int64_t* __restrict__ elem, const double* __restrict__ P,
-                                                  double* __restrict__ out)
+                                                  double* __restrict__ out, const int64_t* __restrict__ gstart)
 {
     const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (k >= npairs) return;
-    const int64_t j = elem[k];
-    const F4 v = PC == 1   ? dev::element_policy(geom + j * KG_STRIDE, recs + j * KEM_REC, policy, P[3 * k],
-                                                 P[3 * k + 1], P[3 * k + 2])
-                 : PC == 2 ? dev::element_any<true>(geom + j * KG_STRIDE, nodes + j * KEM_NODE_STRIDE,
-                                                    recs + j * KEM_REC, policy, P[3 * k], P[3 * k + 1], P[3 * k + 2])
-                           : dev::element_eval<true>(geom + j * KG_STRIDE, nodes + j * KEM_NODE_STRIDE, policy,
-                                                     P[3 * k], P[3 * k + 1], P[3 * k + 2]);
-    out[4 * k + 0] = v.x;
-    out[4 * k + 1] = v.y;
-    out[4 * k + 2] = v.z;
-    out[4 * k + 3] = v.p;
+    // a symmetry group sums its members in order (KElectrostaticElementIntegrator.hh:52-79)
+    const int64_t m0 = gstart ? gstart[elem[k]] : elem[k], m1 = gstart ? gstart[elem[k] + 1] : elem[k] + 1;
+    F4 s{0., 0., 0., 0.};
+    for (int64_t j = m0; j < m1; j++) {
+        const F4 v = PC == 1   ? dev::element_policy(geom + j * KG_STRIDE, recs + j * KEM_REC, policy, P[3 * k],
+                                                     P[3 * k + 1], P[3 * k + 2])
+                     : PC == 2 ? dev::element_any<true>(geom + j * KG_STRIDE, nodes + j * KEM_NODE_STRIDE,
+                                                        recs + j * KEM_REC, policy, P[3 * k], P[3 * k + 1],
+                                                        P[3 * k + 2])
+                               : dev::element_eval<true>(geom + j * KG_STRIDE, nodes + j * KEM_NODE_STRIDE, policy,
+                                                         P[3 * k], P[3 * k + 1], P[3 * k + 2]);
+        s.x += v.x;
+        s.y += v.y;
+        s.z += v.z;
+        s.p += v.p;
+    }
+    out[4 * k + 0] = s.x;
+    out[4 * k + 1] = s.y;
+    out[4 * k + 2] = s.z;
+    out[4 * k + 3] = s.p;
+}
+
+// charge densities per member of symmetry groups: out[s*nm + i] = sigma[s*n + gof[i]]
+__global__ __launch_bounds__(kBlock) void k_expand_groups(const int32_t* __restrict__ gof, int64_t nm, int64_t n,
+                                                          int64_t nsets, const double* __restrict__ sigma,
+                                                          double* __restrict__ out)
+{
+    const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (t >= nm * nsets) return;
+    const int64_t s = t / nm, i = t - s * nm;
+    out[t] = sigma[s * n + gof[i]];
 }
 
 // ------------------------------------------------------------------ assembly
 // Lane = collocation target i (rows), wave-uniform source j (columns).  Column-major storage:
 // A(i, j) at A[j*n + i], so the 64 lanes of a wavefront write 64 consecutive entries.
 // KElectrostaticBoundaryIntegrator::ComputeBoundaryIntegral (Core/include/...hh:260-280).
-template <bool FIELD, int PC>
+template <bool FIELD, int PC, bool GROUPED>
 __global__ __launch_bounds__(kBlock) void k_assemble(const double* __restrict__ geom,
                                                      const double* __restrict__ nodes,
                                                      const double* __restrict__ recs,
                                                      const double* __restrict__ targets, int policy, int64_t n,
                                                      int64_t per_split, double* __restrict__ A64,
-                                                     float* __restrict__ A32)
+                                                     float* __restrict__ A32, const int64_t* __restrict__ gstart)
 {
     int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     const bool live = i < n;
@@ -497,11 +517,25 @@ __global__ __launch_bounds__(kBlock) void k_assemble(const double* __restrict__ 
     int64_t j1 = j0 + per_split;
     if (j1 > n) j1 = n;
     for (int64_t j = j0; j < j1; j++) {
-        const F4 v = PC == 1   ? dev::element_policy(geom + j * KG_STRIDE, recs + j * KEM_REC, policy, cx, cy, cz)
-                     : PC == 2 ? dev::element_any<FIELD>(geom + j * KG_STRIDE, nodes + j * KEM_NODE_STRIDE,
-                                                         recs + j * KEM_REC, policy, cx, cy, cz)
-                               : dev::element_eval<FIELD>(geom + j * KG_STRIDE, nodes + j * KEM_NODE_STRIDE, policy,
-                                                          cx, cy, cz);
+        // column j: element j, or the members of symmetry group j summed in order
+        // (KElectrostaticElementIntegrator.hh:52-79)
+        const int64_t m0 = GROUPED ? gstart[j] : j, m1 = GROUPED ? gstart[j + 1] : j + 1;
+        F4 v{0., 0., 0., 0.};
+        for (int64_t q = m0; q < m1; q++) {
+            const F4 u = PC == 1   ? dev::element_policy(geom + q * KG_STRIDE, recs + q * KEM_REC, policy, cx, cy, cz)
+                         : PC == 2 ? dev::element_any<FIELD>(geom + q * KG_STRIDE, nodes + q * KEM_NODE_STRIDE,
+                                                             recs + q * KEM_REC, policy, cx, cy, cz)
+                                   : dev::element_eval<FIELD>(geom + q * KG_STRIDE, nodes + q * KEM_NODE_STRIDE,
+                                                              policy, cx, cy, cz);
+            if (!GROUPED) {
+                v = u;
+                break;
+            }
+            v.x += u.x;
+            v.y += u.y;
+            v.z += u.z;
+            v.p += u.p;
+        }
         double a;
         if (!FIELD || dirichlet) {
             a = v.p;
@@ -1174,21 +1208,24 @@ hipError_t launch_census(hipStream_t st, const double* geom, int64_t nelem, cons
 }
 
 hipError_t launch_pairs(hipStream_t st, int policy, const double* geom, const double* nodes, const double* recs,
-                        int64_t npairs, const int64_t* elem, const double* P, double* out, bool axisym)
+                        int64_t npairs, const int64_t* elem, const double* P, double* out, bool axisym,
+                        const int64_t* gstart)
 {
     if (npairs <= 0) return hipSuccess;
     if (axisym)
-        k_pairs<2><<<nblocks(npairs), kBlock, 0, st>>>(geom, nodes, recs, policy, npairs, elem, P, out);
+        k_pairs<2><<<nblocks(npairs), kBlock, 0, st>>>(geom, nodes, recs, policy, npairs, elem, P, out, gstart);
     else if (policy >= 2)
-        k_pairs<1><<<nblocks(npairs), kBlock, 0, st>>>(geom, nodes, recs, policy, npairs, elem, P, out);
+        k_pairs<1><<<nblocks(npairs), kBlock, 0, st>>>(geom, nodes, recs, policy, npairs, elem, P, out, gstart);
     else
-        k_pairs<0><<<nblocks(npairs), kBlock, 0, st>>>(geom, nodes, recs, policy, npairs, elem, P, out);
+        k_pairs<0><<<nblocks(npairs), kBlock, 0, st>>>(geom, nodes, recs, policy, npairs, elem, P, out, gstart);
     return hipGetLastError();
 }
 
-hipError_t launch_assemble(hipStream_t st, int policy, bool field, const double* geom, const double* nodes,
-                           const double* recs, const double* targets, int64_t n, double* A64, float* A32,
-                           bool axisym)
+namespace {
+template <bool G>
+hipError_t assemble_kernels(hipStream_t st, int policy, bool field, const double* geom, const double* nodes,
+                            const double* recs, const double* targets, int64_t n, double* A64, float* A32,
+                            bool axisym, const int64_t* gstart)
 {
     if (n <= 0) return hipSuccess;
     const unsigned gx = nblocks(n);
@@ -1199,19 +1236,37 @@ hipError_t launch_assemble(hipStream_t st, int policy, bool field, const double*
     dim3 grid(gx, (unsigned)nsplit);
     if (axisym) {
         if (field)
-            k_assemble<true, 2><<<grid, kBlock, 0, st>>>(geom, nodes, recs, targets, policy, n, per, A64, A32);
+            k_assemble<true, 2, G><<<grid, kBlock, 0, st>>>(geom, nodes, recs, targets, policy, n, per, A64, A32, gstart);
         else
-            k_assemble<false, 2><<<grid, kBlock, 0, st>>>(geom, nodes, recs, targets, policy, n, per, A64, A32);
+            k_assemble<false, 2, G><<<grid, kBlock, 0, st>>>(geom, nodes, recs, targets, policy, n, per, A64, A32, gstart);
     } else if (policy >= 2) {
         if (field)
-            k_assemble<true, 1><<<grid, kBlock, 0, st>>>(geom, nodes, recs, targets, policy, n, per, A64, A32);
+            k_assemble<true, 1, G><<<grid, kBlock, 0, st>>>(geom, nodes, recs, targets, policy, n, per, A64, A32, gstart);
         else
-            k_assemble<false, 1><<<grid, kBlock, 0, st>>>(geom, nodes, recs, targets, policy, n, per, A64, A32);
+            k_assemble<false, 1, G><<<grid, kBlock, 0, st>>>(geom, nodes, recs, targets, policy, n, per, A64, A32, gstart);
     } else if (field) {
-        k_assemble<true, 0><<<grid, kBlock, 0, st>>>(geom, nodes, recs, targets, policy, n, per, A64, A32);
+        k_assemble<true, 0, G><<<grid, kBlock, 0, st>>>(geom, nodes, recs, targets, policy, n, per, A64, A32, gstart);
     } else {
-        k_assemble<false, 0><<<grid, kBlock, 0, st>>>(geom, nodes, recs, targets, policy, n, per, A64, A32);
+        k_assemble<false, 0, G><<<grid, kBlock, 0, st>>>(geom, nodes, recs, targets, policy, n, per, A64, A32, gstart);
     }
+    return hipGetLastError();
+}
+}  // namespace
+
+hipError_t launch_assemble(hipStream_t st, int policy, bool field, const double* geom, const double* nodes,
+                           const double* recs, const double* targets, int64_t n, double* A64, float* A32,
+                           bool axisym, const int64_t* gstart)
+{
+    if (gstart)
+        return assemble_kernels<true>(st, policy, field, geom, nodes, recs, targets, n, A64, A32, axisym, gstart);
+    return assemble_kernels<false>(st, policy, field, geom, nodes, recs, targets, n, A64, A32, axisym, nullptr);
+}
+
+hipError_t launch_expand_groups(hipStream_t st, const int32_t* gof, int64_t nm, int64_t n, int64_t nsets,
+                                const double* sigma, double* out)
+{
+    if (nm <= 0 || nsets <= 0) return hipSuccess;
+    k_expand_groups<<<nblocks(nm * nsets), kBlock, 0, st>>>(gof, nm, n, nsets, sigma, out);
     return hipGetLastError();
 }
 

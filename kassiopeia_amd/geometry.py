@@ -28,21 +28,28 @@ REC = 11
 
 @dataclasses.dataclass
 class Mesh:
-    """A surface container: elements in KSurfaceContainer global order."""
+    """A surface container: elements in KSurfaceContainer global order.  With group_start the
+    unknowns are symmetry groups (KSymmetryGroup): shapes / recs list their members, group g
+    holding members group_start[g] .. group_start[g+1]-1, and the per-unknown arrays (bc_type,
+    bc_value, bc_flux, group) have one entry per group."""
 
-    shapes: np.ndarray  # int32[n]
-    recs: np.ndarray  # float64[n, 11]
+    shapes: np.ndarray  # int32[members]
+    recs: np.ndarray  # float64[members, 11]
     bc_type: np.ndarray  # int32[n]  (0 Dirichlet, 1 Neumann)
     bc_value: np.ndarray  # float64[n]  (Dirichlet voltage)
     bc_flux: np.ndarray  # float64[n]  (Neumann normal boundary flux, eps ratio)
     group: np.ndarray  # int32[n]  (electrode id)
+    group_start: np.ndarray = None  # int64[n+1] (symmetry groups) or None
 
     @property
     def n(self) -> int:
-        return int(self.shapes.shape[0])
+        return int(self.bc_type.shape[0])
+
+    def _first(self, a):
+        return a if self.group_start is None else a[self.group_start[:-1]]
 
     def areas(self) -> np.ndarray:
-        a = np.empty(self.n)
+        a = np.empty(self.shapes.shape[0])
         t = self.shapes == TRIANGLE
         r = self.recs
         cr = np.cross(r[:, 5:8], r[:, 8:11])
@@ -58,11 +65,11 @@ class Mesh:
         a[cs] = np.pi * (r[cs, 0] + r[cs, 3]) * np.sqrt((r[cs, 0] - r[cs, 3]) ** 2 + (r[cs, 2] - r[cs, 5]) ** 2)
         rg = self.shapes == RING
         a[rg] = 2.0 * np.pi * r[rg, 0]
-        return a
+        return self._first(a)  # a group's area is its first member's (KSymmetryGroup.hh:44-47)
 
     def centroids(self) -> np.ndarray:
         r = self.recs
-        c = np.empty((self.n, 3))
+        c = np.empty((self.shapes.shape[0], 3))
         t = self.shapes == TRIANGLE
         c[t] = r[t, 2:5] + (r[t, 0:1] * r[t, 5:8] + r[t, 1:2] * r[t, 8:11]) / 3.0
         q = self.shapes == RECTANGLE
@@ -71,9 +78,11 @@ class Mesh:
         c[ln] = (r[ln, 0:3] + r[ln, 3:6]) * 0.5
         rg = self.shapes == RING
         c[rg] = r[rg, 0:3]
-        return c
+        return self._first(c)
 
     def subset(self, idx) -> "Mesh":
+        if self.group_start is not None:
+            raise ValueError("subset of a mesh with symmetry groups")
         idx = np.asarray(idx)
         return Mesh(self.shapes[idx], self.recs[idx], self.bc_type[idx], self.bc_value[idx],
                     self.bc_flux[idx], self.group[idx])
@@ -116,6 +125,34 @@ def line_records(p0, p1, diameter) -> np.ndarray:
     rec[:, 3:6] = p1
     rec[:, 6] = diameter
     return rec
+
+
+def reflected_records(recs, shapes, point, normal) -> np.ndarray:
+    """Triangle / rectangle records reflected through the plane (point, unit normal): P0 and the
+    side directions N1, N2 mirrored, the side lengths kept (the member a reflection adds to a
+    symmetry group, KSymmetryGroup::AddReflectionThroughPlane)."""
+    recs = np.array(recs, dtype=np.float64, copy=True)
+    if np.any((np.asarray(shapes) != TRIANGLE) & (np.asarray(shapes) != RECTANGLE)):
+        raise ValueError("reflected_records: triangles and rectangles only")
+    o, nrm = np.asarray(point, dtype=np.float64), np.asarray(normal, dtype=np.float64)
+    nrm = nrm / np.linalg.norm(nrm)
+    recs[:, 2:5] -= 2.0 * ((recs[:, 2:5] - o) @ nrm)[:, None] * nrm
+    for k in (5, 8):
+        recs[:, k:k + 3] -= 2.0 * (recs[:, k:k + 3] @ nrm)[:, None] * nrm
+    return recs
+
+
+def mirror_groups(mesh: Mesh, point, normal) -> Mesh:
+    """Each element of `mesh` grouped with its reflection through the plane: a mesh of
+    mesh.n symmetry groups of two members (element, then its mirror image)."""
+    if mesh.group_start is not None:
+        raise ValueError("mirror_groups: the mesh is grouped already")
+    refl = reflected_records(mesh.recs, mesh.shapes, point, normal)
+    n = mesh.n
+    recs = np.empty((2 * n, mesh.recs.shape[1]))
+    recs[0::2], recs[1::2] = mesh.recs, refl
+    return Mesh(np.repeat(mesh.shapes, 2).astype(np.int32), recs, mesh.bc_type.copy(), mesh.bc_value.copy(),
+                mesh.bc_flux.copy(), mesh.group.copy(), np.arange(0, 2 * n + 1, 2, dtype=np.int64))
 
 
 def conic_records(r0, z0, r1, z1) -> np.ndarray:
@@ -195,8 +232,11 @@ def read_triangle_file(path) -> Mesh:
 
 
 def concat(*meshes: Mesh) -> Mesh:
+    """Containers one after another (meshes without symmetry groups)."""
+    if any(m.group_start is not None for m in meshes):
+        raise ValueError("concat of meshes with symmetry groups")
     return Mesh(*[np.concatenate([getattr(m, f.name) for m in meshes])
-                  for f in dataclasses.fields(Mesh)])
+                  for f in dataclasses.fields(Mesh) if f.name != "group_start"])
 
 
 def _quad_grid_triangles(P):

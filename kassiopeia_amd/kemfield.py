@@ -101,8 +101,13 @@ class KSurfaceContainer:
         bc = np.ascontiguousarray(mesh.bc_type, np.int32)
         val = _f64(mesh.bc_value)
         flux = _f64(mesh.bc_flux)
-        ctx.check(ctx.lib.kem_mesh_create(ctx.h, mesh.n, _ptr(shapes), _ptr(recs), _ptr(bc), _ptr(val), _ptr(flux),
-                                          ctypes.byref(h)))
+        if mesh.group_start is None:
+            ctx.check(ctx.lib.kem_mesh_create(ctx.h, mesh.n, _ptr(shapes), _ptr(recs), _ptr(bc), _ptr(val),
+                                              _ptr(flux), ctypes.byref(h)))
+        else:
+            gs = np.ascontiguousarray(mesh.group_start, np.int64)
+            ctx.check(ctx.lib.kem_mesh_create_groups(ctx.h, mesh.n, _ptr(gs), _ptr(shapes), _ptr(recs), _ptr(bc),
+                                                     _ptr(val), _ptr(flux), ctypes.byref(h)))
         self.h = h
         ctx.track(self)
 

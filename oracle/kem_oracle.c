@@ -1635,6 +1635,76 @@ void ko_field_mt(int nthreads, int policy, int64_t n, const int32_t* shape, cons
         field_point(policy, n, shape, recs, sigma, P + 3 * p, phi ? phi + p : 0, E ? E + 3 * p : 0);
 }
 
+/* ------------------------------------------------------ symmetry groups */
+/* A group's integral: its members in order, plainly summed (Core/include/
+   KElectrostaticElementIntegrator.hh:52-79); members [gs[g], gs[g+1]). */
+static void group_eval(int policy, const int64_t* gs, int64_t g, const int32_t* shape, const double* recs,
+                       const double P[3], double out[4])
+{
+    out[0] = out[1] = out[2] = out[3] = 0.;
+    for (int64_t m = gs[g]; m < gs[g + 1]; m++) {
+        double v[4];
+        ko_element_eval(policy, shape[m], recs + KO_REC * m, P, v);
+        for (int k = 0; k < 4; k++) out[k] += v[k];
+    }
+}
+
+/* collocation with groups as sources and targets: the target is the group's first member
+   (KSymmetryGroup::Centroid / Normal, Surfaces/include/KSymmetryGroup.hh:48-59), the Neumann
+   self term compares the groups' (first members') centroids as for single elements
+   (Core/include/KElectrostaticBoundaryIntegrator.hh:260-280) */
+void ko_assemble_groups(int policy, int64_t n, const int64_t* gs, const int32_t* shape, const double* recs,
+                        const int32_t* bc, const double* flux, double* A)
+{
+    for (int64_t i = 0; i < n; i++) {
+        const int64_t fi = gs[i];
+        double ci[3], ni[3];
+        ko_centroid(shape[fi], recs + KO_REC * fi, ci);
+        ko_normal(shape[fi], recs + KO_REC * fi, ni);
+        for (int64_t j = 0; j < n; j++) {
+            double out[4];
+            if (bc == NULL || bc[i] == KO_DIRICHLET) {
+                group_eval(policy, gs, j, shape, recs, ci, out);
+                A[i * n + j] = out[3];
+                continue;
+            }
+            double cj[3];
+            ko_centroid(shape[gs[j]], recs + KO_REC * gs[j], cj);
+            double dd[3] = {cj[0] - ci[0], cj[1] - ci[1], cj[2] - ci[2]};
+            if (norm3(dd) >= 1.e-12) {
+                group_eval(policy, gs, j, shape, recs, ci, out);
+                A[i * n + j] = out[0] * ni[0] + out[1] * ni[1] + out[2] * ni[2];
+            } else {
+                const double f = flux ? flux[i] : 0.;
+                A[i * n + j] = ((1. + f) / (1. - f)) / (2. * KO_EPS0);
+            }
+        }
+    }
+}
+
+/* Kahan-summed field over groups, sigma per group (KElectrostaticIntegratingFieldSolver.hh:262-339
+   with the group integral above) */
+void ko_field_groups(int policy, int64_t n, const int64_t* gs, const int32_t* shape, const double* recs,
+                     const double* sigma, int64_t m, const double* P, double* phi, double* E)
+{
+    for (int64_t p = 0; p < m; p++) {
+        double s[4] = {0., 0., 0., 0.}, c[4] = {0., 0., 0., 0.};
+        for (int64_t j = 0; j < n; j++) {
+            double out[4];
+            group_eval(policy, gs, j, shape, recs, P + 3 * p, out);
+            for (int k = 0; k < 4; k++) {
+                double y = out[k] * sigma[j] - c[k];
+                double t = s[k] + y;
+                c[k] = (t - s[k]) - y;
+                s[k] = t;
+            }
+        }
+        if (phi) phi[p] = s[3];
+        if (E)
+            for (int k = 0; k < 3; k++) E[3 * p + k] = s[k];
+    }
+}
+
 /* ------------------------------------------------------------ solvers */
 static double dotn(int64_t n, const double* a, const double* b)
 {

@@ -97,6 +97,13 @@ double ko_ell_EK(double k);
 void ko_ring(const double* rec, const double P[3], double out[4]);
 void ko_conic(const double* rec, const double P[3], double out[4]);
 
+/* symmetry groups (KSymmetryGroup<Shape>): n groups, members [gs[g], gs[g+1]) of the records;
+   a group's integral sums its members, its target is its first member */
+void ko_assemble_groups(int policy, int64_t n, const int64_t* gs, const int32_t* shape, const double* recs,
+                        const int32_t* bc, const double* flux, double* A);
+void ko_field_groups(int policy, int64_t n, const int64_t* gs, const int32_t* shape, const double* recs,
+                     const double* sigma, int64_t m, const double* P, double* phi, double* E);
+
 /* policy dispatch over shapes: KElectrostaticBoundaryIntegratorFactory.cc:29-88 */
 int ko_element_eval(int policy, int shape, const double* rec, const double P[3], double out[4]);
 

@@ -66,6 +66,8 @@ def _load():
         "ko_gmres_ortho": (ctypes.c_int, [i64, P, P, P, d, ctypes.c_int, ctypes.c_int, ctypes.c_int, P, P]),
         "ko_robin_hood": (i64, [i64, P, P, P, d, i64, i64, P]),
         "ko_gauss": (ctypes.c_int, [i64, P, P, P]),
+        "ko_assemble_groups": (None, [ctypes.c_int, i64, P, P, P, P, P, P]),
+        "ko_field_groups": (None, [ctypes.c_int, i64, P, P, P, P, i64, P, P, P]),
         "ko_ell_K": (d, [d]),
         "ko_ell_E": (d, [d]),
         "ko_ell_EK": (d, [d]),
@@ -246,6 +248,32 @@ def assemble(shapes, recs, bc=None, flux=None, policy=NUMERIC):
     flp = None if flux is None else _p(_f64(flux))
     lib().ko_assemble(policy, n, _p(shapes), _p(recs), bcp, flp, _p(A))
     return A
+
+
+def assemble_groups(group_start, shapes, recs, bc=None, flux=None, policy=NUMERIC):
+    """Collocation matrix over symmetry groups (members group_start[g] .. group_start[g+1]-1)."""
+    gs = np.ascontiguousarray(group_start, dtype=np.int64)
+    shapes = np.ascontiguousarray(shapes, dtype=np.int32)
+    recs = _recs(recs)
+    n = len(gs) - 1
+    A = np.zeros((n, n))
+    bcp = None if bc is None else _p(np.ascontiguousarray(bc, dtype=np.int32))
+    flp = None if flux is None else _p(_f64(flux))
+    lib().ko_assemble_groups(policy, n, _p(gs), _p(shapes), _p(recs), bcp, flp, _p(A))
+    return A
+
+
+def field_groups(group_start, shapes, recs, sigma, points, policy=NUMERIC):
+    gs = np.ascontiguousarray(group_start, dtype=np.int64)
+    shapes = np.ascontiguousarray(shapes, dtype=np.int32)
+    recs = _recs(recs)
+    sigma = _f64(sigma)
+    pts = _f64(points).reshape(-1, 3)
+    m = pts.shape[0]
+    phi = np.zeros(m)
+    E = np.zeros((m, 3))
+    lib().ko_field_groups(policy, len(gs) - 1, _p(gs), _p(shapes), _p(recs), _p(sigma), m, _p(pts), _p(phi), _p(E))
+    return phi, E
 
 
 def matrix_element(i, j, shapes, recs, bc=None, flux=None, policy=NUMERIC):
