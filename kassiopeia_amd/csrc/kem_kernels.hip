@@ -222,9 +222,10 @@ __device__ __forceinline__ double wave_max(double v)
 // proven to contain none.  Writes its own sums (one slice of the partial-sum layout that
 // k_field_combine adds), so it may run beside the far pass.  Blocks stride over the point
 // blocks, so a small grid keeps the pass's footprint to a few wavefronts per SIMD beside the
-// far kernel.
+// far kernel.  Capped at 168 VGPRs (3 wavefronts per SIMD, 16 bytes of spills): uncapped it
+// takes 173 and runs 2 per SIMD (4 M points in uneven batches 253 -> 247 ms with the cap).
 template <bool MULTI, bool AXI>
-__global__ __launch_bounds__(kBlock) void k_field_near_culled(const double* __restrict__ geom,
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) void k_field_near_culled(const double* __restrict__ geom,
                                                               const double* __restrict__ sigma,
                                                               const int32_t* __restrict__ pset, int64_t nsets,
                                                               const double* __restrict__ clusters, int64_t ncl,
