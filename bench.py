@@ -76,7 +76,7 @@ def main():
                     help="only copy 1 GiB device to device (reference kernel for the rocprofv3 byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported", file=sys.stderr)
+        print(f"bench.py: the library runs on one GPU; --gpus {args.gpus} is not supported", file=sys.stderr)
         sys.exit(2)
     if args.copy_calibration:
         return copy_calibration()
@@ -379,7 +379,7 @@ def main():
                    "elements": N, "points": M},
         # the far-field kernel (the dominant one) is bound by the vector ALU (no matrix-shaped
         # work); the peak is MI355X's published figure of the arithmetic type: FP64 78.6 TFLOP/s,
-        # FP32 157.3 TFLOP/s (the FP32 figure needs packed FP32, which this build does not use).
+        # FP32 157.3 TFLOP/s (the FP32 figure needs packed FP32, which the library's kernels do not use).
         # achieved = census FLOPs of the far pairs / far-kernel time from HIP events on the
         # library's stream over the timed steps
         "roofline": {"bound": "mfma", "compute": "vector ALU (" + ("fp32" if mixed_ok else "fp64") + ")",
