@@ -467,6 +467,28 @@ __device__ __forceinline__ void element_far32(const double* __restrict__ g, cons
     return cubature32<33>(nd + 4 * KEM_N33, dx, dy, dz, s0, s1, s2, s3);
 }
 
+// element_far32 with one rule for the whole wavefront (all active lanes evaluate the same
+// element): the highest-degree rule the per-pair ladder picks for any active lane.  Lanes whose
+// ladder would pick a lower rule get one of higher degree, whose error at their distance is
+// below the lower rule's; divergent lanes would instead run every picked rule in turn.
+__device__ __forceinline__ void element_far32_wave(const double* __restrict__ g, const float* __restrict__ nd,
+                                                   int type, double r2, float dx, float dy, float dz, float& s0,
+                                                   float& s1, float& s2, float& s3)
+{
+    if (type == KEM_LINESEGMENT) {
+        if (__builtin_amdgcn_ballot_w64(!(r2 > g[KG_MIX7_2])) == 0)
+            return cubature32<4>(nd + 4 * KEM_L4, dx, dy, dz, s0, s1, s2, s3);
+        return cubature32<16>(nd + 4 * KEM_L16, dx, dy, dz, s0, s1, s2, s3);
+    }
+    if (__builtin_amdgcn_ballot_w64(!(r2 > g[KG_MIX12_2])) != 0)
+        return cubature32<33>(nd + 4 * KEM_N33, dx, dy, dz, s0, s1, s2, s3);
+    if (__builtin_amdgcn_ballot_w64(!(r2 > g[KG_MIX7_2])) != 0)
+        return cubature32<12>(nd + 4 * KEM_N12, dx, dy, dz, s0, s1, s2, s3);
+    if (__builtin_amdgcn_ballot_w64(!(r2 > g[KG_MIX6_2])) != 0)
+        return cubature32<7>(nd + 4 * KEM_N7, dx, dy, dz, s0, s1, s2, s3);
+    return cubature32<6>(nd + 4 * KEM_N6, dx, dy, dz, s0, s1, s2, s3);
+}
+
 // Kahan accumulation of sigma * v into (s, c), component-wise, as the reference sums
 // (KElectrostaticIntegratingFieldSolver.hh:325-333)
 __device__ __forceinline__ void kahan_add(double v, double sj, double& s, double& c)

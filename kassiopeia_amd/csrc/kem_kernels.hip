@@ -335,8 +335,9 @@ __global__ __launch_bounds__(kBlock) void k_far_weights(const double* __restrict
 // sphere; an element whose 6-point reach (KEM_FAR6_REACH) lies beyond that sphere is a 6-point
 // pair for every lane -- the per-pair ladder of k_field_far32 would pick the same rule -- so
 // the wavefront evaluates it from the element's far record without per-lane regime tests; a
-// cluster beyond its own reach skips even the per-element test.  Every other element takes
-// k_field_far32's per-pair path.  A 6-point pair evaluates P - node as (P - cluster centre)
+// cluster beyond its own reach skips even the per-element test.  Every other element skips the
+// near lanes and evaluates the highest rule that k_field_far32's per-pair ladder picks for any
+// far lane, once for all of them (dev::element_far32_wave).  A 6-point pair evaluates P - node as (P - cluster centre)
 // rounded to FP32 once per cluster minus the node's FP32 offset from the cluster centre (the
 // per-pair path rounds P - element centre per pair); the FP64 sums run over elements in
 // cluster order.  Elements
@@ -411,8 +412,8 @@ __global__ __launch_bounds__(kBlock) void k_field_far32c(const double* __restric
                 const double r2 = dev::dist2(g, x, y, z);
                 if (dev::is_near2(g, r2, 0)) continue;
                 const float ex = (float)(x - g[KG_CX]), ey = (float)(y - g[KG_CY]), ez = (float)(z - g[KG_CZ]);
-                dev::element_far32(g, nodes32 + j * KEM_NODE32_STRIDE, (int)g[KG_TYPE], r2, ex, ey, ez, s0, s1, s2,
-                                   s3);
+                dev::element_far32_wave(g, nodes32 + j * KEM_NODE32_STRIDE, (int)g[KG_TYPE], r2, ex, ey, ez, s0,
+                                        s1, s2, s3);
             }
             a0 += (double)s0 * wj;
             a1 += (double)s1 * wj;
