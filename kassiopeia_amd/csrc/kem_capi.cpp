@@ -66,7 +66,8 @@ struct kem_mesh {
     double* clusters = nullptr;  // near-pass element clusters (centre, radius)
     int32_t* order = nullptr;    // elements in cluster order
     float* far6 = nullptr;       // mixed far field: 6-point records in cluster order
-    double* far6_cl = nullptr;   // and the clusters' 6-point reach
+    float* far3 = nullptr;       // 3-point records in cluster order
+    double* far6_cl = nullptr;   // and the clusters' 6- and 3-point reach
     std::vector<int32_t> shape, bc;
     std::vector<double> value, flux;
     bool has_neumann = false;
@@ -419,9 +420,10 @@ int mesh_build(kem_ctx* c, int64_t n, const int64_t* gs, int64_t nm, const int32
     std::vector<int32_t> order((size_t)nm);
     std::vector<double> clusters((size_t)(4 * ((nm + KEM_CLUSTER - 1) / KEM_CLUSTER)));
     kem::build_clusters(nm, g.data(), order.data(), clusters.data());
-    std::vector<float> far6((size_t)nm * KEM_FAR6_STRIDE);
-    std::vector<double> far6_cl(clusters.size());
-    kem::build_far6(nm, g.data(), nd32.data(), order.data(), clusters.data(), far6.data(), far6_cl.data());
+    std::vector<float> far6((size_t)nm * KEM_FAR6_STRIDE), far3((size_t)nm * KEM_FAR3_STRIDE);
+    std::vector<double> far6_cl(clusters.size() / 4 * KEM_FAR_CL_STRIDE);
+    kem::build_far6(nm, g.data(), nd32.data(), order.data(), clusters.data(), far6.data(), far3.data(),
+                    far6_cl.data());
     std::vector<double> recs((size_t)nm * KEM_REC, 0.);
     for (int64_t i = 0; i < nm; i++)
         for (int k = 0; k < 11; k++) recs[(size_t)i * KEM_REC + k] = data[11 * i + k];
@@ -452,6 +454,8 @@ int mesh_build(kem_ctx* c, int64_t n, const int64_t* gs, int64_t nm, const int32
     if (e == hipSuccess) e = hipMemcpy(m->order, order.data(), sizeof(int32_t) * order.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = dalloc(&m->far6, (int64_t)far6.size());
     if (e == hipSuccess) e = hipMemcpy(m->far6, far6.data(), sizeof(float) * far6.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = dalloc(&m->far3, (int64_t)far3.size());
+    if (e == hipSuccess) e = hipMemcpy(m->far3, far3.data(), sizeof(float) * far3.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = dalloc(&m->far6_cl, (int64_t)far6_cl.size());
     if (e == hipSuccess)
         e = hipMemcpy(m->far6_cl, far6_cl.data(), sizeof(double) * far6_cl.size(), hipMemcpyHostToDevice);
@@ -510,6 +514,7 @@ void kem_mesh_destroy(kem_mesh* m)
     dfree(m->clusters);
     dfree(m->order);
     dfree(m->far6);
+    dfree(m->far3);
     dfree(m->far6_cl);
     dfree(m->recs);
     dfree(m->rule);
@@ -573,6 +578,7 @@ void field_mesh(const kem_mesh* m, kem::FieldMesh& f)
     f.clusters = m->clusters;
     f.order = m->order;
     f.far6 = m->far6;
+    f.far3 = m->far3;
     f.far6_cl = m->far6_cl;
     f.axisym = m->has_axisym;
 }

@@ -18,7 +18,8 @@ struct FieldMesh {
     const double* clusters = nullptr;  // element clusters (kem::build_clusters); null: element scan
     const int32_t* order = nullptr;    // elements in cluster order
     const float* far6 = nullptr;       // 6-point far records in cluster order (kem::build_far6)
-    const double* far6_cl = nullptr;   // and the clusters' 6-point reach
+    const float* far3 = nullptr;       // 3-point far records in cluster order
+    const double* far6_cl = nullptr;   // and the clusters' 6- and 3-point reach (KEM_FAR_CL_STRIDE)
     bool axisym = false;               // holds conic sections or rings (kernels with their integrators)
 };
 

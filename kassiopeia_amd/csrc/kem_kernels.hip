@@ -297,6 +297,7 @@ __device__ __forceinline__ double f2d(float lo, float hi)
 
 // weights of the degree-4 6-point rule's two orbits as the FP32 node table stores them
 constexpr float kW6a = (float)0.223381589678011, kW6b = (float)0.109951743655322;
+constexpr float kW3 = 1.f / 3.f;  // the degree-2 3-point rule (interior nodes)
 
 // one FP32 quadrature node at offset (ex, ey, ez) from the point, weight w (dev::cubature32)
 __device__ __forceinline__ void node32(float ex, float ey, float ez, float w, float& s0, float& s1, float& s2,
@@ -335,7 +336,8 @@ __global__ __launch_bounds__(kBlock) void k_far_weights(const double* __restrict
 // sphere; an element whose 6-point reach (KEM_FAR6_REACH) lies beyond that sphere is a 6-point
 // pair for every lane -- the per-pair ladder of k_field_far32 would pick the same rule -- so
 // the wavefront evaluates it from the element's far record without per-lane regime tests; a
-// cluster beyond its own reach skips even the per-element test.  Every other element skips the
+// cluster beyond its own reach skips even the per-element test, and a cluster beyond its 3-point
+// reach (KEM_FAR3_RATIO sizes of each element) takes the 3-point records.  Every other element skips the
 // near lanes and evaluates the highest rule that k_field_far32's per-pair ladder picks for any
 // far lane, once for all of them (dev::element_far32_wave).  A 6-point pair evaluates P - node as (P - cluster centre)
 // rounded to FP32 once per cluster minus the node's FP32 offset from the cluster centre (the
@@ -346,6 +348,7 @@ template <bool MULTI>
 __global__ __launch_bounds__(kBlock) void k_field_far32c(const double* __restrict__ geom,
                                                          const float* __restrict__ nodes32,
                                                          const float* __restrict__ far6,
+                                                         const float* __restrict__ far3,
                                                          const double* __restrict__ far6_cl,
                                                          const int32_t* __restrict__ order,
                                                          const double* __restrict__ wk,
@@ -376,15 +379,33 @@ __global__ __launch_bounds__(kBlock) void k_field_far32c(const double* __restric
     if (c1 > ncl) c1 = ncl;
     double a0 = 0., a1 = 0., a2 = 0., a3 = 0.;
     for (int64_t c = c0; c < c1; c++) {
-        const double* cl = far6_cl + 4 * c;
+        const double* cl = far6_cl + KEM_FAR_CL_STRIDE * c;
         const double qx = wx - cl[0], qy = wy - cl[1], qz = wz - cl[2];
-        const double tc = cl[3] + wr;
-        const bool all6 = qx * qx + qy * qy + qz * qz > tc * tc;
+        const double q2 = qx * qx + qy * qy + qz * qz;
+        const double tc = cl[3] + wr, t3 = cl[4] + wr;
+        const bool all6 = q2 > tc * tc;
         // P - cluster centre, rounded to FP32 once per cluster; the far records hold the nodes
         // relative to the cluster centre
         const float dx = (float)(x - cl[0]), dy = (float)(y - cl[1]), dz = (float)(z - cl[2]);
         int64_t k1 = (c + 1) * KEM_CLUSTER;
         if (k1 > nelem) k1 = nelem;
+        if (q2 > t3 * t3) {
+            // every element of the cluster lies beyond KEM_FAR3_RATIO sizes of every lane: the
+            // degree-2 3-point rule
+            for (int64_t k = c * KEM_CLUSTER; k < k1; k++) {
+                const v16f R = *reinterpret_cast<const v16f*>(far3 + k * KEM_FAR3_STRIDE);
+                const double wj = wl[MULTI ? k * nsets : k];
+                float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f;
+                node32(dx - R[0], dy - R[1], dz - R[2], kW3, s0, s1, s2, s3);
+                node32(dx - R[4], dy - R[5], dz - R[6], kW3, s0, s1, s2, s3);
+                node32(dx - R[8], dy - R[9], dz - R[10], kW3, s0, s1, s2, s3);
+                a0 += (double)s0 * wj;
+                a1 += (double)s1 * wj;
+                a2 += (double)s2 * wj;
+                a3 += (double)s3 * wj;
+            }
+            continue;
+        }
         for (int64_t k = c * KEM_CLUSTER; k < k1; k++) {
             // the whole far record in two 16-dword loads: nodes 0-3, then nodes 4-5, centre, reach
             const v16f* rv = reinterpret_cast<const v16f*>(far6 + k * KEM_FAR6_STRIDE);
@@ -1054,7 +1075,7 @@ void far_kernel(hipStream_t st, dim3 grid, int kind, const FieldMesh& m, double 
     const int64_t nelem = m.n;
     if (kind == 2) {
         const int64_t ncl = (nelem + KEM_CLUSTER - 1) / KEM_CLUSTER;
-        k_field_far32c<MULTI><<<grid, kBlock, 0, st>>>(m.geom, m.nodes32, m.far6, m.far6_cl, m.order, wk, pset, nsets,
+        k_field_far32c<MULTI><<<grid, kBlock, 0, st>>>(m.geom, m.nodes32, m.far6, m.far3, m.far6_cl, m.order, wk, pset, nsets,
                                                        nelem, ncl, per / KEM_CLUSTER, px, py, pz, npts, o0, o1, o2,
                                                        o3, stride, xcd_pb);
     } else if (kind == 1) {
@@ -1076,7 +1097,8 @@ void field_passes(hipStream_t st, int policy, const FieldMesh& m, bool mixed, do
     const unsigned gx = nblocks(npts);
     // ws: the cluster pass's element weights, then (several sets) the sets transposed to
     // element-major order, then the partial sums of split element ranges
-    const bool clustered = policy == 0 && mixed && m.nodes32 && fp64_ratio == 0. && m.far6 && m.far6_cl && m.order;
+    const bool clustered = policy == 0 && mixed && m.nodes32 && fp64_ratio == 0. && m.far6 && m.far3 && m.far6_cl &&
+                           m.order;
     const int kind = clustered ? 2 : (policy == 0 && mixed && m.nodes32) ? 1 : 0;
     const int64_t nsig = (nsets * nelem + 31) / 32 * 32;
     double* wk = ws;

@@ -90,6 +90,19 @@ enum kem_geom_field {
 #define KEM_FAR6_CENTRE 24
 #define KEM_FAR6_REACH 30
 
+/* 3-point far records, one per element in cluster order (16 floats = 64 B): the degree-2 rule
+   with nodes at barycentric (2/3, 1/6, 1/6) and its permutations, weights 1/3, as (x, y, z, 0)
+   FP32 offsets from the cluster centre (computed in FP64, rounded once), for triangles beyond
+   KEM_FAR3_RATIO sizes, where its truncation (<= 3.0e-8 relative, measured) is on the level of
+   the mixed ladder's other rules.  Zero records for other shapes. */
+#define KEM_FAR3_STRIDE 16
+#define KEM_FAR3_RATIO 80.0
+
+/* per-cluster far data (doubles): centre x, y, z, 6-point reach, 3-point reach (the largest
+   |c_j - centre| + KEM_FAR3_RATIO size_j over the cluster, infinite when it holds any other
+   shape than triangles), 3 pad */
+#define KEM_FAR_CL_STRIDE 8
+
 /* collocation target record (one per element): centre, unit normal, boundary type, flux */
 enum kem_target_field { KT_CX = 0, KT_CY, KT_CZ, KT_NX, KT_NY, KT_NZ, KT_BC, KT_FLUX, KT_STRIDE = 8 };
 

@@ -542,12 +542,12 @@ void build_clusters(int64_t n, const double* g, int32_t* order, double* clusters
 }
 
 void build_far6(int64_t n, const double* g, const float* nodes32, const int32_t* order, const double* clusters,
-                float* far6, double* far6_cl)
+                float* far6, float* far3, double* far_cl)
 {
     const int64_t ncl = (n + KEM_CLUSTER - 1) / KEM_CLUSTER;
     for (int64_t c = 0; c < ncl; c++) {
         const int64_t e0 = c * KEM_CLUSTER, e1 = std::min(n, e0 + KEM_CLUSTER);
-        double reach = 0.;
+        double reach = 0., reach3 = 0.;
         for (int64_t e = e0; e < e1; e++) {
             const int64_t j = order[e];
             const double* ge = g + j * (int64_t)KG_STRIDE;
@@ -568,9 +568,27 @@ void build_far6(int64_t n, const double* g, const float* nodes32, const int32_t*
             const double d = std::sqrt(sq(ce[0] - clusters[4 * c]) + sq(ce[1] - clusters[4 * c + 1]) +
                                        sq(ce[2] - clusters[4 * c + 2]));
             reach = std::max(reach, d + ce[3]);
+            // the 3-point rule: nodes (2 V_q + (V_q+1 + V_q+2) / 2) / 3 relative to the cluster centre
+            float* r3 = far3 + e * KEM_FAR3_STRIDE;
+            for (int k = 0; k < KEM_FAR3_STRIDE; k++) r3[k] = 0.f;
+            if ((int)ge[KG_TYPE] == KEM_TRIANGLE) {
+                const double* v = ge + KG_V;
+                for (int q = 0; q < 3; q++)
+                    for (int k = 0; k < 3; k++) {
+                        const double x = (2. / 3.) * v[3 * q + k] + (1. / 6.) * v[3 * ((q + 1) % 3) + k] +
+                                         (1. / 6.) * v[3 * ((q + 2) % 3) + k];
+                        r3[4 * q + k] = (float)(x - clusters[4 * c + k]);
+                    }
+                reach3 = std::max(reach3, d + KEM_FAR3_RATIO / ge[KG_INVSIZE] * (1. + 1e-9));
+            } else {
+                reach3 = INFINITY;
+            }
         }
-        for (int k = 0; k < 3; k++) far6_cl[4 * c + k] = clusters[4 * c + k];
-        far6_cl[4 * c + 3] = reach * (1. + 1e-12);
+        double* cl = far_cl + KEM_FAR_CL_STRIDE * c;
+        for (int k = 0; k < KEM_FAR_CL_STRIDE; k++) cl[k] = 0.;
+        for (int k = 0; k < 3; k++) cl[k] = clusters[4 * c + k];
+        cl[3] = reach * (1. + 1e-12);
+        cl[4] = reach3 * (1. + 1e-12);
     }
 }
 

@@ -37,11 +37,13 @@ void prep_target(int type, const double* shape_data, int bc, double flux, double
 // every point within the near distance of a member lies within R of the centre.
 void build_clusters(int64_t n, const double* g, int32_t* order, double* clusters);
 
-// Far records of the mixed far field in cluster order (layout: KEM_FAR6_* in kem_layout.h):
-// far6[KEM_FAR6_STRIDE * n] floats from the FP32 node table nodes32 and the element records g;
-// far6_cl[4 * ceil(n / KEM_CLUSTER)] doubles with the clusters' centres taken from clusters.
+// Far records of the mixed far field in cluster order (layout: KEM_FAR6_*, KEM_FAR3_* and
+// KEM_FAR_CL_STRIDE in kem_layout.h): far6[KEM_FAR6_STRIDE * n] floats from the FP32 node table
+// nodes32 and the element records g, far3[KEM_FAR3_STRIDE * n] floats from the vertices in g,
+// far_cl[KEM_FAR_CL_STRIDE * ceil(n / KEM_CLUSTER)] doubles with the clusters' centres taken from
+// clusters.
 void build_far6(int64_t n, const double* g, const float* nodes32, const int32_t* order, const double* clusters,
-                float* far6, double* far6_cl);
+                float* far6, float* far3, double* far_cl);
 
 }  // namespace kem
 
