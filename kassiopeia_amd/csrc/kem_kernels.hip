@@ -600,10 +600,10 @@ __global__ __launch_bounds__(kBlock) void k_gemv_part(const T* __restrict__ A, i
 }
 
 // y[i] = alpha * sum_s part[s][i] + beta * y[i]
-// Same as k_gemv_part with R consecutive rows per thread read as one 16-byte vector per column
-// (used for FP32 matrices, R = 4; needs lda and A 16-byte aligned in R-row units and
-// nrows % R == 0).  Measured on the 20,000-unknown FP32 matrix: 5.7 TB/s against 5.3 TB/s
-// for one row per thread; FP64 matrices stay with k_gemv_part (5.9 TB/s either way).
+// Same as k_gemv_part with R consecutive rows per thread read as one vector per column (used for
+// FP32 matrices, R = 2: one 8-byte vector; needs lda and A aligned in R-row units and
+// nrows % R == 0).  Measured on the 20,000-unknown FP32 matrix: 5.4 TB/s with R = 2 against
+// 4.9-5.1 TB/s with R = 4 (16-byte vectors); FP64 matrices stay with k_gemv_part (5.9 TB/s).
 template <typename T, int R>
 __global__ __launch_bounds__(kBlock) void k_gemv_part_vec(const T* __restrict__ A, int64_t lda, int64_t nrows,
                                                           int64_t ncols, int64_t per_split,
