@@ -46,8 +46,8 @@ CPU_ORACLE_OVER_REFERENCE = 3.68 / 2.95
 # 1,000,000 points): rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE of one dispatch, scaled by the
 # readings of a 1 GiB device copy (bench.py --copy-calibration: FETCH_SIZE x 1.99990, WRITE_SIZE
 # x 1.0), both element ranges at their default of 2,560 elements.
-TRAFFIC_BYTES = {"k_field_far32c": (149954.6875 * 1.99990 + 250000.0) * 1024,
-                 "k_field_far": (1216095.875 * 1.99990 + 250000.0) * 1024}
+TRAFFIC_BYTES = {"k_field_far32c": (149889.4375 * 1.99990 + 250000.0) * 1024,
+                 "k_field_far": (1223041.5 * 1.99990 + 250000.0) * 1024}
 
 
 def log(*a):
