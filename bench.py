@@ -61,7 +61,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--points", type=int, default=1_000_000)
     ap.add_argument("--cpu-points", type=int, default=0,
-                    help="points of the bounded CPU-baseline sample (x 20,000 elements); 0: 1500 per thread")
+                    help="points of the bounded CPU-baseline sample (x 20,000 elements); 0: 4000 per thread")
     ap.add_argument("--cpu-threads", type=int, default=0,
                     help="OpenMP threads of the CPU baseline; 0: the host's cores, at most 16")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -341,7 +341,7 @@ def main():
         from oracle import oracle as OR
 
         threads = args.cpu_threads or max(1, min(16, os.cpu_count() or 1))
-        k = min(args.cpu_points or 1500 * threads, M)
+        k = min(args.cpu_points or 4000 * threads, M)
         t0 = time.perf_counter()
         phr, Er = OR.field(mesh.shapes, mesh.recs, sigma, pts[:k], threads=threads)
         tc = time.perf_counter() - t0
