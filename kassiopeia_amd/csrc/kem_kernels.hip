@@ -389,9 +389,10 @@ __global__ __launch_bounds__(kBlock) void k_field_far32c(const double* __restric
         const float dx = (float)(x - cl[0]), dy = (float)(y - cl[1]), dz = (float)(z - cl[2]);
         int64_t k1 = (c + 1) * KEM_CLUSTER;
         if (k1 > nelem) k1 = nelem;
-        if (q2 > t3 * t3) {
+        if (!MULTI && q2 > t3 * t3) {
             // every element of the cluster lies beyond KEM_FAR3_RATIO sizes of every lane: the
-            // degree-2 3-point rule
+            // degree-2 3-point rule (single-set passes: in the several-set kernel the branch
+            // costs 3 % on the 4 M-point batches, which have no such clusters)
             for (int64_t k = c * KEM_CLUSTER; k < k1; k++) {
                 const v16f R = *reinterpret_cast<const v16f*>(far3 + k * KEM_FAR3_STRIDE);
                 const double wj = wl[MULTI ? k * nsets : k];
