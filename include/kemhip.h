@@ -190,7 +190,10 @@ int kem_field_eval_device(kem_ctx* ctx, const kem_mesh* mesh, const double* d_si
    kem_graph_launch (asynchronous on the context's stream): for repeated evaluations at points
    that change in place, e.g. the particles of a tracking step.  The buffers, sigma and the
    point count are fixed at capture; their contents may change between launches.  No
-   reference counterpart.  The graph must be destroyed before its context and mesh. */
+   reference counterpart.  The graph owns the workspace its kernels address (allocated here),
+   so eager calls on the context between replays are allowed; it must be destroyed before its
+   context and mesh.  phi, ex, ey, ez: all four required (also for kem_field_eval_device and
+   kem_field_eval_sets). */
 int kem_field_graph_create(kem_ctx* ctx, const kem_mesh* mesh, const double* sigma, int64_t npts, const double* x,
                            const double* y, const double* z, int precision, double* phi, double* ex, double* ey,
                            double* ez, kem_graph** out);
@@ -207,6 +210,18 @@ int kem_field_eval_sets(kem_ctx* ctx, const kem_mesh* mesh, int64_t nsets, const
    ladder only).  Device points. */
 int kem_field_census(kem_ctx* ctx, const kem_mesh* mesh, int64_t npts, const double* d_x, const double* d_y,
                      const double* d_z, int precision, int64_t* counts);
+
+/* Instrumentation (no reference counterpart; host arithmetic only, no device access): how a
+   field evaluation of npts points over nelem elements splits into element ranges and partial
+   sums, for a mesh with (clustered = 1: triangles / rectangles / segments, numeric policy) or
+   without element clusters, at a precision, with per-point set indices (multi = 1, as
+   kem_field_eval_sets) or without, and a far-pass element range (KEM_TUNE_FAR_RANGE_*).
+   ws_doubles < 0 plans for the workspace the library allocates.  out[8] = workspace doubles,
+   element-weight doubles, partial-sum doubles written after them, element ranges, partial
+   slices, points per chunk, XCD split (0 / 1), far kernel (0 FP64, 1 FP32 per pair, 2 FP32
+   clusters).  The passes write only weights + partials <= workspace doubles. */
+int kem_field_plan(int policy, int clustered, int precision, int multi, int64_t nelem, int64_t nsets, int64_t npts,
+                   int64_t range_elems, int64_t ws_doubles, int64_t* out);
 
 /* ---- dense influence matrix: KBoundaryIntegralMatrix<KElectrostaticBoundaryIntegrator, true>
    (BoundaryIntegrals/Core/include/KBoundaryIntegralMatrix.hh:16-99) as a device-resident
@@ -260,6 +275,41 @@ int kem_field_vjp(kem_ctx* ctx, const kem_mesh* mesh, const double* sigma, int64
    matrix A_ij = G(c_i; T_j), c_i = centroid of T_i (the term lambda^T (dA) sigma of the adjoint
    of A sigma = b).  KEM_E_STATE for meshes with Neumann rows. */
 int kem_assemble_vjp(kem_ctx* ctx, const kem_mesh* mesh, const double* lambda, const double* sigma, double* dvert);
+
+/* ---- charge-density solution cache (host code, no device): KChargeDensitySolver::FindSolution
+   / SaveSolution (Interface/ChargeDensitySolvers/Electric/src/KChargeDensitySolver.cc:51-280)
+   over KEMFileInterface's directory of .kbd files (IO/FileManipulation/include/
+   KEMFileInterface.hh:23-81, KEMFile.hh), keyed by masked-bit MD5 hashes of the surfaces
+   (IO/HashGenerator/include/KMD5HashGenerator.hh).  Surfaces as for kem_mesh_create (element
+   records, boundary types and values, KSurfaceContainer order; no symmetry groups).  Errors
+   return KEM_E_ARG with the message in kem_cache_error(). */
+#define KEM_HASH_OMIT_BASIS 1     /* leave out the charge densities (KElectrostaticBasis) */
+#define KEM_HASH_OMIT_DIRICHLET 2 /* leave out the Dirichlet values (KBoundaryType<.., KDirichletBoundary>) */
+const char* kem_cache_error(void);
+/* MD5 (RFC 1321) of bytes: 32 lowercase hex digits and a NUL into hex33 (md5.hh hex_digest) */
+int kem_md5_hex(const void* data, int64_t bytes, char* hex33);
+/* KMD5HashGenerator over n doubles streamed in order: each rounded half up at its highest
+   masked bit and the masked_bits low bits cleared, values of magnitude below threshold left out
+   (MaskedBits / Threshold; the reference's defaults are 1 and 1e-12, its solvers' 20 and 1e-14) */
+int kem_hash_doubles(const double* x, int64_t n, int masked_bits, double threshold, char* hex33);
+/* KMD5HashGenerator::GenerateHash(KSurfaceContainer) of the surfaces with charge densities sigma
+   (nullable when omitted), leaving out the policies in omit (KEM_HASH_OMIT_*) */
+int kem_hash_surfaces(int64_t n, const int32_t* shape_type, const double* shape_data, const int32_t* bc_type,
+                      const double* bc_value, const double* bc_flux, const double* sigma, int omit, int masked_bits,
+                      double threshold, char* hex33);
+/* SaveSolution: appends to directory/file (created if missing) the residual threshold record,
+   labelled with the shape, shape + boundary and solution hashes, and the solved container */
+int kem_cache_save(const char* directory, const char* file, int64_t n, const int32_t* shape_type,
+                   const double* shape_data, const int32_t* bc_type, const double* bc_value, const double* bc_flux,
+                   const double* sigma, double residual_threshold, int masked_bits, double hash_threshold);
+/* FindSolution: *found = 1 when a stored solution of these surfaces and boundary values has a
+   residual threshold <= tolerance (the smallest such), 2 when stored solutions of the same
+   surfaces with other boundary values superpose to these (KSuperpositionSolver), 0 otherwise;
+   sigma (n) receives the charge densities, file_out (nullable, file_cap bytes) the file read */
+int kem_cache_find(const char* directory, int64_t n, const int32_t* shape_type, const double* shape_data,
+                   const int32_t* bc_type, const double* bc_value, const double* bc_flux, double tolerance,
+                   int masked_bits, double hash_threshold, double* sigma, int* found, char* file_out,
+                   int64_t file_cap);
 
 /* Y = A X for nrhs vectors (host, column-major n x nrhs): KSquareMatrix::Multiply
    (KMatrix.hh:92-111) over a block; KEM_MIXED uses the FP32 matrix on MFMA. */

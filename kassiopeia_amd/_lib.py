@@ -85,6 +85,7 @@ SIGNATURES = {
     "kem_graph_destroy": (None, [_P]),
     "kem_field_eval_sets": (_INT, [_P, _P, _I64, _P, _P, _I64, _P, _P, _P, _INT, _P, _P, _P, _P]),
     "kem_field_census": (_INT, [_P, _P, _I64, _P, _P, _P, _INT, _P]),
+    "kem_field_plan": (_INT, [_INT, _INT, _INT, _INT, _I64, _I64, _I64, _I64, _I64, _P]),
     "kem_matrix_assemble": (_INT, [_P, _P, _INT, ctypes.POINTER(_P)]),
     "kem_matrix_destroy": (None, [_P]),
     "kem_matrix_dimension": (_I64, [_P]),
@@ -98,6 +99,13 @@ SIGNATURES = {
     "kem_assemble_vjp": (_INT, [_P, _P, _P, _P, _P]),
     "kem_solve_batch": (_INT, [_P, ctypes.POINTER(SolveOpts), _I64, _P, _P, _P, ctypes.POINTER(SolveStats)]),
     "kem_matrix_multiply_batch": (_INT, [_P, _INT, _I64, _P, _P]),
+    "kem_cache_error": (ctypes.c_char_p, []),
+    "kem_md5_hex": (_INT, [_P, _I64, ctypes.c_char_p]),
+    "kem_hash_doubles": (_INT, [_P, _I64, _INT, _D, ctypes.c_char_p]),
+    "kem_hash_surfaces": (_INT, [_I64, _P, _P, _P, _P, _P, _P, _INT, _INT, _D, ctypes.c_char_p]),
+    "kem_cache_save": (_INT, [ctypes.c_char_p, ctypes.c_char_p, _I64, _P, _P, _P, _P, _P, _P, _D, _INT, _D]),
+    "kem_cache_find": (_INT, [ctypes.c_char_p, _I64, _P, _P, _P, _P, _P, _D, _INT, _D, _P, ctypes.POINTER(_INT),
+                              ctypes.c_char_p, _I64]),
 }
 
 _lib = None

@@ -29,6 +29,7 @@ SOURCES = [
     ("kem_adjoint.hip", True),
     ("kem_capi.cpp", False),
     ("kem_prep.cpp", False),
+    ("kem_cache.cpp", False),
 ]
 
 

@@ -24,6 +24,22 @@ constexpr double kMixedFp64RatioDefault = 0.0;
 // elements per XCD range of the large single-set far passes (kem::launch_field); 0 = no split
 constexpr int64_t kFarRangeDefault = 2560;
 
+// Workspace of one field evaluation: partial sums and per-call element weights (ws), the
+// charge densities of symmetry-group members (gsig) and the point ordering (sort).  A context
+// owns one that grows with its calls; a captured graph owns one sized at capture, since a later
+// call may reallocate the context's while the graph still holds its addresses; the
+// no-allocation entry point uses one carved from the device's static scratch (fixed: never
+// reallocated, a call that needs more is refused).
+struct FieldWs {
+    double* ws = nullptr;
+    int64_t ws_doubles = 0;
+    double* gsig = nullptr;
+    int64_t gsig_doubles = 0;
+    void* sort = nullptr;
+    size_t sort_bytes = 0;
+    bool fixed = false;
+};
+
 struct kem_ctx {
     int device = 0;
     hipStream_t own = nullptr;
@@ -34,14 +50,11 @@ struct kem_ctx {
     int64_t far_range_fp64 = kFarRangeDefault, far_range_mixed = kFarRangeDefault;
     kem::LaunchTuning tuning;
     std::string err;
-    double* ws = nullptr;  // field partial sums / gemv partials
+    double* ws = nullptr;  // matrix-vector partial sums (grow-only)
     int64_t ws_doubles = 0;
     double* scal = nullptr;  // small device scalars (dot products, Hessenberg column)
     int64_t scal_doubles = 0;
-    double* gsig = nullptr;  // charge densities per member of symmetry groups (grow-only)
-    int64_t gsig_doubles = 0;
-    void* sort_ws = nullptr;  // point ordering workspace (grow-only)
-    size_t sort_ws_bytes = 0;
+    FieldWs fws;  // workspace of eager field evaluations (grow-only)
     // far-kernel timing: one event pair per field evaluation since the last query
     static constexpr int kTimedMax = 256;
     bool timing = false;
@@ -77,6 +90,7 @@ struct kem_graph {
     kem_ctx* ctx = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    FieldWs fws;  // the buffers the captured kernels address, owned by the graph
 };
 
 struct kem_matrix {
@@ -127,14 +141,48 @@ hipError_t ensure_ws(kem_ctx* c, int64_t doubles)
     return e;
 }
 
-hipError_t ensure_gsig(kem_ctx* c, int64_t doubles)
+// grow w to at least the given sizes (a fixed workspace is never reallocated: a call that needs
+// more than it holds is refused with hipErrorInvalidValue)
+hipError_t ws_reserve(FieldWs& w, int64_t ws_doubles, int64_t gsig_doubles, size_t sort_bytes)
 {
-    if (c->gsig_doubles >= doubles) return hipSuccess;
-    dfree(c->gsig);
-    c->gsig_doubles = 0;
-    hipError_t e = dalloc(&c->gsig, doubles);
-    if (e == hipSuccess) c->gsig_doubles = doubles;
-    return e;
+    if (w.fixed)
+        return (w.ws_doubles >= ws_doubles && w.gsig_doubles >= gsig_doubles && w.sort_bytes >= sort_bytes)
+                   ? hipSuccess
+                   : hipErrorInvalidValue;
+    hipError_t e = hipSuccess;
+    if (w.ws_doubles < ws_doubles) {
+        dfree(w.ws);
+        w.ws_doubles = 0;
+        e = dalloc(&w.ws, ws_doubles);
+        if (e != hipSuccess) return e;
+        w.ws_doubles = ws_doubles;
+    }
+    if (w.gsig_doubles < gsig_doubles) {
+        dfree(w.gsig);
+        w.gsig_doubles = 0;
+        e = dalloc(&w.gsig, gsig_doubles);
+        if (e != hipSuccess) return e;
+        w.gsig_doubles = gsig_doubles;
+    }
+    if (w.sort_bytes < sort_bytes) {
+        if (w.sort) (void)hipFree(w.sort);
+        w.sort = nullptr;
+        w.sort_bytes = 0;
+        e = hipMalloc(&w.sort, sort_bytes);
+        if (e != hipSuccess) return e;
+        w.sort_bytes = sort_bytes;
+    }
+    return hipSuccess;
+}
+
+void ws_release(FieldWs& w)
+{
+    if (!w.fixed) {
+        dfree(w.ws);
+        dfree(w.gsig);
+        if (w.sort) (void)hipFree(w.sort);
+    }
+    w = FieldWs();
 }
 
 hipError_t ensure_scal(kem_ctx* c, int64_t doubles)
@@ -176,6 +224,21 @@ struct DVec {
 // point counts from which field evaluation visits points in Morton order
 constexpr int64_t kSortMinPoints = 4096;
 
+// Workspace sizes of one field evaluation with `range` elements per XCD range
+struct WsNeed {
+    int64_t ws = 0, gsig = 0;
+    size_t sort = 0;
+};
+WsNeed field_ws_need(const kem_ctx* c, const kem_mesh* m, const kem::FieldMesh& fm, bool multi, int64_t nsets,
+                     int64_t npts, bool mixed, int64_t range)
+{
+    WsNeed n;
+    n.ws = kem::field_workspace_doubles(c->policy, fm, mixed, c->mixed_fp64_ratio, multi, nsets, npts, range);
+    n.gsig = m->gstart ? nsets * m->nm : 0;
+    if (npts >= kSortMinPoints && npts <= INT32_MAX) n.sort = kem::sort_workspace_bytes(npts);
+    return n;
+}
+
 double now_s()
 {
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
@@ -216,8 +279,7 @@ void kem_ctx_destroy(kem_ctx* c)
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     dfree(c->ws);
     dfree(c->scal);
-    dfree(c->gsig);
-    if (c->sort_ws) (void)hipFree(c->sort_ws);
+    ws_release(c->fws);
     for (hipEvent_t& e : c->ev)
         if (e) (void)hipEventDestroy(e);
     if (c->side.st) (void)hipStreamSynchronize(c->side.st);
@@ -583,42 +645,38 @@ void field_mesh(const kem_mesh* m, kem::FieldMesh& f)
     f.axisym = m->has_axisym;
 }
 
-// side: run the near pass on the context's side stream (not under graph capture)
+// Field evaluation on device arrays into workspace w; pset (nullable): per-point
+// charge-density set.  side: run the near pass on the context's side stream (not under graph
+// capture).  range < 0: the context's element ranges for the precision.
 int field_eval(kem_ctx* c, const kem_mesh* m, const double* sigma, const int32_t* pset, int64_t nsets, int64_t npts,
                const double* x, const double* y, const double* z, int precision, double* phi, double* ex, double* ey,
-               double* ez, bool side = true)
+               double* ez, FieldWs& w, bool side, int64_t range = -1)
 {
     const kem::SideStream* ss = side ? &c->side : nullptr;
     kem::FieldMesh fm;
     field_mesh(m, fm);
     const bool mixed = precision == KEM_MIXED;
-    const int64_t range = mixed ? c->far_range_mixed : c->far_range_fp64;
+    if (range < 0) range = mixed ? c->far_range_mixed : c->far_range_fp64;
+    const WsNeed need = field_ws_need(c, m, fm, pset != nullptr, nsets, npts, mixed, range);
+    // a fixed workspace may hold less than the preferred size: the plan then takes fewer
+    // element ranges, and launch_field refuses one that does not fit
+    KEM_CK(c, ws_reserve(w, w.fixed ? 0 : need.ws, need.gsig, need.sort));
     if (m->gstart) {
         // symmetry groups: every member carries its group's charge density
-        KEM_CK(c, ensure_gsig(c, nsets * m->nm));
-        KEM_CK(c, kem::launch_expand_groups(c->stream, m->gof, m->nm, m->n, nsets, sigma, c->gsig));
-        sigma = c->gsig;
+        KEM_CK(c, kem::launch_expand_groups(c->stream, m->gof, m->nm, m->n, nsets, sigma, w.gsig));
+        sigma = w.gsig;
     }
     hipEvent_t t0, t1;
     timing_pair(c, &t0, &t1);
-    KEM_CK(c, ensure_ws(c, kem::field_workspace_doubles(c->policy, fm, nsets, npts)));
-    if (npts >= kSortMinPoints && npts <= INT32_MAX) {
+    if (need.sort) {
         // visit the points in Morton order so the lanes of a wavefront share quadrature rules
-        const size_t bytes = kem::sort_workspace_bytes(npts);
-        if (c->sort_ws_bytes < bytes) {
-            if (c->sort_ws) (void)hipFree(c->sort_ws);
-            c->sort_ws = nullptr;
-            c->sort_ws_bytes = 0;
-            KEM_CK(c, hipMalloc(&c->sort_ws, bytes));
-            c->sort_ws_bytes = bytes;
-        }
         KEM_CK(c, kem::launch_field_sorted(c->stream, c->policy, fm, mixed, c->mixed_fp64_ratio, sigma, pset, nsets,
-                                           x, y, z, npts, phi, ex, ey, ez, c->sort_ws, c->sort_ws_bytes, c->ws,
-                                           c->ws_doubles, range, t0, t1, ss));
+                                           x, y, z, npts, phi, ex, ey, ez, w.sort, w.sort_bytes, w.ws, w.ws_doubles,
+                                           range, t0, t1, ss));
         return KEM_OK;
     }
     KEM_CK(c, kem::launch_field(c->stream, c->policy, fm, mixed, c->mixed_fp64_ratio, sigma, pset, nsets, x, y, z,
-                                npts, phi, ex, ey, ez, c->ws, c->ws_doubles, range, t0, t1, ss));
+                                npts, phi, ex, ey, ez, w.ws, w.ws_doubles, range, t0, t1, ss));
     return KEM_OK;
 }
 }  // namespace
@@ -627,49 +685,46 @@ int kem_field_eval_device(kem_ctx* c, const kem_mesh* m, const double* sigma, in
                           const double* y, const double* z, int precision, double* phi, double* ex, double* ey,
                           double* ez)
 {
-    if (!c || !m || !sigma || (npts > 0 && (!x || !y || !z))) return KEM_E_ARG;
+    if (!c || !m || !sigma || (npts > 0 && (!x || !y || !z || !phi || !ex || !ey || !ez))) return KEM_E_ARG;
     if (m->ctx != c) return fail(c, KEM_E_STATE, "mesh belongs to another context");
     if (precision != KEM_FP64 && precision != KEM_MIXED) return fail(c, KEM_E_ARG, "field: unknown precision mode");
     if (npts <= 0) return KEM_OK;
     KEM_CK(c, enter(c));
-    return field_eval(c, m, sigma, nullptr, 1, npts, x, y, z, precision, phi, ex, ey, ez);
+    return field_eval(c, m, sigma, nullptr, 1, npts, x, y, z, precision, phi, ex, ey, ez, c->fws, true);
 }
 
 int kem_field_graph_create(kem_ctx* c, const kem_mesh* m, const double* sigma, int64_t npts, const double* x,
                            const double* y, const double* z, int precision, double* phi, double* ex, double* ey,
                            double* ez, kem_graph** out)
 {
-    if (!c || !m || !sigma || !out || npts <= 0 || !x || !y || !z) return KEM_E_ARG;
+    if (!c || !m || !sigma || !out || npts <= 0 || !x || !y || !z || !phi || !ex || !ey || !ez) return KEM_E_ARG;
     *out = nullptr;
     if (m->ctx != c) return fail(c, KEM_E_STATE, "mesh belongs to another context");
     if (precision != KEM_FP64 && precision != KEM_MIXED) return fail(c, KEM_E_ARG, "field: unknown precision mode");
     KEM_CK(c, enter(c));
-    // workspaces reach their final size before the capture (no allocation inside a graph)
+    kem_graph* g = new kem_graph();
+    g->ctx = c;
+    // the graph's own workspace reaches its final size before the capture (no allocation
+    // inside a graph); later calls on the context cannot move it
     {
         kem::FieldMesh fm;
         field_mesh(m, fm);
-        KEM_CK(c, ensure_ws(c, kem::field_workspace_doubles(c->policy, fm, 1, npts)));
-        if (m->gstart) KEM_CK(c, ensure_gsig(c, m->nm));
-    }
-    if (npts >= kSortMinPoints && npts <= INT32_MAX) {
-        const size_t bytes = kem::sort_workspace_bytes(npts);
-        if (c->sort_ws_bytes < bytes) {
-            if (c->sort_ws) (void)hipFree(c->sort_ws);
-            c->sort_ws = nullptr;
-            c->sort_ws_bytes = 0;
-            KEM_CK(c, hipMalloc(&c->sort_ws, bytes));
-            c->sort_ws_bytes = bytes;
+        const bool mixed = precision == KEM_MIXED;
+        const WsNeed need =
+            field_ws_need(c, m, fm, false, 1, npts, mixed, mixed ? c->far_range_mixed : c->far_range_fp64);
+        const hipError_t e = ws_reserve(g->fws, need.ws, need.gsig, need.sort);
+        if (e != hipSuccess) {
+            kem_graph_destroy(g);
+            return fail(c, KEM_E_HIP, std::string("field graph workspace: ") + hipGetErrorString(e));
         }
     }
     KEM_CK(c, hipStreamSynchronize(c->stream));
     const bool timing = c->timing;
     c->timing = false;
-    kem_graph* g = new kem_graph();
-    g->ctx = c;
     hipError_t e = hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal);
     int rc = KEM_OK;
     if (e == hipSuccess) {
-        rc = field_eval(c, m, sigma, nullptr, 1, npts, x, y, z, precision, phi, ex, ey, ez, false);
+        rc = field_eval(c, m, sigma, nullptr, 1, npts, x, y, z, precision, phi, ex, ey, ez, g->fws, false);
         hipGraph_t graph = nullptr;
         const hipError_t e2 = hipStreamEndCapture(c->stream, &graph);
         g->graph = graph;
@@ -698,9 +753,14 @@ int kem_graph_launch(kem_graph* g)
 void kem_graph_destroy(kem_graph* g)
 {
     if (!g) return;
-    if (g->ctx) (void)hipSetDevice(g->ctx->device);
+    if (g->ctx) {
+        (void)hipSetDevice(g->ctx->device);
+        // a replay still queued on the context's stream reads the graph's workspace
+        (void)hipStreamSynchronize(g->ctx->stream);
+    }
     if (g->exec) (void)hipGraphExecDestroy(g->exec);
     if (g->graph) (void)hipGraphDestroy(g->graph);
+    ws_release(g->fws);
     delete g;
 }
 
@@ -708,7 +768,8 @@ int kem_field_eval_sets(kem_ctx* c, const kem_mesh* m, int64_t nsets, const doub
                         int64_t npts, const double* x, const double* y, const double* z, int precision, double* phi,
                         double* ex, double* ey, double* ez)
 {
-    if (!c || !m || !sigma || nsets <= 0 || (npts > 0 && (!x || !y || !z || !set))) return KEM_E_ARG;
+    if (!c || !m || !sigma || nsets <= 0 || (npts > 0 && (!x || !y || !z || !set || !phi || !ex || !ey || !ez)))
+        return KEM_E_ARG;
     if (m->ctx != c) return fail(c, KEM_E_STATE, "mesh belongs to another context");
     if (precision != KEM_FP64 && precision != KEM_MIXED) return fail(c, KEM_E_ARG, "field: unknown precision mode");
     if (nsets > (1 << 6)) return fail(c, KEM_E_ARG, "field: at most 64 charge-density sets per call");
@@ -722,7 +783,39 @@ int kem_field_eval_sets(kem_ctx* c, const kem_mesh* m, int64_t nsets, const doub
     KEM_CK(c, hipMemcpyAsync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost, c->stream));
     KEM_CK(c, hipStreamSynchronize(c->stream));
     if (hbad) return fail(c, KEM_E_ARG, "field: set index outside [0, nsets)");
-    return field_eval(c, m, sigma, set, nsets, npts, x, y, z, precision, phi, ex, ey, ez);
+    return field_eval(c, m, sigma, set, nsets, npts, x, y, z, precision, phi, ex, ey, ez, c->fws, true);
+}
+
+int kem_field_plan(int policy, int clustered, int precision, int multi, int64_t nelem, int64_t nsets, int64_t npts,
+                   int64_t range_elems, int64_t ws_doubles, int64_t* out)
+{
+    if (!out || nelem <= 0 || nsets < 1 || npts < 0 || range_elems < 0) return KEM_E_ARG;
+    if (precision != KEM_FP64 && precision != KEM_MIXED) return KEM_E_ARG;
+    // the plan only tests which mesh arrays exist: stand-in addresses, never dereferenced
+    static const char present = 0;
+    kem::FieldMesh fm;
+    fm.n = nelem;
+    if (clustered) {
+        fm.clusters = (const double*)&present;
+        fm.order = (const int32_t*)&present;
+        fm.nodes32 = (const float*)&present;
+        fm.far6 = fm.far3 = (const float*)&present;
+        fm.far6_cl = (const double*)&present;
+    }
+    const bool mixed = precision == KEM_MIXED;
+    const int64_t need =
+        kem::field_workspace_doubles(policy, fm, mixed, kMixedFp64RatioDefault, multi != 0, nsets, npts, range_elems);
+    const kem::FieldPlan p = kem::plan_field(policy, fm, mixed, kMixedFp64RatioDefault, multi != 0, nsets, npts,
+                                             range_elems, ws_doubles < 0 ? need : ws_doubles);
+    out[0] = ws_doubles < 0 ? need : ws_doubles;
+    out[1] = p.weights;
+    out[2] = p.partials;
+    out[3] = p.ranges;
+    out[4] = p.slices;
+    out[5] = p.chunk;
+    out[6] = p.xcd ? 1 : 0;
+    out[7] = p.kind;
+    return KEM_OK;
 }
 
 int kem_field_census(kem_ctx* c, const kem_mesh* m, int64_t npts, const double* x, const double* y, const double* z,

@@ -31,8 +31,26 @@ struct SideStream {
     hipEvent_t fork = nullptr, join = nullptr;
 };
 
-// Doubles of the ws argument of launch_field for npts points and nsets charge-density sets.
-int64_t field_workspace_doubles(int policy, const FieldMesh& m, int64_t nsets, int64_t npts);
+// Doubles of the ws argument of launch_field for npts points and nsets charge-density sets
+// (multi: per-point set indices given) with the element ranges range_elems asks for.
+int64_t field_workspace_doubles(int policy, const FieldMesh& m, bool mixed, double fp64_ratio, bool multi,
+                                int64_t nsets, int64_t npts, int64_t range_elems);
+
+// How launch_field splits one evaluation (host arithmetic, no device access).
+struct FieldPlan {
+    int kind = 0;          // far kernel: 0 FP64, 1 FP32 per pair, 2 FP32 over element clusters
+    bool xcd = false;      // 8 R element ranges, R per XCD, over chunks of points
+    int64_t ranges = 1;    // element ranges of the far pass
+    int64_t slices = 1;    // partial-sum slices: the ranges, and one for a culled near pass
+    int64_t per = 0;       // elements per range
+    int64_t chunk = 0;     // points per chunk
+    int64_t weights = 0;   // doubles of per-call element weights at the start of ws
+    int64_t partials = 0;  // doubles of partial sums written after them (0: straight to the outputs)
+};
+// The plan for a workspace of ws_doubles doubles (< 0: field_workspace_doubles); launch_field
+// refuses to run when weights + partials exceed its ws_doubles.
+FieldPlan plan_field(int policy, const FieldMesh& m, bool mixed, double fp64_ratio, bool multi, int64_t nsets,
+                     int64_t npts, int64_t range_elems, int64_t ws_doubles);
 
 // phi/E at npts points (SoA) from the mesh's elements with charge densities sigma.  pset
 // (nullable): per-point index of the charge-density set, point i using
@@ -42,8 +60,10 @@ int64_t field_workspace_doubles(int policy, const FieldMesh& m, int64_t nsets, i
 // order select the culled near pass under the numeric policy; null scans every element.
 // range_elems > 0: from 262,144 points on, a single-set numeric-policy far pass splits the
 // elements into 8 R ranges of at most range_elems elements, R per XCD (0: no split).
-// ws: scratch of ws_doubles >= field_workspace_doubles(policy, m, nsets, npts) doubles
-// (element-range partial sums, per-set element weights).  far_begin / far_end (nullable): events recorded on
+// ws: scratch of ws_doubles doubles (element-range partial sums, per-set element weights); a
+// smaller workspace than field_workspace_doubles asks for takes fewer element ranges, and one
+// smaller than the plan's weights + partials returns hipErrorInvalidValue.  phi, ex, ey, ez:
+// npts doubles each, all four required.  far_begin / far_end (nullable): events recorded on
 // st around the far-field pass.  side (nullable): the culled near pass runs on side->st; the
 // sums are the same with and without it (the near pass fills its own partial slice).
 hipError_t launch_field(hipStream_t st, int policy, const FieldMesh& m, bool mixed, double fp64_ratio,

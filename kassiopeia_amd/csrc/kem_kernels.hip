@@ -1019,13 +1019,6 @@ bool xcd_split(int policy, const FieldMesh& m, int64_t npts, int64_t range_elems
 // the culled near pass writes one more slice of partial sums beside the far pass's
 bool culled_near(int policy, const FieldMesh& m) { return policy == 0 && m.clusters && m.order; }
 
-int64_t partial_doubles(int policy, const FieldMesh& m, int64_t npts)
-{
-    if (xcd_split(policy, m, npts, 1)) return std::max(kPartialDoubles, kXcdPartialDoubles);
-    if (culled_near(policy, m)) return std::max(kPartialDoubles, 8 * npts);
-    return kPartialDoubles;
-}
-
 // Culled near pass over points [p0, p0 + np) into slice `slice` of the partial layout q (ns
 // slices of np doubles per component).  With a side stream it runs there, forked from st
 // after the work queued so far; join_side() makes st wait for it.
@@ -1089,57 +1082,46 @@ void far_kernel(hipStream_t st, dim3 grid, int kind, const FieldMesh& m, double 
 }
 
 template <bool MULTI>
-void field_passes(hipStream_t st, int policy, const FieldMesh& m, bool mixed, double fp64_ratio, const double* sigma,
-                  const int32_t* pset, int64_t nsets, const double* px, const double* py, const double* pz,
-                  int64_t npts, double* phi, double* ex, double* ey, double* ez, double* ws, int64_t ws_doubles,
-                  int64_t range_elems, hipEvent_t far_begin, hipEvent_t far_end, const SideStream* side)
+hipError_t field_passes(hipStream_t st, int policy, const FieldMesh& m, bool mixed, double fp64_ratio,
+                        const double* sigma, const int32_t* pset, int64_t nsets, const double* px, const double* py,
+                        const double* pz, int64_t npts, double* phi, double* ex, double* ey, double* ez, double* ws,
+                        int64_t ws_doubles, int64_t range_elems, hipEvent_t far_begin, hipEvent_t far_end,
+                        const SideStream* side)
 {
     const int64_t nelem = m.n;
-    const unsigned gx = nblocks(npts);
+    const FieldPlan p = plan_field(policy, m, mixed, fp64_ratio, MULTI, nsets, npts, range_elems, ws_doubles);
+    // every partial sum the passes write lies inside ws
+    if (p.weights + p.partials > ws_doubles) return hipErrorInvalidValue;
+    const bool clustered = p.kind == 2;
     // ws: the cluster pass's element weights, then (several sets) the sets transposed to
     // element-major order, then the partial sums of split element ranges
-    const bool clustered = policy == 0 && mixed && m.nodes32 && fp64_ratio == 0. && m.far6 && m.far3 && m.far6_cl &&
-                           m.order;
-    const int kind = clustered ? 2 : (policy == 0 && mixed && m.nodes32) ? 1 : 0;
     const int64_t nsig = (nsets * nelem + 31) / 32 * 32;
     double* wk = ws;
-    if (clustered) {
-        ws += nsig;
-        ws_doubles -= nsig;
-    }
+    if (clustered) ws += nsig;
     const double* sg = sigma;
     if (MULTI) {
         k_far_weights<<<nblocks(nsets * nelem), kBlock, 0, st>>>(m.geom, nullptr, sigma, nelem, nsets, ws);
         sg = ws;
         ws += nsig;
-        ws_doubles -= nsig;
     }
     const bool culled = culled_near(policy, m);
-    const int64_t extra = culled ? 1 : 0;  // the near pass's slice, after the far pass's
-    if (!MULTI && xcd_split(policy, m, npts, range_elems)) {
+    const int64_t nr = p.ranges, ns = p.slices;
+    if (p.xcd) {
         // 8 R element ranges of at most range_elems elements, R per XCD; partial sums per
         // range (and of the near pass) over chunks of points
-        const int64_t r = std::min<int64_t>(kXcdPartialDoubles / (4 * kXcds * kBlock) - 1,
-                                            std::max<int64_t>(1, ((nelem + kXcds - 1) / kXcds + range_elems - 1) /
-                                                                     range_elems));
-        const int64_t nr = kXcds * r;
-        const int64_t ns = nr + extra;
-        int64_t per = (nelem + nr - 1) / nr;
-        if (clustered) per = (per + KEM_CLUSTER - 1) / KEM_CLUSTER * KEM_CLUSTER;
-        const int64_t chunk = std::max<int64_t>(kBlock, kXcdPartialDoubles / (4 * ns) / kBlock * kBlock);
         if (far_begin) (void)hipEventRecord(far_begin, st);
         if (clustered)
             k_far_weights<<<nblocks(nsets * nelem), kBlock, 0, st>>>(m.geom, m.order, sigma, nelem, nsets, wk);
-        for (int64_t p0 = 0; p0 < npts; p0 += chunk) {
-            const int64_t np = std::min(chunk, npts - p0);
+        for (int64_t p0 = 0; p0 < npts; p0 += p.chunk) {
+            const int64_t np = std::min(p.chunk, npts - p0);
             const unsigned gc = nblocks(np);
             double* q = ws;
             if (culled)
                 near_slice<MULTI>(st, side, m, sg, MULTI ? pset + p0 : nullptr, nsets, px + p0, py + p0, pz + p0, np,
-                                  q, ns, nr, kind != 0);
-            far_kernel<MULTI>(st, dim3((unsigned)(gc * nr)), kind, m, fp64_ratio, wk, sg, MULTI ? pset + p0 : nullptr,
-                              nsets, per, px + p0, py + p0, pz + p0, np, q, q + ns * np, q + 2 * ns * np,
-                              q + 3 * ns * np, np, (int64_t)gc);
+                                  q, ns, nr, p.kind != 0);
+            far_kernel<MULTI>(st, dim3((unsigned)(gc * nr)), p.kind, m, fp64_ratio, wk, sg,
+                              MULTI ? pset + p0 : nullptr, nsets, p.per, px + p0, py + p0, pz + p0, np, q,
+                              q + ns * np, q + 2 * ns * np, q + 3 * ns * np, np, (int64_t)gc);
             // far_end right after the last far kernel: the events time the far pass (with the
             // joins and combines of earlier chunks), not the wait for the near pass
             if (far_end && p0 + np >= npts) (void)hipEventRecord(far_end, st);
@@ -1147,54 +1129,96 @@ void field_passes(hipStream_t st, int policy, const FieldMesh& m, bool mixed, do
             k_field_combine<<<gc, kBlock, 0, st>>>(ws, (int)ns, np, phi + p0, ex + p0, ey + p0, ez + p0);
         }
     } else {
-        // enough blocks to fill 256 CUs several times over; each range >= 32 elements (one cluster)
-        int64_t nsplit = (2048 + gx - 1) / gx;
-        if (nsplit > (nelem + 31) / 32) nsplit = (nelem + 31) / 32;
-        if (nsplit < 1) nsplit = 1;
-        while (nsplit > 1 && 4 * (nsplit + extra) * npts > ws_doubles) nsplit--;
-        int64_t per = (nelem + nsplit - 1) / nsplit;
-        if (clustered) per = (per + KEM_CLUSTER - 1) / KEM_CLUSTER * KEM_CLUSTER;
-        nsplit = (nelem + per - 1) / per;
-        const int64_t ns = nsplit + extra;
-        dim3 grid(gx, (unsigned)nsplit);
+        const unsigned gx = nblocks(npts);
+        dim3 grid(gx, (unsigned)nr);
         double *o0 = ex, *o1 = ey, *o2 = ez, *o3 = phi;
         int64_t stride = 0;
-        if (ns > 1) {
+        if (p.partials > 0) {
             o0 = ws;
             o1 = ws + ns * npts;
             o2 = ws + 2 * ns * npts;
             o3 = ws + 3 * ns * npts;
             stride = npts;
         }
-        if (culled) near_slice<MULTI>(st, side, m, sg, pset, nsets, px, py, pz, npts, ws, ns, nsplit, kind != 0);
+        if (culled) near_slice<MULTI>(st, side, m, sg, pset, nsets, px, py, pz, npts, ws, ns, nr, p.kind != 0);
         if (far_begin) (void)hipEventRecord(far_begin, st);
         if (clustered)
             k_far_weights<<<nblocks(nsets * nelem), kBlock, 0, st>>>(m.geom, m.order, sigma, nelem, nsets, wk);
-        if (policy == 0) far_kernel<MULTI>(st, grid, kind, m, fp64_ratio, wk, sg, pset, nsets, per, px, py, pz, npts,
-                                           o0, o1, o2, o3, stride, 0);
+        if (policy == 0) far_kernel<MULTI>(st, grid, p.kind, m, fp64_ratio, wk, sg, pset, nsets, p.per, px, py, pz,
+                                           npts, o0, o1, o2, o3, stride, 0);
         if (far_end) (void)hipEventRecord(far_end, st);
         if (!culled) {
             if (m.axisym)
-                k_field_near<2, MULTI><<<grid, kBlock, 0, st>>>(m.geom, m.nodes, m.recs, sg, pset, nsets, nelem, per,
+                k_field_near<2, MULTI><<<grid, kBlock, 0, st>>>(m.geom, m.nodes, m.recs, sg, pset, nsets, nelem, p.per,
                                                                 px, py, pz, npts, policy, policy == 0 ? 1 : 0, o0, o1,
                                                                 o2, o3, stride);
             else if (policy >= 2)
-                k_field_near<1, MULTI><<<grid, kBlock, 0, st>>>(m.geom, m.nodes, m.recs, sg, pset, nsets, nelem, per,
+                k_field_near<1, MULTI><<<grid, kBlock, 0, st>>>(m.geom, m.nodes, m.recs, sg, pset, nsets, nelem, p.per,
                                                                 px, py, pz, npts, policy, 0, o0, o1, o2, o3, stride);
             else
-                k_field_near<0, MULTI><<<grid, kBlock, 0, st>>>(m.geom, m.nodes, m.recs, sg, pset, nsets, nelem, per,
+                k_field_near<0, MULTI><<<grid, kBlock, 0, st>>>(m.geom, m.nodes, m.recs, sg, pset, nsets, nelem, p.per,
                                                                 px, py, pz, npts, policy, policy == 0 ? 1 : 0, o0, o1,
                                                                 o2, o3, stride);
         }
         if (culled) join_side(st, side);
-        if (ns > 1) k_field_combine<<<gx, kBlock, 0, st>>>(ws, (int)ns, npts, phi, ex, ey, ez);
+        if (p.partials > 0) k_field_combine<<<gx, kBlock, 0, st>>>(ws, (int)ns, npts, phi, ex, ey, ez);
     }
+    return hipSuccess;
 }
 }  // namespace
 
-int64_t field_workspace_doubles(int policy, const FieldMesh& m, int64_t nsets, int64_t npts)
+FieldPlan plan_field(int policy, const FieldMesh& m, bool mixed, double fp64_ratio, bool multi, int64_t nsets,
+                     int64_t npts, int64_t range_elems, int64_t ws_doubles)
 {
-    return partial_doubles(policy, m, npts) + 2 * ((nsets * m.n + 31) / 32 * 32);
+    FieldPlan p;
+    const int64_t nelem = m.n;
+    if (npts <= 0 || nelem <= 0 || nsets < 1) return p;
+    const bool clustered = policy == 0 && mixed && m.nodes32 && fp64_ratio == 0. && m.far6 && m.far3 &&
+                           m.far6_cl && m.order;
+    p.kind = clustered ? 2 : (policy == 0 && mixed && m.nodes32) ? 1 : 0;
+    const int64_t nsig = (nsets * nelem + 31) / 32 * 32;
+    p.weights = (clustered ? nsig : 0) + (multi ? nsig : 0);
+    if (ws_doubles < 0)
+        ws_doubles = field_workspace_doubles(policy, m, mixed, fp64_ratio, multi, nsets, npts, range_elems);
+    const int64_t avail = std::max<int64_t>(0, ws_doubles - p.weights);
+    const int64_t extra = culled_near(policy, m) ? 1 : 0;  // the near pass's slice, after the far pass's
+    p.xcd = !multi && xcd_split(policy, m, npts, range_elems) && avail >= kXcdPartialDoubles;
+    if (p.xcd) {
+        const int64_t r = std::min<int64_t>(kXcdPartialDoubles / (4 * kXcds * kBlock) - 1,
+                                            std::max<int64_t>(1, ((nelem + kXcds - 1) / kXcds + range_elems - 1) /
+                                                                     range_elems));
+        p.ranges = kXcds * r;
+        p.slices = p.ranges + extra;
+        p.per = (nelem + p.ranges - 1) / p.ranges;
+        if (clustered) p.per = (p.per + KEM_CLUSTER - 1) / KEM_CLUSTER * KEM_CLUSTER;
+        p.chunk = std::max<int64_t>(kBlock, kXcdPartialDoubles / (4 * p.slices) / kBlock * kBlock);
+        p.partials = 4 * p.slices * std::min(p.chunk, npts);
+        return p;
+    }
+    // enough blocks to fill 256 CUs several times over; each range >= 32 elements (one cluster);
+    // fewer ranges when their partial sums would not fit the workspace
+    const int64_t gx = nblocks(npts);
+    int64_t nsplit = (2048 + gx - 1) / gx;
+    if (nsplit > (nelem + 31) / 32) nsplit = (nelem + 31) / 32;
+    if (nsplit < 1) nsplit = 1;
+    while (nsplit > 1 && 4 * (nsplit + extra) * npts > avail) nsplit--;
+    p.per = (nelem + nsplit - 1) / nsplit;
+    if (clustered) p.per = (p.per + KEM_CLUSTER - 1) / KEM_CLUSTER * KEM_CLUSTER;
+    p.ranges = (nelem + p.per - 1) / p.per;
+    p.slices = p.ranges + extra;
+    p.chunk = npts;
+    // one range and no near slice: the passes write the outputs directly
+    p.partials = p.slices > 1 ? 4 * p.slices * npts : 0;
+    return p;
+}
+
+int64_t field_workspace_doubles(int policy, const FieldMesh& m, bool mixed, double fp64_ratio, bool multi,
+                                int64_t nsets, int64_t npts, int64_t range_elems)
+{
+    const int64_t nsig = (nsets * m.n + 31) / 32 * 32;
+    int64_t partial = std::max(kPartialDoubles, 8 * npts);
+    if (!multi && xcd_split(policy, m, npts, range_elems)) partial = std::max(partial, kXcdPartialDoubles);
+    return partial + 2 * nsig;
 }
 
 hipError_t launch_field(hipStream_t st, int policy, const FieldMesh& m, bool mixed, double fp64_ratio,
@@ -1204,14 +1228,16 @@ hipError_t launch_field(hipStream_t st, int policy, const FieldMesh& m, bool mix
                         const SideStream* side)
 {
     if (npts <= 0) return hipSuccess;
-    if (nsets < 1 || ws_doubles < field_workspace_doubles(policy, m, nsets, npts)) return hipErrorInvalidValue;
+    if (nsets < 1 || !phi || !ex || !ey || !ez) return hipErrorInvalidValue;
     if (side && side->st && (!side->fork || !side->join)) return hipErrorInvalidValue;
+    hipError_t e;
     if (pset)
-        field_passes<true>(st, policy, m, mixed, fp64_ratio, sigma, pset, nsets, px, py, pz, npts, phi, ex, ey, ez, ws,
-                           ws_doubles, range_elems, far_begin, far_end, side);
+        e = field_passes<true>(st, policy, m, mixed, fp64_ratio, sigma, pset, nsets, px, py, pz, npts, phi, ex, ey,
+                               ez, ws, ws_doubles, range_elems, far_begin, far_end, side);
     else
-        field_passes<false>(st, policy, m, mixed, fp64_ratio, sigma, nullptr, 1, px, py, pz, npts, phi, ex, ey, ez,
-                            ws, ws_doubles, range_elems, far_begin, far_end, side);
+        e = field_passes<false>(st, policy, m, mixed, fp64_ratio, sigma, nullptr, 1, px, py, pz, npts, phi, ex, ey,
+                                ez, ws, ws_doubles, range_elems, far_begin, far_end, side);
+    if (e != hipSuccess) return e;
     return hipGetLastError();
 }
 

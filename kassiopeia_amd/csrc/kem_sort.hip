@@ -148,6 +148,7 @@ hipError_t launch_field_sorted(hipStream_t st, int policy, const FieldMesh& m, b
 {
     if (npts <= 0) return hipSuccess;
     if (npts > INT32_MAX || sort_ws_bytes < sort_workspace_bytes(npts)) return hipErrorInvalidValue;
+    if (!phi || !ex || !ey || !ez) return hipErrorInvalidValue;
     char* p = (char*)sort_ws;
     auto take = [&](size_t bytes) {
         char* r = p;

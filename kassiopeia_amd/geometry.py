@@ -428,27 +428,35 @@ def ring_bore_mesh(nrings: int = 32, r_ring: float = 1.9, ring_width: float = 0.
                    z_span: float = 8.0, r_outer: float = 2.0, n_target: int = 100000, volts=None,
                    seed: int = 99) -> Mesh:
     """Coaxial ring electrodes (short cylinders of radius r_ring and axial width ring_width,
-    evenly spaced over z_span) inside a grounded outer cylinder of radius r_outer; about
-    n_target triangles split between rings and the outer cylinder by area."""
+    evenly spaced over z_span) inside a grounded outer cylinder of radius r_outer: exactly
+    n_target triangles (even), split between rings and the outer cylinder by area.  The outer
+    cylinder takes an nz x nphi grid; the rings share the remaining cells, one axial row each
+    (rings this narrow get one), the first (remainder % nrings) rings one azimuthal cell more."""
+    if n_target % 2:
+        raise ValueError("ring_bore_mesh: n_target must be even (two triangles per grid cell)")
     rng = np.random.default_rng(seed)
     zc = (np.arange(nrings) - (nrings - 1) / 2.0) * (z_span / nrings)
     a_ring = 2 * np.pi * r_ring * ring_width
     a_out = 2 * np.pi * r_outer * (z_span + 0.5)
     total = nrings * a_ring + a_out
-    n_ring = int(n_target * a_ring / total) // 2
-    n_out = (n_target // 2) - nrings * n_ring
+    cells = n_target // 2
+    n_out = cells - nrings * (int(n_target * a_ring / total) // 2)
+    nz = max(1, int(round(np.sqrt(n_out * (z_span + 0.5) / (2 * np.pi * r_outer)))))
+    nphi_out = max(8, n_out // nz)
+    left = cells - nz * nphi_out  # cells of the rings
+    if left < 8 * nrings:
+        raise ValueError("ring_bore_mesh: n_target too small for the rings")
     parts = []
     for k in range(nrings):
-        nz = max(1, int(round(np.sqrt(n_ring * ring_width / (2 * np.pi * r_ring)))))
-        nphi = max(8, n_ring // nz)
-        recs = _cylinder(r_ring, zc[k] - ring_width / 2, zc[k] + ring_width / 2, nphi, nz, rng, 0.0)
+        nphi = left // nrings + (1 if k < left % nrings else 0)
+        recs = _cylinder(r_ring, zc[k] - ring_width / 2, zc[k] + ring_width / 2, nphi, 1, rng, 0.0)
         v = 0.0 if volts is None else volts[k]
         parts.append(_mesh(recs, TRIANGLE, v, k))
-    nz = max(1, int(round(np.sqrt(n_out * (z_span + 0.5) / (2 * np.pi * r_outer)))))
-    nphi = max(8, n_out // nz)
-    recs = _cylinder(r_outer, -(z_span + 0.5) / 2, (z_span + 0.5) / 2, nphi, nz, rng, 0.0, inward=True)
+    recs = _cylinder(r_outer, -(z_span + 0.5) / 2, (z_span + 0.5) / 2, nphi_out, nz, rng, 0.0, inward=True)
     parts.append(_mesh(recs, TRIANGLE, 0.0, nrings))
-    return concat(*parts)
+    mesh = concat(*parts)
+    assert mesh.n == n_target
+    return mesh
 
 
 def sphere_mesh(subdiv: int = 3, radius: float = 1.0, volts: float = 1.0) -> Mesh:

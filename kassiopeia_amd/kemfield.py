@@ -262,12 +262,48 @@ class KBoundaryIntegralMatrix:
             pass
 
 
-class KKrylovChargeDensitySolver:
+class KChargeDensitySolver:
+    """Base of the charge-density solvers: the solution cache's hash properties and the
+    thresholds a solver looks up and stores solutions with (KChargeDensitySolver.hh:17-44;
+    FindSolution / SaveSolution in KChargeDensitySolver.cc:51-280, through
+    kassiopeia_amd.cache)."""
+
+    def __init__(self):
+        self.hash_masked_bits = 20
+        self.hash_threshold = 1e-14
+
+    def SetHashProperties(self, masked_bits, hash_threshold):
+        self.hash_masked_bits = int(masked_bits)
+        self.hash_threshold = float(hash_threshold)
+
+    # the tolerance a stored solution must meet, and the threshold a new one is stored with
+    # (KKrylovChargeDensitySolver.cc:66-75: the tolerance / the final residual norm)
+    def find_threshold(self):
+        return self.kw["tolerance"]
+
+    def save_threshold(self, stats):
+        return stats.residual
+
+    def FindSolution(self, directory, mesh):
+        from . import cache
+
+        return cache.find_solution(directory, mesh, self.find_threshold(), self.hash_masked_bits,
+                                   self.hash_threshold)
+
+    def SaveSolution(self, directory, file, mesh, sigma, stats):
+        from . import cache
+
+        cache.save_solution(directory, file, mesh, sigma, self.save_threshold(stats), self.hash_masked_bits,
+                            self.hash_threshold)
+
+
+class KKrylovChargeDensitySolver(KChargeDensitySolver):
     """GMRES charge-density solve (krylov_bem_solver: solver_name, tolerance, max_iterations,
     iterations_between_restarts, time_limit_in_seconds)."""
 
     def __init__(self, tolerance=1e-8, max_iterations=1 << 30, iterations_between_restarts=0, solver_name="gmres",
                  precision="fp64", time_limit_in_seconds=0.0):
+        super().__init__()
         if solver_name != "gmres":
             raise KEMSimpleException(f"unknown Krylov solver '{solver_name}'")
         self.kw = dict(method="gmres", tolerance=tolerance, max_iterations=max_iterations,
@@ -277,15 +313,20 @@ class KKrylovChargeDensitySolver:
         return matrix.solve(b, **self.kw)
 
 
-class KRobinHoodChargeDensitySolver:
+class KRobinHoodChargeDensitySolver(KChargeDensitySolver):
     """Robin Hood charge-density solve (robin_hood_bem_solver: tolerance, check_sub_interval)."""
 
     def __init__(self, tolerance=1e-8, check_sub_interval=0, max_checks=1 << 30, time_limit_in_seconds=0.0):
+        super().__init__()
         self.kw = dict(method="robin_hood", tolerance=tolerance, check_interval=check_sub_interval,
                        max_iterations=max_checks, time_limit_s=time_limit_in_seconds)
 
     def solve(self, matrix, b):
         return matrix.solve(b, **self.kw)
+
+    # found and stored at the tolerance (KRobinHoodChargeDensitySolver.cc:89, 157, 212)
+    def save_threshold(self, stats):
+        return self.kw["tolerance"]
 
 
 class DeviceArray:
@@ -348,11 +389,20 @@ def field_sets(container: "KSurfaceContainer", sigma_sets, set_index, points, pr
             a.close()
 
 
-class KGaussianEliminationChargeDensitySolver:
+class KGaussianEliminationChargeDensitySolver(KChargeDensitySolver):
     """Direct solve (gaussian_elimination_bem_solver; KGaussianEliminationChargeDensitySolver.cc:29-46)."""
+
+    kw = {"method": "gauss"}
 
     def solve(self, matrix, b):
         return matrix.solve(b, method="gauss")
+
+    # found and stored at threshold 0 (KGaussianEliminationChargeDensitySolver.cc:35, 44)
+    def find_threshold(self):
+        return 0.0
+
+    def save_threshold(self, stats):
+        return 0.0
 
 
 class KIntegratingElectrostaticFieldSolver:
@@ -381,9 +431,14 @@ class KIntegratingElectrostaticFieldSolver:
 
 
 class KElectrostaticBoundaryField:
-    """Top-level field: Initialize() solves for the charge densities, then evaluates fields."""
+    """Top-level field: Initialize() finds or solves the charge densities, then evaluates
+    fields (KElectrostaticBoundaryField.cc:77-95).  With a directory set (SetDirectory), solved
+    charge densities are stored in SetFile's .kbd file there and found again by later
+    Initialize() calls on the same geometry (the reference's cache, with its hash properties:
+    20 masked bits, threshold 1e-14); without one nothing is read or written."""
 
-    def __init__(self, ctx: Context, mesh: Mesh, integrator="numeric", solver=None, precision="fp64"):
+    def __init__(self, ctx: Context, mesh: Mesh, integrator="numeric", solver=None, precision="fp64",
+                 directory=None, file=None):
         self.ctx = ctx
         self.mesh = mesh
         self.integrator = integrator
@@ -392,17 +447,44 @@ class KElectrostaticBoundaryField:
         self.container = None
         self.sigma = None
         self.stats = None
+        self.directory = directory
+        self.file = file
+        self.hash_masked_bits = 20
+        self.hash_threshold = 1e-14
+        self.cache_result = 0  # 0 solved, 1 stored solution, 2 superposed stored solutions
+        self.cache_file = ""
 
     def SetChargeDensitySolver(self, solver):
         self.solver = solver
 
+    def SetDirectory(self, directory):
+        self.directory = directory
+
+    def SetFile(self, file):
+        self.file = file
+
     def Initialize(self):
         self.ctx.set_integrator(self.integrator)
         self.container = KSurfaceContainer(self.ctx, self.mesh)
-        need32 = self.solver.kw.get("precision") == "mixed"
-        A = KBoundaryIntegralMatrix(self.container, fp64=True, fp32=need32)
-        self.sigma, self.stats, _ = self.solver.solve(A, self.container.rhs())
-        A.close()
+        cached = self.directory is not None and isinstance(self.solver, KChargeDensitySolver) and \
+            self.mesh.group_start is None
+        self.cache_result, self.cache_file, self.stats = 0, "", None
+        if cached:
+            self.solver.SetHashProperties(self.hash_masked_bits, self.hash_threshold)
+            kind, sigma, fname = self.solver.FindSolution(self.directory, self.mesh)
+            if kind:
+                self.cache_result, self.cache_file, self.sigma = kind, fname, sigma
+        if not self.cache_result:
+            need32 = self.solver.kw.get("precision") == "mixed"
+            A = KBoundaryIntegralMatrix(self.container, fp64=True, fp32=need32)
+            self.sigma, self.stats, _ = self.solver.solve(A, self.container.rhs())
+            A.close()
+            if cached:
+                from . import cache
+
+                if self.file is None:
+                    self.file = cache.default_file_name()
+                self.solver.SaveSolution(self.directory, self.file, self.mesh, self.sigma, self.stats)
         self.field = KIntegratingElectrostaticFieldSolver(self.container, self.sigma, self.precision)
 
     def _ready(self):
