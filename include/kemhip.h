@@ -45,6 +45,8 @@ extern "C" {
 /* ---- precision modes */
 #define KEM_FP64 0  /* plain path: FP64 everywhere */
 #define KEM_MIXED 1 /* FP32 far-field cubature / FP32 matrix; FP64 near field, refinement, sums */
+#define KEM_MIXED_BF16 2 /* solves and block products only: BF16 matrix on BF16 MFMA (FP32 accumulation),
+                            FP64 refinement */
 
 /* ---- matrix storage flags for kem_matrix_assemble */
 /* shape types and boundary types of kem_mesh_create (KTriangle / KRectangle / KLineSegment /
@@ -59,6 +61,7 @@ extern "C" {
 
 #define KEM_STORE_FP64 1
 #define KEM_STORE_FP32 2
+#define KEM_STORE_BF16 4 /* entries rounded to FP32, then BF16 (nearest even); packed for the MFMA operand */
 
 /* ---- solvers (KKrylovSolverFactory.hh:40-199, KRobinHood.hh, KGaussianElimination.hh) */
 #define KEM_SOLVER_GMRES 0
@@ -72,7 +75,8 @@ typedef struct kem_matrix kem_matrix;
 
 typedef struct kem_solve_opts {
     int method;             /* KEM_SOLVER_* */
-    int precision;          /* KEM_FP64, or KEM_MIXED: FP32-matrix inner solves + FP64 refinement */
+    int precision;          /* KEM_FP64, or KEM_MIXED: FP32-matrix inner solves + FP64 refinement, or
+                               KEM_MIXED_BF16: BF16-matrix inner solves (GMRES) + FP64 refinement */
     double tolerance;       /* relative: ||r||/||b|| (GMRES), ||r||_inf/||b||_inf (Robin Hood) */
     int max_iterations;     /* GMRES iterations (all restarts), Robin Hood norm checks */
     int restart;            /* GMRES iterations between restarts; 0 = never */
@@ -116,6 +120,10 @@ int kem_ctx_set_mixed_fp64_ratio(kem_ctx* ctx, double ratio);
 #define KEM_TUNE_GEMV_BLOCKS 3
 #define KEM_TUNE_BMM_BLOCKS 4
 #define KEM_TUNE_NEAR_BLOCKS 5 /* blocks of the near pass beside a single-set mixed far pass (0: one per 256 points; default 384) */
+/* wavefront tile of the FP32 MFMA block product: 64 rows (one float per lane and column), or
+   128 rows (four consecutive rows per lane in one 16-byte load; matrices with n % 4 == 0);
+   default 64 */
+#define KEM_TUNE_BMM_ROWS 6
 int kem_ctx_set_tuning(kem_ctx* ctx, int param, int64_t value);
 int kem_ctx_get_tuning(const kem_ctx* ctx, int param, int64_t* value);
 /* Instrumentation, no reference counterpart: with timing on, each field evaluation records HIP
@@ -199,6 +207,17 @@ int kem_field_graph_create(kem_ctx* ctx, const kem_mesh* mesh, const double* sig
                            double* ez, kem_graph** out);
 int kem_graph_launch(kem_graph* graph);
 void kem_graph_destroy(kem_graph* graph);
+/* Step-level field of several boundary fields: KSRootElectricField::CalculateFieldAndPotential
+   (Kassiopeia/Simulation/Source/KSRootElectricField.cxx:104-121) sums every field's (E, phi) per
+   sample point, in the order the fields were added, from zero.  Here nfields meshes of this
+   context with their charge densities (device arrays, one per mesh) and precisions (KEM_FP64 /
+   KEM_MIXED per field) are evaluated over all npts sample points of a step, one pass per field,
+   and summed on the device in the same order (the same sums as per point).  Device SoA arrays,
+   asynchronous on the context's stream; nfields = 0 gives zeros.  The reference rejects
+   non-finite sample points (KSFieldError); here they give non-finite results. */
+int kem_field_eval_fields(kem_ctx* ctx, int64_t nfields, const kem_mesh* const* meshes, const double* const* d_sigmas,
+                          const int* precisions, int64_t npts, const double* d_x, const double* d_y,
+                          const double* d_z, double* d_phi, double* d_ex, double* d_ey, double* d_ez);
 int kem_field_eval_sets(kem_ctx* ctx, const kem_mesh* mesh, int64_t nsets, const double* sigma, const int32_t* set,
                         int64_t npts, const double* x, const double* y, const double* z, int precision, double* phi,
                         double* ex, double* ey, double* ez);
@@ -312,7 +331,9 @@ int kem_cache_find(const char* directory, int64_t n, const int32_t* shape_type, 
                    int64_t file_cap);
 
 /* Y = A X for nrhs vectors (host, column-major n x nrhs): KSquareMatrix::Multiply
-   (KMatrix.hh:92-111) over a block; KEM_MIXED uses the FP32 matrix on MFMA. */
+   (KMatrix.hh:92-111) over a block; KEM_MIXED uses the FP32 matrix on FP32 MFMA, KEM_MIXED_BF16
+   the BF16 matrix on BF16 MFMA (X rounded to BF16, FP32 accumulation, FP64 sum of the split-K
+   partials). */
 int kem_matrix_multiply_batch(kem_matrix* A, int precision, int64_t nrhs, const double* X, double* Y);
 
 #ifdef __cplusplus

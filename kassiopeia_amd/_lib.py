@@ -11,11 +11,12 @@ LIBPATH = os.path.join(HERE, "libkemhip.so")
 KEM_OK = 0
 KEM_W_NOT_CONVERGED = 1
 KEM_E_ARG, KEM_E_HIP, KEM_E_NAME, KEM_E_GEOM, KEM_E_STATE = -1, -2, -3, -4, -5
-KEM_FP64, KEM_MIXED = 0, 1
-KEM_STORE_FP64, KEM_STORE_FP32 = 1, 2
+KEM_FP64, KEM_MIXED, KEM_MIXED_BF16 = 0, 1, 2
+KEM_STORE_FP64, KEM_STORE_FP32, KEM_STORE_BF16 = 1, 2, 4
 KEM_SOLVER_GMRES, KEM_SOLVER_ROBIN_HOOD, KEM_SOLVER_GAUSS = 0, 1, 2
 KEM_TUNE_FAR_RANGE_FP64, KEM_TUNE_FAR_RANGE_MIXED, KEM_TUNE_GEMV_BLOCKS, KEM_TUNE_BMM_BLOCKS = 1, 2, 3, 4
 KEM_TUNE_NEAR_BLOCKS = 5
+KEM_TUNE_BMM_ROWS = 6
 
 
 class KemError(RuntimeError):
@@ -83,6 +84,7 @@ SIGNATURES = {
     "kem_field_graph_create": (_INT, [_P, _P, _P, _I64, _P, _P, _P, _INT, _P, _P, _P, _P, ctypes.POINTER(_P)]),
     "kem_graph_launch": (_INT, [_P]),
     "kem_graph_destroy": (None, [_P]),
+    "kem_field_eval_fields": (_INT, [_P, _I64, _P, _P, _P, _I64, _P, _P, _P, _P, _P, _P, _P]),
     "kem_field_eval_sets": (_INT, [_P, _P, _I64, _P, _P, _I64, _P, _P, _P, _INT, _P, _P, _P, _P]),
     "kem_field_census": (_INT, [_P, _P, _I64, _P, _P, _P, _INT, _P]),
     "kem_field_plan": (_INT, [_INT, _INT, _INT, _INT, _I64, _I64, _I64, _I64, _I64, _P]),

@@ -10,6 +10,8 @@ The C-ABI exposes four launch-shape knobs (``kem_ctx_set_tuning``):
 * ``KEM_TUNE_BMM_BLOCKS``: target block count of the multi-right-hand-side block product.
 * ``KEM_TUNE_NEAR_BLOCKS``: blocks of the near pass that runs beside the mixed far pass on a
   second stream (0: one per 256 points).  Its timed region is the whole field pass.
+* ``KEM_TUNE_BMM_ROWS``: the wavefront tile of the FP32 MFMA block product, 64 rows (one float
+  per lane and column) or 128 rows (a 16-byte load of four rows per lane feeding four MFMAs).
 
 Each knob is swept over its candidates; every candidate runs once untimed and then ``repeats``
 times, timed by the HIP events the library records around the kernels the knob shapes
@@ -29,10 +31,11 @@ FAR_RANGE_CANDIDATES = (0, 640, 1280, 2560, 5120, 10240)
 GEMV_CANDIDATES = (512, 1024, 2048, 4096, 8192)
 BMM_CANDIDATES = (640, 1280, 2560, 5120, 10240)
 NEAR_CANDIDATES = (0, 128, 256, 384, 512)
+BMM_ROW_CANDIDATES = (64, 128)
 
 _KNOB = {"far_range_fp64": _lib.KEM_TUNE_FAR_RANGE_FP64, "far_range_mixed": _lib.KEM_TUNE_FAR_RANGE_MIXED,
          "gemv_blocks": _lib.KEM_TUNE_GEMV_BLOCKS, "bmm_blocks": _lib.KEM_TUNE_BMM_BLOCKS,
-         "near_blocks": _lib.KEM_TUNE_NEAR_BLOCKS}
+         "near_blocks": _lib.KEM_TUNE_NEAR_BLOCKS, "bmm_rows": _lib.KEM_TUNE_BMM_ROWS}
 
 
 def get(ctx: K.Context, name: str) -> int:
@@ -127,10 +130,19 @@ def tune_matvec(matrix: K.KBoundaryIntegralMatrix, repeats=5, candidates=GEMV_CA
 
 
 def tune_block_product(matrix: K.KBoundaryIntegralMatrix, nrhs=64, precision="mixed", repeats=3,
-                       candidates=BMM_CANDIDATES):
-    """Split-K target of the block product with nrhs right-hand sides (MFMA for "mixed")."""
+                       candidates=BMM_CANDIDATES, rows=BMM_ROW_CANDIDATES):
+    """Wavefront tile and split-K target of the block product with nrhs right-hand sides (MFMA
+    for "mixed" / "bf16"): the tile first at the current split target, then the split target
+    with the chosen tile.  Returns the split target's sweep with the tile's under "rows"."""
     X = np.random.default_rng(1).normal(size=(matrix.n, nrhs))
-    return _sweep(matrix.ctx, "bmm_blocks", candidates, lambda: matrix.MultiplyBatch(X, precision), repeats)
+
+    def run():
+        matrix.MultiplyBatch(X, precision)
+
+    tile = _sweep(matrix.ctx, "bmm_rows", rows if matrix.n % 4 == 0 else (64,), run, repeats)
+    res = _sweep(matrix.ctx, "bmm_blocks", candidates, run, repeats)
+    res["rows"] = tile
+    return res
 
 
 def current(ctx: K.Context) -> dict:

@@ -18,8 +18,17 @@ namespace {
 
 constexpr int kBlock = 256;
 typedef float f16v __attribute__((ext_vector_type(16)));
+typedef __bf16 bf8v __attribute__((ext_vector_type(8)));
 
 inline unsigned nblocks(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+
+// FP32 -> BF16 bit pattern, round to nearest even (finite values)
+__device__ __forceinline__ uint16_t bf16_bits(float f)
+{
+    uint32_t u = __float_as_uint(f);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
 
 // ------------------------------------------------------------------ FP32 MFMA block product
 // part[s][i][c] = sum_{j in split s} A32(i, j) * Vt[j][c]   (c < KP = 32*NT)
@@ -83,13 +92,87 @@ __global__ __launch_bounds__(kBlock) void k_bmm32(const float* __restrict__ A, i
             acc1[t] = __builtin_amdgcn_mfma_f32_32x32x2f32(a1, b[t], acc1[t], 0, 0, 0);
         }
     }
-    float* out = part + (int64_t)blockIdx.y * n * KP;
+    // rows i0 + 4 lk + rr, rr = 8 (v / 4) + v % 4 (+ 32 for the second tile): one base address
+    // and constant offsets (per-store 64-bit addresses took 135 VGPRs and 2 wavefronts per SIMD)
+    float* out = part + (int64_t)blockIdx.y * n * KP + (i0 + 4 * lk) * KP + lr;
+    const int64_t lim = n - i0 - 4 * lk;
     for (int t = 0; t < NT; t++)
         for (int v = 0; v < 16; v++) {
-            const int64_t r = 8 * (v >> 2) + 4 * lk + (v & 3);
-            if (i0 + r < n) out[(i0 + r) * KP + 32 * t + lr] = acc0[t][v];
-            if (i0 + 32 + r < n) out[(i0 + 32 + r) * KP + 32 * t + lr] = acc1[t][v];
+            const int rr = 8 * (v >> 2) + (v & 3);
+            if (rr < lim) out[rr * KP + 32 * t] = acc0[t][v];
+            if (rr + 32 < lim) out[(rr + 32) * KP + 32 * t] = acc1[t][v];
         }
+}
+
+// The same product with 16-byte row loads (n % 4 == 0): lane l loads rows i0 + 4 (l % 32) .. + 3
+// of column j + l / 32 as one float4 and feeds its four values to four MFMAs, row subtile m
+// taking rows i0 + 4 r + m (r = the MFMA row index), so a wavefront covers 128 rows, 32 lanes read
+// 512 contiguous bytes per column, and each panel load serves four MFMAs instead of two.
+template <int NT>
+__global__ __launch_bounds__(kBlock) void k_bmm32q(const float* __restrict__ A, int64_t n,
+                                                   const float* __restrict__ Vt, int64_t per_split,
+                                                   float* __restrict__ part, const int* __restrict__ skip)
+{
+    if (skip && *skip) return;
+    constexpr int KP = 32 * NT;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int64_t i0 = ((int64_t)blockIdx.x * 4 + wave) * 128;
+    if (i0 >= n) return;  // whole wavefront beyond the last row
+    const int64_t j0 = (int64_t)blockIdx.y * per_split;
+    const int64_t j1 = std::min<int64_t>(n, j0 + per_split);
+    const int lr = lane & 31, lk = lane >> 5;
+    // rows past n read the last four rows (n % 4 == 0 keeps the float4 aligned); never stored
+    const int64_t ra = std::min<int64_t>(i0 + 4 * lr, n - 4);
+    const float4* __restrict__ A4 = reinterpret_cast<const float4*>(A);
+    f16v acc[4][NT];
+    for (int m = 0; m < 4; m++)
+        for (int t = 0; t < NT; t++)
+            for (int v = 0; v < 16; v++) acc[m][t][v] = 0.f;
+    constexpr int KB = 4;
+    int64_t j = j0;
+    for (; j + 2 * KB <= j1; j += 2 * KB) {
+        float4 a[KB];
+        float b[KB][NT];
+        for (int q = 0; q < KB; q++) {
+            const int64_t js = j + 2 * q + lk;
+            a[q] = A4[(js * n + ra) >> 2];
+            for (int t = 0; t < NT; t++) b[q][t] = Vt[js * KP + 32 * t + lr];
+        }
+        for (int q = 0; q < KB; q++)
+            for (int t = 0; t < NT; t++) {
+                acc[0][t] = __builtin_amdgcn_mfma_f32_32x32x2f32(a[q].x, b[q][t], acc[0][t], 0, 0, 0);
+                acc[1][t] = __builtin_amdgcn_mfma_f32_32x32x2f32(a[q].y, b[q][t], acc[1][t], 0, 0, 0);
+                acc[2][t] = __builtin_amdgcn_mfma_f32_32x32x2f32(a[q].z, b[q][t], acc[2][t], 0, 0, 0);
+                acc[3][t] = __builtin_amdgcn_mfma_f32_32x32x2f32(a[q].w, b[q][t], acc[3][t], 0, 0, 0);
+            }
+    }
+    for (; j < j1; j += 2) {
+        const int64_t jj = j + lk;
+        const bool ok = jj < j1;
+        const int64_t js = ok ? jj : j;  // j < j1: a valid column
+        float4 a = A4[(js * n + ra) >> 2];
+        float b[NT];
+        for (int t = 0; t < NT; t++) b[t] = ok ? Vt[js * KP + 32 * t + lr] : 0.f;
+        for (int t = 0; t < NT; t++) {
+            acc[0][t] = __builtin_amdgcn_mfma_f32_32x32x2f32(a.x, b[t], acc[0][t], 0, 0, 0);
+            acc[1][t] = __builtin_amdgcn_mfma_f32_32x32x2f32(a.y, b[t], acc[1][t], 0, 0, 0);
+            acc[2][t] = __builtin_amdgcn_mfma_f32_32x32x2f32(a.z, b[t], acc[2][t], 0, 0, 0);
+            acc[3][t] = __builtin_amdgcn_mfma_f32_32x32x2f32(a.w, b[t], acc[3][t], 0, 0, 0);
+        }
+    }
+    // accumulator v of subtile m: MFMA row 8 (v / 4) + 4 lk + v % 4, matrix row i0 + 4 (that) + m;
+    // stored in groups of 4 v (32 rows apart) from a base that advances by 32 rows
+    float* out = part + (int64_t)blockIdx.y * n * KP + (i0 + 16 * lk) * KP + lr;
+    const int64_t lim = n - i0 - 16 * lk;
+    for (int v4 = 0; v4 < 4; v4++) {
+        for (int m = 0; m < 4; m++)
+            for (int t = 0; t < NT; t++)
+                for (int v = 0; v < 4; v++) {
+                    const int rr = 4 * v + m;  // row offset within this group of 16 rows
+                    if (32 * v4 + rr < lim) out[rr * KP + 32 * t] = acc[m][t][4 * v4 + v];
+                }
+        out += 32 * KP;
+    }
 }
 
 // W[c*ldw + i] = sum_s part[s][i][c] for c < ncols: 64-row tiles summed in FP64 (splits in
@@ -116,6 +199,125 @@ __global__ __launch_bounds__(kBlock) void k_bmm32_reduce(const float* __restrict
         const int64_t i = i0 + ii;
         if (c < ncols && i < n) W[c * ldw + i] = sh[c][ii];
     }
+}
+
+// ------------------------------------------------------------------ BF16 MFMA block product
+// BF16 matrix storage, packed for the MFMA operand: A16[(g*n + i)*8 + m] = A(i, 8g + m) (row
+// i, column 8g + m; columns past n are zero), so lane i of a wavefront loads the 8 consecutive
+// columns of its row in one 16-byte load and 32 lanes read 512 contiguous bytes.  Entries are
+// rounded to FP32 and then to BF16, each to nearest even.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_pack_bf16(const T* __restrict__ A, int64_t n, int64_t ngroups,
+                                                      uint4* __restrict__ A16)
+{
+    const int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x;  // e = g*n + i
+    if (e >= ngroups * n) return;
+    const int64_t g = e / n, i = e - g * n;
+    uint16_t v[8];
+    for (int m = 0; m < 8; m++) {
+        const int64_t j = 8 * g + m;
+        v[m] = j < n ? bf16_bits((float)A[j * n + i]) : (uint16_t)0;
+    }
+    uint4 w;
+    w.x = (uint32_t)v[0] | ((uint32_t)v[1] << 16);
+    w.y = (uint32_t)v[2] | ((uint32_t)v[3] << 16);
+    w.z = (uint32_t)v[4] | ((uint32_t)v[5] << 16);
+    w.w = (uint32_t)v[6] | ((uint32_t)v[7] << 16);
+    A16[e] = w;
+}
+
+// BF16 panel of the right-hand sides: Vp[(g*kp + c)*8 + m] = X(8g + m, c) (zero past n rows and
+// ncols columns)
+__global__ __launch_bounds__(kBlock) void k_panel_bf16(const double* __restrict__ X, int64_t ldx, int64_t n,
+                                                       int ncols, int kp, int64_t ngroups, uint4* __restrict__ Vp)
+{
+    const int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x;  // e = g*kp + c
+    if (e >= ngroups * kp) return;
+    const int64_t g = e / kp;
+    const int c = (int)(e - g * kp);
+    uint16_t v[8];
+    for (int m = 0; m < 8; m++) {
+        const int64_t i = 8 * g + m;
+        v[m] = (c < ncols && i < n) ? bf16_bits((float)X[c * ldx + i]) : (uint16_t)0;
+    }
+    uint4 w;
+    w.x = (uint32_t)v[0] | ((uint32_t)v[1] << 16);
+    w.y = (uint32_t)v[2] | ((uint32_t)v[3] << 16);
+    w.z = (uint32_t)v[4] | ((uint32_t)v[5] << 16);
+    w.w = (uint32_t)v[6] | ((uint32_t)v[7] << 16);
+    Vp[e] = w;
+}
+
+// part[s][i][c] = sum_{j in split s} A16(i, j) * Vp(j, c) with v_mfma_f32_32x32x16_bf16 (FP32
+// accumulation).  A wavefront computes a 64-row x KP tile: per step of 16 columns lane l loads
+// row l % 32 (+ 32 for the second row tile) of column group g + l / 32 (8 BF16 values, one
+// 16-byte load) as operand A, and the same column group of panel column l % 32 (+ 32 t) as
+// operand B; the accumulators are laid out as in k_bmm32.  Column groups [g0, g1) per
+// blockIdx.y; the loads of KB steps are issued before their MFMAs.
+template <int NT>
+__global__ __launch_bounds__(kBlock) void k_bmm16(const bf8v* __restrict__ A16, int64_t n,
+                                                  const bf8v* __restrict__ Vp, int64_t ngroups, int64_t gper,
+                                                  float* __restrict__ part, const int* __restrict__ skip)
+{
+    if (skip && *skip) return;
+    constexpr int KP = 32 * NT;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int64_t i0 = ((int64_t)blockIdx.x * 4 + wave) * 64;
+    if (i0 >= n) return;  // whole wavefront beyond the last row
+    const int64_t g0 = (int64_t)blockIdx.y * gper;
+    const int64_t g1 = std::min<int64_t>(ngroups, g0 + gper);
+    const int lr = lane & 31, lk = lane >> 5;
+    const int64_t ra = std::min<int64_t>(i0 + lr, n - 1), rb = std::min<int64_t>(i0 + 32 + lr, n - 1);
+    f16v acc0[NT], acc1[NT];
+    for (int t = 0; t < NT; t++)
+        for (int v = 0; v < 16; v++) {
+            acc0[t][v] = 0.f;
+            acc1[t][v] = 0.f;
+        }
+    constexpr int KB = NT == 1 ? 4 : 2;  // keeps 64 right-hand sides at 2 wavefronts per SIMD
+    int64_t g = g0;
+    for (; g + 2 * KB <= g1; g += 2 * KB) {
+        bf8v a0[KB], a1[KB], b[KB][NT];
+        for (int q = 0; q < KB; q++) {
+            const int64_t gg = g + 2 * q + lk;
+            a0[q] = A16[gg * n + ra];
+            a1[q] = A16[gg * n + rb];
+            for (int t = 0; t < NT; t++) b[q][t] = Vp[gg * KP + 32 * t + lr];
+        }
+        for (int q = 0; q < KB; q++)
+            for (int t = 0; t < NT; t++) {
+                acc0[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0[q], b[q][t], acc0[t], 0, 0, 0);
+                acc1[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1[q], b[q][t], acc1[t], 0, 0, 0);
+            }
+    }
+    for (; g < g1; g += 2) {
+        const int64_t gg = g + lk;
+        const bool ok = gg < g1;
+        const int64_t gs = ok ? gg : g;  // g < g1: a valid column group
+        bf8v a0 = A16[gs * n + ra], a1 = A16[gs * n + rb];
+        bf8v b[NT];
+        for (int t = 0; t < NT; t++) b[t] = Vp[gs * KP + 32 * t + lr];
+        if (!ok) {
+            for (int m = 0; m < 8; m++) {
+                a0[m] = (__bf16)0.f;
+                a1[m] = (__bf16)0.f;
+            }
+        }
+        for (int t = 0; t < NT; t++) {
+            acc0[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, b[t], acc0[t], 0, 0, 0);
+            acc1[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, b[t], acc1[t], 0, 0, 0);
+        }
+    }
+    // the lane's rows are i0 + 4 lk + rr with rr = 8 (v / 4) + v % 4 (+ 32 for the second tile):
+    // one base address and constant offsets, so the stores need no per-store address registers
+    float* out = part + (int64_t)blockIdx.y * n * KP + (i0 + 4 * lk) * KP + lr;
+    const int64_t lim = n - i0 - 4 * lk;
+    for (int t = 0; t < NT; t++)
+        for (int v = 0; v < 16; v++) {
+            const int rr = 8 * (v >> 2) + (v & 3);
+            if (rr < lim) out[rr * KP + 32 * t] = acc0[t][v];
+            if (rr + 32 < lim) out[(rr + 32) * KP + 32 * t] = acc1[t][v];
+        }
 }
 
 // ------------------------------------------------------------------ FP64 block product (VALU)
@@ -231,6 +433,30 @@ __global__ __launch_bounds__(kBlock) void k_bnorm(const double* __restrict__ W, 
     if (threadIdx.x == 0) nrm[r * stride] = sqrt(s);
 }
 
+// Panel entry (row i, column r) of a block operand: FP32 / FP64 panels are row-major n x kp;
+// BF16 panels (uint16_t bit patterns) are packed in groups of 8 rows, Vp[((i/8)*kp + r)*8 + i%8],
+// so that one 16-byte load holds the 8 consecutive k-values an MFMA lane takes.
+template <typename T>
+__device__ __forceinline__ int64_t panel_index(int64_t i, int r, int kp)
+{
+    return i * kp + r;
+}
+template <>
+__device__ __forceinline__ int64_t panel_index<uint16_t>(int64_t i, int r, int kp)
+{
+    return ((i >> 3) * kp + r) * 8 + (i & 7);
+}
+template <typename T>
+__device__ __forceinline__ T panel_value(double v)
+{
+    return (T)v;
+}
+template <>
+__device__ __forceinline__ uint16_t panel_value<uint16_t>(double v)
+{
+    return bf16_bits((float)v);
+}
+
 // V_r q = W_r * (1 / nrm_r); panel column r = V_r q for active r, 0 otherwise
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_bnormalize(const double* __restrict__ W, const double* __restrict__ nrm,
@@ -244,7 +470,7 @@ __global__ __launch_bounds__(kBlock) void k_bnormalize(const double* __restrict_
     const double d = nrm[r * stride];
     const double v = active[r] ? W[(int64_t)r * n + i] * (1.0 / d) : 0.;
     V[r * vstride + (int64_t)q * n + i] = v;
-    Vt[i * kp + r] = (T)v;
+    Vt[panel_index<T>(i, r, kp)] = panel_value<T>(v);
 }
 
 // Z_r = a X_r + b Y_r over all r (column-major, ld n), elementwise
@@ -287,7 +513,7 @@ __global__ __launch_bounds__(kBlock) void k_bnormalize_idx(const double* __restr
     if (i >= n) return;
     const double v = active[r] ? W[(int64_t)r * n + i] * (1.0 / nrm[r]) : 0.;
     V[r * vstride + (int64_t)(ints[0] + 1) * n + i] = v;
-    Vt[i * kp + r] = (T)v;
+    Vt[panel_index<T>(i, r, kp)] = panel_value<T>(v);
 }
 
 // Column j of each active right-hand side's Hessenberg matrix, its Givens rotations and the
@@ -381,10 +607,15 @@ int bmm_splits(int64_t n, int64_t tiles)
 
 }  // namespace
 
-size_t bmm_workspace_bytes(int64_t n, int nrhs, bool fp32)
+size_t bmm_workspace_bytes(int64_t n, int nrhs, int prec)
 {
     const int kp = nrhs <= 32 ? 32 : 64;
-    if (fp32) {
+    if (prec == kBmmBf16) {
+        const int64_t tiles = (n + 63) / 64, ngroups = (n + 7) / 8;
+        const int s = bmm_splits(n, tiles);
+        return sizeof(float) * (size_t)s * n * kp + sizeof(uint16_t) * (size_t)ngroups * 8 * kp + 512;
+    }
+    if (prec == kBmmFp32) {
         const int64_t tiles = (n + 63) / 64;
         const int s = bmm_splits(n, tiles);
         return sizeof(float) * ((size_t)s * n * kp + (size_t)n * kp) + 256;
@@ -399,7 +630,7 @@ hipError_t launch_bmm(hipStream_t st, const float* A32, const double* A64, int64
                       int nrhs, double* Y, int64_t ldy, void* ws, size_t ws_bytes, bool x_is_panel, const int* skip)
 {
     if (n <= 0 || nrhs <= 0) return hipSuccess;
-    if (nrhs > 64 || ws_bytes < bmm_workspace_bytes(n, nrhs, A32 != nullptr)) return hipErrorInvalidValue;
+    if (nrhs > 64 || ws_bytes < bmm_workspace_bytes(n, nrhs, A32 ? kBmmFp32 : kBmmFp64)) return hipErrorInvalidValue;
     if (A32) {
         const int nt = nrhs <= 32 ? 1 : 2, kp = 32 * nt;
         const int64_t tiles = (n + 63) / 64;
@@ -412,11 +643,20 @@ hipError_t launch_bmm(hipStream_t st, const float* A32, const double* A64, int64
         const float* P = x_is_panel ? (const float*)X : Vt;
         if (!x_is_panel)
             k_panel<float><<<nblocks(n * kp), kBlock, 0, st>>>(X, ldx, n, nrhs, kp, Vt);
-        dim3 grid((unsigned)((tiles + 3) / 4), (unsigned)ns);
-        if (nt == 1)
-            k_bmm32<1><<<grid, kBlock, 0, st>>>(A32, n, P, per, part, skip);
-        else
-            k_bmm32<2><<<grid, kBlock, 0, st>>>(A32, n, P, per, part, skip);
+        if (launch_tuning().bmm_rows == 128 && n % 4 == 0 && n >= 4) {
+            // 128-row wavefront tiles, 512 rows per block
+            dim3 grid((unsigned)((n + 511) / 512), (unsigned)ns);
+            if (nt == 1)
+                k_bmm32q<1><<<grid, kBlock, 0, st>>>(A32, n, P, per, part, skip);
+            else
+                k_bmm32q<2><<<grid, kBlock, 0, st>>>(A32, n, P, per, part, skip);
+        } else {
+            dim3 grid((unsigned)((tiles + 3) / 4), (unsigned)ns);
+            if (nt == 1)
+                k_bmm32<1><<<grid, kBlock, 0, st>>>(A32, n, P, per, part, skip);
+            else
+                k_bmm32<2><<<grid, kBlock, 0, st>>>(A32, n, P, per, part, skip);
+        }
         const unsigned gr = (unsigned)((n + 63) / 64);
         if (kp == 32)
             k_bmm32_reduce<32><<<gr, kBlock, 0, st>>>(part, ns, n, nrhs, Y, ldy, skip);
@@ -439,6 +679,52 @@ hipError_t launch_bmm(hipStream_t st, const float* A32, const double* A64, int64
     else
         k_bmm64<64><<<grid, kBlock, 0, st>>>(A64, n, P, per, part, skip);
     k_bmm64_reduce<<<dim3((unsigned)blocks, (unsigned)nrhs), kBlock, 0, st>>>(part, ns, kb, n, nrhs, Y, ldy, skip);
+    return hipGetLastError();
+}
+
+hipError_t launch_pack_bf16(hipStream_t st, const double* A64, const float* A32, int64_t n, uint16_t* A16)
+{
+    if (n <= 0) return hipSuccess;
+    const int64_t ngroups = (n + 7) / 8;
+    if (A64)
+        k_pack_bf16<double><<<nblocks(ngroups * n), kBlock, 0, st>>>(A64, n, ngroups, (uint4*)A16);
+    else if (A32)
+        k_pack_bf16<float><<<nblocks(ngroups * n), kBlock, 0, st>>>(A32, n, ngroups, (uint4*)A16);
+    else
+        return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+size_t bf16_matrix_elements(int64_t n) { return (size_t)((n + 7) / 8) * 8 * (size_t)n; }
+
+hipError_t launch_bmm16(hipStream_t st, const uint16_t* A16, int64_t n, const void* X, int64_t ldx, int nrhs,
+                        double* Y, int64_t ldy, void* ws, size_t ws_bytes, bool x_is_panel, const int* skip)
+{
+    if (n <= 0 || nrhs <= 0) return hipSuccess;
+    if (nrhs > 64 || ws_bytes < bmm_workspace_bytes(n, nrhs, kBmmBf16)) return hipErrorInvalidValue;
+    const int nt = nrhs <= 32 ? 1 : 2, kp = 32 * nt;
+    const int64_t tiles = (n + 63) / 64, ngroups = (n + 7) / 8;
+    const int s = bmm_splits(n, tiles);
+    const int64_t gper = (ngroups + s - 1) / s;
+    const int ns = (int)((ngroups + gper - 1) / gper);
+    float* part = (float*)ws;
+    // the panel after the partial sums, 256-byte aligned
+    uintptr_t pv = (uintptr_t)(part + (size_t)s * n * kp);
+    pv = (pv + 255) & ~(uintptr_t)255;
+    uint4* Vp = (uint4*)pv;
+    const bf8v* P = x_is_panel ? (const bf8v*)X : (const bf8v*)Vp;
+    if (!x_is_panel)
+        k_panel_bf16<<<nblocks(ngroups * kp), kBlock, 0, st>>>((const double*)X, ldx, n, nrhs, kp, ngroups, Vp);
+    dim3 grid((unsigned)((tiles + 3) / 4), (unsigned)ns);
+    if (nt == 1)
+        k_bmm16<1><<<grid, kBlock, 0, st>>>((const bf8v*)A16, n, P, ngroups, gper, part, skip);
+    else
+        k_bmm16<2><<<grid, kBlock, 0, st>>>((const bf8v*)A16, n, P, ngroups, gper, part, skip);
+    const unsigned gr = (unsigned)((n + 63) / 64);
+    if (kp == 32)
+        k_bmm32_reduce<32><<<gr, kBlock, 0, st>>>(part, ns, n, nrhs, Y, ldy, skip);
+    else
+        k_bmm32_reduce<64><<<gr, kBlock, 0, st>>>(part, ns, n, nrhs, Y, ldy, skip);
     return hipGetLastError();
 }
 
@@ -467,11 +753,13 @@ hipError_t launch_bnorm(hipStream_t st, const double* W, int64_t n, int nrhs, do
 
 hipError_t launch_bnormalize(hipStream_t st, const double* W, const double* nrm, int stride, const int* active,
                              int64_t n, int nrhs, double* V, int64_t vstride, int q, int kp, float* Vt32,
-                             double* Vt64)
+                             double* Vt64, uint16_t* Vt16)
 {
     if (nrhs <= 0) return hipSuccess;
     dim3 grid(nblocks(n), (unsigned)nrhs);
-    if (Vt32)
+    if (Vt16)
+        k_bnormalize<uint16_t><<<grid, kBlock, 0, st>>>(W, nrm, stride, active, n, V, vstride, q, kp, Vt16);
+    else if (Vt32)
         k_bnormalize<float><<<grid, kBlock, 0, st>>>(W, nrm, stride, active, n, V, vstride, q, kp, Vt32);
     else
         k_bnormalize<double><<<grid, kBlock, 0, st>>>(W, nrm, stride, active, n, V, vstride, q, kp, Vt64);
@@ -483,8 +771,10 @@ hipError_t launch_batch_arnoldi_step(hipStream_t st, const BatchArnoldi& a)
     const int64_t n = a.n;
     const int nrhs = a.nrhs;
     int* stop = a.ints + 1;
-    hipError_t e = launch_bmm(st, a.use32 ? a.A32 : nullptr, a.use32 ? nullptr : a.A64, n,
-                              a.use32 ? (const double*)a.P32 : a.P64, n, nrhs, a.W, n, a.ws, a.ws_bytes, true, stop);
+    hipError_t e = a.A16 ? launch_bmm16(st, a.A16, n, a.P16, n, nrhs, a.W, n, a.ws, a.ws_bytes, true, stop)
+                         : launch_bmm(st, a.use32 ? a.A32 : nullptr, a.use32 ? nullptr : a.A64, n,
+                                      a.use32 ? (const double*)a.P32 : a.P64, n, nrhs, a.W, n, a.ws, a.ws_bytes, true,
+                                      stop);
     if (e != hipSuccess) return e;
     for (double* h : {a.h1, a.h2}) {
         k_bproj_idx<<<dim3((unsigned)(a.kmax + 1), (unsigned)nrhs), kBlock, 0, st>>>(a.V, a.vstride, a.W, n, h, a.ldh,
@@ -493,7 +783,10 @@ hipError_t launch_batch_arnoldi_step(hipStream_t st, const BatchArnoldi& a)
                                                                       stop);
     }
     k_bnorm<<<(unsigned)nrhs, kBlock, 0, st>>>(a.W, n, a.nrm, 1, stop);
-    if (a.use32)
+    if (a.A16)
+        k_bnormalize_idx<uint16_t><<<dim3(nblocks(n), (unsigned)nrhs), kBlock, 0, st>>>(a.W, a.nrm, a.active, n, a.V,
+                                                                                       a.vstride, a.kp, a.P16, a.ints);
+    else if (a.use32)
         k_bnormalize_idx<float><<<dim3(nblocks(n), (unsigned)nrhs), kBlock, 0, st>>>(a.W, a.nrm, a.active, n, a.V,
                                                                                     a.vstride, a.kp, a.P32, a.ints);
     else

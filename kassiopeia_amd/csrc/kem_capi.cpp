@@ -21,6 +21,9 @@
 // mixed precision: far pairs up to this distance ratio keep FP64 cubature; 0 = every far pair
 // in FP32 (the FP32 rule ladder alone keeps the global field error near 3e-8)
 constexpr double kMixedFp64RatioDefault = 0.0;
+// the inner tolerance of BF16-matrix solves: the BF16 product rounds its input vector to 8
+// significant bits, so an inner solve cannot reduce its true residual much below 2^-8
+constexpr double kBf16InnerTolerance = 1.0 / 256.0;
 // elements per XCD range of the large single-set far passes (kem::launch_field); 0 = no split
 constexpr int64_t kFarRangeDefault = 2560;
 
@@ -55,6 +58,8 @@ struct kem_ctx {
     double* scal = nullptr;  // small device scalars (dot products, Hessenberg column)
     int64_t scal_doubles = 0;
     FieldWs fws;  // workspace of eager field evaluations (grow-only)
+    double* acc4 = nullptr;  // one field's (phi, Ex, Ey, Ez) of a step-level sum over fields (grow-only)
+    int64_t acc4_doubles = 0;
     // far-kernel timing: one event pair per field evaluation since the last query
     static constexpr int kTimedMax = 256;
     bool timing = false;
@@ -98,6 +103,7 @@ struct kem_matrix {
     int64_t n = 0;
     double* A64 = nullptr;
     float* A32 = nullptr;
+    uint16_t* A16 = nullptr;  // BF16, packed in groups of 8 columns (kem::launch_pack_bf16)
     int trans = 0;  // solves run on A^T (kem_solve_transposed)
 };
 
@@ -279,6 +285,7 @@ void kem_ctx_destroy(kem_ctx* c)
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     dfree(c->ws);
     dfree(c->scal);
+    dfree(c->acc4);
     ws_release(c->fws);
     for (hipEvent_t& e : c->ev)
         if (e) (void)hipEventDestroy(e);
@@ -324,6 +331,10 @@ int kem_ctx_set_tuning(kem_ctx* c, int param, int64_t value)
         if (value < 0 || value > (1 << 20)) return fail(c, KEM_E_ARG, "tuning: block count out of [0, 2^20]");
         c->tuning.near_blocks = value;
         return KEM_OK;
+    case KEM_TUNE_BMM_ROWS:
+        if (value != 64 && value != 128) return fail(c, KEM_E_ARG, "tuning: block-product rows per wavefront 64 or 128");
+        c->tuning.bmm_rows = value;
+        return KEM_OK;
     default:
         return fail(c, KEM_E_ARG, "tuning: unknown parameter");
     }
@@ -338,6 +349,7 @@ int kem_ctx_get_tuning(const kem_ctx* c, int param, int64_t* value)
     case KEM_TUNE_GEMV_BLOCKS: *value = c->tuning.gemv_blocks; return KEM_OK;
     case KEM_TUNE_BMM_BLOCKS: *value = c->tuning.bmm_blocks; return KEM_OK;
     case KEM_TUNE_NEAR_BLOCKS: *value = c->tuning.near_blocks; return KEM_OK;
+    case KEM_TUNE_BMM_ROWS: *value = c->tuning.bmm_rows; return KEM_OK;
     default: return KEM_E_ARG;
     }
 }
@@ -764,6 +776,46 @@ void kem_graph_destroy(kem_graph* g)
     delete g;
 }
 
+int kem_field_eval_fields(kem_ctx* c, int64_t nfields, const kem_mesh* const* meshes, const double* const* sigmas,
+                          const int* precisions, int64_t npts, const double* x, const double* y, const double* z,
+                          double* phi, double* ex, double* ey, double* ez)
+{
+    if (!c || nfields < 0 || (nfields > 0 && (!meshes || !sigmas || !precisions))) return KEM_E_ARG;
+    if (npts > 0 && (!x || !y || !z || !phi || !ex || !ey || !ez)) return KEM_E_ARG;
+    for (int64_t f = 0; f < nfields; f++) {
+        if (!meshes[f] || !sigmas[f]) return fail(c, KEM_E_ARG, "fields: null mesh or charge densities");
+        if (meshes[f]->ctx != c) return fail(c, KEM_E_STATE, "fields: mesh belongs to another context");
+        if (precisions[f] != KEM_FP64 && precisions[f] != KEM_MIXED)
+            return fail(c, KEM_E_ARG, "fields: unknown precision mode");
+    }
+    if (npts <= 0) return KEM_OK;
+    KEM_CK(c, enter(c));
+    if (nfields == 0) {
+        // no field: zero field and potential (the reference's sums start from zero)
+        for (double* o : {phi, ex, ey, ez}) KEM_CK(c, hipMemsetAsync(o, 0, sizeof(double) * npts, c->stream));
+        return KEM_OK;
+    }
+    // the first field writes the outputs, each further one is evaluated aside and added
+    int rc = field_eval(c, meshes[0], sigmas[0], nullptr, 1, npts, x, y, z, precisions[0], phi, ex, ey, ez, c->fws,
+                        true);
+    if (rc != KEM_OK) return rc;
+    if (nfields > 1 && c->acc4_doubles < 4 * npts) {
+        KEM_CK(c, hipStreamSynchronize(c->stream));
+        dfree(c->acc4);
+        c->acc4_doubles = 0;
+        KEM_CK(c, dalloc(&c->acc4, 4 * npts));
+        c->acc4_doubles = 4 * npts;
+    }
+    double* t = c->acc4;
+    for (int64_t f = 1; f < nfields; f++) {
+        rc = field_eval(c, meshes[f], sigmas[f], nullptr, 1, npts, x, y, z, precisions[f], t, t + npts, t + 2 * npts,
+                        t + 3 * npts, c->fws, true);
+        if (rc != KEM_OK) return rc;
+        KEM_CK(c, kem::launch_accumulate4(c->stream, t, npts, phi, ex, ey, ez));
+    }
+    return KEM_OK;
+}
+
 int kem_field_eval_sets(kem_ctx* c, const kem_mesh* m, int64_t nsets, const double* sigma, const int32_t* set,
                         int64_t npts, const double* x, const double* y, const double* z, int precision, double* phi,
                         double* ex, double* ey, double* ez)
@@ -886,7 +938,8 @@ int kem_matrix_assemble(kem_ctx* c, const kem_mesh* m, int storage, kem_matrix**
     if (!c || !m || !out) return KEM_E_ARG;
     *out = nullptr;
     if (m->ctx != c) return fail(c, KEM_E_STATE, "mesh belongs to another context");
-    if (!(storage & (KEM_STORE_FP64 | KEM_STORE_FP32))) return fail(c, KEM_E_ARG, "matrix: no storage selected");
+    if (!(storage & (KEM_STORE_FP64 | KEM_STORE_FP32 | KEM_STORE_BF16)))
+        return fail(c, KEM_E_ARG, "matrix: no storage selected");
     KEM_CK(c, enter(c));
     kem_matrix* A = new kem_matrix();
     A->ctx = c;
@@ -894,11 +947,17 @@ int kem_matrix_assemble(kem_ctx* c, const kem_mesh* m, int storage, kem_matrix**
     const int64_t nn = m->n * m->n;
     hipError_t e = hipSuccess;
     if (storage & KEM_STORE_FP64) e = dalloc(&A->A64, nn);
-    if (e == hipSuccess && (storage & KEM_STORE_FP32)) e = dalloc(&A->A32, nn);
+    // BF16 storage is packed from the FP64 matrix, else from the FP32 one (assembled for it
+    // and dropped again when FP32 storage was not asked for)
+    const bool need32 = (storage & KEM_STORE_FP32) || ((storage & KEM_STORE_BF16) && !(storage & KEM_STORE_FP64));
+    if (e == hipSuccess && need32) e = dalloc(&A->A32, nn);
+    if (e == hipSuccess && (storage & KEM_STORE_BF16)) e = dalloc(&A->A16, (int64_t)kem::bf16_matrix_elements(m->n));
     if (e == hipSuccess)
         e = kem::launch_assemble(c->stream, c->policy, m->has_neumann, m->geom, m->nodes, m->recs, m->targets, m->n, A->A64,
                                  A->A32, m->has_axisym, m->gstart);
+    if (e == hipSuccess && A->A16) e = kem::launch_pack_bf16(c->stream, A->A64, A->A32, m->n, A->A16);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e == hipSuccess && A->A32 && !(storage & KEM_STORE_FP32)) dfree(A->A32);
     if (e != hipSuccess) {
         kem_matrix_destroy(A);
         return fail(c, KEM_E_HIP, std::string("assemble: ") + hipGetErrorString(e));
@@ -913,6 +972,7 @@ void kem_matrix_destroy(kem_matrix* A)
     if (A->ctx) (void)hipSetDevice(A->ctx->device);
     dfree(A->A64);
     dfree(A->A32);
+    dfree(A->A16);
     delete A;
 }
 
@@ -1025,6 +1085,10 @@ struct SolveCtl {
 // turns the remaining replays into no-ops.  Device vectors d_b, d_x (x holds x0).  On restart
 // (or at the basis cap) x <- x0 + V y.
 constexpr int kGmresReplays = 4;
+
+int gmres_batch(kem_matrix* A, int prec, int nrhs, const double* d_B, double* d_X, const std::vector<double>& tol,
+                int maxit, int restart, const SolveCtl& ctl, int* iters, std::vector<double>& resid,
+                std::vector<int>& conv_out);
 
 int gmres_dev(kem_matrix* A, bool use32, const double* d_b, double* d_x, double tol, int maxit, int restart,
               const SolveCtl& ctl, int* iters, double* resid)
@@ -1256,9 +1320,12 @@ int kem_solve(kem_matrix* A, const kem_solve_opts* opts, const double* b, double
             if (!A->A64) return fail(c, KEM_E_STATE, "FP64 solve needs FP64 matrix storage");
             status = gmres_dev(A, false, db.p, dx.p, o.tolerance, o.max_iterations, o.restart, ctl, &s.iterations,
                                &s.residual);
-        } else {
-            // FP32-matrix GMRES inner solves, FP64 residual refinement
-            if (!A->A32) return fail(c, KEM_E_STATE, "mixed solve needs FP32 matrix storage");
+        } else if (o.precision == KEM_MIXED || o.precision == KEM_MIXED_BF16) {
+            // FP32- (or BF16-) matrix GMRES inner solves, FP64 residual refinement
+            const bool bf16 = o.precision == KEM_MIXED_BF16;
+            if (!bf16 && !A->A32) return fail(c, KEM_E_STATE, "mixed solve needs FP32 matrix storage");
+            if (bf16 && !A->A16) return fail(c, KEM_E_STATE, "BF16 solve needs BF16 matrix storage");
+            if (bf16 && A->trans) return fail(c, KEM_E_ARG, "BF16 solves run on A, not on its transpose");
             if (!A->A64) return fail(c, KEM_E_STATE, "mixed solve needs FP64 matrix storage for the residual");
             KEM_CK(c, dalloc(&dr.p, n));
             KEM_CK(c, dalloc(&dd.p, n));
@@ -1275,9 +1342,18 @@ int kem_solve(kem_matrix* A, const kem_solve_opts* opts, const double* b, double
                 KEM_CK(c, hipMemsetAsync(dd.p, 0, sizeof(double) * n, c->stream));
                 int its = 0;
                 double res = 0.;
-                const double inner = std::max(o.inner_tolerance, 0.5 * o.tolerance / std::max(rel, 1e-300));
-                rc = gmres_dev(A, true, dr.p, dd.p, std::min(inner, 0.5), o.max_iterations - s.iterations, 0, ctl,
-                               &its, &res);
+                double inner = std::max(o.inner_tolerance, 0.5 * o.tolerance / std::max(rel, 1e-300));
+                if (bf16) inner = std::max(inner, kBf16InnerTolerance);
+                if (bf16) {
+                    // the lockstep solver with one right-hand side on BF16 MFMA block products
+                    std::vector<double> tol1(1, std::min(inner, 0.5)), res1;
+                    std::vector<int> conv1;
+                    rc = gmres_batch(A, kem::kBmmBf16, 1, dr.p, dd.p, tol1, o.max_iterations - s.iterations, 0,
+                                     ctl, &its, res1, conv1);
+                } else {
+                    rc = gmres_dev(A, true, dr.p, dd.p, std::min(inner, 0.5), o.max_iterations - s.iterations, 0,
+                                   ctl, &its, &res);
+                }
                 if (rc < 0) return rc;
                 s.iterations += its;
                 s.refinements++;
@@ -1285,6 +1361,8 @@ int kem_solve(kem_matrix* A, const kem_solve_opts* opts, const double* b, double
             }
             s.residual = rel;
             if (rel > o.tolerance) status = KEM_W_NOT_CONVERGED;
+        } else {
+            return fail(c, KEM_E_ARG, "unknown precision mode");
         }
     } else if (o.method == KEM_SOLVER_GAUSS) {
         // dense direct solve on a copy of the FP64 matrix (KGaussianEliminationChargeDensitySolver)
@@ -1398,13 +1476,19 @@ int kem_assemble_vjp(kem_ctx* c, const kem_mesh* m, const double* lambda, const 
 namespace {
 
 // panel width of the block product for nrhs right-hand sides
-int panel_cols(int nrhs, bool fp32) { return fp32 ? (nrhs <= 32 ? 32 : 64) : (nrhs <= 16 ? 16 : 64); }
+int panel_cols(int nrhs, int prec) { return prec != kem::kBmmFp64 ? (nrhs <= 32 ? 32 : 64) : (nrhs <= 16 ? 16 : 64); }
 
 // Y = A X for up to 64 right-hand sides (device, column-major with leading dimension n)
-int block_product(kem_matrix* A, bool use32, int nrhs, const double* d_X, double* d_Y, void** ws, size_t* ws_bytes)
+// the block-product precision of a solve precision: FP64 matrix, FP32 matrix or BF16 matrix
+int bmm_prec(int precision)
+{
+    return precision == KEM_MIXED_BF16 ? kem::kBmmBf16 : precision == KEM_MIXED ? kem::kBmmFp32 : kem::kBmmFp64;
+}
+
+int block_product(kem_matrix* A, int prec, int nrhs, const double* d_X, double* d_Y, void** ws, size_t* ws_bytes)
 {
     kem_ctx* c = A->ctx;
-    const size_t need = kem::bmm_workspace_bytes(A->n, nrhs, use32);
+    const size_t need = kem::bmm_workspace_bytes(A->n, nrhs, prec);
     if (*ws_bytes < need) {
         if (*ws) (void)hipFree(*ws);
         *ws = nullptr;
@@ -1412,8 +1496,12 @@ int block_product(kem_matrix* A, bool use32, int nrhs, const double* d_X, double
         KEM_CK(c, hipMalloc(ws, need));
         *ws_bytes = need;
     }
-    KEM_CK(c, kem::launch_bmm(c->stream, use32 ? A->A32 : nullptr, use32 ? nullptr : A->A64, A->n, d_X, A->n, nrhs,
-                              d_Y, A->n, *ws, *ws_bytes, false));
+    if (prec == kem::kBmmBf16)
+        KEM_CK(c, kem::launch_bmm16(c->stream, A->A16, A->n, d_X, A->n, nrhs, d_Y, A->n, *ws, *ws_bytes, false));
+    else
+        KEM_CK(c, kem::launch_bmm(c->stream, prec == kem::kBmmFp32 ? A->A32 : nullptr,
+                                  prec == kem::kBmmFp32 ? nullptr : A->A64, A->n, d_X, A->n, nrhs, d_Y, A->n, *ws,
+                                  *ws_bytes, false));
     return KEM_OK;
 }
 
@@ -1431,7 +1519,7 @@ struct DBuf {
 // one block product A [v_1 .. v_k] per iteration serves all of them (FP32 matrix: MFMA).  A
 // right-hand side leaves the lockstep when it converges (|p_{j+1}| <= tol_r ||b_r||) or breaks
 // down.  d_B, d_X: n x nrhs column-major; X holds x0 on entry.
-int gmres_batch(kem_matrix* A, bool use32, int nrhs, const double* d_B, double* d_X, const std::vector<double>& tol,
+int gmres_batch(kem_matrix* A, int prec, int nrhs, const double* d_B, double* d_X, const std::vector<double>& tol,
                 int maxit, int restart, const SolveCtl& ctl, int* iters, std::vector<double>& resid,
                 std::vector<int>& conv_out)
 {
@@ -1442,7 +1530,7 @@ int gmres_batch(kem_matrix* A, bool use32, int nrhs, const double* d_B, double* 
     if (restart > 0) kmax = std::min(kmax, restart);
     if (kmax < 1) kmax = 1;
     const int ldh = kmax + 2;
-    const int kp = panel_cols(nrhs, use32);
+    const int kp = panel_cols(nrhs, prec);
     const int64_t vstride = (int64_t)(kmax + 1) * n;
     DVec V, W, h1, h2, Y, H, rot, vec;
     DBuf panel, ws, ints;
@@ -1454,10 +1542,14 @@ int gmres_batch(kem_matrix* A, bool use32, int nrhs, const double* d_B, double* 
     KEM_CK(c, dalloc(&H.p, (int64_t)nrhs * ldh * ldh));
     KEM_CK(c, dalloc(&rot.p, 3 * (int64_t)ldh * nrhs));
     KEM_CK(c, dalloc(&vec.p, 4 * (int64_t)nrhs));
-    KEM_CK(c, hipMalloc(&panel.p, (use32 ? sizeof(float) : sizeof(double)) * (size_t)n * kp));
-    KEM_CK(c, hipMemsetAsync(panel.p, 0, (use32 ? sizeof(float) : sizeof(double)) * (size_t)n * kp, st));
+    // the panel's rows past n (BF16: the last group of 8) stay zero
+    const size_t panel_bytes = prec == kem::kBmmBf16 ? sizeof(uint16_t) * (size_t)((n + 7) / 8) * 8 * kp
+                                                     : (prec == kem::kBmmFp32 ? sizeof(float) : sizeof(double)) *
+                                                           (size_t)n * kp;
+    KEM_CK(c, hipMalloc(&panel.p, panel_bytes));
+    KEM_CK(c, hipMemsetAsync(panel.p, 0, panel_bytes, st));
     KEM_CK(c, hipMalloc(&ints.p, sizeof(int) * (4 * (size_t)nrhs + 4)));
-    const size_t wsb = kem::bmm_workspace_bytes(n, nrhs, use32);
+    const size_t wsb = kem::bmm_workspace_bytes(n, nrhs, prec);
     KEM_CK(c, hipMalloc(&ws.p, wsb));
     ws.bytes = wsb;
     kem::BatchArnoldi a;
@@ -1466,14 +1558,16 @@ int gmres_batch(kem_matrix* A, bool use32, int nrhs, const double* d_B, double* 
     a.kmax = kmax;
     a.ldh = ldh;
     a.kp = kp;
-    a.use32 = use32;
+    a.use32 = prec == kem::kBmmFp32;
     a.A32 = A->A32;
     a.A64 = A->A64;
     a.V = V.p;
     a.vstride = vstride;
     a.W = W.p;
-    a.P32 = use32 ? (float*)panel.p : nullptr;
-    a.P64 = use32 ? nullptr : (double*)panel.p;
+    a.A16 = prec == kem::kBmmBf16 ? A->A16 : nullptr;
+    a.P32 = prec == kem::kBmmFp32 ? (float*)panel.p : nullptr;
+    a.P64 = prec == kem::kBmmFp64 ? (double*)panel.p : nullptr;
+    a.P16 = prec == kem::kBmmBf16 ? (uint16_t*)panel.p : nullptr;
     a.ws = ws.p;
     a.ws_bytes = ws.bytes;
     a.h1 = h1.p;
@@ -1532,7 +1626,7 @@ int gmres_batch(kem_matrix* A, bool use32, int nrhs, const double* d_B, double* 
     int it = 0, status = KEM_OK;
     for (;;) {
         // W = B - A X; V_0 = W / ||W||
-        int rc = block_product(A, use32, nrhs, d_X, W.p, &ws.p, &ws.bytes);
+        int rc = block_product(A, prec, nrhs, d_X, W.p, &ws.p, &ws.bytes);
         if (rc != KEM_OK) return rc;
         KEM_CK(c, kem::launch_baxpby(st, 1.0, d_B, -1.0, W.p, W.p, n * nrhs));
         KEM_CK(c, kem::launch_bnorm(st, W.p, n, nrhs, h1.p, ldh));
@@ -1557,7 +1651,8 @@ int gmres_batch(kem_matrix* A, bool use32, int nrhs, const double* d_B, double* 
         KEM_CK(c, hipMemcpyAsync(a.p, hp.data(), sizeof(double) * ldh * nrhs, hipMemcpyHostToDevice, st));
         KEM_CK(c, hipMemcpyAsync(a.resid, resid.data(), sizeof(double) * nrhs, hipMemcpyHostToDevice, st));
         KEM_CK(c, hipMemsetAsync(a.jr, 0, sizeof(int) * nrhs, st));
-        KEM_CK(c, kem::launch_bnormalize(st, W.p, h1.p, ldh, a.active, n, nrhs, V.p, vstride, 0, kp, a.P32, a.P64));
+        KEM_CK(c, kem::launch_bnormalize(st, W.p, h1.p, ldh, a.active, n, nrhs, V.p, vstride, 0, kp, a.P32, a.P64,
+                                         a.P16));
         int got[4] = {0, 0, 0, 0};
         bool expired = false;
         for (;;) {
@@ -1598,7 +1693,7 @@ int block_residual(kem_matrix* A, int nrhs, const double* d_B, const double* d_X
     kem_ctx* c = A->ctx;
     hipStream_t st = c->stream;
     const int64_t n = A->n;
-    int rc = block_product(A, A->A64 == nullptr, nrhs, d_X, d_R, ws, wsb);
+    int rc = block_product(A, A->A64 ? kem::kBmmFp64 : kem::kBmmFp32, nrhs, d_X, d_R, ws, wsb);
     if (rc != KEM_OK) return rc;
     KEM_CK(c, kem::launch_baxpby(st, 1.0, d_B, -1.0, d_R, d_R, n * nrhs));
     KEM_CK(c, ensure_scal(c, 2 * (int64_t)nrhs));
@@ -1624,12 +1719,15 @@ int solve_group(kem_matrix* A, const kem_solve_opts& o, int nrhs, const double* 
     if (o.precision == KEM_FP64) {
         if (!A->A64) return fail(c, KEM_E_STATE, "FP64 solve needs FP64 matrix storage");
         std::vector<double> tol(nrhs, o.tolerance);
-        int status = gmres_batch(A, false, nrhs, d_B, d_X, tol, o.max_iterations, o.restart, ctl, &its, rel, conv);
+        int status =
+            gmres_batch(A, kem::kBmmFp64, nrhs, d_B, d_X, tol, o.max_iterations, o.restart, ctl, &its, rel, conv);
         if (status < 0) return status;
         s->iterations = std::max(s->iterations, its);
         return status;
     }
-    if (!A->A32) return fail(c, KEM_E_STATE, "mixed solve needs FP32 matrix storage");
+    const int inner = bmm_prec(o.precision);
+    if (inner == kem::kBmmFp32 && !A->A32) return fail(c, KEM_E_STATE, "mixed solve needs FP32 matrix storage");
+    if (inner == kem::kBmmBf16 && !A->A16) return fail(c, KEM_E_STATE, "BF16 solve needs BF16 matrix storage");
     if (!A->A64) return fail(c, KEM_E_STATE, "mixed solve needs FP64 matrix storage for the residual");
     DVec R, D;
     DBuf ws;
@@ -1645,7 +1743,9 @@ int solve_group(kem_matrix* A, const kem_solve_opts& o, int nrhs, const double* 
             worst = std::max(worst, rel[r]);
             // inner tolerance relative to the residual right-hand side; converged ones are skipped
             tol[r] = rel[r] <= o.tolerance ? 2.0
-                                            : std::min(0.5, std::max(o.inner_tolerance, 0.5 * o.tolerance / rel[r]));
+                                            : std::min(0.5, std::max(inner == kem::kBmmBf16 ? kBf16InnerTolerance
+                                                                                              : o.inner_tolerance,
+                                                                     0.5 * o.tolerance / rel[r]));
         }
         if (worst <= o.tolerance) break;
         if (total >= o.max_iterations || ctl.expired()) {
@@ -1653,8 +1753,8 @@ int solve_group(kem_matrix* A, const kem_solve_opts& o, int nrhs, const double* 
             break;
         }
         KEM_CK(c, hipMemsetAsync(D.p, 0, sizeof(double) * n * nrhs, st));
-        std::vector<double> inner;
-        rc = gmres_batch(A, true, nrhs, R.p, D.p, tol, o.max_iterations - total, 0, ctl, &its, inner, conv);
+        std::vector<double> inner_rel;
+        rc = gmres_batch(A, inner, nrhs, R.p, D.p, tol, o.max_iterations - total, 0, ctl, &its, inner_rel, conv);
         if (rc < 0) return rc;
         total += its;
         refinements++;
@@ -1673,10 +1773,12 @@ int kem_matrix_multiply_batch(kem_matrix* A, int precision, int64_t nrhs, const 
 {
     if (!A || nrhs < 0 || (nrhs > 0 && (!X || !Y))) return KEM_E_ARG;
     kem_ctx* c = A->ctx;
-    if (precision != KEM_FP64 && precision != KEM_MIXED) return fail(c, KEM_E_ARG, "unknown precision mode");
-    const bool use32 = precision == KEM_MIXED;
-    if (use32 && !A->A32) return fail(c, KEM_E_STATE, "FP32 product needs FP32 matrix storage");
-    if (!use32 && !A->A64) return fail(c, KEM_E_STATE, "FP64 product needs FP64 matrix storage");
+    if (precision != KEM_FP64 && precision != KEM_MIXED && precision != KEM_MIXED_BF16)
+        return fail(c, KEM_E_ARG, "unknown precision mode");
+    const int prec = bmm_prec(precision);
+    if (prec == kem::kBmmFp32 && !A->A32) return fail(c, KEM_E_STATE, "FP32 product needs FP32 matrix storage");
+    if (prec == kem::kBmmBf16 && !A->A16) return fail(c, KEM_E_STATE, "BF16 product needs BF16 matrix storage");
+    if (prec == kem::kBmmFp64 && !A->A64) return fail(c, KEM_E_STATE, "FP64 product needs FP64 matrix storage");
     if (nrhs == 0) return KEM_OK;
     KEM_CK(c, enter(c));
     const int64_t n = A->n;
@@ -1685,7 +1787,7 @@ int kem_matrix_multiply_batch(kem_matrix* A, int precision, int64_t nrhs, const 
     DBuf ws;
     KEM_CK(c, dalloc(&dX.p, n * g));
     KEM_CK(c, dalloc(&dY.p, n * g));
-    ws.bytes = kem::bmm_workspace_bytes(n, g, use32);
+    ws.bytes = kem::bmm_workspace_bytes(n, g, prec);
     KEM_CK(c, hipMalloc(&ws.p, ws.bytes));
     for (int64_t r0 = 0; r0 < nrhs; r0 += 64) {
         const int k = (int)std::min<int64_t>(64, nrhs - r0);
@@ -1693,7 +1795,7 @@ int kem_matrix_multiply_batch(kem_matrix* A, int precision, int64_t nrhs, const 
         hipEvent_t t0, t1;
         timing_pair(c, &t0, &t1);
         if (t0) KEM_CK(c, hipEventRecord(t0, c->stream));
-        int rc = block_product(A, use32, k, dX.p, dY.p, &ws.p, &ws.bytes);
+        int rc = block_product(A, prec, k, dX.p, dY.p, &ws.p, &ws.bytes);
         if (rc != KEM_OK) return rc;
         if (t1) KEM_CK(c, hipEventRecord(t1, c->stream));
         KEM_CK(c, hipMemcpyAsync(Y + r0 * n, dY.p, sizeof(double) * n * k, hipMemcpyDeviceToHost, c->stream));
@@ -1713,7 +1815,8 @@ int kem_solve_batch(kem_matrix* A, const kem_solve_opts* opts, int64_t nrhs, con
     if (o.method != KEM_SOLVER_GMRES) return fail(c, KEM_E_ARG, "batched solve supports GMRES only");
     if (o.max_iterations <= 0) return fail(c, KEM_E_ARG, "max_iterations must be positive");
     if (!(o.tolerance > 0.)) return fail(c, KEM_E_ARG, "tolerance must be positive");
-    if (o.precision != KEM_FP64 && o.precision != KEM_MIXED) return fail(c, KEM_E_ARG, "unknown precision mode");
+    if (o.precision != KEM_FP64 && o.precision != KEM_MIXED && o.precision != KEM_MIXED_BF16)
+        return fail(c, KEM_E_ARG, "unknown precision mode");
     kem_solve_stats s{};
     if (nrhs == 0) {
         if (stats) *stats = s;

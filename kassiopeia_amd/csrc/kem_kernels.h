@@ -96,6 +96,9 @@ hipError_t launch_pairs(hipStream_t st, int policy, const double* geom, const do
 hipError_t launch_assemble(hipStream_t st, int policy, bool field, const double* geom, const double* nodes,
                            const double* recs, const double* targets, int64_t n, double* A64, float* A32,
                            bool axisym = false, const int64_t* gstart = nullptr);
+// phi[i] += t[i], ex[i] += t[npts + i], ey[i] += t[2 npts + i], ez[i] += t[3 npts + i]
+hipError_t launch_accumulate4(hipStream_t st, const double* t, int64_t npts, double* phi, double* ex, double* ey,
+                              double* ez);
 // symmetry groups: out[s*nm + i] = sigma[s*n + gof[i]] (each member takes its group's density)
 hipError_t launch_expand_groups(hipStream_t st, const int32_t* gof, int64_t nm, int64_t n, int64_t nsets,
                                 const double* sigma, double* out);
@@ -106,6 +109,7 @@ struct LaunchTuning {
     int64_t gemv_blocks = 2048;
     int64_t bmm_blocks = 2560;
     int64_t near_blocks = 384;  // grid of the near pass beside the mixed far pass (0: one block per point block)
+    int64_t bmm_rows = 64;      // rows per wavefront tile of the FP32 block product (64, or 128 when n % 4 == 0)
 };
 void set_launch_tuning(const LaunchTuning& t);
 const LaunchTuning& launch_tuning();
@@ -168,8 +172,23 @@ hipError_t launch_robin_hood(hipStream_t st, const double* A, const double* b, d
                              int64_t steps, double* rnorm);
 
 // ---- multi-right-hand-side kernels (kem_batch.hip)
-// Workspace of launch_bmm for nrhs <= 64 right-hand sides (FP32 MFMA path or FP64 path).
-size_t bmm_workspace_bytes(int64_t n, int nrhs, bool fp32);
+// Block-product precisions: FP64 matrix (VALU), FP32 matrix (FP32 MFMA), BF16 matrix (BF16 MFMA),
+// all accumulating in FP32 (MFMA) or FP64 (VALU) and summing the split-K partials in FP64.
+constexpr int kBmmFp64 = 0, kBmmFp32 = 1, kBmmBf16 = 2;
+// Workspace of launch_bmm / launch_bmm16 for nrhs <= 64 right-hand sides.
+size_t bmm_workspace_bytes(int64_t n, int nrhs, int prec);
+// BF16 matrix storage of an n x n matrix: uint16_t elements (columns padded to a multiple of 8),
+// packed from the FP64 (or, when A64 is null, FP32) column-major matrix: A16[(g*n + i)*8 + m] =
+// A(i, 8g + m), each entry rounded to FP32 and then to BF16 to nearest even.
+size_t bf16_matrix_elements(int64_t n);
+hipError_t launch_pack_bf16(hipStream_t st, const double* A64, const float* A32, int64_t n, uint16_t* A16);
+// Y[c*ldy + i] = sum_j A16(i, j) X(j, c) for c < nrhs <= 64 on BF16 MFMA (X rounded to BF16,
+// FP32 accumulation, FP64 sum of the split-K partials).  x_is_panel: X is the packed BF16 panel
+// (uint16_t, (n + 7) / 8 groups of 8 rows x (nrhs <= 32 ? 32 : 64) columns); else X is FP64,
+// column-major with leading dimension ldx.
+hipError_t launch_bmm16(hipStream_t st, const uint16_t* A16, int64_t n, const void* X, int64_t ldx, int nrhs,
+                        double* Y, int64_t ldy, void* ws, size_t ws_bytes, bool x_is_panel,
+                        const int* skip = nullptr);
 // Y[c*ldy + i] = sum_j A(i, j) X(j, c) for c < nrhs <= 64: A32 != null selects the FP32 MFMA
 // product (X rounded to FP32), else the FP64 product with A64.  x_is_panel: X already is the
 // row-major panel (FP32: n x (nrhs <= 32 ? 32 : 64) floats; FP64: n x (nrhs <= 16 ? 16 : 64)).
@@ -184,7 +203,7 @@ hipError_t launch_bupdate(hipStream_t st, const double* V, int64_t vstride, cons
 hipError_t launch_bnorm(hipStream_t st, const double* W, int64_t n, int nrhs, double* nrm, int stride);
 hipError_t launch_bnormalize(hipStream_t st, const double* W, const double* nrm, int stride, const int* active,
                              int64_t n, int nrhs, double* V, int64_t vstride, int q, int kp, float* Vt32,
-                             double* Vt64);
+                             double* Vt64, uint16_t* Vt16 = nullptr);
 hipError_t launch_baxpby(hipStream_t st, double a, const double* X, double b, const double* Y, double* Z, int64_t len);
 
 // Lockstep GMRES over nrhs <= 64 right-hand sides on device state: after one step the
@@ -196,11 +215,13 @@ struct BatchArnoldi {
     bool use32 = false;
     const float* A32 = nullptr;
     const double* A64 = nullptr;
+    const uint16_t* A16 = nullptr;  // non-null: BF16 block products (panel P16)
     double* V = nullptr;  // nrhs bases of kmax + 1 columns, vstride apart
     int64_t vstride = 0;
     double* W = nullptr;  // n x nrhs work block
     float* P32 = nullptr;  // row-major panel of the current basis vectors (n x kp)
     double* P64 = nullptr;
+    uint16_t* P16 = nullptr;  // packed BF16 panel (groups of 8 rows, zero past n)
     void* ws = nullptr;  // block-product workspace
     size_t ws_bytes = 0;
     double *h1 = nullptr, *h2 = nullptr;  // projections, ldh x nrhs

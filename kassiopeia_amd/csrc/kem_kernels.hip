@@ -516,6 +516,20 @@ __global__ __launch_bounds__(kBlock) void k_expand_groups(const int32_t* __restr
     out[t] = sigma[s * n + gof[i]];
 }
 
+// Step-level sum over fields: out_c[i] += t[c*npts + i] (c = phi, Ex, Ey, Ez), one field at a time
+// in the order the fields were added (KSRootElectricField.cxx:104-121 adds per sample point)
+__global__ __launch_bounds__(kBlock) void k_accumulate4(const double* __restrict__ t, int64_t npts,
+                                                        double* __restrict__ phi, double* __restrict__ ex,
+                                                        double* __restrict__ ey, double* __restrict__ ez)
+{
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= npts) return;
+    phi[i] += t[i];
+    ex[i] += t[npts + i];
+    ey[i] += t[2 * npts + i];
+    ez[i] += t[3 * npts + i];
+}
+
 // ------------------------------------------------------------------ assembly
 // Lane = collocation target i (rows), wave-uniform source j (columns).  Column-major storage:
 // A(i, j) at A[j*n + i], so the 64 lanes of a wavefront write 64 consecutive entries.
@@ -1311,6 +1325,14 @@ hipError_t launch_assemble(hipStream_t st, int policy, bool field, const double*
     if (gstart)
         return assemble_kernels<true>(st, policy, field, geom, nodes, recs, targets, n, A64, A32, axisym, gstart);
     return assemble_kernels<false>(st, policy, field, geom, nodes, recs, targets, n, A64, A32, axisym, nullptr);
+}
+
+hipError_t launch_accumulate4(hipStream_t st, const double* t, int64_t npts, double* phi, double* ex, double* ey,
+                              double* ez)
+{
+    if (npts <= 0) return hipSuccess;
+    k_accumulate4<<<nblocks(npts), kBlock, 0, st>>>(t, npts, phi, ex, ey, ez);
+    return hipGetLastError();
 }
 
 hipError_t launch_expand_groups(hipStream_t st, const int32_t* gof, int64_t nm, int64_t n, int64_t nsets,

@@ -27,6 +27,10 @@ from .geometry import Mesh
 _P = ctypes.c_void_p
 
 
+# precision modes of solves and block products by name (fields take "fp64" and "mixed")
+PRECISIONS = {"fp64": _lib.KEM_FP64, "mixed": _lib.KEM_MIXED, "bf16": _lib.KEM_MIXED_BF16}
+
+
 class KEMSimpleException(RuntimeError):
     """Raised for unknown integrator / solver names (KEMSimpleException in the reference)."""
 
@@ -165,9 +169,10 @@ class KElectrostaticBoundaryIntegrator:
 class KBoundaryIntegralMatrix:
     """Dense, device-resident collocation matrix (a KSquareMatrix<double>)."""
 
-    def __init__(self, container: KSurfaceContainer, fp64: bool = True, fp32: bool = False):
+    def __init__(self, container: KSurfaceContainer, fp64: bool = True, fp32: bool = False, bf16: bool = False):
         self.ctx = container.ctx
-        storage = (_lib.KEM_STORE_FP64 if fp64 else 0) | (_lib.KEM_STORE_FP32 if fp32 else 0)
+        storage = (_lib.KEM_STORE_FP64 if fp64 else 0) | (_lib.KEM_STORE_FP32 if fp32 else 0) | \
+            (_lib.KEM_STORE_BF16 if bf16 else 0)
         h = _P()
         self.ctx.check(self.ctx.lib.kem_matrix_assemble(self.ctx.h, container.h, storage, ctypes.byref(h)))
         self.h = h
@@ -194,10 +199,11 @@ class KBoundaryIntegralMatrix:
         return y
 
     def MultiplyBatch(self, X, precision="fp64"):
-        """Y = A X for the columns of X (n x k); precision "mixed" uses the FP32 matrix (MFMA)."""
+        """Y = A X for the columns of X (n x k); precision "mixed" uses the FP32 matrix (FP32
+        MFMA), "bf16" the BF16 matrix (BF16 MFMA, X rounded to BF16)."""
         X = np.asfortranarray(np.asarray(X, dtype=np.float64).reshape(self.n, -1))
         Y = np.zeros_like(X, order="F")
-        prec = {"fp64": _lib.KEM_FP64, "mixed": _lib.KEM_MIXED}[precision]
+        prec = PRECISIONS[precision]
         self.ctx.check(self.ctx.lib.kem_matrix_multiply_batch(self.h, prec, X.shape[1], _ptr(X), _ptr(Y)))
         return Y
 
@@ -208,7 +214,7 @@ class KBoundaryIntegralMatrix:
         o = _lib.SolveOpts()
         self.ctx.lib.kem_solve_opts_default(ctypes.byref(o))
         o.method = _lib.KEM_SOLVER_GMRES
-        o.precision = {"fp64": _lib.KEM_FP64, "mixed": _lib.KEM_MIXED}[precision]
+        o.precision = PRECISIONS[precision]
         o.tolerance = tolerance
         o.max_iterations = int(min(max_iterations, (1 << 31) - 1))
         o.restart = restart
@@ -237,7 +243,7 @@ class KBoundaryIntegralMatrix:
         if method not in methods:
             raise KEMSimpleException(f"unknown solver '{method}'")
         o.method = methods[method]
-        o.precision = {"fp64": _lib.KEM_FP64, "mixed": _lib.KEM_MIXED}[precision]
+        o.precision = PRECISIONS[precision]
         o.tolerance = tolerance
         o.max_iterations = int(min(max_iterations, (1 << 31) - 1))
         o.restart = restart

@@ -61,11 +61,18 @@ def test_tune_products(setup):
     X = np.random.default_rng(3).normal(size=(A.n, 40))
     ref = A.MultiplyBatch(X, "mixed")
     r = T.tune_block_product(A, nrhs=40, repeats=1, candidates=(320, 2560))
-    assert T.get(ctx, "bmm_blocks") == r["best"]
-    for c in (64, 10240):
-        T.set_knob(ctx, "bmm_blocks", c)
-        assert np.linalg.norm(A.MultiplyBatch(X, "mixed") - ref) <= 1e-6 * np.linalg.norm(ref)
-    T.apply(ctx, {"gemv_blocks": 2048, "bmm_blocks": 2560})
+    assert T.get(ctx, "bmm_blocks") == r["best"] and T.get(ctx, "bmm_rows") == r["rows"]["best"]
+    assert set(r["rows"]["ms"]) == {64, 128}
+    # both wavefront tiles, every split target: the product of the FP32 matrix to FP32 accuracy
+    for rows in (64, 128):
+        T.set_knob(ctx, "bmm_rows", rows)
+        for nrhs in (7, 40):
+            R = A.MultiplyBatch(X[:, :nrhs], "mixed")
+            assert np.linalg.norm(R - ref[:, :nrhs]) <= 1e-6 * np.linalg.norm(ref[:, :nrhs]), (rows, nrhs)
+        for c in (64, 10240):
+            T.set_knob(ctx, "bmm_blocks", c)
+            assert np.linalg.norm(A.MultiplyBatch(X, "mixed") - ref) <= 1e-6 * np.linalg.norm(ref), (rows, c)
+    T.apply(ctx, {"gemv_blocks": 2048, "bmm_blocks": 2560, "bmm_rows": 64})
 
 
 def test_tuning_table_roundtrip_and_errors(setup, tmp_path):
@@ -74,11 +81,11 @@ def test_tuning_table_roundtrip_and_errors(setup, tmp_path):
 
     ctx = setup[0]
     table = {"far_range_fp64": 1280, "far_range_mixed": {"best": 5120, "ms": {}}, "gemv_blocks": 1024,
-             "bmm_blocks": 2560, "near_blocks": 256}
+             "bmm_blocks": 2560, "near_blocks": 256, "bmm_rows": 128}
     T.save(table, tmp_path / "t.json")
     loaded = T.load(tmp_path / "t.json")
     assert loaded == {"far_range_fp64": 1280, "far_range_mixed": 5120, "gemv_blocks": 1024, "bmm_blocks": 2560,
-                      "near_blocks": 256}
+                      "near_blocks": 256, "bmm_rows": 128}
     T.apply(ctx, loaded)
     assert T.current(ctx) == loaded
     with pytest.raises(_lib.KemError):
@@ -87,7 +94,10 @@ def test_tuning_table_roundtrip_and_errors(setup, tmp_path):
         ctx.check(ctx.lib.kem_ctx_set_tuning(ctx.h, 99, 1))
     with pytest.raises(_lib.KemError):
         T.set_knob(ctx, "near_blocks", -1)
-    T.apply(ctx, {"far_range_fp64": 2560, "far_range_mixed": 2560, "gemv_blocks": 2048, "near_blocks": 384})
+    with pytest.raises(_lib.KemError):
+        T.set_knob(ctx, "bmm_rows", 96)
+    T.apply(ctx, {"far_range_fp64": 2560, "far_range_mixed": 2560, "gemv_blocks": 2048, "near_blocks": 384,
+                  "bmm_rows": 64})
 
 
 def test_tune_near(setup):
