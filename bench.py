@@ -151,6 +151,24 @@ def _plan(ctx, n, npts, range_elems, precision):
     return {"ranges": out[3], "slices": out[4], "chunk": out[5], "xcd": out[6]}
 
 
+# FLOPs per pair of each path of the mixed cluster far pass (kem_field_census_paths order):
+# 3-point cluster pairs, 6-point record pairs, per-pair path by the pair's own rule (6, 7, 12,
+# 33 points; segments 4, 16 nodes); near pairs belong to the near pass
+FLOPS_PER_PATH = np.array([14 + 21 * 3, 140, 140, 161, 266, 707, 98, 350, 0, 0], dtype=np.float64)
+
+
+def _census_paths(ctx, cont, X, m):
+    """pairs per path of the mixed cluster far pass, in the order the pass visits the points;
+    returns (counts, algorithmic FLOPs = each pair at the rule its path evaluates (per-pair
+    pairs at their own rule), executed FLOPs (the per-pair path at its wavefront rule))"""
+    c = (ctypes.c_int64 * 10)()
+    ctx.check(ctx.lib.kem_field_census_paths(ctx.h, cont.h, m, X[0].ptr, X[1].ptr, X[2].ptr, c))
+    c = np.array(c[:], dtype=np.float64)
+    alg = float((c * FLOPS_PER_PATH).sum())
+    executed = float(c[0] * FLOPS_PER_PATH[0] + c[1] * FLOPS_PER_PATH[1] + 14.0 * c[2:8].sum() + 21.0 * c[9])
+    return c, alg, executed
+
+
 def _census(ctx, cont, X, m, precision):
     c = (ctypes.c_int64 * 12)()
     ctx.check(ctx.lib.kem_field_census(ctx.h, cont.h, m, X[0].ptr, X[1].ptr, X[2].ptr, precision, c))
@@ -322,8 +340,7 @@ def run_bore(ctx, args):
 
     # ------------------------------------------------------------ roofline of the mixed far kernel
     prec_mx = _lib.KEM_MIXED
-    cmx = _census(ctx, cont, X, M, prec_mx)
-    far_flops = float((cmx * FLOPS_PER_PAIR * FAR).sum())
+    cmx, far_flops, far_flops_executed = _census_paths(ctx, cont, X, M)
     far_ms = float(np.mean([s["far_ms"] / max(s["far_regions"], 1) for s in steps]))
     rng_used = T.get(ctx, "far_range_mixed")
     plan = _plan(ctx, N, M, rng_used, prec_mx)
@@ -442,7 +459,11 @@ def run_bore(ctx, args):
                      "traffic_source": TRAFFIC["profile"], "kernel": "k_field_far32c", "kernel_ms_per_pass": far_ms,
                      "launches_per_pass": launches, "kernel_ms_per_launch": far_ms / launches,
                      "flops_per_pass": far_flops, "frac_of_unpacked_fp32_peak": achieved / PEAK_FP64_TFLOPS,
-                     "pairs_by_regime": cmx.astype(int).tolist()},
+                     "flops_executed_per_pass": far_flops_executed,
+                     "pairs_by_path": cmx.astype(int).tolist(),
+                     "pairs_by_path_keys": ["3-point cluster", "6-point record", "per-pair 6", "per-pair 7",
+                                            "per-pair 12", "per-pair 33", "segment 4", "segment 16", "near",
+                                            "per-pair nodes executed"]},
         "cpu_baseline": cpu,
         "tuning": None if tuning is None else {k: v["best"] for k, v in tuning.items()},
     }
@@ -515,9 +536,8 @@ def run_coax(ctx, args):
     ms64, kms64 = timed(_lib.KEM_FP64, O, 2)
     msmx, kmsmx = timed(_lib.KEM_MIXED, Om, 5)
     c64 = _census(ctx, cont, X, M, _lib.KEM_FP64)
-    cmx = _census(ctx, cont, X, M, _lib.KEM_MIXED)
     f64 = float((c64 * FLOPS_PER_PAIR * FAR).sum())
-    fmx = float((cmx * FLOPS_PER_PAIR * FAR).sum())
+    _, fmx, _ = _census_paths(ctx, cont, X, M)
     phi = O[0].get()
     E = np.stack([O[k].get() for k in (1, 2, 3)], 1)
     err = _errors(E, phi, np.stack([Om[k].get() for k in (1, 2, 3)], 1), Om[0].get(), 18600.0)

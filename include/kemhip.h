@@ -230,6 +230,16 @@ int kem_field_eval_sets(kem_ctx* ctx, const kem_mesh* mesh, int64_t nsets, const
 int kem_field_census(kem_ctx* ctx, const kem_mesh* mesh, int64_t npts, const double* d_x, const double* d_y,
                      const double* d_z, int precision, int64_t* counts);
 
+/* Instrumentation (no reference counterpart): the paths of the mixed-precision cluster far
+   pass (single charge-density set, numeric policy) over device points, visited in the order a
+   kem_field_eval_device pass of the same points visits them.  counts[10] = (point, element)
+   pairs on the 3-point cluster path, on the 6-point record path, on the per-pair path by the
+   rule each pair's own ladder picks (6, 7, 12, 33 points; segments 4, 16 nodes), near pairs
+   left to the near pass, and the node evaluations the per-pair path runs (its wavefront rule
+   for every far lane).  Synchronous. */
+int kem_field_census_paths(kem_ctx* ctx, const kem_mesh* mesh, int64_t npts, const double* d_x, const double* d_y,
+                           const double* d_z, int64_t* counts);
+
 /* Instrumentation (no reference counterpart; host arithmetic only, no device access): how a
    field evaluation of npts points over nelem elements splits into element ranges and partial
    sums, for a mesh with (clustered = 1: triangles / rectangles / segments, numeric policy) or

@@ -87,6 +87,7 @@ SIGNATURES = {
     "kem_field_eval_fields": (_INT, [_P, _I64, _P, _P, _P, _I64, _P, _P, _P, _P, _P, _P, _P]),
     "kem_field_eval_sets": (_INT, [_P, _P, _I64, _P, _P, _I64, _P, _P, _P, _INT, _P, _P, _P, _P]),
     "kem_field_census": (_INT, [_P, _P, _I64, _P, _P, _P, _INT, _P]),
+    "kem_field_census_paths": (_INT, [_P, _P, _I64, _P, _P, _P, _P]),
     "kem_field_plan": (_INT, [_INT, _INT, _INT, _INT, _I64, _I64, _I64, _I64, _I64, _P]),
     "kem_matrix_assemble": (_INT, [_P, _P, _INT, ctypes.POINTER(_P)]),
     "kem_matrix_destroy": (None, [_P]),

@@ -30,9 +30,8 @@ constexpr int64_t kFarRangeDefault = 2560;
 // Workspace of one field evaluation: partial sums and per-call element weights (ws), the
 // charge densities of symmetry-group members (gsig) and the point ordering (sort).  A context
 // owns one that grows with its calls; a captured graph owns one sized at capture, since a later
-// call may reallocate the context's while the graph still holds its addresses; the
-// no-allocation entry point uses one carved from the device's static scratch (fixed: never
-// reallocated, a call that needs more is refused).
+// call may reallocate the context's while the graph still holds its addresses.  A fixed
+// workspace is never reallocated (a call that needs more is refused).
 struct FieldWs {
     double* ws = nullptr;
     int64_t ws_doubles = 0;
@@ -890,6 +889,38 @@ int kem_field_census(kem_ctx* c, const kem_mesh* m, int64_t npts, const double* 
     return KEM_OK;
 }
 
+int kem_field_census_paths(kem_ctx* c, const kem_mesh* m, int64_t npts, const double* x, const double* y,
+                           const double* z, int64_t* counts)
+{
+    if (!c || !m || !counts || (npts > 0 && (!x || !y || !z))) return KEM_E_ARG;
+    if (m->ctx != c) return fail(c, KEM_E_STATE, "mesh belongs to another context");
+    for (int k = 0; k < 10; k++) counts[k] = 0;
+    if (npts <= 0) return KEM_OK;
+    kem::FieldMesh fm;
+    field_mesh(m, fm);
+    if (c->policy != KEM_POLICY_NUMERIC || !fm.far6 || !fm.far6_cl || !fm.order || c->mixed_fp64_ratio != 0.)
+        return fail(c, KEM_E_STATE, "path census: the mesh's mixed passes do not take the cluster far pass");
+    KEM_CK(c, enter(c));
+    unsigned long long* d = nullptr;
+    void* sort = nullptr;
+    const bool sorted = npts >= kSortMinPoints && npts <= INT32_MAX;
+    const size_t sort_bytes = sorted ? kem::sort_workspace_bytes(npts) : 0;
+    hipError_t e = dalloc(&d, 10);
+    if (e == hipSuccess && sorted) e = hipMalloc(&sort, sort_bytes);
+    if (e == hipSuccess) e = hipMemsetAsync(d, 0, sizeof(unsigned long long) * 10, c->stream);
+    if (e == hipSuccess)
+        e = sorted ? kem::launch_census_paths_sorted(c->stream, fm, x, y, z, npts, sort, sort_bytes, d)
+                   : kem::launch_census_paths(c->stream, fm, x, y, z, npts, d);
+    unsigned long long h[10];
+    if (e == hipSuccess) e = hipMemcpyAsync(h, d, sizeof(h), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    dfree(d);
+    if (sort) (void)hipFree(sort);
+    if (e != hipSuccess) return fail(c, KEM_E_HIP, std::string("path census: ") + hipGetErrorString(e));
+    for (int k = 0; k < 10; k++) counts[k] = (int64_t)h[k];
+    return KEM_OK;
+}
+
 int kem_field_eval(kem_ctx* c, const kem_mesh* m, const double* sigma, int64_t npts, const double* P,
                    int precision, double* phi, double* E)
 {
@@ -954,7 +985,7 @@ int kem_matrix_assemble(kem_ctx* c, const kem_mesh* m, int storage, kem_matrix**
     if (e == hipSuccess && (storage & KEM_STORE_BF16)) e = dalloc(&A->A16, (int64_t)kem::bf16_matrix_elements(m->n));
     if (e == hipSuccess)
         e = kem::launch_assemble(c->stream, c->policy, m->has_neumann, m->geom, m->nodes, m->recs, m->targets, m->n, A->A64,
-                                 A->A32, m->has_axisym, m->gstart);
+                                 A->A32, m->has_axisym, m->gstart, m->clusters, m->order);
     if (e == hipSuccess && A->A16) e = kem::launch_pack_bf16(c->stream, A->A64, A->A32, m->n, A->A16);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e == hipSuccess && A->A32 && !(storage & KEM_STORE_FP32)) dfree(A->A32);

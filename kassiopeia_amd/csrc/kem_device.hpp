@@ -333,6 +333,17 @@ __device__ __forceinline__ double distance_ratio(const double* __restrict__ g, d
     return sqrt(__builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx))) * g[KG_INVSIZE];
 }
 
+// regime of a collocation pair exactly as element_eval picks it under the numeric policy:
+// 0 / 1 / 2 = 7 / 12 / 33-point cubature (segments: 4 / 16 nodes, no 2), 3 = closed form
+__device__ __forceinline__ int assemble_regime(const double* __restrict__ g, double px, double py, double pz)
+{
+    const int type = (int)g[KG_TYPE];
+    const double dr = distance_ratio(g, px, py, pz);
+    if (type == KEM_TRIANGLE) return dr > kTriCub12 ? 0 : dr > kTriCub33 ? 1 : dr > kTriRWG ? 2 : 3;
+    if (type == KEM_RECTANGLE) return dr > kRectCub12 ? 0 : dr > kRectCub33 ? 1 : dr > kRectRWG ? 2 : 3;
+    return dr > kLineQuad16 ? 0 : dr > kLineAna ? 1 : 3;
+}
+
 // One (element, point) pair under the "numeric" policy (default) or the closed form everywhere
 // ("rwg").  Returns the unit-charge (E, phi).
 template <bool FIELD>

@@ -89,13 +89,25 @@ hipError_t launch_check_sets(hipStream_t st, const int32_t* pset, int64_t n, int
 hipError_t launch_census(hipStream_t st, const double* geom, int64_t nelem, const double* px, const double* py,
                          const double* pz, int64_t npts, int mixed, double fp64_ratio, unsigned long long* counts);
 
+// path census of the mixed cluster far pass (single set) over points in visiting order
+// (10 counters, see k_census_far32c); launch_census_paths_sorted visits them in the Morton
+// order of a pass of npts >= 4096 points (sort_ws: sort_workspace_bytes(npts) bytes)
+hipError_t launch_census_paths(hipStream_t st, const FieldMesh& m, const double* px, const double* py,
+                               const double* pz, int64_t npts, unsigned long long* counts);
+hipError_t launch_census_paths_sorted(hipStream_t st, const FieldMesh& m, const double* px, const double* py,
+                                      const double* pz, int64_t npts, void* sort_ws, size_t sort_ws_bytes,
+                                      unsigned long long* counts);
+
 hipError_t launch_pairs(hipStream_t st, int policy, const double* geom, const double* nodes, const double* recs,
                         int64_t npairs, const int64_t* elem, const double* P, double* out, bool axisym = false,
                         const int64_t* gstart = nullptr);
 
+// collocation matrix; with element clusters (centre, radius per KEM_CLUSTER elements in the
+// Morton order `order`) the numeric policy splits it into cubature and closed-form kernels
 hipError_t launch_assemble(hipStream_t st, int policy, bool field, const double* geom, const double* nodes,
                            const double* recs, const double* targets, int64_t n, double* A64, float* A32,
-                           bool axisym = false, const int64_t* gstart = nullptr);
+                           bool axisym = false, const int64_t* gstart = nullptr, const double* clusters = nullptr,
+                           const int32_t* order = nullptr);
 // phi[i] += t[i], ex[i] += t[npts + i], ey[i] += t[2 npts + i], ez[i] += t[3 npts + i]
 hipError_t launch_accumulate4(hipStream_t st, const double* t, int64_t npts, double* phi, double* ex, double* ey,
                               double* ez);

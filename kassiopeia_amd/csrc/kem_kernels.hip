@@ -452,6 +452,97 @@ __global__ __launch_bounds__(kBlock) void k_field_far32c(const double* __restric
     }
 }
 
+// Path census of k_field_far32c over a single charge-density set: the kernel's control flow
+// (wavefront bounding sphere, cluster and element reach tests, near skip, wave rule) for points
+// in the order the pass visits them, counting per live lane instead of evaluating.
+// counts[0] 3-point cluster pairs, [1] 6-point record pairs, [2..5] pairs of the per-pair path
+// by the rule the lane's own ladder picks (6, 7, 12, 33 points), [6, 7] segment pairs with 4 /
+// 16 nodes, [8] near pairs the per-pair path skips, [9] node evaluations the per-pair path
+// runs (the wave rule's nodes for each far lane).  Element ranges are whole clusters, so one
+// range over all clusters takes the same decisions as any split of them.
+__global__ __launch_bounds__(kBlock) void k_census_far32c(const double* __restrict__ geom,
+                                                          const float* __restrict__ far6,
+                                                          const double* __restrict__ far6_cl,
+                                                          const int32_t* __restrict__ order, int64_t nelem,
+                                                          int64_t ncl, const double* __restrict__ px,
+                                                          const double* __restrict__ py,
+                                                          const double* __restrict__ pz, int64_t npts,
+                                                          unsigned long long* __restrict__ counts)
+{
+    __shared__ unsigned long long sc[10];
+    if (threadIdx.x < 10) sc[threadIdx.x] = 0;
+    __syncthreads();
+    int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const bool live = i < npts;
+    if (!live) i = npts - 1;
+    const double x = px[i], y = py[i], z = pz[i];
+    const double lx = wave_min(x), ly = wave_min(y), lz = wave_min(z);
+    const double hx = wave_max(x), hy = wave_max(y), hz = wave_max(z);
+    const double wx = uniform(0.5 * (lx + hx)), wy = uniform(0.5 * (ly + hy)), wz = uniform(0.5 * (lz + hz));
+    const double wr =
+        uniform(0.5 * sqrt((hx - lx) * (hx - lx) + (hy - ly) * (hy - ly) + (hz - lz) * (hz - lz)) * (1. + 1e-9) +
+                1e-12);
+    unsigned long long c[10] = {0};
+    for (int64_t cc = 0; cc < ncl; cc++) {
+        const double* cl = far6_cl + KEM_FAR_CL_STRIDE * cc;
+        const double qx = wx - cl[0], qy = wy - cl[1], qz = wz - cl[2];
+        const double q2 = qx * qx + qy * qy + qz * qz;
+        const double tc = cl[3] + wr, t3 = cl[4] + wr;
+        const bool all6 = q2 > tc * tc;
+        int64_t k1 = (cc + 1) * KEM_CLUSTER;
+        if (k1 > nelem) k1 = nelem;
+        const int64_t cnt = k1 - cc * KEM_CLUSTER;
+        if (q2 > t3 * t3) {
+            c[0] += cnt;
+            continue;
+        }
+        for (int64_t k = cc * KEM_CLUSTER; k < k1; k++) {
+            const v16f B = reinterpret_cast<const v16f*>(far6 + k * KEM_FAR6_STRIDE)[1];
+            bool six = all6;
+            if (!six) {
+                const double cx = f2d(B[8], B[9]), cy = f2d(B[10], B[11]), cz = f2d(B[12], B[13]);
+                const double ux = wx - cx, uy = wy - cy, uz = wz - cz;
+                const double te = f2d(B[14], B[15]) + wr;
+                six = ux * ux + uy * uy + uz * uz > te * te;
+            }
+            if (six) {
+                c[1]++;
+                continue;
+            }
+            const int64_t j = order[k];
+            const double* g = geom + j * KG_STRIDE;
+            const double r2 = dev::dist2(g, x, y, z);
+            const bool near = dev::is_near2(g, r2, 0);
+            const int type = (int)g[KG_TYPE];
+            // the wave rule (dev::element_far32_wave) over the far lanes; as in the pass, lanes
+            // past the last point stand in for it
+            const bool fl = !near;
+            int wn;
+            if (type == KEM_LINESEGMENT) {
+                wn = __builtin_amdgcn_ballot_w64(fl && !(r2 > g[KG_MIX7_2])) == 0 ? 4 : 16;
+            } else {
+                wn = __builtin_amdgcn_ballot_w64(fl && !(r2 > g[KG_MIX12_2])) != 0   ? 33
+                     : __builtin_amdgcn_ballot_w64(fl && !(r2 > g[KG_MIX7_2])) != 0 ? 12
+                     : __builtin_amdgcn_ballot_w64(fl && !(r2 > g[KG_MIX6_2])) != 0 ? 7
+                                                                                      : 6;
+            }
+            if (near) {
+                c[8]++;
+                continue;
+            }
+            c[9] += wn;
+            if (type == KEM_LINESEGMENT)
+                c[r2 > g[KG_MIX7_2] ? 6 : 7]++;
+            else
+                c[r2 > g[KG_MIX6_2] ? 2 : r2 > g[KG_MIX7_2] ? 3 : r2 > g[KG_MIX12_2] ? 4 : 5]++;
+        }
+    }
+    for (int k = 0; k < 10; k++)
+        if (live && c[k]) atomicAdd(&sc[k], c[k]);
+    __syncthreads();
+    if (threadIdx.x < 10 && sc[threadIdx.x]) atomicAdd(&counts[threadIdx.x], sc[threadIdx.x]);
+}
+
 // Kahan-sum the per-range partials (ws[c][s][i], c = Ex, Ey, Ez, phi) in range order.
 __global__ __launch_bounds__(kBlock) void k_field_combine(const double* __restrict__ ws, int nsplit, int64_t npts,
                                                           double* __restrict__ phi, double* __restrict__ ex,
@@ -585,6 +676,114 @@ __global__ __launch_bounds__(kBlock) void k_assemble(const double* __restrict__ 
         if (live) {
             if (A64) A64[j * n + i] = a;
             if (A32) A32[j * n + i] = (float)a;
+        }
+    }
+}
+
+// ------------------------------------------------------------------ assembly, split by regime
+// The numeric-policy assembly of a mesh without symmetry groups or axisymmetric shapes runs in
+// two kernels, as the field pass does: k_assemble_far evaluates the cubature / Gauss-Legendre
+// entries and skips the closed-form ones, k_assemble_near evaluates only the closed-form
+// entries for the targets whose wavefront can reach an element cluster.  Both classify a pair
+// with element_eval's own test (assemble_near), so every entry is written by exactly one of
+// them, with the value k_assemble gives it.
+
+// Far entries: lane = collocation target i, wave-uniform source column j, as k_assemble, but
+// the closed-form entries are left to k_assemble_near (no closed form inlined here: the kernel
+// keeps the far field kernels' register budget).  KElectrostaticBoundaryIntegrator::
+// ComputeBoundaryIntegral (Core/include/KElectrostaticBoundaryIntegrator.hh:260-280).
+template <bool FIELD>
+__global__ __launch_bounds__(kBlock) void k_assemble_far(const double* __restrict__ geom,
+                                                         const double* __restrict__ nodes,
+                                                         const double* __restrict__ targets, int64_t n,
+                                                         int64_t per_split, double* __restrict__ A64,
+                                                         float* __restrict__ A32)
+{
+    int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const bool live = i < n;
+    if (!live) i = n - 1;
+    const double* t = targets + i * KT_STRIDE;
+    const double cx = t[KT_CX], cy = t[KT_CY], cz = t[KT_CZ];
+    const double nx = t[KT_NX], ny = t[KT_NY], nz = t[KT_NZ];
+    const bool dirichlet = t[KT_BC] == 0.;
+    const int64_t j0 = (int64_t)blockIdx.y * per_split;
+    int64_t j1 = j0 + per_split;
+    if (j1 > n) j1 = n;
+    for (int64_t j = j0; j < j1; j++) {
+        const double* g = geom + j * KG_STRIDE;
+        const double* nd = nodes + j * KEM_NODE_STRIDE;
+        const int reg = dev::assemble_regime(g, cx, cy, cz);
+        if (reg == 3) continue;
+        const double pref = g[KG_PREF];
+        F4 v;
+        if ((int)g[KG_TYPE] == KEM_LINESEGMENT)
+            v = reg == 0 ? dev::line_quadrature<2, FIELD>(nd + 4 * KEM_L4, pref, cx, cy, cz)
+                         : dev::line_quadrature<8, FIELD>(nd + 4 * KEM_L16, pref, cx, cy, cz);
+        else
+            v = reg == 0   ? dev::cubature<7, FIELD>(nd + 4 * KEM_N7, pref, cx, cy, cz)
+                : reg == 1 ? dev::cubature<12, FIELD>(nd + 4 * KEM_N12, pref, cx, cy, cz)
+                           : dev::cubature<33, FIELD>(nd + 4 * KEM_N33, pref, cx, cy, cz);
+        // far entries lie beyond 2 element sizes: never the Neumann self term
+        const double a = (!FIELD || dirichlet) ? v.p : (v.x * nx + v.y * ny + v.z * nz);
+        if (live) {
+            if (A64) A64[j * n + i] = a;
+            if (A32) A32[j * n + i] = (float)a;
+        }
+    }
+}
+
+// Closed-form entries: targets in the Morton order of the element centres (order[k]), so a
+// wavefront's targets are compact; each wavefront bounds its targets by a sphere and skips the
+// element clusters that cannot hold a near pair of any of them (the culling of
+// k_field_near_culled); the entries it finds are written in place (a few per target).
+template <bool FIELD>
+__global__ __launch_bounds__(kBlock) void k_assemble_near(const double* __restrict__ geom,
+                                                          const double* __restrict__ targets,
+                                                          const double* __restrict__ clusters, int64_t ncl,
+                                                          const int32_t* __restrict__ order, int64_t n,
+                                                          double* __restrict__ A64, float* __restrict__ A32)
+{
+    int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const bool live = k < n;
+    if (!live) k = n - 1;
+    const int64_t i = order[k];
+    const double* t = targets + i * KT_STRIDE;
+    const double cx = t[KT_CX], cy = t[KT_CY], cz = t[KT_CZ];
+    const double nx = t[KT_NX], ny = t[KT_NY], nz = t[KT_NZ];
+    const bool dirichlet = t[KT_BC] == 0.;
+    const double f = t[KT_FLUX];
+    const double diag = ((1. + f) / (1. - f)) / (2. * dev::kEps0);
+    const double lx = wave_min(cx), ly = wave_min(cy), lz = wave_min(cz);
+    const double hx = wave_max(cx), hy = wave_max(cy), hz = wave_max(cz);
+    const double wx = 0.5 * (lx + hx), wy = 0.5 * (ly + hy), wz = 0.5 * (lz + hz);
+    const double wr = 0.5 * sqrt((hx - lx) * (hx - lx) + (hy - ly) * (hy - ly) + (hz - lz) * (hz - lz));
+    for (int64_t c = 0; c < ncl; c++) {
+        const double* cl = clusters + 4 * c;
+        const double dx = cl[0] - wx, dy = cl[1] - wy, dz = cl[2] - wz;
+        const double reach = (cl[3] + wr) * (1. + 1e-9) + 1e-12;
+        if (dx * dx + dy * dy + dz * dz > reach * reach) continue;
+        int64_t e1 = (c + 1) * KEM_CLUSTER;
+        if (e1 > n) e1 = n;
+        for (int64_t e = c * KEM_CLUSTER; e < e1; e++) {
+            const int64_t j = order[e];
+            const double* g = geom + j * KG_STRIDE;
+            if (dev::assemble_regime(g, cx, cy, cz) != 3) continue;
+            const int type = (int)g[KG_TYPE];
+            const F4 v = type == KEM_TRIANGLE    ? dev::closed_form<3, FIELD>(g, cx, cy, cz)
+                         : type == KEM_RECTANGLE ? dev::closed_form<4, FIELD>(g, cx, cy, cz)
+                                                 : dev::line_analytic(g, cx, cy, cz);
+            double a;
+            if (!FIELD || dirichlet) {
+                a = v.p;
+            } else {
+                const double* tj = targets + j * KT_STRIDE;
+                const double ex = tj[KT_CX] - cx, ey = tj[KT_CY] - cy, ez = tj[KT_CZ] - cz;
+                a = (sqrt(ex * ex + ey * ey + ez * ez) >= 1.e-12) ? (v.x * nx + v.y * ny + v.z * nz) : diag;
+            }
+            if (live) {
+                if (A64) A64[j * n + i] = a;
+                if (A32) A32[j * n + i] = (float)a;
+            }
         }
     }
 }
@@ -1272,6 +1471,17 @@ hipError_t launch_census(hipStream_t st, const double* geom, int64_t nelem, cons
     return hipGetLastError();
 }
 
+hipError_t launch_census_paths(hipStream_t st, const FieldMesh& m, const double* px, const double* py,
+                               const double* pz, int64_t npts, unsigned long long* counts)
+{
+    if (npts <= 0 || m.n <= 0) return hipSuccess;
+    if (!m.far6 || !m.far6_cl || !m.order) return hipErrorInvalidValue;
+    const int64_t ncl = (m.n + KEM_CLUSTER - 1) / KEM_CLUSTER;
+    k_census_far32c<<<nblocks(npts), kBlock, 0, st>>>(m.geom, m.far6, m.far6_cl, m.order, m.n, ncl, px, py, pz, npts,
+                                                      counts);
+    return hipGetLastError();
+}
+
 hipError_t launch_pairs(hipStream_t st, int policy, const double* geom, const double* nodes, const double* recs,
                         int64_t npairs, const int64_t* elem, const double* P, double* out, bool axisym,
                         const int64_t* gstart)
@@ -1290,7 +1500,7 @@ namespace {
 template <bool G>
 hipError_t assemble_kernels(hipStream_t st, int policy, bool field, const double* geom, const double* nodes,
                             const double* recs, const double* targets, int64_t n, double* A64, float* A32,
-                            bool axisym, const int64_t* gstart)
+                            bool axisym, const int64_t* gstart, const double* clusters, const int32_t* order)
 {
     if (n <= 0) return hipSuccess;
     const unsigned gx = nblocks(n);
@@ -1299,6 +1509,19 @@ hipError_t assemble_kernels(hipStream_t st, int policy, bool field, const double
     const int64_t per = (n + nsplit - 1) / nsplit;
     nsplit = (n + per - 1) / per;
     dim3 grid(gx, (unsigned)nsplit);
+    if (!G && !axisym && policy == 0 && clusters && order) {
+        // cubature entries and closed-form entries in separate kernels (measured on the
+        // 20,000-triangle coax: 12.8 ms in one kernel at 129 VGPRs)
+        const int64_t ncl = (n + KEM_CLUSTER - 1) / KEM_CLUSTER;
+        if (field) {
+            k_assemble_far<true><<<grid, kBlock, 0, st>>>(geom, nodes, targets, n, per, A64, A32);
+            k_assemble_near<true><<<gx, kBlock, 0, st>>>(geom, targets, clusters, ncl, order, n, A64, A32);
+        } else {
+            k_assemble_far<false><<<grid, kBlock, 0, st>>>(geom, nodes, targets, n, per, A64, A32);
+            k_assemble_near<false><<<gx, kBlock, 0, st>>>(geom, targets, clusters, ncl, order, n, A64, A32);
+        }
+        return hipGetLastError();
+    }
     if (axisym) {
         if (field)
             k_assemble<true, 2, G><<<grid, kBlock, 0, st>>>(geom, nodes, recs, targets, policy, n, per, A64, A32, gstart);
@@ -1320,11 +1543,13 @@ hipError_t assemble_kernels(hipStream_t st, int policy, bool field, const double
 
 hipError_t launch_assemble(hipStream_t st, int policy, bool field, const double* geom, const double* nodes,
                            const double* recs, const double* targets, int64_t n, double* A64, float* A32,
-                           bool axisym, const int64_t* gstart)
+                           bool axisym, const int64_t* gstart, const double* clusters, const int32_t* order)
 {
     if (gstart)
-        return assemble_kernels<true>(st, policy, field, geom, nodes, recs, targets, n, A64, A32, axisym, gstart);
-    return assemble_kernels<false>(st, policy, field, geom, nodes, recs, targets, n, A64, A32, axisym, nullptr);
+        return assemble_kernels<true>(st, policy, field, geom, nodes, recs, targets, n, A64, A32, axisym, gstart,
+                                      nullptr, nullptr);
+    return assemble_kernels<false>(st, policy, field, geom, nodes, recs, targets, n, A64, A32, axisym, nullptr,
+                                   clusters, order);
 }
 
 hipError_t launch_accumulate4(hipStream_t st, const double* t, int64_t npts, double* phi, double* ex, double* ey,

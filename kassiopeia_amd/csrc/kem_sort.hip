@@ -183,4 +183,40 @@ hipError_t launch_field_sorted(hipStream_t st, int policy, const FieldMesh& m, b
     return hipGetLastError();
 }
 
+hipError_t launch_census_paths_sorted(hipStream_t st, const FieldMesh& m, const double* px, const double* py,
+                                      const double* pz, int64_t npts, void* sort_ws, size_t sort_ws_bytes,
+                                      unsigned long long* counts)
+{
+    if (npts <= 0) return hipSuccess;
+    if (npts > INT32_MAX || sort_ws_bytes < sort_workspace_bytes(npts)) return hipErrorInvalidValue;
+    // the same carving and ordering as launch_field_sorted
+    char* p = (char*)sort_ws;
+    auto take = [&](size_t bytes) {
+        char* r = p;
+        p += align256(bytes);
+        return r;
+    };
+    uint64_t* k0 = (uint64_t*)take(8 * npts);
+    uint64_t* k1 = (uint64_t*)take(8 * npts);
+    int32_t* i0 = (int32_t*)take(4 * npts);
+    int32_t* i1 = (int32_t*)take(4 * npts);
+    int32_t* ss = (int32_t*)take(4 * npts);
+    double* sx = (double*)take(8 * npts);
+    double* sy = (double*)take(8 * npts);
+    double* sz = (double*)take(8 * npts);
+    for (int c = 0; c < 4; c++) (void)take(8 * npts);
+    double* box = (double*)take(8 * 6 * kBoxParts);
+    size_t tb = cub_temp_bytes(npts);
+    void* tmp = take(tb);
+
+    int nb = (int)((npts + kBlock - 1) / kBlock);
+    if (nb > kBoxParts) nb = kBoxParts;
+    k_bbox_part<<<nb, kBlock, 0, st>>>(px, py, pz, npts, box);
+    k_morton<<<nblocks(npts), kBlock, 0, st>>>(px, py, pz, npts, box, nb, k0, i0);
+    hipError_t e = hipcub::DeviceRadixSort::SortPairs(tmp, tb, k0, k1, i0, i1, (int)npts, 0, 63, st);
+    if (e != hipSuccess) return e;
+    k_gather3<<<nblocks(npts), kBlock, 0, st>>>(i1, px, py, pz, nullptr, npts, sx, sy, sz, ss);
+    return launch_census_paths(st, m, sx, sy, sz, npts, counts);
+}
+
 }  // namespace kem

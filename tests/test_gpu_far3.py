@@ -49,6 +49,29 @@ def test_far3_against_fp64(sphere, shell):
     # outside a uniformly charged sphere at potential 1: phi = 1/r, within the mesh's accuracy
     r = np.linalg.norm(pts, axis=1)
     assert np.abs(pm * r - 1.0).max() < 5e-3
+    # the paths the pass took: the far shell runs the 3-point branch, the near shell never
+    c = paths(cont, pts)
+    assert c.sum() - c[9] == len(pts) * m.n
+    if shell == (10.0, 40.0):
+        assert c[0] > 0, c
+    if shell == (1.5, 4.0):
+        assert c[0] == 0 and c[1] > 0, c
+
+
+def paths(cont, pts):
+    import ctypes
+
+    from kassiopeia_amd import kemfield as K
+
+    ctx = cont.ctx
+    X = [K.DeviceArray(ctx, np.ascontiguousarray(pts[:, k])) for k in range(3)]
+    out = (ctypes.c_int64 * 10)()
+    try:
+        ctx.check(ctx.lib.kem_field_census_paths(ctx.h, cont.h, len(pts), X[0].ptr, X[1].ptr, X[2].ptr, out))
+    finally:
+        for a in X:
+            a.close()
+    return np.array(out[:], dtype=np.int64)
 
 
 def test_far3_sets(ctx, sphere):

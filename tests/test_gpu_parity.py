@@ -29,6 +29,18 @@ def relerr(a, b):
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
 
 
+def assert_field(E, Er, tol=TOL):
+    """E against the oracle's Er: every point's |E - Er| within tol of the largest |Er| of the
+    point set (points inside a conductor, where |E| is a cancellation residue, are held to
+    the same absolute bar as the rest), and within 10 tol of its own |Er| where that is at
+    least 1e-3 of the largest."""
+    d = np.linalg.norm(np.asarray(E) - np.asarray(Er), axis=1)
+    en = np.linalg.norm(Er, axis=1)
+    assert d.max() <= tol * en.max(), (d.max(), en.max())
+    big = en >= 1e-3 * en.max()
+    assert (d[big] / en[big]).max() < 10 * tol
+
+
 def random_elements(rng, n, shape):
     recs = []
     for _ in range(n):
@@ -251,7 +263,7 @@ def test_field_matches_oracle(ctx, oracle, cube6):
     E, phi = K.KIntegratingElectrostaticFieldSolver(cont, sigma).ElectricFieldAndPotential(pts)
     phr, Er = oracle.field(m.shapes, m.recs, sigma, pts)
     assert (np.abs(phi - phr) / np.abs(phr)).max() < TOL
-    assert max(relerr(E[k], Er[k]) for k in range(len(pts))) < 1e-11
+    assert_field(E, Er)
     # split over element ranges (few points) and one range (many points) agree
     E2, phi2 = K.KIntegratingElectrostaticFieldSolver(cont, sigma).ElectricFieldAndPotential(pts[:3])
     assert (np.abs(phi2 - phi[:3]) / np.abs(phi[:3])).max() < 1e-14
@@ -282,7 +294,7 @@ def test_policy_assembly_and_field(ctx, oracle, policy):
         E, phi = K.KIntegratingElectrostaticFieldSolver(cont, sigma).ElectricFieldAndPotential(pts)
         phr, Er = oracle.field(m.shapes, m.recs, sigma, pts, policy=pol)
         assert (np.abs(phi - phr) / np.abs(phr)).max() < TOL
-        assert max(relerr(E[k], Er[k]) for k in range(len(pts))) < 1e-11
+        assert_field(E, Er)
     finally:
         ctx.set_integrator("numeric")
 
@@ -307,7 +319,7 @@ def test_field_sorted_path(ctx, oracle, cube6):
     sel = np.arange(0, 9000, 45)
     phr, Er = oracle.field(m.shapes, m.recs, sigma, pts[sel])
     assert (np.abs(phi[sel] - phr) / np.abs(phr)).max() < TOL
-    assert max(relerr(E[k], Er[i]) for i, k in enumerate(sel)) < 1e-11
+    assert_field(E[sel], Er)
 
 
 @pytest.mark.parametrize("policy", ["numeric", "rwg"])
@@ -331,7 +343,7 @@ def test_field_mixed_shapes(ctx, oracle, policy):
     phr, Er = oracle.field(m.shapes, m.recs, sigma, pts, policy=0 if policy == "numeric" else 1)
     tol = TOL if policy == "numeric" else 1e-10
     assert (np.abs(phi - phr) / np.abs(phr)).max() < tol
-    assert max(relerr(E[k], Er[k]) for k in range(len(pts))) < 10 * tol
+    assert_field(E, Er, tol)
 
 
 @pytest.fixture(scope="module")
