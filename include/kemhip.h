@@ -194,6 +194,20 @@ int kem_field_eval_device(kem_ctx* ctx, const kem_mesh* mesh, const double* d_si
    (device arrays: sigma n x nsets column-major, set[npts] in [0, nsets), nsets <= 64).  No
    reference counterpart (one KIntegratingFieldSolver per solution there).  Validates the set
    indices (waits for the stream), then runs asynchronously like kem_field_eval_device. */
+/* No-allocation variant of kem_field_eval_device, with the same arguments and results (the
+   same kernels, element ranges and sums, so bit-equal outputs).  No counterpart in the
+   reference: it is for callers that cannot allocate device memory, such as code under HIP
+   graph capture or a tracking loop that forbids hipMalloc.  It calls no hipMalloc / hipFree:
+   its partial sums, element weights and point ordering live in a static 48 MiB device scratch
+   array, one per GPU, held until the process ends.  At most 65,536 points per call, and the
+   point ordering, group charge densities and the workspace the eager path would allocate
+   must fit the 48 MiB (about 1.5 M elements at 65,536 points); larger calls are refused with
+   KEM_E_ARG before any device work.  All calls on one GPU share its scratch: calls that
+   overlap on one GPU (several contexts or streams of one device) race, and must be
+   serialised by the caller.  Asynchronous on the context's stream. */
+int kem_field_eval_device_noalloc(kem_ctx* ctx, const kem_mesh* mesh, const double* d_sigma, int64_t npts,
+                                  const double* d_x, const double* d_y, const double* d_z, int precision,
+                                  double* d_phi, double* d_ex, double* d_ey, double* d_ez);
 /* A field evaluation on fixed device buffers captured once as a HIP graph and replayed with
    kem_graph_launch (asynchronous on the context's stream): for repeated evaluations at points
    that change in place, e.g. the particles of a tracking step.  The buffers, sigma and the

@@ -81,6 +81,7 @@ SIGNATURES = {
     "kem_element_eval": (_INT, [_P, _P, _I64, _P, _P, _P]),
     "kem_field_eval": (_INT, [_P, _P, _P, _I64, _P, _INT, _P, _P]),
     "kem_field_eval_device": (_INT, [_P, _P, _P, _I64, _P, _P, _P, _INT, _P, _P, _P, _P]),
+    "kem_field_eval_device_noalloc": (_INT, [_P, _P, _P, _I64, _P, _P, _P, _INT, _P, _P, _P, _P]),
     "kem_field_graph_create": (_INT, [_P, _P, _P, _I64, _P, _P, _P, _INT, _P, _P, _P, _P, ctypes.POINTER(_P)]),
     "kem_graph_launch": (_INT, [_P]),
     "kem_graph_destroy": (None, [_P]),

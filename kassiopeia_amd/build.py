@@ -27,7 +27,7 @@ SOURCES = [
     ("kem_sort.hip", True),
     ("kem_batch.hip", True),
     ("kem_adjoint.hip", True),
-    ("kem_capi.cpp", False),
+    ("kem_capi.cpp", True),  # holds the no-allocation scratch (a __device__ array)
     ("kem_prep.cpp", False),
     ("kem_cache.cpp", False),
 ]

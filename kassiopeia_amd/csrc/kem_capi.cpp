@@ -10,6 +10,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -27,11 +28,29 @@ constexpr double kBf16InnerTolerance = 1.0 / 256.0;
 // elements per XCD range of the large single-set far passes (kem::launch_field); 0 = no split
 constexpr int64_t kFarRangeDefault = 2560;
 
+// Device scratch of the no-allocation field entry point (kem_field_eval_device_noalloc): one
+// 48 MiB array per GPU, held until the process ends.  Host code never passes the array's name
+// to a kernel (on the host it is a stand-in, not a device address): the address of each
+// device's copy is looked up once, at the first context created on that device, and kept in
+// g_scratch.  Calls on one GPU share it, so calls that overlap on one GPU race.
+constexpr size_t kScratchBytes = (size_t)48 << 20;
+constexpr int64_t kNoAllocMaxPoints = 65536;
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) char scratch[kScratchBytes];
+struct ScratchSlot {
+    bool looked_up = false;
+    hipError_t status = hipSuccess;
+    void* ptr = nullptr;
+};
+std::mutex g_scratch_mu;
+ScratchSlot g_scratch[kMaxDevices];
+
 // Workspace of one field evaluation: partial sums and per-call element weights (ws), the
 // charge densities of symmetry-group members (gsig) and the point ordering (sort).  A context
 // owns one that grows with its calls; a captured graph owns one sized at capture, since a later
-// call may reallocate the context's while the graph still holds its addresses.  A fixed
-// workspace is never reallocated (a call that needs more is refused).
+// call may reallocate the context's while the graph still holds its addresses; the
+// no-allocation entry point uses one carved from the device's static scratch (fixed: never
+// reallocated, a call that needs more is refused).
 struct FieldWs {
     double* ws = nullptr;
     int64_t ws_doubles = 0;
@@ -273,6 +292,18 @@ int kem_ctx_create(int device, kem_ctx** out)
         return KEM_E_HIP;
     }
     c->stream = c->own;
+    if (device < kMaxDevices) {
+        // the device's copy of the no-allocation scratch, once per device, with it current
+        std::lock_guard<std::mutex> lock(g_scratch_mu);
+        ScratchSlot& s = g_scratch[device];
+        if (!s.looked_up) {
+            void* p = nullptr;
+            s.status = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+            s.ptr = s.status == hipSuccess ? p : nullptr;
+            s.looked_up = true;
+            (void)hipGetLastError();
+        }
+    }
     *out = c;
     return KEM_OK;
 }
@@ -702,6 +733,59 @@ int kem_field_eval_device(kem_ctx* c, const kem_mesh* m, const double* sigma, in
     if (npts <= 0) return KEM_OK;
     KEM_CK(c, enter(c));
     return field_eval(c, m, sigma, nullptr, 1, npts, x, y, z, precision, phi, ex, ey, ez, c->fws, true);
+}
+
+int kem_field_eval_device_noalloc(kem_ctx* c, const kem_mesh* m, const double* sigma, int64_t npts,
+                                  const double* x, const double* y, const double* z, int precision, double* phi,
+                                  double* ex, double* ey, double* ez)
+{
+    if (!c || !m || !sigma || (npts > 0 && (!x || !y || !z || !phi || !ex || !ey || !ez))) return KEM_E_ARG;
+    if (m->ctx != c) return fail(c, KEM_E_STATE, "mesh belongs to another context");
+    if (precision != KEM_FP64 && precision != KEM_MIXED) return fail(c, KEM_E_ARG, "field: unknown precision mode");
+    if (npts > kNoAllocMaxPoints)
+        return fail(c, KEM_E_ARG, "no-allocation field: at most " + std::to_string(kNoAllocMaxPoints) +
+                                      " points per call (" + std::to_string(npts) + " given)");
+    if (npts <= 0) return KEM_OK;
+    void* base = nullptr;
+    {
+        std::lock_guard<std::mutex> lock(g_scratch_mu);
+        if (c->device < kMaxDevices && g_scratch[c->device].looked_up) {
+            if (g_scratch[c->device].status != hipSuccess)
+                return fail(c, KEM_E_HIP, std::string("no-allocation field: scratch address: ") +
+                                              hipGetErrorString(g_scratch[c->device].status));
+            base = g_scratch[c->device].ptr;
+        }
+    }
+    if (!base) return fail(c, KEM_E_STATE, "no-allocation field: no scratch on this device");
+    KEM_CK(c, enter(c));
+    // the scratch holds the point ordering, the group members' charge densities and the
+    // partial sums + element weights, each 256-byte aligned; the partial sums get what is left,
+    // which must hold the workspace the eager path would allocate (the same plan, the same sums)
+    kem::FieldMesh fm;
+    field_mesh(m, fm);
+    const bool mixed = precision == KEM_MIXED;
+    const int64_t range = mixed ? c->far_range_mixed : c->far_range_fp64;
+    const WsNeed need = field_ws_need(c, m, fm, false, 1, npts, mixed, range);
+    auto up = [](size_t b) { return (b + 255) / 256 * 256; };
+    const size_t sort_b = up(need.sort), gsig_b = up(sizeof(double) * (size_t)need.gsig);
+    const size_t ws_b = sizeof(double) * (size_t)need.ws;
+    if (sort_b + gsig_b + ws_b > kScratchBytes)
+        return fail(c, KEM_E_ARG, "no-allocation field: " + std::to_string(npts) + " points over " +
+                                      std::to_string(m->nm) + " elements need " +
+                                      std::to_string((sort_b + gsig_b + ws_b) >> 20) + " MiB of scratch; it holds " +
+                                      std::to_string(kScratchBytes >> 20) + " MiB");
+    char* p = (char*)base;
+    FieldWs w;
+    w.fixed = true;
+    w.sort = need.sort ? (void*)p : nullptr;
+    w.sort_bytes = need.sort;
+    p += sort_b;
+    w.gsig = need.gsig ? (double*)p : nullptr;
+    w.gsig_doubles = need.gsig;
+    p += gsig_b;
+    w.ws = (double*)p;
+    w.ws_doubles = (int64_t)((kScratchBytes - sort_b - gsig_b) / sizeof(double));
+    return field_eval(c, m, sigma, nullptr, 1, npts, x, y, z, precision, phi, ex, ey, ez, w, true);
 }
 
 int kem_field_graph_create(kem_ctx* c, const kem_mesh* m, const double* sigma, int64_t npts, const double* x,
