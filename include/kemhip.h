@@ -72,6 +72,7 @@ typedef struct kem_ctx kem_ctx;
 typedef struct kem_graph kem_graph;
 typedef struct kem_mesh kem_mesh;
 typedef struct kem_matrix kem_matrix;
+typedef struct kem_zh kem_zh;
 
 typedef struct kem_solve_opts {
     int method;             /* KEM_SOLVER_* */
@@ -127,9 +128,10 @@ int kem_ctx_set_mixed_fp64_ratio(kem_ctx* ctx, double ratio);
 int kem_ctx_set_tuning(kem_ctx* ctx, int param, int64_t value);
 int kem_ctx_get_tuning(const kem_ctx* ctx, int param, int64_t* value);
 /* Instrumentation, no reference counterpart: with timing on, each field evaluation records HIP
-   events on the context's stream around its far-field pass, each kem_matrix_multiply around
-   its matrix-vector kernels and each 64-column block of kem_matrix_multiply_batch around its
-   block product (up to 256 timed regions between queries); kem_ctx_kernel_time waits for them
+   events on the context's stream around its far-field pass, each kem_matrix_assemble around
+   its assembly kernels, each kem_matrix_multiply around its matrix-vector kernels and each
+   64-column block of kem_matrix_multiply_batch around its block product (up to 256 timed
+   regions between queries); kem_ctx_kernel_time waits for them
    and returns the summed time and the number of regions timed, then starts a new count. */
 int kem_ctx_set_kernel_timing(kem_ctx* ctx, int on);
 int kem_ctx_kernel_time(kem_ctx* ctx, double* total_ms, int* launches);
@@ -359,6 +361,46 @@ int kem_cache_find(const char* directory, int64_t n, const int32_t* shape_type, 
    the BF16 matrix on BF16 MFMA (X rounded to BF16, FP32 accumulation, FP64 sum of the split-K
    partials). */
 int kem_matrix_multiply_batch(kem_matrix* A, int precision, int64_t nrhs, const double* X, double* Y);
+
+/* ---- zonal harmonics: the field of axisymmetric elements (conic sections, rings) about the z
+   axis as Legendre expansions about source points on the axis.
+   KZonalHarmonicContainer<KElectrostaticBasis> + ComputeCoefficients (FieldSolvers/ZonalHarmonic/
+   Generator/include/KZonalHarmonicContainer.hh:246-345, KZonalHarmonicCoefficientGenerator.hh,
+   KZHCoefficientGeneratorConicSection.cc, KZHCoefficientGeneratorRing.cc) -> kem_zh_create;
+   KZonalHarmonicFieldSolver<KElectrostaticBasis>::ElectricFieldAndPotential (Solver/src/
+   KElectrostaticZonalHarmonicFieldSolver.cc:98-124) -> kem_zh_field_eval(_device).  The
+   parameters are KZonalHarmonicParameters' (Generator/include/KZonalHarmonicParameters.hh). */
+typedef struct kem_zh_params {
+    int32_t n_bifurcations;              /* 5 */
+    double convergence_ratio;            /* 0.99 */
+    double proximity_to_source_point;    /* 1e-14 */
+    double convergence_parameter;        /* 1e-15 */
+    int32_t n_central_coefficients;      /* 500 */
+    int32_t central_fractional_spacing;  /* 1 */
+    double central_fractional_distance;  /* 0.2 */
+    double central_delta_z;              /* 0.01 */
+    double central_z1, central_z2;       /* 0, 0: the elements' axial extent */
+    int32_t n_remote_coefficients;       /* 500 */
+    int32_t n_remote_source_points;      /* 3 */
+    double remote_z1, remote_z2;         /* 0, 0 */
+} kem_zh_params;
+void kem_zh_params_default(kem_zh_params* params);
+/* The expansion of a mesh (without symmetry groups) with charge densities sigma (host, mesh
+   size; rings: total charge).  Elements that are not axisymmetric are integrated directly
+   wherever the expansion is evaluated.  params may be NULL (the defaults).  The mesh must
+   outlive the expansion. */
+int kem_zh_create(kem_ctx* ctx, const kem_mesh* mesh, const double* sigma, const kem_zh_params* params,
+                  kem_zh** out);
+void kem_zh_destroy(kem_zh* zh);
+/* out[4]: tree nodes, central source points, remote source points, tree depth */
+int kem_zh_info(const kem_zh* zh, int64_t* out);
+/* (E, phi) at device points (SoA; asynchronous unless counts is given).  counts (host, 3,
+   nullable): central expansions, remote expansions and direct integrations the call took
+   (KZonalHarmonicComputer's execution counts); with counts the call waits for the stream. */
+int kem_zh_field_eval_device(kem_zh* zh, int64_t npts, const double* d_x, const double* d_y, const double* d_z,
+                             double* d_phi, double* d_ex, double* d_ey, double* d_ez, int64_t* counts);
+/* Same on host arrays (P npts x 3; phi, E nullable), synchronous. */
+int kem_zh_field_eval(kem_zh* zh, int64_t npts, const double* P, double* phi, double* E, int64_t* counts);
 
 #ifdef __cplusplus
 }

@@ -49,6 +49,26 @@ class SolveStats(ctypes.Structure):
     ]
 
 
+class ZHParams(ctypes.Structure):
+    """kem_zh_params (KZonalHarmonicParameters)"""
+    _fields_ = [
+        ("n_bifurcations", ctypes.c_int32),
+        ("convergence_ratio", ctypes.c_double),
+        ("proximity_to_source_point", ctypes.c_double),
+        ("convergence_parameter", ctypes.c_double),
+        ("n_central_coefficients", ctypes.c_int32),
+        ("central_fractional_spacing", ctypes.c_int32),
+        ("central_fractional_distance", ctypes.c_double),
+        ("central_delta_z", ctypes.c_double),
+        ("central_z1", ctypes.c_double),
+        ("central_z2", ctypes.c_double),
+        ("n_remote_coefficients", ctypes.c_int32),
+        ("n_remote_source_points", ctypes.c_int32),
+        ("remote_z1", ctypes.c_double),
+        ("remote_z2", ctypes.c_double),
+    ]
+
+
 # name -> (restype, argtypes); the exported symbol set of include/kemhip.h
 _P = ctypes.c_void_p
 _I64 = ctypes.c_int64
@@ -89,6 +109,12 @@ SIGNATURES = {
     "kem_field_eval_sets": (_INT, [_P, _P, _I64, _P, _P, _I64, _P, _P, _P, _INT, _P, _P, _P, _P]),
     "kem_field_census": (_INT, [_P, _P, _I64, _P, _P, _P, _INT, _P]),
     "kem_field_census_paths": (_INT, [_P, _P, _I64, _P, _P, _P, _P]),
+    "kem_zh_params_default": (None, [ctypes.POINTER(ZHParams)]),
+    "kem_zh_create": (_INT, [_P, _P, _P, ctypes.POINTER(ZHParams), ctypes.POINTER(_P)]),
+    "kem_zh_destroy": (None, [_P]),
+    "kem_zh_info": (_INT, [_P, _P]),
+    "kem_zh_field_eval_device": (_INT, [_P, _I64, _P, _P, _P, _P, _P, _P, _P, _P]),
+    "kem_zh_field_eval": (_INT, [_P, _I64, _P, _P, _P, _P]),
     "kem_field_plan": (_INT, [_INT, _INT, _INT, _INT, _I64, _I64, _I64, _I64, _I64, _P]),
     "kem_matrix_assemble": (_INT, [_P, _P, _INT, ctypes.POINTER(_P)]),
     "kem_matrix_destroy": (None, [_P]),

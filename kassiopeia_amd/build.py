@@ -27,9 +27,11 @@ SOURCES = [
     ("kem_sort.hip", True),
     ("kem_batch.hip", True),
     ("kem_adjoint.hip", True),
+    ("kem_zh.hip", True),
     ("kem_capi.cpp", True),  # holds the no-allocation scratch (a __device__ array)
     ("kem_prep.cpp", False),
     ("kem_cache.cpp", False),
+    ("kem_zh_gen.cpp", False),
 ]
 
 

@@ -18,6 +18,7 @@
 #include "kem_kernels.h"
 #include "kem_layout.h"
 #include "kem_prep.h"
+#include "kem_zh.h"
 
 // mixed precision: far pairs up to this distance ratio keep FP64 cubature; 0 = every far pair
 // in FP32 (the FP32 rule ladder alone keeps the global field error near 3e-8)
@@ -114,6 +115,19 @@ struct kem_graph {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     FieldWs fws;  // the buffers the captured kernels address, owned by the graph
+};
+
+// zonal-harmonic expansion of a mesh's axisymmetric elements: the tree and coefficients on
+// the device, the charge densities of the direct fallback, and the counts of the last call
+struct kem_zh {
+    kem_ctx* ctx = nullptr;
+    const kem_mesh* mesh = nullptr;
+    kem::ZHDevice dev;
+    int32_t *node = nullptr, *child = nullptr;
+    int64_t* elem = nullptr;
+    double *sp = nullptr, *coeff = nullptr, *leg = nullptr, *sigma = nullptr;
+    unsigned long long* counts = nullptr;
+    int64_t nnodes = 0, ncentral = 0, nremote = 0;
 };
 
 struct kem_matrix {
@@ -1067,9 +1081,13 @@ int kem_matrix_assemble(kem_ctx* c, const kem_mesh* m, int storage, kem_matrix**
     const bool need32 = (storage & KEM_STORE_FP32) || ((storage & KEM_STORE_BF16) && !(storage & KEM_STORE_FP64));
     if (e == hipSuccess && need32) e = dalloc(&A->A32, nn);
     if (e == hipSuccess && (storage & KEM_STORE_BF16)) e = dalloc(&A->A16, (int64_t)kem::bf16_matrix_elements(m->n));
+    hipEvent_t t0, t1;
+    timing_pair(c, &t0, &t1);
+    if (e == hipSuccess && t0) e = hipEventRecord(t0, c->stream);
     if (e == hipSuccess)
         e = kem::launch_assemble(c->stream, c->policy, m->has_neumann, m->geom, m->nodes, m->recs, m->targets, m->n, A->A64,
                                  A->A32, m->has_axisym, m->gstart, m->clusters, m->order);
+    if (e == hipSuccess && t1) e = hipEventRecord(t1, c->stream);
     if (e == hipSuccess && A->A16) e = kem::launch_pack_bf16(c->stream, A->A64, A->A32, m->n, A->A16);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e == hipSuccess && A->A32 && !(storage & KEM_STORE_FP32)) dfree(A->A32);
@@ -1978,6 +1996,214 @@ int kem_solve_batch(kem_matrix* A, const kem_solve_opts* opts, int64_t nrhs, con
     s.seconds = now_s() - ctl.t0;
     if (stats) *stats = s;
     return status;
+}
+
+// ------------------------------------------------------------------ zonal harmonics
+void kem_zh_params_default(kem_zh_params* p)
+{
+    if (!p) return;
+    const kem::ZHParams d;
+    p->n_bifurcations = d.nbif;
+    p->convergence_ratio = d.conv_ratio;
+    p->proximity_to_source_point = d.prox;
+    p->convergence_parameter = d.conv_param;
+    p->n_central_coefficients = d.ncen;
+    p->central_fractional_spacing = d.frac_spacing ? 1 : 0;
+    p->central_fractional_distance = d.frac_dist;
+    p->central_delta_z = d.cen_dz;
+    p->central_z1 = d.cen_z1;
+    p->central_z2 = d.cen_z2;
+    p->n_remote_coefficients = d.nrem;
+    p->n_remote_source_points = d.nrem_sp;
+    p->remote_z1 = d.rem_z1;
+    p->remote_z2 = d.rem_z2;
+}
+
+void kem_zh_destroy(kem_zh* z)
+{
+    if (!z) return;
+    if (z->ctx) {
+        (void)hipSetDevice(z->ctx->device);
+        (void)hipStreamSynchronize(z->ctx->stream);
+    }
+    dfree(z->node);
+    dfree(z->child);
+    dfree(z->elem);
+    dfree(z->sp);
+    dfree(z->coeff);
+    dfree(z->leg);
+    dfree(z->sigma);
+    dfree(z->counts);
+    delete z;
+}
+
+int kem_zh_create(kem_ctx* c, const kem_mesh* m, const double* sigma, const kem_zh_params* params, kem_zh** out)
+{
+    if (!c || !m || !sigma || !out) return KEM_E_ARG;
+    *out = nullptr;
+    if (m->ctx != c) return fail(c, KEM_E_STATE, "mesh belongs to another context");
+    if (m->gstart) return fail(c, KEM_E_ARG, "zonal harmonics: meshes with symmetry groups are not supported");
+    kem_zh_params dp;
+    kem_zh_params_default(&dp);
+    const kem_zh_params& q = params ? *params : dp;
+    kem::ZHParams p;
+    p.nbif = q.n_bifurcations;
+    p.conv_ratio = q.convergence_ratio;
+    p.prox = q.proximity_to_source_point;
+    p.conv_param = q.convergence_parameter;
+    p.ncen = q.n_central_coefficients;
+    p.frac_spacing = q.central_fractional_spacing != 0;
+    p.frac_dist = q.central_fractional_distance;
+    p.cen_dz = q.central_delta_z;
+    p.cen_z1 = q.central_z1;
+    p.cen_z2 = q.central_z2;
+    p.nrem = q.n_remote_coefficients;
+    p.nrem_sp = q.n_remote_source_points;
+    p.rem_z1 = q.remote_z1;
+    p.rem_z2 = q.remote_z2;
+    if (p.nbif < 0 || p.nbif > 20 || p.ncen < 3 || p.nrem < 3 || p.nrem_sp < 1 || !(p.conv_ratio > 0.) ||
+        !(p.frac_dist > 0.) || !(p.cen_dz > 0.) || !(p.conv_param > 0.))
+        return fail(c, KEM_E_ARG, "zonal harmonics: invalid parameters");
+    KEM_CK(c, enter(c));
+    // the elements as the generators see them (radius = x, axial position = z)
+    const int64_t n = m->n;
+    std::vector<double> g((size_t)n * KG_STRIDE);
+    KEM_CK(c, hipMemcpy(g.data(), m->geom, sizeof(double) * g.size(), hipMemcpyDeviceToHost));
+    std::vector<kem::ZHElement> el((size_t)n);
+    for (int64_t i = 0; i < n; i++) {
+        const double* d = g.data() + i * KG_STRIDE + KG_V;
+        kem::ZHElement& e = el[(size_t)i];
+        e.sigma = sigma[i];
+        if (m->shape[(size_t)i] == KEM_SHAPE_RING) {
+            e.type = kem::ZHElement::kRing;
+            e.r0 = d[0];
+            e.z0 = d[2];
+        } else if (m->shape[(size_t)i] == KEM_SHAPE_CONICSECTION) {
+            e.type = kem::ZHElement::kConic;
+            e.r0 = d[0];
+            e.z0 = d[2];
+            e.r1 = d[3];
+            e.z1 = d[5];
+        }
+    }
+    const kem::ZHTables t = kem::zh_build(el, p);
+    kem_zh* z = new kem_zh();
+    z->ctx = c;
+    z->mesh = m;
+    std::vector<double> spf(4 * t.sp.size());
+    for (size_t i = 0; i < t.sp.size(); i++) {
+        spf[4 * i + 0] = t.sp[i].z0;
+        spf[4 * i + 1] = t.sp[i].rho;
+        spf[4 * i + 2] = (double)t.sp[i].offset;
+        spf[4 * i + 3] = (double)t.sp[i].ncoef;
+    }
+    hipError_t e = hipSuccess;
+    auto up = [&](auto** dst, const auto& v) {
+        if (e != hipSuccess || v.empty()) return;
+        e = dalloc(dst, (int64_t)v.size());
+        if (e == hipSuccess) e = hipMemcpy(*dst, v.data(), sizeof(v[0]) * v.size(), hipMemcpyHostToDevice);
+    };
+    up(&z->node, t.node);
+    up(&z->child, t.child);
+    up(&z->elem, t.elem);
+    up(&z->sp, spf);
+    up(&z->coeff, t.coeff);
+    up(&z->leg, t.leg);
+    up(&z->sigma, std::vector<double>(sigma, sigma + n));
+    if (e == hipSuccess) e = dalloc(&z->counts, 3);
+    if (e != hipSuccess) {
+        kem_zh_destroy(z);
+        return fail(c, KEM_E_HIP, std::string("zonal harmonics: ") + hipGetErrorString(e));
+    }
+    z->dev.node = z->node;
+    z->dev.child = z->child;
+    z->dev.elem = z->elem;
+    z->dev.sp = z->sp;
+    z->dev.coeff = z->coeff;
+    z->dev.leg = z->leg;
+    z->dev.nleg = t.nleg;
+    z->dev.depth = t.depth;
+    z->dev.conv_ratio = p.conv_ratio;
+    z->dev.prox = p.prox;
+    z->dev.conv_param = p.conv_param;
+    z->nnodes = (int64_t)t.node.size() / 8;
+    for (size_t k = 0; k < t.node.size(); k += 8) {
+        z->ncentral += t.node[k + 1];
+        z->nremote += t.node[k + 3];
+    }
+    *out = z;
+    return KEM_OK;
+}
+
+int kem_zh_info(const kem_zh* z, int64_t* out)
+{
+    if (!z || !out) return KEM_E_ARG;
+    out[0] = z->nnodes;
+    out[1] = z->ncentral;
+    out[2] = z->nremote;
+    out[3] = z->dev.depth;
+    return KEM_OK;
+}
+
+int kem_zh_field_eval_device(kem_zh* z, int64_t npts, const double* x, const double* y, const double* zz,
+                             double* phi, double* ex, double* ey, double* ez, int64_t* counts)
+{
+    if (!z || (npts > 0 && (!x || !y || !zz || !phi || !ex || !ey || !ez))) return KEM_E_ARG;
+    kem_ctx* c = z->ctx;
+    if (counts) counts[0] = counts[1] = counts[2] = 0;
+    if (npts <= 0) return KEM_OK;
+    KEM_CK(c, enter(c));
+    if (counts) KEM_CK(c, hipMemsetAsync(z->counts, 0, 3 * sizeof(unsigned long long), c->stream));
+    const kem_mesh* m = z->mesh;
+    KEM_CK(c, kem::launch_zh_field(c->stream, z->dev, m->geom, m->nodes, m->recs, z->sigma, c->policy, x, y, zz,
+                                   npts, phi, ex, ey, ez, counts ? z->counts : nullptr));
+    if (counts) {
+        unsigned long long h[3];
+        KEM_CK(c, hipMemcpyAsync(h, z->counts, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+        KEM_CK(c, hipStreamSynchronize(c->stream));
+        for (int k = 0; k < 3; k++) counts[k] = (int64_t)h[k];
+    }
+    return KEM_OK;
+}
+
+int kem_zh_field_eval(kem_zh* z, int64_t npts, const double* P, double* phi, double* E, int64_t* counts)
+{
+    if (!z || (npts > 0 && !P)) return KEM_E_ARG;
+    kem_ctx* c = z->ctx;
+    if (npts <= 0) {
+        if (counts) counts[0] = counts[1] = counts[2] = 0;
+        return KEM_OK;
+    }
+    KEM_CK(c, enter(c));
+    std::vector<double> soa((size_t)npts * 3);
+    for (int64_t i = 0; i < npts; i++)
+        for (int k = 0; k < 3; k++) soa[(size_t)k * npts + i] = P[3 * i + k];
+    double *dP = nullptr, *dO = nullptr;
+    hipError_t e = dalloc(&dP, 3 * npts);
+    if (e == hipSuccess) e = dalloc(&dO, 4 * npts);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(dP, soa.data(), sizeof(double) * 3 * npts, hipMemcpyHostToDevice, c->stream);
+    int rc = KEM_OK;
+    if (e == hipSuccess) {
+        rc = kem_zh_field_eval_device(z, npts, dP, dP + npts, dP + 2 * npts, dO, dO + npts, dO + 2 * npts,
+                                      dO + 3 * npts, counts);
+        if (rc == KEM_OK) {
+            std::vector<double> o((size_t)npts * 4);
+            e = hipMemcpyAsync(o.data(), dO, sizeof(double) * 4 * npts, hipMemcpyDeviceToHost, c->stream);
+            if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+            if (e == hipSuccess)
+                for (int64_t i = 0; i < npts; i++) {
+                    if (phi) phi[i] = o[(size_t)i];
+                    if (E)
+                        for (int k = 0; k < 3; k++) E[3 * i + k] = o[(size_t)(k + 1) * npts + i];
+                }
+        }
+    }
+    dfree(dP);
+    dfree(dO);
+    if (rc != KEM_OK) return rc;
+    if (e != hipSuccess) return fail(c, KEM_E_HIP, std::string("zonal harmonics: ") + hipGetErrorString(e));
+    return KEM_OK;
 }
 
 }  // extern "C"

@@ -138,6 +138,15 @@ int64_t ko_robin_hood(int64_t n, const double* A, const double* b, double* x, do
 /* Gauss-Jordan without pivoting: LinearAlgebra/Solvers/include/KGaussianElimination.hh:24-58 */
 int ko_gauss(int64_t n, const double* A, const double* b, double* x);
 
+/* zonal-harmonic field solver of rings and conic sections (kem_oracle_zh.c): build over n
+   elements (arrays must outlive the handle), evaluate (E, phi) at m points (out m x 4) with
+   counts[3] = central, remote and direct evaluations, tree sizes, a central source point */
+void* ko_zh_build(int64_t n, const int32_t* shape, const double* recs, const double* sigma, const double* params);
+void ko_zh_eval(void* h, int64_t m, const double* P, double* out, int64_t* counts);
+void ko_zh_info(void* h, int64_t* out);
+int ko_zh_central_sp(void* h, int k, double* out);
+void ko_zh_free(void* h);
+
 #ifdef __cplusplus
 }
 #endif
