@@ -47,10 +47,11 @@ PEAK_HBM_GBS = 8000.0
 # same host type: 3.68 / 2.95 M point-element pairs/s (1,728-triangle cube, 4,096 points)
 CPU_ORACLE_OVER_REFERENCE = 3.68 / 2.95
 
-# HBM bytes per launch (one dispatch) of the mixed far kernel on the headline workload, from
-# rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs scaled by the readings of a 1 GiB device copy
-# (FETCH_SIZE x 1.99990, WRITE_SIZE x 1.0); None until measured.  Source: the profile named here.
-TRAFFIC = {"bytes": None, "profile": "profiles/pmc_traffic_k_field_far32c_bore100k_10Mpts.md"}
+# HBM bytes per launch (one dispatch of 1,973,760 points, 16 element ranges) of the mixed far
+# kernel on the headline workload, from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs scaled by
+# the readings of a 1 GiB device copy (FETCH_SIZE x 1.99990, WRITE_SIZE x 1.0): 0.996 GB read,
+# 1.011 GB written (the 16 partial-sum slices).  Source: the profile named here.
+TRAFFIC = {"bytes": 2.006e9, "profile": "profiles/pmc_traffic_k_field_far32c_bore100k_10Mpts.md"}
 
 METRIC = "point-element field evals/s + solve time, mixed vs FP64 at 1e-6 rel. error"
 
@@ -169,6 +170,42 @@ def _census_paths(ctx, cont, X, m):
     return c, alg, executed
 
 
+# FLOPs of one collocation entry (Dirichlet row: potential only) by regime, census order
+# (triangle 7/12/33/closed, rectangle 7/12/33/closed, segment 4/16/analytic, unused)
+ASM_FLOPS = np.array([95, 155, 407, 200, 95, 155, 407, 200, 59, 203, 100, 0], dtype=np.float64)
+ASM_FAR = np.ones(12)
+ASM_FAR[[3, 7, 10, 11]] = 0.0
+
+
+def _assembly_flops(ctx, cont, mesh):
+    """census of the collocation pairs (targets at the element centroids) by FP64 regime:
+    (FLOPs of the cubature kernel's entries, FLOPs of the closed-form entries)"""
+    from kassiopeia_amd import _lib
+    from kassiopeia_amd import kemfield as K
+
+    C = mesh.centroids()
+    T = [K.DeviceArray(ctx, np.ascontiguousarray(C[:, k])) for k in range(3)]
+    c = _census(ctx, cont, T, mesh.n, _lib.KEM_FP64)
+    for a in T:
+        a.close()
+    return float((c * ASM_FLOPS * ASM_FAR).sum()), float((c * ASM_FLOPS * (1 - ASM_FAR)).sum())
+
+
+def _assemble_timed(ctx, cont, **storage):
+    """assemble with kernel timing: (matrix, wall seconds incl. allocation, kernel ms)"""
+    from kassiopeia_amd import kemfield as K
+
+    ctx.sync()
+    _timing(ctx, True)
+    t0 = time.perf_counter()
+    A = K.KBoundaryIntegralMatrix(cont, **storage)
+    ctx.sync()
+    t = time.perf_counter() - t0
+    kt, kn = _kernel_time(ctx)
+    _timing(ctx, False)
+    return A, t, kt
+
+
 def _census(ctx, cont, X, m, precision):
     c = (ctypes.c_int64 * 12)()
     ctx.check(ctx.lib.kem_field_census(ctx.h, cont.h, m, X[0].ptr, X[1].ptr, X[2].ptr, precision, c))
@@ -284,11 +321,8 @@ def run_bore(ctx, args):
 
     def pipeline(mixed, outs, keep_matrix=False):
         """assembly + GMRES 1e-10 + field pass, wall time between stream synchronisations"""
-        ctx.sync()
         t0 = time.perf_counter()
-        A = K.KBoundaryIntegralMatrix(cont, fp64=True, fp32=mixed)
-        ctx.sync()
-        t_asm = time.perf_counter() - t0
+        A, t_asm, asm_ms = _assemble_timed(ctx, cont, fp64=True, fp32=mixed)
         x, st, _ = A.solve(b, tolerance=1e-10, precision="mixed" if mixed else "fp64")
         t_solve = time.perf_counter() - t0 - t_asm
         t_field, kt = field(x, _lib.KEM_MIXED if mixed else _lib.KEM_FP64, outs, timed=True)
@@ -296,7 +330,8 @@ def run_bore(ctx, args):
         if not keep_matrix:
             A.close()
             A = None
-        return {"total": total, "assembly": t_asm, "solve": t_solve, "field": t_field, "far_ms": kt[0],
+        return {"total": total, "assembly": t_asm, "asm_ms": asm_ms, "solve": t_solve, "field": t_field,
+                "far_ms": kt[0],
                 "far_regions": kt[1], "iterations": st.iterations, "refinements": st.refinements,
                 "residual": st.true_residual, "sigma": x, "A": A}
 
@@ -348,6 +383,38 @@ def run_bore(ctx, args):
     achieved = far_flops / (far_ms * 1e-3) / 1e12
     log(f"far kernel (mixed): {far_ms:.1f} ms per pass in {launches} launches, census {far_flops:.3e} FLOP "
         f"-> {achieved:.1f} TFLOP/s; pairs by regime {cmx.astype(int).tolist()}")
+
+    # ------------------------------------------------------------ the assembly kernels' roofline
+    asm_ms = pm["asm_ms"]
+    asm_far_flops, asm_near_flops = _assembly_flops(ctx, cont, mesh)
+    asm_tflops = (asm_far_flops + asm_near_flops) / (asm_ms * 1e-3) / 1e12
+    log(f"assembly kernels (FP64 + FP32 storage): {asm_ms:.1f} ms for {asm_far_flops + asm_near_flops:.3e} FLOP "
+        f"= {asm_tflops:.1f} TFLOP/s ({asm_tflops / PEAK_FP64_TFLOPS:.2f} of the FP64 peak)")
+    out.update(assembly_kernel_ms=asm_ms, assembly_flops=asm_far_flops + asm_near_flops,
+               assembly_tflops=asm_tflops, assembly_frac_fp64_peak=asm_tflops / PEAK_FP64_TFLOPS)
+
+    # ------------------------------------------------------------ BF16 inner solves (FP64 refinement)
+    ctx.sync()
+    t0 = time.perf_counter()
+    Ab = K.KBoundaryIntegralMatrix(cont, fp64=True, fp32=False, bf16=True)
+    ctx.sync()
+    tb_asm = time.perf_counter() - t0
+    xb, stb16, _ = Ab.solve(b, tolerance=1e-10, precision="bf16")
+    tb_solve = time.perf_counter() - t0 - tb_asm
+    Ab.close()
+    Otmp = [K.DeviceArray.empty(ctx, M) for _ in range(4)]
+    tb_field, _ = field(xb, _lib.KEM_MIXED, Otmp)
+    eb = _errors(E64, phi64, np.stack([Otmp[k].get() for k in (1, 2, 3)], 1), Otmp[0].get(), vscale)
+    for a in Otmp:
+        a.close()
+    tb_total = tb_asm + tb_solve + tb_field
+    log(f"BF16-inner pipeline: assembly {tb_asm:.2f} s, GMRES {stb16.iterations} it / {stb16.refinements} ref "
+        f"{tb_solve:.2f} s (res {stb16.true_residual:.1e}), field {tb_field:.2f} s = {tb_total:.2f} s "
+        f"({p64['total'] / tb_total:.2f}x FP64); error max/rms {eb['E_max_over_rms']:.2e}")
+    out.update(bf16_pipeline_s=tb_total, bf16_assembly_s=tb_asm, bf16_solve_s=tb_solve,
+               bf16_gmres_iterations=stb16.iterations, bf16_refinements=stb16.refinements,
+               bf16_residual=stb16.true_residual, bf16_pipeline_speedup=p64["total"] / tb_total,
+               bf16_err_E_max_over_rms=eb["E_max_over_rms"], bf16_err_E_l2=eb["E_l2"])
 
     # ------------------------------------------------------------ where the error comes from
     sig64 = p64["sigma"]
@@ -565,6 +632,41 @@ def run_coax(ctx, args):
     out.update(batch64_solve_ms=stb.seconds * 1e3, batch64_iterations=stb.iterations,
                batch64_refinements=stb.refinements, batch64_worst_residual=float(res.max()),
                batch64_speedup_vs_64_single=64 * stmx.seconds / stb.seconds)
+    # block products of 64 right-hand sides on MFMA: FP32 matrix (k_bmm32) and BF16 matrix
+    # (k_bmm16), kernel time from HIP events; and the 64 sets in lockstep on the BF16 matrix
+    Ab = K.KBoundaryIntegralMatrix(cont, fp64=True, fp32=False, bf16=True)
+    Xr = np.random.default_rng(9).normal(size=(N, 64))
+    for mat, prec, key, bpe in ((A, "mixed", "fp32", 4), (Ab, "bf16", "bf16", 2)):
+        mat.MultiplyBatch(Xr, prec)
+        _timing(ctx, True)
+        for _ in range(5):
+            mat.MultiplyBatch(Xr, prec)
+        kt, kn = _kernel_time(ctx)
+        _timing(ctx, False)
+        ms = kt / max(kn, 1)
+        gbs = bpe * N * N / (ms * 1e-3) / 1e9
+        tfl = 2.0 * N * N * 64 / (ms * 1e-3) / 1e12
+        log(f"block product, 64 right-hand sides, {key} matrix: {ms:.3f} ms = {gbs:.0f} GB/s of matrix, "
+            f"{tfl:.1f} TFLOP/s")
+        out.update({f"coax20k_bmm64_ms_{key}": ms, f"coax20k_bmm64_gbs_{key}": gbs,
+                    f"coax20k_bmm64_tflops_{key}": tfl, f"coax20k_bmm64_frac_hbm_{key}": gbs / PEAK_HBM_GBS})
+    Ab.solve_batch(B, tolerance=1e-10, precision="bf16")  # warm
+    Xb16, res16, st16, _ = Ab.solve_batch(B, tolerance=1e-10, precision="bf16")
+    xs16, sts16, _ = Ab.solve(b, tolerance=1e-10, precision="bf16")
+    log(f"64 voltage sets, lockstep BF16 GMRES: {st16.seconds*1e3:.1f} ms ({st16.iterations} it, "
+        f"{st16.refinements} ref, worst {res16.max():.2e}); single right-hand side BF16 {sts16.seconds*1e3:.1f} ms "
+        f"({sts16.iterations} it, {sts16.refinements} ref)")
+    out.update(batch64_solve_ms_bf16=st16.seconds * 1e3, batch64_iterations_bf16=st16.iterations,
+               batch64_refinements_bf16=st16.refinements, batch64_worst_residual_bf16=float(res16.max()),
+               coax20k_solve_ms_bf16=sts16.seconds * 1e3, coax20k_gmres_iterations_bf16=sts16.iterations)
+    Ab.close()
+    # Robin Hood (one correction per step), to 1e-8 or 3 s
+    xr, strh, _ = A.solve(b, method="robin_hood", tolerance=1e-8, time_limit_s=3.0)
+    log(f"Robin Hood: {strh.sub_iterations} corrections in {strh.seconds:.2f} s "
+        f"({strh.sub_iterations / strh.seconds:.3e}/s), residual {strh.residual:.2e}")
+    out.update(coax20k_robin_hood_s=strh.seconds, coax20k_robin_hood_corrections=strh.sub_iterations,
+               coax20k_robin_hood_corrections_per_s=strh.sub_iterations / strh.seconds,
+               coax20k_robin_hood_residual=strh.residual)
     npb = 4_000_000
     rng = np.random.default_rng(4096)
     sizes = []
@@ -604,6 +706,10 @@ def run_coax(ctx, args):
     ctx.sync()
     t_casm = time.perf_counter() - t0
     xc, stc, _ = Ac.solve(ccont.rhs(), tolerance=1e-10)
+    xrh, strc, _ = Ac.solve(ccont.rhs(), method="robin_hood", tolerance=1e-8)
+    log(f"cube 1728 Robin Hood: {strc.sub_iterations} corrections in {strc.seconds*1e3:.1f} ms, residual "
+        f"{strc.residual:.1e}")
+    out.update(cube1728_robin_hood_ms=strc.seconds * 1e3, cube1728_robin_hood_corrections=strc.sub_iterations)
     Ac.close()
     cap = float((cube.areas() * xc).sum() / (4 * np.pi * 8.8541878128e-12))
     cpts = G.points_shell(4096, 0.9, 2.0, seed=7)

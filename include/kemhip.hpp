@@ -453,6 +453,7 @@ class KSElectricKEMField {
     }
 
   private:
+    friend class KSRootElectricField;
     const Context& ctx() const { return field_.Container().context(); }
     double* alloc(int64_t count)
     {
@@ -484,6 +485,96 @@ class KSElectricKEMField {
     double* sigma_ = nullptr;
     double* x_[3] = {nullptr, nullptr, nullptr};
     double* out_[4] = {nullptr, nullptr, nullptr, nullptr};  // Ex, Ey, Ez, phi
+    int64_t cap_ = 0;
+    std::vector<double> host_;
+};
+
+// KSRootElectricField (Kassiopeia/Simulation/Source/KSRootElectricField.cxx:104-121): the sum
+// of several electric-field components at the sample points of a step, added per point in the
+// order the components were added, from zero.  Here a step's sample points are evaluated in
+// one call for all components (kem_field_eval_fields: one pass per component, summed on the
+// device in the same order).  Components must share one context and be initialized.
+class KSRootElectricField {
+  public:
+    KSRootElectricField() = default;
+    ~KSRootElectricField() { release(); }
+    KSRootElectricField(const KSRootElectricField&) = delete;
+    KSRootElectricField& operator=(const KSRootElectricField&) = delete;
+
+    void AddElectricField(KSElectricKEMField* f) { fields_.push_back(f); }
+    void RemoveElectricField(KSElectricKEMField* f)
+    {
+        for (size_t k = 0; k < fields_.size(); k++)
+            if (fields_[k] == f) {
+                fields_.erase(fields_.begin() + (long)k);
+                return;
+            }
+        throw Error(KEM_E_ARG, "KSRootElectricField: field not found");
+    }
+
+    void CalculateFieldAndPotential(const Position& P, const double& t, FieldVector& E, double& phi)
+    {
+        std::vector<FieldVector> Ev;
+        std::vector<double> pv;
+        CalculateFieldAndPotential(std::vector<Position>{P}, t, Ev, pv);
+        E = Ev[0];
+        phi = pv[0];
+    }
+    void CalculateFieldAndPotential(const std::vector<Position>& P, const double& /*t*/, std::vector<FieldVector>& E,
+                                    std::vector<double>& phi)
+    {
+        const int64_t n = (int64_t)P.size();
+        E.assign(P.size(), FieldVector{0., 0., 0.});
+        phi.assign(P.size(), 0.);
+        if (n == 0 || fields_.empty()) return;
+        const Context& c = fields_[0]->ctx();
+        std::vector<const kem_mesh*> meshes;
+        std::vector<const double*> sigmas;
+        std::vector<int> prec;
+        for (KSElectricKEMField* f : fields_) {
+            if (!f->sigma_) throw Error(KEM_E_STATE, "KSRootElectricField: a component is not initialized");
+            if (f->ctx().get() != c.get()) throw Error(KEM_E_STATE, "KSRootElectricField: components on two contexts");
+            meshes.push_back(f->field_.Container().get());
+            sigmas.push_back(f->sigma_);
+            prec.push_back(f->field_.MixedPrecision() ? KEM_MIXED : KEM_FP64);
+        }
+        reserve(c, n);
+        host_.resize((size_t)(4 * n));
+        for (int64_t k = 0; k < n; k++)
+            for (int q = 0; q < 3; q++) host_[(size_t)(q * n + k)] = P[(size_t)k][(size_t)q];
+        const int64_t bytes = (int64_t)sizeof(double) * n;
+        for (int q = 0; q < 3; q++) c.check(kem_dev_upload(c.get(), buf_[q], host_.data() + q * n, bytes));
+        c.check(kem_field_eval_fields(c.get(), (int64_t)fields_.size(), meshes.data(), sigmas.data(), prec.data(), n,
+                                      buf_[0], buf_[1], buf_[2], buf_[6], buf_[3], buf_[4], buf_[5]));
+        for (int q = 0; q < 4; q++) c.check(kem_dev_download(c.get(), host_.data() + q * n, buf_[3 + q], bytes));
+        for (int64_t k = 0; k < n; k++) {
+            for (int q = 0; q < 3; q++) E[(size_t)k][(size_t)q] = host_[(size_t)(q * n + k)];
+            phi[(size_t)k] = host_[(size_t)(3 * n + k)];
+        }
+    }
+
+  private:
+    void reserve(const Context& c, int64_t n)
+    {
+        if (n <= cap_ && ctx_ == c.get()) return;
+        release();
+        ctx_ = c.get();
+        for (double*& p : buf_) {
+            void* v = nullptr;
+            c.check(kem_dev_alloc(ctx_, (int64_t)sizeof(double) * n, &v));
+            p = (double*)v;
+        }
+        cap_ = n;
+    }
+    void release()
+    {
+        for (double*& p : buf_)
+            if (p && ctx_) (void)kem_dev_free(ctx_, p), p = nullptr;
+        cap_ = 0;
+    }
+    std::vector<KSElectricKEMField*> fields_;
+    kem_ctx* ctx_ = nullptr;
+    double* buf_[7] = {};  // x, y, z, Ex, Ey, Ez, phi
     int64_t cap_ = 0;
     std::vector<double> host_;
 };

@@ -4,6 +4,7 @@
 // (CubeCapacitanceTest.cc-style).  Prints "facade OK" and exits 0 when every check passes.
 #include <cmath>
 #include <cstdio>
+#include <cstring>
 #include <vector>
 
 #include "kemhip.hpp"
@@ -17,7 +18,17 @@ static void check(bool ok, const char* what, double v)
     if (!ok) fails++;
 }
 
-int main()
+// raw dump of the step-level root field for the oracle check (tests/test_gpu_facade.py)
+static void dump_mesh(std::FILE* f, const SurfaceData& s, const std::vector<double>& sigma)
+{
+    const int64_t n = (int64_t)s.size();
+    std::fwrite(&n, sizeof n, 1, f);
+    std::fwrite(s.shape.data(), sizeof(int32_t), (size_t)n, f);
+    std::fwrite(s.data.data(), sizeof(double), (size_t)(11 * n), f);
+    std::fwrite(sigma.data(), sizeof(double), (size_t)n, f);
+}
+
+int main(int argc, char** argv)
 {
     Context ctx(0);
     SurfaceData s;
@@ -203,6 +214,55 @@ int main()
         const double pr = ring.Potential({0., 0., 0.3});
         const double want = 1. / (4. * M_PI * 8.8541878128E-12 * std::sqrt(0.25 + 0.09));
         check(std::fabs(pr - want) < 1e-12 * want, "ring potential on the axis (closed form)", pr / want - 1.);
+    }
+
+    // Kassiopeia's root field over two components (the triangle cube and the rectangle cube):
+    // one call per step for all sample points, the components summed per point in the order
+    // they were added; equal to the two components' own batches added
+    {
+        KElectrostaticBoundaryField field_r(rcont, integ, gmres);
+        KSElectricKEMField k1(field), k2(field_r);
+        k1.InitializeComponent();
+        k2.InitializeComponent();
+        KSRootElectricField root;
+        root.AddElectricField(&k1);
+        root.AddElectricField(&k2);
+        std::vector<Position> pts;
+        for (int k = 0; k < 3000; k++) {
+            const double t = 0.71 * k, r = 0.7 + 2.5 * (k % 23) / 23.;
+            pts.push_back({r * std::cos(t) * std::sin(0.2 + 0.0009 * k), r * std::sin(t) * std::sin(0.2 + 0.0009 * k),
+                           r * std::cos(0.2 + 0.0009 * k)});
+        }
+        std::vector<FieldVector> E, E1, E2;
+        std::vector<double> phi, p1, p2;
+        root.CalculateFieldAndPotential(pts, 0., E, phi);
+        k1.CalculateFieldAndPotential(pts, 0., E1, p1);
+        k2.CalculateFieldAndPotential(pts, 0., E2, p2);
+        double worst = 0.;
+        for (size_t k = 0; k < pts.size(); k++) {
+            worst = std::max(worst, std::fabs(phi[k] - (p1[k] + p2[k])));
+            for (int c = 0; c < 3; c++) worst = std::max(worst, std::fabs(E[k][(size_t)c] - (E1[k][(size_t)c] + E2[k][(size_t)c])));
+        }
+        check(worst == 0., "KSRootElectricField == sum of its components", worst);
+        FieldVector Es;
+        double ps;
+        root.CalculateFieldAndPotential(pts[17], 0., Es, ps);
+        check(ps == phi[17] && Es[2] == E[17][2], "KSRootElectricField single point == batch", ps - phi[17]);
+        if (argc > 1) {
+            std::FILE* f = std::fopen(argv[1], "wb");
+            if (!f) {
+                check(false, "dump file opens", 0.);
+            } else {
+                const int64_t m = (int64_t)pts.size();
+                std::fwrite(&m, sizeof m, 1, f);
+                for (const Position& P : pts) std::fwrite(P.data(), sizeof(double), 3, f);
+                for (const FieldVector& v : E) std::fwrite(v.data(), sizeof(double), 3, f);
+                std::fwrite(phi.data(), sizeof(double), (size_t)m, f);
+                dump_mesh(f, s, field.ChargeDensities());
+                dump_mesh(f, r, field_r.ChargeDensities());
+                std::fclose(f);
+            }
+        }
     }
 
     bool threw = false;
