@@ -125,6 +125,9 @@ int kem_ctx_set_mixed_fp64_ratio(kem_ctx* ctx, double ratio);
    128 rows (four consecutive rows per lane in one 16-byte load; matrices with n % 4 == 0);
    default 64 */
 #define KEM_TUNE_BMM_ROWS 6
+/* Robin Hood sub-iterations: 0 = one workgroup up to 8,192 unknowns and several co-resident
+   workgroups above (default), 1 = always one workgroup (the same corrections either way) */
+#define KEM_TUNE_ROBIN_HOOD_GROUPS 7
 int kem_ctx_set_tuning(kem_ctx* ctx, int param, int64_t value);
 int kem_ctx_get_tuning(const kem_ctx* ctx, int param, int64_t* value);
 /* Instrumentation, no reference counterpart: with timing on, each field evaluation records HIP

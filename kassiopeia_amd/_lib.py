@@ -17,6 +17,7 @@ KEM_SOLVER_GMRES, KEM_SOLVER_ROBIN_HOOD, KEM_SOLVER_GAUSS = 0, 1, 2
 KEM_TUNE_FAR_RANGE_FP64, KEM_TUNE_FAR_RANGE_MIXED, KEM_TUNE_GEMV_BLOCKS, KEM_TUNE_BMM_BLOCKS = 1, 2, 3, 4
 KEM_TUNE_NEAR_BLOCKS = 5
 KEM_TUNE_BMM_ROWS = 6
+KEM_TUNE_ROBIN_HOOD_GROUPS = 7
 
 
 class KemError(RuntimeError):

@@ -12,6 +12,8 @@ The C-ABI exposes four launch-shape knobs (``kem_ctx_set_tuning``):
   second stream (0: one per 256 points).  Its timed region is the whole field pass.
 * ``KEM_TUNE_BMM_ROWS``: the wavefront tile of the FP32 MFMA block product, 64 rows (one float
   per lane and column) or 128 rows (a 16-byte load of four rows per lane feeding four MFMAs).
+* ``KEM_TUNE_ROBIN_HOOD_GROUPS``: Robin Hood over several co-resident workgroups from 8,192
+  unknowns on (0) or always one workgroup (1); both take the same corrections.
 
 Each knob is swept over its candidates; every candidate runs once untimed and then ``repeats``
 times, timed by the HIP events the library records around the kernels the knob shapes
@@ -35,7 +37,8 @@ BMM_ROW_CANDIDATES = (64, 128)
 
 _KNOB = {"far_range_fp64": _lib.KEM_TUNE_FAR_RANGE_FP64, "far_range_mixed": _lib.KEM_TUNE_FAR_RANGE_MIXED,
          "gemv_blocks": _lib.KEM_TUNE_GEMV_BLOCKS, "bmm_blocks": _lib.KEM_TUNE_BMM_BLOCKS,
-         "near_blocks": _lib.KEM_TUNE_NEAR_BLOCKS, "bmm_rows": _lib.KEM_TUNE_BMM_ROWS}
+         "near_blocks": _lib.KEM_TUNE_NEAR_BLOCKS, "bmm_rows": _lib.KEM_TUNE_BMM_ROWS,
+         "robin_hood_groups": _lib.KEM_TUNE_ROBIN_HOOD_GROUPS}
 
 
 def get(ctx: K.Context, name: str) -> int:

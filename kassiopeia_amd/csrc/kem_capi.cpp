@@ -379,6 +379,10 @@ int kem_ctx_set_tuning(kem_ctx* c, int param, int64_t value)
         if (value != 64 && value != 128) return fail(c, KEM_E_ARG, "tuning: block-product rows per wavefront 64 or 128");
         c->tuning.bmm_rows = value;
         return KEM_OK;
+    case KEM_TUNE_ROBIN_HOOD_GROUPS:
+        if (value != 0 && value != 1) return fail(c, KEM_E_ARG, "tuning: Robin Hood workgroups 0 (by size) or 1");
+        c->tuning.rh_groups = value;
+        return KEM_OK;
     default:
         return fail(c, KEM_E_ARG, "tuning: unknown parameter");
     }
@@ -394,6 +398,7 @@ int kem_ctx_get_tuning(const kem_ctx* c, int param, int64_t* value)
     case KEM_TUNE_BMM_BLOCKS: *value = c->tuning.bmm_blocks; return KEM_OK;
     case KEM_TUNE_NEAR_BLOCKS: *value = c->tuning.near_blocks; return KEM_OK;
     case KEM_TUNE_BMM_ROWS: *value = c->tuning.bmm_rows; return KEM_OK;
+    case KEM_TUNE_ROBIN_HOOD_GROUPS: *value = c->tuning.rh_groups; return KEM_OK;
     default: return KEM_E_ARG;
     }
 }
@@ -1401,6 +1406,18 @@ int robin_hood_dev(kem_matrix* A, const double* d_b, double* d_x, const std::vec
     if (binf == 0.) binf = 1.;
     const int64_t interval = o->check_interval > 0 ? o->check_interval : n;
     KEM_CK(c, ensure_scal(c, 260));
+    // the several-workgroup sub-iterations' slots and barrier flags
+    DVec slots;
+    unsigned int* flags = nullptr;
+    const bool grid = kem::robin_hood_groups(n) > 1;
+    if (grid) {
+        KEM_CK(c, dalloc(&slots.p, 2 * 256 * 3));
+        KEM_CK(c, dalloc(&flags, 2));
+    }
+    struct Flags {
+        unsigned int*& p;
+        ~Flags() { dfree(p); }
+    } flags_guard{flags};
     double rnorm = 1.;  // KIterativeSolver::fResidualNorm starts at 1
     int checks = 0;
     int status = KEM_OK;
@@ -1409,10 +1426,13 @@ int robin_hood_dev(kem_matrix* A, const double* d_b, double* d_x, const std::vec
             status = KEM_W_NOT_CONVERGED;
             break;
         }
-        KEM_CK(c, kem::launch_robin_hood(st, A->A64, d_b, d_x, bi.p, n, interval, c->scal + 256));
+        KEM_CK(c, kem::launch_robin_hood(st, A->A64, d_b, d_x, bi.p, n, interval, c->scal + 256, slots.p, flags));
         double v;
+        unsigned int hf[2] = {0, 0};
         KEM_CK(c, hipMemcpyAsync(&v, c->scal + 256, sizeof(double), hipMemcpyDeviceToHost, st));
+        if (grid) KEM_CK(c, hipMemcpyAsync(hf, flags, sizeof(hf), hipMemcpyDeviceToHost, st));
         KEM_CK(c, hipStreamSynchronize(st));
+        if (hf[1]) return fail(c, KEM_E_HIP, "Robin Hood: a workgroup timed out at the grid barrier");
         rnorm = v / binf;
         checks++;
     }

@@ -333,17 +333,6 @@ __device__ __forceinline__ double distance_ratio(const double* __restrict__ g, d
     return sqrt(__builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx))) * g[KG_INVSIZE];
 }
 
-// regime of a collocation pair exactly as element_eval picks it under the numeric policy:
-// 0 / 1 / 2 = 7 / 12 / 33-point cubature (segments: 4 / 16 nodes, no 2), 3 = closed form
-__device__ __forceinline__ int assemble_regime(const double* __restrict__ g, double px, double py, double pz)
-{
-    const int type = (int)g[KG_TYPE];
-    const double dr = distance_ratio(g, px, py, pz);
-    if (type == KEM_TRIANGLE) return dr > kTriCub12 ? 0 : dr > kTriCub33 ? 1 : dr > kTriRWG ? 2 : 3;
-    if (type == KEM_RECTANGLE) return dr > kRectCub12 ? 0 : dr > kRectCub33 ? 1 : dr > kRectRWG ? 2 : 3;
-    return dr > kLineQuad16 ? 0 : dr > kLineAna ? 1 : 3;
-}
-
 // One (element, point) pair under the "numeric" policy (default) or the closed form everywhere
 // ("rwg").  Returns the unit-charge (E, phi).
 template <bool FIELD>
@@ -397,17 +386,19 @@ __device__ __forceinline__ bool is_near2(const double* __restrict__ g, double r2
 }
 
 // Far pass of one pair (reference ladder, FP64).  Called only for pairs outside the near set.
+// FIELD = false: the potential only (collocation rows of Dirichlet elements).
+template <bool FIELD = true>
 __device__ __forceinline__ F4 element_far(const double* __restrict__ g, const double* __restrict__ nd, int type,
                                           double r2, double px, double py, double pz)
 {
     const double pref = g[KG_PREF];
     if (type == KEM_LINESEGMENT) {
-        if (r2 > g[KG_REF7_2]) return line_quadrature<2, true>(nd + 4 * KEM_L4, pref, px, py, pz);
-        return line_quadrature<8, true>(nd + 4 * KEM_L16, pref, px, py, pz);
+        if (r2 > g[KG_REF7_2]) return line_quadrature<2, FIELD>(nd + 4 * KEM_L4, pref, px, py, pz);
+        return line_quadrature<8, FIELD>(nd + 4 * KEM_L16, pref, px, py, pz);
     }
-    if (r2 > g[KG_REF7_2]) return cubature<7, true>(nd + 4 * KEM_N7, pref, px, py, pz);
-    if (r2 > g[KG_REF12_2]) return cubature<12, true>(nd + 4 * KEM_N12, pref, px, py, pz);
-    return cubature<33, true>(nd + 4 * KEM_N33, pref, px, py, pz);
+    if (r2 > g[KG_REF7_2]) return cubature<7, FIELD>(nd + 4 * KEM_N7, pref, px, py, pz);
+    if (r2 > g[KG_REF12_2]) return cubature<12, FIELD>(nd + 4 * KEM_N12, pref, px, py, pz);
+    return cubature<33, FIELD>(nd + 4 * KEM_N33, pref, px, py, pz);
 }
 
 // Near pass of one pair (closed forms).

@@ -122,6 +122,7 @@ struct LaunchTuning {
     int64_t bmm_blocks = 2560;
     int64_t near_blocks = 384;  // grid of the near pass beside the mixed far pass (0: one block per point block)
     int64_t bmm_rows = 64;      // rows per wavefront tile of the FP32 block product (64, or 128 when n % 4 == 0)
+    int64_t rh_groups = 0;      // workgroups of the Robin Hood sub-iterations (0: by size, 1: one)
 };
 void set_launch_tuning(const LaunchTuning& t);
 const LaunchTuning& launch_tuning();
@@ -180,8 +181,12 @@ hipError_t launch_axpby(hipStream_t st, double a, const double* x, double b, con
 // f: scratch of n doubles; x: n doubles
 hipError_t launch_gauss_jordan(hipStream_t st, double* M, int64_t n, double* f, double* x);
 
+// Robin Hood sub-iterations: one workgroup up to 8,192 unknowns, above that robin_hood_groups(n)
+// co-resident workgroups (cooperative launch) with slots (2 x 256 x 3 doubles) and flags (2
+// uints: arrival counter, fault; flags[1] != 0 after the call: a workgroup timed out waiting)
+int robin_hood_groups(int64_t n);
 hipError_t launch_robin_hood(hipStream_t st, const double* A, const double* b, double* x, double* bi, int64_t n,
-                             int64_t steps, double* rnorm);
+                             int64_t steps, double* rnorm, double* slots, unsigned int* flags);
 
 // ---- multi-right-hand-side kernels (kem_batch.hip)
 // Block-product precisions: FP64 matrix (VALU), FP32 matrix (FP32 MFMA), BF16 matrix (BF16 MFMA),

@@ -685,8 +685,10 @@ __global__ __launch_bounds__(kBlock) void k_assemble(const double* __restrict__ 
 // two kernels, as the field pass does: k_assemble_far evaluates the cubature / Gauss-Legendre
 // entries and skips the closed-form ones, k_assemble_near evaluates only the closed-form
 // entries for the targets whose wavefront can reach an element cluster.  Both classify a pair
-// with element_eval's own test (assemble_near), so every entry is written by exactly one of
-// them, with the value k_assemble gives it.
+// by the field passes' squared-distance tests (dist2 against the element's (dr_cut size)^2,
+// no square root per pair), so every entry is written by exactly one of them; the rule a pair
+// gets differs from the reference's dr > cut only within an ulp of a cut-off, where both rules
+// meet the reference's accuracy.
 
 // Far entries: lane = collocation target i, wave-uniform source column j, as k_assemble, but
 // the closed-form entries are left to k_assemble_near (no closed form inlined here: the kernel
@@ -711,18 +713,9 @@ __global__ __launch_bounds__(kBlock) void k_assemble_far(const double* __restric
     if (j1 > n) j1 = n;
     for (int64_t j = j0; j < j1; j++) {
         const double* g = geom + j * KG_STRIDE;
-        const double* nd = nodes + j * KEM_NODE_STRIDE;
-        const int reg = dev::assemble_regime(g, cx, cy, cz);
-        if (reg == 3) continue;
-        const double pref = g[KG_PREF];
-        F4 v;
-        if ((int)g[KG_TYPE] == KEM_LINESEGMENT)
-            v = reg == 0 ? dev::line_quadrature<2, FIELD>(nd + 4 * KEM_L4, pref, cx, cy, cz)
-                         : dev::line_quadrature<8, FIELD>(nd + 4 * KEM_L16, pref, cx, cy, cz);
-        else
-            v = reg == 0   ? dev::cubature<7, FIELD>(nd + 4 * KEM_N7, pref, cx, cy, cz)
-                : reg == 1 ? dev::cubature<12, FIELD>(nd + 4 * KEM_N12, pref, cx, cy, cz)
-                           : dev::cubature<33, FIELD>(nd + 4 * KEM_N33, pref, cx, cy, cz);
+        const double r2 = dev::dist2(g, cx, cy, cz);
+        if (dev::is_near2(g, r2, 0)) continue;
+        const F4 v = dev::element_far<FIELD>(g, nodes + j * KEM_NODE_STRIDE, (int)g[KG_TYPE], r2, cx, cy, cz);
         // far entries lie beyond 2 element sizes: never the Neumann self term
         const double a = (!FIELD || dirichlet) ? v.p : (v.x * nx + v.y * ny + v.z * nz);
         if (live) {
@@ -767,7 +760,7 @@ __global__ __launch_bounds__(kBlock) void k_assemble_near(const double* __restri
         for (int64_t e = c * KEM_CLUSTER; e < e1; e++) {
             const int64_t j = order[e];
             const double* g = geom + j * KG_STRIDE;
-            if (dev::assemble_regime(g, cx, cy, cz) != 3) continue;
+            if (!dev::is_near2(g, dev::dist2(g, cx, cy, cz), 0)) continue;
             const int type = (int)g[KG_TYPE];
             const F4 v = type == KEM_TRIANGLE    ? dev::closed_form<3, FIELD>(g, cx, cy, cz)
                          : type == KEM_RECTANGLE ? dev::closed_form<4, FIELD>(g, cx, cy, cz)
@@ -1113,6 +1106,149 @@ __global__ __launch_bounds__(kRHBlock) void k_robin_hood(const double* __restric
         for (int64_t i = threadIdx.x; i < n; i += kRHBlock) bi[i] += col[i] * corr;
         __syncthreads();
     }
+}
+
+// The same sub-iterations over G co-resident workgroups (cooperative launch), for matrices too
+// large for one workgroup to scan each step.  Workgroup g owns rows [g*per, g*per + per) and
+// keeps their b and b_iter in LDS; per step it publishes its first argmax of |b_i - b_iter_i|
+// (value, index, signed residual) in a slot, all workgroups meet at a grid barrier, each
+// reduces the G slots to the same global first argmax (ties to the smaller index, as the scan
+// of k_robin_hood), divides the owner's residual by A_kk, workgroup 0 adds it to x_k, and
+// every workgroup adds the column's share to its rows.  The arithmetic of every value is
+// k_robin_hood's, so both take the same corrections.  Slots alternate between two buffers by
+// step parity; the barrier is a monotonic arrival counter (zero on entry) read with
+// device-scope acquire loads, its wait bounded (a workgroup that waits past the bound sets
+// *fault and the grid stops).
+constexpr int kRHGBlock = 256;
+constexpr int kRHRowsMax = 4096;  // rows per workgroup: b and b_iter in 64 KB of LDS
+// from this many unknowns on, the sub-iterations run over several workgroups
+constexpr int64_t kRobinHoodGridMin = 8192;
+
+__device__ __forceinline__ void rh_store(double* p, double v)
+{
+    __hip_atomic_store(reinterpret_cast<unsigned long long*>(p), (unsigned long long)__double_as_longlong(v),
+                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ double rh_load(const double* p)
+{
+    return __longlong_as_double((long long)__hip_atomic_load(reinterpret_cast<const unsigned long long*>(p),
+                                                             __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+}
+
+__global__ __launch_bounds__(kRHGBlock) void k_robin_hood_grid(const double* __restrict__ A,
+                                                               const double* __restrict__ b, double* __restrict__ x,
+                                                               double* __restrict__ bi, int64_t n, int64_t per,
+                                                               int64_t steps, double* __restrict__ rnorm,
+                                                               double* slots, unsigned int* bar,
+                                                               unsigned int* fault)
+{
+    __shared__ double lb[kRHRowsMax], lbi[kRHRowsMax];
+    __shared__ double sv[kRHGBlock / 64], sr[kRHGBlock / 64];
+    __shared__ int64_t si[kRHGBlock / 64];
+    __shared__ double s_corr;
+    __shared__ int64_t s_idx;
+    __shared__ int s_stop;
+    const int G = gridDim.x;
+    const int64_t r0 = (int64_t)blockIdx.x * per;
+    const int64_t r1 = r0 + per < n ? r0 + per : n;
+    const int64_t nr = r1 > r0 ? r1 - r0 : 0;
+    for (int64_t i = threadIdx.x; i < nr; i += kRHGBlock) {
+        lb[i] = b[r0 + i];
+        lbi[i] = bi[r0 + i];
+    }
+    if (threadIdx.x == 0) s_stop = 0;
+    __syncthreads();
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    for (int64_t s = 0; s < steps; s++) {
+        // this workgroup's first argmax, with the signed residual there
+        double best = -1., bres = 0.;
+        int64_t bidx = n;
+        for (int64_t i = threadIdx.x; i < nr; i += kRHGBlock) {
+            const double d = lb[i] - lbi[i];
+            const double r = fabs(d);
+            if (r > best) {
+                best = r;
+                bidx = r0 + i;
+                bres = d;
+            }
+        }
+        for (int o = 32; o > 0; o >>= 1) {
+            const double ov = __shfl_down(best, o, 64);
+            const int64_t oi = __shfl_down(bidx, o, 64);
+            const double orr = __shfl_down(bres, o, 64);
+            if (ov > best || (ov == best && oi < bidx)) {
+                best = ov;
+                bidx = oi;
+                bres = orr;
+            }
+        }
+        if (lane == 0) {
+            sv[w] = best;
+            si[w] = bidx;
+            sr[w] = bres;
+        }
+        __syncthreads();
+        double* slot = slots + ((s & 1) * (int64_t)G + blockIdx.x) * 3;
+        if (threadIdx.x == 0) {
+            double bv = sv[0], br = sr[0];
+            int64_t bx = si[0];
+            for (int k = 1; k < kRHGBlock / 64; k++)
+                if (sv[k] > bv || (sv[k] == bv && si[k] < bx)) {
+                    bv = sv[k];
+                    bx = si[k];
+                    br = sr[k];
+                }
+            rh_store(slot, bv);
+            rh_store(slot + 1, (double)bx);
+            rh_store(slot + 2, br);
+            // arrive, then wait for every workgroup's arrival at this step
+            __hip_atomic_fetch_add(bar, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            const unsigned int want = (unsigned int)(G * (s + 1));
+            unsigned int spins = 0;
+            while (__hip_atomic_load(bar, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < want) {
+                if (++spins > (1u << 26) || __hip_atomic_load(fault, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                    __hip_atomic_store(fault, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    s_stop = 1;
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+            }
+            if (!s_stop) {
+                // the global first argmax over the slots of this step
+                const double* sl = slots + (s & 1) * (int64_t)G * 3;
+                double gv = rh_load(sl);
+                int64_t gx = (int64_t)rh_load(sl + 1);
+                double gr = rh_load(sl + 2);
+                for (int k = 1; k < G; k++) {
+                    const double v = rh_load(sl + 3 * k);
+                    const int64_t ix = (int64_t)rh_load(sl + 3 * k + 1);
+                    if (v > gv || (v == gv && ix < gx)) {
+                        gv = v;
+                        gx = ix;
+                        gr = rh_load(sl + 3 * k + 2);
+                    }
+                }
+                // no workgroup found a residual > -1: every residual is NaN, the reference
+                // corrects index 0 by its (NaN) residual
+                if (gx >= n) {
+                    gx = 0;
+                    gr = __builtin_nan("");
+                }
+                if (s == steps - 1 && blockIdx.x == 0) *rnorm = gv;
+                const double corr = gr / A[gx * n + gx];
+                if (blockIdx.x == 0) x[gx] += corr;
+                s_corr = corr;
+                s_idx = gx;
+            }
+        }
+        __syncthreads();
+        if (s_stop) break;
+        const double corr = s_corr;
+        const double* col = A + s_idx * n + r0;
+        for (int64_t i = threadIdx.x; i < nr; i += kRHGBlock) lbi[i] += col[i] * corr;
+        __syncthreads();
+    }
+    for (int64_t i = threadIdx.x; i < nr; i += kRHGBlock) bi[r0 + i] = lbi[i];
 }
 
 // ------------------------------------------------------------------ Gauss-Jordan
@@ -1511,13 +1647,21 @@ hipError_t assemble_kernels(hipStream_t st, int policy, bool field, const double
     dim3 grid(gx, (unsigned)nsplit);
     if (!G && !axisym && policy == 0 && clusters && order) {
         // cubature entries and closed-form entries in separate kernels (measured on the
-        // 20,000-triangle coax: 12.8 ms in one kernel at 129 VGPRs)
+        // 20,000-triangle coax: 12.8 ms in one kernel at 129 VGPRs).  The cubature kernel's
+        // grid: enough column splits for ~16 rounds of wavefronts over 256 CUs x 32 slots, so
+        // the last round's partial occupancy costs little (4,096 blocks were 2.1 rounds, a
+        // third of the last one idle)
+        int64_t sp = (32768 + gx - 1) / gx;
+        if (sp > n) sp = n;
+        const int64_t per2 = (n + sp - 1) / sp;
+        sp = (n + per2 - 1) / per2;
+        grid = dim3(gx, (unsigned)sp);
         const int64_t ncl = (n + KEM_CLUSTER - 1) / KEM_CLUSTER;
         if (field) {
-            k_assemble_far<true><<<grid, kBlock, 0, st>>>(geom, nodes, targets, n, per, A64, A32);
+            k_assemble_far<true><<<grid, kBlock, 0, st>>>(geom, nodes, targets, n, per2, A64, A32);
             k_assemble_near<true><<<gx, kBlock, 0, st>>>(geom, targets, clusters, ncl, order, n, A64, A32);
         } else {
-            k_assemble_far<false><<<grid, kBlock, 0, st>>>(geom, nodes, targets, n, per, A64, A32);
+            k_assemble_far<false><<<grid, kBlock, 0, st>>>(geom, nodes, targets, n, per2, A64, A32);
             k_assemble_near<false><<<gx, kBlock, 0, st>>>(geom, targets, clusters, ncl, order, n, A64, A32);
         }
         return hipGetLastError();
@@ -1681,10 +1825,33 @@ hipError_t launch_gauss_jordan(hipStream_t st, double* M, int64_t n, double* f, 
     return hipGetLastError();
 }
 
-hipError_t launch_robin_hood(hipStream_t st, const double* A, const double* b, double* x, double* bi, int64_t n,
-                             int64_t steps, double* rnorm)
+int robin_hood_groups(int64_t n)
 {
-    k_robin_hood<<<1, kRHBlock, 0, st>>>(A, b, x, bi, n, steps, rnorm);
+    if (n <= kRobinHoodGridMin || g_tuning.rh_groups == 1) return 1;
+    int64_t g = (n + 1023) / 1024;
+    g = std::max<int64_t>(g, (n + kRHRowsMax - 1) / kRHRowsMax);
+    return (int)std::min<int64_t>(g, 256);
+}
+
+hipError_t launch_robin_hood(hipStream_t st, const double* A, const double* b, double* x, double* bi, int64_t n,
+                             int64_t steps, double* rnorm, double* slots, unsigned int* flags)
+{
+    const int G = robin_hood_groups(n);
+    if (G == 1) {
+        k_robin_hood<<<1, kRHBlock, 0, st>>>(A, b, x, bi, n, steps, rnorm);
+        return hipGetLastError();
+    }
+    int64_t per = (n + G - 1) / G;
+    if (per > kRHRowsMax || !slots || !flags) return hipErrorInvalidValue;
+    // arrival counter and fault flag start at zero
+    hipError_t e = hipMemsetAsync(flags, 0, 2 * sizeof(unsigned int), st);
+    if (e != hipSuccess) return e;
+    unsigned int* bar = flags;
+    unsigned int* fault = flags + 1;
+    void* args[] = {(void*)&A, (void*)&b,     (void*)&x,     (void*)&bi,  (void*)&n,    (void*)&per,
+                    (void*)&steps, (void*)&rnorm, (void*)&slots, (void*)&bar, (void*)&fault};
+    e = hipLaunchCooperativeKernel((const void*)k_robin_hood_grid, dim3((unsigned)G), dim3(kRHGBlock), args, 0, st);
+    if (e != hipSuccess) return e;
     return hipGetLastError();
 }
 

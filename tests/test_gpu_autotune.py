@@ -81,11 +81,11 @@ def test_tuning_table_roundtrip_and_errors(setup, tmp_path):
 
     ctx = setup[0]
     table = {"far_range_fp64": 1280, "far_range_mixed": {"best": 5120, "ms": {}}, "gemv_blocks": 1024,
-             "bmm_blocks": 2560, "near_blocks": 256, "bmm_rows": 128}
+             "bmm_blocks": 2560, "near_blocks": 256, "bmm_rows": 128, "robin_hood_groups": 1}
     T.save(table, tmp_path / "t.json")
     loaded = T.load(tmp_path / "t.json")
     assert loaded == {"far_range_fp64": 1280, "far_range_mixed": 5120, "gemv_blocks": 1024, "bmm_blocks": 2560,
-                      "near_blocks": 256, "bmm_rows": 128}
+                      "near_blocks": 256, "bmm_rows": 128, "robin_hood_groups": 1}
     T.apply(ctx, loaded)
     assert T.current(ctx) == loaded
     with pytest.raises(_lib.KemError):
@@ -96,8 +96,10 @@ def test_tuning_table_roundtrip_and_errors(setup, tmp_path):
         T.set_knob(ctx, "near_blocks", -1)
     with pytest.raises(_lib.KemError):
         T.set_knob(ctx, "bmm_rows", 96)
+    with pytest.raises(_lib.KemError):
+        T.set_knob(ctx, "robin_hood_groups", 2)
     T.apply(ctx, {"far_range_fp64": 2560, "far_range_mixed": 2560, "gemv_blocks": 2048, "near_blocks": 384,
-                  "bmm_rows": 64})
+                  "bmm_rows": 64, "robin_hood_groups": 0})
 
 
 def test_tune_near(setup):
