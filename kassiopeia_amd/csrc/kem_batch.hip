@@ -175,29 +175,48 @@ __global__ __launch_bounds__(kBlock) void k_bmm32q(const float* __restrict__ A, 
     }
 }
 
-// W[c*ldw + i] = sum_s part[s][i][c] for c < ncols: 64-row tiles summed in FP64 (splits in
-// order) and transposed through LDS so that both the reads and the writes are coalesced.
+// W[c*ldw + i] = sum_s part[s][i][c] for c < ncols, splits summed in order in FP64.  A block
+// takes 32 rows: entry e = threadIdx.x + 256 m (row e / KP, column e % KP) is contiguous in
+// e for each split, so every split's read is coalesced; a thread loads four splits of its
+// 32 KP / 256 entries before adding them (the loads in flight together: 9 splits of a
+// 20,000 x 64 block reduce in 25 us with one split per round trip); the sums are transposed
+// through LDS so that the writes run along rows.
 template <int KP>
 __global__ __launch_bounds__(kBlock) void k_bmm32_reduce(const float* __restrict__ part, int nsplit, int64_t n,
                                                          int ncols, double* __restrict__ W, int64_t ldw,
                                                          const int* __restrict__ skip)
 {
     if (skip && *skip) return;
-    __shared__ double sh[KP][65];
-    const int64_t i0 = (int64_t)blockIdx.x * 64;
-    for (int e = threadIdx.x; e < 64 * KP; e += kBlock) {
-        const int ii = e / KP, c = e % KP;
-        const int64_t i = i0 + ii;
-        double s = 0.;
-        if (i < n)
-            for (int k = 0; k < nsplit; k++) s += (double)part[((int64_t)k * n + i) * KP + c];
-        sh[c][ii] = s;
+    constexpr int R = 32, E = R * KP / kBlock;
+    __shared__ double sh[KP][R + 1];
+    const int64_t i0 = (int64_t)blockIdx.x * R;
+    const int rows = (int)std::min<int64_t>(R, n - i0);
+    const int lim = rows * KP;
+    const float* base = part + i0 * KP;
+    double s[E];
+    for (int m = 0; m < E; m++) s[m] = 0.;
+    constexpr int KS = 4;
+    for (int k0 = 0; k0 < nsplit; k0 += KS) {
+        float v[KS][E];
+        for (int kk = 0; kk < KS; kk++) {
+            const float* p = base + (int64_t)(k0 + kk) * n * KP;
+            for (int m = 0; m < E; m++) {
+                const int e = threadIdx.x + kBlock * m;
+                v[kk][m] = (k0 + kk < nsplit && e < lim) ? p[e] : 0.f;
+            }
+        }
+        for (int kk = 0; kk < KS; kk++)
+            if (k0 + kk < nsplit)
+                for (int m = 0; m < E; m++) s[m] += (double)v[kk][m];
+    }
+    for (int m = 0; m < E; m++) {
+        const int e = threadIdx.x + kBlock * m;
+        sh[e % KP][e / KP] = s[m];
     }
     __syncthreads();
-    for (int e = threadIdx.x; e < 64 * KP; e += kBlock) {
-        const int c = e / 64, ii = e % 64;
-        const int64_t i = i0 + ii;
-        if (c < ncols && i < n) W[c * ldw + i] = sh[c][ii];
+    for (int e = threadIdx.x; e < R * KP; e += kBlock) {
+        const int c = e / R, ii = e % R;
+        if (c < ncols && ii < rows) W[c * ldw + i0 + ii] = sh[c][ii];
     }
 }
 
@@ -274,7 +293,9 @@ __global__ __launch_bounds__(kBlock) void k_bmm16(const bf8v* __restrict__ A16, 
             acc0[t][v] = 0.f;
             acc1[t][v] = 0.f;
         }
-    constexpr int KB = NT == 1 ? 4 : 2;  // keeps 64 right-hand sides at 2 wavefronts per SIMD
+    // column steps per load batch (64 right-hand sides: one step measured 3 % faster than two,
+    // 4 wavefronts per SIMD either way)
+    constexpr int KB = NT == 1 ? 4 : 1;
     int64_t g = g0;
     for (; g + 2 * KB <= g1; g += 2 * KB) {
         bf8v a0[KB], a1[KB], b[KB][NT];
@@ -318,6 +339,61 @@ __global__ __launch_bounds__(kBlock) void k_bmm16(const bf8v* __restrict__ A16, 
             if (rr < lim) out[rr * KP + 32 * t] = acc0[t][v];
             if (rr + 32 < lim) out[(rr + 32) * KP + 32 * t] = acc1[t][v];
         }
+}
+
+// One right-hand side on the BF16 matrix: part[s][i] = sum_{g in split s} sum_m A16(i, 8g + m) *
+// x(8g + m) on the vector ALU (the block product would pad one column to 32 MFMA columns).
+// Lane = row i: one 16-byte load per column group holds the row's 8 BF16 entries; the group's
+// 8 panel values are wave-uniform (one 16-byte scalar load).  FP32 products and sums, as the
+// MFMA product accumulates.  Groups [s*gper, (s+1)*gper) per blockIdx.y.
+__device__ __forceinline__ float bf16_lo(uint32_t w) { return __uint_as_float(w << 16); }
+__device__ __forceinline__ float bf16_hi(uint32_t w) { return __uint_as_float(w & 0xffff0000u); }
+
+__global__ __launch_bounds__(kBlock) void k_gemv16_part(const uint4* __restrict__ A16, int64_t n,
+                                                        const uint4* __restrict__ Vp, int kp, int64_t ngroups,
+                                                        int64_t gper, float* __restrict__ part,
+                                                        const int* __restrict__ skip)
+{
+    if (skip && *skip) return;
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const int64_t g0 = (int64_t)blockIdx.y * gper;
+    const int64_t g1 = g0 + gper < ngroups ? g0 + gper : ngroups;
+    float a0 = 0.f, a1 = 0.f, a2 = 0.f, a3 = 0.f;
+    int64_t g = g0;
+    for (; g + 2 <= g1; g += 2) {
+        const uint4 u = A16[g * n + i], v = A16[(g + 1) * n + i];
+        const uint4 x = Vp[g * kp], y = Vp[(g + 1) * kp];  // column 0 of the panel: 8 values
+        a0 += bf16_lo(u.x) * bf16_lo(x.x) + bf16_hi(u.x) * bf16_hi(x.x);
+        a1 += bf16_lo(u.y) * bf16_lo(x.y) + bf16_hi(u.y) * bf16_hi(x.y);
+        a2 += bf16_lo(u.z) * bf16_lo(x.z) + bf16_hi(u.z) * bf16_hi(x.z);
+        a3 += bf16_lo(u.w) * bf16_lo(x.w) + bf16_hi(u.w) * bf16_hi(x.w);
+        a0 += bf16_lo(v.x) * bf16_lo(y.x) + bf16_hi(v.x) * bf16_hi(y.x);
+        a1 += bf16_lo(v.y) * bf16_lo(y.y) + bf16_hi(v.y) * bf16_hi(y.y);
+        a2 += bf16_lo(v.z) * bf16_lo(y.z) + bf16_hi(v.z) * bf16_hi(y.z);
+        a3 += bf16_lo(v.w) * bf16_lo(y.w) + bf16_hi(v.w) * bf16_hi(y.w);
+    }
+    for (; g < g1; g++) {
+        const uint4 u = A16[g * n + i];
+        const uint4 x = Vp[g * kp];
+        a0 += bf16_lo(u.x) * bf16_lo(x.x) + bf16_hi(u.x) * bf16_hi(x.x);
+        a1 += bf16_lo(u.y) * bf16_lo(x.y) + bf16_hi(u.y) * bf16_hi(x.y);
+        a2 += bf16_lo(u.z) * bf16_lo(x.z) + bf16_hi(u.z) * bf16_hi(x.z);
+        a3 += bf16_lo(u.w) * bf16_lo(x.w) + bf16_hi(u.w) * bf16_hi(x.w);
+    }
+    part[(int64_t)blockIdx.y * n + i] = (a0 + a1) + (a2 + a3);
+}
+
+// W[i] = sum_s part[s][i] (FP64)
+__global__ __launch_bounds__(kBlock) void k_gemv16_reduce(const float* __restrict__ part, int nsplit, int64_t n,
+                                                          double* __restrict__ W, const int* __restrict__ skip)
+{
+    if (skip && *skip) return;
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    double s = 0.;
+    for (int k = 0; k < nsplit; k++) s += (double)part[(int64_t)k * n + i];
+    W[i] = s;
 }
 
 // ------------------------------------------------------------------ FP64 block product (VALU)
@@ -657,7 +733,7 @@ hipError_t launch_bmm(hipStream_t st, const float* A32, const double* A64, int64
             else
                 k_bmm32<2><<<grid, kBlock, 0, st>>>(A32, n, P, per, part, skip);
         }
-        const unsigned gr = (unsigned)((n + 63) / 64);
+        const unsigned gr = (unsigned)((n + 31) / 32);
         if (kp == 32)
             k_bmm32_reduce<32><<<gr, kBlock, 0, st>>>(part, ns, n, nrhs, Y, ldy, skip);
         else
@@ -715,12 +791,26 @@ hipError_t launch_bmm16(hipStream_t st, const uint16_t* A16, int64_t n, const vo
     const bf8v* P = x_is_panel ? (const bf8v*)X : (const bf8v*)Vp;
     if (!x_is_panel)
         k_panel_bf16<<<nblocks(ngroups * kp), kBlock, 0, st>>>((const double*)X, ldx, n, nrhs, kp, ngroups, Vp);
+    if (nrhs == 1) {
+        // one right-hand side: the vector-ALU product, split over column groups into about 768
+        // blocks (3 per CU: 6.0 TB/s on a 20,000-unknown matrix, against 4.5 TB/s at 22,000
+        // blocks of short columns); its partials fit the block product's (s16 <= s kp)
+        const int64_t rb = (n + kBlock - 1) / kBlock;
+        int64_t s16 = (768 + rb - 1) / rb;
+        s16 = std::max<int64_t>(1, std::min<int64_t>({s16, ngroups, (int64_t)s * kp}));
+        const int64_t g16 = (ngroups + s16 - 1) / s16;
+        const int n16 = (int)((ngroups + g16 - 1) / g16);
+        k_gemv16_part<<<dim3((unsigned)rb, (unsigned)n16), kBlock, 0, st>>>((const uint4*)A16, n, (const uint4*)P,
+                                                                            kp, ngroups, g16, part, skip);
+        k_gemv16_reduce<<<nblocks(n), kBlock, 0, st>>>(part, n16, n, Y, skip);
+        return hipGetLastError();
+    }
     dim3 grid((unsigned)((tiles + 3) / 4), (unsigned)ns);
     if (nt == 1)
         k_bmm16<1><<<grid, kBlock, 0, st>>>((const bf8v*)A16, n, P, ngroups, gper, part, skip);
     else
         k_bmm16<2><<<grid, kBlock, 0, st>>>((const bf8v*)A16, n, P, ngroups, gper, part, skip);
-    const unsigned gr = (unsigned)((n + 63) / 64);
+    const unsigned gr = (unsigned)((n + 31) / 32);
     if (kp == 32)
         k_bmm32_reduce<32><<<gr, kBlock, 0, st>>>(part, ns, n, nrhs, Y, ldy, skip);
     else

@@ -1213,21 +1213,36 @@ __global__ __launch_bounds__(kRHGBlock) void k_robin_hood_grid(const double* __r
                 }
                 __builtin_amdgcn_s_sleep(1);
             }
-            if (!s_stop) {
-                // the global first argmax over the slots of this step
-                const double* sl = slots + (s & 1) * (int64_t)G * 3;
-                double gv = rh_load(sl);
-                int64_t gx = (int64_t)rh_load(sl + 1);
-                double gr = rh_load(sl + 2);
-                for (int k = 1; k < G; k++) {
-                    const double v = rh_load(sl + 3 * k);
-                    const int64_t ix = (int64_t)rh_load(sl + 3 * k + 1);
-                    if (v > gv || (v == gv && ix < gx)) {
-                        gv = v;
-                        gx = ix;
-                        gr = rh_load(sl + 3 * k + 2);
-                    }
+        }
+        __syncthreads();
+        if (s_stop) break;
+        if (w == 0) {
+            // the global first argmax over the slots of this step: lane k reads slots k, k + 64,
+            // ... (loads in flight together), then the wavefront reduces with first-index ties
+            const double* sl = slots + (s & 1) * (int64_t)G * 3;
+            double gv = -1., gr = 0.;
+            int64_t gx = n;
+            for (int k = lane; k < G; k += 64) {
+                const double v = rh_load(sl + 3 * k);
+                const int64_t ix = (int64_t)rh_load(sl + 3 * k + 1);
+                const double rr = rh_load(sl + 3 * k + 2);
+                if (v > gv || (v == gv && ix < gx)) {
+                    gv = v;
+                    gx = ix;
+                    gr = rr;
                 }
+            }
+            for (int o = 32; o > 0; o >>= 1) {
+                const double ov = __shfl_down(gv, o, 64);
+                const int64_t oi = __shfl_down(gx, o, 64);
+                const double orr = __shfl_down(gr, o, 64);
+                if (ov > gv || (ov == gv && oi < gx)) {
+                    gv = ov;
+                    gx = oi;
+                    gr = orr;
+                }
+            }
+            if (lane == 0) {
                 // no workgroup found a residual > -1: every residual is NaN, the reference
                 // corrects index 0 by its (NaN) residual
                 if (gx >= n) {
@@ -1242,7 +1257,6 @@ __global__ __launch_bounds__(kRHGBlock) void k_robin_hood_grid(const double* __r
             }
         }
         __syncthreads();
-        if (s_stop) break;
         const double corr = s_corr;
         const double* col = A + s_idx * n + r0;
         for (int64_t i = threadIdx.x; i < nr; i += kRHGBlock) lbi[i] += col[i] * corr;
