@@ -639,8 +639,8 @@ def run_coax(ctx, args):
     for mat, prec, key, bpe in ((A, "mixed", "fp32", 4), (Ab, "bf16", "bf16", 2)):
         if not args.no_autotune:
             # the split-K target (and for FP32 the wavefront tile) tuned per matrix precision
-            tb = T.tune_block_product(mat, nrhs=64, precision=prec, candidates=(1280, 2560, 5120, 10240, 20480))
-            out[f"coax20k_bmm64_tuned_{key}"] = {"bmm_blocks": tb["best"], "bmm_rows": tb["rows"]["best"]}
+            tb = T.tune_block_product(mat, nrhs=64, precision=prec)
+            out[f"coax20k_bmm64_tuned_{key}"] = {tb["knob"]: tb["best"]}
         mat.MultiplyBatch(Xr, prec)
         _timing(ctx, True)
         for _ in range(5):

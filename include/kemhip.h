@@ -114,8 +114,8 @@ int kem_ctx_set_mixed_fp64_ratio(kem_ctx* ctx, double ratio);
    or more points of one charge-density set; the elements split into 8 R ranges, R per XCD, so
    each XCD's range stays in its L2 (0: no split; default 2560).  KEM_TUNE_GEMV_BLOCKS /
    KEM_TUNE_BMM_BLOCKS = target block counts of the split-K matrix-vector product (default 2048)
-   and of the multi-RHS block product (default 2560); they set how many partial sums are
-   reduced. */
+   and of the multi-RHS block product in 64-row tiles times column splits (default 2560); they
+   set how many partial sums are reduced. */
 #define KEM_TUNE_FAR_RANGE_FP64 1
 #define KEM_TUNE_FAR_RANGE_MIXED 2
 #define KEM_TUNE_GEMV_BLOCKS 3
@@ -128,6 +128,10 @@ int kem_ctx_set_mixed_fp64_ratio(kem_ctx* ctx, double ratio);
 /* Robin Hood sub-iterations: 0 = one workgroup up to 8,192 unknowns and several co-resident
    workgroups above (default), 1 = always one workgroup (the same corrections either way) */
 #define KEM_TUNE_ROBIN_HOOD_GROUPS 7
+/* target workgroup count of the block products of 33-64 right-hand sides that stream the
+   matrix through LDS (FP32 matrices with n % 4 == 0, and BF16): 256 rows per workgroup times
+   column splits; default 480 */
+#define KEM_TUNE_BMM_STREAM_BLOCKS 8
 int kem_ctx_set_tuning(kem_ctx* ctx, int param, int64_t value);
 int kem_ctx_get_tuning(const kem_ctx* ctx, int param, int64_t* value);
 /* Instrumentation, no reference counterpart: with timing on, each field evaluation records HIP

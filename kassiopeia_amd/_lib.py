@@ -18,6 +18,7 @@ KEM_TUNE_FAR_RANGE_FP64, KEM_TUNE_FAR_RANGE_MIXED, KEM_TUNE_GEMV_BLOCKS, KEM_TUN
 KEM_TUNE_NEAR_BLOCKS = 5
 KEM_TUNE_BMM_ROWS = 6
 KEM_TUNE_ROBIN_HOOD_GROUPS = 7
+KEM_TUNE_BMM_STREAM_BLOCKS = 8
 
 
 class KemError(RuntimeError):

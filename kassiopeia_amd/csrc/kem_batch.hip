@@ -341,6 +341,192 @@ __global__ __launch_bounds__(kBlock) void k_bmm16(const bf8v* __restrict__ A16, 
         }
 }
 
+// ------------------------------------------------------------------ block products streamed through LDS
+// 64 right-hand sides.  A workgroup takes 256 rows (wavefront w: rows 64 w .. 64 w + 63, the
+// 64 x 64 tile of the kernels above, same accumulator layout) and a range of columns.  The
+// matrix and the panel stream in chunks into three LDS buffers by direct-to-LDS loads
+// (global_load_lds_dwordx4: lane l writes its 16 bytes at the buffer base + 16 l), two chunks
+// ahead of the MFMAs.  Per chunk a wavefront waits for its own loads of this chunk (those of
+// the next chunk stay in flight), meets the others at a bare barrier (a fence would also wait
+// for the loads in flight), reads its operands into registers, issues the loads two chunks
+// ahead into the buffer every wavefront finished reading one chunk ago, then runs the MFMAs.
+// Loads into LDS hold no registers, so two chunks are in flight at any occupancy.  Measured on
+// a 20,000-unknown matrix: BF16 0.148 ms at 3 column splits (the register-loaded k_bmm16: 0.150
+// ms at 9 splits, three times the partial sums), FP32 0.569 ms at 6 splits (k_bmm32: 0.61-0.65).
+
+// s_waitcnt with vmcnt = N and the other counters left open (gfx9 encoding)
+template <int N>
+__device__ __forceinline__ void wait_vmcnt()
+{
+    static_assert(N >= 0 && N < 64, "vmcnt");
+    __builtin_amdgcn_s_waitcnt((N & 0xF) | ((N >> 4) << 14) | (0x7 << 4) | (0xF << 8));
+}
+
+constexpr int kStreamRows = 256;  // rows per workgroup
+constexpr int kStreamKC = 16;     // FP32: matrix columns per chunk (3 buffers: 60 KB of LDS)
+constexpr int kStreamG = 4;       // BF16: column groups of 8 per chunk (3 buffers: 60 KB)
+
+// FP32 matrix (n % 4 == 0, A and Vt 16-byte aligned), KP = 64.  Chunk: KC columns x 256 rows
+// of A (one 1 KB load per column: lane l takes rows 4l .. 4l + 3) and KC panel rows (4 per 1 KB
+// load).  Splits start at multiples of KC columns.
+__global__ __launch_bounds__(kBlock) void k_bmm32s(const float* __restrict__ A, int64_t n,
+                                                   const float* __restrict__ Vt, int64_t per_split,
+                                                   float* __restrict__ part, const int* __restrict__ skip)
+{
+    if (skip && *skip) return;
+    constexpr int KP = 64, KC = kStreamKC, CH = KC * kStreamRows + KC * KP, NW = KC / 4 + KC / 16;
+    static_assert(KC % 16 == 0, "one panel load per wavefront and chunk");
+    __shared__ float lds[3 * CH];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lr = lane & 31, lk = lane >> 5;
+    const int64_t r0 = (int64_t)blockIdx.x * kStreamRows;
+    const int64_t j0 = (int64_t)blockIdx.y * per_split;
+    const int64_t j1 = std::min<int64_t>(n, j0 + per_split);
+    if (j0 >= j1) return;
+    const int64_t nch = (j1 - j0 + KC - 1) / KC;
+    // this lane's 4 rows of a column (rows past n read the last 4 rows; never stored)
+    const int64_t rr = std::min<int64_t>(r0 + 4 * lane, n - 4);
+    auto issue = [&](int64_t k) {
+        float* buf = lds + (k % 3) * CH;
+        const int64_t jc = j0 + k * KC;
+        for (int q = 0; q < KC / 4; q++) {
+            const int c = wave + 4 * q;
+            const int64_t col = std::min<int64_t>(jc + c, j1 - 1);  // past j1: a valid column, unused
+            __builtin_amdgcn_global_load_lds(A + col * n + rr, buf + c * kStreamRows, 16, 0, 0);
+        }
+        for (int q = 0; q < KC / 16; q++) {
+            // panel rows jc + 16 q + 4 wave .. + 3; jc and n are multiples of 4, so a group of
+            // rows is either below n or wholly past it (then clamped, unused)
+            const int rb = 16 * q + 4 * wave;
+            const int64_t row = std::min<int64_t>(jc + rb, n - 4);
+            __builtin_amdgcn_global_load_lds(Vt + row * KP + lane * 4, buf + KC * kStreamRows + rb * KP, 16, 0, 0);
+        }
+    };
+    f16v acc0[2], acc1[2];
+    for (int t = 0; t < 2; t++)
+        for (int v = 0; v < 16; v++) {
+            acc0[t][v] = 0.f;
+            acc1[t][v] = 0.f;
+        }
+    issue(0);
+    if (nch > 1) issue(1);
+    for (int64_t k = 0; k < nch; k++) {
+        if (k + 1 < nch)
+            wait_vmcnt<NW>();
+        else
+            wait_vmcnt<0>();
+        __builtin_amdgcn_s_barrier();
+        asm volatile("" ::: "memory");
+        const float* buf = lds + (k % 3) * CH;
+        const int kv = (int)std::min<int64_t>(KC, j1 - (j0 + k * KC));  // valid columns
+        float a0[KC / 2], a1[KC / 2], b[KC / 2][2];
+        for (int p = 0; p < KC / 2; p++) {
+            const int col = 2 * p + lk;
+            a0[p] = buf[col * kStreamRows + wave * 64 + lr];
+            a1[p] = buf[col * kStreamRows + wave * 64 + 32 + lr];
+            b[p][0] = buf[KC * kStreamRows + col * KP + lr];
+            b[p][1] = buf[KC * kStreamRows + col * KP + 32 + lr];
+            if (col >= kv) a0[p] = a1[p] = 0.f;
+        }
+        if (k + 2 < nch) issue(k + 2);
+        for (int p = 0; p < KC / 2; p++)
+            for (int t = 0; t < 2; t++) {
+                acc0[t] = __builtin_amdgcn_mfma_f32_32x32x2f32(a0[p], b[p][t], acc0[t], 0, 0, 0);
+                acc1[t] = __builtin_amdgcn_mfma_f32_32x32x2f32(a1[p], b[p][t], acc1[t], 0, 0, 0);
+            }
+    }
+    const int64_t i0 = r0 + wave * 64;
+    if (i0 >= n) return;
+    float* out = part + (int64_t)blockIdx.y * n * KP + (i0 + 4 * lk) * KP + lr;
+    const int64_t lim = n - i0 - 4 * lk;
+    for (int t = 0; t < 2; t++)
+        for (int v = 0; v < 16; v++) {
+            const int rr2 = 8 * (v >> 2) + (v & 3);
+            if (rr2 < lim) out[rr2 * KP + 32 * t] = acc0[t][v];
+            if (rr2 + 32 < lim) out[(rr2 + 32) * KP + 32 * t] = acc1[t][v];
+        }
+}
+
+// BF16 matrix, KP = 64.  Chunk: G column groups x 256 rows of A16 (one 1 KB load per group and
+// wavefront: lane l takes row 64 w + l) and the G groups of the panel (one 1 KB load per
+// group); the panel is read from LDS by all four wavefronts.
+__global__ __launch_bounds__(kBlock) void k_bmm16s(const bf8v* __restrict__ A16, int64_t n,
+                                                   const bf8v* __restrict__ Vp, int64_t ngroups, int64_t gper,
+                                                   float* __restrict__ part, const int* __restrict__ skip)
+{
+    if (skip && *skip) return;
+    constexpr int KP = 64, G = kStreamG, CH = G * kStreamRows + G * KP, NW = G + G / 4;
+    static_assert(G % 4 == 0, "panel loads spread over the four wavefronts");
+    __shared__ bf8v lds[3 * CH];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lr = lane & 31, lk = lane >> 5;
+    const int64_t r0 = (int64_t)blockIdx.x * kStreamRows;
+    const int64_t g0 = (int64_t)blockIdx.y * gper;
+    const int64_t g1 = std::min<int64_t>(ngroups, g0 + gper);
+    if (g0 >= g1) return;
+    const int64_t nch = (g1 - g0 + G - 1) / G;
+    const int64_t row = std::min<int64_t>(r0 + 64 * wave + lane, n - 1);  // past n: unused
+    auto issue = [&](int64_t k) {
+        bf8v* buf = lds + (k % 3) * CH;
+        const int64_t gc = g0 + k * G;
+        for (int q = 0; q < G; q++) {
+            const int64_t g = std::min<int64_t>(gc + q, g1 - 1);  // past g1: a valid group, unused
+            __builtin_amdgcn_global_load_lds(A16 + g * n + row, buf + q * kStreamRows + 64 * wave, 16, 0, 0);
+        }
+        for (int q = wave; q < G; q += 4) {
+            const int64_t g = std::min<int64_t>(gc + q, g1 - 1);
+            __builtin_amdgcn_global_load_lds(Vp + g * KP + lane, buf + G * kStreamRows + q * KP, 16, 0, 0);
+        }
+    };
+    f16v acc0[2], acc1[2];
+    for (int t = 0; t < 2; t++)
+        for (int v = 0; v < 16; v++) {
+            acc0[t][v] = 0.f;
+            acc1[t][v] = 0.f;
+        }
+    issue(0);
+    if (nch > 1) issue(1);
+    for (int64_t k = 0; k < nch; k++) {
+        if (k + 1 < nch)
+            wait_vmcnt<NW>();
+        else
+            wait_vmcnt<0>();
+        __builtin_amdgcn_s_barrier();
+        asm volatile("" ::: "memory");
+        const bf8v* buf = lds + (k % 3) * CH;
+        const int kv = (int)std::min<int64_t>(G, g1 - (g0 + k * G));  // valid groups
+        bf8v a0[G / 2], a1[G / 2], b[G / 2][2];
+        for (int p = 0; p < G / 2; p++) {
+            const int q = 2 * p + lk;
+            a0[p] = buf[q * kStreamRows + wave * 64 + lr];
+            a1[p] = buf[q * kStreamRows + wave * 64 + 32 + lr];
+            b[p][0] = buf[G * kStreamRows + q * KP + lr];
+            b[p][1] = buf[G * kStreamRows + q * KP + 32 + lr];
+            if (q >= kv)
+                for (int m = 0; m < 8; m++) {
+                    a0[p][m] = (__bf16)0.f;
+                    a1[p][m] = (__bf16)0.f;
+                }
+        }
+        if (k + 2 < nch) issue(k + 2);
+        for (int p = 0; p < G / 2; p++)
+            for (int t = 0; t < 2; t++) {
+                acc0[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0[p], b[p][t], acc0[t], 0, 0, 0);
+                acc1[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1[p], b[p][t], acc1[t], 0, 0, 0);
+            }
+    }
+    const int64_t i0 = r0 + wave * 64;
+    if (i0 >= n) return;
+    float* out = part + (int64_t)blockIdx.y * n * KP + (i0 + 4 * lk) * KP + lr;
+    const int64_t lim = n - i0 - 4 * lk;
+    for (int t = 0; t < 2; t++)
+        for (int v = 0; v < 16; v++) {
+            const int rr2 = 8 * (v >> 2) + (v & 3);
+            if (rr2 < lim) out[rr2 * KP + 32 * t] = acc0[t][v];
+            if (rr2 + 32 < lim) out[(rr2 + 32) * KP + 32 * t] = acc1[t][v];
+        }
+}
+
 // One right-hand side on the BF16 matrix: part[s][i] = sum_{g in split s} sum_m A16(i, 8g + m) *
 // x(8g + m) on the vector ALU (the block product would pad one column to 32 MFMA columns).
 // Lane = row i: one 16-byte load per column group holds the row's 8 BF16 entries; the group's
@@ -674,6 +860,16 @@ __global__ __launch_bounds__(64) void k_btrisolve(const double* __restrict__ H, 
     cnt[r] = m;
 }
 
+// column splits of the LDS-streamed products: the workgroup count of 256 rows x splits nearest
+// bmm_stream_blocks (the measured optima sit at 3 and 6 splits of a 79-workgroup matrix)
+int bmm_stream_splits(int64_t n)
+{
+    const int64_t rb = (n + 255) / 256;
+    int64_t s = (launch_tuning().bmm_stream_blocks + rb / 2) / rb;
+    s = std::min<int64_t>(s, std::max<int64_t>(1, n / 256));
+    return (int)std::max<int64_t>(1, std::min<int64_t>(s, 64));
+}
+
 int bmm_splits(int64_t n, int64_t tiles)
 {
     int64_t s = (launch_tuning().bmm_blocks + tiles - 1) / std::max<int64_t>(tiles, 1);
@@ -688,12 +884,12 @@ size_t bmm_workspace_bytes(int64_t n, int nrhs, int prec)
     const int kp = nrhs <= 32 ? 32 : 64;
     if (prec == kBmmBf16) {
         const int64_t tiles = (n + 63) / 64, ngroups = (n + 7) / 8;
-        const int s = bmm_splits(n, tiles);
+        const int s = std::max(bmm_splits(n, tiles), kp == 64 ? bmm_stream_splits(n) : 1);
         return sizeof(float) * (size_t)s * n * kp + sizeof(uint16_t) * (size_t)ngroups * 8 * kp + 512;
     }
     if (prec == kBmmFp32) {
         const int64_t tiles = (n + 63) / 64;
-        const int s = bmm_splits(n, tiles);
+        const int s = std::max(bmm_splits(n, tiles), kp == 64 ? bmm_stream_splits(n) : 1);
         return sizeof(float) * ((size_t)s * n * kp + (size_t)n * kp) + 256;
     }
     const int kb = nrhs <= 16 ? 16 : 64;
@@ -715,10 +911,23 @@ hipError_t launch_bmm(hipStream_t st, const float* A32, const double* A64, int64
         per += per & 1;
         const int ns = (int)((n + per - 1) / per);
         float* part = (float*)ws;
-        float* Vt = part + (size_t)s * n * kp;
+        // the panel after the partial sums of either kernel
+        const int sw = std::max(s, nt == 2 ? bmm_stream_splits(n) : 1);
+        float* Vt = part + (size_t)sw * n * kp;
         const float* P = x_is_panel ? (const float*)X : Vt;
         if (!x_is_panel)
             k_panel<float><<<nblocks(n * kp), kBlock, 0, st>>>(X, ldx, n, nrhs, kp, Vt);
+        if (nt == 2 && n % 4 == 0 && (uintptr_t)A32 % 16 == 0 && (uintptr_t)P % 16 == 0) {
+            // streamed through LDS; splits start at multiples of the chunk width
+            const int ss = bmm_stream_splits(n);
+            int64_t pc = (n + ss - 1) / ss;
+            pc = (pc + kStreamKC - 1) / kStreamKC * kStreamKC;
+            const int nc = (int)((n + pc - 1) / pc);
+            dim3 grid((unsigned)((n + kStreamRows - 1) / kStreamRows), (unsigned)nc);
+            k_bmm32s<<<grid, kBlock, 0, st>>>(A32, n, P, pc, part, skip);
+            k_bmm32_reduce<64><<<(unsigned)((n + 31) / 32), kBlock, 0, st>>>(part, nc, n, nrhs, Y, ldy, skip);
+            return hipGetLastError();
+        }
         if (launch_tuning().bmm_rows == 128 && n % 4 == 0 && n >= 4) {
             // 128-row wavefront tiles, 512 rows per block
             dim3 grid((unsigned)((n + 511) / 512), (unsigned)ns);
@@ -784,8 +993,9 @@ hipError_t launch_bmm16(hipStream_t st, const uint16_t* A16, int64_t n, const vo
     const int64_t gper = (ngroups + s - 1) / s;
     const int ns = (int)((ngroups + gper - 1) / gper);
     float* part = (float*)ws;
-    // the panel after the partial sums, 256-byte aligned
-    uintptr_t pv = (uintptr_t)(part + (size_t)s * n * kp);
+    // the panel after the partial sums of either kernel, 256-byte aligned
+    const int sw = std::max(s, nt == 2 ? bmm_stream_splits(n) : 1);
+    uintptr_t pv = (uintptr_t)(part + (size_t)sw * n * kp);
     pv = (pv + 255) & ~(uintptr_t)255;
     uint4* Vp = (uint4*)pv;
     const bf8v* P = x_is_panel ? (const bf8v*)X : (const bf8v*)Vp;
@@ -805,11 +1015,18 @@ hipError_t launch_bmm16(hipStream_t st, const uint16_t* A16, int64_t n, const vo
         k_gemv16_reduce<<<nblocks(n), kBlock, 0, st>>>(part, n16, n, Y, skip);
         return hipGetLastError();
     }
-    dim3 grid((unsigned)((tiles + 3) / 4), (unsigned)ns);
-    if (nt == 1)
+    if (nt == 2) {
+        const int ss = bmm_stream_splits(n);
+        const int64_t gs = (ngroups + ss - 1) / ss;
+        const int nss = (int)((ngroups + gs - 1) / gs);
+        dim3 grid((unsigned)((n + kStreamRows - 1) / kStreamRows), (unsigned)nss);
+        k_bmm16s<<<grid, kBlock, 0, st>>>((const bf8v*)A16, n, P, ngroups, gs, part, skip);
+        k_bmm32_reduce<64><<<(unsigned)((n + 31) / 32), kBlock, 0, st>>>(part, nss, n, nrhs, Y, ldy, skip);
+        return hipGetLastError();
+    } else {
+        dim3 grid((unsigned)((tiles + 3) / 4), (unsigned)ns);
         k_bmm16<1><<<grid, kBlock, 0, st>>>((const bf8v*)A16, n, P, ngroups, gper, part, skip);
-    else
-        k_bmm16<2><<<grid, kBlock, 0, st>>>((const bf8v*)A16, n, P, ngroups, gper, part, skip);
+    }
     const unsigned gr = (unsigned)((n + 31) / 32);
     if (kp == 32)
         k_bmm32_reduce<32><<<gr, kBlock, 0, st>>>(part, ns, n, nrhs, Y, ldy, skip);

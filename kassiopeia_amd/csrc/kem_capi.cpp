@@ -383,6 +383,10 @@ int kem_ctx_set_tuning(kem_ctx* c, int param, int64_t value)
         if (value != 0 && value != 1) return fail(c, KEM_E_ARG, "tuning: Robin Hood workgroups 0 (by size) or 1");
         c->tuning.rh_groups = value;
         return KEM_OK;
+    case KEM_TUNE_BMM_STREAM_BLOCKS:
+        if (value < 1 || value > (1 << 20)) return fail(c, KEM_E_ARG, "tuning: block count out of [1, 2^20]");
+        c->tuning.bmm_stream_blocks = value;
+        return KEM_OK;
     default:
         return fail(c, KEM_E_ARG, "tuning: unknown parameter");
     }
@@ -399,6 +403,7 @@ int kem_ctx_get_tuning(const kem_ctx* c, int param, int64_t* value)
     case KEM_TUNE_NEAR_BLOCKS: *value = c->tuning.near_blocks; return KEM_OK;
     case KEM_TUNE_BMM_ROWS: *value = c->tuning.bmm_rows; return KEM_OK;
     case KEM_TUNE_ROBIN_HOOD_GROUPS: *value = c->tuning.rh_groups; return KEM_OK;
+    case KEM_TUNE_BMM_STREAM_BLOCKS: *value = c->tuning.bmm_stream_blocks; return KEM_OK;
     default: return KEM_E_ARG;
     }
 }

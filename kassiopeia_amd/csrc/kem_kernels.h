@@ -116,10 +116,12 @@ hipError_t launch_expand_groups(hipStream_t st, const int32_t* gof, int64_t nm, 
                                 const double* sigma, double* out);
 
 // Launch shapes of the calling thread, set from the context at every C-ABI entry point:
-// target block counts of the split-K matrix-vector product and of the FP32 / FP64 block product.
+// target block counts of the split-K matrix-vector product and of the FP32 / FP64 block product
+// (register-loaded kernels in 64-row tiles; the LDS-streamed ones in 256-row workgroups).
 struct LaunchTuning {
     int64_t gemv_blocks = 2048;
     int64_t bmm_blocks = 2560;
+    int64_t bmm_stream_blocks = 480;  // workgroups of the LDS-streamed 64-RHS block products
     int64_t near_blocks = 384;  // grid of the near pass beside the mixed far pass (0: one block per point block)
     int64_t bmm_rows = 64;      // rows per wavefront tile of the FP32 block product (64, or 128 when n % 4 == 0)
     int64_t rh_groups = 0;      // workgroups of the Robin Hood sub-iterations (0: by size, 1: one)

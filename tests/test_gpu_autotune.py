@@ -60,9 +60,14 @@ def test_tune_products(setup):
         assert np.linalg.norm(A.Multiply(v) - ref) <= 1e-14 * np.linalg.norm(ref)
     X = np.random.default_rng(3).normal(size=(A.n, 40))
     ref = A.MultiplyBatch(X, "mixed")
-    r = T.tune_block_product(A, nrhs=40, repeats=1, candidates=(320, 2560))
-    assert T.get(ctx, "bmm_blocks") == r["best"] and T.get(ctx, "bmm_rows") == r["rows"]["best"]
-    assert set(r["rows"]["ms"]) == {64, 128}
+    # 40 right-hand sides on this FP32 matrix (n % 4 == 0) stream through LDS: their workgroup
+    # target is the knob swept; 7 take the register-loaded kernels (tile and split target)
+    r = T.tune_block_product(A, nrhs=40, repeats=1, candidates=(60, 480))
+    assert r["knob"] == "bmm_stream_blocks" and T.get(ctx, "bmm_stream_blocks") == r["best"]
+    assert set(r["ms"]) == {60, 480}
+    r = T.tune_block_product(A, nrhs=7, repeats=1, candidates=(320, 2560))
+    assert r["knob"] == "bmm_blocks" and T.get(ctx, "bmm_blocks") == r["best"]
+    assert T.get(ctx, "bmm_rows") == r["rows"]["best"] and set(r["rows"]["ms"]) == {64, 128}
     # both wavefront tiles, every split target: the product of the FP32 matrix to FP32 accuracy
     for rows in (64, 128):
         T.set_knob(ctx, "bmm_rows", rows)
@@ -71,8 +76,9 @@ def test_tune_products(setup):
             assert np.linalg.norm(R - ref[:, :nrhs]) <= 1e-6 * np.linalg.norm(ref[:, :nrhs]), (rows, nrhs)
         for c in (64, 10240):
             T.set_knob(ctx, "bmm_blocks", c)
+            T.set_knob(ctx, "bmm_stream_blocks", c)
             assert np.linalg.norm(A.MultiplyBatch(X, "mixed") - ref) <= 1e-6 * np.linalg.norm(ref), (rows, c)
-    T.apply(ctx, {"gemv_blocks": 2048, "bmm_blocks": 2560, "bmm_rows": 64})
+    T.apply(ctx, {"gemv_blocks": 2048, "bmm_blocks": 2560, "bmm_rows": 64, "bmm_stream_blocks": 480})
 
 
 def test_tuning_table_roundtrip_and_errors(setup, tmp_path):
@@ -81,11 +87,12 @@ def test_tuning_table_roundtrip_and_errors(setup, tmp_path):
 
     ctx = setup[0]
     table = {"far_range_fp64": 1280, "far_range_mixed": {"best": 5120, "ms": {}}, "gemv_blocks": 1024,
-             "bmm_blocks": 2560, "near_blocks": 256, "bmm_rows": 128, "robin_hood_groups": 1}
+             "bmm_blocks": 2560, "near_blocks": 256, "bmm_rows": 128, "robin_hood_groups": 1,
+             "bmm_stream_blocks": 240}
     T.save(table, tmp_path / "t.json")
     loaded = T.load(tmp_path / "t.json")
     assert loaded == {"far_range_fp64": 1280, "far_range_mixed": 5120, "gemv_blocks": 1024, "bmm_blocks": 2560,
-                      "near_blocks": 256, "bmm_rows": 128, "robin_hood_groups": 1}
+                      "near_blocks": 256, "bmm_rows": 128, "robin_hood_groups": 1, "bmm_stream_blocks": 240}
     T.apply(ctx, loaded)
     assert T.current(ctx) == loaded
     with pytest.raises(_lib.KemError):
@@ -98,8 +105,10 @@ def test_tuning_table_roundtrip_and_errors(setup, tmp_path):
         T.set_knob(ctx, "bmm_rows", 96)
     with pytest.raises(_lib.KemError):
         T.set_knob(ctx, "robin_hood_groups", 2)
+    with pytest.raises(_lib.KemError):
+        T.set_knob(ctx, "bmm_stream_blocks", 0)
     T.apply(ctx, {"far_range_fp64": 2560, "far_range_mixed": 2560, "gemv_blocks": 2048, "near_blocks": 384,
-                  "bmm_rows": 64, "robin_hood_groups": 0})
+                  "bmm_rows": 64, "robin_hood_groups": 0, "bmm_stream_blocks": 480})
 
 
 def test_tune_near(setup):

@@ -45,6 +45,59 @@ def test_block_product_fp64(coax, k):
         assert np.abs(Y[:, 0] - A.Multiply(X[:, 0])).max() / np.abs(ref).max() < 1e-13
 
 
+def _first_elements(m, n):
+    return G.Mesh(m.shapes[:n].copy(), m.recs[:n].copy(), m.bc_type[:n].copy(), m.bc_value[:n].copy(),
+                  m.bc_flux[:n].copy(), m.group[:n].copy())
+
+
+@pytest.mark.parametrize("n", [1352, 1349])
+def test_block_product_streamed_shapes(ctx, n):
+    """33-64 right-hand sides: FP32 matrices with n % 4 == 0 and BF16 matrices stream through
+    LDS (256-row workgroups; chunks of 16 FP32 columns or 4 BF16 column groups), FP32 with
+    n % 4 != 0 loads into registers.  n = 1,352 leaves 72 rows past the last full workgroup and
+    partial last chunks; the workgroup target 1 gives one column split, 480 five, 4,096 the
+    most (one per 256 columns).  Exact check: unit right-hand sides e_j pick columns j of the
+    stored matrix bit for bit (one nonzero term, products by 1 and sums with 0 exact) at 64
+    columns spread over every chunk and split, first and last included.  Random right-hand
+    sides against the FP64 product of the rounded operands, within the recursive-summation
+    bound (n + 2) 2^-24 of the sum of |terms|."""
+    from kassiopeia_amd import autotune as T
+    from kassiopeia_amd import kemfield as K
+
+    m = _first_elements(G.coax_mesh(nphi_in=12, nphi_out=24, nz=16, ncap_r=2, ncap_phi=25), n)
+    cont = K.KSurfaceContainer(ctx, m)
+    A = K.KBoundaryIntegralMatrix(cont, fp64=True, fp32=True, bf16=True)
+    Ah = A.to_numpy()
+
+    def rnd16(a):
+        u = np.asarray(a, np.float32).view(np.uint32).astype(np.uint64)
+        u = (u + 0x7FFF + ((u >> 16) & 1)) >> 16 << 16
+        return u.astype(np.uint32).view(np.float32).astype(np.float64)
+
+    rounded = {"mixed": Ah.astype(np.float32).astype(np.float64), "bf16": rnd16(Ah.astype(np.float32))}
+    cols = np.linspace(0, n - 1, 64).astype(int)
+    E = np.zeros((n, 64))
+    E[cols, np.arange(64)] = 1.0
+    X = np.random.default_rng(n).normal(size=(n, 64))
+    Xr = {"mixed": X.astype(np.float32).astype(np.float64), "bf16": rnd16(X)}
+    bound = (n + 2) * 2.0 ** -24
+    try:
+        for blocks in (1, 480, 4096):
+            T.set_knob(ctx, "bmm_stream_blocks", blocks)
+            for k in (33, 64):
+                for prec in ("mixed", "bf16"):
+                    print(f"streamed product: n {n} blocks {blocks} k {k} {prec}", flush=True)
+                    Y = A.MultiplyBatch(E[:, :k], prec)
+                    assert np.array_equal(Y, rounded[prec][:, cols[:k]]), (prec, blocks, k)
+                    Y = A.MultiplyBatch(X[:, :k], prec)
+                    Ar, Xk = rounded[prec], Xr[prec][:, :k]
+                    assert np.all(np.abs(Y - Ar @ Xk) <= bound * (np.abs(Ar) @ np.abs(Xk))), (prec, blocks, k)
+    finally:
+        T.set_knob(ctx, "bmm_stream_blocks", 480)
+        A.close()
+        cont.close()
+
+
 @pytest.mark.parametrize("k", [1, 7, 32, 33, 64, 70])
 def test_block_product_fp32_mfma(coax, k):
     """FP32 MFMA product against the FP64 product of the FP32-rounded operands: FP32 products

@@ -23,7 +23,7 @@ def setup():
     from kassiopeia_amd import kemfield as K
 
     ctx = K.Context(0)
-    m = G.coax_mesh(nphi_in=12, nphi_out=24, nz=16, ncap_r=2, ncap_phi=17)  # 1,352 triangles
+    m = G.coax_mesh(nphi_in=12, nphi_out=24, nz=16, ncap_r=2, ncap_phi=17)  # 1,288 triangles
     cont = K.KSurfaceContainer(ctx, m)
     A = K.KBoundaryIntegralMatrix(cont, fp64=True, fp32=True, bf16=True)
     yield ctx, m, cont, A
