@@ -93,6 +93,9 @@ def main():
                          "steps x, for profiling")
     ap.add_argument("--far-range", type=int, default=-1,
                     help="--field-only: elements per far-pass element range (-1: the library's default)")
+    ap.add_argument("--block-products", action="store_true",
+                    help="only the block products of 64 right-hand sides on the 20,000-triangle coax matrix "
+                         "(FP32 and BF16 storage), steps x each, for profiling")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 1 GiB device to device (reference kernel for the rocprofv3 byte counters)")
     args = ap.parse_args()
@@ -107,6 +110,8 @@ def main():
     ctx = K.Context(0)
     if args.field_only:
         return field_only(ctx, args)
+    if args.block_products:
+        return block_products(ctx, args)
     t_start = time.perf_counter()
     warm_up(ctx)
     line = run_bore(ctx, args)
@@ -115,6 +120,32 @@ def main():
     line["bench_wall_s"] = round(time.perf_counter() - t_start, 1)
     ctx.close()
     print(json.dumps(line), flush=True)
+
+
+def block_products(ctx, args):
+    """steps block products of 64 right-hand sides on the 20,000-triangle coax matrix, FP32 then
+    BF16 storage, at the library's default launch shapes: the command the rocprofv3 summaries of
+    the streamed block-product kernels run."""
+    from kassiopeia_amd import geometry as G
+    from kassiopeia_amd import kemfield as K
+
+    mesh = G.coax_mesh()
+    cont = K.KSurfaceContainer(ctx, mesh)
+    A = K.KBoundaryIntegralMatrix(cont, fp64=False, fp32=True, bf16=True)
+    X = np.random.default_rng(9).normal(size=(mesh.n, 64))
+    out = {"block_products": mesh.n, "nrhs": 64}
+    for prec in ("mixed", "bf16"):
+        A.MultiplyBatch(X, prec)
+        _timing(ctx, True)
+        for _ in range(args.steps):
+            A.MultiplyBatch(X, prec)
+        kt, kn = _kernel_time(ctx)
+        _timing(ctx, False)
+        out[prec + "_ms"] = kt / max(kn, 1)
+    A.close()
+    cont.close()
+    ctx.close()
+    print(json.dumps(out), flush=True)
 
 
 def warm_up(ctx):

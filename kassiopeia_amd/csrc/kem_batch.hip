@@ -175,19 +175,22 @@ __global__ __launch_bounds__(kBlock) void k_bmm32q(const float* __restrict__ A, 
     }
 }
 
+constexpr int kReduceRows = 16;  // rows per block of k_bmm32_reduce
+
 // W[c*ldw + i] = sum_s part[s][i][c] for c < ncols, splits summed in order in FP64.  A block
-// takes 32 rows: entry e = threadIdx.x + 256 m (row e / KP, column e % KP) is contiguous in
-// e for each split, so every split's read is coalesced; a thread loads four splits of its
-// 32 KP / 256 entries before adding them (the loads in flight together: 9 splits of a
-// 20,000 x 64 block reduce in 25 us with one split per round trip); the sums are transposed
-// through LDS so that the writes run along rows.
+// takes kReduceRows rows: entry e = threadIdx.x + 256 m (row e / KP, column e % KP) is contiguous
+// in e for each split, so every split's read is coalesced; a thread loads up to eight splits of
+// its kReduceRows KP / 256 entries before adding them, so a reduce is about one round trip to
+// memory (one split per round trip: 9 splits of a 20,000 x 64 block in 41 us; 32 rows and
+// four splits per round trip: 20 us for 6 splits); the sums are transposed through LDS so that
+// the writes run along rows.
 template <int KP>
 __global__ __launch_bounds__(kBlock) void k_bmm32_reduce(const float* __restrict__ part, int nsplit, int64_t n,
                                                          int ncols, double* __restrict__ W, int64_t ldw,
                                                          const int* __restrict__ skip)
 {
     if (skip && *skip) return;
-    constexpr int R = 32, E = R * KP / kBlock;
+    constexpr int R = kReduceRows, E = R * KP / kBlock;
     __shared__ double sh[KP][R + 1];
     const int64_t i0 = (int64_t)blockIdx.x * R;
     const int rows = (int)std::min<int64_t>(R, n - i0);
@@ -195,7 +198,7 @@ __global__ __launch_bounds__(kBlock) void k_bmm32_reduce(const float* __restrict
     const float* base = part + i0 * KP;
     double s[E];
     for (int m = 0; m < E; m++) s[m] = 0.;
-    constexpr int KS = 4;
+    constexpr int KS = 8;
     for (int k0 = 0; k0 < nsplit; k0 += KS) {
         float v[KS][E];
         for (int kk = 0; kk < KS; kk++) {
@@ -246,25 +249,31 @@ __global__ __launch_bounds__(kBlock) void k_pack_bf16(const T* __restrict__ A, i
 }
 
 // BF16 panel of the right-hand sides: Vp[(g*kp + c)*8 + m] = X(8g + m, c) (zero past n rows and
-// ncols columns)
+// ncols columns).  A block takes 2 groups (16 rows) x kp columns through LDS, so the reads run
+// down the columns (128 bytes) and the 16-byte writes along the panel.
 __global__ __launch_bounds__(kBlock) void k_panel_bf16(const double* __restrict__ X, int64_t ldx, int64_t n,
                                                        int ncols, int kp, int64_t ngroups, uint4* __restrict__ Vp)
 {
-    const int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x;  // e = g*kp + c
-    if (e >= ngroups * kp) return;
-    const int64_t g = e / kp;
-    const int c = (int)(e - g * kp);
-    uint16_t v[8];
-    for (int m = 0; m < 8; m++) {
-        const int64_t i = 8 * g + m;
-        v[m] = (c < ncols && i < n) ? bf16_bits((float)X[c * ldx + i]) : (uint16_t)0;
+    __shared__ float sh[64][17];
+    const int64_t g0 = (int64_t)blockIdx.x * 2, i0 = g0 * 8;
+    for (int e = threadIdx.x; e < kp * 16; e += kBlock) {
+        const int c = e / 16, r = e % 16;
+        const int64_t i = i0 + r;
+        sh[c][r] = (c < ncols && i < n) ? (float)X[c * ldx + i] : 0.f;
     }
-    uint4 w;
-    w.x = (uint32_t)v[0] | ((uint32_t)v[1] << 16);
-    w.y = (uint32_t)v[2] | ((uint32_t)v[3] << 16);
-    w.z = (uint32_t)v[4] | ((uint32_t)v[5] << 16);
-    w.w = (uint32_t)v[6] | ((uint32_t)v[7] << 16);
-    Vp[e] = w;
+    __syncthreads();
+    for (int e = threadIdx.x; e < 2 * kp; e += kBlock) {
+        const int g = e / kp, c = e % kp;
+        if (g0 + g >= ngroups) continue;
+        uint16_t v[8];
+        for (int m = 0; m < 8; m++) v[m] = bf16_bits(sh[c][8 * g + m]);
+        uint4 w;
+        w.x = (uint32_t)v[0] | ((uint32_t)v[1] << 16);
+        w.y = (uint32_t)v[2] | ((uint32_t)v[3] << 16);
+        w.z = (uint32_t)v[4] | ((uint32_t)v[5] << 16);
+        w.w = (uint32_t)v[6] | ((uint32_t)v[7] << 16);
+        Vp[(g0 + g) * kp + c] = w;
+    }
 }
 
 // part[s][i][c] = sum_{j in split s} A16(i, j) * Vp(j, c) with v_mfma_f32_32x32x16_bf16 (FP32
@@ -623,15 +632,26 @@ __global__ __launch_bounds__(kBlock) void k_bmm64_reduce(const double* __restric
 }
 
 // Vt[i][c] = X[c*ldx + i] (c < ncols), 0 for the padding columns; T = float or double
+// (a block transposes kPanelRows rows x kp <= 64 columns through LDS: 128-byte column reads,
+// row writes contiguous)
+constexpr int kPanelRows = 16;
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_panel(const double* __restrict__ X, int64_t ldx, int64_t n, int ncols,
                                                   int kp, T* __restrict__ Vt)
 {
-    const int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (e >= n * kp) return;
-    const int64_t i = e / kp;
-    const int c = (int)(e % kp);
-    Vt[e] = c < ncols ? (T)X[c * ldx + i] : (T)0;
+    __shared__ double sh[64][kPanelRows + 1];
+    const int64_t i0 = (int64_t)blockIdx.x * kPanelRows;
+    for (int e = threadIdx.x; e < kp * kPanelRows; e += kBlock) {
+        const int c = e / kPanelRows, r = e % kPanelRows;
+        const int64_t i = i0 + r;
+        sh[c][r] = (c < ncols && i < n) ? X[c * ldx + i] : 0.;
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < kPanelRows * kp; e += kBlock) {
+        const int r = e / kp, c = e % kp;
+        const int64_t i = i0 + r;
+        if (i < n) Vt[i * kp + c] = (T)sh[c][r];
+    }
 }
 
 // ------------------------------------------------------------------ batched Krylov operations
@@ -916,7 +936,7 @@ hipError_t launch_bmm(hipStream_t st, const float* A32, const double* A64, int64
         float* Vt = part + (size_t)sw * n * kp;
         const float* P = x_is_panel ? (const float*)X : Vt;
         if (!x_is_panel)
-            k_panel<float><<<nblocks(n * kp), kBlock, 0, st>>>(X, ldx, n, nrhs, kp, Vt);
+            k_panel<float><<<(unsigned)((n + kPanelRows - 1) / kPanelRows), kBlock, 0, st>>>(X, ldx, n, nrhs, kp, Vt);
         if (nt == 2 && n % 4 == 0 && (uintptr_t)A32 % 16 == 0 && (uintptr_t)P % 16 == 0) {
             // streamed through LDS; splits start at multiples of the chunk width
             const int ss = bmm_stream_splits(n);
@@ -925,7 +945,7 @@ hipError_t launch_bmm(hipStream_t st, const float* A32, const double* A64, int64
             const int nc = (int)((n + pc - 1) / pc);
             dim3 grid((unsigned)((n + kStreamRows - 1) / kStreamRows), (unsigned)nc);
             k_bmm32s<<<grid, kBlock, 0, st>>>(A32, n, P, pc, part, skip);
-            k_bmm32_reduce<64><<<(unsigned)((n + 31) / 32), kBlock, 0, st>>>(part, nc, n, nrhs, Y, ldy, skip);
+            k_bmm32_reduce<64><<<(unsigned)((n + kReduceRows - 1) / kReduceRows), kBlock, 0, st>>>(part, nc, n, nrhs, Y, ldy, skip);
             return hipGetLastError();
         }
         if (launch_tuning().bmm_rows == 128 && n % 4 == 0 && n >= 4) {
@@ -942,7 +962,7 @@ hipError_t launch_bmm(hipStream_t st, const float* A32, const double* A64, int64
             else
                 k_bmm32<2><<<grid, kBlock, 0, st>>>(A32, n, P, per, part, skip);
         }
-        const unsigned gr = (unsigned)((n + 31) / 32);
+        const unsigned gr = (unsigned)((n + kReduceRows - 1) / kReduceRows);
         if (kp == 32)
             k_bmm32_reduce<32><<<gr, kBlock, 0, st>>>(part, ns, n, nrhs, Y, ldy, skip);
         else
@@ -957,7 +977,8 @@ hipError_t launch_bmm(hipStream_t st, const float* A32, const double* A64, int64
     double* part = (double*)ws;
     double* Xt = part + (size_t)s * n * kb;
     const double* P = x_is_panel ? X : Xt;
-    if (!x_is_panel) k_panel<double><<<nblocks(n * kb), kBlock, 0, st>>>(X, ldx, n, nrhs, kb, Xt);
+    if (!x_is_panel)
+        k_panel<double><<<(unsigned)((n + kPanelRows - 1) / kPanelRows), kBlock, 0, st>>>(X, ldx, n, nrhs, kb, Xt);
     dim3 grid((unsigned)blocks, (unsigned)ns);
     if (kb == 16)
         k_bmm64<16><<<grid, kBlock, 0, st>>>(A64, n, P, per, part, skip);
@@ -1000,7 +1021,7 @@ hipError_t launch_bmm16(hipStream_t st, const uint16_t* A16, int64_t n, const vo
     uint4* Vp = (uint4*)pv;
     const bf8v* P = x_is_panel ? (const bf8v*)X : (const bf8v*)Vp;
     if (!x_is_panel)
-        k_panel_bf16<<<nblocks(ngroups * kp), kBlock, 0, st>>>((const double*)X, ldx, n, nrhs, kp, ngroups, Vp);
+        k_panel_bf16<<<(unsigned)((ngroups + 1) / 2), kBlock, 0, st>>>((const double*)X, ldx, n, nrhs, kp, ngroups, Vp);
     if (nrhs == 1) {
         // one right-hand side: the vector-ALU product, split over column groups into about 768
         // blocks (3 per CU: 6.0 TB/s on a 20,000-unknown matrix, against 4.5 TB/s at 22,000
@@ -1021,13 +1042,13 @@ hipError_t launch_bmm16(hipStream_t st, const uint16_t* A16, int64_t n, const vo
         const int nss = (int)((ngroups + gs - 1) / gs);
         dim3 grid((unsigned)((n + kStreamRows - 1) / kStreamRows), (unsigned)nss);
         k_bmm16s<<<grid, kBlock, 0, st>>>((const bf8v*)A16, n, P, ngroups, gs, part, skip);
-        k_bmm32_reduce<64><<<(unsigned)((n + 31) / 32), kBlock, 0, st>>>(part, nss, n, nrhs, Y, ldy, skip);
+        k_bmm32_reduce<64><<<(unsigned)((n + kReduceRows - 1) / kReduceRows), kBlock, 0, st>>>(part, nss, n, nrhs, Y, ldy, skip);
         return hipGetLastError();
     } else {
         dim3 grid((unsigned)((tiles + 3) / 4), (unsigned)ns);
         k_bmm16<1><<<grid, kBlock, 0, st>>>((const bf8v*)A16, n, P, ngroups, gper, part, skip);
     }
-    const unsigned gr = (unsigned)((n + 31) / 32);
+    const unsigned gr = (unsigned)((n + kReduceRows - 1) / kReduceRows);
     if (kp == 32)
         k_bmm32_reduce<32><<<gr, kBlock, 0, st>>>(part, ns, n, nrhs, Y, ldy, skip);
     else
