@@ -376,7 +376,8 @@ class KGBEMConverter:
 
 
 class KGBEMMeshConverter(KGBEMConverter):
-    """Triangles, rectangles and wires of a surface's mesh (KGBEMConverter.cc:229-300)."""
+    """Triangles, rectangles and wires of a surface's mesh (KGBEMConverter.cc:229-300); ``ignored``
+    counts the elements not added."""
 
     def VisitSurface(self, s: KGSurface) -> bool:
         self._set_current(s)
@@ -391,7 +392,7 @@ class KGBEMMeshConverter(KGBEMConverter):
             elif isinstance(e, KGMeshWire) and self._keep(e):
                 line.append(_line_record(self.LocalToInternal(e.GetP0()), self.LocalToInternal(e.GetP1()),
                                          e.diameter))
-            elif not isinstance(e, (KGMeshTriangle, KGMeshRectangle, KGMeshWire)):
+            else:  # another element type, or one the filters drop (KGBEMConverter.cc:292-294)
                 ignored += 1
         for shape, recs in (("triangle", tri), ("rectangle", rect), ("line", line)):
             for r in recs:

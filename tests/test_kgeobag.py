@@ -89,7 +89,7 @@ def test_filters_and_ignored():
     conv.SetMaximumAspectRatio(100.0)
     conv.VisitSurface(KG.KGSurface(mesh=[good, tiny, sliver, wire, "not an element"]))
     mesh = conv.container.mesh()
-    assert list(mesh.shapes) == [G.TRIANGLE, G.LINESEGMENT] and conv.ignored == 1
+    assert list(mesh.shapes) == [G.TRIANGLE, G.LINESEGMENT] and conv.ignored == 3
     # the bounds are strict: area > minimum, aspect < maximum
     conv = KG.KGBEMMeshConverter()
     conv.SetMinimumArea(0.5)
