@@ -13,7 +13,9 @@ with the FP32 far field (FP64 near field and sums).  `value` = point-element pai
 pass per second of the whole step (field evaluations per second with the solve time included),
 valid when the step's field is within 1e-6 of the FP64 pipeline's (norm-wise, max / rms and
 potential / voltage scale).  The FP64 pipeline (FP64 matrix, FP64 GMRES, FP64 field) is timed
-the same way on the same inputs; `pipeline_speedup` is the ratio of the two.
+the same way on the same inputs; `pipeline_speedup` is the ratio of the two.  Launch shapes
+are tuned on the workload before they are timed (the far pass's element ranges on a 1 M-point
+sample, the matrix-vector product's split target per precision before its own pipeline).
 
 Also reported, as extra keys: the 20,000-triangle coax (1 M points in the gap; FP64 and mixed
 field passes, GMRES, matrix-vector bandwidth), 64 voltage sets on it (lockstep GMRES on MFMA
@@ -366,12 +368,28 @@ def run_bore(ctx, args):
                 "far_regions": kt[1], "iterations": st.iterations, "refinements": st.refinements,
                 "residual": st.true_residual, "sigma": x, "A": A}
 
+    def tune_matvec(fp32):
+        """the matrix-vector product's split target on this matrix, FP64 storage for the FP64
+        pipeline, FP32 for the mixed one (93 of its ~95 products), each tuned before its own
+        pipeline (the knob is shared)"""
+        if args.no_autotune:
+            return
+        t0 = time.perf_counter()
+        A = K.KBoundaryIntegralMatrix(cont, fp64=not fp32, fp32=fp32)
+        r = T.tune_matvec(A, repeats=3, candidates=(256, 512, 768, 1024, 2048, 4096))
+        A.close()
+        key = "gemv_blocks_" + ("fp32" if fp32 else "fp64")
+        tuning[key] = r
+        log(f"{key} {r['best']} {r['ms']} ({time.perf_counter() - t0:.1f} s)")
+
     # ------------------------------------------------------------ FP64 pipeline (the plain path)
+    tune_matvec(False)
     p64 = pipeline(False, O64)
     log(f"FP64 pipeline {p64['total']:.2f} s: assembly {p64['assembly']:.2f} s, GMRES {p64['iterations']} it "
         f"{p64['solve']:.2f} s (res {p64['residual']:.1e}), field {p64['field']:.2f} s")
 
     # ------------------------------------------------------------ mixed pipeline: warm-up + timed steps
+    tune_matvec(True)
     for _ in range(args.warmup):
         pipeline(True, Omx)
     steps = [pipeline(True, Omx, keep_matrix=(k == args.steps - 1)) for k in range(args.steps)]
