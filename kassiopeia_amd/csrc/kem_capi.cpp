@@ -137,6 +137,10 @@ struct kem_matrix {
     float* A32 = nullptr;
     uint16_t* A16 = nullptr;  // BF16, packed in groups of 8 columns (kem::launch_pack_bf16)
     int trans = 0;  // solves run on A^T (kem_solve_transposed)
+    // the lockstep solver's device workspace (Krylov bases, Hessenberg matrices, panel, split
+    // partials), kept from one solve to the next and grown when a solve needs more
+    void* bws = nullptr;
+    size_t bws_bytes = 0;
 };
 
 namespace {
@@ -1116,6 +1120,7 @@ void kem_matrix_destroy(kem_matrix* A)
     dfree(A->A64);
     dfree(A->A32);
     dfree(A->A16);
+    if (A->bws) (void)hipFree(A->bws);
     delete A;
 }
 
@@ -1643,6 +1648,20 @@ int bmm_prec(int precision)
     return precision == KEM_MIXED_BF16 ? kem::kBmmBf16 : precision == KEM_MIXED ? kem::kBmmFp32 : kem::kBmmFp64;
 }
 
+// Y = A X into a workspace of at least kem::bmm_workspace_bytes(n, nrhs, prec) bytes
+int block_product_in(kem_matrix* A, int prec, int nrhs, const double* d_X, double* d_Y, void* ws, size_t ws_bytes)
+{
+    kem_ctx* c = A->ctx;
+    if (prec == kem::kBmmBf16)
+        KEM_CK(c, kem::launch_bmm16(c->stream, A->A16, A->n, d_X, A->n, nrhs, d_Y, A->n, ws, ws_bytes, false));
+    else
+        KEM_CK(c, kem::launch_bmm(c->stream, prec == kem::kBmmFp32 ? A->A32 : nullptr,
+                                  prec == kem::kBmmFp32 ? nullptr : A->A64, A->n, d_X, A->n, nrhs, d_Y, A->n, ws,
+                                  ws_bytes, false));
+    return KEM_OK;
+}
+
+// as block_product_in, growing the caller's workspace *ws (*ws_bytes bytes) first if needed
 int block_product(kem_matrix* A, int prec, int nrhs, const double* d_X, double* d_Y, void** ws, size_t* ws_bytes)
 {
     kem_ctx* c = A->ctx;
@@ -1654,13 +1673,7 @@ int block_product(kem_matrix* A, int prec, int nrhs, const double* d_X, double* 
         KEM_CK(c, hipMalloc(ws, need));
         *ws_bytes = need;
     }
-    if (prec == kem::kBmmBf16)
-        KEM_CK(c, kem::launch_bmm16(c->stream, A->A16, A->n, d_X, A->n, nrhs, d_Y, A->n, *ws, *ws_bytes, false));
-    else
-        KEM_CK(c, kem::launch_bmm(c->stream, prec == kem::kBmmFp32 ? A->A32 : nullptr,
-                                  prec == kem::kBmmFp32 ? nullptr : A->A64, A->n, d_X, A->n, nrhs, d_Y, A->n, *ws,
-                                  *ws_bytes, false));
-    return KEM_OK;
+    return block_product_in(A, prec, nrhs, d_X, d_Y, *ws, *ws_bytes);
 }
 
 struct DBuf {
@@ -1690,26 +1703,49 @@ int gmres_batch(kem_matrix* A, int prec, int nrhs, const double* d_B, double* d_
     const int ldh = kmax + 2;
     const int kp = panel_cols(nrhs, prec);
     const int64_t vstride = (int64_t)(kmax + 1) * n;
-    DVec V, W, h1, h2, Y, H, rot, vec;
-    DBuf panel, ws, ints;
-    KEM_CK(c, dalloc(&V.p, vstride * nrhs));
-    KEM_CK(c, dalloc(&W.p, n * nrhs));
-    KEM_CK(c, dalloc(&h1.p, (int64_t)ldh * nrhs));
-    KEM_CK(c, dalloc(&h2.p, (int64_t)ldh * nrhs));
-    KEM_CK(c, dalloc(&Y.p, (int64_t)ldh * nrhs));
-    KEM_CK(c, dalloc(&H.p, (int64_t)nrhs * ldh * ldh));
-    KEM_CK(c, dalloc(&rot.p, 3 * (int64_t)ldh * nrhs));
-    KEM_CK(c, dalloc(&vec.p, 4 * (int64_t)nrhs));
     // the panel's rows past n (BF16: the last group of 8) stay zero
     const size_t panel_bytes = prec == kem::kBmmBf16 ? sizeof(uint16_t) * (size_t)((n + 7) / 8) * 8 * kp
                                                      : (prec == kem::kBmmFp32 ? sizeof(float) : sizeof(double)) *
                                                            (size_t)n * kp;
-    KEM_CK(c, hipMalloc(&panel.p, panel_bytes));
-    KEM_CK(c, hipMemsetAsync(panel.p, 0, panel_bytes, st));
-    KEM_CK(c, hipMalloc(&ints.p, sizeof(int) * (4 * (size_t)nrhs + 4)));
     const size_t wsb = kem::bmm_workspace_bytes(n, nrhs, prec);
-    KEM_CK(c, hipMalloc(&ws.p, wsb));
-    ws.bytes = wsb;
+    // one allocation, carved into 256-byte aligned pieces, kept on the matrix between solves
+    size_t off = 0;
+    auto carve = [&off](size_t bytes) {
+        const size_t at = off;
+        off += (bytes + 255) & ~(size_t)255;
+        return at;
+    };
+    const size_t oV = carve(sizeof(double) * (size_t)(vstride * nrhs));
+    const size_t oW = carve(sizeof(double) * (size_t)(n * nrhs));
+    const size_t oh1 = carve(sizeof(double) * (size_t)ldh * nrhs);
+    const size_t oh2 = carve(sizeof(double) * (size_t)ldh * nrhs);
+    const size_t oY = carve(sizeof(double) * (size_t)ldh * nrhs);
+    const size_t oH = carve(sizeof(double) * (size_t)nrhs * ldh * ldh);
+    const size_t orot = carve(sizeof(double) * 3 * (size_t)ldh * nrhs);
+    const size_t ovec = carve(sizeof(double) * 4 * (size_t)nrhs);
+    const size_t opanel = carve(panel_bytes);
+    const size_t oints = carve(sizeof(int) * (4 * (size_t)nrhs + 4));
+    const size_t ows = carve(wsb);
+    if (A->bws_bytes < off) {
+        if (A->bws) (void)hipFree(A->bws);
+        A->bws = nullptr;
+        A->bws_bytes = 0;
+        KEM_CK(c, hipMalloc(&A->bws, off));
+        A->bws_bytes = off;
+    }
+    char* base = (char*)A->bws;
+    double* const V = (double*)(base + oV);
+    double* const W = (double*)(base + oW);
+    double* const h1 = (double*)(base + oh1);
+    double* const h2 = (double*)(base + oh2);
+    double* const Y = (double*)(base + oY);
+    double* const H = (double*)(base + oH);
+    double* const rot = (double*)(base + orot);
+    double* const vec = (double*)(base + ovec);
+    void* const panel = base + opanel;
+    void* const ints = base + oints;
+    void* const ws = base + ows;
+    KEM_CK(c, hipMemsetAsync(panel, 0, panel_bytes, st));
     kem::BatchArnoldi a;
     a.n = n;
     a.nrhs = nrhs;
@@ -1719,26 +1755,26 @@ int gmres_batch(kem_matrix* A, int prec, int nrhs, const double* d_B, double* d_
     a.use32 = prec == kem::kBmmFp32;
     a.A32 = A->A32;
     a.A64 = A->A64;
-    a.V = V.p;
+    a.V = V;
     a.vstride = vstride;
-    a.W = W.p;
+    a.W = W;
     a.A16 = prec == kem::kBmmBf16 ? A->A16 : nullptr;
-    a.P32 = prec == kem::kBmmFp32 ? (float*)panel.p : nullptr;
-    a.P64 = prec == kem::kBmmFp64 ? (double*)panel.p : nullptr;
-    a.P16 = prec == kem::kBmmBf16 ? (uint16_t*)panel.p : nullptr;
-    a.ws = ws.p;
-    a.ws_bytes = ws.bytes;
-    a.h1 = h1.p;
-    a.h2 = h2.p;
-    a.H = H.p;
-    a.cs = rot.p;
-    a.sn = rot.p + (int64_t)ldh * nrhs;
-    a.p = rot.p + 2 * (int64_t)ldh * nrhs;
-    a.nrm = vec.p;
-    a.bnorm = vec.p + nrhs;
-    a.tol = vec.p + 2 * nrhs;
-    a.resid = vec.p + 3 * nrhs;
-    a.active = (int*)ints.p;
+    a.P32 = prec == kem::kBmmFp32 ? (float*)panel : nullptr;
+    a.P64 = prec == kem::kBmmFp64 ? (double*)panel : nullptr;
+    a.P16 = prec == kem::kBmmBf16 ? (uint16_t*)panel : nullptr;
+    a.ws = ws;
+    a.ws_bytes = wsb;
+    a.h1 = h1;
+    a.h2 = h2;
+    a.H = H;
+    a.cs = rot;
+    a.sn = rot + (int64_t)ldh * nrhs;
+    a.p = rot + 2 * (int64_t)ldh * nrhs;
+    a.nrm = vec;
+    a.bnorm = vec + nrhs;
+    a.tol = vec + 2 * nrhs;
+    a.resid = vec + 3 * nrhs;
+    a.active = (int*)ints;
     a.done = a.active + nrhs;
     a.jr = a.done + nrhs;
     a.cnt = a.jr + nrhs;
@@ -1746,10 +1782,14 @@ int gmres_batch(kem_matrix* A, int prec, int nrhs, const double* d_B, double* d_
 
     std::vector<double> bnorm(nrhs), hp((size_t)ldh * nrhs, 0.), hh1(nrhs);
     std::vector<int> active(nrhs), done(nrhs, 0), ones(nrhs, 1);
-    KEM_CK(c, kem::launch_bnorm(st, d_B, n, nrhs, h1.p, ldh));
-    KEM_CK(c, hipMemcpy2DAsync(bnorm.data(), sizeof(double), h1.p, sizeof(double) * ldh, sizeof(double), nrhs,
-                               hipMemcpyDeviceToHost, st));
+    // the norms land ldh apart (h1's layout); one contiguous copy of that span reads them back (a
+    // strided 2D copy into pageable memory took 2.6 ms per call)
+    const size_t span = (size_t)(nrhs - 1) * ldh + 1;
+    std::vector<double> stage(span);
+    KEM_CK(c, kem::launch_bnorm(st, d_B, n, nrhs, h1, ldh));
+    KEM_CK(c, hipMemcpyAsync(stage.data(), h1, sizeof(double) * span, hipMemcpyDeviceToHost, st));
     KEM_CK(c, hipStreamSynchronize(st));
+    for (int r = 0; r < nrhs; r++) bnorm[r] = stage[(size_t)r * ldh];
     for (auto& b : bnorm)
         if (b == 0.) b = 1.;
     KEM_CK(c, hipMemcpyAsync(a.bnorm, bnorm.data(), sizeof(double) * nrhs, hipMemcpyHostToDevice, st));
@@ -1784,13 +1824,13 @@ int gmres_batch(kem_matrix* A, int prec, int nrhs, const double* d_B, double* d_
     int it = 0, status = KEM_OK;
     for (;;) {
         // W = B - A X; V_0 = W / ||W||
-        int rc = block_product(A, prec, nrhs, d_X, W.p, &ws.p, &ws.bytes);
+        int rc = block_product_in(A, prec, nrhs, d_X, W, ws, wsb);
         if (rc != KEM_OK) return rc;
-        KEM_CK(c, kem::launch_baxpby(st, 1.0, d_B, -1.0, W.p, W.p, n * nrhs));
-        KEM_CK(c, kem::launch_bnorm(st, W.p, n, nrhs, h1.p, ldh));
-        KEM_CK(c, hipMemcpy2DAsync(hh1.data(), sizeof(double), h1.p, sizeof(double) * ldh, sizeof(double), nrhs,
-                                   hipMemcpyDeviceToHost, st));
+        KEM_CK(c, kem::launch_baxpby(st, 1.0, d_B, -1.0, W, W, n * nrhs));
+        KEM_CK(c, kem::launch_bnorm(st, W, n, nrhs, h1, ldh));
+        KEM_CK(c, hipMemcpyAsync(stage.data(), h1, sizeof(double) * span, hipMemcpyDeviceToHost, st));
         KEM_CK(c, hipStreamSynchronize(st));
+        for (int r = 0; r < nrhs; r++) hh1[r] = stage[(size_t)r * ldh];
         int nact = 0;
         for (int r = 0; r < nrhs; r++) {
             hp[(size_t)r * ldh] = hh1[r];
@@ -1809,7 +1849,7 @@ int gmres_batch(kem_matrix* A, int prec, int nrhs, const double* d_B, double* d_
         KEM_CK(c, hipMemcpyAsync(a.p, hp.data(), sizeof(double) * ldh * nrhs, hipMemcpyHostToDevice, st));
         KEM_CK(c, hipMemcpyAsync(a.resid, resid.data(), sizeof(double) * nrhs, hipMemcpyHostToDevice, st));
         KEM_CK(c, hipMemsetAsync(a.jr, 0, sizeof(int) * nrhs, st));
-        KEM_CK(c, kem::launch_bnormalize(st, W.p, h1.p, ldh, a.active, n, nrhs, V.p, vstride, 0, kp, a.P32, a.P64,
+        KEM_CK(c, kem::launch_bnormalize(st, W, h1, ldh, a.active, n, nrhs, V, vstride, 0, kp, a.P32, a.P64,
                                          a.P16));
         int got[4] = {0, 0, 0, 0};
         bool expired = false;
@@ -1826,8 +1866,8 @@ int gmres_batch(kem_matrix* A, int prec, int nrhs, const double* d_B, double* d_
         }
         it += got[3];
         // x_r += V_r y_r with H_r y_r = p_r (upper triangular, jr[r] columns)
-        KEM_CK(c, kem::launch_batch_trisolve(st, a, Y.p));
-        KEM_CK(c, kem::launch_bupdate(st, V.p, vstride, Y.p, ldh, a.cnt, 1.0, n, nrhs, d_X));
+        KEM_CK(c, kem::launch_batch_trisolve(st, a, Y));
+        KEM_CK(c, kem::launch_bupdate(st, V, vstride, Y, ldh, a.cnt, 1.0, n, nrhs, d_X));
         KEM_CK(c, hipMemcpyAsync(done.data(), a.done, sizeof(int) * nrhs, hipMemcpyDeviceToHost, st));
         KEM_CK(c, hipMemcpyAsync(resid.data(), a.resid, sizeof(double) * nrhs, hipMemcpyDeviceToHost, st));
         KEM_CK(c, hipStreamSynchronize(st));
