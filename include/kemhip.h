@@ -114,16 +114,17 @@ int kem_ctx_set_mixed_fp64_ratio(kem_ctx* ctx, double ratio);
    or more points of one charge-density set; the elements split into 8 R ranges, R per XCD, so
    each XCD's range stays in its L2 (0: no split; default 2560).  KEM_TUNE_GEMV_BLOCKS /
    KEM_TUNE_BMM_BLOCKS = target block counts of the split-K matrix-vector product (default 2048)
-   and of the multi-RHS block product in 64-row tiles times column splits (default 2560); they
-   set how many partial sums are reduced. */
+   and of the register-loaded multi-RHS block products in 64-row tiles times column splits
+   (BF16 with up to 32 right-hand sides, FP32 with n % 4 != 0, FP64 with n odd; default 2560);
+   they set how many partial sums are reduced. */
 #define KEM_TUNE_FAR_RANGE_FP64 1
 #define KEM_TUNE_FAR_RANGE_MIXED 2
 #define KEM_TUNE_GEMV_BLOCKS 3
 #define KEM_TUNE_BMM_BLOCKS 4
 #define KEM_TUNE_NEAR_BLOCKS 5 /* blocks of the near pass beside a single-set mixed far pass (0: one per 256 points; default 384) */
-/* wavefront tile of the FP32 MFMA block product: 64 rows (one float per lane and column), or
-   128 rows (four consecutive rows per lane in one 16-byte load; matrices with n % 4 == 0);
-   default 64 */
+/* wavefront tile of the register-loaded FP32 MFMA block product: 64 rows (one float per lane
+   and column), or 128 rows (four consecutive rows per lane in one 16-byte load; n % 4 == 0,
+   which the streamed kernels take whenever the buffers are 16-byte aligned); default 64 */
 #define KEM_TUNE_BMM_ROWS 6
 /* Robin Hood sub-iterations: 0 = one workgroup up to 8,192 unknowns and several co-resident
    workgroups above (default), 1 = always one workgroup (the same corrections either way) */
@@ -132,6 +133,11 @@ int kem_ctx_set_mixed_fp64_ratio(kem_ctx* ctx, double ratio);
    matrix through LDS (FP32 matrices with n % 4 == 0, and BF16): 256 rows per workgroup times
    column splits; default 480 */
 #define KEM_TUNE_BMM_STREAM_BLOCKS 8
+/* largest workgroup count of the block products on 16 x 16 MFMA tiles streamed through LDS
+   (FP32 matrices with n % 4 == 0 and up to 32 right-hand sides; FP64 matrices with n even): 256
+   rows per workgroup times column splits, the split count within it whose last round of two
+   workgroups per compute unit is fullest; default 4096 */
+#define KEM_TUNE_BMM_TILE_BLOCKS 9
 int kem_ctx_set_tuning(kem_ctx* ctx, int param, int64_t value);
 int kem_ctx_get_tuning(const kem_ctx* ctx, int param, int64_t* value);
 /* Instrumentation, no reference counterpart: with timing on, each field evaluation records HIP

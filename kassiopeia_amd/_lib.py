@@ -19,6 +19,7 @@ KEM_TUNE_NEAR_BLOCKS = 5
 KEM_TUNE_BMM_ROWS = 6
 KEM_TUNE_ROBIN_HOOD_GROUPS = 7
 KEM_TUNE_BMM_STREAM_BLOCKS = 8
+KEM_TUNE_BMM_TILE_BLOCKS = 9
 
 
 class KemError(RuntimeError):

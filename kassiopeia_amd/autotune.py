@@ -7,10 +7,14 @@ The C-ABI exposes these launch-shape knobs (``kem_ctx_set_tuning``):
   range an XCD works on stays in its L2; 0 disables the split.
 * ``KEM_TUNE_GEMV_BLOCKS``: target block count of the split-K matrix-vector product (GMRES,
   Robin Hood residuals, ``KSquareMatrix::Multiply``).
-* ``KEM_TUNE_BMM_BLOCKS``: target block count of the multi-right-hand-side block product
-  (kernels that load the matrix into registers: up to 32 right-hand sides, FP64 matrices).
+* ``KEM_TUNE_BMM_BLOCKS``: target block count of the multi-right-hand-side block products
+  that load the matrix into registers (BF16 up to 32 right-hand sides, FP32 when n % 4 != 0,
+  FP64 when n is odd).
 * ``KEM_TUNE_BMM_STREAM_BLOCKS``: target workgroup count of the block products of 33-64
   right-hand sides that stream the matrix through LDS (BF16, and FP32 when n % 4 == 0).
+* ``KEM_TUNE_BMM_TILE_BLOCKS``: target workgroup count of the block products on 16 x 16 MFMA
+  tiles streamed through LDS (FP32 up to 32 right-hand sides when n % 4 == 0, FP64 when n is
+  even).
 * ``KEM_TUNE_NEAR_BLOCKS``: blocks of the near pass that runs beside the mixed far pass on a
   second stream (0: one per 256 points).  Its timed region is the whole field pass.
 * ``KEM_TUNE_BMM_ROWS``: the wavefront tile of the FP32 MFMA block product, 64 rows (one float
@@ -36,13 +40,15 @@ FAR_RANGE_CANDIDATES = (0, 640, 1280, 2560, 5120, 10240)
 GEMV_CANDIDATES = (512, 1024, 2048, 4096, 8192)
 BMM_CANDIDATES = (640, 1280, 2560, 5120, 10240)
 STREAM_CANDIDATES = (160, 240, 320, 400, 480, 640, 960)
+TILE_CANDIDATES = (256, 512, 768, 1024, 1536, 2048, 4096)
 NEAR_CANDIDATES = (0, 128, 256, 384, 512)
 BMM_ROW_CANDIDATES = (64, 128)
 
 _KNOB = {"far_range_fp64": _lib.KEM_TUNE_FAR_RANGE_FP64, "far_range_mixed": _lib.KEM_TUNE_FAR_RANGE_MIXED,
          "gemv_blocks": _lib.KEM_TUNE_GEMV_BLOCKS, "bmm_blocks": _lib.KEM_TUNE_BMM_BLOCKS,
          "near_blocks": _lib.KEM_TUNE_NEAR_BLOCKS, "bmm_rows": _lib.KEM_TUNE_BMM_ROWS,
-         "robin_hood_groups": _lib.KEM_TUNE_ROBIN_HOOD_GROUPS, "bmm_stream_blocks": _lib.KEM_TUNE_BMM_STREAM_BLOCKS}
+         "robin_hood_groups": _lib.KEM_TUNE_ROBIN_HOOD_GROUPS, "bmm_stream_blocks": _lib.KEM_TUNE_BMM_STREAM_BLOCKS,
+         "bmm_tile_blocks": _lib.KEM_TUNE_BMM_TILE_BLOCKS}
 
 
 def get(ctx: K.Context, name: str) -> int:
@@ -138,13 +144,16 @@ def tune_matvec(matrix: K.KBoundaryIntegralMatrix, repeats=5, candidates=GEMV_CA
 
 def tune_block_product(matrix: K.KBoundaryIntegralMatrix, nrhs=64, precision="mixed", repeats=3,
                        candidates=None, rows=BMM_ROW_CANDIDATES):
-    """Launch shape of the block product with nrhs right-hand sides (MFMA for "mixed" / "bf16").
-    33-64 right-hand sides on a BF16 matrix, or on an FP32 matrix with n % 4 == 0, run the
-    kernels that stream the matrix through LDS: their workgroup target ``bmm_stream_blocks`` is
-    swept (candidates default STREAM_CANDIDATES).  Otherwise the FP32 wavefront tile is swept
-    first at the current split target, then the split target ``bmm_blocks`` with the chosen tile
-    (candidates default BMM_CANDIDATES; the tile's sweep under "rows").  Returns the sweep of the
-    split knob with its name under "knob"."""
+    """Launch shape of the block product with nrhs right-hand sides ("fp64", "mixed" = FP32
+    matrix, "bf16").  33-64 right-hand sides on a BF16 matrix, or on an FP32 matrix with
+    n % 4 == 0, run the 32 x 32-tile kernels that stream the matrix through LDS: their workgroup
+    target ``bmm_stream_blocks`` is swept (candidates default STREAM_CANDIDATES).  Up to 32
+    right-hand sides on an FP32 matrix with n % 4 == 0, and an FP64 matrix with n even, run the
+    16 x 16-tile streamed kernels: ``bmm_tile_blocks`` is swept (TILE_CANDIDATES).  Otherwise
+    (register-loaded kernels) the FP32 wavefront tile is swept first at the current split
+    target, then the split target ``bmm_blocks`` with the chosen tile (candidates default
+    BMM_CANDIDATES; the tile's sweep under "rows").  Returns the sweep of the split knob with
+    its name under "knob"."""
     X = np.random.default_rng(1).normal(size=(matrix.n, nrhs))
 
     def run():
@@ -154,6 +163,12 @@ def tune_block_product(matrix: K.KBoundaryIntegralMatrix, nrhs=64, precision="mi
     if streamed:
         res = _sweep(matrix.ctx, "bmm_stream_blocks", candidates or STREAM_CANDIDATES, run, repeats)
         res["knob"] = "bmm_stream_blocks"
+        return res
+    tiled = ((precision == "mixed" and nrhs <= 32 and matrix.n % 4 == 0)
+             or (precision == "fp64" and matrix.n % 2 == 0))
+    if tiled:
+        res = _sweep(matrix.ctx, "bmm_tile_blocks", candidates or TILE_CANDIDATES, run, repeats)
+        res["knob"] = "bmm_tile_blocks"
         return res
     tile = None
     if precision == "mixed":

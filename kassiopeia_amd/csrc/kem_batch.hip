@@ -184,28 +184,29 @@ constexpr int kReduceRows = 16;  // rows per block of k_bmm32_reduce
 // memory (one split per round trip: 9 splits of a 20,000 x 64 block in 41 us; 32 rows and
 // four splits per round trip: 20 us for 6 splits); the sums are transposed through LDS so that
 // the writes run along rows.
-template <int KP>
-__global__ __launch_bounds__(kBlock) void k_bmm32_reduce(const float* __restrict__ part, int nsplit, int64_t n,
+template <int KP, typename P = float>
+__global__ __launch_bounds__(kBlock) void k_bmm32_reduce(const P* __restrict__ part, int nsplit, int64_t n,
                                                          int ncols, double* __restrict__ W, int64_t ldw,
                                                          const int* __restrict__ skip)
 {
     if (skip && *skip) return;
     constexpr int R = kReduceRows, E = R * KP / kBlock;
+    static_assert(E >= 1, "at least one entry per thread");
     __shared__ double sh[KP][R + 1];
     const int64_t i0 = (int64_t)blockIdx.x * R;
     const int rows = (int)std::min<int64_t>(R, n - i0);
     const int lim = rows * KP;
-    const float* base = part + i0 * KP;
+    const P* base = part + i0 * KP;
     double s[E];
     for (int m = 0; m < E; m++) s[m] = 0.;
     constexpr int KS = 8;
     for (int k0 = 0; k0 < nsplit; k0 += KS) {
-        float v[KS][E];
+        P v[KS][E];
         for (int kk = 0; kk < KS; kk++) {
-            const float* p = base + (int64_t)(k0 + kk) * n * KP;
+            const P* p = base + (int64_t)(k0 + kk) * n * KP;
             for (int m = 0; m < E; m++) {
                 const int e = threadIdx.x + kBlock * m;
-                v[kk][m] = (k0 + kk < nsplit && e < lim) ? p[e] : 0.f;
+                v[kk][m] = (k0 + kk < nsplit && e < lim) ? p[e] : (P)0;
             }
         }
         for (int kk = 0; kk < KS; kk++)
@@ -533,6 +534,131 @@ __global__ __launch_bounds__(kBlock) void k_bmm16s(const bf8v* __restrict__ A16,
             const int rr2 = 8 * (v >> 2) + (v & 3);
             if (rr2 < lim) out[rr2 * KP + 32 * t] = acc0[t][v];
             if (rr2 + 32 < lim) out[(rr2 + 32) * KP + 32 * t] = acc1[t][v];
+        }
+}
+
+// FP32 or FP64 matrix, KP = 16, 32 or 64 panel columns, on 16 x 16 MFMA tiles
+// (v_mfma_f32_16x16x4f32 / v_mfma_f64_16x16x4f64: operand A is 16 rows x 4 columns of the
+// matrix, lane l holding row l % 16 of column l / 16; operand B is 4 panel rows x 16 columns, lane
+// l holding panel row l / 16, column l % 16; accumulator register v of lane l holds column l % 16
+// of tile row 4 (l / 16) + v in FP32, of tile row l / 16 + 4 v in FP64: layouts measured on
+// gfx950).  Sixteen columns need no padding to 32, so 16 right-hand sides
+// cost half the MFMA work of the 32 x 32 tiles, and the same kernel serves the FP64 matrix.  A
+// workgroup takes 256 rows (wavefront w: rows 64 w .. 64 w + 63, four row tiles x KP / 16 column
+// tiles) and streams a range of columns through three LDS buffers as k_bmm32s does.  Chunk:
+// 64 bytes of every row, i.e. KC = 16 FP32 or 8 FP64 columns (sixteen 1 KB loads, four per
+// wavefront: FP32 lane l takes rows 4 l .. 4 l + 3 of a column, FP64 rows 2 l, 2 l + 1 of one
+// half), and the KC panel rows (KC KP elements, a quarter per wavefront, in one load whose
+// lanes past the quarter are idle).  Columns sit 64 bytes apart beyond their 256 rows in LDS so
+// that the two column halves a wavefront's operand read touches fall in different banks.
+// Requires n % (16 / sizeof(T)) == 0 and 16-byte aligned A and Vt.  part[s][i][c] in T.
+template <typename T>
+struct Mfma16;
+template <>
+struct Mfma16<float> {
+    typedef float v4 __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ v4 step(float a, float b, v4 c)
+    {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    // tile row of accumulator register v in lanes 16 lk .. 16 lk + 15
+    static constexpr __device__ int row(int lk, int v) { return 4 * lk + v; }
+};
+template <>
+struct Mfma16<double> {
+    typedef double v4 __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ v4 step(double a, double b, v4 c)
+    {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    static constexpr __device__ int row(int lk, int v) { return lk + 4 * v; }
+};
+
+// the loads of chunk k of k_bmmt into its LDS buffer (see there)
+template <typename T, int KP>
+__device__ __forceinline__ void bmmt_issue(const T* __restrict__ A, int64_t n, const T* __restrict__ Vt, T* lds,
+                                           int64_t r0, int64_t j0, int64_t j1, int64_t k, int lane, int wave)
+{
+    constexpr int SZ = sizeof(T), KC = 64 / SZ, PER = 16 / SZ, LPC = kStreamRows * SZ / 1024;
+    constexpr int CS = kStreamRows + KC, PC = KC * KP, PQ = PC / 4, CH = KC * CS + PC;
+    T* buf = lds + (k % 3) * CH;
+    const int64_t jc = j0 + k * KC;
+    for (int q = 0; q < 4; q++) {
+        const int u = wave + 4 * q;  // load u of the chunk: column u / LPC, part u % LPC
+        const int c = u / LPC, h = u % LPC;
+        const int64_t col = std::min<int64_t>(jc + c, j1 - 1);  // past j1: a valid column, unused
+        // rows past n read the last rows (never stored)
+        const int64_t row = std::min<int64_t>(r0 + 1024 / SZ * h + PER * lane, n - PER);
+        __builtin_amdgcn_global_load_lds(A + col * n + row, buf + c * CS + 1024 / SZ * h, 16, 0, 0);
+    }
+    // this wavefront's quarter of the chunk's panel rows; rows past n read row n - 1 (their
+    // matrix operands are zero)
+    if (lane * PER < PQ) {
+        const int e = wave * PQ + lane * PER;
+        int64_t prow = jc + e / KP;
+        if (prow > n - 1) prow = n - 1;
+        __builtin_amdgcn_global_load_lds(Vt + prow * KP + e % KP, buf + KC * CS + wave * PQ, 16, 0, 0);
+    }
+}
+
+template <typename T, int KP>
+__global__ __launch_bounds__(kBlock, 2) void k_bmmt(const T* __restrict__ A, int64_t n, const T* __restrict__ Vt,
+                                                 int64_t per_split, T* __restrict__ part,
+                                                 const int* __restrict__ skip)
+{
+    if (skip && *skip) return;
+    constexpr int SZ = sizeof(T), KC = 64 / SZ, PER = 16 / SZ;  // PER: elements per 16-byte lane load
+    constexpr int LPC = kStreamRows * SZ / 1024;                 // 1 KB loads per column
+    constexpr int CS = kStreamRows + KC;                          // LDS column stride (elements)
+    constexpr int PC = KC * KP, PQ = PC / 4;                      // panel elements per chunk, per wavefront
+    constexpr int CH = KC * CS + PC, NW = KC * LPC / 4 + 1;
+    constexpr int CT = KP / 16;
+    static_assert(KC * LPC == 16 && PQ * SZ <= 1024 && PQ % PER == 0 && KP % 16 == 0, "chunk shape");
+    typedef typename Mfma16<T>::v4 acc_t;
+    __shared__ T lds[3 * CH];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lr = lane & 15, lk = lane >> 4;
+    const int64_t r0 = (int64_t)blockIdx.x * kStreamRows;
+    const int64_t j0 = (int64_t)blockIdx.y * per_split;
+    const int64_t j1 = std::min<int64_t>(n, j0 + per_split);
+    if (j0 >= j1) return;
+    const int64_t nch = (j1 - j0 + KC - 1) / KC;
+    acc_t acc[4][CT];
+    for (int m = 0; m < 4; m++)
+        for (int t = 0; t < CT; t++)
+            for (int v = 0; v < 4; v++) acc[m][t][v] = (T)0;
+    bmmt_issue<T, KP>(A, n, Vt, lds, r0, j0, j1, 0, lane, wave);
+    if (nch > 1) bmmt_issue<T, KP>(A, n, Vt, lds, r0, j0, j1, 1, lane, wave);
+    for (int64_t k = 0; k < nch; k++) {
+        if (k + 1 < nch)
+            wait_vmcnt<NW>();
+        else
+            wait_vmcnt<0>();
+        __builtin_amdgcn_s_barrier();
+        asm volatile("" ::: "memory");
+        const T* buf = lds + (k % 3) * CH;
+        const int kv = (int)std::min<int64_t>(KC, j1 - (j0 + k * KC));  // valid columns
+        T a[KC / 4][4], b[KC / 4][CT];
+        for (int p = 0; p < KC / 4; p++) {
+            const int col = 4 * p + lk;
+            for (int m = 0; m < 4; m++) a[p][m] = col < kv ? buf[col * CS + wave * 64 + 16 * m + lr] : (T)0;
+            for (int t = 0; t < CT; t++) b[p][t] = buf[KC * CS + col * KP + 16 * t + lr];
+        }
+        if (k + 2 < nch) bmmt_issue<T, KP>(A, n, Vt, lds, r0, j0, j1, k + 2, lane, wave);
+        for (int p = 0; p < KC / 4; p++)
+            for (int m = 0; m < 4; m++)
+                for (int t = 0; t < CT; t++) acc[m][t] = Mfma16<T>::step(a[p][m], b[p][t], acc[m][t]);
+    }
+    const int64_t i0 = r0 + wave * 64;
+    if (i0 >= n) return;
+    // tile m, register v: row i0 + 16 m + Mfma16<T>::row(lk, v), column 16 t + lr
+    T* out = part + (int64_t)blockIdx.y * n * KP + i0 * KP + lr;
+    const int64_t lim = n - i0;
+    for (int m = 0; m < 4; m++)
+        for (int v = 0; v < 4; v++) {
+            const int rr = 16 * m + Mfma16<T>::row(lk, v);
+            if (rr < lim)
+                for (int t = 0; t < CT; t++) out[rr * KP + 16 * t] = acc[m][t][v];
         }
 }
 
@@ -890,6 +1016,32 @@ int bmm_stream_splits(int64_t n)
     return (int)std::max<int64_t>(1, std::min<int64_t>(s, 64));
 }
 
+// column splits of k_bmmt for a matrix of sz-byte entries and kp panel columns.  Two
+// workgroups fit a compute unit (54-62 KB of LDS each), so the grid of 256-row blocks x splits
+// runs in rounds of 2 x CUs workgroups of equal work; of the split counts whose grid stays
+// within bmm_tile_blocks, the one of least modelled cost per row: the matrix row (n sz bytes)
+// stretched by the idle part of the last round, plus 8 kp sz bytes per split for its partials
+// (written, read back and summed; the weight fits the measured optima: on the 20,000-unknown
+// coax, 79 row blocks, 6 splits, 0.277 ms for 16 FP32 right-hand sides against 0.291 ms at 19
+// and 0.33 ms at 10 or 13; 9 splits on a 100,000-unknown matrix, 7.3 ms against 8.5 ms at 4).
+int bmm_tile_splits(int64_t n, int kp, int sz)
+{
+    const int64_t rb = (n + 255) / 256, cap = 2 * std::max<int64_t>(1, launch_tuning().cus);
+    const int64_t smax = std::max<int64_t>(
+        1, std::min<int64_t>({launch_tuning().bmm_tile_blocks / rb, std::max<int64_t>(1, n / 256), 64}));
+    int64_t best = 1;
+    double cost = 0.;
+    for (int64_t s = 1; s <= smax; s++) {
+        const int64_t g = rb * s;
+        const double c = (double)(cap * ((g + cap - 1) / cap)) / (double)g * (double)n * sz + 8.0 * s * kp * sz;
+        if (s == 1 || c < cost) {
+            cost = c;
+            best = s;
+        }
+    }
+    return (int)best;
+}
+
 int bmm_splits(int64_t n, int64_t tiles)
 {
     int64_t s = (launch_tuning().bmm_blocks + tiles - 1) / std::max<int64_t>(tiles, 1);
@@ -899,9 +1051,16 @@ int bmm_splits(int64_t n, int64_t tiles)
 
 }  // namespace
 
+int bmm_panel_cols(int64_t n, int nrhs, int prec)
+{
+    if (prec == kBmmFp64) return nrhs <= 16 ? 16 : 64;
+    if (prec == kBmmFp32 && nrhs <= 16 && n % 4 == 0) return 16;  // k_bmmt only
+    return nrhs <= 32 ? 32 : 64;
+}
+
 size_t bmm_workspace_bytes(int64_t n, int nrhs, int prec)
 {
-    const int kp = nrhs <= 32 ? 32 : 64;
+    const int kp = bmm_panel_cols(n, nrhs, prec);
     if (prec == kBmmBf16) {
         const int64_t tiles = (n + 63) / 64, ngroups = (n + 7) / 8;
         const int s = std::max(bmm_splits(n, tiles), kp == 64 ? bmm_stream_splits(n) : 1);
@@ -909,13 +1068,13 @@ size_t bmm_workspace_bytes(int64_t n, int nrhs, int prec)
     }
     if (prec == kBmmFp32) {
         const int64_t tiles = (n + 63) / 64;
-        const int s = std::max(bmm_splits(n, tiles), kp == 64 ? bmm_stream_splits(n) : 1);
+        const int s = std::max({bmm_splits(n, tiles), kp == 64 ? bmm_stream_splits(n) : 1,
+                                kp <= 32 && n % 4 == 0 ? bmm_tile_splits(n, kp, 4) : 1});
         return sizeof(float) * ((size_t)s * n * kp + (size_t)n * kp) + 256;
     }
-    const int kb = nrhs <= 16 ? 16 : 64;
     const int64_t blocks = (n + kBlock - 1) / kBlock;
-    const int s = bmm_splits(n, blocks * 4);
-    return sizeof(double) * ((size_t)s * n * kb + (size_t)n * kb) + 256;
+    const int s = std::max(bmm_splits(n, blocks * 4), n % 2 == 0 ? bmm_tile_splits(n, kp, 8) : 1);
+    return sizeof(double) * ((size_t)s * n * kp + (size_t)n * kp) + 256;
 }
 
 hipError_t launch_bmm(hipStream_t st, const float* A32, const double* A64, int64_t n, const double* X, int64_t ldx,
@@ -923,21 +1082,42 @@ hipError_t launch_bmm(hipStream_t st, const float* A32, const double* A64, int64
 {
     if (n <= 0 || nrhs <= 0) return hipSuccess;
     if (nrhs > 64 || ws_bytes < bmm_workspace_bytes(n, nrhs, A32 ? kBmmFp32 : kBmmFp64)) return hipErrorInvalidValue;
+    const int kp = bmm_panel_cols(n, nrhs, A32 ? kBmmFp32 : kBmmFp64);
     if (A32) {
-        const int nt = nrhs <= 32 ? 1 : 2, kp = 32 * nt;
+        const int nt = kp == 64 ? 2 : 1;
         const int64_t tiles = (n + 63) / 64;
         const int s = bmm_splits(n, tiles);
         int64_t per = (n + s - 1) / s;
         per += per & 1;
         const int ns = (int)((n + per - 1) / per);
         float* part = (float*)ws;
-        // the panel after the partial sums of either kernel
-        const int sw = std::max(s, nt == 2 ? bmm_stream_splits(n) : 1);
+        // the panel after the partial sums of any of the kernels
+        const int sw = std::max({s, nt == 2 ? bmm_stream_splits(n) : 1,
+                                 kp <= 32 && n % 4 == 0 ? bmm_tile_splits(n, kp, 4) : 1});
         float* Vt = part + (size_t)sw * n * kp;
         const float* P = x_is_panel ? (const float*)X : Vt;
         if (!x_is_panel)
             k_panel<float><<<(unsigned)((n + kPanelRows - 1) / kPanelRows), kBlock, 0, st>>>(X, ldx, n, nrhs, kp, Vt);
-        if (nt == 2 && n % 4 == 0 && (uintptr_t)A32 % 16 == 0 && (uintptr_t)P % 16 == 0) {
+        const bool a16 = n % 4 == 0 && (uintptr_t)A32 % 16 == 0 && (uintptr_t)P % 16 == 0;
+        if (kp <= 32 && a16) {
+            // 16 x 16 tiles streamed through LDS; splits start at multiples of the chunk width
+            const int ss = bmm_tile_splits(n, kp, 4);
+            int64_t pc = (n + ss - 1) / ss;
+            pc = (pc + 15) / 16 * 16;
+            const int nc = (int)((n + pc - 1) / pc);
+            dim3 grid((unsigned)((n + kStreamRows - 1) / kStreamRows), (unsigned)nc);
+            const unsigned gr = (unsigned)((n + kReduceRows - 1) / kReduceRows);
+            if (kp == 16) {
+                k_bmmt<float, 16><<<grid, kBlock, 0, st>>>(A32, n, P, pc, part, skip);
+                k_bmm32_reduce<16><<<gr, kBlock, 0, st>>>(part, nc, n, nrhs, Y, ldy, skip);
+            } else {
+                k_bmmt<float, 32><<<grid, kBlock, 0, st>>>(A32, n, P, pc, part, skip);
+                k_bmm32_reduce<32><<<gr, kBlock, 0, st>>>(part, nc, n, nrhs, Y, ldy, skip);
+            }
+            return hipGetLastError();
+        }
+        if (kp == 16) return hipErrorInvalidValue;  // a 16-column panel needs the 16 x 16 tiles
+        if (nt == 2 && a16) {
             // streamed through LDS; splits start at multiples of the chunk width
             const int ss = bmm_stream_splits(n);
             int64_t pc = (n + ss - 1) / ss;
@@ -969,16 +1149,34 @@ hipError_t launch_bmm(hipStream_t st, const float* A32, const double* A64, int64
             k_bmm32_reduce<64><<<gr, kBlock, 0, st>>>(part, ns, n, nrhs, Y, ldy, skip);
         return hipGetLastError();
     }
-    const int kb = nrhs <= 16 ? 16 : 64;
+    const int kb = kp;
     const int64_t blocks = (n + kBlock - 1) / kBlock;
     const int s = bmm_splits(n, blocks * 4);
     const int64_t per = (n + s - 1) / s;
     const int ns = (int)((n + per - 1) / per);
     double* part = (double*)ws;
-    double* Xt = part + (size_t)s * n * kb;
+    const int sw = std::max(s, n % 2 == 0 ? bmm_tile_splits(n, kb, 8) : 1);
+    double* Xt = part + (size_t)sw * n * kb;
     const double* P = x_is_panel ? X : Xt;
     if (!x_is_panel)
         k_panel<double><<<(unsigned)((n + kPanelRows - 1) / kPanelRows), kBlock, 0, st>>>(X, ldx, n, nrhs, kb, Xt);
+    if (n % 2 == 0 && (uintptr_t)A64 % 16 == 0 && (uintptr_t)P % 16 == 0) {
+        // FP64 MFMA on 16 x 16 tiles streamed through LDS
+        const int ss = bmm_tile_splits(n, kb, 8);
+        int64_t pc = (n + ss - 1) / ss;
+        pc = (pc + 7) / 8 * 8;
+        const int nc = (int)((n + pc - 1) / pc);
+        dim3 tgrid((unsigned)((n + kStreamRows - 1) / kStreamRows), (unsigned)nc);
+        const unsigned gr = (unsigned)((n + kReduceRows - 1) / kReduceRows);
+        if (kb == 16) {
+            k_bmmt<double, 16><<<tgrid, kBlock, 0, st>>>(A64, n, P, pc, part, skip);
+            k_bmm32_reduce<16, double><<<gr, kBlock, 0, st>>>(part, nc, n, nrhs, Y, ldy, skip);
+        } else {
+            k_bmmt<double, 64><<<tgrid, kBlock, 0, st>>>(A64, n, P, pc, part, skip);
+            k_bmm32_reduce<64, double><<<gr, kBlock, 0, st>>>(part, nc, n, nrhs, Y, ldy, skip);
+        }
+        return hipGetLastError();
+    }
     dim3 grid((unsigned)blocks, (unsigned)ns);
     if (kb == 16)
         k_bmm64<16><<<grid, kBlock, 0, st>>>(A64, n, P, per, part, skip);

@@ -310,6 +310,10 @@ int kem_ctx_create(int device, kem_ctx** out)
         return KEM_E_HIP;
     }
     c->stream = c->own;
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cus > 0)
+        c->tuning.cus = cus;
+    (void)hipGetLastError();
     if (device < kMaxDevices) {
         // the device's copy of the no-allocation scratch, once per device, with it current
         std::lock_guard<std::mutex> lock(g_scratch_mu);
@@ -388,8 +392,9 @@ int kem_ctx_set_tuning(kem_ctx* c, int param, int64_t value)
         c->tuning.rh_groups = value;
         return KEM_OK;
     case KEM_TUNE_BMM_STREAM_BLOCKS:
+    case KEM_TUNE_BMM_TILE_BLOCKS:
         if (value < 1 || value > (1 << 20)) return fail(c, KEM_E_ARG, "tuning: block count out of [1, 2^20]");
-        c->tuning.bmm_stream_blocks = value;
+        (param == KEM_TUNE_BMM_STREAM_BLOCKS ? c->tuning.bmm_stream_blocks : c->tuning.bmm_tile_blocks) = value;
         return KEM_OK;
     default:
         return fail(c, KEM_E_ARG, "tuning: unknown parameter");
@@ -408,6 +413,7 @@ int kem_ctx_get_tuning(const kem_ctx* c, int param, int64_t* value)
     case KEM_TUNE_BMM_ROWS: *value = c->tuning.bmm_rows; return KEM_OK;
     case KEM_TUNE_ROBIN_HOOD_GROUPS: *value = c->tuning.rh_groups; return KEM_OK;
     case KEM_TUNE_BMM_STREAM_BLOCKS: *value = c->tuning.bmm_stream_blocks; return KEM_OK;
+    case KEM_TUNE_BMM_TILE_BLOCKS: *value = c->tuning.bmm_tile_blocks; return KEM_OK;
     default: return KEM_E_ARG;
     }
 }
@@ -1639,7 +1645,6 @@ int kem_assemble_vjp(kem_ctx* c, const kem_mesh* m, const double* lambda, const 
 namespace {
 
 // panel width of the block product for nrhs right-hand sides
-int panel_cols(int nrhs, int prec) { return prec != kem::kBmmFp64 ? (nrhs <= 32 ? 32 : 64) : (nrhs <= 16 ? 16 : 64); }
 
 // Y = A X for up to 64 right-hand sides (device, column-major with leading dimension n)
 // the block-product precision of a solve precision: FP64 matrix, FP32 matrix or BF16 matrix
@@ -1701,7 +1706,7 @@ int gmres_batch(kem_matrix* A, int prec, int nrhs, const double* d_B, double* d_
     if (restart > 0) kmax = std::min(kmax, restart);
     if (kmax < 1) kmax = 1;
     const int ldh = kmax + 2;
-    const int kp = panel_cols(nrhs, prec);
+    const int kp = kem::bmm_panel_cols(n, nrhs, prec);
     const int64_t vstride = (int64_t)(kmax + 1) * n;
     // the panel's rows past n (BF16: the last group of 8) stay zero
     const size_t panel_bytes = prec == kem::kBmmBf16 ? sizeof(uint16_t) * (size_t)((n + 7) / 8) * 8 * kp

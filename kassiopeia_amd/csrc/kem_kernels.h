@@ -122,6 +122,8 @@ struct LaunchTuning {
     int64_t gemv_blocks = 2048;
     int64_t bmm_blocks = 2560;
     int64_t bmm_stream_blocks = 480;  // workgroups of the LDS-streamed 64-RHS block products
+    int64_t bmm_tile_blocks = 4096;   // most workgroups of the 16 x 16-tile products (FP32 <= 32 RHS, FP64)
+    int64_t cus = 256;                // compute units of the device (set from the device at context creation)
     int64_t near_blocks = 384;  // grid of the near pass beside the mixed far pass (0: one block per point block)
     int64_t bmm_rows = 64;      // rows per wavefront tile of the FP32 block product (64, or 128 when n % 4 == 0)
     int64_t rh_groups = 0;      // workgroups of the Robin Hood sub-iterations (0: by size, 1: one)
@@ -191,11 +193,15 @@ hipError_t launch_robin_hood(hipStream_t st, const double* A, const double* b, d
                              int64_t steps, double* rnorm, double* slots, unsigned int* flags);
 
 // ---- multi-right-hand-side kernels (kem_batch.hip)
-// Block-product precisions: FP64 matrix (VALU), FP32 matrix (FP32 MFMA), BF16 matrix (BF16 MFMA),
-// all accumulating in FP32 (MFMA) or FP64 (VALU) and summing the split-K partials in FP64.
+// Block-product precisions: FP64 matrix (FP64 MFMA; the VALU when n is odd), FP32 matrix (FP32
+// MFMA), BF16 matrix (BF16 MFMA), each accumulating in its matrix's precision (BF16: FP32) and
+// summing the split-K partials in FP64.
 constexpr int kBmmFp64 = 0, kBmmFp32 = 1, kBmmBf16 = 2;
 // Workspace of launch_bmm / launch_bmm16 for nrhs <= 64 right-hand sides.
 size_t bmm_workspace_bytes(int64_t n, int nrhs, int prec);
+// panel columns of a block product of nrhs right-hand sides (16, 32 or 64): the row stride of the
+// FP32 / FP64 panels and the column count of the BF16 panel groups
+int bmm_panel_cols(int64_t n, int nrhs, int prec);
 // BF16 matrix storage of an n x n matrix: uint16_t elements (columns padded to a multiple of 8),
 // packed from the FP64 (or, when A64 is null, FP32) column-major matrix: A16[(g*n + i)*8 + m] =
 // A(i, 8g + m), each entry rounded to FP32 and then to BF16 to nearest even.

@@ -50,6 +50,8 @@ def test_tune_field(setup, precision):
 def test_tune_products(setup):
     from kassiopeia_amd import autotune as T
 
+    from kassiopeia_amd import kemfield as K
+
     ctx, m, cont, A, x = setup
     v = np.random.default_rng(2).normal(size=A.n)
     ref = A.Multiply(v)
@@ -60,25 +62,45 @@ def test_tune_products(setup):
         assert np.linalg.norm(A.Multiply(v) - ref) <= 1e-14 * np.linalg.norm(ref)
     X = np.random.default_rng(3).normal(size=(A.n, 40))
     ref = A.MultiplyBatch(X, "mixed")
-    # 40 right-hand sides on this FP32 matrix (n % 4 == 0) stream through LDS: their workgroup
-    # target is the knob swept; 7 take the register-loaded kernels (tile and split target)
+    ref64 = A.MultiplyBatch(X[:, :20], "fp64")
+    # on this FP32 matrix (n % 4 == 0) 40 right-hand sides stream through LDS on 32 x 32 tiles
+    # and 7 on 16 x 16 tiles, as the FP64 matrix (n even) does: each sweeps its workgroup target
     r = T.tune_block_product(A, nrhs=40, repeats=1, candidates=(60, 480))
     assert r["knob"] == "bmm_stream_blocks" and T.get(ctx, "bmm_stream_blocks") == r["best"]
     assert set(r["ms"]) == {60, 480}
-    r = T.tune_block_product(A, nrhs=7, repeats=1, candidates=(320, 2560))
-    assert r["knob"] == "bmm_blocks" and T.get(ctx, "bmm_blocks") == r["best"]
-    assert T.get(ctx, "bmm_rows") == r["rows"]["best"] and set(r["rows"]["ms"]) == {64, 128}
-    # both wavefront tiles, every split target: the product of the FP32 matrix to FP32 accuracy
-    for rows in (64, 128):
-        T.set_knob(ctx, "bmm_rows", rows)
-        for nrhs in (7, 40):
+    r = T.tune_block_product(A, nrhs=7, repeats=1, candidates=(256, 2048))
+    assert r["knob"] == "bmm_tile_blocks" and T.get(ctx, "bmm_tile_blocks") == r["best"]
+    assert set(r["ms"]) == {256, 2048}
+    r = T.tune_block_product(A, nrhs=5, precision="fp64", repeats=1, candidates=(256, 2048))
+    assert r["knob"] == "bmm_tile_blocks" and set(r["ms"]) == {256, 2048}
+    # every workgroup target: the product of the FP32 matrix to FP32 accuracy, FP64 to FP64's
+    for c in (1, 64, 10240):
+        T.set_knob(ctx, "bmm_stream_blocks", c)
+        T.set_knob(ctx, "bmm_tile_blocks", c)
+        for nrhs in (7, 20, 40):
             R = A.MultiplyBatch(X[:, :nrhs], "mixed")
-            assert np.linalg.norm(R - ref[:, :nrhs]) <= 1e-6 * np.linalg.norm(ref[:, :nrhs]), (rows, nrhs)
+            assert np.linalg.norm(R - ref[:, :nrhs]) <= 1e-6 * np.linalg.norm(ref[:, :nrhs]), (c, nrhs)
+        R = A.MultiplyBatch(X[:, :20], "fp64")
+        assert np.linalg.norm(R - ref64) <= 1e-14 * np.linalg.norm(ref64), c
+    # the register-loaded kernels (an FP32 matrix with n % 4 != 0): split target and tile
+    sub = G.Mesh(m.shapes[:1278].copy(), m.recs[:1278].copy(), m.bc_type[:1278].copy(), m.bc_value[:1278].copy(),
+                 m.bc_flux[:1278].copy(), m.group[:1278].copy())
+    cont2 = K.KSurfaceContainer(ctx, sub)
+    A2 = K.KBoundaryIntegralMatrix(cont2, fp64=False, fp32=True)
+    try:
+        X2 = X[:1278, :7]
+        ref2 = A2.MultiplyBatch(X2, "mixed")
+        r = T.tune_block_product(A2, nrhs=7, repeats=1, candidates=(320, 2560))
+        assert r["knob"] == "bmm_blocks" and T.get(ctx, "bmm_blocks") == r["best"]
+        assert T.get(ctx, "bmm_rows") == r["rows"]["best"] and set(r["rows"]["ms"]) == {64}
         for c in (64, 10240):
             T.set_knob(ctx, "bmm_blocks", c)
-            T.set_knob(ctx, "bmm_stream_blocks", c)
-            assert np.linalg.norm(A.MultiplyBatch(X, "mixed") - ref) <= 1e-6 * np.linalg.norm(ref), (rows, c)
-    T.apply(ctx, {"gemv_blocks": 2048, "bmm_blocks": 2560, "bmm_rows": 64, "bmm_stream_blocks": 480})
+            assert np.linalg.norm(A2.MultiplyBatch(X2, "mixed") - ref2) <= 1e-6 * np.linalg.norm(ref2), c
+    finally:
+        A2.close()
+        cont2.close()
+    T.apply(ctx, {"gemv_blocks": 2048, "bmm_blocks": 2560, "bmm_rows": 64, "bmm_stream_blocks": 480,
+                  "bmm_tile_blocks": 4096})
 
 
 def test_tuning_table_roundtrip_and_errors(setup, tmp_path):
@@ -88,11 +110,12 @@ def test_tuning_table_roundtrip_and_errors(setup, tmp_path):
     ctx = setup[0]
     table = {"far_range_fp64": 1280, "far_range_mixed": {"best": 5120, "ms": {}}, "gemv_blocks": 1024,
              "bmm_blocks": 2560, "near_blocks": 256, "bmm_rows": 128, "robin_hood_groups": 1,
-             "bmm_stream_blocks": 240}
+             "bmm_stream_blocks": 240, "bmm_tile_blocks": 2048}
     T.save(table, tmp_path / "t.json")
     loaded = T.load(tmp_path / "t.json")
     assert loaded == {"far_range_fp64": 1280, "far_range_mixed": 5120, "gemv_blocks": 1024, "bmm_blocks": 2560,
-                      "near_blocks": 256, "bmm_rows": 128, "robin_hood_groups": 1, "bmm_stream_blocks": 240}
+                      "near_blocks": 256, "bmm_rows": 128, "robin_hood_groups": 1, "bmm_stream_blocks": 240,
+                      "bmm_tile_blocks": 2048}
     T.apply(ctx, loaded)
     assert T.current(ctx) == loaded
     with pytest.raises(_lib.KemError):
@@ -107,8 +130,10 @@ def test_tuning_table_roundtrip_and_errors(setup, tmp_path):
         T.set_knob(ctx, "robin_hood_groups", 2)
     with pytest.raises(_lib.KemError):
         T.set_knob(ctx, "bmm_stream_blocks", 0)
+    with pytest.raises(_lib.KemError):
+        T.set_knob(ctx, "bmm_tile_blocks", 0)
     T.apply(ctx, {"far_range_fp64": 2560, "far_range_mixed": 2560, "gemv_blocks": 2048, "near_blocks": 384,
-                  "bmm_rows": 64, "robin_hood_groups": 0, "bmm_stream_blocks": 480})
+                  "bmm_rows": 64, "robin_hood_groups": 0, "bmm_stream_blocks": 480, "bmm_tile_blocks": 4096})
 
 
 def test_tune_near(setup):

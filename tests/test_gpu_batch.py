@@ -98,6 +98,49 @@ def test_block_product_streamed_shapes(ctx, n):
         cont.close()
 
 
+@pytest.mark.parametrize("n", [1352, 1350, 1349])
+def test_block_product_tile_shapes(ctx, n):
+    """Block products on 16 x 16 MFMA tiles streamed through LDS: FP32 matrices with n % 4 == 0
+    and up to 32 right-hand sides (16-column panels up to 16, 32 columns above), FP64 matrices
+    with n even (16 or 64 columns); the other shapes (n = 1,350 FP32, n = 1,349 both) take the
+    register-loaded kernels.  Chunks of 16 FP32 or 8 FP64 columns; the workgroup target 1 gives
+    one column split, 1,024 and 4,096 more.  Exact check: unit right-hand sides pick columns of
+    the stored matrix bit for bit.  Random right-hand sides against the FP64 product of the
+    rounded operands within the recursive-summation bound (n + 2) u of the sum of |terms|
+    (u = 2^-24 for FP32, 2^-53 for FP64)."""
+    from kassiopeia_amd import autotune as T
+    from kassiopeia_amd import kemfield as K
+
+    m = _first_elements(G.coax_mesh(nphi_in=12, nphi_out=24, nz=16, ncap_r=2, ncap_phi=25), n)
+    cont = K.KSurfaceContainer(ctx, m)
+    A = K.KBoundaryIntegralMatrix(cont, fp64=True, fp32=True)
+    Ah = A.to_numpy()
+    rounded = {"mixed": Ah.astype(np.float32).astype(np.float64), "fp64": Ah}
+    unit = {"mixed": 2.0 ** -24, "fp64": 2.0 ** -53}
+    cols = np.linspace(0, n - 1, 64).astype(int)
+    E = np.zeros((n, 64))
+    E[cols, np.arange(64)] = 1.0
+    X = np.random.default_rng(n + 7).normal(size=(n, 64))
+    Xr = {"mixed": X.astype(np.float32).astype(np.float64), "fp64": X}
+    cases = [("mixed", k) for k in (2, 5, 16, 17, 32)] + [("fp64", k) for k in (1, 5, 16, 17, 64)]
+    try:
+        for blocks in (1, 1024, 4096):
+            T.set_knob(ctx, "bmm_tile_blocks", blocks)
+            for prec, k in cases:
+                print(f"tile product: n {n} blocks {blocks} k {k} {prec}", flush=True)
+                Y = A.MultiplyBatch(E[:, :k], prec)
+                assert np.array_equal(Y, rounded[prec][:, cols[:k]]), (prec, blocks, k)
+                Y = A.MultiplyBatch(X[:, :k], prec)
+                Ar, Xk = rounded[prec], Xr[prec][:, :k]
+                # FP64: the reference product's own rounding doubles the bound
+                bound = (n + 2) * unit[prec] * (2 if prec == "fp64" else 1)
+                assert np.all(np.abs(Y - Ar @ Xk) <= bound * (np.abs(Ar) @ np.abs(Xk))), (prec, blocks, k)
+    finally:
+        T.set_knob(ctx, "bmm_tile_blocks", 4096)
+        A.close()
+        cont.close()
+
+
 @pytest.mark.parametrize("k", [1, 7, 32, 33, 64, 70])
 def test_block_product_fp32_mfma(coax, k):
     """FP32 MFMA product against the FP64 product of the FP32-rounded operands: FP32 products
