@@ -96,8 +96,8 @@ def main():
     ap.add_argument("--far-range", type=int, default=-1,
                     help="--field-only: elements per far-pass element range (-1: the library's default)")
     ap.add_argument("--block-products", action="store_true",
-                    help="only the block products of 64 right-hand sides on the 20,000-triangle coax matrix "
-                         "(FP32 and BF16 storage), steps x each, for profiling")
+                    help="only block products on the 20,000-triangle coax matrix (64 right-hand sides on "
+                         "FP32 and BF16 storage, 16 on FP32, 16 and 64 on FP64), steps x each, for profiling")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 1 GiB device to device (reference kernel for the rocprofv3 byte counters)")
     args = ap.parse_args()
@@ -125,25 +125,26 @@ def main():
 
 
 def block_products(ctx, args):
-    """steps block products of 64 right-hand sides on the 20,000-triangle coax matrix, FP32 then
-    BF16 storage, at the library's default launch shapes: the command the rocprofv3 summaries of
-    the streamed block-product kernels run."""
+    """steps block products on the 20,000-triangle coax matrix at the library's default launch
+    shapes: 64 right-hand sides on the FP32 and the BF16 storage (32 x 32 MFMA tiles), then 16 on
+    the FP32 storage and 16 and 64 on the FP64 storage (16 x 16 MFMA tiles): the command the
+    rocprofv3 summaries of the streamed block-product kernels run."""
     from kassiopeia_amd import geometry as G
     from kassiopeia_amd import kemfield as K
 
     mesh = G.coax_mesh()
     cont = K.KSurfaceContainer(ctx, mesh)
-    A = K.KBoundaryIntegralMatrix(cont, fp64=False, fp32=True, bf16=True)
+    A = K.KBoundaryIntegralMatrix(cont, fp64=True, fp32=True, bf16=True)
     X = np.random.default_rng(9).normal(size=(mesh.n, 64))
-    out = {"block_products": mesh.n, "nrhs": 64}
-    for prec in ("mixed", "bf16"):
-        A.MultiplyBatch(X, prec)
+    out = {"block_products": mesh.n}
+    for prec, k in (("mixed", 64), ("bf16", 64), ("mixed", 16), ("fp64", 16), ("fp64", 64)):
+        A.MultiplyBatch(X[:, :k], prec)
         _timing(ctx, True)
         for _ in range(args.steps):
-            A.MultiplyBatch(X, prec)
+            A.MultiplyBatch(X[:, :k], prec)
         kt, kn = _kernel_time(ctx)
         _timing(ctx, False)
-        out[prec + "_ms"] = kt / max(kn, 1)
+        out[f"{prec}_{k}_ms"] = kt / max(kn, 1)
     A.close()
     cont.close()
     ctx.close()
@@ -490,6 +491,17 @@ def run_bore(ctx, args):
             f"worst {res.max():.1e})")
         out.update(batch16_solve_s=stb.seconds, batch16_iterations=stb.iterations,
                    batch16_worst_residual=float(res.max()))
+        # its block products: 16 right-hand sides on the 40 GB FP32 and the 80 GB FP64 matrix
+        for prec, key, bpe in (("mixed", "fp32", 4), ("fp64", "fp64", 8)):
+            _timing(ctx, True)
+            for _ in range(3):
+                A.MultiplyBatch(Xb, prec)
+            kt, kn = _kernel_time(ctx)
+            _timing(ctx, False)
+            ms = kt / max(kn, 1)
+            gbs = bpe * N * N / (ms * 1e-3) / 1e9
+            log(f"  block product of the 16 sets, {key} matrix: {ms:.2f} ms = {gbs:.0f} GB/s of matrix")
+            out.update({f"batch16_bmm_ms_{key}": ms, f"batch16_bmm_gbs_{key}": gbs})
         # adjoint of L = mean (phi(axis) - target)^2 over 10k axial points
         zax = np.linspace(-3.5, 3.5, 10000)
         pax = np.ascontiguousarray(np.stack([0 * zax, 0 * zax, zax], 1))
@@ -685,6 +697,22 @@ def run_coax(ctx, args):
     # (k_bmm16), kernel time from HIP events; and the 64 sets in lockstep on the BF16 matrix
     Ab = K.KBoundaryIntegralMatrix(cont, fp64=True, fp32=False, bf16=True)
     Xr = np.random.default_rng(9).normal(size=(N, 64))
+    # 16 right-hand sides on the FP32 matrix and 16 / 64 on the FP64 matrix: the streamed
+    # 16 x 16-tile products (k_bmmt), the matrix read once per call
+    for prec, key, bpe, k in (("mixed", "fp32", 4, 16), ("fp64", "fp64", 8, 16), ("fp64", "fp64", 8, 64)):
+        A.MultiplyBatch(Xr[:, :k], prec)
+        _timing(ctx, True)
+        for _ in range(5):
+            A.MultiplyBatch(Xr[:, :k], prec)
+        kt, kn = _kernel_time(ctx)
+        _timing(ctx, False)
+        ms = kt / max(kn, 1)
+        gbs = bpe * N * N / (ms * 1e-3) / 1e9
+        tfl = 2.0 * N * N * k / (ms * 1e-3) / 1e12
+        log(f"block product, {k} right-hand sides, {key} matrix (16 x 16 tiles): {ms:.3f} ms = {gbs:.0f} GB/s of "
+            f"matrix, {tfl:.1f} TFLOP/s")
+        out.update({f"coax20k_bmm{k}_ms_{key}": ms, f"coax20k_bmm{k}_gbs_{key}": gbs,
+                    f"coax20k_bmm{k}_tflops_{key}": tfl})
     for mat, prec, key, bpe in ((A, "mixed", "fp32", 4), (Ab, "bf16", "bf16", 2)):
         if not args.no_autotune:
             # the split-K target (and for FP32 the wavefront tile) tuned per matrix precision

@@ -1731,6 +1731,7 @@ int gmres_batch(kem_matrix* A, int prec, int nrhs, const double* d_B, double* d_
     const size_t opanel = carve(panel_bytes);
     const size_t oints = carve(sizeof(int) * (4 * (size_t)nrhs + 4));
     const size_t ows = carve(wsb);
+    const size_t oproj = carve(sizeof(double) * (size_t)kem::batch_proj_doubles(n, nrhs, ldh));
     if (A->bws_bytes < off) {
         if (A->bws) (void)hipFree(A->bws);
         A->bws = nullptr;
@@ -1784,6 +1785,7 @@ int gmres_batch(kem_matrix* A, int prec, int nrhs, const double* d_B, double* d_
     a.jr = a.done + nrhs;
     a.cnt = a.jr + nrhs;
     a.ints = a.cnt + nrhs;
+    a.proj = (double*)(base + oproj);
 
     std::vector<double> bnorm(nrhs), hp((size_t)ldh * nrhs, 0.), hh1(nrhs);
     std::vector<int> active(nrhs), done(nrhs, 0), ones(nrhs, 1);

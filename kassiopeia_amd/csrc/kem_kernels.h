@@ -256,7 +256,10 @@ struct BatchArnoldi {
     double *bnorm = nullptr, *tol = nullptr, *resid = nullptr;  // nrhs
     int *active = nullptr, *done = nullptr, *jr = nullptr, *cnt = nullptr;  // nrhs
     int* ints = nullptr;  // [0] j, [1] stop, [2] jlimit, [3] steps
+    double* proj = nullptr;  // projection partials, batch_proj_doubles(n, nrhs, ldh)
 };
+// partial sums of the Arnoldi projections: row chunks x nrhs x ldh doubles
+int64_t batch_proj_doubles(int64_t n, int nrhs, int ldh);
 hipError_t launch_batch_arnoldi_step(hipStream_t st, const BatchArnoldi& a);
 // Y (ldh x nrhs) = the least-squares coefficients of every right-hand side; cnt = jr
 hipError_t launch_batch_trisolve(hipStream_t st, const BatchArnoldi& a, double* Y);
