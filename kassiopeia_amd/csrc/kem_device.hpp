@@ -401,6 +401,29 @@ __device__ __forceinline__ F4 element_far(const double* __restrict__ g, const do
     return cubature<33, FIELD>(nd + 4 * KEM_N33, pref, px, py, pz);
 }
 
+// element_far with one rule for the whole wavefront: the most nodes the reference's ladder
+// picks for any active lane (the lanes of near pairs are inactive here).  The lanes of a
+// wavefront straddling a cut-off then run one rule instead of both; a lane given more nodes than
+// its own rule gets the same integral to within the rules' truncation, below 1e-15 relative
+// beyond their cut-offs.  Used by the assembly, whose census counts 16.6 node evaluations per
+// lane and pair when each lane runs its own rule, 14.9 with one rule per wavefront, 13.6 useful.
+template <bool FIELD = true>
+__device__ __forceinline__ F4 element_far_wave(const double* __restrict__ g, const double* __restrict__ nd, int type,
+                                               double r2, double px, double py, double pz)
+{
+    const double pref = g[KG_PREF];
+    if (type == KEM_LINESEGMENT) {
+        if (__builtin_amdgcn_ballot_w64(!(r2 > g[KG_REF7_2])) == 0)
+            return line_quadrature<2, FIELD>(nd + 4 * KEM_L4, pref, px, py, pz);
+        return line_quadrature<8, FIELD>(nd + 4 * KEM_L16, pref, px, py, pz);
+    }
+    if (__builtin_amdgcn_ballot_w64(!(r2 > g[KG_REF12_2])) != 0)
+        return cubature<33, FIELD>(nd + 4 * KEM_N33, pref, px, py, pz);
+    if (__builtin_amdgcn_ballot_w64(!(r2 > g[KG_REF7_2])) != 0)
+        return cubature<12, FIELD>(nd + 4 * KEM_N12, pref, px, py, pz);
+    return cubature<7, FIELD>(nd + 4 * KEM_N7, pref, px, py, pz);
+}
+
 // Near pass of one pair (closed forms).
 __device__ __forceinline__ F4 element_near(const double* __restrict__ g, int type, double px, double py, double pz)
 {

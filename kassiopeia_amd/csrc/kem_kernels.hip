@@ -692,7 +692,11 @@ __global__ __launch_bounds__(kBlock) void k_assemble(const double* __restrict__ 
 
 // Far entries: lane = collocation target i, wave-uniform source column j, as k_assemble, but
 // the closed-form entries are left to k_assemble_near (no closed form inlined here: the kernel
-// keeps the far field kernels' register budget).  KElectrostaticBoundaryIntegrator::
+// keeps the far field kernels' register budget), and the far lanes of a wavefront share one
+// cubature rule per column (dev::element_far_wave: 112.9 -> 108 ms on the 100,000-triangle
+// assembly).  Measured and not taken: potential nodes stored relative to the element centre as
+// (-2 q', |q'|^2, w), which cuts r^2 from 6 to 4 operations per node but needs 5 scalar
+// values per node instead of 4 (120 ms at an unroll of 4, 149 ms at 16, SGPR spills unrolled).  KElectrostaticBoundaryIntegrator::
 // ComputeBoundaryIntegral (Core/include/KElectrostaticBoundaryIntegrator.hh:260-280).
 template <bool FIELD>
 __global__ __launch_bounds__(kBlock) void k_assemble_far(const double* __restrict__ geom,
@@ -715,7 +719,7 @@ __global__ __launch_bounds__(kBlock) void k_assemble_far(const double* __restric
         const double* g = geom + j * KG_STRIDE;
         const double r2 = dev::dist2(g, cx, cy, cz);
         if (dev::is_near2(g, r2, 0)) continue;
-        const F4 v = dev::element_far<FIELD>(g, nodes + j * KEM_NODE_STRIDE, (int)g[KG_TYPE], r2, cx, cy, cz);
+        const F4 v = dev::element_far_wave<FIELD>(g, nodes + j * KEM_NODE_STRIDE, (int)g[KG_TYPE], r2, cx, cy, cz);
         // far entries lie beyond 2 element sizes: never the Neumann self term
         const double a = (!FIELD || dirichlet) ? v.p : (v.x * nx + v.y * ny + v.z * nz);
         if (live) {
