@@ -424,6 +424,56 @@ __device__ __forceinline__ F4 element_far_wave(const double* __restrict__ g, con
     return cubature<7, FIELD>(nd + 4 * KEM_N7, pref, px, py, pz);
 }
 
+// The potential of cubature<NQ, false> at T points per lane from one pass over the nodes.
+template <int NQ, int T>
+__device__ __forceinline__ void cubature_pot_n(const double* __restrict__ nd, double pref, const double (&px)[T],
+                                               const double (&py)[T], const double (&pz)[T], double (&v)[T])
+{
+    double s[T];
+#pragma unroll
+    for (int t = 0; t < T; t++) s[t] = 0.;
+#pragma unroll
+    for (int g = 0; g < NQ; g++) {
+        const double qx = nd[4 * g + 0], qy = nd[4 * g + 1], qz = nd[4 * g + 2], w = nd[4 * g + 3];
+#pragma unroll
+        for (int t = 0; t < T; t++) {
+            const double dx = px[t] - qx, dy = py[t] - qy, dz = pz[t] - qz;
+            s[t] += w * rsqrt_acc(dx * dx + dy * dy + dz * dz);
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < T; t++) v[t] = pref * s[t];
+}
+
+// element_far_wave's potential at T points per lane (squared distances r2[t] from the element
+// centre; a point takes part in the rule choice only if its pair is far, far[t])
+template <int T>
+__device__ __forceinline__ void element_far_wave_pot_n(const double* __restrict__ g, const double* __restrict__ nd,
+                                                       int type, const double (&r2)[T], const bool (&far)[T],
+                                                       const double (&px)[T], const double (&py)[T],
+                                                       const double (&pz)[T], double (&v)[T])
+{
+    const double pref = g[KG_PREF];
+    auto needs = [&](double cut) {
+        bool b = false;
+#pragma unroll
+        for (int t = 0; t < T; t++) b = b || (far[t] && !(r2[t] > cut));
+        return __builtin_amdgcn_ballot_w64(b) != 0;
+    };
+    if (type == KEM_LINESEGMENT) {
+        const bool l16 = needs(g[KG_REF7_2]);
+#pragma unroll
+        for (int t = 0; t < T; t++)
+            v[t] = (l16 ? line_quadrature<8, false>(nd + 4 * KEM_L16, pref, px[t], py[t], pz[t])
+                        : line_quadrature<2, false>(nd + 4 * KEM_L4, pref, px[t], py[t], pz[t]))
+                       .p;
+        return;
+    }
+    if (needs(g[KG_REF12_2])) return cubature_pot_n<33, T>(nd + 4 * KEM_N33, pref, px, py, pz, v);
+    if (needs(g[KG_REF7_2])) return cubature_pot_n<12, T>(nd + 4 * KEM_N12, pref, px, py, pz, v);
+    return cubature_pot_n<7, T>(nd + 4 * KEM_N7, pref, px, py, pz, v);
+}
+
 // Near pass of one pair (closed forms).
 __device__ __forceinline__ F4 element_near(const double* __restrict__ g, int type, double px, double py, double pz)
 {

@@ -729,6 +729,54 @@ __global__ __launch_bounds__(kBlock) void k_assemble_far(const double* __restric
     }
 }
 
+// Far entries of Dirichlet rows with T targets per lane: wavefront w of a block takes rows
+// 64 T w .. 64 T (w + 1) - 1 of the block's 256 T, lane l rows l + 64 t, so each wave-uniform
+// node load of a column serves T evaluations.
+template <int T>
+__global__ __launch_bounds__(kBlock) void k_assemble_far_pot(const double* __restrict__ geom,
+                                                             const double* __restrict__ nodes,
+                                                             const double* __restrict__ targets, int64_t n,
+                                                             int64_t per_split, double* __restrict__ A64,
+                                                             float* __restrict__ A32)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int64_t i0 = (int64_t)blockIdx.x * (T * kBlock) + wave * (64 * T) + lane;
+    int64_t row[T];
+    bool live[T];
+    double px[T], py[T], pz[T];
+#pragma unroll
+    for (int t = 0; t < T; t++) {
+        row[t] = i0 + 64 * t;
+        live[t] = row[t] < n;
+        if (!live[t]) row[t] = n - 1;
+        px[t] = targets[row[t] * KT_STRIDE + KT_CX];
+        py[t] = targets[row[t] * KT_STRIDE + KT_CY];
+        pz[t] = targets[row[t] * KT_STRIDE + KT_CZ];
+    }
+    const int64_t j0 = (int64_t)blockIdx.y * per_split;
+    int64_t j1 = j0 + per_split;
+    if (j1 > n) j1 = n;
+    for (int64_t j = j0; j < j1; j++) {
+        const double* g = geom + j * KG_STRIDE;
+        double r2[T], v[T];
+        bool far[T], any = false;
+#pragma unroll
+        for (int t = 0; t < T; t++) {
+            r2[t] = dev::dist2(g, px[t], py[t], pz[t]);
+            far[t] = !dev::is_near2(g, r2[t], 0);
+            any = any || far[t];
+        }
+        if (__builtin_amdgcn_ballot_w64(any) == 0) continue;
+        dev::element_far_wave_pot_n<T>(g, nodes + j * KEM_NODE_STRIDE, (int)g[KG_TYPE], r2, far, px, py, pz, v);
+#pragma unroll
+        for (int t = 0; t < T; t++)
+            if (live[t] && far[t]) {
+                if (A64) A64[j * n + row[t]] = v[t];
+                if (A32) A32[j * n + row[t]] = (float)v[t];
+            }
+    }
+}
+
 // Closed-form entries: targets in the Morton order of the element centres (order[k]), so a
 // wavefront's targets are compact; each wavefront bounds its targets by a sphere and skips the
 // element clusters that cannot hold a near pair of any of them (the culling of
@@ -1651,6 +1699,10 @@ hipError_t launch_pairs(hipStream_t st, int policy, const double* geom, const do
 }
 
 namespace {
+// targets per lane of the Dirichlet cubature kernel (100,000-triangle assembly, both kernels:
+// 108 ms at one target per lane, 100-102 ms at 2, 106 ms at 3, 108 ms at 4)
+constexpr int kAsmTargets = 2;
+
 template <bool G>
 hipError_t assemble_kernels(hipStream_t st, int policy, bool field, const double* geom, const double* nodes,
                             const double* recs, const double* targets, int64_t n, double* A64, float* A32,
@@ -1679,7 +1731,14 @@ hipError_t assemble_kernels(hipStream_t st, int policy, bool field, const double
             k_assemble_far<true><<<grid, kBlock, 0, st>>>(geom, nodes, targets, n, per2, A64, A32);
             k_assemble_near<true><<<gx, kBlock, 0, st>>>(geom, targets, clusters, ncl, order, n, A64, A32);
         } else {
-            k_assemble_far<false><<<grid, kBlock, 0, st>>>(geom, nodes, targets, n, per2, A64, A32);
+            // kAsmTargets targets per lane: fewer row blocks, more column splits
+            constexpr int TT = kAsmTargets;
+            const unsigned gx2 = (unsigned)((n + TT * kBlock - 1) / (TT * kBlock));
+            int64_t sp2 = (32768 + gx2 - 1) / gx2;
+            if (sp2 > n) sp2 = n;
+            const int64_t per3 = (n + sp2 - 1) / sp2;
+            sp2 = (n + per3 - 1) / per3;
+            k_assemble_far_pot<TT><<<dim3(gx2, (unsigned)sp2), kBlock, 0, st>>>(geom, nodes, targets, n, per3, A64, A32);
             k_assemble_near<false><<<gx, kBlock, 0, st>>>(geom, targets, clusters, ncl, order, n, A64, A32);
         }
         return hipGetLastError();
