@@ -49,11 +49,14 @@ PEAK_HBM_GBS = 8000.0
 # same host type: 3.68 / 2.95 M point-element pairs/s (1,728-triangle cube, 4,096 points)
 CPU_ORACLE_OVER_REFERENCE = 3.68 / 2.95
 
-# HBM bytes per launch (one dispatch of 1,973,760 points, 16 element ranges) of the mixed far
-# kernel on the headline workload, from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs scaled by
-# the readings of a 1 GiB device copy (FETCH_SIZE x 1.99990, WRITE_SIZE x 1.0): 0.996 GB read,
-# 1.011 GB written (the 16 partial-sum slices).  Source: the profile named here.
-TRAFFIC = {"bytes": 2.006e9, "profile": "profiles/pmc_traffic_k_field_far32c_bore100k_10Mpts.md"}
+# HBM bytes per launch of the mixed far kernel on the headline workload, by element-range size
+# (which sets the dispatch: 640 -> 208,384 points x 160 ranges, 10,240 -> 1,973,760 points x 16
+# ranges), from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs scaled by the readings of a 1 GiB
+# device copy (FETCH_SIZE x 1.99990, WRITE_SIZE x 1.0): 640: 0.847 GB read, 1.067 GB written;
+# 10,240: 0.996 GB read, 1.011 GB written (the writes are the partial-sum slices of the ranges).
+# Source: the profile named here; other range sizes were not measured (traffic null).
+TRAFFIC = {"bytes": {640: 1.914e9, 10240: 2.006e9},
+           "profile": "profiles/pmc_traffic_k_field_far32c_bore100k_10Mpts.md"}
 
 METRIC = "point-element field evals/s + solve time, mixed vs FP64 at 1e-6 rel. error"
 
@@ -79,8 +82,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=1, help="untimed mixed pipelines before them")
     ap.add_argument("--points", type=int, default=10_000_000)
     ap.add_argument("--elements", type=int, default=100_000)
-    ap.add_argument("--tune-points", type=int, default=1_000_000,
-                    help="points of the sample the far-pass element ranges are tuned on")
+    ap.add_argument("--tune-points", type=int, default=0,
+                    help="points of the sample the far-pass element ranges are tuned on (0: all of them; "
+                         "a 1 M-point sample preferred 10,240-element ranges, which are the slowest at 10 M)")
     ap.add_argument("--cpu-points", type=int, default=0,
                     help="points of the bounded CPU-baseline sample; 0: 1,600 per thread")
     ap.add_argument("--cpu-threads", type=int, default=0, help="OpenMP threads of the CPU baseline; 0: all cores")
@@ -327,9 +331,13 @@ def run_bore(ctx, args):
     if not args.no_autotune:
         t0 = time.perf_counter()
         sig0 = np.full(N, 1e-9)  # the time per pair does not depend on the charge densities
-        sample = pts[:min(args.tune_points, M)]
-        cands = (0, 640, 2560, 10240)
-        tuning = {"far_range_mixed": T.tune_field(cont, sig0, sample, "mixed", repeats=2, candidates=cands),
+        sample = pts[:min(args.tune_points, M)] if args.tune_points > 0 else pts
+        # the same candidates for both precisions, on the whole point set: the chunking of the
+        # per-XCD element ranges depends on the point count (10 M points, s per pass: mixed
+        # {640: 2.10, 1280: 2.13, 2560: 2.14, 5120: 2.19, 10240: 2.19}, FP64 {640: 9.67,
+        # 1280: 9.70, 2560: 9.85, 5120: 9.98, 10240: 10.16})
+        cands = (640, 2560, 10240)
+        tuning = {"far_range_mixed": T.tune_field(cont, sig0, sample, "mixed", repeats=1, candidates=cands),
                   "far_range_fp64": T.tune_field(cont, sig0, sample, "fp64", repeats=1, candidates=cands)}
         log(f"autotune on {sample.shape[0]} points ({time.perf_counter() - t0:.1f} s): " +
             ", ".join(f"{k} {v['best']} {v['ms']}" for k, v in tuning.items()))
@@ -582,7 +590,7 @@ def run_bore(ctx, args):
         # of points), so this equals FLOPs per dispatch / average dispatch duration
         "roofline": {"bound": "valu", "compute": "vector ALU, fp32 (unpacked)", "achieved": achieved,
                      "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
-                     "traffic": TRAFFIC["bytes"] if (N == 100_000 and M == 10_000_000) else None,
+                     "traffic": TRAFFIC["bytes"].get(rng_used) if (N == 100_000 and M == 10_000_000) else None,
                      "traffic_unit": "HBM bytes per dispatch (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)",
                      "traffic_source": TRAFFIC["profile"], "kernel": "k_field_far32c", "kernel_ms_per_pass": far_ms,
                      "launches_per_pass": launches, "kernel_ms_per_launch": far_ms / launches,
