@@ -809,8 +809,10 @@ __global__ __launch_bounds__(kBlock) void k_bproj(const double* __restrict__ V, 
     if (threadIdx.x == 0) h[r * ldh + q] = s;
 }
 
-// Y_r += sign * sum_{q < cnt_r} V_r q * h[r*ldh + q]   (grid: rows x r)
-__global__ __launch_bounds__(kBlock) void k_bupdate(const double* __restrict__ V, int64_t vstride,
+// Y_r += sign * sum_{q < cnt_r} V_r q * h[r*ldh + q]   (grid: rows x r); the basis V in FP64,
+// or in FP32 for the inner solves of mixed precision (VT = float), summed in FP64
+template <typename VT>
+__global__ __launch_bounds__(kBlock) void k_bupdate(const VT* __restrict__ V, int64_t vstride,
                                                     const double* __restrict__ h, int ldh,
                                                     const int* __restrict__ cnt, double sign, int64_t n,
                                                     double* __restrict__ Y, const int* __restrict__ skip)
@@ -819,11 +821,11 @@ __global__ __launch_bounds__(kBlock) void k_bupdate(const double* __restrict__ V
     const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     const int r = blockIdx.y;
     if (i >= n) return;
-    const double* v = V + r * vstride + i;
+    const VT* v = V + r * vstride + i;
     const double* hr = h + r * ldh;
     const int m = cnt[r];
     double s = 0.;
-    for (int q = 0; q < m; q++) s += v[(int64_t)q * n] * hr[q];
+    for (int q = 0; q < m; q++) s += (double)v[(int64_t)q * n] * hr[q];
     Y[(int64_t)r * n + i] += sign * s;
 }
 
@@ -865,11 +867,11 @@ __device__ __forceinline__ uint16_t panel_value<uint16_t>(double v)
     return bf16_bits((float)v);
 }
 
-// V_r q = W_r * (1 / nrm_r); panel column r = V_r q for active r, 0 otherwise
-template <typename T>
+// V_r q = W_r * (1 / nrm_r); panel column r = V_r q for active r, 0 otherwise (the basis in VT)
+template <typename T, typename VT>
 __global__ __launch_bounds__(kBlock) void k_bnormalize(const double* __restrict__ W, const double* __restrict__ nrm,
                                                        int stride, const int* __restrict__ active, int64_t n,
-                                                       double* __restrict__ V, int64_t vstride, int q, int kp,
+                                                       VT* __restrict__ V, int64_t vstride, int q, int kp,
                                                        T* __restrict__ Vt)
 {
     const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
@@ -877,7 +879,7 @@ __global__ __launch_bounds__(kBlock) void k_bnormalize(const double* __restrict_
     if (i >= n) return;
     const double d = nrm[r * stride];
     const double v = active[r] ? W[(int64_t)r * n + i] * (1.0 / d) : 0.;
-    V[r * vstride + (int64_t)q * n + i] = v;
+    V[r * vstride + (int64_t)q * n + i] = (VT)v;
     Vt[panel_index<T>(i, r, kp)] = panel_value<T>(v);
 }
 
@@ -898,7 +900,8 @@ __global__ __launch_bounds__(kBlock) void k_baxpby(double a, const double* __res
 // block per (q, r), every q up to kmax launched: 42.6 us per pass for 64 right-hand sides of
 // 20,000 rows, most of its blocks returning at once.)
 constexpr int kProjRows = 1024;
-__global__ __launch_bounds__(kBlock) void k_bproj_part(const double* __restrict__ V, int64_t vstride,
+template <typename VT>
+__global__ __launch_bounds__(kBlock) void k_bproj_part(const VT* __restrict__ V, int64_t vstride,
                                                        const double* __restrict__ W, int64_t n,
                                                        double* __restrict__ part, int ldh,
                                                        const int* __restrict__ ints)
@@ -915,13 +918,13 @@ __global__ __launch_bounds__(kBlock) void k_bproj_part(const double* __restrict_
         const int64_t i = i0 + 64 * k;
         w[k] = i < n ? wr[i] : 0.;
     }
-    const double* vr = V + r * vstride;
+    const VT* vr = V + r * vstride;
     for (int q = wave; q <= j; q += 4) {
-        const double* v = vr + (int64_t)q * n;
+        const VT* v = vr + (int64_t)q * n;
         double s = 0.;
         for (int k = 0; k < K; k++) {
             const int64_t i = i0 + 64 * k;
-            if (i < n) s += v[i] * w[k];
+            if (i < n) s += (double)v[i] * w[k];
         }
         for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
         if (lane == 0) part[((int64_t)c * nrhs + r) * ldh + q] = s;
@@ -942,11 +945,11 @@ __global__ __launch_bounds__(64) void k_bproj_sum(const double* __restrict__ par
 }
 
 // V_r (j+1) = W_r / nrm_r; panel column r = that for active r, 0 otherwise
-template <typename T>
+template <typename T, typename VT>
 __global__ __launch_bounds__(kBlock) void k_bnormalize_idx(const double* __restrict__ W,
                                                            const double* __restrict__ nrm,
                                                            const int* __restrict__ active, int64_t n,
-                                                           double* __restrict__ V, int64_t vstride, int kp,
+                                                           VT* __restrict__ V, int64_t vstride, int kp,
                                                            T* __restrict__ Vt, const int* __restrict__ ints)
 {
     if (ints[1]) return;
@@ -954,7 +957,7 @@ __global__ __launch_bounds__(kBlock) void k_bnormalize_idx(const double* __restr
     const int r = blockIdx.y;
     if (i >= n) return;
     const double v = active[r] ? W[(int64_t)r * n + i] * (1.0 / nrm[r]) : 0.;
-    V[r * vstride + (int64_t)(ints[0] + 1) * n + i] = v;
+    V[r * vstride + (int64_t)(ints[0] + 1) * n + i] = (VT)v;
     Vt[panel_index<T>(i, r, kp)] = panel_value<T>(v);
 }
 
@@ -1300,7 +1303,15 @@ hipError_t launch_bupdate(hipStream_t st, const double* V, int64_t vstride, cons
                           double sign, int64_t n, int nrhs, double* Y)
 {
     if (nrhs <= 0) return hipSuccess;
-    k_bupdate<<<dim3(nblocks(n), (unsigned)nrhs), kBlock, 0, st>>>(V, vstride, h, ldh, cnt, sign, n, Y, nullptr);
+    k_bupdate<double><<<dim3(nblocks(n), (unsigned)nrhs), kBlock, 0, st>>>(V, vstride, h, ldh, cnt, sign, n, Y, nullptr);
+    return hipGetLastError();
+}
+
+hipError_t launch_bupdate(hipStream_t st, const float* V, int64_t vstride, const double* h, int ldh, const int* cnt,
+                          double sign, int64_t n, int nrhs, double* Y)
+{
+    if (nrhs <= 0) return hipSuccess;
+    k_bupdate<float><<<dim3(nblocks(n), (unsigned)nrhs), kBlock, 0, st>>>(V, vstride, h, ldh, cnt, sign, n, Y, nullptr);
     return hipGetLastError();
 }
 
@@ -1311,19 +1322,64 @@ hipError_t launch_bnorm(hipStream_t st, const double* W, int64_t n, int nrhs, do
     return hipGetLastError();
 }
 
-hipError_t launch_bnormalize(hipStream_t st, const double* W, const double* nrm, int stride, const int* active,
-                             int64_t n, int nrhs, double* V, int64_t vstride, int q, int kp, float* Vt32,
-                             double* Vt64, uint16_t* Vt16)
+namespace {
+template <typename VT>
+hipError_t bnormalize_any(hipStream_t st, const double* W, const double* nrm, int stride, const int* active,
+                          int64_t n, int nrhs, VT* V, int64_t vstride, int q, int kp, float* Vt32, double* Vt64,
+                          uint16_t* Vt16)
 {
     if (nrhs <= 0) return hipSuccess;
     dim3 grid(nblocks(n), (unsigned)nrhs);
     if (Vt16)
-        k_bnormalize<uint16_t><<<grid, kBlock, 0, st>>>(W, nrm, stride, active, n, V, vstride, q, kp, Vt16);
+        k_bnormalize<uint16_t, VT><<<grid, kBlock, 0, st>>>(W, nrm, stride, active, n, V, vstride, q, kp, Vt16);
     else if (Vt32)
-        k_bnormalize<float><<<grid, kBlock, 0, st>>>(W, nrm, stride, active, n, V, vstride, q, kp, Vt32);
+        k_bnormalize<float, VT><<<grid, kBlock, 0, st>>>(W, nrm, stride, active, n, V, vstride, q, kp, Vt32);
     else
-        k_bnormalize<double><<<grid, kBlock, 0, st>>>(W, nrm, stride, active, n, V, vstride, q, kp, Vt64);
+        k_bnormalize<double, VT><<<grid, kBlock, 0, st>>>(W, nrm, stride, active, n, V, vstride, q, kp, Vt64);
     return hipGetLastError();
+}
+
+// the Gram-Schmidt passes, norm and normalisation of one lockstep step on a basis of type VT
+template <typename VT>
+void batch_orthonormalize(hipStream_t st, const BatchArnoldi& a, VT* V)
+{
+    const int64_t n = a.n;
+    const int nrhs = a.nrhs;
+    const int* stop = a.ints + 1;
+    const unsigned nchunks = (unsigned)((n + kProjRows - 1) / kProjRows);
+    for (double* h : {a.h1, a.h2}) {
+        k_bproj_part<VT><<<dim3(nchunks, (unsigned)nrhs), kBlock, 0, st>>>(V, a.vstride, a.W, n, a.proj, a.ldh,
+                                                                            a.ints);
+        k_bproj_sum<<<(unsigned)nrhs, 64, 0, st>>>(a.proj, (int)nchunks, a.ldh, h, a.ints);
+        k_bupdate<VT><<<dim3(nblocks(n), (unsigned)nrhs), kBlock, 0, st>>>(V, a.vstride, h, a.ldh, a.cnt, -1.0, n,
+                                                                          a.W, stop);
+    }
+    k_bnorm<<<(unsigned)nrhs, kBlock, 0, st>>>(a.W, n, a.nrm, 1, stop);
+    const dim3 grid(nblocks(n), (unsigned)nrhs);
+    if (a.A16)
+        k_bnormalize_idx<uint16_t, VT><<<grid, kBlock, 0, st>>>(a.W, a.nrm, a.active, n, V, a.vstride, a.kp, a.P16,
+                                                               a.ints);
+    else if (a.use32)
+        k_bnormalize_idx<float, VT><<<grid, kBlock, 0, st>>>(a.W, a.nrm, a.active, n, V, a.vstride, a.kp, a.P32,
+                                                            a.ints);
+    else
+        k_bnormalize_idx<double, VT><<<grid, kBlock, 0, st>>>(a.W, a.nrm, a.active, n, V, a.vstride, a.kp, a.P64,
+                                                             a.ints);
+}
+}  // namespace
+
+hipError_t launch_bnormalize(hipStream_t st, const double* W, const double* nrm, int stride, const int* active,
+                             int64_t n, int nrhs, double* V, int64_t vstride, int q, int kp, float* Vt32,
+                             double* Vt64, uint16_t* Vt16)
+{
+    return bnormalize_any(st, W, nrm, stride, active, n, nrhs, V, vstride, q, kp, Vt32, Vt64, Vt16);
+}
+
+hipError_t launch_bnormalize(hipStream_t st, const double* W, const double* nrm, int stride, const int* active,
+                             int64_t n, int nrhs, float* V, int64_t vstride, int q, int kp, float* Vt32, double* Vt64,
+                             uint16_t* Vt16)
+{
+    return bnormalize_any(st, W, nrm, stride, active, n, nrhs, V, vstride, q, kp, Vt32, Vt64, Vt16);
 }
 
 int64_t batch_proj_doubles(int64_t n, int nrhs, int ldh)
@@ -1341,24 +1397,10 @@ hipError_t launch_batch_arnoldi_step(hipStream_t st, const BatchArnoldi& a)
                                       a.use32 ? (const double*)a.P32 : a.P64, n, nrhs, a.W, n, a.ws, a.ws_bytes, true,
                                       stop);
     if (e != hipSuccess) return e;
-    const unsigned nchunks = (unsigned)((n + kProjRows - 1) / kProjRows);
-    for (double* h : {a.h1, a.h2}) {
-        k_bproj_part<<<dim3(nchunks, (unsigned)nrhs), kBlock, 0, st>>>(a.V, a.vstride, a.W, n, a.proj, a.ldh,
-                                                                        a.ints);
-        k_bproj_sum<<<(unsigned)nrhs, 64, 0, st>>>(a.proj, (int)nchunks, a.ldh, h, a.ints);
-        k_bupdate<<<dim3(nblocks(n), (unsigned)nrhs), kBlock, 0, st>>>(a.V, a.vstride, h, a.ldh, a.cnt, -1.0, n, a.W,
-                                                                      stop);
-    }
-    k_bnorm<<<(unsigned)nrhs, kBlock, 0, st>>>(a.W, n, a.nrm, 1, stop);
-    if (a.A16)
-        k_bnormalize_idx<uint16_t><<<dim3(nblocks(n), (unsigned)nrhs), kBlock, 0, st>>>(a.W, a.nrm, a.active, n, a.V,
-                                                                                       a.vstride, a.kp, a.P16, a.ints);
-    else if (a.use32)
-        k_bnormalize_idx<float><<<dim3(nblocks(n), (unsigned)nrhs), kBlock, 0, st>>>(a.W, a.nrm, a.active, n, a.V,
-                                                                                    a.vstride, a.kp, a.P32, a.ints);
+    if (a.V32)
+        batch_orthonormalize(st, a, a.V32);
     else
-        k_bnormalize_idx<double><<<dim3(nblocks(n), (unsigned)nrhs), kBlock, 0, st>>>(a.W, a.nrm, a.active, n, a.V,
-                                                                                     a.vstride, a.kp, a.P64, a.ints);
+        batch_orthonormalize(st, a, a.V);
     k_bgivens<<<1, 64, 0, st>>>(a.h1, a.h2, a.nrm, a.H, a.ldh, a.cs, a.sn, a.p, a.bnorm, a.tol, a.resid, a.active,
                                 a.done, a.jr, a.cnt, a.ints, nrhs);
     return hipGetLastError();

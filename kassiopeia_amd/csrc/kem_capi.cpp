@@ -1720,7 +1720,10 @@ int gmres_batch(kem_matrix* A, int prec, int nrhs, const double* d_B, double* d_
         off += (bytes + 255) & ~(size_t)255;
         return at;
     };
-    const size_t oV = carve(sizeof(double) * (size_t)(vstride * nrhs));
+    // the Krylov bases of the FP32 / BF16 inner solves in FP32 (they solve to >= 1e-5; the
+    // orthogonalisation passes move half the bytes), of FP64 solves in FP64
+    const bool v32 = prec != kem::kBmmFp64;
+    const size_t oV = carve((v32 ? sizeof(float) : sizeof(double)) * (size_t)(vstride * nrhs));
     const size_t oW = carve(sizeof(double) * (size_t)(n * nrhs));
     const size_t oh1 = carve(sizeof(double) * (size_t)ldh * nrhs);
     const size_t oh2 = carve(sizeof(double) * (size_t)ldh * nrhs);
@@ -1740,7 +1743,8 @@ int gmres_batch(kem_matrix* A, int prec, int nrhs, const double* d_B, double* d_
         A->bws_bytes = off;
     }
     char* base = (char*)A->bws;
-    double* const V = (double*)(base + oV);
+    double* const V = v32 ? nullptr : (double*)(base + oV);
+    float* const V32 = v32 ? (float*)(base + oV) : nullptr;
     double* const W = (double*)(base + oW);
     double* const h1 = (double*)(base + oh1);
     double* const h2 = (double*)(base + oh2);
@@ -1762,6 +1766,7 @@ int gmres_batch(kem_matrix* A, int prec, int nrhs, const double* d_B, double* d_
     a.A32 = A->A32;
     a.A64 = A->A64;
     a.V = V;
+    a.V32 = V32;
     a.vstride = vstride;
     a.W = W;
     a.A16 = prec == kem::kBmmBf16 ? A->A16 : nullptr;
@@ -1856,8 +1861,10 @@ int gmres_batch(kem_matrix* A, int prec, int nrhs, const double* d_B, double* d_
         KEM_CK(c, hipMemcpyAsync(a.p, hp.data(), sizeof(double) * ldh * nrhs, hipMemcpyHostToDevice, st));
         KEM_CK(c, hipMemcpyAsync(a.resid, resid.data(), sizeof(double) * nrhs, hipMemcpyHostToDevice, st));
         KEM_CK(c, hipMemsetAsync(a.jr, 0, sizeof(int) * nrhs, st));
-        KEM_CK(c, kem::launch_bnormalize(st, W, h1, ldh, a.active, n, nrhs, V, vstride, 0, kp, a.P32, a.P64,
-                                         a.P16));
+        KEM_CK(c, v32 ? kem::launch_bnormalize(st, W, h1, ldh, a.active, n, nrhs, V32, vstride, 0, kp, a.P32, a.P64,
+                                               a.P16)
+                      : kem::launch_bnormalize(st, W, h1, ldh, a.active, n, nrhs, V, vstride, 0, kp, a.P32, a.P64,
+                                               a.P16));
         int got[4] = {0, 0, 0, 0};
         bool expired = false;
         for (;;) {
@@ -1874,7 +1881,8 @@ int gmres_batch(kem_matrix* A, int prec, int nrhs, const double* d_B, double* d_
         it += got[3];
         // x_r += V_r y_r with H_r y_r = p_r (upper triangular, jr[r] columns)
         KEM_CK(c, kem::launch_batch_trisolve(st, a, Y));
-        KEM_CK(c, kem::launch_bupdate(st, V, vstride, Y, ldh, a.cnt, 1.0, n, nrhs, d_X));
+        KEM_CK(c, v32 ? kem::launch_bupdate(st, V32, vstride, Y, ldh, a.cnt, 1.0, n, nrhs, d_X)
+                      : kem::launch_bupdate(st, V, vstride, Y, ldh, a.cnt, 1.0, n, nrhs, d_X));
         KEM_CK(c, hipMemcpyAsync(done.data(), a.done, sizeof(int) * nrhs, hipMemcpyDeviceToHost, st));
         KEM_CK(c, hipMemcpyAsync(resid.data(), a.resid, sizeof(double) * nrhs, hipMemcpyDeviceToHost, st));
         KEM_CK(c, hipStreamSynchronize(st));

@@ -225,9 +225,14 @@ hipError_t launch_bproj(hipStream_t st, const double* V, int64_t vstride, const 
                         int nrhs, double* h, int ldh);
 hipError_t launch_bupdate(hipStream_t st, const double* V, int64_t vstride, const double* h, int ldh, const int* cnt,
                           double sign, int64_t n, int nrhs, double* Y);
+hipError_t launch_bupdate(hipStream_t st, const float* V, int64_t vstride, const double* h, int ldh, const int* cnt,
+                          double sign, int64_t n, int nrhs, double* Y);
 hipError_t launch_bnorm(hipStream_t st, const double* W, int64_t n, int nrhs, double* nrm, int stride);
 hipError_t launch_bnormalize(hipStream_t st, const double* W, const double* nrm, int stride, const int* active,
                              int64_t n, int nrhs, double* V, int64_t vstride, int q, int kp, float* Vt32,
+                             double* Vt64, uint16_t* Vt16 = nullptr);
+hipError_t launch_bnormalize(hipStream_t st, const double* W, const double* nrm, int stride, const int* active,
+                             int64_t n, int nrhs, float* V, int64_t vstride, int q, int kp, float* Vt32,
                              double* Vt64, uint16_t* Vt16 = nullptr);
 hipError_t launch_baxpby(hipStream_t st, double a, const double* X, double b, const double* Y, double* Z, int64_t len);
 
@@ -242,6 +247,7 @@ struct BatchArnoldi {
     const double* A64 = nullptr;
     const uint16_t* A16 = nullptr;  // non-null: BF16 block products (panel P16)
     double* V = nullptr;  // nrhs bases of kmax + 1 columns, vstride apart
+    float* V32 = nullptr;  // the same in FP32 (used instead of V when set: inner solves of mixed precision)
     int64_t vstride = 0;
     double* W = nullptr;  // n x nrhs work block
     float* P32 = nullptr;  // row-major panel of the current basis vectors (n x kp)
