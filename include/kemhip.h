@@ -313,7 +313,10 @@ int kem_solve(kem_matrix* A, const kem_solve_opts* opts, const double* b, double
    (KEM_MIXED: FP32 matrix on MFMA, FP64 refinement against the FP64 matrix).  B, X: host,
    column-major n x nrhs (X holds the initial guesses and receives the solutions).
    residuals (optional, nrhs): final ||b - A x|| / ||b|| per right-hand side (FP64 matrix).
-   stats: iterations / refinements = maximum over groups of 64, residual = worst. */
+   stats: iterations / refinements = maximum over groups of 64, residual = worst.  The matrix
+   keeps the lockstep workspace of its largest solve until kem_matrix_destroy (mostly the
+   Krylov bases: (min(max_iterations, 300) + 1) x n x min(nrhs, 64) values, FP32 for the inner
+   solves of KEM_MIXED / KEM_MIXED_BF16, FP64 for KEM_FP64). */
 int kem_solve_batch(kem_matrix* A, const kem_solve_opts* opts, int64_t nrhs, const double* B, double* X,
                     double* residuals, kem_solve_stats* stats);
 /* ---- adjoint (no reference counterpart: KEMField has no derivatives).  Gradients of a loss
