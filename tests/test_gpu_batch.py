@@ -98,7 +98,7 @@ def test_block_product_streamed_shapes(ctx, n):
         cont.close()
 
 
-@pytest.mark.parametrize("n", [1352, 1350, 1349])
+@pytest.mark.parametrize("n", [1352, 1350, 1349, 260, 8, 6, 4])
 def test_block_product_tile_shapes(ctx, n):
     """Block products on 16 x 16 MFMA tiles streamed through LDS: FP32 matrices with n % 4 == 0
     and up to 32 right-hand sides (16-column panels up to 16, 32 columns above), FP64 matrices
@@ -123,6 +123,7 @@ def test_block_product_tile_shapes(ctx, n):
     X = np.random.default_rng(n + 7).normal(size=(n, 64))
     Xr = {"mixed": X.astype(np.float32).astype(np.float64), "fp64": X}
     cases = [("mixed", k) for k in (2, 5, 16, 17, 32)] + [("fp64", k) for k in (1, 5, 16, 17, 64)]
+    # matrices smaller than one workgroup's 256 rows and one chunk's columns: clamped loads
     try:
         for blocks in (1, 1024, 4096):
             T.set_knob(ctx, "bmm_tile_blocks", blocks)
