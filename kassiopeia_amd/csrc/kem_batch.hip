@@ -277,79 +277,6 @@ __global__ __launch_bounds__(kBlock) void k_panel_bf16(const double* __restrict_
     }
 }
 
-// part[s][i][c] = sum_{j in split s} A16(i, j) * Vp(j, c) with v_mfma_f32_32x32x16_bf16 (FP32
-// accumulation).  A wavefront computes a 64-row x KP tile: per step of 16 columns lane l loads
-// row l % 32 (+ 32 for the second row tile) of column group g + l / 32 (8 BF16 values, one
-// 16-byte load) as operand A, and the same column group of panel column l % 32 (+ 32 t) as
-// operand B; the accumulators are laid out as in k_bmm32.  Column groups [g0, g1) per
-// blockIdx.y; the loads of KB steps are issued before their MFMAs.
-template <int NT>
-__global__ __launch_bounds__(kBlock) void k_bmm16(const bf8v* __restrict__ A16, int64_t n,
-                                                  const bf8v* __restrict__ Vp, int64_t ngroups, int64_t gper,
-                                                  float* __restrict__ part, const int* __restrict__ skip)
-{
-    if (skip && *skip) return;
-    constexpr int KP = 32 * NT;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int64_t i0 = ((int64_t)blockIdx.x * 4 + wave) * 64;
-    if (i0 >= n) return;  // whole wavefront beyond the last row
-    const int64_t g0 = (int64_t)blockIdx.y * gper;
-    const int64_t g1 = std::min<int64_t>(ngroups, g0 + gper);
-    const int lr = lane & 31, lk = lane >> 5;
-    const int64_t ra = std::min<int64_t>(i0 + lr, n - 1), rb = std::min<int64_t>(i0 + 32 + lr, n - 1);
-    f16v acc0[NT], acc1[NT];
-    for (int t = 0; t < NT; t++)
-        for (int v = 0; v < 16; v++) {
-            acc0[t][v] = 0.f;
-            acc1[t][v] = 0.f;
-        }
-    // column steps per load batch (64 right-hand sides: one step measured 3 % faster than two,
-    // 4 wavefronts per SIMD either way)
-    constexpr int KB = NT == 1 ? 4 : 1;
-    int64_t g = g0;
-    for (; g + 2 * KB <= g1; g += 2 * KB) {
-        bf8v a0[KB], a1[KB], b[KB][NT];
-        for (int q = 0; q < KB; q++) {
-            const int64_t gg = g + 2 * q + lk;
-            a0[q] = A16[gg * n + ra];
-            a1[q] = A16[gg * n + rb];
-            for (int t = 0; t < NT; t++) b[q][t] = Vp[gg * KP + 32 * t + lr];
-        }
-        for (int q = 0; q < KB; q++)
-            for (int t = 0; t < NT; t++) {
-                acc0[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0[q], b[q][t], acc0[t], 0, 0, 0);
-                acc1[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1[q], b[q][t], acc1[t], 0, 0, 0);
-            }
-    }
-    for (; g < g1; g += 2) {
-        const int64_t gg = g + lk;
-        const bool ok = gg < g1;
-        const int64_t gs = ok ? gg : g;  // g < g1: a valid column group
-        bf8v a0 = A16[gs * n + ra], a1 = A16[gs * n + rb];
-        bf8v b[NT];
-        for (int t = 0; t < NT; t++) b[t] = Vp[gs * KP + 32 * t + lr];
-        if (!ok) {
-            for (int m = 0; m < 8; m++) {
-                a0[m] = (__bf16)0.f;
-                a1[m] = (__bf16)0.f;
-            }
-        }
-        for (int t = 0; t < NT; t++) {
-            acc0[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, b[t], acc0[t], 0, 0, 0);
-            acc1[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, b[t], acc1[t], 0, 0, 0);
-        }
-    }
-    // the lane's rows are i0 + 4 lk + rr with rr = 8 (v / 4) + v % 4 (+ 32 for the second tile):
-    // one base address and constant offsets, so the stores need no per-store address registers
-    float* out = part + (int64_t)blockIdx.y * n * KP + (i0 + 4 * lk) * KP + lr;
-    const int64_t lim = n - i0 - 4 * lk;
-    for (int t = 0; t < NT; t++)
-        for (int v = 0; v < 16; v++) {
-            const int rr = 8 * (v >> 2) + (v & 3);
-            if (rr < lim) out[rr * KP + 32 * t] = acc0[t][v];
-            if (rr + 32 < lim) out[(rr + 32) * KP + 32 * t] = acc1[t][v];
-        }
-}
 
 // ------------------------------------------------------------------ block products streamed through LDS
 // 64 right-hand sides.  A workgroup takes 256 rows (wavefront w: rows 64 w .. 64 w + 63, the
@@ -457,15 +384,35 @@ __global__ __launch_bounds__(kBlock) void k_bmm32s(const float* __restrict__ A, 
         }
 }
 
-// BF16 matrix, KP = 64.  Chunk: G column groups x 256 rows of A16 (one 1 KB load per group and
-// wavefront: lane l takes row 64 w + l) and the G groups of the panel (one 1 KB load per
-// group); the panel is read from LDS by all four wavefronts.
+// BF16 matrix, KP = 32 NT (NT = 1: 2-32 right-hand sides, 2: 33-64).  Chunk: G column groups x
+// 256 rows of A16 (one 1 KB load per group and wavefront: lane l takes row 64 w + l) and the G
+// groups of the panel (one load per group, its lanes past KP idle); the panel is read from LDS
+// by all four wavefronts.
+template <int NT>
+__device__ __forceinline__ void bmm16s_issue(const bf8v* __restrict__ A16, int64_t n, const bf8v* __restrict__ Vp,
+                                             bf8v* lds, int64_t row, int64_t g0, int64_t g1, int64_t k, int lane,
+                                             int wave)
+{
+    constexpr int KP = 32 * NT, G = kStreamG, CH = G * kStreamRows + G * KP;
+    bf8v* buf = lds + (k % 3) * CH;
+    const int64_t gc = g0 + k * G;
+    for (int q = 0; q < G; q++) {
+        const int64_t g = std::min<int64_t>(gc + q, g1 - 1);  // past g1: a valid group, unused
+        __builtin_amdgcn_global_load_lds(A16 + g * n + row, buf + q * kStreamRows + 64 * wave, 16, 0, 0);
+    }
+    for (int q = wave; q < G; q += 4) {
+        const int64_t g = std::min<int64_t>(gc + q, g1 - 1);
+        if (lane < KP) __builtin_amdgcn_global_load_lds(Vp + g * KP + lane, buf + G * kStreamRows + q * KP, 16, 0, 0);
+    }
+}
+
+template <int NT>
 __global__ __launch_bounds__(kBlock) void k_bmm16s(const bf8v* __restrict__ A16, int64_t n,
                                                    const bf8v* __restrict__ Vp, int64_t ngroups, int64_t gper,
                                                    float* __restrict__ part, const int* __restrict__ skip)
 {
     if (skip && *skip) return;
-    constexpr int KP = 64, G = kStreamG, CH = G * kStreamRows + G * KP, NW = G + G / 4;
+    constexpr int KP = 32 * NT, G = kStreamG, CH = G * kStreamRows + G * KP, NW = G + G / 4;
     static_assert(G % 4 == 0, "panel loads spread over the four wavefronts");
     __shared__ bf8v lds[3 * CH];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -476,26 +423,14 @@ __global__ __launch_bounds__(kBlock) void k_bmm16s(const bf8v* __restrict__ A16,
     if (g0 >= g1) return;
     const int64_t nch = (g1 - g0 + G - 1) / G;
     const int64_t row = std::min<int64_t>(r0 + 64 * wave + lane, n - 1);  // past n: unused
-    auto issue = [&](int64_t k) {
-        bf8v* buf = lds + (k % 3) * CH;
-        const int64_t gc = g0 + k * G;
-        for (int q = 0; q < G; q++) {
-            const int64_t g = std::min<int64_t>(gc + q, g1 - 1);  // past g1: a valid group, unused
-            __builtin_amdgcn_global_load_lds(A16 + g * n + row, buf + q * kStreamRows + 64 * wave, 16, 0, 0);
-        }
-        for (int q = wave; q < G; q += 4) {
-            const int64_t g = std::min<int64_t>(gc + q, g1 - 1);
-            __builtin_amdgcn_global_load_lds(Vp + g * KP + lane, buf + G * kStreamRows + q * KP, 16, 0, 0);
-        }
-    };
-    f16v acc0[2], acc1[2];
-    for (int t = 0; t < 2; t++)
+    f16v acc0[NT], acc1[NT];
+    for (int t = 0; t < NT; t++)
         for (int v = 0; v < 16; v++) {
             acc0[t][v] = 0.f;
             acc1[t][v] = 0.f;
         }
-    issue(0);
-    if (nch > 1) issue(1);
+    bmm16s_issue<NT>(A16, n, Vp, lds, row, g0, g1, 0, lane, wave);
+    if (nch > 1) bmm16s_issue<NT>(A16, n, Vp, lds, row, g0, g1, 1, lane, wave);
     for (int64_t k = 0; k < nch; k++) {
         if (k + 1 < nch)
             wait_vmcnt<NW>();
@@ -505,22 +440,21 @@ __global__ __launch_bounds__(kBlock) void k_bmm16s(const bf8v* __restrict__ A16,
         asm volatile("" ::: "memory");
         const bf8v* buf = lds + (k % 3) * CH;
         const int kv = (int)std::min<int64_t>(G, g1 - (g0 + k * G));  // valid groups
-        bf8v a0[G / 2], a1[G / 2], b[G / 2][2];
+        bf8v a0[G / 2], a1[G / 2], b[G / 2][NT];
         for (int p = 0; p < G / 2; p++) {
             const int q = 2 * p + lk;
             a0[p] = buf[q * kStreamRows + wave * 64 + lr];
             a1[p] = buf[q * kStreamRows + wave * 64 + 32 + lr];
-            b[p][0] = buf[G * kStreamRows + q * KP + lr];
-            b[p][1] = buf[G * kStreamRows + q * KP + 32 + lr];
+            for (int t = 0; t < NT; t++) b[p][t] = buf[G * kStreamRows + q * KP + 32 * t + lr];
             if (q >= kv)
                 for (int m = 0; m < 8; m++) {
                     a0[p][m] = (__bf16)0.f;
                     a1[p][m] = (__bf16)0.f;
                 }
         }
-        if (k + 2 < nch) issue(k + 2);
+        if (k + 2 < nch) bmm16s_issue<NT>(A16, n, Vp, lds, row, g0, g1, k + 2, lane, wave);
         for (int p = 0; p < G / 2; p++)
-            for (int t = 0; t < 2; t++) {
+            for (int t = 0; t < NT; t++) {
                 acc0[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0[p], b[p][t], acc0[t], 0, 0, 0);
                 acc1[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1[p], b[p][t], acc1[t], 0, 0, 0);
             }
@@ -529,7 +463,7 @@ __global__ __launch_bounds__(kBlock) void k_bmm16s(const bf8v* __restrict__ A16,
     if (i0 >= n) return;
     float* out = part + (int64_t)blockIdx.y * n * KP + (i0 + 4 * lk) * KP + lr;
     const int64_t lim = n - i0 - 4 * lk;
-    for (int t = 0; t < 2; t++)
+    for (int t = 0; t < NT; t++)
         for (int v = 0; v < 16; v++) {
             const int rr2 = 8 * (v >> 2) + (v & 3);
             if (rr2 < lim) out[rr2 * KP + 32 * t] = acc0[t][v];
@@ -1100,7 +1034,8 @@ size_t bmm_workspace_bytes(int64_t n, int nrhs, int prec)
     const int kp = bmm_panel_cols(n, nrhs, prec);
     if (prec == kBmmBf16) {
         const int64_t tiles = (n + 63) / 64, ngroups = (n + 7) / 8;
-        const int s = std::max(bmm_splits(n, tiles), kp == 64 ? bmm_stream_splits(n) : 1);
+        const int s = std::max({bmm_splits(n, tiles), kp == 64 ? bmm_stream_splits(n) : 1,
+                                kp == 32 ? bmm_tile_splits(n, 32, 2) : 1});
         return sizeof(float) * (size_t)s * n * kp + sizeof(uint16_t) * (size_t)ngroups * 8 * kp + 512;
     }
     if (prec == kBmmFp32) {
@@ -1246,11 +1181,9 @@ hipError_t launch_bmm16(hipStream_t st, const uint16_t* A16, int64_t n, const vo
     const int nt = nrhs <= 32 ? 1 : 2, kp = 32 * nt;
     const int64_t tiles = (n + 63) / 64, ngroups = (n + 7) / 8;
     const int s = bmm_splits(n, tiles);
-    const int64_t gper = (ngroups + s - 1) / s;
-    const int ns = (int)((ngroups + gper - 1) / gper);
     float* part = (float*)ws;
-    // the panel after the partial sums of either kernel, 256-byte aligned
-    const int sw = std::max(s, nt == 2 ? bmm_stream_splits(n) : 1);
+    // the panel after the partial sums of any of the kernels, 256-byte aligned
+    const int sw = std::max({s, nt == 2 ? bmm_stream_splits(n) : 1, nt == 1 ? bmm_tile_splits(n, 32, 2) : 1});
     uintptr_t pv = (uintptr_t)(part + (size_t)sw * n * kp);
     pv = (pv + 255) & ~(uintptr_t)255;
     uint4* Vp = (uint4*)pv;
@@ -1276,19 +1209,20 @@ hipError_t launch_bmm16(hipStream_t st, const uint16_t* A16, int64_t n, const vo
         const int64_t gs = (ngroups + ss - 1) / ss;
         const int nss = (int)((ngroups + gs - 1) / gs);
         dim3 grid((unsigned)((n + kStreamRows - 1) / kStreamRows), (unsigned)nss);
-        k_bmm16s<<<grid, kBlock, 0, st>>>((const bf8v*)A16, n, P, ngroups, gs, part, skip);
+        k_bmm16s<2><<<grid, kBlock, 0, st>>>((const bf8v*)A16, n, P, ngroups, gs, part, skip);
         k_bmm32_reduce<64><<<(unsigned)((n + kReduceRows - 1) / kReduceRows), kBlock, 0, st>>>(part, nss, n, nrhs, Y, ldy, skip);
         return hipGetLastError();
     } else {
-        dim3 grid((unsigned)((tiles + 3) / 4), (unsigned)ns);
-        k_bmm16<1><<<grid, kBlock, 0, st>>>((const bf8v*)A16, n, P, ngroups, gper, part, skip);
+        // 2-32 right-hand sides: streamed through LDS on one 32-column tile, HBM-bound, split
+        // like the 16 x 16-tile products
+        const int ss = bmm_tile_splits(n, 32, 2);
+        const int64_t gs = (ngroups + ss - 1) / ss;
+        const int nss = (int)((ngroups + gs - 1) / gs);
+        dim3 grid((unsigned)((n + kStreamRows - 1) / kStreamRows), (unsigned)nss);
+        k_bmm16s<1><<<grid, kBlock, 0, st>>>((const bf8v*)A16, n, P, ngroups, gs, part, skip);
+        k_bmm32_reduce<32><<<(unsigned)((n + kReduceRows - 1) / kReduceRows), kBlock, 0, st>>>(part, nss, n, nrhs, Y, ldy, skip);
+        return hipGetLastError();
     }
-    const unsigned gr = (unsigned)((n + kReduceRows - 1) / kReduceRows);
-    if (kp == 32)
-        k_bmm32_reduce<32><<<gr, kBlock, 0, st>>>(part, ns, n, nrhs, Y, ldy, skip);
-    else
-        k_bmm32_reduce<64><<<gr, kBlock, 0, st>>>(part, ns, n, nrhs, Y, ldy, skip);
-    return hipGetLastError();
 }
 
 hipError_t launch_bproj(hipStream_t st, const double* V, int64_t vstride, const double* W, int64_t n, int nq,
