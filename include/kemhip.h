@@ -115,7 +115,7 @@ int kem_ctx_set_mixed_fp64_ratio(kem_ctx* ctx, double ratio);
    each XCD's range stays in its L2 (0: no split; default 2560).  KEM_TUNE_GEMV_BLOCKS /
    KEM_TUNE_BMM_BLOCKS = target block counts of the split-K matrix-vector product (default 2048)
    and of the register-loaded multi-RHS block products in 64-row tiles times column splits
-   (BF16 with up to 32 right-hand sides, FP32 with n % 4 != 0, FP64 with n odd; default 2560);
+   (FP32 with n % 4 != 0, FP64 with n odd; default 2560);
    they set how many partial sums are reduced. */
 #define KEM_TUNE_FAR_RANGE_FP64 1
 #define KEM_TUNE_FAR_RANGE_MIXED 2
@@ -133,10 +133,11 @@ int kem_ctx_set_mixed_fp64_ratio(kem_ctx* ctx, double ratio);
    matrix through LDS (FP32 matrices with n % 4 == 0, and BF16): 256 rows per workgroup times
    column splits; default 480 */
 #define KEM_TUNE_BMM_STREAM_BLOCKS 8
-/* largest workgroup count of the block products on 16 x 16 MFMA tiles streamed through LDS
-   (FP32 matrices with n % 4 == 0 and up to 32 right-hand sides; FP64 matrices with n even): 256
-   rows per workgroup times column splits, the split count within it whose last round of two
-   workgroups per compute unit is fullest; default 4096 */
+/* largest workgroup count of the streamed block products of up to 32 right-hand sides (16 x
+   16 MFMA tiles: FP32 matrices with n % 4 == 0, FP64 matrices with n even, also for 33-64; one
+   32-column tile: BF16 matrices): 256 rows per workgroup times column splits, the split count
+   within it chosen by a traffic model over rounds of two workgroups per compute unit; default
+   4096 */
 #define KEM_TUNE_BMM_TILE_BLOCKS 9
 int kem_ctx_set_tuning(kem_ctx* ctx, int param, int64_t value);
 int kem_ctx_get_tuning(const kem_ctx* ctx, int param, int64_t* value);

@@ -8,13 +8,12 @@ The C-ABI exposes these launch-shape knobs (``kem_ctx_set_tuning``):
 * ``KEM_TUNE_GEMV_BLOCKS``: target block count of the split-K matrix-vector product (GMRES,
   Robin Hood residuals, ``KSquareMatrix::Multiply``).
 * ``KEM_TUNE_BMM_BLOCKS``: target block count of the multi-right-hand-side block products
-  that load the matrix into registers (BF16 up to 32 right-hand sides, FP32 when n % 4 != 0,
-  FP64 when n is odd).
+  that load the matrix into registers (FP32 when n % 4 != 0, FP64 when n is odd).
 * ``KEM_TUNE_BMM_STREAM_BLOCKS``: target workgroup count of the block products of 33-64
   right-hand sides that stream the matrix through LDS (BF16, and FP32 when n % 4 == 0).
-* ``KEM_TUNE_BMM_TILE_BLOCKS``: target workgroup count of the block products on 16 x 16 MFMA
-  tiles streamed through LDS (FP32 up to 32 right-hand sides when n % 4 == 0, FP64 when n is
-  even).
+* ``KEM_TUNE_BMM_TILE_BLOCKS``: largest workgroup count of the streamed block products of up
+  to 32 right-hand sides (16 x 16 MFMA tiles: FP32 when n % 4 == 0, FP64 when n is even; one
+  32-column tile: BF16).
 * ``KEM_TUNE_NEAR_BLOCKS``: blocks of the near pass that runs beside the mixed far pass on a
   second stream (0: one per 256 points).  Its timed region is the whole field pass.
 * ``KEM_TUNE_BMM_ROWS``: the wavefront tile of the FP32 MFMA block product, 64 rows (one float
@@ -149,7 +148,8 @@ def tune_block_product(matrix: K.KBoundaryIntegralMatrix, nrhs=64, precision="mi
     n % 4 == 0, run the 32 x 32-tile kernels that stream the matrix through LDS: their workgroup
     target ``bmm_stream_blocks`` is swept (candidates default STREAM_CANDIDATES).  Up to 32
     right-hand sides on an FP32 matrix with n % 4 == 0, and an FP64 matrix with n even, run the
-    16 x 16-tile streamed kernels: ``bmm_tile_blocks`` is swept (TILE_CANDIDATES).  Otherwise
+    16 x 16-tile streamed kernels, and 2-32 on a BF16 matrix the one-tile streamed kernel:
+    ``bmm_tile_blocks`` is swept (TILE_CANDIDATES).  Otherwise
     (register-loaded kernels) the FP32 wavefront tile is swept first at the current split
     target, then the split target ``bmm_blocks`` with the chosen tile (candidates default
     BMM_CANDIDATES; the tile's sweep under "rows").  Returns the sweep of the split knob with
@@ -165,7 +165,7 @@ def tune_block_product(matrix: K.KBoundaryIntegralMatrix, nrhs=64, precision="mi
         res["knob"] = "bmm_stream_blocks"
         return res
     tiled = ((precision == "mixed" and nrhs <= 32 and matrix.n % 4 == 0)
-             or (precision == "fp64" and matrix.n % 2 == 0))
+             or (precision == "fp64" and matrix.n % 2 == 0) or (precision == "bf16" and 2 <= nrhs <= 32))
     if tiled:
         res = _sweep(matrix.ctx, "bmm_tile_blocks", candidates or TILE_CANDIDATES, run, repeats)
         res["knob"] = "bmm_tile_blocks"
