@@ -50,12 +50,12 @@ PEAK_HBM_GBS = 8000.0
 CPU_ORACLE_OVER_REFERENCE = 3.68 / 2.95
 
 # HBM bytes per launch of the mixed far kernel on the headline workload, by element-range size
-# (which sets the dispatch: 640 -> 208,384 points x 160 ranges, 10,240 -> 1,973,760 points x 16
-# ranges), from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs scaled by the readings of a 1 GiB
-# device copy (FETCH_SIZE x 1.99990, WRITE_SIZE x 1.0): 640: 0.847 GB read, 1.067 GB written;
-# 10,240: 0.996 GB read, 1.011 GB written (the writes are the partial-sum slices of the ranges).
-# Source: the profile named here; other range sizes were not measured (traffic null).
-TRAFFIC = {"bytes": {640: 1.914e9, 10240: 2.006e9},
+# (which sets the dispatch: 640 -> 208,384 points x 160 ranges, 2,560 -> 818,176 x 40, 10,240 ->
+# 1,973,760 x 16), from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs scaled by the readings of
+# a 1 GiB device copy (FETCH_SIZE x 1.99990, WRITE_SIZE x 1.0): 640: 0.847 GB read, 1.067 GB
+# written; 2,560: 0.885 / 1.047 GB; 10,240: 0.996 / 1.011 GB (the writes are the partial-sum
+# slices of the ranges).  Source: the profile named here; other sizes not measured (null).
+TRAFFIC = {"bytes": {640: 1.914e9, 2560: 1.932e9, 10240: 2.006e9},
            "profile": "profiles/pmc_traffic_k_field_far32c_bore100k_10Mpts.md"}
 
 METRIC = "point-element field evals/s + solve time, mixed vs FP64 at 1e-6 rel. error"
