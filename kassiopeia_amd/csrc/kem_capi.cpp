@@ -1101,12 +1101,17 @@ int kem_matrix_assemble(kem_ctx* c, const kem_mesh* m, int storage, kem_matrix**
     const bool need32 = (storage & KEM_STORE_FP32) || ((storage & KEM_STORE_BF16) && !(storage & KEM_STORE_FP64));
     if (e == hipSuccess && need32) e = dalloc(&A->A32, nn);
     if (e == hipSuccess && (storage & KEM_STORE_BF16)) e = dalloc(&A->A16, (int64_t)kem::bf16_matrix_elements(m->n));
+    // the potential node table of the split numeric-policy assembly (Dirichlet rows)
+    DVec an;
+    if (e == hipSuccess && c->policy == 0 && !m->has_neumann && !m->has_axisym && !m->gstart && m->clusters &&
+        m->order)
+        e = dalloc(&an.p, kem::assemble_nodes_doubles(m->n));
     hipEvent_t t0, t1;
     timing_pair(c, &t0, &t1);
     if (e == hipSuccess && t0) e = hipEventRecord(t0, c->stream);
     if (e == hipSuccess)
         e = kem::launch_assemble(c->stream, c->policy, m->has_neumann, m->geom, m->nodes, m->recs, m->targets, m->n, A->A64,
-                                 A->A32, m->has_axisym, m->gstart, m->clusters, m->order);
+                                 A->A32, m->has_axisym, m->gstart, m->clusters, m->order, an.p);
     if (e == hipSuccess && t1) e = hipEventRecord(t1, c->stream);
     if (e == hipSuccess && A->A16) e = kem::launch_pack_bf16(c->stream, A->A64, A->A32, m->n, A->A16);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);

@@ -424,40 +424,47 @@ __device__ __forceinline__ F4 element_far_wave(const double* __restrict__ g, con
     return cubature<7, FIELD>(nd + 4 * KEM_N7, pref, px, py, pz);
 }
 
-// The potential of cubature<NQ, false> at T points per lane from one pass over the nodes.
+// The potential of cubature<NQ, false> at T points per lane from one pass over the nodes of the
+// assembly's node table (KEM_ANODE_STRIDE layout): (dx, dy, dz)[t] = P_t - c, d2[t] = |P_t - c|^2,
+// r^2 = d2 + |q'|^2 - 2 (P - c) . q'.  Beyond the near cut-offs |P - c| is at least 2 element
+// sizes and |q'| about one, so the cancellation costs at most a few ulps of r^2.
 template <int NQ, int T>
-__device__ __forceinline__ void cubature_pot_n(const double* __restrict__ nd, double pref, const double (&px)[T],
-                                               const double (&py)[T], const double (&pz)[T], double (&v)[T])
+__device__ __forceinline__ void cubature_pot_n(const double* __restrict__ an, double pref, const double (&dx)[T],
+                                               const double (&dy)[T], const double (&dz)[T], const double (&d2)[T],
+                                               double (&v)[T])
 {
     double s[T];
 #pragma unroll
     for (int t = 0; t < T; t++) s[t] = 0.;
 #pragma unroll
     for (int g = 0; g < NQ; g++) {
-        const double qx = nd[4 * g + 0], qy = nd[4 * g + 1], qz = nd[4 * g + 2], w = nd[4 * g + 3];
+        const double* q = an + 5 * g;
 #pragma unroll
         for (int t = 0; t < T; t++) {
-            const double dx = px[t] - qx, dy = py[t] - qy, dz = pz[t] - qz;
-            s[t] += w * rsqrt_acc(dx * dx + dy * dy + dz * dz);
+            const double r2 = __builtin_fma(dx[t], q[0], __builtin_fma(dy[t], q[1], __builtin_fma(dz[t], q[2], d2[t] + q[3])));
+            s[t] = __builtin_fma(q[4], rsqrt_acc(r2), s[t]);
         }
     }
 #pragma unroll
     for (int t = 0; t < T; t++) v[t] = pref * s[t];
 }
 
-// element_far_wave's potential at T points per lane (squared distances r2[t] from the element
-// centre; a point takes part in the rule choice only if its pair is far, far[t])
+// element_far_wave's potential at T points per lane (squared distances d2[t] from the element
+// centre, offsets (dx, dy, dz)[t] = P_t - c; a point takes part in the rule choice only if its
+// pair is far, far[t]); segments from the cubature nodes nd, surfaces from the table an
 template <int T>
 __device__ __forceinline__ void element_far_wave_pot_n(const double* __restrict__ g, const double* __restrict__ nd,
-                                                       int type, const double (&r2)[T], const bool (&far)[T],
-                                                       const double (&px)[T], const double (&py)[T],
-                                                       const double (&pz)[T], double (&v)[T])
+                                                       const double* __restrict__ an, int type,
+                                                       const double (&d2)[T], const bool (&far)[T],
+                                                       const double (&dx)[T], const double (&dy)[T],
+                                                       const double (&dz)[T], const double (&px)[T],
+                                                       const double (&py)[T], const double (&pz)[T], double (&v)[T])
 {
     const double pref = g[KG_PREF];
     auto needs = [&](double cut) {
         bool b = false;
 #pragma unroll
-        for (int t = 0; t < T; t++) b = b || (far[t] && !(r2[t] > cut));
+        for (int t = 0; t < T; t++) b = b || (far[t] && !(d2[t] > cut));
         return __builtin_amdgcn_ballot_w64(b) != 0;
     };
     if (type == KEM_LINESEGMENT) {
@@ -469,9 +476,9 @@ __device__ __forceinline__ void element_far_wave_pot_n(const double* __restrict_
                        .p;
         return;
     }
-    if (needs(g[KG_REF12_2])) return cubature_pot_n<33, T>(nd + 4 * KEM_N33, pref, px, py, pz, v);
-    if (needs(g[KG_REF7_2])) return cubature_pot_n<12, T>(nd + 4 * KEM_N12, pref, px, py, pz, v);
-    return cubature_pot_n<7, T>(nd + 4 * KEM_N7, pref, px, py, pz, v);
+    if (needs(g[KG_REF12_2])) return cubature_pot_n<33, T>(an + 5 * KEM_N33, pref, dx, dy, dz, d2, v);
+    if (needs(g[KG_REF7_2])) return cubature_pot_n<12, T>(an + 5 * KEM_N12, pref, dx, dy, dz, d2, v);
+    return cubature_pot_n<7, T>(an + 5 * KEM_N7, pref, dx, dy, dz, d2, v);
 }
 
 // Near pass of one pair (closed forms).

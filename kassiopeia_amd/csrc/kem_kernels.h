@@ -103,11 +103,14 @@ hipError_t launch_pairs(hipStream_t st, int policy, const double* geom, const do
                         const int64_t* gstart = nullptr);
 
 // collocation matrix; with element clusters (centre, radius per KEM_CLUSTER elements in the
-// Morton order `order`) the numeric policy splits it into cubature and closed-form kernels
+// Morton order `order`) the numeric policy splits it into cubature and closed-form kernels, and
+// without Neumann rows (field = false) that split needs anodes, assemble_nodes_doubles(n)
+// doubles of workspace for the potential node table
+int64_t assemble_nodes_doubles(int64_t n);
 hipError_t launch_assemble(hipStream_t st, int policy, bool field, const double* geom, const double* nodes,
                            const double* recs, const double* targets, int64_t n, double* A64, float* A32,
                            bool axisym = false, const int64_t* gstart = nullptr, const double* clusters = nullptr,
-                           const int32_t* order = nullptr);
+                           const int32_t* order = nullptr, double* anodes = nullptr);
 // phi[i] += t[i], ex[i] += t[npts + i], ey[i] += t[2 npts + i], ez[i] += t[3 npts + i]
 hipError_t launch_accumulate4(hipStream_t st, const double* t, int64_t npts, double* phi, double* ex, double* ey,
                               double* ez);

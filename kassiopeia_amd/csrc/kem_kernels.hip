@@ -694,9 +694,7 @@ __global__ __launch_bounds__(kBlock) void k_assemble(const double* __restrict__ 
 // the closed-form entries are left to k_assemble_near (no closed form inlined here: the kernel
 // keeps the far field kernels' register budget), and the far lanes of a wavefront share one
 // cubature rule per column (dev::element_far_wave: 112.9 -> 108 ms on the 100,000-triangle
-// assembly).  Measured and not taken: potential nodes stored relative to the element centre as
-// (-2 q', |q'|^2, w), which cuts r^2 from 6 to 4 operations per node but needs 5 scalar
-// values per node instead of 4 (120 ms at an unroll of 4, 149 ms at 16, SGPR spills unrolled).  KElectrostaticBoundaryIntegrator::
+// assembly).  Dirichlet rows take k_assemble_far_pot instead.  KElectrostaticBoundaryIntegrator::
 // ComputeBoundaryIntegral (Core/include/KElectrostaticBoundaryIntegrator.hh:260-280).
 template <bool FIELD>
 __global__ __launch_bounds__(kBlock) void k_assemble_far(const double* __restrict__ geom,
@@ -731,10 +729,15 @@ __global__ __launch_bounds__(kBlock) void k_assemble_far(const double* __restric
 
 // Far entries of Dirichlet rows with T targets per lane: wavefront w of a block takes rows
 // 64 T w .. 64 T (w + 1) - 1 of the block's 256 T, lane l rows l + 64 t, so each wave-uniform
-// node load of a column serves T evaluations.
+// node load of a column serves T evaluations; the nodes come from the centre-relative table an
+// (k_asm_nodes: r^2 in 4 operations per node instead of 6, for 5 scalar values per node instead
+// of 4).  100,000-triangle assembly, both kernels: 108 ms at one target per lane, 100-102 ms at
+// 2 with the plain nodes, 96-98 ms with the table; the table at one target per lane was slower
+// (120 ms at an unroll of 4, 149 ms at 16, SGPR spills fully unrolled).
 template <int T>
 __global__ __launch_bounds__(kBlock) void k_assemble_far_pot(const double* __restrict__ geom,
                                                              const double* __restrict__ nodes,
+                                                             const double* __restrict__ an,
                                                              const double* __restrict__ targets, int64_t n,
                                                              int64_t per_split, double* __restrict__ A64,
                                                              float* __restrict__ A32)
@@ -758,16 +761,20 @@ __global__ __launch_bounds__(kBlock) void k_assemble_far_pot(const double* __res
     if (j1 > n) j1 = n;
     for (int64_t j = j0; j < j1; j++) {
         const double* g = geom + j * KG_STRIDE;
-        double r2[T], v[T];
+        double r2[T], v[T], dx[T], dy[T], dz[T];
         bool far[T], any = false;
 #pragma unroll
         for (int t = 0; t < T; t++) {
-            r2[t] = dev::dist2(g, px[t], py[t], pz[t]);
+            dx[t] = px[t] - g[KG_CX];
+            dy[t] = py[t] - g[KG_CY];
+            dz[t] = pz[t] - g[KG_CZ];
+            r2[t] = __builtin_fma(dz[t], dz[t], __builtin_fma(dy[t], dy[t], dx[t] * dx[t]));  // dev::dist2
             far[t] = !dev::is_near2(g, r2[t], 0);
             any = any || far[t];
         }
         if (__builtin_amdgcn_ballot_w64(any) == 0) continue;
-        dev::element_far_wave_pot_n<T>(g, nodes + j * KEM_NODE_STRIDE, (int)g[KG_TYPE], r2, far, px, py, pz, v);
+        dev::element_far_wave_pot_n<T>(g, nodes + j * KEM_NODE_STRIDE, an + j * KEM_ANODE_STRIDE, (int)g[KG_TYPE], r2,
+                                       far, dx, dy, dz, px, py, pz, v);
 #pragma unroll
         for (int t = 0; t < T; t++)
             if (live[t] && far[t]) {
@@ -775,6 +782,27 @@ __global__ __launch_bounds__(kBlock) void k_assemble_far_pot(const double* __res
                 if (A32) A32[j * n + row[t]] = (float)v[t];
             }
     }
+}
+
+// The assembly's potential node table (KEM_ANODE_STRIDE layout) from the cubature nodes: one
+// thread per (element, node slot); unused slots hold values never read.
+__global__ __launch_bounds__(kBlock) void k_asm_nodes(const double* __restrict__ geom,
+                                                      const double* __restrict__ nodes, int64_t n,
+                                                      double* __restrict__ an)
+{
+    const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (t >= n * KEM_NODE_SLOTS) return;
+    const int64_t j = t / KEM_NODE_SLOTS;
+    const int slot = (int)(t - j * KEM_NODE_SLOTS);
+    const double* g = geom + j * KG_STRIDE;
+    const double* q = nodes + j * KEM_NODE_STRIDE + 4 * slot;
+    const double x = q[0] - g[KG_CX], y = q[1] - g[KG_CY], z = q[2] - g[KG_CZ];
+    double* o = an + j * KEM_ANODE_STRIDE + 5 * slot;
+    o[0] = -2. * x;
+    o[1] = -2. * y;
+    o[2] = -2. * z;
+    o[3] = __builtin_fma(z, z, __builtin_fma(y, y, x * x));
+    o[4] = q[3];
 }
 
 // Closed-form entries: targets in the Morton order of the element centres (order[k]), so a
@@ -1699,14 +1727,15 @@ hipError_t launch_pairs(hipStream_t st, int policy, const double* geom, const do
 }
 
 namespace {
-// targets per lane of the Dirichlet cubature kernel (100,000-triangle assembly, both kernels:
-// 108 ms at one target per lane, 100-102 ms at 2, 106 ms at 3, 108 ms at 4)
+// targets per lane of the Dirichlet cubature kernel (100,000-triangle assembly, both kernels,
+// plain node table: 108 ms at one target per lane, 100-102 ms at 2, 106 ms at 3, 108 ms at 4)
 constexpr int kAsmTargets = 2;
 
 template <bool G>
 hipError_t assemble_kernels(hipStream_t st, int policy, bool field, const double* geom, const double* nodes,
                             const double* recs, const double* targets, int64_t n, double* A64, float* A32,
-                            bool axisym, const int64_t* gstart, const double* clusters, const int32_t* order)
+                            bool axisym, const int64_t* gstart, const double* clusters, const int32_t* order,
+                            double* anodes)
 {
     if (n <= 0) return hipSuccess;
     const unsigned gx = nblocks(n);
@@ -1738,7 +1767,10 @@ hipError_t assemble_kernels(hipStream_t st, int policy, bool field, const double
             if (sp2 > n) sp2 = n;
             const int64_t per3 = (n + sp2 - 1) / sp2;
             sp2 = (n + per3 - 1) / per3;
-            k_assemble_far_pot<TT><<<dim3(gx2, (unsigned)sp2), kBlock, 0, st>>>(geom, nodes, targets, n, per3, A64, A32);
+            if (!anodes) return hipErrorInvalidValue;
+            k_asm_nodes<<<nblocks(n * KEM_NODE_SLOTS), kBlock, 0, st>>>(geom, nodes, n, anodes);
+            k_assemble_far_pot<TT><<<dim3(gx2, (unsigned)sp2), kBlock, 0, st>>>(geom, nodes, anodes, targets, n, per3,
+                                                                              A64, A32);
             k_assemble_near<false><<<gx, kBlock, 0, st>>>(geom, targets, clusters, ncl, order, n, A64, A32);
         }
         return hipGetLastError();
@@ -1762,15 +1794,18 @@ hipError_t assemble_kernels(hipStream_t st, int policy, bool field, const double
 }
 }  // namespace
 
+int64_t assemble_nodes_doubles(int64_t n) { return n * KEM_ANODE_STRIDE; }
+
 hipError_t launch_assemble(hipStream_t st, int policy, bool field, const double* geom, const double* nodes,
                            const double* recs, const double* targets, int64_t n, double* A64, float* A32,
-                           bool axisym, const int64_t* gstart, const double* clusters, const int32_t* order)
+                           bool axisym, const int64_t* gstart, const double* clusters, const int32_t* order,
+                           double* anodes)
 {
     if (gstart)
         return assemble_kernels<true>(st, policy, field, geom, nodes, recs, targets, n, A64, A32, axisym, gstart,
-                                      nullptr, nullptr);
+                                      nullptr, nullptr, nullptr);
     return assemble_kernels<false>(st, policy, field, geom, nodes, recs, targets, n, A64, A32, axisym, nullptr,
-                                   clusters, order);
+                                   clusters, order, anodes);
 }
 
 hipError_t launch_accumulate4(hipStream_t st, const double* t, int64_t npts, double* phi, double* ex, double* ey,

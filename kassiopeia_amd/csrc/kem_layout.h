@@ -62,6 +62,11 @@ enum kem_geom_field {
 
 #define KEM_NODE_SLOTS 52
 #define KEM_NODE_STRIDE (4 * KEM_NODE_SLOTS)
+/* the assembly's potential node table (Dirichlet rows, numeric policy; built on the device per
+   assembly): per element KEM_NODE_SLOTS nodes of 5 doubles relative to the element centre c,
+   (-2 q'x, -2 q'y, -2 q'z, |q'|^2, w) with q' = Q - c, so that |P - Q|^2 = |P - c|^2 + |q'|^2 -
+   2 (P - c) . q' costs 4 operations per node instead of 6 */
+#define KEM_ANODE_STRIDE (5 * KEM_NODE_SLOTS)
 #define KEM_N7 0
 #define KEM_N12 7
 #define KEM_N33 19
