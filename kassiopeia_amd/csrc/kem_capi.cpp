@@ -1266,7 +1266,8 @@ int gmres_dev(kem_matrix* A, bool use32, const double* d_b, double* d_x, double 
     KEM_CK(c, dalloc(&H.p, (int64_t)ld * ld));
     KEM_CK(c, dalloc(&rot.p, 3 * (int64_t)(kmax + 2)));
     KEM_CK(c, dalloc(&hh.p, 2 * (int64_t)(kmax + 8) + 8));
-    KEM_CK(c, dalloc(&part.p, 256 + (int64_t)kem::gemv_splits(n, ld) * n));
+    const int64_t gparts = 256 + (int64_t)kem::gemv_splits(n, ld) * n;
+    KEM_CK(c, dalloc(&part.p, gparts + (int64_t)ld * kem::kArnProjSplits));
     KEM_CK(c, dalloc(&scal.p, 8));
     int* d_ints = nullptr;
     KEM_CK(c, hipMalloc((void**)&d_ints, 4 * sizeof(int)));
@@ -1289,6 +1290,7 @@ int gmres_dev(kem_matrix* A, bool use32, const double* d_b, double* d_x, double 
     a.h2 = hh.p + (kmax + 8);
     a.nrm = hh.p + 2 * (kmax + 8);
     a.part = part.p;
+    a.hp = part.p + gparts;
     a.ints = d_ints;
     a.scal = scal.p;
     double bnorm;

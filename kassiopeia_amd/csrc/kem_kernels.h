@@ -151,6 +151,8 @@ hipError_t launch_gemv_t(hipStream_t st, const T* A, int64_t lda, int64_t nrows,
 // dimension ld = kmax+1), rotations cs / sn, rotated right-hand side p, projections h1 / h2,
 // norm scalar, the column index j and the stop flag done (ints), the cycle's column limit
 // jlimit, and the tolerance tol * ||b|| against which |p[j+1]| is tested.
+constexpr int kArnProjSplits = 16;  // most row ranges of a GMRES projection
+int arn_proj_splits(int64_t n);
 struct ArnoldiState {
     int64_t n = 0;
     int kmax = 0;
@@ -165,6 +167,7 @@ struct ArnoldiState {
     double* h2 = nullptr;
     double* nrm = nullptr;
     double* part = nullptr;  // 256 dot partials + gemv partials (gemv_splits(n, kmax + 1) * n)
+    double* hp = nullptr;    // projection partials, (kmax + 1) * kArnProjSplits
     int* ints = nullptr;     // [0] j, [1] done, [2] jlimit, [3] converged
     double* scal = nullptr;  // [0] tol * ||b||, [1] |p[j+1]| after the last step
 };

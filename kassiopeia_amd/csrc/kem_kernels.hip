@@ -1035,19 +1035,37 @@ __global__ __launch_bounds__(kBlock) void k_div_dev(const double* __restrict__ d
 // z = a*x + b*y
 // ---- GMRES Arnoldi step on device state (ints: [0] j, [1] done, [2] jlimit, [3] converged)
 // h[q] = V[:, q] . w for q <= j (one block per column of the basis's capacity)
+// hp[q*S + s] = <V q, w> over rows [s per, (s+1) per) for q <= j (S = gridDim.y row ranges:
+// a block per (q, range), so the latency chain of a column's dot product is 1/S as long; one
+// block per column took 27 us per pass at 20,000 rows and 102 us at 100,000)
 __global__ __launch_bounds__(kBlock) void k_arn_proj(const double* __restrict__ V, int64_t n,
-                                                     const double* __restrict__ w, double* __restrict__ h,
-                                                     const int* __restrict__ ints)
+                                                     const double* __restrict__ w, double* __restrict__ hp,
+                                                     int64_t per, const int* __restrict__ ints)
 {
     if (ints[1]) return;
     const int64_t q = blockIdx.x;
     if (q > ints[0]) return;
+    const int S = gridDim.y, r = blockIdx.y;
     __shared__ double sh[kBlock / 64];
     const double* col = V + q * n;
+    const int64_t i1 = std::min<int64_t>(n, (r + 1) * per);
     double s = 0.;
-    for (int64_t i = threadIdx.x; i < n; i += kBlock) s += col[i] * w[i];
+    for (int64_t i = r * per + threadIdx.x; i < i1; i += kBlock) s += col[i] * w[i];
     s = block_sum(s, sh);
-    if (threadIdx.x == 0) h[q] = s;
+    if (threadIdx.x == 0) hp[q * S + r] = s;
+}
+
+// h[q] = sum_s hp[q*S + s] in order, q <= j
+__global__ __launch_bounds__(kBlock) void k_arn_hsum(const double* __restrict__ hp, int S, double* __restrict__ h,
+                                                     const int* __restrict__ ints)
+{
+    if (ints[1]) return;
+    const int j = ints[0];
+    for (int q = threadIdx.x; q <= j; q += kBlock) {
+        double s = 0.;
+        for (int r = 0; r < S; r++) s += hp[(int64_t)q * S + r];
+        h[q] = s;
+    }
 }
 // part[k][i] = sum over the k-th column range (of columns 0..j) of V[i, q] h[q]
 __global__ __launch_bounds__(kBlock) void k_arn_update_part(const double* __restrict__ V, int64_t n,
@@ -1872,6 +1890,8 @@ template hipError_t launch_gemv_t<double>(hipStream_t, const double*, int64_t, i
 template hipError_t launch_gemv_t<float>(hipStream_t, const float*, int64_t, int64_t, int64_t, const double*,
                                          double*, const int*);
 
+int arn_proj_splits(int64_t n) { return (int)std::max<int64_t>(1, std::min<int64_t>(n / 4096, kArnProjSplits)); }
+
 hipError_t launch_arnoldi_step(hipStream_t st, const ArnoldiState& a)
 {
     const int64_t n = a.n, ncap = (int64_t)a.kmax + 1;
@@ -1880,9 +1900,12 @@ hipError_t launch_arnoldi_step(hipStream_t st, const ArnoldiState& a)
     nsplit = (int)((ncap + per - 1) / per);
     const int* ints = a.ints;
     double* gpart = a.part + 256;
+    const int ps = arn_proj_splits(n);
+    const int64_t pper = (n + ps - 1) / ps;
     // CGS2: h1 = V^T w, w -= V h1; h2 = V^T w, w -= V h2 (columns 0..j)
     for (double* h : {a.h1, a.h2}) {
-        k_arn_proj<<<(unsigned)ncap, kBlock, 0, st>>>(a.V, n, a.w, h, ints);
+        k_arn_proj<<<dim3((unsigned)ncap, (unsigned)ps), kBlock, 0, st>>>(a.V, n, a.w, a.hp, pper, ints);
+        k_arn_hsum<<<1, kBlock, 0, st>>>(a.hp, ps, h, ints);
         k_arn_update_part<<<dim3(nblocks(n), (unsigned)nsplit), kBlock, 0, st>>>(a.V, n, per, h, gpart, ints);
         k_gemv_reduce<<<nblocks(n), kBlock, 0, st>>>(gpart, nsplit, n, -1.0, 1.0, a.w, ints + 1);
     }
