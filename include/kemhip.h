@@ -289,6 +289,10 @@ int kem_field_plan(int policy, int clustered, int precision, int multi, int64_t 
    Neumann: E.n_i, diagonal (1+f)/(1-f)/(2 eps0)).  storage: KEM_STORE_FP64 and/or
    KEM_STORE_FP32. */
 int kem_matrix_assemble(kem_ctx* ctx, const kem_mesh* mesh, int storage, kem_matrix** out);
+/* Destroys a matrix (before its context).  Its FP64 / FP32 / BF16 storage returns to the
+   context, which hands it to the next kem_matrix_assemble that needs a block of the same size
+   (reallocating tens of GB right after freeing them waits for the freed memory to be cleared)
+   and frees it in kem_ctx_destroy, or when a new allocation would otherwise fail. */
 void kem_matrix_destroy(kem_matrix* A);
 int64_t kem_matrix_dimension(const kem_matrix* A);
 /* KSquareMatrix::operator()(i, j) */

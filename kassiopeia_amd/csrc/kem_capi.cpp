@@ -79,6 +79,10 @@ struct kem_ctx {
     FieldWs fws;  // workspace of eager field evaluations (grow-only)
     double* acc4 = nullptr;  // one field's (phi, Ex, Ey, Ez) of a step-level sum over fields (grow-only)
     int64_t acc4_doubles = 0;
+    // storage of destroyed matrices, kept for the next matrix of the same size: freeing and
+    // allocating tens of GB again costs up to a second on this hardware (the freed memory is
+    // cleared before it is handed out again), as much as a 100,000-triangle assembly ten times
+    std::vector<std::pair<void*, size_t>> store_cache;
     // far-kernel timing: one event pair per field evaluation since the last query
     static constexpr int kTimedMax = 256;
     bool timing = false;
@@ -170,6 +174,37 @@ template <typename T>
 void dfree(T*& p)
 {
     if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
+// matrix storage: a cached block of exactly this size, else a new allocation (after releasing
+// the cache if the first attempt fails)
+template <typename T>
+hipError_t store_alloc(kem_ctx* c, T** p, int64_t count)
+{
+    *p = nullptr;
+    if (count <= 0) return hipSuccess;
+    const size_t bytes = sizeof(T) * (size_t)count;
+    for (size_t k = 0; k < c->store_cache.size(); k++)
+        if (c->store_cache[k].second == bytes) {
+            *p = (T*)c->store_cache[k].first;
+            c->store_cache.erase(c->store_cache.begin() + (std::ptrdiff_t)k);
+            return hipSuccess;
+        }
+    hipError_t e = hipMalloc((void**)p, bytes);
+    if (e != hipSuccess && !c->store_cache.empty()) {
+        (void)hipGetLastError();
+        for (auto& b : c->store_cache) (void)hipFree(b.first);
+        c->store_cache.clear();
+        e = hipMalloc((void**)p, bytes);
+    }
+    return e;
+}
+
+template <typename T>
+void store_free(kem_ctx* c, T*& p, int64_t count)
+{
+    if (p) c->store_cache.emplace_back((void*)p, sizeof(T) * (size_t)count);
     p = nullptr;
 }
 
@@ -339,6 +374,8 @@ void kem_ctx_destroy(kem_ctx* c)
     dfree(c->scal);
     dfree(c->acc4);
     ws_release(c->fws);
+    for (auto& b : c->store_cache) (void)hipFree(b.first);
+    c->store_cache.clear();
     for (hipEvent_t& e : c->ev)
         if (e) (void)hipEventDestroy(e);
     if (c->side.st) (void)hipStreamSynchronize(c->side.st);
@@ -1095,12 +1132,13 @@ int kem_matrix_assemble(kem_ctx* c, const kem_mesh* m, int storage, kem_matrix**
     A->n = m->n;
     const int64_t nn = m->n * m->n;
     hipError_t e = hipSuccess;
-    if (storage & KEM_STORE_FP64) e = dalloc(&A->A64, nn);
+    if (storage & KEM_STORE_FP64) e = store_alloc(c, &A->A64, nn);
     // BF16 storage is packed from the FP64 matrix, else from the FP32 one (assembled for it
     // and dropped again when FP32 storage was not asked for)
     const bool need32 = (storage & KEM_STORE_FP32) || ((storage & KEM_STORE_BF16) && !(storage & KEM_STORE_FP64));
-    if (e == hipSuccess && need32) e = dalloc(&A->A32, nn);
-    if (e == hipSuccess && (storage & KEM_STORE_BF16)) e = dalloc(&A->A16, (int64_t)kem::bf16_matrix_elements(m->n));
+    if (e == hipSuccess && need32) e = store_alloc(c, &A->A32, nn);
+    if (e == hipSuccess && (storage & KEM_STORE_BF16))
+        e = store_alloc(c, &A->A16, (int64_t)kem::bf16_matrix_elements(m->n));
     // the potential node table of the split numeric-policy assembly (Dirichlet rows)
     DVec an;
     if (e == hipSuccess && c->policy == 0 && !m->has_neumann && !m->has_axisym && !m->gstart && m->clusters &&
@@ -1115,7 +1153,7 @@ int kem_matrix_assemble(kem_ctx* c, const kem_mesh* m, int storage, kem_matrix**
     if (e == hipSuccess && t1) e = hipEventRecord(t1, c->stream);
     if (e == hipSuccess && A->A16) e = kem::launch_pack_bf16(c->stream, A->A64, A->A32, m->n, A->A16);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-    if (e == hipSuccess && A->A32 && !(storage & KEM_STORE_FP32)) dfree(A->A32);
+    if (e == hipSuccess && A->A32 && !(storage & KEM_STORE_FP32)) store_free(c, A->A32, nn);
     if (e != hipSuccess) {
         kem_matrix_destroy(A);
         return fail(c, KEM_E_HIP, std::string("assemble: ") + hipGetErrorString(e));
@@ -1128,9 +1166,17 @@ void kem_matrix_destroy(kem_matrix* A)
 {
     if (!A) return;
     if (A->ctx) (void)hipSetDevice(A->ctx->device);
-    dfree(A->A64);
-    dfree(A->A32);
-    dfree(A->A16);
+    // the storage goes back to the context for the next matrix of the same size
+    const int64_t nn = A->n * A->n;
+    if (A->ctx) {
+        store_free(A->ctx, A->A64, nn);
+        store_free(A->ctx, A->A32, nn);
+        store_free(A->ctx, A->A16, (int64_t)kem::bf16_matrix_elements(A->n));
+    } else {
+        dfree(A->A64);
+        dfree(A->A32);
+        dfree(A->A16);
+    }
     if (A->bws) (void)hipFree(A->bws);
     delete A;
 }
