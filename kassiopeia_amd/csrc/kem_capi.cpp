@@ -145,6 +145,9 @@ struct kem_matrix {
     // partials), kept from one solve to the next and grown when a solve needs more
     void* bws = nullptr;
     size_t bws_bytes = 0;
+    // the single-right-hand-side solver's workspace (Krylov basis, Hessenberg matrix, partials)
+    void* sws = nullptr;
+    size_t sws_bytes = 0;
 };
 
 namespace {
@@ -1178,6 +1181,7 @@ void kem_matrix_destroy(kem_matrix* A)
         dfree(A->A16);
     }
     if (A->bws) (void)hipFree(A->bws);
+    if (A->sws) (void)hipFree(A->sws);
     delete A;
 }
 
@@ -1305,22 +1309,38 @@ int gmres_dev(kem_matrix* A, bool use32, const double* d_b, double* d_x, double 
     if (restart > 0) kmax = std::min(kmax, restart);
     if (kmax < 1) kmax = 1;
     const int ld = kmax + 1;
-    DVec V, w, vcur, H, rot, hh, part, scal;
-    KEM_CK(c, dalloc(&V.p, (int64_t)ld * n));
-    KEM_CK(c, dalloc(&w.p, n));
-    KEM_CK(c, dalloc(&vcur.p, n));
-    KEM_CK(c, dalloc(&H.p, (int64_t)ld * ld));
-    KEM_CK(c, dalloc(&rot.p, 3 * (int64_t)(kmax + 2)));
-    KEM_CK(c, dalloc(&hh.p, 2 * (int64_t)(kmax + 8) + 8));
     const int64_t gparts = 256 + (int64_t)kem::gemv_splits(n, ld) * n;
-    KEM_CK(c, dalloc(&part.p, gparts + (int64_t)ld * kem::kArnProjSplits));
-    KEM_CK(c, dalloc(&scal.p, 8));
-    int* d_ints = nullptr;
-    KEM_CK(c, hipMalloc((void**)&d_ints, 4 * sizeof(int)));
-    struct IntBuf {
-        int* p;
-        ~IntBuf() { (void)hipFree(p); }
-    } ints_guard{d_ints};
+    // one allocation, carved into 256-byte aligned pieces, kept on the matrix between solves
+    // (the Krylov basis of a 100,000-unknown solve is 800 MB)
+    size_t off = 0;
+    auto carve = [&off](size_t bytes) {
+        const size_t at = off;
+        off += (bytes + 255) & ~(size_t)255;
+        return at;
+    };
+    const size_t oV = carve(sizeof(double) * (size_t)ld * n);
+    const size_t ow = carve(sizeof(double) * (size_t)n);
+    const size_t ovc = carve(sizeof(double) * (size_t)n);
+    const size_t oH = carve(sizeof(double) * (size_t)ld * ld);
+    const size_t orot = carve(sizeof(double) * 3 * (size_t)(kmax + 2));
+    const size_t ohh = carve(sizeof(double) * (2 * (size_t)(kmax + 8) + 8));
+    const size_t opart = carve(sizeof(double) * (size_t)(gparts + (int64_t)ld * kem::kArnProjSplits));
+    const size_t oscal = carve(sizeof(double) * 8);
+    const size_t oints = carve(sizeof(int) * 4);
+    if (A->sws_bytes < off) {
+        if (A->sws) (void)hipFree(A->sws);
+        A->sws = nullptr;
+        A->sws_bytes = 0;
+        KEM_CK(c, hipMalloc(&A->sws, off));
+        A->sws_bytes = off;
+    }
+    char* base = (char*)A->sws;
+    struct {
+        double* p;
+    } V{(double*)(base + oV)}, w{(double*)(base + ow)}, vcur{(double*)(base + ovc)}, H{(double*)(base + oH)},
+        rot{(double*)(base + orot)}, hh{(double*)(base + ohh)}, part{(double*)(base + opart)},
+        scal{(double*)(base + oscal)};
+    int* const d_ints = (int*)(base + oints);
     KEM_CK(c, ensure_ws(c, (int64_t)kem::gemv_splits(n, n) * n));
     kem::ArnoldiState a;
     a.n = n;
