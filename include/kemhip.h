@@ -307,7 +307,9 @@ int kem_matrix_multiply(kem_matrix* A, int transpose, const double* x, double* y
    .../KRobinHoodChargeDensitySolver.cc:83-212) over KSimpleIterativeKrylovSolver<GMRES>
    (LinearAlgebra/Solvers/include/KSimpleIterativeKrylovSolver.hh:140-194) and KRobinHood
    (KRobinHood.hh:62-104).  x holds the start vector on entry (KBoundaryIntegralSolutionVector)
-   and the solution on exit.  Returns KEM_OK, or KEM_W_NOT_CONVERGED at the iteration/time limit. */
+   and the solution on exit.  Returns KEM_OK, or KEM_W_NOT_CONVERGED at the iteration/time limit.
+   The matrix keeps the GMRES workspace of its largest solve until kem_matrix_destroy (mostly
+   the Krylov basis, (min(max_iterations, restart if set, 1000) + 1) x n doubles). */
 void kem_solve_opts_default(kem_solve_opts* opts);
 int kem_solve(kem_matrix* A, const kem_solve_opts* opts, const double* b, double* x, kem_solve_stats* stats);
 
