@@ -771,9 +771,19 @@ __global__ __launch_bounds__(kBlock) void k_bnorm(const double* __restrict__ W, 
     __shared__ double sh[kBlock / 64];
     const int r = blockIdx.x;
     const double* w = W + (int64_t)r * n;
-    double s = 0.;
-    for (int64_t i = threadIdx.x; i < n; i += kBlock) s += w[i] * w[i];
-    s = block_sum(s, sh);
+    // four independent sums, so four loads per thread are in flight (one block per right-hand
+    // side is otherwise bound by one load latency per 256 entries)
+    double s0 = 0., s1 = 0., s2 = 0., s3 = 0.;
+    int64_t i = threadIdx.x;
+    for (; i + 3 * kBlock < n; i += 4 * kBlock) {
+        const double a = w[i], b = w[i + kBlock], c = w[i + 2 * kBlock], d = w[i + 3 * kBlock];
+        s0 += a * a;
+        s1 += b * b;
+        s2 += c * c;
+        s3 += d * d;
+    }
+    for (; i < n; i += kBlock) s0 += w[i] * w[i];
+    double s = block_sum((s0 + s1) + (s2 + s3), sh);
     if (threadIdx.x == 0) nrm[r * stride] = sqrt(s);
 }
 
@@ -872,8 +882,18 @@ __global__ __launch_bounds__(64) void k_bproj_sum(const double* __restrict__ par
     const int j = min(ints[0], ldh - 2);
     const int r = blockIdx.x, nrhs = gridDim.x;
     for (int q = threadIdx.x; q <= j; q += 64) {
+        // the chunks in order, four loads in flight
         double s = 0.;
-        for (int c = 0; c < nchunks; c++) s += part[((int64_t)c * nrhs + r) * ldh + q];
+        int c = 0;
+        for (; c + 3 < nchunks; c += 4) {
+            const double a = part[((int64_t)c * nrhs + r) * ldh + q], b = part[((int64_t)(c + 1) * nrhs + r) * ldh + q],
+                         d = part[((int64_t)(c + 2) * nrhs + r) * ldh + q], e = part[((int64_t)(c + 3) * nrhs + r) * ldh + q];
+            s += a;
+            s += b;
+            s += d;
+            s += e;
+        }
+        for (; c < nchunks; c++) s += part[((int64_t)c * nrhs + r) * ldh + q];
         h[r * ldh + q] = s;
     }
 }
