@@ -758,9 +758,17 @@ __global__ __launch_bounds__(kBlock) void k_bupdate(const VT* __restrict__ V, in
     const VT* v = V + r * vstride + i;
     const double* hr = h + r * ldh;
     const int m = cnt[r];
-    double s = 0.;
-    for (int q = 0; q < m; q++) s += (double)v[(int64_t)q * n] * hr[q];
-    Y[(int64_t)r * n + i] += sign * s;
+    // four independent sums, four basis loads in flight per thread
+    double s0 = 0., s1 = 0., s2 = 0., s3 = 0.;
+    int q = 0;
+    for (; q + 3 < m; q += 4) {
+        s0 += (double)v[(int64_t)q * n] * hr[q];
+        s1 += (double)v[(int64_t)(q + 1) * n] * hr[q + 1];
+        s2 += (double)v[(int64_t)(q + 2) * n] * hr[q + 2];
+        s3 += (double)v[(int64_t)(q + 3) * n] * hr[q + 3];
+    }
+    for (; q < m; q++) s0 += (double)v[(int64_t)q * n] * hr[q];
+    Y[(int64_t)r * n + i] += sign * ((s0 + s1) + (s2 + s3));
 }
 
 // nrm[r] = ||W_r|| (block per r)
